@@ -1,0 +1,147 @@
+"""Benchmark of the streaming-BDPT hot path on one MI355X.
+
+Workload (one step): the 262,144-triangle sphere-grid scene (64 x 64
+64-triangle spheres on a ground plane, 2 emissive sphere lights), 1280 x 720,
+eye depth 8, 64 frames = 64 samples per pixel through the wavefront BDPT
+(light-vertex cache every third frame).  Inputs are built and uploaded to HBM
+before timing; the timed region is K renders of 64 frames each.
+
+metric/value: Mrays/s = (extension + shadow rays traced) / wall time.
+roofline: the closest-hit extension kernel, algorithmic bytes per launch =
+  48 B per ray (32 B ray in + 16 B hit out) + 128 B per 4-wide node visited +
+  48 B per triangle tested, over its HIP-event duration on the renderer stream.
+cpu_baseline: the oracle's FP32 reference estimator on the host cores (OpenMP),
+  a bounded sample of the same frame (a band of rows at 1 spp).
+
+python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WIDTH, HEIGHT, SPP, EYE_DEPTH, LIGHT_DEPTH = 1280, 720, 64, 8, 8
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(sc, seconds_target=10.0, max_frames=8):
+    """Oracle FP32 reference estimator over whole 1280x720 frames at 1 spp each
+    (frames 0, 1, ... of the same stream) until ~seconds_target of wall time."""
+    import oracle
+    O = oracle.Scene(sc)
+    threads = oracle.max_threads()
+    p = oracle.render_params(WIDTH, HEIGHT, sc["camera"], eye_depth=EYE_DEPTH, light_depth=LIGHT_DEPTH)
+    R = oracle.Renderer(p, 32)
+    t_used, frames = 0.0, 0
+    while t_used < seconds_target and frames < max_frames:
+        t0 = time.perf_counter()
+        R.render(O, frames, 1)
+        t_used += time.perf_counter() - t0
+        frames += 1
+    e, s = R.ray_counts()
+    return {"value": round((e + s) / t_used / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"oracle FP32 reference estimator, {frames} full {WIDTH}x{HEIGHT} frame(s) at 1 spp "
+                      f"({e + s} rays in {t_used:.1f} s, OpenMP {threads} threads, "
+                      f"{platform.processor() or platform.machine()})"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--frames", type=int, default=SPP)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (one-GPU build; "
+              "see the sharding note in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import cudapt_amd as ca
+    from cudapt_amd import scenes
+
+    t0 = time.perf_counter()
+    sc = scenes.sphere_grid()
+    S = ca.Scene(sc)
+    info = S.bvh4_info()
+    log(f"scene: {len(sc['idx'])} triangles, bvh4 {info}, built+uploaded in {time.perf_counter() - t0:.1f} s")
+    R = ca.Renderer(WIDTH, HEIGHT, eye_depth=EYE_DEPTH, light_depth=LIGHT_DEPTH)
+    cam = sc["camera"]
+    for w in range(args.warmup):
+        R.render(S, cam, frames=args.frames, result=False)
+        log(f"warmup {w + 1}/{args.warmup} done")
+    R.stats(reset=True)
+    R.set_profiling(True)
+    ca.synchronize()
+    times = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        R.render(S, cam, frames=args.frames, result=False)   # ends with a stream synchronisation
+        ca.synchronize()
+        times.append(time.perf_counter() - ts)
+        log(f"step {k + 1}/{args.steps}: {times[-1] * 1e3:.1f} ms")
+    wall = time.perf_counter() - t_start
+    st = R.stats(reset=True)
+    R.set_profiling(False)
+    rays = st["extension_rays"] + st["shadow_rays"]
+    mrays = rays / wall / 1e6
+    launches = max(st["extend_launches"], 1)
+    bytes_alg = 48 * st["extension_rays"] + 128 * st["extend_nodes"] + 48 * st["extend_tris"]
+    ext_s = st["extend_ms"] / 1e3
+    achieved = bytes_alg / ext_s / 1e9 if ext_s > 0 else 0.0
+    res = {
+        "metric": "Mrays/s + time-to-equal-MSE (learned vs plain adaptive BDPT), 1 MI355X",
+        "value": round(mrays, 2),
+        "unit": "Mrays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded procedural scene)",
+        "config": {"workload": "streaming BDPT, sphere grid 262,144 tris + 2 sphere lights, 1280x720, "
+                               "depth 8, 64 spp per step, light-vertex cache every 3rd frame",
+                   "triangles": int(len(sc["idx"])), "width": WIDTH, "height": HEIGHT, "spp": args.frames,
+                   "eye_depth": EYE_DEPTH, "light_depth": LIGHT_DEPTH},
+        "samples_per_s": round(st["eye_paths"] / wall, 1),
+        "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},
+        "kernel_ms": {"extend": round(st["extend_ms"], 3), "shade": round(st["shade_ms"], 3),
+                      "shadow": round(st["shadow_ms"], 3), "light_pass": round(st["light_ms"], 3),
+                      "all": round(st["total_ms"], 3)},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "kernel": "k_trace<0,false,1> (closest-hit extension)",
+                     "launches": launches, "avg_launch_us": round(st["extend_ms"] * 1e3 / launches, 2),
+                     "bytes_per_launch": int(bytes_alg / launches),
+                     "nodes_per_ray": round(st["extend_nodes"] / max(st["extension_rays"], 1), 2),
+                     "tris_per_ray": round(st["extend_tris"] / max(st["extension_rays"], 1), 2)},
+        "time_to_equal_mse": None,
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        res["cpu_baseline"] = cpu_baseline(sc)
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,294 @@
+"""cudapt_amd: MI355X (gfx950) streaming-BDPT hot path behind a C ABI.
+
+Python front end over libcpt_hip.so (include/cpt.h) through ctypes.  There is
+no CPU fallback: importing the library fails loudly when the HIP build is
+missing, and every call raises CptError on a non-zero return code.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+from . import scenes  # noqa: F401  (procedural inputs)
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libcpt_hip.so")
+_lib = None
+
+CPT_MODE_BDPT_STREAM = 0
+CPT_MODE_BDPT_ADAPTIVE_PLL = 1
+CPT_BUF_RESULT, CPT_BUF_SAMPLE_N = 0, 1
+
+RAY_DTYPE = np.dtype([("o", "<f4", 3), ("max_dist", "<f4"), ("d", "<f4", 3), ("eps", "<f4")])
+HIT_DTYPE = np.dtype([("tri", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+MISS = 0xFFFFFFFF
+
+
+class CptError(RuntimeError):
+    pass
+
+
+class _Stats(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint64) for n in ("extension_rays", "shadow_rays", "eye_paths", "frames",
+                                              "extend_launches", "extend_nodes", "extend_tris")] + \
+               [(n, ctypes.c_double) for n in ("extend_ms", "shadow_ms", "shade_ms", "light_ms", "total_ms")]
+
+
+def lib():
+    """The loaded HIP library; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CptError(f"{LIB_PATH} is missing: run `python -m cudapt_amd.build` (no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        vp, c32, i32, f32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int, ctypes.c_float
+        L.cpt_last_error.restype = ctypes.c_char_p
+        sigs = {
+            "cpt_device_count": [vp],
+            "cpt_device_alloc": [vp, ctypes.c_uint64],
+            "cpt_device_free": [vp],
+            "cpt_memcpy_htod": [vp, vp, ctypes.c_uint64],
+            "cpt_memcpy_dtoh": [vp, vp, ctypes.c_uint64],
+            "cpt_device_synchronize": [],
+            "cpt_scene_create": [vp],
+            "cpt_scene_set_mesh": [vp, vp, vp, vp, c32, vp, vp, c32],
+            "cpt_scene_set_materials": [vp, vp, c32],
+            "cpt_scene_update_materials": [vp],
+            "cpt_bvh_build_sah": [vp, c32],
+            "cpt_bvh_export": [vp, vp, vp, vp, vp],
+            "cpt_bvh4_info": [vp, vp, vp, vp],
+            "cpt_scene_upload": [vp],
+            "cpt_scene_destroy": [vp],
+            "cpt_trace_closest": [vp, vp, c32, c32, vp],
+            "cpt_trace_closest_device": [vp, vp, c32, c32, vp, vp, c32],
+            "cpt_trace_occluded_device": [vp, vp, c32, c32, vp, vp],
+            "cpt_trace_counters": [vp, vp, vp, vp, i32],
+            "cpt_renderer_create": [vp, i32, c32, c32],
+            "cpt_render_iteration": [vp, vp, vp, vp, vp, f32, vp],
+            "cpt_render_frames": [vp, vp, vp, vp, vp, f32, c32, vp],
+            "cpt_renderer_attrib_count": [vp, vp],
+            "cpt_renderer_get_attrib": [vp, c32, vp, c32, vp, vp],
+            "cpt_renderer_set_attrib": [vp, c32, ctypes.c_double],
+            "cpt_renderer_reset": [vp],
+            "cpt_renderer_read_buffer": [vp, i32, vp],
+            "cpt_renderer_stats": [vp, vp, i32],
+            "cpt_renderer_set_profiling": [vp, i32],
+            "cpt_renderer_destroy": [vp],
+        }
+        for name, args in sigs.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc: int, what: str):
+    if rc != 0:
+        raise CptError(f"{what} failed ({rc}): {lib().cpt_last_error().decode()}")
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f3(v):
+    return (ctypes.c_float * 3)(*[float(x) for x in v])
+
+
+def device_count() -> int:
+    n = ctypes.c_int(0)
+    _check(lib().cpt_device_count(ctypes.byref(n)), "cpt_device_count")
+    return n.value
+
+
+class DeviceBuffer:
+    """HBM allocation owned through the library's HIP runtime."""
+
+    def __init__(self, nbytes: int):
+        p = ctypes.c_void_p()
+        _check(lib().cpt_device_alloc(ctypes.byref(p), int(nbytes)), "cpt_device_alloc")
+        self.ptr, self.nbytes = p.value or 0, int(nbytes)
+
+    @classmethod
+    def from_array(cls, a: np.ndarray) -> "DeviceBuffer":
+        a = np.ascontiguousarray(a)
+        b = cls(a.nbytes)
+        b.upload(a)
+        return b
+
+    def upload(self, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert a.nbytes <= self.nbytes
+        _check(lib().cpt_memcpy_htod(ctypes.c_void_p(self.ptr), _ptr(a), a.nbytes), "cpt_memcpy_htod")
+
+    def download(self, dtype, count: int) -> np.ndarray:
+        out = np.zeros(count, dtype)
+        assert out.nbytes <= self.nbytes
+        _check(lib().cpt_memcpy_dtoh(_ptr(out), ctypes.c_void_p(self.ptr), out.nbytes), "cpt_memcpy_dtoh")
+        return out
+
+    def __del__(self):
+        if getattr(self, "ptr", 0) and _lib is not None:
+            _lib.cpt_device_free(ctypes.c_void_p(self.ptr))
+            self.ptr = 0
+
+
+def synchronize():
+    _check(lib().cpt_device_synchronize(), "cpt_device_synchronize")
+
+
+class Scene:
+    """RTScene counterpart: mesh + materials, SAH BVH, device upload."""
+
+    def __init__(self, sc: dict | None = None, max_depth: int = 32, upload: bool = True):
+        L = lib()
+        h = ctypes.c_void_p()
+        _check(L.cpt_scene_create(ctypes.byref(h)), "cpt_scene_create")
+        self.h = h
+        if sc is not None:
+            self.set_mesh(sc["pos"], sc.get("nrm"), sc["idx"], sc["mat"])
+            self.set_materials(sc["mats"])
+            self.build_bvh(max_depth)
+            if upload:
+                self.upload()
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cpt_scene_destroy(self.h)
+            self.h = None
+
+    def set_mesh(self, pos, nrm, idx, mat):
+        self._pos = np.ascontiguousarray(pos, np.float32)
+        self._nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32)
+        self._idx = np.ascontiguousarray(idx, np.uint32)
+        self._mat = np.ascontiguousarray(mat, np.uint32)
+        self.nt = len(self._idx)
+        _check(lib().cpt_scene_set_mesh(self.h, _ptr(self._pos), _ptr(self._nrm), None, len(self._pos),
+                                        _ptr(self._idx), _ptr(self._mat), self.nt), "cpt_scene_set_mesh")
+
+    def set_materials(self, mats):
+        self._mats = np.ascontiguousarray(mats)
+        assert self._mats.dtype.itemsize == 80
+        _check(lib().cpt_scene_set_materials(self.h, _ptr(self._mats), len(self._mats)), "cpt_scene_set_materials")
+
+    def update_materials(self):
+        _check(lib().cpt_scene_update_materials(self.h), "cpt_scene_update_materials")
+
+    def build_bvh(self, max_depth: int = 32):
+        _check(lib().cpt_bvh_build_sah(self.h, max_depth), "cpt_bvh_build_sah")
+
+    def upload(self):
+        _check(lib().cpt_scene_upload(self.h), "cpt_scene_upload")
+
+    def bvh(self):
+        n = ctypes.c_uint32(0)
+        _check(lib().cpt_bvh_export(self.h, None, None, None, ctypes.byref(n)), "cpt_bvh_export")
+        bounds = np.zeros((n.value, 6), np.float32)
+        off = np.zeros((n.value, 2), np.uint32)
+        reorder = np.zeros(self.nt, np.uint32)
+        _check(lib().cpt_bvh_export(self.h, _ptr(bounds), _ptr(off), _ptr(reorder), ctypes.byref(n)),
+               "cpt_bvh_export")
+        return bounds, off, reorder
+
+    def bvh4_info(self):
+        a, b, c = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()
+        _check(lib().cpt_bvh4_info(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "cpt_bvh4_info")
+        return dict(nodes=a.value, depth=b.value, leaves=c.value)
+
+    def trace(self, rays: np.ndarray, cull: bool = False) -> np.ndarray:
+        """rays: structured RAY_DTYPE or (n, 8) float32. Host arrays in and out."""
+        r = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+        hits = np.zeros(len(r), HIT_DTYPE)
+        _check(lib().cpt_trace_closest(self.h, _ptr(r), len(r), int(cull), _ptr(hits)), "cpt_trace_closest")
+        return hits
+
+    def trace_device(self, d_rays_ptr: int, n: int, d_hits_ptr: int, stream_ptr: int = 0, layout: int = 0,
+                     cull: bool = False):
+        _check(lib().cpt_trace_closest_device(self.h, ctypes.c_void_p(d_rays_ptr), n, int(cull),
+                                              ctypes.c_void_p(d_hits_ptr),
+                                              ctypes.c_void_p(stream_ptr) if stream_ptr else None, layout),
+               "cpt_trace_closest_device")
+
+    def occluded_device(self, d_rays_ptr: int, n: int, d_out_ptr: int, stream_ptr: int = 0, cull: bool = False):
+        _check(lib().cpt_trace_occluded_device(self.h, ctypes.c_void_p(d_rays_ptr), n, int(cull),
+                                               ctypes.c_void_p(d_out_ptr),
+                                               ctypes.c_void_p(stream_ptr) if stream_ptr else None),
+               "cpt_trace_occluded_device")
+
+    def counters(self, reset: bool = False):
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _check(lib().cpt_trace_counters(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c), int(reset)),
+               "cpt_trace_counters")
+        return dict(nodes=a.value, tris=b.value)
+
+
+class Renderer:
+    """RTRenderer + <ns>::Render counterpart (one call = one accumulation frame)."""
+
+    def __init__(self, width: int, height: int, mode: int = CPT_MODE_BDPT_STREAM, **attribs):
+        h = ctypes.c_void_p()
+        _check(lib().cpt_renderer_create(ctypes.byref(h), mode, width, height), "cpt_renderer_create")
+        self.h, self.W, self.H = h, width, height
+        for k, v in attribs.items():
+            self.set_attrib(k, v)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cpt_renderer_destroy(self.h)
+            self.h = None
+
+    def attribs(self):
+        n = ctypes.c_uint32()
+        _check(lib().cpt_renderer_attrib_count(self.h, ctypes.byref(n)), "cpt_renderer_attrib_count")
+        out = {}
+        for i in range(n.value):
+            name = ctypes.create_string_buffer(64)
+            t, v = ctypes.c_int(), ctypes.c_double()
+            _check(lib().cpt_renderer_get_attrib(self.h, i, name, 64, ctypes.byref(t), ctypes.byref(v)),
+                   "cpt_renderer_get_attrib")
+            out[name.value.decode()] = (i, t.value, v.value)
+        return out
+
+    def set_attrib(self, name, value):
+        key = {"eye_depth": "Eye Depth", "light_depth": "Light Depth", "light_paths": "Light Paths",
+               "seed": "Seed", "literal_shadow": "Literal Shadow Endpoint"}.get(name, name)
+        idx = self.attribs()[key][0]
+        _check(lib().cpt_renderer_set_attrib(self.h, idx, float(value)), "cpt_renderer_set_attrib")
+
+    def render(self, scene: Scene, camera: dict, frames: int = 1, result: bool = True):
+        out = np.zeros((self.H, self.W, 3), np.float32) if result else None
+        args = (self.h, scene.h, _f3(camera["pos"]), _f3(camera["dir"]), _f3(camera["up"]),
+                float(camera["fov"]))
+        if frames == 1:
+            _check(lib().cpt_render_iteration(*args, _ptr(out)), "cpt_render_iteration")
+        else:
+            _check(lib().cpt_render_frames(*args, frames, _ptr(out)), "cpt_render_frames")
+        return out
+
+    def sample_counts(self):
+        out = np.zeros((self.H, self.W), np.uint32)
+        _check(lib().cpt_renderer_read_buffer(self.h, CPT_BUF_SAMPLE_N, _ptr(out)), "cpt_renderer_read_buffer")
+        return out
+
+    def reset(self):
+        _check(lib().cpt_renderer_reset(self.h), "cpt_renderer_reset")
+
+    def set_profiling(self, on: bool):
+        _check(lib().cpt_renderer_set_profiling(self.h, int(on)), "cpt_renderer_set_profiling")
+
+    def stats(self, reset: bool = False) -> dict:
+        s = _Stats()
+        _check(lib().cpt_renderer_stats(self.h, ctypes.byref(s), int(reset)), "cpt_renderer_stats")
+        return {n: getattr(s, n) for n, _ in s._fields_}
+
+
+def make_rays(o, d, max_dist=1e20, eps=1e-4) -> np.ndarray:
+    o = np.asarray(o, np.float32)
+    d = np.asarray(d, np.float32)
+    r = np.zeros(len(o), RAY_DTYPE)
+    r["o"], r["d"] = o, d
+    r["max_dist"], r["eps"] = max_dist, eps
+    return r

@@ -1,0 +1,257 @@
+// Binned-SAH BVH with NPBVHHelper semantics and its 4-wide re-layout.
+//
+// build_bvh_sah reproduces src/bvhhelper.cpp:55-311 bit for bit (same float
+// expressions, same bin boundaries, same strict-'<' cost minimum, same Hoare
+// partition with '<=' going left) but bins every triangle in one pass instead
+// of rescanning the node once per bin, and emits the CompactBVH preorder
+// directly.  Compiled with -ffp-contract=off so the host float arithmetic is
+// the reference's.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <queue>
+
+#include "cpt_internal.h"
+
+namespace cpt {
+namespace {
+
+constexpr int kBins = 16;                     // BVH_SAH_BIN_N (bvhhelper.cpp:9)
+
+struct Box {
+  float mn[3], mx[3];
+};
+inline void box_empty(Box& b) {
+  for (int k = 0; k < 3; k++) { b.mn[k] = kInfF; b.mx[k] = -kInfF; }
+}
+inline void box_grow(Box& b, const float* p) {
+  for (int k = 0; k < 3; k++) { b.mn[k] = std::fmin(b.mn[k], p[k]); b.mx[k] = std::fmax(b.mx[k], p[k]); }
+}
+inline void box_merge(Box& b, const Box& o) {
+  for (int k = 0; k < 3; k++) { b.mn[k] = std::fmin(b.mn[k], o.mn[k]); b.mx[k] = std::fmax(b.mx[k], o.mx[k]); }
+}
+// AABBSurfaceArea (bvhhelper.cpp:33-39)
+inline float surface_area(const Box& b) {
+  float dx = b.mx[0] - b.mn[0], dy = b.mx[1] - b.mn[1], dz = b.mx[2] - b.mn[2];
+  float len = std::sqrt(dx * dx + dy * dy + dz * dz);
+  if (dx < 0 || dy < 0 || dz < 0 || (double)len < kEpsilon) return 0;
+  return dx * dy + dx * dz + dy * dz;
+}
+
+struct Builder {
+  const float* cen;        // 3 per original triangle
+  const Box* tbox;         // per original triangle
+  u32* order;
+  std::vector<BinNode>* nodes;
+
+  void leaf(u32 me, u32 start, u32 n) { (*nodes)[me].a = start; (*nodes)[me].n = n; }
+
+  void build(u32 start, u32 n, u32 budget) {
+    u32 me = (u32)nodes->size();
+    nodes->push_back(BinNode());
+    BinNode& init = (*nodes)[me];
+    for (int k = 0; k < 3; k++) { init.mn[k] = kInfF; init.mx[k] = -kInfF; }
+    init.a = 0; init.n = 0;
+    if (n == 0) return;                                   // bvhhelper.cpp:58-59
+    Box nb, cb;
+    box_empty(nb); box_empty(cb);
+    for (u32 i = 0; i < n; i++) {
+      u32 t = order[start + i];
+      box_merge(nb, tbox[t]);
+      box_grow(cb, cen + (size_t)t * 3);
+    }
+    {
+      BinNode& me_n = (*nodes)[me];
+      for (int k = 0; k < 3; k++) { me_n.mn[k] = nb.mn[k]; me_n.mx[k] = nb.mx[k]; }
+    }
+    float dia[3] = {cb.mx[0] - cb.mn[0], cb.mx[1] - cb.mn[1], cb.mx[2] - cb.mn[2]};
+    if (dia[0] < 0 || dia[1] < 0 || dia[2] < 0) return;   // :74-75
+    float dlen = std::sqrt(dia[0] * dia[0] + dia[1] * dia[1] + dia[2] * dia[2]);
+    if (n == 1 || budget <= 1 || (double)dlen < kEpsilon) { leaf(me, start, n); return; }
+    int axis = 0;                                         // :80-83
+    if (std::fabs(dia[1]) > std::fabs(dia[0])) axis = 1;
+    if (std::fabs(dia[2]) > std::fabs(dia[1]) && std::fabs(dia[2]) > std::fabs(dia[0])) axis = 2;
+    float len = std::fabs(dia[axis]);
+    float c0 = cb.mn[axis];
+    // bin boundaries exactly as :93-96; bin i = [edge[i], edge[i+1])
+    float edge[kBins + 1];
+    for (int i = 0; i < kBins; i++) edge[i] = c0 + (float)i * len / (float)kBins;
+    edge[kBins] = (float)((double)cb.mx[axis] + kBiasEps);
+    u32 binN[kBins];
+    Box binB[kBins];
+    for (int i = 0; i < kBins; i++) { binN[i] = 0; box_empty(binB[i]); }
+    float scale = len > 0 ? (float)kBins / len : 0.f;
+    for (u32 i = 0; i < n; i++) {
+      u32 t = order[start + i];
+      const float* c = cen + (size_t)t * 3;
+      float p = c[axis];
+      int k = (int)((p - c0) * scale);
+      k = k < 0 ? 0 : (k > kBins - 1 ? kBins - 1 : k);
+      while (k > 0 && p < edge[k]) k--;
+      while (k < kBins - 1 && p >= edge[k + 1]) k++;
+      if (!(p >= edge[k] && p < edge[k + 1])) continue;   // outside every bin (:101)
+      binN[k]++;
+      box_grow(binB[k], c);
+    }
+    // split costs (:120-164): heu[k] = bins 0..k left, k+1..15 right
+    float nodeSA = surface_area(nb);
+    u32 preN[kBins], sufN[kBins];
+    Box preB[kBins], sufB[kBins];
+    for (int i = 0; i < kBins; i++) {
+      preN[i] = binN[i] + (i ? preN[i - 1] : 0u);
+      preB[i] = binB[i];
+      if (i) box_merge(preB[i], preB[i - 1]);
+    }
+    for (int i = kBins - 1; i >= 0; i--) {
+      sufN[i] = binN[i] + (i < kBins - 1 ? sufN[i + 1] : 0u);
+      sufB[i] = binB[i];
+      if (i < kBins - 1) box_merge(sufB[i], sufB[i + 1]);
+    }
+    float minCost = kInfF, minBin = (float)(kBins - 1);
+    for (int k = 0; k < kBins; k++) {
+      u32 nl = preN[k], nr = k < kBins - 1 ? sufN[k + 1] : 0u;
+      Box br;
+      if (k < kBins - 1) br = sufB[k + 1]; else box_empty(br);
+      float heu = 0.125f + ((float)nl * surface_area(preB[k]) + (float)nr * surface_area(br)) / nodeSA;
+      if (heu < minCost) { minCost = heu; minBin = (float)k; }
+    }
+    // partition (:190-211)
+    float split = c0 + (minBin + 1) * len / (float)kBins;
+    u32 left = start, right = start + n - 1;
+    while (left < right) {
+      while (cen[(size_t)order[left] * 3 + axis] <= split && left < right) left++;
+      while (cen[(size_t)order[right] * 3 + axis] > split && left < right) right--;
+      if (left < right) { std::swap(order[left], order[right]); left++; right--; }
+    }
+    u32 mid = (cen[(size_t)order[left] * 3 + axis] <= split) ? left : left - 1;
+    u32 nl = mid - start + 1;
+    build(start, nl, budget - 1);
+    u32 rightIdx = (u32)nodes->size();
+    build(mid + 1, n - nl, budget - 1);
+    (*nodes)[me].a = rightIdx - me;                       // CompactBVH offset (:294-297)
+    (*nodes)[me].n = 0;
+  }
+};
+
+}  // namespace
+
+bool build_bvh_sah(const float* pos3, const u32* idx3, u32 nt, u32 max_depth, BvhHost& out) {
+  out.nodes.clear();
+  out.reorder.assign(nt, 0);
+  if (nt == 0) return false;
+  std::vector<float> cen((size_t)nt * 3);
+  std::vector<Box> tb(nt);
+  for (u32 i = 0; i < nt; i++) {                          // :248-259
+    const float* a = pos3 + (size_t)idx3[i * 3] * 3;
+    const float* b = pos3 + (size_t)idx3[i * 3 + 1] * 3;
+    const float* c = pos3 + (size_t)idx3[i * 3 + 2] * 3;
+    for (int k = 0; k < 3; k++) {
+      cen[(size_t)i * 3 + k] = ((a[k] + b[k]) + c[k]) / 3.f;
+      tb[i].mn[k] = std::fmin(c[k], std::fmin(b[k], a[k]));
+      tb[i].mx[k] = std::fmax(c[k], std::fmax(b[k], a[k]));
+    }
+    out.reorder[i] = i;
+  }
+  out.nodes.reserve((size_t)nt * 2);
+  Builder bld{cen.data(), tb.data(), out.reorder.data(), &out.nodes};
+  bld.build(0, nt, max_depth);
+  return !out.nodes.empty();
+}
+
+void collapse_bvh4(const float* /*pos3*/, const u32* /*idx3*/, BvhHost& b, u32 leaf_max) {
+  const std::vector<BinNode>& bn = b.nodes;
+  u32 N = (u32)bn.size();
+  std::vector<u32> cnt(N, 0), first(N, 0);
+  auto is_empty = [&](u32 i) { return bn[i].n == 0 && bn[i].mn[0] > bn[i].mx[0]; };
+  auto is_leaf = [&](u32 i) { return bn[i].n > 0; };
+  for (u32 ii = N; ii-- > 0;) {
+    if (is_empty(ii)) { cnt[ii] = 0; first[ii] = kInvalid; continue; }
+    if (is_leaf(ii)) { cnt[ii] = bn[ii].n; first[ii] = bn[ii].a; continue; }
+    u32 l = ii + 1, r = ii + bn[ii].a;
+    cnt[ii] = cnt[l] + cnt[r];
+    first[ii] = first[l] != kInvalid ? first[l] : first[r];
+  }
+  auto leafish = [&](u32 i) { return is_leaf(i) || cnt[i] <= leaf_max; };
+  auto area = [&](u32 i) {
+    float dx = bn[i].mx[0] - bn[i].mn[0], dy = bn[i].mx[1] - bn[i].mn[1], dz = bn[i].mx[2] - bn[i].mn[2];
+    return dx * dy + dx * dz + dy * dz;
+  };
+  b.nodes4.clear();
+  b.depth4 = 0;
+  b.leaves4 = 0;
+  auto set_child = [&](Node4& n4, int c, u32 binIdx) {
+    const BinNode& s = bn[binIdx];
+    n4.lox[c] = s.mn[0]; n4.loy[c] = s.mn[1]; n4.loz[c] = s.mn[2];
+    n4.hix[c] = s.mx[0]; n4.hiy[c] = s.mx[1]; n4.hiz[c] = s.mx[2];
+  };
+  auto init4 = [](Node4& n4) {
+    for (int c = 0; c < 4; c++) {
+      n4.lox[c] = n4.loy[c] = n4.loz[c] = kInfF;
+      n4.hix[c] = n4.hiy[c] = n4.hiz[c] = -kInfF;
+      n4.child[c] = kInvalid;
+      n4.count[c] = 0;
+    }
+  };
+  if (N == 0) return;
+  // queue of (binary node, node4 slot, depth)
+  struct Item { u32 bin, slot, depth; };
+  std::queue<Item> q;
+  b.nodes4.push_back(Node4());
+  init4(b.nodes4[0]);
+  if (leafish(0)) {
+    set_child(b.nodes4[0], 0, 0);
+    b.nodes4[0].child[0] = first[0];
+    b.nodes4[0].count[0] = cnt[0];
+    b.leaves4 = 1;
+    b.depth4 = 1;
+    return;
+  }
+  q.push({0, 0, 1});
+  while (!q.empty()) {
+    Item it = q.front();
+    q.pop();
+    b.depth4 = std::max(b.depth4, it.depth);
+    u32 set[4];
+    int ns = 0;
+    u32 l = it.bin + 1, r = it.bin + bn[it.bin].a;
+    if (cnt[l]) set[ns++] = l;
+    if (cnt[r]) set[ns++] = r;
+    while (ns < 4) {
+      int best = -1;
+      float bestA = -1.f;
+      for (int i = 0; i < ns; i++)
+        if (!leafish(set[i]) && area(set[i]) > bestA) { bestA = area(set[i]); best = i; }
+      if (best < 0) break;
+      u32 x = set[best];
+      u32 xl = x + 1, xr = x + bn[x].a;
+      u32 repl[2];
+      int nr = 0;
+      if (cnt[xl]) repl[nr++] = xl;
+      if (cnt[xr]) repl[nr++] = xr;
+      set[best] = repl[0];
+      if (nr == 2) set[ns++] = repl[1];
+    }
+    Node4 n4;
+    init4(n4);
+    std::vector<Item> kids;
+    for (int c = 0; c < ns; c++) {
+      u32 x = set[c];
+      set_child(n4, c, x);
+      if (leafish(x)) {
+        n4.child[c] = first[x];
+        n4.count[c] = cnt[x];
+        b.leaves4++;
+      } else {
+        u32 slot = (u32)b.nodes4.size();
+        b.nodes4.push_back(Node4());
+        n4.child[c] = slot;
+        n4.count[c] = 0;
+        kids.push_back({x, slot, it.depth + 1});
+      }
+    }
+    b.nodes4[it.slot] = n4;
+    for (auto& k : kids) q.push(k);
+  }
+}
+
+}  // namespace cpt

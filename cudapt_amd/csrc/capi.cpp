@@ -1,0 +1,716 @@
+// C ABI of libcpt_hip.so (include/cpt.h): scene store, BVH build + upload,
+// batched traversal, and the wavefront BDPT renderer driver.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/cpt.h"
+#include "cpt_internal.h"
+#include "kernels.h"
+
+using namespace cpt;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess) return fail(CPT_E_HIP, "%s: %s", #expr, hipGetErrorString(e_)); \
+  } while (0)
+
+template <class T>
+struct DevArray {
+  T* p = nullptr;
+  size_t n = 0;
+  ~DevArray() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  hipError_t alloc(size_t count) {
+    if (count <= n && p) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc((void**)&p, sizeof(T) * (count ? count : 1));
+    if (e == hipSuccess) n = count;
+    return e;
+  }
+  hipError_t upload(const T* src, size_t count, hipStream_t st) {
+    hipError_t e = alloc(count);
+    if (e != hipSuccess || count == 0) return e;
+    e = hipMemcpyAsync(p, src, sizeof(T) * count, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(st);
+  }
+};
+
+// splitmix64 finaliser (Steele, Lea, Flood 2014) keyed by (seed, frame,
+// domain, sample): the per-frame stream key that takes the place of
+// curand_init(WangHash(frameN) + index, 0, 0) (src/cudaRTBDPTStream.cu:552,633).
+uint64_t stream_key(uint64_t seed, uint32_t frame, uint32_t domain, uint32_t sample) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(frame + 1) + ((uint64_t)sample << 32) + ((uint64_t)domain << 60);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+enum { DOMAIN_EYE = 1, DOMAIN_LIGHT = 2 };
+
+}  // namespace
+
+struct cpt_scene {
+  std::vector<float> pos, nrm, uv;
+  std::vector<u32> idx, mat;
+  std::vector<cpt_material> mats;
+  u32 nv = 0, nt = 0;
+  BvhHost bvh;
+  bool has_bvh = false;
+  bool uploaded = false;
+  u32 mesh_version = 0, mat_version = 0, leaf_max = 4;
+  hipStream_t stream = nullptr;
+  DevArray<Node4> d_nodes4;
+  DevArray<float4> d_tris, d_vpos, d_vnrm, d_bnodes;
+  DevArray<uint4> d_tri_verts;
+  DevArray<uint2> d_boff;
+  DevArray<DevMaterial> d_mats;
+  DevArray<u32> d_light;
+  DevArray<float4> d_rays, d_hits;
+  DevArray<DevStats> d_stats;
+  u32 n_light = 0;
+  DevScene dev() const {
+    DevScene s;
+    s.nodes4 = d_nodes4.p;
+    s.tris = d_tris.p;
+    s.tri_verts = d_tri_verts.p;
+    s.vpos = d_vpos.p;
+    s.vnrm = d_vnrm.p;
+    s.mats = d_mats.p;
+    s.light_tris = d_light.p;
+    s.n_light_tris = n_light;
+    s.n_tris = nt;
+    s.bnodes = d_bnodes.p;
+    s.boff = d_boff.p;
+    return s;
+  }
+  ~cpt_scene() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+int ensure_stream(hipStream_t& st) {
+  if (!st) HIP_TRY(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  return CPT_OK;
+}
+
+// updateLightTriCudaMem (src/cudaRTBDPTStream.cu:57-77) over leaf order.
+int upload_materials(cpt_scene* s) {
+  std::vector<DevMaterial> dm(s->mats.size());
+  static_assert(sizeof(DevMaterial) == sizeof(cpt_material), "material layouts agree");
+  if (!s->mats.empty()) std::memcpy(dm.data(), s->mats.data(), sizeof(cpt_material) * s->mats.size());
+  HIP_TRY(s->d_mats.upload(dm.data(), dm.size(), s->stream));
+  std::vector<u32> light;
+  for (u32 i = 0; i < s->nt; i++) {
+    u32 o = s->bvh.reorder[i];
+    u32 m = s->mat[o];
+    const float* e = s->mats[m].emissive;
+    if (std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) > 0.f) light.push_back(i);
+  }
+  s->n_light = (u32)light.size();
+  HIP_TRY(s->d_light.upload(light.data(), light.size(), s->stream));
+  return CPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* cpt_last_error(void) { return g_err.c_str(); }
+
+int cpt_device_count(int* n) {
+  if (!n) return fail(CPT_E_ARG, "cpt_device_count: null output");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  *n = e == hipSuccess ? c : 0;
+  return CPT_OK;
+}
+
+int cpt_device_alloc(void** ptr, uint64_t bytes) {
+  if (!ptr) return fail(CPT_E_ARG, "cpt_device_alloc: null output");
+  *ptr = nullptr;
+  hipError_t e = hipMalloc(ptr, bytes ? bytes : 1);
+  if (e != hipSuccess) return fail(CPT_E_NOMEM, "cpt_device_alloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e));
+  return CPT_OK;
+}
+int cpt_device_free(void* ptr) {
+  if (ptr) HIP_TRY(hipFree(ptr));
+  return CPT_OK;
+}
+int cpt_memcpy_htod(void* dst, const void* src, uint64_t bytes) {
+  if (bytes && (!dst || !src)) return fail(CPT_E_ARG, "cpt_memcpy_htod: null pointer");
+  if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return CPT_OK;
+}
+int cpt_memcpy_dtoh(void* dst, const void* src, uint64_t bytes) {
+  if (bytes && (!dst || !src)) return fail(CPT_E_ARG, "cpt_memcpy_dtoh: null pointer");
+  if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return CPT_OK;
+}
+int cpt_device_synchronize(void) {
+  HIP_TRY(hipDeviceSynchronize());
+  return CPT_OK;
+}
+
+int cpt_scene_create(cpt_scene** out) {
+  if (!out) return fail(CPT_E_ARG, "cpt_scene_create: null output");
+  *out = new cpt_scene();
+  return CPT_OK;
+}
+
+int cpt_scene_set_mesh(cpt_scene* s, const float* pos3, const float* nrm3, const float* uv2, uint32_t nv,
+                       const uint32_t* idx3, const uint32_t* mat, uint32_t nt) {
+  if (!s || (!pos3 && nv) || (!idx3 && nt)) return fail(CPT_E_ARG, "cpt_scene_set_mesh: null argument");
+  for (size_t i = 0; i < (size_t)nt * 3; i++)
+    if (idx3[i] >= nv) return fail(CPT_E_ARG, "cpt_scene_set_mesh: index %u out of range (nv=%u)", idx3[i], nv);
+  s->pos.assign(pos3, pos3 + (size_t)nv * 3);
+  if (nrm3) s->nrm.assign(nrm3, nrm3 + (size_t)nv * 3); else s->nrm.assign((size_t)nv * 3, 0.f);
+  if (uv2) s->uv.assign(uv2, uv2 + (size_t)nv * 2); else s->uv.assign((size_t)nv * 2, 0.f);
+  s->idx.assign(idx3, idx3 + (size_t)nt * 3);
+  if (mat) s->mat.assign(mat, mat + nt); else s->mat.assign(nt, 0u);
+  s->nv = nv;
+  s->nt = nt;
+  s->has_bvh = false;
+  s->uploaded = false;
+  s->mesh_version++;
+  return CPT_OK;
+}
+
+int cpt_scene_set_materials(cpt_scene* s, const cpt_material* m, uint32_t n) {
+  if (!s || (!m && n)) return fail(CPT_E_ARG, "cpt_scene_set_materials: null argument");
+  s->mats.assign(m, m + n);
+  return CPT_OK;
+}
+
+int cpt_scene_update_materials(cpt_scene* s) {
+  if (!s) return fail(CPT_E_ARG, "cpt_scene_update_materials: null scene");
+  if (!s->uploaded) return fail(CPT_E_STATE, "cpt_scene_update_materials: scene not uploaded");
+  for (u32 m : s->mat)
+    if (m >= s->mats.size()) return fail(CPT_E_ARG, "cpt_scene_update_materials: material %u missing", m);
+  int rc = upload_materials(s);
+  if (rc) return rc;
+  s->mat_version++;
+  return CPT_OK;
+}
+
+int cpt_bvh_build_sah(cpt_scene* s, uint32_t max_depth) {
+  if (!s) return fail(CPT_E_ARG, "cpt_bvh_build_sah: null scene");
+  if (s->nt == 0) return fail(CPT_E_STATE, "cpt_bvh_build_sah: empty mesh");
+  if (max_depth == 0 || max_depth > 64) return fail(CPT_E_ARG, "cpt_bvh_build_sah: max_depth %u not in 1..64", max_depth);
+  if (!build_bvh_sah(s->pos.data(), s->idx.data(), s->nt, max_depth, s->bvh))
+    return fail(CPT_E_STATE, "cpt_bvh_build_sah: build failed");
+  collapse_bvh4(s->pos.data(), s->idx.data(), s->bvh, s->leaf_max);
+  if (s->bvh.depth4 > 31) return fail(CPT_E_LIMIT, "cpt_bvh_build_sah: 4-wide depth %u exceeds 31", s->bvh.depth4);
+  s->has_bvh = true;
+  s->uploaded = false;
+  return CPT_OK;
+}
+
+int cpt_bvh_export(cpt_scene* s, float* bounds6, uint32_t* off2, uint32_t* reorder, uint32_t* node_n) {
+  if (!s) return fail(CPT_E_ARG, "cpt_bvh_export: null scene");
+  if (!s->has_bvh) return fail(CPT_E_STATE, "cpt_bvh_export: no BVH");
+  const auto& nodes = s->bvh.nodes;
+  if (node_n) *node_n = (u32)nodes.size();
+  for (size_t i = 0; i < nodes.size(); i++) {
+    if (bounds6)
+      for (int k = 0; k < 3; k++) { bounds6[i * 6 + k] = nodes[i].mn[k]; bounds6[i * 6 + 3 + k] = nodes[i].mx[k]; }
+    if (off2) { off2[i * 2] = nodes[i].a; off2[i * 2 + 1] = nodes[i].n; }
+  }
+  if (reorder) std::memcpy(reorder, s->bvh.reorder.data(), sizeof(u32) * s->nt);
+  return CPT_OK;
+}
+
+int cpt_bvh4_info(cpt_scene* s, uint32_t* node_n, uint32_t* depth, uint32_t* leaf_n) {
+  if (!s) return fail(CPT_E_ARG, "cpt_bvh4_info: null scene");
+  if (!s->has_bvh) return fail(CPT_E_STATE, "cpt_bvh4_info: no BVH");
+  if (node_n) *node_n = (u32)s->bvh.nodes4.size();
+  if (depth) *depth = s->bvh.depth4;
+  if (leaf_n) *leaf_n = s->bvh.leaves4;
+  return CPT_OK;
+}
+
+int cpt_scene_upload(cpt_scene* s) {
+  if (!s) return fail(CPT_E_ARG, "cpt_scene_upload: null scene");
+  if (!s->has_bvh) return fail(CPT_E_STATE, "cpt_scene_upload: build the BVH first");
+  for (u32 m : s->mat)
+    if (m >= s->mats.size()) return fail(CPT_E_ARG, "cpt_scene_upload: material %u missing", m);
+  int rc = ensure_stream(s->stream);
+  if (rc) return rc;
+  const u32 nt = s->nt;
+  std::vector<float4> tris((size_t)nt * 3);
+  std::vector<uint4> tv(nt);
+  for (u32 i = 0; i < nt; i++) {                // src/cudaRTCommon.cu:503-514, host FP32 edges
+    u32 o = s->bvh.reorder[i];
+    const float* a = &s->pos[(size_t)s->idx[o * 3] * 3];
+    const float* b = &s->pos[(size_t)s->idx[o * 3 + 1] * 3];
+    const float* c = &s->pos[(size_t)s->idx[o * 3 + 2] * 3];
+    float ow;
+    std::memcpy(&ow, &o, 4);
+    tris[i * 3] = make_float4(a[0], a[1], a[2], ow);
+    tris[i * 3 + 1] = make_float4(b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.f);
+    tris[i * 3 + 2] = make_float4(c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.f);
+    tv[i] = make_uint4(s->idx[o * 3], s->idx[o * 3 + 1], s->idx[o * 3 + 2], s->mat[o]);
+  }
+  std::vector<float4> vp(s->nv), vn(s->nv);
+  for (u32 i = 0; i < s->nv; i++) {
+    vp[i] = make_float4(s->pos[i * 3], s->pos[i * 3 + 1], s->pos[i * 3 + 2], 0.f);
+    vn[i] = make_float4(s->nrm[i * 3], s->nrm[i * 3 + 1], s->nrm[i * 3 + 2], 0.f);
+  }
+  const auto& bn = s->bvh.nodes;
+  std::vector<float4> bnodes(bn.size() * 2);
+  std::vector<uint2> boff(bn.size());
+  for (size_t i = 0; i < bn.size(); i++) {
+    bnodes[i * 2] = make_float4(bn[i].mn[0], bn[i].mn[1], bn[i].mn[2], 0.f);
+    bnodes[i * 2 + 1] = make_float4(bn[i].mx[0], bn[i].mx[1], bn[i].mx[2], 0.f);
+    boff[i] = make_uint2(bn[i].a, bn[i].n);
+  }
+  HIP_TRY(s->d_nodes4.upload(s->bvh.nodes4.data(), s->bvh.nodes4.size(), s->stream));
+  HIP_TRY(s->d_tris.upload(tris.data(), tris.size(), s->stream));
+  HIP_TRY(s->d_tri_verts.upload(tv.data(), tv.size(), s->stream));
+  HIP_TRY(s->d_vpos.upload(vp.data(), vp.size(), s->stream));
+  HIP_TRY(s->d_vnrm.upload(vn.data(), vn.size(), s->stream));
+  HIP_TRY(s->d_bnodes.upload(bnodes.data(), bnodes.size(), s->stream));
+  HIP_TRY(s->d_boff.upload(boff.data(), boff.size(), s->stream));
+  rc = upload_materials(s);
+  if (rc) return rc;
+  if (!s->d_stats.p) {
+    HIP_TRY(s->d_stats.alloc(1));
+    HIP_TRY(hipMemsetAsync(s->d_stats.p, 0, sizeof(DevStats), s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+  }
+  s->uploaded = true;
+  s->mat_version++;
+  return CPT_OK;
+}
+
+int cpt_scene_destroy(cpt_scene* s) {
+  if (!s) return fail(CPT_E_ARG, "cpt_scene_destroy: null scene");
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  delete s;
+  return CPT_OK;
+}
+
+int cpt_trace_closest_device(cpt_scene* s, const cpt_ray* d_rays, uint32_t n, uint32_t cull, cpt_hit* d_hits,
+                             void* stream, uint32_t layout) {
+  if (!s || (n && (!d_rays || !d_hits))) return fail(CPT_E_ARG, "cpt_trace_closest_device: null argument");
+  if (!s->uploaded) return fail(CPT_E_STATE, "cpt_trace_closest_device: scene not uploaded");
+  if (layout > 1) return fail(CPT_E_ARG, "cpt_trace_closest_device: layout %u not 0 or 1", layout);
+  hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+  HIP_TRY(launch_trace(s->dev(), (const float4*)d_rays, n, nullptr, cull, (float4*)d_hits, 1, 0, layout,
+                       s->d_stats.p, st));
+  if (!stream) HIP_TRY(hipStreamSynchronize(st));
+  return CPT_OK;
+}
+
+int cpt_trace_occluded_device(cpt_scene* s, const cpt_ray* d_rays, uint32_t n, uint32_t cull, uint32_t* d_occluded,
+                              void* stream) {
+  if (!s || (n && (!d_rays || !d_occluded))) return fail(CPT_E_ARG, "cpt_trace_occluded_device: null argument");
+  if (!s->uploaded) return fail(CPT_E_STATE, "cpt_trace_occluded_device: scene not uploaded");
+  hipStream_t st = stream ? (hipStream_t)stream : s->stream;
+  HIP_TRY(launch_trace(s->dev(), (const float4*)d_rays, n, nullptr, cull, (float4*)d_occluded, 0, 1, 0, s->d_stats.p,
+                       st));
+  if (!stream) HIP_TRY(hipStreamSynchronize(st));
+  return CPT_OK;
+}
+
+int cpt_trace_closest(cpt_scene* s, const cpt_ray* rays, uint32_t n, uint32_t cull, cpt_hit* hits) {
+  if (!s || (n && (!rays || !hits))) return fail(CPT_E_ARG, "cpt_trace_closest: null argument");
+  if (!s->uploaded) return fail(CPT_E_STATE, "cpt_trace_closest: scene not uploaded");
+  if (n == 0) return CPT_OK;
+  HIP_TRY(s->d_rays.alloc((size_t)n * 2));
+  HIP_TRY(s->d_hits.alloc(n));
+  HIP_TRY(hipMemcpyAsync(s->d_rays.p, rays, sizeof(cpt_ray) * n, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(launch_trace(s->dev(), s->d_rays.p, n, nullptr, cull, s->d_hits.p, 1, 0, 0, s->d_stats.p, s->stream));
+  HIP_TRY(hipMemcpyAsync(hits, s->d_hits.p, sizeof(cpt_hit) * n, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return CPT_OK;
+}
+
+int cpt_trace_counters(cpt_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* rays, int reset) {
+  if (!s) return fail(CPT_E_ARG, "cpt_trace_counters: null scene");
+  if (!s->d_stats.p) return fail(CPT_E_STATE, "cpt_trace_counters: scene not uploaded");
+  DevStats h;
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  HIP_TRY(hipMemcpy(&h, s->d_stats.p, sizeof h, hipMemcpyDeviceToHost));
+  if (nodes) *nodes = h.ext_nodes;
+  if (tris) *tris = h.ext_tris;
+  if (rays) *rays = h.ext_rays;
+  if (reset) HIP_TRY(hipMemset(s->d_stats.p, 0, sizeof(DevStats)));
+  return CPT_OK;
+}
+
+}  // extern "C"
+
+// ============================================================== renderer
+
+namespace {
+
+struct Attrib {
+  const char* name;
+  int type;  // 0 uint, 1 int, 2 bool, 3 float, 4 enum
+  double value;
+};
+
+struct PathSet {
+  DevArray<float4> ray_o, ray_d, mul, smp, acc;
+  DevArray<u32> pix, depth;
+  DevArray<uint64_t> rng;
+  hipError_t alloc(size_t n) {
+    hipError_t e;
+    if ((e = ray_o.alloc(n)) || (e = ray_d.alloc(n)) || (e = mul.alloc(n)) || (e = smp.alloc(n)) ||
+        (e = acc.alloc(n)) || (e = pix.alloc(n)) || (e = depth.alloc(n)) || (e = rng.alloc(n)))
+      return e;
+    return hipSuccess;
+  }
+  PathBuf view() {
+    PathBuf b;
+    b.ray_o = ray_o.p; b.ray_d = ray_d.p; b.pix = pix.p; b.depth = depth.p;
+    b.mul = mul.p; b.smp = smp.p; b.acc = acc.p; b.rng = rng.p;
+    return b;
+  }
+};
+
+enum { ATTR_EYE_DEPTH = 0, ATTR_LIGHT_DEPTH, ATTR_LIGHT_PATHS, ATTR_SEED, ATTR_LITERAL_SHADOW, ATTR_N };
+
+}  // namespace
+
+struct cpt_renderer {
+  int mode = 0;
+  u32 W = 0, H = 0;
+  std::vector<Attrib> attribs;
+  hipStream_t stream = nullptr;
+  PathSet A, B;
+  DevArray<float4> hits;
+  DevArray<float4> sh_o, sh_d, sh_c;
+  DevArray<LightVert> lv;
+  DevArray<FrameCounters> fc;
+  DevArray<DevStats> stats;
+  DevArray<float> result;
+  DevArray<u32> sample_n;
+  bool allocated = false;
+  u32 frame = 0;
+  bool have_cam = false;
+  float cam[9] = {0};
+  const cpt_scene* scene = nullptr;
+  u32 scene_mesh_version = 0, scene_mat_version = 0;
+  bool profiling = false;
+  struct Ev { hipEvent_t a, b; int kind; };
+  std::vector<Ev> events;          // recorded pairs awaiting readback
+  std::vector<hipEvent_t> pool;    // reusable events
+  double ms[5] = {0, 0, 0, 0, 0};  // extend, shadow, shade, light, total
+  ~cpt_renderer() {
+    for (auto& e : events) { pool.push_back(e.a); pool.push_back(e.b); }
+    for (auto e : pool) (void)hipEventDestroy(e);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  hipError_t take(hipEvent_t* e) {
+    if (!pool.empty()) { *e = pool.back(); pool.pop_back(); return hipSuccess; }
+    return hipEventCreate(e);
+  }
+  double attr(int i) const { return attribs[i].value; }
+};
+
+namespace {
+
+int renderer_alloc(cpt_renderer* r) {
+  size_t n = (size_t)r->W * r->H;
+  HIP_TRY(r->A.alloc(n));
+  HIP_TRY(r->B.alloc(n));
+  HIP_TRY(r->hits.alloc(n));
+  HIP_TRY(r->sh_o.alloc(n));
+  HIP_TRY(r->sh_d.alloc(n));
+  HIP_TRY(r->sh_c.alloc(n));
+  HIP_TRY(r->fc.alloc(1));
+  HIP_TRY(r->stats.alloc(1));
+  HIP_TRY(r->result.alloc(n * 3));
+  HIP_TRY(r->sample_n.alloc(n));
+  HIP_TRY(hipMemsetAsync(r->stats.p, 0, sizeof(DevStats), r->stream));
+  HIP_TRY(hipMemsetAsync(r->result.p, 0, sizeof(float) * n * 3, r->stream));
+  HIP_TRY(hipMemsetAsync(r->sample_n.p, 0, sizeof(u32) * n, r->stream));
+  HIP_TRY(hipMemsetAsync(r->fc.p, 0, sizeof(FrameCounters), r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  r->allocated = true;
+  return CPT_OK;
+}
+
+int lv_alloc(cpt_renderer* r) {
+  size_t need = (size_t)r->attr(ATTR_LIGHT_PATHS) * ((size_t)r->attr(ATTR_LIGHT_DEPTH) + 1);
+  HIP_TRY(r->lv.alloc(need));
+  return CPT_OK;
+}
+
+hipError_t timed_begin(cpt_renderer* r, int kind, hipEvent_t* a) {
+  if (!r->profiling) return hipSuccess;
+  hipError_t e = r->take(a);
+  if (e) return e;
+  (void)kind;
+  return hipEventRecord(*a, r->stream);
+}
+hipError_t timed_end(cpt_renderer* r, int kind, hipEvent_t a) {
+  if (!r->profiling) return hipSuccess;
+  hipEvent_t b;
+  hipError_t e = r->take(&b);
+  if (e) return e;
+  e = hipEventRecord(b, r->stream);
+  r->events.push_back({a, b, kind});
+  return e;
+}
+
+// Camera basis of src/cudaRTBDPTStream.cu:823-824 (host float Vec3 math).
+void camera_basis(const float* d, const float* up, float* right, float* up2) {
+  float rx = d[1] * up[2] - d[2] * up[1], ry = d[2] * up[0] - d[0] * up[2], rz = d[0] * up[1] - d[1] * up[0];
+  float rl = std::sqrt(rx * rx + ry * ry + rz * rz);
+  right[0] = rx / rl; right[1] = ry / rl; right[2] = rz / rl;
+  float ux = right[1] * d[2] - right[2] * d[1], uy = right[2] * d[0] - right[0] * d[2], uz = right[0] * d[1] - right[1] * d[0];
+  float ul = std::sqrt(ux * ux + uy * uy + uz * uz);
+  up2[0] = ux / ul; up2[1] = uy / ul; up2[2] = uz / ul;
+}
+
+int render_one_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
+  DevScene sc = s->dev();
+  const u32 n = r->W * r->H;
+  hipEvent_t ev = nullptr;
+  if (fa.frame % 3 == 0) {                        // light paths every third frame (:875)
+    HIP_TRY(timed_begin(r, 3, &ev));
+    HIP_TRY(launch_light_pass(sc, fa, r->lv.p, r->fc.p, r->stream));
+    HIP_TRY(timed_end(r, 3, ev));
+  }
+  HIP_TRY(launch_frame_begin(r->fc.p, n, r->stream));
+  HIP_TRY(launch_gen_eye(fa, r->A.view(), r->stream));
+  PathSet* cur = &r->A;
+  PathSet* nxt = &r->B;
+  ShadowBuf sb;
+  sb.ray_o = r->sh_o.p; sb.ray_d = r->sh_d.p; sb.contrib = r->sh_c.p;
+  for (u32 b = 0; b < fa.eye_depth; b++) {
+    HIP_TRY(timed_begin(r, 0, &ev));
+    HIP_TRY(launch_extend(sc, cur->view(), r->fc.p, (int)b, n, r->hits.p, r->stats.p, r->stream));
+    HIP_TRY(timed_end(r, 0, ev));
+    HIP_TRY(timed_begin(r, 2, &ev));
+    HIP_TRY(launch_shade(sc, fa, cur->view(), nxt->view(), r->hits.p, r->fc.p, (int)b, n, sb, r->lv.p, r->result.p,
+                         r->sample_n.p, r->stream));
+    HIP_TRY(timed_end(r, 2, ev));
+    if (b + 1 < fa.eye_depth) {
+      HIP_TRY(timed_begin(r, 1, &ev));
+      HIP_TRY(launch_shadow(sc, sb, nxt->view(), r->fc.p, (int)b, n, r->stream));
+      HIP_TRY(timed_end(r, 1, ev));
+    }
+    std::swap(cur, nxt);
+  }
+  HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, n, fa.eye_depth, r->stream));
+  return CPT_OK;
+}
+
+int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* dir, const float* up, float fov,
+                  uint32_t nframes, float* result) {
+  if (!r || !s || !pos || !dir || !up) return fail(CPT_E_ARG, "render: null argument");
+  if (r->mode != CPT_MODE_BDPT_STREAM) return fail(CPT_E_ARG, "render: mode %d not available", r->mode);
+  if (!s->has_bvh) return fail(CPT_E_STATE, "render: scene has no BVH");
+  if (!s->uploaded) {                              // GetIsCudaDirty -> initAllSceneCudaMem (:830-838)
+    int rc = cpt_scene_upload(s);
+    if (rc) return rc;
+  }
+  u32 eye_depth = (u32)r->attr(ATTR_EYE_DEPTH), light_depth = (u32)r->attr(ATTR_LIGHT_DEPTH);
+  u32 light_paths = (u32)r->attr(ATTR_LIGHT_PATHS);
+  if (eye_depth < 1 || eye_depth > (u32)kMaxDepth) return fail(CPT_E_LIMIT, "render: eye depth %u not in 1..15", eye_depth);
+  if (light_depth < 1 || light_depth > 64) return fail(CPT_E_LIMIT, "render: light depth %u not in 1..64", light_depth);
+  if (light_paths < 1 || light_paths > (u32)kBlock) return fail(CPT_E_LIMIT, "render: light paths %u not in 1..256", light_paths);
+  if (!r->allocated) {
+    int rc = renderer_alloc(r);
+    if (rc) return rc;
+  }
+  int rc = lv_alloc(r);
+  if (rc) return rc;
+  float cam[9] = {pos[0], pos[1], pos[2], dir[0], dir[1], dir[2], up[0], up[1], up[2]};
+  bool reset = !r->have_cam || std::memcmp(cam, r->cam, sizeof cam) != 0 || r->scene != s ||
+               r->scene_mesh_version != s->mesh_version || r->scene_mat_version != s->mat_version;
+  std::memcpy(r->cam, cam, sizeof cam);
+  r->have_cam = true;
+  r->scene = s;
+  r->scene_mesh_version = s->mesh_version;
+  r->scene_mat_version = s->mat_version;
+  FrameArgs fa;
+  std::memset(&fa, 0, sizeof fa);
+  fa.width = r->W;
+  fa.height = r->H;
+  for (int k = 0; k < 3; k++) { fa.cam_pos[k] = pos[k]; fa.cam_dir[k] = dir[k]; }
+  camera_basis(dir, up, fa.cam_right, fa.cam_up);
+  fa.fov = fov;
+  fa.eye_depth = eye_depth;
+  fa.light_depth = light_depth;
+  fa.light_paths = light_paths;
+  fa.flags = r->attr(ATTR_LITERAL_SHADOW) != 0 ? 1u : 0u;
+  uint64_t seed = (uint64_t)r->attr(ATTR_SEED);
+  hipEvent_t ev_all = nullptr;
+  HIP_TRY(timed_begin(r, 4, &ev_all));
+  for (u32 f = 0; f < nframes; f++) {
+    if (reset && f == 0) r->frame = 0;
+    else r->frame++;
+    fa.frame = r->frame;
+    fa.eye_key = stream_key(seed, fa.frame, DOMAIN_EYE, 0);
+    fa.light_key = stream_key(seed, fa.frame, DOMAIN_LIGHT, 0);
+    rc = render_one_frame(r, s, fa);
+    if (rc) return rc;
+  }
+  HIP_TRY(timed_end(r, 4, ev_all));
+  if (result)
+    HIP_TRY(hipMemcpyAsync(result, r->result.p, sizeof(float) * 3 * r->W * r->H, hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return CPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t height) {
+  if (!out) return fail(CPT_E_ARG, "cpt_renderer_create: null output");
+  if (mode != CPT_MODE_BDPT_STREAM && mode != CPT_MODE_BDPT_ADAPTIVE_PLL)
+    return fail(CPT_E_ARG, "cpt_renderer_create: unknown mode %d", mode);
+  if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 26))
+    return fail(CPT_E_LIMIT, "cpt_renderer_create: %ux%u outside 1..2^26 pixels", width, height);
+  cpt_renderer* r = new cpt_renderer();
+  r->mode = mode;
+  r->W = width;
+  r->H = height;
+  r->attribs = {{"Eye Depth", 0, 5}, {"Light Depth", 0, 5}, {"Light Paths", 0, 128},
+                {"Seed", 0, (double)0x5EED0001u}, {"Literal Shadow Endpoint", 2, 0}};
+  int rc = ensure_stream(r->stream);
+  if (rc) { delete r; return rc; }
+  *out = r;
+  return CPT_OK;
+}
+
+int cpt_render_iteration(cpt_renderer* r, cpt_scene* s, const float cam_pos[3], const float cam_dir[3],
+                         const float cam_up[3], float fov, float* result) {
+  return render_frames(r, s, cam_pos, cam_dir, cam_up, fov, 1, result);
+}
+
+int cpt_render_frames(cpt_renderer* r, cpt_scene* s, const float cam_pos[3], const float cam_dir[3],
+                      const float cam_up[3], float fov, uint32_t n_frames, float* result) {
+  if (n_frames == 0) return fail(CPT_E_ARG, "cpt_render_frames: n_frames = 0");
+  return render_frames(r, s, cam_pos, cam_dir, cam_up, fov, n_frames, result);
+}
+
+int cpt_renderer_attrib_count(cpt_renderer* r, uint32_t* n) {
+  if (!r || !n) return fail(CPT_E_ARG, "cpt_renderer_attrib_count: null argument");
+  *n = (u32)r->attribs.size();
+  return CPT_OK;
+}
+
+int cpt_renderer_get_attrib(cpt_renderer* r, uint32_t idx, char* name, uint32_t name_cap, int* type, double* value) {
+  if (!r) return fail(CPT_E_ARG, "cpt_renderer_get_attrib: null renderer");
+  if (idx >= r->attribs.size()) return fail(CPT_E_ARG, "cpt_renderer_get_attrib: index %u out of range", idx);
+  const Attrib& a = r->attribs[idx];
+  if (name && name_cap) { std::strncpy(name, a.name, name_cap - 1); name[name_cap - 1] = 0; }
+  if (type) *type = a.type;
+  if (value) *value = a.value;
+  return CPT_OK;
+}
+
+int cpt_renderer_set_attrib(cpt_renderer* r, uint32_t idx, double value) {
+  if (!r) return fail(CPT_E_ARG, "cpt_renderer_set_attrib: null renderer");
+  if (idx >= r->attribs.size()) return fail(CPT_E_ARG, "cpt_renderer_set_attrib: index %u out of range", idx);
+  if (!(value >= 0) || value > 1.8e19) return fail(CPT_E_ARG, "cpt_renderer_set_attrib: value out of range");
+  r->attribs[idx].value = value;
+  r->have_cam = false;                             // parameters change the estimator: restart
+  return CPT_OK;
+}
+
+int cpt_renderer_reset(cpt_renderer* r) {
+  if (!r) return fail(CPT_E_ARG, "cpt_renderer_reset: null renderer");
+  r->have_cam = false;
+  return CPT_OK;
+}
+
+int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out) {
+  if (!r || !out) return fail(CPT_E_ARG, "cpt_renderer_read_buffer: null argument");
+  if (!r->allocated) return fail(CPT_E_STATE, "cpt_renderer_read_buffer: nothing rendered yet");
+  size_t n = (size_t)r->W * r->H;
+  switch (which) {
+    case CPT_BUF_RESULT:
+      HIP_TRY(hipMemcpyAsync(out, r->result.p, n * 12, hipMemcpyDeviceToHost, r->stream));
+      break;
+    case CPT_BUF_SAMPLE_N:
+      HIP_TRY(hipMemcpyAsync(out, r->sample_n.p, n * 4, hipMemcpyDeviceToHost, r->stream));
+      break;
+    default:
+      return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d not produced by this mode", which);
+  }
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return CPT_OK;
+}
+
+int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset) {
+  if (!r || !out) return fail(CPT_E_ARG, "cpt_renderer_stats: null argument");
+  std::memset(out, 0, sizeof *out);
+  if (!r->allocated) return CPT_OK;
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  DevStats h;
+  HIP_TRY(hipMemcpy(&h, r->stats.p, sizeof h, hipMemcpyDeviceToHost));
+  for (auto& e : r->events) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, e.a, e.b));
+    r->ms[e.kind] += ms;
+    r->pool.push_back(e.a);
+    r->pool.push_back(e.b);
+  }
+  r->events.clear();
+  out->extension_rays = h.ext_rays;
+  out->shadow_rays = h.shadow_rays;
+  out->eye_paths = h.eye_paths;
+  out->frames = h.frames;
+  out->extend_nodes = h.ext_nodes;
+  out->extend_tris = h.ext_tris;
+  out->extend_launches = h.frames * (uint64_t)r->attr(ATTR_EYE_DEPTH);
+  out->extend_ms = r->ms[0];
+  out->shadow_ms = r->ms[1];
+  out->shade_ms = r->ms[2];
+  out->light_ms = r->ms[3];
+  out->total_ms = r->ms[4];
+  if (reset) {
+    HIP_TRY(hipMemset(r->stats.p, 0, sizeof(DevStats)));
+    for (double& m : r->ms) m = 0;
+  }
+  return CPT_OK;
+}
+
+int cpt_renderer_set_profiling(cpt_renderer* r, int on) {
+  if (!r) return fail(CPT_E_ARG, "cpt_renderer_set_profiling: null renderer");
+  r->profiling = on != 0;
+  return CPT_OK;
+}
+
+int cpt_renderer_destroy(cpt_renderer* r) {
+  if (!r) return fail(CPT_E_ARG, "cpt_renderer_destroy: null renderer");
+  if (r->stream) (void)hipStreamSynchronize(r->stream);
+  delete r;
+  return CPT_OK;
+}
+
+}  // extern "C"

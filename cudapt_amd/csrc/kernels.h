@@ -1,0 +1,104 @@
+// Host-visible launch interface of the gfx950 kernels (kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "cpt_internal.h"
+
+namespace cpt {
+
+// Material record: RTMaterial field order (include/raytracer.h:82-100).
+struct DevMaterial {
+  float diffuse[3];
+  float emissive[3];
+  int32_t diffuseTexId, normalTexId, emissiveTexId, matType;
+  float transparency, subsurface, specular, metallic, roughness, anisotropic;
+  float sheen, sheenTint, clearcoat, clearcoatGloss;
+};
+
+// Device-resident scene.  Triangles are in BVH leaf order.
+struct DevScene {
+  const Node4* nodes4;         // 128-byte 4-wide nodes
+  const float4* tris;          // 3 per triangle: p0 (w = original id bits), e0 = p1-p0, e1 = p2-p0
+  const uint4* tri_verts;      // v0, v1, v2, material
+  const float4* vpos;          // vertex positions
+  const float4* vnrm;          // vertex normals
+  const DevMaterial* mats;
+  const uint32_t* light_tris;  // leaf-order emissive triangles
+  uint32_t n_light_tris;
+  uint32_t n_tris;
+  const float4* bnodes;        // reference binary layout: 2 per node (min, max)
+  const uint2* boff;           //   (offset-or-triStart, triN)
+};
+
+// Per-path SoA state of the wavefront (one slot per live eye path).
+struct PathBuf {
+  float4* ray_o;     // origin xyz, w = max distance (1e20)
+  float4* ray_d;     // direction xyz, w = ray epsilon (1e-4)
+  uint32_t* pix;     // pixel index
+  uint32_t* depth;   // bounce count
+  float4* mul;       // path throughput (pathOutMulTerm)
+  float4* smp;       // emitted radiance gathered along the path (pathSample)
+  float4* acc;       // xyz accumulated sample, w = sample count bits
+  uint64_t* rng;     // PCG32 state
+};
+
+// Shadow (connection) rays: segment + the contribution to add when visible.
+struct ShadowBuf {
+  float4* ray_o;     // origin, w = max distance
+  float4* ray_d;     // direction, w = epsilon
+  float4* contrib;   // xyz contribution, w = target slot bits
+};
+
+// Light-vertex cache entry (LightVertex, src/cudaRTBDPTStream.cu:19-36).
+struct LightVert {
+  float pos[3], norm[3], irrad[3], irradDir[3], diff[3], emissive[3];
+  float specular, metallic, roughness, pad[3];
+};
+static_assert(sizeof(LightVert) == 96, "LightVert is 96 bytes");
+
+struct FrameArgs {
+  uint32_t width, height;
+  float cam_pos[3], cam_dir[3], cam_up[3], cam_right[3];
+  float fov;
+  uint32_t frame;        // frame counter since the last reset
+  uint64_t eye_key;      // PCG32 initstate for eye paths of this frame
+  uint64_t light_key;    // PCG32 initstate for light paths
+  uint32_t eye_depth, light_depth, light_paths;
+  uint32_t flags;        // bit 0: literal shadow endpoint dist - eps
+};
+
+// Device counters of one frame (zeroed by frame_begin).
+struct FrameCounters {
+  uint32_t active[16];   // live paths entering bounce b
+  uint32_t shadow[16];   // shadow rays emitted at bounce b
+  uint32_t light_n;      // light-vertex cache size
+  uint32_t pad[31];
+};
+
+// Cumulative device statistics.
+struct DevStats {
+  unsigned long long ext_rays, shadow_rays, eye_paths, frames;
+  unsigned long long ext_nodes, ext_tris;
+  unsigned long long pad[2];
+};
+
+constexpr int kMaxDepth = 15;
+constexpr int kBlock = 256;
+
+// ---- launches -------------------------------------------------------------
+hipError_t launch_trace(const DevScene& sc, const float4* rays, uint32_t n, const uint32_t* count, uint32_t cull,
+                        float4* hits_out, int out_orig, int any_hit, uint32_t layout, DevStats* stats, hipStream_t st);
+hipError_t launch_frame_begin(FrameCounters* fc, uint32_t n_paths, hipStream_t st);
+hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, LightVert* lv, FrameCounters* fc, hipStream_t st);
+hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st);
+hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, uint32_t max_n,
+                         float4* hits, DevStats* stats, hipStream_t st);
+hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, PathBuf nxt, const float4* hits,
+                        FrameCounters* fc, int bounce, uint32_t max_n, ShadowBuf sb, const LightVert* lv,
+                        float* result, uint32_t* sample_n, hipStream_t st);
+hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
+                         uint32_t max_n, hipStream_t st);
+hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t n_paths, uint32_t depth, hipStream_t st);
+
+}  // namespace cpt

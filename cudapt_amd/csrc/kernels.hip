@@ -1,0 +1,745 @@
+// gfx950 kernels of the streaming-BDPT hot path.
+//
+//  traverse4      closest/any hit over 128-byte 4-wide nodes, ordered
+//                 near-first, stack top 16 entries in LDS (column layout, no
+//                 bank conflicts), deeper entries in private memory.
+//                 Replaces TracePrimitive (src/cudaRTCommon.cu:66-127).
+//  traverse_bin   the same query over the reference's binary CompactBVH
+//                 layout (src/cudaRTCommon.cu:527-541), kept as the baseline.
+//  k_light_pass   light-vertex cache: pt_genLightPathQueue_kernel +
+//                 pt_traceLight_kernel loop (src/cudaRTBDPTStream.cu:180-363,
+//                 545-617, 875-906) in one workgroup; stream order of the
+//                 reference's stable compaction from a workgroup prefix sum.
+//  k_gen_eye      pt_genPathQueue_kernel (src/cudaRTBDPTStream.cu:619-640).
+//  k_extend       closest hit of every live path (SoA queue).
+//  k_shade        pt_traceSample_kernel body + connection set-up of
+//                 pt_connectEyeLightPath_kernel + pt_applyPathQueueResult_kernel
+//                 for terminated paths (src/cudaRTBDPTStream.cu:365-543,
+//                 665-778); survivors and shadow rays compacted with a wave64
+//                 ballot + per-workgroup prefix, one atomic per workgroup.
+//  k_shadow       occlusion of the connection segment and the N += 4 update.
+//
+// Hit semantics: the closest hit is the minimum of (t, original triangle id)
+// over all triangles with 0 <= t < maxDist, each tested with IntersectTri's
+// exact arithmetic (src/cudaRTCommon.cu:42-64).  Box culling is conservative
+// (Ize 2013 slab widening + a slack covering IntersectTri's eps origin shift),
+// so the answer does not depend on the node layout or visiting order.
+#include <hip/hip_runtime.h>
+
+#include "devmath.h"
+#include "kernels.h"
+
+namespace cpt {
+
+#define CPT_LDS_STACK 16
+#define CPT_PRIV_STACK 80
+
+// Smallest float >= 1e-9: float |det| < it  <=>  (double)|det| < M_EPSILON.
+__device__ constexpr float kDetEps = 0x1.12e0c0p-30f;
+__device__ constexpr float kPiF = 3.14159265358979323846f;
+constexpr double kPiD = 3.14159265358979323846;
+
+struct TraceHit {
+  float t, u, v;
+  uint32_t leaf, orig;
+};
+
+__device__ __forceinline__ float escape_zero(float v) {
+  const float e = 1e-9f;
+  if (fabsf(v) < e) v = (v > 0.f) ? v + e : v - e;
+  return v;
+}
+__device__ __forceinline__ float cull_slack(float best, float eps) { return eps + best * 1e-4f + 1e-5f; }
+
+// CURay::IntersectTri (src/cudaRTCommon.cu:42-64); o2 = orig + eps * dir.
+__device__ __forceinline__ float isect_tri(f3 o2, f3 d, f3 p0, f3 e0, f3 e1, bool cull, float& u, float& v) {
+  if (cull && dot(cross(e0, e1), d) > 0.f) return kInfF;
+  f3 de2 = cross(d, e1);
+  float div = dot(de2, e0);
+  if (fabsf(div) < kDetEps) return kInfF;
+  div = 1.0f / div;
+  f3 t = o2 - p0;
+  f3 te1 = cross(t, e0);
+  float rT = dot(te1, e1) * div;
+  if (rT < 0.f) return kInfF;
+  u = dot(de2, t) * div;
+  if (u < 0.f || u > 1.f) return kInfF;
+  v = dot(te1, d) * div;
+  if (v < 0.f || (u + v) > 1.f) return kInfF;
+  return rT;
+}
+
+__device__ __forceinline__ bool better(float t, uint32_t oid, const TraceHit& h) {
+  return t < h.t || (t == h.t && h.leaf != kInvalid && oid < h.orig);
+}
+
+__device__ __forceinline__ void test_leaf(const float4* __restrict__ tris, uint32_t first, uint32_t count, f3 o2, f3 d,
+                                          bool cull, TraceHit& h, uint32_t& nt) {
+  for (uint32_t i = first; i < first + count; i++) {
+    float4 a = tris[3 * i], b = tris[3 * i + 1], c = tris[3 * i + 2];
+    float u = 0.f, v = 0.f;
+    float t = isect_tri(o2, d, ld3(a), ld3(b), ld3(c), cull, u, v);
+    uint32_t oid = __float_as_uint(a.w);
+    if (t >= 0.f && better(t, oid, h)) { h.t = t; h.u = u; h.v = v; h.leaf = i; h.orig = oid; }
+  }
+  nt += count;
+}
+
+__device__ __forceinline__ void cswap(uint64_t& a, uint64_t& b) {
+  uint64_t lo = a < b ? a : b, hi = a < b ? b : a;
+  a = lo; b = hi;
+}
+
+// lstk points at this thread's column of a [CPT_LDS_STACK][kBlock] u64 array.
+template <bool ANY>
+__device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps, bool cull, uint64_t* lstk,
+                          TraceHit& h, uint32_t& nn, uint32_t& nt) {
+  const f3 idir = mk3(1.f / escape_zero(d.x), 1.f / escape_zero(d.y), 1.f / escape_zero(d.z));
+  const f3 o2 = o + d * eps;
+  h.t = maxd; h.u = h.v = 0.f; h.leaf = kInvalid; h.orig = kInvalid;
+  uint64_t pstk[CPT_PRIV_STACK];
+  int sp = 0;
+  uint32_t node = 0;
+  for (;;) {
+    const Node4* N = sc.nodes4 + node;
+    const float4 lx = *(const float4*)N->lox, hx = *(const float4*)N->hix;
+    const float4 ly = *(const float4*)N->loy, hy = *(const float4*)N->hiy;
+    const float4 lz = *(const float4*)N->loz, hz = *(const float4*)N->hiz;
+    const uint4 ch = *(const uint4*)N->child, cn = *(const uint4*)N->count;
+    nn++;
+    const float lx_[4] = {lx.x, lx.y, lx.z, lx.w}, hx_[4] = {hx.x, hx.y, hx.z, hx.w};
+    const float ly_[4] = {ly.x, ly.y, ly.z, ly.w}, hy_[4] = {hy.x, hy.y, hy.z, hy.w};
+    const float lz_[4] = {lz.x, lz.y, lz.z, lz.w}, hz_[4] = {hz.x, hz.y, hz.z, hz.w};
+    const uint32_t ch_[4] = {ch.x, ch.y, ch.z, ch.w}, cn_[4] = {cn.x, cn.y, cn.z, cn.w};
+    uint64_t key[4];
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      float t0x = (lx_[c] - o.x) * idir.x, t1x = (hx_[c] - o.x) * idir.x;
+      float t0y = (ly_[c] - o.y) * idir.y, t1y = (hy_[c] - o.y) * idir.y;
+      float t0z = (lz_[c] - o.z) * idir.z, t1z = (hz_[c] - o.z) * idir.z;
+      float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
+      float tfar = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * 1.0000004f;
+      bool hit = ch_[c] != kInvalid && tnear <= tfar && tnear <= h.t + cull_slack(h.t, eps);
+      key[c] = ~0ull;
+      if (hit) {
+        if (cn_[c] > 0) {
+          test_leaf(sc.tris, ch_[c], cn_[c], o2, d, cull, h, nt);
+          if (ANY && h.leaf != kInvalid) return true;
+        } else {
+          key[c] = ((uint64_t)__float_as_uint(tnear) << 32) | ch_[c];
+        }
+      }
+    }
+    // sort interior children near-first (tnear >= 0, so float bits order = value order)
+    cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[1], key[2]);
+    if (key[0] != ~0ull) {
+      node = (uint32_t)key[0];
+#pragma unroll
+      for (int c = 3; c >= 1; c--) {
+        // 3 pushes per level and a 4-wide depth <= 31 (checked at build) keep
+        // sp below CPT_LDS_STACK + CPT_PRIV_STACK = 96.
+        if (key[c] != ~0ull && sp < CPT_LDS_STACK + CPT_PRIV_STACK) {
+          if (sp < CPT_LDS_STACK) lstk[sp * kBlock] = key[c];
+          else pstk[sp - CPT_LDS_STACK] = key[c];
+          sp++;
+        }
+      }
+      continue;
+    }
+    // pop the nearest pending node still in front of the best hit
+    bool found = false;
+    while (sp > 0) {
+      sp--;
+      uint64_t e = sp < CPT_LDS_STACK ? lstk[sp * kBlock] : pstk[sp - CPT_LDS_STACK];
+      float tn = __uint_as_float((uint32_t)(e >> 32));
+      if (tn <= h.t + cull_slack(h.t, eps)) { node = (uint32_t)e; found = true; break; }
+    }
+    if (!found) break;
+  }
+  return h.leaf != kInvalid;
+}
+
+// Same query over the reference's binary layout (right child popped first).
+template <bool ANY>
+__device__ bool traverse_bin(const DevScene& sc, f3 o, f3 d, float maxd, float eps, bool cull, TraceHit& h,
+                             uint32_t& nn, uint32_t& nt) {
+  const f3 idir = mk3(1.f / escape_zero(d.x), 1.f / escape_zero(d.y), 1.f / escape_zero(d.z));
+  const f3 o2 = o + d * eps;
+  h.t = maxd; h.u = h.v = 0.f; h.leaf = kInvalid; h.orig = kInvalid;
+  uint32_t stk[64];
+  int sp = 0;
+  stk[0] = 0;
+  while (sp >= 0) {
+    uint32_t n = stk[sp--];
+    float4 mn = sc.bnodes[2 * n], mx = sc.bnodes[2 * n + 1];
+    uint2 off = sc.boff[n];
+    nn++;
+    float t0x = (mn.x - o.x) * idir.x, t1x = (mx.x - o.x) * idir.x;
+    float t0y = (mn.y - o.y) * idir.y, t1y = (mx.y - o.y) * idir.y;
+    float t0z = (mn.z - o.z) * idir.z, t1z = (mx.z - o.z) * idir.z;
+    float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
+    float tfar = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * 1.0000004f;
+    if (!(tnear <= tfar && tnear <= h.t + cull_slack(h.t, eps))) continue;
+    if (off.y == 0) {
+      if (mn.x > mx.x) continue;                     // empty leaf of a degenerate split
+      if (sp < 62) { stk[++sp] = n + 1; stk[++sp] = n + off.x; }
+    } else {
+      test_leaf(sc.tris, off.x, off.y, o2, d, cull, h, nt);
+      if (ANY && h.leaf != kInvalid) return true;
+    }
+  }
+  return h.leaf != kInvalid;
+}
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// ---------------------------------------------------------------- trace API
+// out mode 0: cpt_hit {orig id, t, u, v}; 1: {t, leaf bits, u, v}; 2: occluded flag.
+template <int LAYOUT, bool ANY, int OUT>
+__global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, const float4* __restrict__ ro,
+                                                  const float4* __restrict__ rd, uint32_t stride,
+                                                  const uint32_t* __restrict__ countp, uint32_t n, uint32_t cull,
+                                                  float4* __restrict__ out, DevStats* __restrict__ stats) {
+  __shared__ uint64_t lstk[CPT_LDS_STACK * kBlock];
+  uint32_t count = countp ? *countp : n;
+  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t nn = 0, nt = 0;
+  if (i < count) {
+    float4 a = ro[(size_t)i * stride], b = rd[(size_t)i * stride];
+    TraceHit h;
+    bool hit = LAYOUT == 0 ? traverse4<ANY>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, lstk + threadIdx.x, h, nn, nt)
+                           : traverse_bin<ANY>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, h, nn, nt);
+    if (OUT == 0) {
+      float4 r;
+      r.x = __uint_as_float(hit ? h.orig : kInvalid);
+      r.y = hit ? h.t : kInfF;
+      r.z = hit ? h.u : 0.f;
+      r.w = hit ? h.v : 0.f;
+      out[i] = r;
+    } else if (OUT == 1) {
+      float4 r;
+      r.x = hit ? h.t : kInfF;
+      r.y = __uint_as_float(hit ? h.leaf : kInvalid);
+      r.z = h.u;
+      r.w = h.v;
+      out[i] = r;
+    } else {
+      ((uint32_t*)out)[i] = hit ? 1u : 0u;
+    }
+  }
+  if (stats) {
+    uint32_t sn = wave_sum(nn), st = wave_sum(nt);
+    if ((threadIdx.x & 63) == 0 && (sn | st)) {
+      atomicAdd(&stats->ext_nodes, (unsigned long long)sn);
+      atomicAdd(&stats->ext_tris, (unsigned long long)st);
+    }
+  }
+}
+
+hipError_t launch_trace(const DevScene& sc, const float4* rays, uint32_t n, const uint32_t* count, uint32_t cull,
+                        float4* hits_out, int out_orig, int any_hit, uint32_t layout, DevStats* stats,
+                        hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
+  const float4* ro = rays;
+  const float4* rd = rays + 1;
+  if (any_hit) {
+    if (layout == 0) k_trace<0, true, 2><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
+    else k_trace<1, true, 2><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
+  } else if (out_orig) {
+    if (layout == 0) k_trace<0, false, 0><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
+    else k_trace<1, false, 0><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
+  } else {
+    if (layout == 0) k_trace<0, false, 1><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
+    else k_trace<1, false, 1><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- shading
+// src/cudaRTCommon.cu:284-365 with the reference's mixed float/double
+// expressions (M_PI and the bias epsilons are double literals there).
+__device__ __forceinline__ bool prob_rand(uint64_t& st, uint64_t inc, float prob) {
+  if ((double)prob < kBiasEps) return false;
+  if ((double)prob + kBiasEps > 1.0) return true;
+  return uniform01(st, inc) < prob;
+}
+__device__ __forceinline__ float vis_smith(float r, float nov, float nol) {
+  float k = (r * r) / 2.f;
+  return (nov / (nov * (1.f - k) + k)) * (nol / (nol * (1.f - k) + k));
+}
+__device__ __forceinline__ float d_ggx(float r, float noh) {
+  float m = r * r, m2 = m * m;
+  float d = (noh * m2 - noh) * noh + 1.f;
+  return (float)((double)m2 / (kPiD * (double)d * (double)d));
+}
+__device__ __forceinline__ f3 f_schlick(f3 s, float voh) {
+  float fc = powf(1.f - voh, 5.f);
+  double z = 50.0 * (double)s.z;
+  float first = (float)(z < 0.0 ? 0.0 : (z > 1.0 ? 1.0 : z)) * fc;
+  return mk3(first, first, first) + s * (1.f - fc);
+}
+__device__ __forceinline__ void basis(f3 w, f3& u, f3& v) {
+  f3 a = (double)fabsf(w.x) > 0.1 ? mk3(0.f, 1.f, 0.f) : mk3(1.f, 0.f, 0.f);
+  u = normalize(cross(a, w));
+  v = cross(w, u);
+  u = cross(v, w);
+}
+__device__ __forceinline__ f3 sample_ggx(float xi0, float xi1, float rough, f3 n) {
+  float a = rough * rough;
+  float phi = (float)(2.0 * kPiD * (double)xi0);
+  float ct = sqrtf((1.f - xi1) / (1.f + (a * a - 1.f) * xi1));
+  float st = sqrtf(1.f - ct * ct);
+  f3 h = mk3(st * cosf(phi), st * sinf(phi), ct);
+  f3 u, v;
+  basis(n, u, v);
+  return (u * h.x + v * h.y) + n * h.z;
+}
+__device__ __forceinline__ f3 lambert(f3 diff) { return diff * (float)(1.0 / kPiD); }
+// GetShadingResult (src/cudaRTBDPTStream.cu:642-663)
+__device__ __forceinline__ f3 shading(f3 outDir, f3 inDir, f3 irrad, f3 n, f3 diff, float metal, float rough,
+                                      float spec, float dw, float sw) {
+  if (dot(n, inDir) <= 0.f) return mk3(0.f, 0.f, 0.f);
+  f3 h = normalize(outDir + inDir);
+  float voh = dot(outDir, h), nov = dot(n, outDir), noh = dot(n, h), nol = dot(n, inDir);
+  f3 f0 = vlerp(mk3(1.f, 1.f, 1.f) * (0.08f * spec), diff, metal);
+  f3 F = f_schlick(f0, voh);
+  float dd = d_ggx(rough, noh), vv = vis_smith(rough, nov, nol);
+  f3 specI = F * (dd * vv);
+  f3 diffI = vmul(mk3(1.f, 1.f, 1.f) - F, lambert(diff));
+  return vmul(irrad * nol, specI * sw + diffI * dw);
+}
+
+struct Surf {
+  f3 norm, pos, diff, emissive;
+  float trans, spec, metal, rough;
+};
+__device__ __forceinline__ Surf fetch_surf(const DevScene& sc, uint32_t leaf, float w, float u, float v, bool want_pos) {
+  Surf s;
+  uint4 tv = sc.tri_verts[leaf];
+  const DevMaterial& m = sc.mats[tv.w];
+  f3 n0 = ld3(sc.vnrm[tv.x]), n1 = ld3(sc.vnrm[tv.y]), n2 = ld3(sc.vnrm[tv.z]);
+  s.norm = (n0 * w + n1 * u) + n2 * v;
+  if (want_pos) {
+    f3 q0 = ld3(sc.vpos[tv.x]), q1 = ld3(sc.vpos[tv.y]), q2 = ld3(sc.vpos[tv.z]);
+    s.pos = (q0 * w + q1 * u) + q2 * v;
+  } else {
+    s.pos = mk3(0.f, 0.f, 0.f);
+  }
+  s.diff = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+  s.emissive = mk3(m.emissive[0], m.emissive[1], m.emissive[2]);
+  s.trans = m.transparency; s.spec = m.specular; s.metal = m.metallic; s.rough = m.roughness;
+  return s;
+}
+
+enum { RT_EYE = 0, RT_DIFF = 1, RT_SPEC = 2, RT_LIGHT = 3 };
+struct Bounce {
+  f3 orig, dir, mul;
+  int type;
+};
+// BRDF sample of one bounce (src/cudaRTBDPTStream.cu:228-329 and :402-502).
+__device__ Bounce sample_bounce(uint64_t& st, uint64_t inc, f3 ro, f3 rd, float t, const Surf& s, f3 nl, int cur) {
+  float x0 = uniform01(st, inc), x1 = uniform01(st, inc);
+  f3 hDir = sample_ggx(x0, x1, s.rough, nl);
+  f3 ndir = rd * -1.f;
+  float voH = dot(ndir, hDir);
+  f3 f0 = vlerp(mk3(0.08f * s.spec, 0.08f * s.spec, 0.08f * s.spec), s.diff, s.metal);
+  f3 F = f_schlick(f0, voH);
+  float reflProb = lerpf(length(F), 1.f, s.metal);
+  float refrProb = s.trans;
+  f3 reflDir = mk3(0.f, 0.f, 0.f), refrDir = mk3(0.f, 0.f, 0.f);
+  Bounce b;
+  b.orig = ro; b.dir = rd; b.type = cur; b.mul = mk3(0.f, 0.f, 0.f);
+  if (refrProb > 0.f) {
+    bool into = dot(nl, s.norm) > 0.f;
+    float nt = s.spec * 0.8f + 1.f, nc = 1.f;
+    float nnt = into ? nc / nt : nt / nc;
+    float ddn = dot(hDir, rd);
+    float cos2t = 1.f - nnt * nnt * (1.f - ddn * ddn);
+    refrDir = normalize(rd * nnt - hDir * (ddn * nnt + sqrtf(cos2t)));
+  }
+  if (reflProb > 0.f) {
+    reflDir = normalize(rd - (hDir * 2.f) * dot(hDir, rd));
+    if (dot(reflDir, nl) < 0.f) reflProb = 0.f;
+  }
+  if (prob_rand(st, inc, reflProb)) {
+    b.orig = ro + rd * (float)((double)t - kBiasEps);
+    b.dir = reflDir;
+    float VoH = dot(ndir, hDir), NoV = dot(nl, ndir), NoH = dot(nl, hDir), NoL = dot(nl, reflDir);
+    float G = vis_smith(s.rough, NoV, NoL);
+    b.mul = (F * G * VoH) / (NoV * NoH * reflProb);
+    b.type = RT_SPEC;
+  } else if (prob_rand(st, inc, refrProb)) {
+    b.orig = (ro + rd * (float)((double)t + kBiasEps)) + refrDir * (float)kBiasEps;
+    b.dir = refrDir;
+    float cosine = dot(nl * -1.f, refrDir);
+    b.mul = s.diff * cosine / (refrProb * (1.f - reflProb));
+    b.type = RT_SPEC;
+  } else {
+    f3 u, v;
+    basis(nl, u, v);
+    float r1 = (float)(2.0 * kPiD * (double)uniform01(st, inc));
+    float r2cos = sqrtf(uniform01(st, inc));
+    float r2sin = 1.f - r2cos * r2cos;
+    f3 dd = normalize((nl * r2cos + (u * r2sin) * cosf(r1)) + (v * r2sin) * sinf(r1));
+    b.orig = (ro + rd * t) + dd * (float)kBiasEps;
+    b.dir = dd;
+    b.mul = lambert(s.diff) * (float)kPiD / ((1.f - refrProb) * (1.f - reflProb));
+    b.type = RT_DIFF;
+  }
+  return b;
+}
+
+__device__ __forceinline__ double shadow_max(double dist, uint32_t flags) {
+  return (flags & 1u) ? dist - kBiasEps : dist - 2.0 * kBiasEps;
+}
+
+// ---------------------------------------------------------------- frame
+__global__ void k_frame_begin(FrameCounters* fc, uint32_t n_paths) {
+  uint32_t i = threadIdx.x;
+  if (i < 16) { fc->active[i] = i == 0 ? n_paths : 0u; fc->shadow[i] = 0u; }
+}
+hipError_t launch_frame_begin(FrameCounters* fc, uint32_t n_paths, hipStream_t st) {
+  k_frame_begin<<<1, 64, 0, st>>>(fc, n_paths);
+  return hipGetLastError();
+}
+
+__global__ void k_frame_end(FrameCounters* fc, DevStats* stats, uint32_t n_paths, uint32_t depth) {
+  if (threadIdx.x == 0) {
+    unsigned long long e = 0, s = 0;
+    for (uint32_t b = 0; b < depth && b < 16; b++) { e += fc->active[b]; s += fc->shadow[b]; }
+    stats->ext_rays += e;
+    stats->shadow_rays += s;
+    stats->eye_paths += n_paths;
+    stats->frames += 1;
+  }
+}
+hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t n_paths, uint32_t depth, hipStream_t st) {
+  k_frame_end<<<1, 64, 0, st>>>(fc, stats, n_paths, depth);
+  return hipGetLastError();
+}
+
+__device__ __forceinline__ void store_lv(LightVert& V, f3 pos, f3 norm, f3 irrad, f3 irradDir, const Surf& s) {
+  V.pos[0] = pos.x; V.pos[1] = pos.y; V.pos[2] = pos.z;
+  V.norm[0] = norm.x; V.norm[1] = norm.y; V.norm[2] = norm.z;
+  V.irrad[0] = irrad.x; V.irrad[1] = irrad.y; V.irrad[2] = irrad.z;
+  V.irradDir[0] = irradDir.x; V.irradDir[1] = irradDir.y; V.irradDir[2] = irradDir.z;
+  V.diff[0] = s.diff.x; V.diff[1] = s.diff.y; V.diff[2] = s.diff.z;
+  V.emissive[0] = s.emissive.x; V.emissive[1] = s.emissive.y; V.emissive[2] = s.emissive.z;
+  V.specular = s.spec; V.metallic = s.metal; V.roughness = s.rough;
+  V.pad[0] = V.pad[1] = V.pad[2] = 0.f;
+}
+
+// Light-vertex cache, one workgroup (light_paths <= kBlock).
+__global__ __launch_bounds__(kBlock) void k_light_pass(DevScene sc, FrameArgs fa, LightVert* __restrict__ lv,
+                                                       FrameCounters* __restrict__ fc) {
+  __shared__ uint64_t lstk[CPT_LDS_STACK * kBlock];
+  __shared__ uint32_t wcnt[kBlock / 64];
+  const uint32_t x = threadIdx.x, L = fa.light_paths;
+  const uint64_t inc = ((uint64_t)x << 1) | 1u;
+  uint64_t st = 0;
+  bool live = false;
+  f3 orig = mk3(0.f, 0.f, 0.f), dir = orig, smp = orig, mul = mk3(1.f, 1.f, 1.f);
+  int type = RT_LIGHT;
+  uint32_t depth = 0;
+  if (sc.n_light_tris == 0) {
+    if (x == 0) fc->light_n = 0;
+    return;
+  }
+  if (x < L) {
+    pcg32_seed(st, inc, fa.light_key);
+    uint32_t id = (uint32_t)(uniform01(st, inc) * (float)sc.n_light_tris);
+    if (id >= sc.n_light_tris) id = sc.n_light_tris - 1;
+    float lw = uniform01(st, inc), lu = uniform01(st, inc);
+    if (lw + lu > 1.f) { lw = 1.f - lw; lu = 1.f - lu; }
+    float lvv = (1.f - lw) - lu;
+    Surf s = fetch_surf(sc, sc.light_tris[id], lw, lu, lvv, true);
+    f3 u, v;
+    basis(s.norm, u, v);
+    float r1 = (float)(2.0 * kPiD * (double)uniform01(st, inc));
+    float r2cos = sqrtf(uniform01(st, inc));
+    float r2sin = 1.f - r2cos * r2cos;
+    f3 dd = normalize((s.norm * r2cos + (u * r2sin) * cosf(r1)) + (v * r2sin) * sinf(r1));
+    orig = s.pos + s.norm * (float)kBiasEps;
+    dir = dd;
+    smp = s.emissive;
+    live = true;
+    store_lv(lv[x], s.pos, s.norm, s.emissive, mk3(0.f, 0.f, 0.f), s);
+  }
+  __syncthreads();
+  uint32_t cur = L;
+  const int lane = x & 63, wid = x >> 6;
+  for (;;) {
+    unsigned long long m = __ballot(live);
+    uint32_t below = __popcll(m & ((1ull << lane) - 1ull));
+    if (lane == 0) wcnt[wid] = __popcll(m);
+    __syncthreads();
+    uint32_t pre = 0, total = 0;
+    for (int w = 0; w < kBlock / 64; w++) { if (w < wid) pre += wcnt[w]; total += wcnt[w]; }
+    __syncthreads();
+    if (total == 0) break;
+    if (live) {
+      uint32_t slot = cur + pre + below;
+      TraceHit h;
+      uint32_t nn = 0, nt = 0;
+      if (!traverse4<false>(sc, orig, dir, kInfF, (float)kBiasEps, false, lstk + x, h, nn, nt)) {
+        lv[slot] = lv[x];
+        live = false;
+      } else {
+        float w = (1.f - h.u) - h.v;
+        Surf s = fetch_surf(sc, h.leaf, w, h.u, h.v, true);
+        f3 nl = dot(s.norm, dir) < 0.f ? s.norm : s.norm * -1.f;
+        Bounce b = sample_bounce(st, inc, orig, dir, h.t, s, nl, type);
+        f3 irr = b.type != RT_DIFF ? mk3(0.f, 0.f, 0.f) : smp;
+        f3 vn = dot(b.dir, nl) < 0.f ? nl * -1.f : nl;
+        store_lv(lv[slot], s.pos, vn, irr, dir * -1.f, s);
+        smp = s.emissive + vmul(smp, b.mul);
+        float contrib = length(mul) * length(b.mul);
+        if (length(s.emissive) > 0.f) contrib = 0.f;
+        if (uniform01(st, inc) > contrib || depth + 1 >= fa.light_depth) {
+          live = false;
+        } else {
+          mul = vmul(mul, b.mul);
+          dir = b.dir;
+          orig = b.orig;
+          type = b.type;
+          depth++;
+        }
+      }
+    }
+    cur += total;
+    __syncthreads();
+  }
+  if (x == 0) fc->light_n = cur;
+}
+hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, LightVert* lv, FrameCounters* fc,
+                             hipStream_t st) {
+  k_light_pass<<<1, kBlock, 0, st>>>(sc, fa, lv, fc);
+  return hipGetLastError();
+}
+
+// pt_genPathQueue_kernel (src/cudaRTBDPTStream.cu:619-640); slot = pixel.
+__global__ __launch_bounds__(kBlock) void k_gen_eye(FrameArgs fa, PathBuf pb) {
+  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t n = fa.width * fa.height;
+  if (i >= n) return;
+  uint32_t y = i / fa.width, x = i - y * fa.width;
+  float W = (float)fa.width, H = (float)fa.height;
+  float tf = tanf(fa.fov * 0.5f);
+  float u = ((((2.f * ((float)x + 0.5f)) / W - 1.f) * tf) * W) / H;
+  float v = ((2.f * ((float)y + 0.5f)) / H - 1.f) * tf;
+  uint64_t inc = ((uint64_t)i << 1) | 1u, st;
+  pcg32_seed(st, inc, fa.eye_key);
+  float au = u + ((uniform01(st, inc) - 0.5f) / H) * tf;
+  float av = v + ((uniform01(st, inc) - 0.5f) / H) * tf;
+  f3 right = mk3(fa.cam_right[0], fa.cam_right[1], fa.cam_right[2]);
+  f3 up = mk3(fa.cam_up[0], fa.cam_up[1], fa.cam_up[2]);
+  f3 cd = mk3(fa.cam_dir[0], fa.cam_dir[1], fa.cam_dir[2]);
+  f3 d = normalize((right * au + up * av) + cd);
+  pb.ray_o[i] = make_float4(fa.cam_pos[0], fa.cam_pos[1], fa.cam_pos[2], kInfF);
+  pb.ray_d[i] = make_float4(d.x, d.y, d.z, (float)kBiasEps);
+  pb.pix[i] = i;
+  pb.depth[i] = 0;
+  pb.mul[i] = make_float4(1.f, 1.f, 1.f, 0.f);
+  pb.smp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.rng[i] = st;
+}
+hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st) {
+  uint32_t n = fa.width * fa.height;
+  k_gen_eye<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(fa, pb);
+  return hipGetLastError();
+}
+
+hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, uint32_t max_n,
+                         float4* hits, DevStats* stats, hipStream_t st) {
+  if (max_n == 0) return hipSuccess;
+  k_trace<0, false, 1><<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, pb.ray_o, pb.ray_d, 1,
+                                                                            &fc->active[bounce], max_n, 0, hits, stats);
+  return hipGetLastError();
+}
+
+// Running-mean update of pt_applyPathQueueResult_kernel
+// (src/cudaRTBDPTStream.cu:754-776); one path per pixel per frame.
+__device__ __forceinline__ void apply_pixel(float* result, uint32_t* sample_n, uint32_t pix, uint32_t frame, f3 acc,
+                                            uint32_t n) {
+  uint32_t nold = frame == 0 ? 0u : sample_n[pix];
+  if (frame == 0) sample_n[pix] = 0u;
+  uint32_t nn = nold + n;
+  if (!(nn > nold) || isinf3(acc)) return;
+  float ri = 1.f / (float)nn, oi = (float)nold * ri;
+  float* r = result + (size_t)pix * 3;
+  r[0] = fmaxf(ri * acc.x + oi * r[0], 0.f);
+  r[1] = fmaxf(ri * acc.y + oi * r[1], 0.f);
+  r[2] = fmaxf(ri * acc.z + oi * r[2], 0.f);
+  sample_n[pix] = nn;
+}
+
+// Append with one atomic per workgroup: returns this lane's slot when pred.
+__device__ __forceinline__ void block_append2(bool p0, bool p1, uint32_t* ctr0, uint32_t* ctr1, uint32_t& s0,
+                                              uint32_t& s1) {
+  __shared__ uint32_t wc0[kBlock / 64], wc1[kBlock / 64], base0, base1;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  unsigned long long m0 = __ballot(p0), m1 = __ballot(p1);
+  unsigned long long below = (1ull << lane) - 1ull;
+  if (lane == 0) { wc0[wid] = __popcll(m0); wc1[wid] = __popcll(m1); }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t0 = 0, t1 = 0;
+    for (int w = 0; w < kBlock / 64; w++) { t0 += wc0[w]; t1 += wc1[w]; }
+    base0 = t0 ? atomicAdd(ctr0, t0) : 0u;
+    base1 = t1 ? atomicAdd(ctr1, t1) : 0u;
+  }
+  __syncthreads();
+  uint32_t pre0 = 0, pre1 = 0;
+  for (int w = 0; w < wid; w++) { pre0 += wc0[w]; pre1 += wc1[w]; }
+  s0 = base0 + pre0 + __popcll(m0 & below);
+  s1 = base1 + pre1 + __popcll(m1 & below);
+}
+
+__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, PathBuf cur, PathBuf nxt,
+                                                  const float4* __restrict__ hits, FrameCounters* __restrict__ fc,
+                                                  int bounce, ShadowBuf sb, const LightVert* __restrict__ lv,
+                                                  float* __restrict__ result, uint32_t* __restrict__ sample_n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t count = fc->active[bounce];
+  if (blockIdx.x * kBlock >= count) return;               // whole workgroup idle
+  const uint32_t lvN = fc->light_n;
+  bool survive = false, emit = false;
+  float4 o_out = make_float4(0.f, 0.f, 0.f, 0.f), d_out = o_out, mul_out = o_out, smp_out = o_out, acc_out = o_out;
+  float4 so = o_out, sd = o_out, sc4 = o_out;
+  uint32_t pix = 0, depth = 0;
+  uint64_t st = 0;
+  if (i < count) {
+    float4 ro = cur.ray_o[i], rd = cur.ray_d[i];
+    pix = cur.pix[i];
+    depth = cur.depth[i];
+    float4 m4 = cur.mul[i], s4 = cur.smp[i], a4 = cur.acc[i];
+    st = cur.rng[i];
+    const uint64_t inc = ((uint64_t)pix << 1) | 1u;
+    float4 h = hits[i];
+    f3 orig = ld3(ro), dir = ld3(rd), mul = ld3(m4), psample = ld3(s4), acc = ld3(a4);
+    uint32_t N = __float_as_uint(a4.w);
+    uint32_t leaf = __float_as_uint(h.y);
+    bool term = false;
+    if (leaf == kInvalid) {
+      acc = acc + psample; N++; term = true;
+    } else {
+      float w = (1.f - h.z) - h.w;
+      Surf s = fetch_surf(sc, leaf, w, h.z, h.w, false);
+      f3 nl = dot(s.norm, dir) < 0.f ? s.norm : s.norm * -1.f;
+      Bounce b = sample_bounce(st, inc, orig, dir, h.x, s, nl, RT_EYE);
+      psample = psample + vmul(s.emissive, mul);
+      f3 inMul = mul, inDir = dir * -1.f;
+      float contrib = length(mul) * length(b.mul);
+      if (length(s.emissive) > 0.f) contrib = 0.f;
+      if (uniform01(st, inc) > contrib || depth + 1 >= fa.eye_depth) {
+        acc = acc + psample; N++; term = true;
+      } else {
+        mul = vmul(mul, b.mul);
+        dir = b.dir;
+        depth++;
+        orig = b.orig;
+        if (b.type == RT_DIFF && lvN > 0) {
+          f3 pos = orig + nl * (float)kBiasEps;
+          uint32_t li = (uint32_t)(uniform01(st, inc) * (float)lvN);
+          if (li >= lvN) li = lvN - 1;
+          const LightVert& L = lv[li];
+          f3 lpos = mk3(L.pos[0], L.pos[1], L.pos[2]);
+          f3 lirr = mk3(L.irrad[0], L.irrad[1], L.irrad[2]);
+          f3 toL = normalize(lpos - pos);
+          float dist = length(lpos - pos);
+          if (length(lirr) > 0.f && dot(nl, toL) > 0.f) {
+            f3 irrad, irradDir;
+            if ((double)dist > kFltEps) {
+              irrad = shading(toL * -1.f, mk3(L.irradDir[0], L.irradDir[1], L.irradDir[2]), lirr,
+                              mk3(L.norm[0], L.norm[1], L.norm[2]), mk3(L.diff[0], L.diff[1], L.diff[2]), L.metallic,
+                              L.roughness, L.specular, 1.f, 1.f) +
+                      mk3(L.emissive[0], L.emissive[1], L.emissive[2]);
+              irradDir = toL;
+            } else {
+              irrad = lirr;
+              irradDir = mk3(L.irradDir[0], L.irradDir[1], L.irradDir[2]) * -1.f;
+            }
+            f3 c = vmax0(shading(inDir, irradDir, irrad, nl, s.diff, s.metal, s.rough, s.spec, 1.f - s.trans, 1.f));
+            if (length(c) > 0.f) {
+              f3 pay = vmul(c, inMul);
+              emit = true;
+              so = make_float4(pos.x, pos.y, pos.z, (float)shadow_max((double)dist, fa.flags));
+              sd = make_float4(toL.x, toL.y, toL.z, (float)kBiasEps);
+              sc4 = make_float4(pay.x, pay.y, pay.z, 0.f);
+            }
+          }
+        }
+      }
+    }
+    if (term) {
+      apply_pixel(result, sample_n, pix, fa.frame, acc, N);
+    } else {
+      survive = true;
+      o_out = make_float4(orig.x, orig.y, orig.z, kInfF);
+      d_out = make_float4(dir.x, dir.y, dir.z, (float)kBiasEps);
+      mul_out = make_float4(mul.x, mul.y, mul.z, 0.f);
+      smp_out = make_float4(psample.x, psample.y, psample.z, 0.f);
+      acc_out = make_float4(acc.x, acc.y, acc.z, __uint_as_float(N));
+    }
+  }
+  uint32_t slot = 0, sslot = 0;
+  block_append2(survive, emit, &fc->active[bounce + 1], &fc->shadow[bounce], slot, sslot);
+  if (survive) {
+    nxt.ray_o[slot] = o_out;
+    nxt.ray_d[slot] = d_out;
+    nxt.pix[slot] = pix;
+    nxt.depth[slot] = depth;
+    nxt.mul[slot] = mul_out;
+    nxt.smp[slot] = smp_out;
+    nxt.acc[slot] = acc_out;
+    nxt.rng[slot] = st;
+  }
+  if (emit) {
+    sc4.w = __uint_as_float(slot);
+    sb.ray_o[sslot] = so;
+    sb.ray_d[sslot] = sd;
+    sb.contrib[sslot] = sc4;
+  }
+}
+hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, PathBuf nxt, const float4* hits,
+                        FrameCounters* fc, int bounce, uint32_t max_n, ShadowBuf sb, const LightVert* lv,
+                        float* result, uint32_t* sample_n, hipStream_t st) {
+  if (max_n == 0) return hipSuccess;
+  k_shade<<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, fa, cur, nxt, hits, fc, bounce, sb, lv, result,
+                                                             sample_n);
+  return hipGetLastError();
+}
+
+__global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, ShadowBuf sb, PathBuf nxt,
+                                                   const FrameCounters* __restrict__ fc, int bounce) {
+  __shared__ uint64_t lstk[CPT_LDS_STACK * kBlock];
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t count = fc->shadow[bounce];
+  if (i >= count) return;
+  float4 a = sb.ray_o[i], b = sb.ray_d[i], c = sb.contrib[i];
+  TraceHit h;
+  uint32_t nn = 0, nt = 0;
+  if (!traverse4<true>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt)) {
+    uint32_t slot = __float_as_uint(c.w);
+    float4 acc = nxt.acc[slot];
+    acc.x = acc.x + c.x;
+    acc.y = acc.y + c.y;
+    acc.z = acc.z + c.z;
+    acc.w = __uint_as_float(__float_as_uint(acc.w) + 4u);
+    nxt.acc[slot] = acc;
+  }
+}
+hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
+                         uint32_t max_n, hipStream_t st) {
+  if (max_n == 0) return hipSuccess;
+  k_shadow<<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, sb, nxt, fc, bounce);
+  return hipGetLastError();
+}
+
+}  // namespace cpt

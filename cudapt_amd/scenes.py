@@ -1,0 +1,261 @@
+"""Procedural scenes and a minimal OBJ reader, as flat numpy arrays.
+
+A scene is a dict with
+  pos   float32 [nv, 3]   vertex positions
+  nrm   float32 [nv, 3]   vertex normals (RTVertex::norm, include/raytracer.h:58-64)
+  idx   uint32  [nt, 3]   triangle vertex indices
+  mat   uint32  [nt]      material index per triangle (RTTriangle::matInd)
+  mats  numpy structured array of MATERIAL_DTYPE, RTMaterial field order
+        (include/raytracer.h:82-100)
+  camera dict(pos, dir, up, fov) with fov in radians
+
+The reference builds scenes by assimp import (src/raytracer.cpp:238-282); the
+workloads here are procedural and seeded, so every run sees the same bytes.
+Scene-level randomness uses numpy's PCG64 generator; per-pixel randomness is the
+renderer's own PCG32 stream.
+"""
+from __future__ import annotations
+
+import math
+import numpy as np
+
+MATERIAL_DTYPE = np.dtype([
+    ("diffuse", "<f4", 3), ("emissive", "<f4", 3),
+    ("diffuseTexId", "<i4"), ("normalTexId", "<i4"), ("emissiveTexId", "<i4"), ("matType", "<i4"),
+    ("transparency", "<f4"), ("subsurface", "<f4"), ("specular", "<f4"), ("metallic", "<f4"),
+    ("roughness", "<f4"), ("anisotropic", "<f4"), ("sheen", "<f4"), ("sheenTint", "<f4"),
+    ("clearcoat", "<f4"), ("clearcoatGloss", "<f4"),
+])
+assert MATERIAL_DTYPE.itemsize == 80
+
+
+def material(diffuse=(0.8, 0.8, 0.8), emissive=(0.0, 0.0, 0.0), transparency=0.0, specular=0.0,
+             metallic=0.0, roughness=1.0, sheen=0.0, clearcoat=0.0):
+    m = np.zeros((), MATERIAL_DTYPE)
+    m["diffuse"] = diffuse
+    m["emissive"] = emissive
+    m["diffuseTexId"] = m["normalTexId"] = m["emissiveTexId"] = -1
+    m["transparency"] = transparency
+    m["specular"] = specular
+    m["metallic"] = metallic
+    m["roughness"] = roughness
+    m["sheen"] = sheen
+    m["clearcoat"] = clearcoat
+    return m
+
+
+class _Builder:
+    def __init__(self):
+        self.pos, self.nrm, self.idx, self.mat, self.mats = [], [], [], [], []
+        self.nv = 0
+
+    def add_material(self, m) -> int:
+        self.mats.append(m)
+        return len(self.mats) - 1
+
+    def add_quad(self, a, b, c, d, normal, m):
+        """Quad a-b-c-d (counter-clockwise seen from `normal`) as two triangles."""
+        base = self.nv
+        self.pos += [a, b, c, d]
+        self.nrm += [normal] * 4
+        self.idx += [(base, base + 1, base + 2), (base, base + 2, base + 3)]
+        self.mat += [m, m]
+        self.nv += 4
+
+    def add_mesh(self, pos, nrm, idx, m):
+        base = self.nv
+        self.pos += [tuple(p) for p in pos]
+        self.nrm += [tuple(n) for n in nrm]
+        self.idx += [tuple(int(i) + base for i in t) for t in idx]
+        self.mat += [m] * len(idx)
+        self.nv += len(pos)
+
+    def add_box(self, center, half, yaw, m):
+        """Axis-aligned box rotated by `yaw` about +y; flat normals, 12 triangles."""
+        c, s = math.cos(yaw), math.sin(yaw)
+        rot = lambda v: (c * v[0] + s * v[2], v[1], -s * v[0] + c * v[2])
+        hx, hy, hz = half
+        faces = [  # (normal, four corners in local space, ccw from outside)
+            ((1, 0, 0), [(hx, -hy, hz), (hx, -hy, -hz), (hx, hy, -hz), (hx, hy, hz)]),
+            ((-1, 0, 0), [(-hx, -hy, -hz), (-hx, -hy, hz), (-hx, hy, hz), (-hx, hy, -hz)]),
+            ((0, 1, 0), [(-hx, hy, hz), (hx, hy, hz), (hx, hy, -hz), (-hx, hy, -hz)]),
+            ((0, -1, 0), [(-hx, -hy, -hz), (hx, -hy, -hz), (hx, -hy, hz), (-hx, -hy, hz)]),
+            ((0, 0, 1), [(-hx, -hy, hz), (hx, -hy, hz), (hx, hy, hz), (-hx, hy, hz)]),
+            ((0, 0, -1), [(hx, -hy, -hz), (-hx, -hy, -hz), (-hx, hy, -hz), (hx, hy, -hz)]),
+        ]
+        for n, corners in faces:
+            pts = [tuple(np.add(rot(p), center)) for p in corners]
+            self.add_quad(*pts, rot(n), m)
+
+    def build(self, camera):
+        mats = np.zeros(len(self.mats), MATERIAL_DTYPE)
+        for i, m in enumerate(self.mats):
+            mats[i] = m
+        return dict(pos=np.asarray(self.pos, np.float32).reshape(-1, 3),
+                    nrm=np.asarray(self.nrm, np.float32).reshape(-1, 3),
+                    idx=np.asarray(self.idx, np.uint32).reshape(-1, 3),
+                    mat=np.asarray(self.mat, np.uint32), mats=mats, camera=camera)
+
+
+def cornell(seed: int = 0xC1, light_radiance: float = 15.0):
+    """Cornell box, 5 quads + 2 yaw-rotated cubes + 1 ceiling light = 36 triangles.
+
+    Lambertian albedo U(0.2, 0.8) per RGB, expressed in the reference material
+    model as specular = metallic = transparency = 0 (F = 0, reflProb = 0).
+    """
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    alb = lambda: material(diffuse=tuple(rng.uniform(0.2, 0.8, 3)))
+    floor, ceil, back, left, right = (b.add_material(alb()) for _ in range(5))
+    b.add_quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1), (0, 1, 0), floor)
+    b.add_quad((-1, 2, -1), (1, 2, -1), (1, 2, 1), (-1, 2, 1), (0, -1, 0), ceil)
+    b.add_quad((-1, 0, -1), (1, 0, -1), (1, 2, -1), (-1, 2, -1), (0, 0, 1), back)
+    b.add_quad((-1, 0, 1), (-1, 0, -1), (-1, 2, -1), (-1, 2, 1), (1, 0, 0), left)
+    b.add_quad((1, 0, -1), (1, 0, 1), (1, 2, 1), (1, 2, -1), (-1, 0, 0), right)
+    tall, short = b.add_material(alb()), b.add_material(alb())
+    yaw = rng.uniform(-math.pi / 4, math.pi / 4, 2)
+    b.add_box((-0.35, 0.6, -0.3), (0.3, 0.6, 0.3), yaw[0], tall)
+    b.add_box((0.35, 0.3, 0.3), (0.3, 0.3, 0.3), yaw[1], short)
+    light = b.add_material(material(diffuse=(0.0, 0.0, 0.0), emissive=(light_radiance,) * 3))
+    b.add_quad((-0.25, 1.98, -0.25), (0.25, 1.98, -0.25), (0.25, 1.98, 0.25), (-0.25, 1.98, 0.25), (0, -1, 0), light)
+    cam = dict(pos=(0.0, 1.0, 3.4), dir=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fov=math.radians(40.0))
+    return b.build(cam)
+
+
+def uv_sphere(center, radius, n_lon=8, n_lat=5):
+    """UV sphere with n_lat bands: 2*n_lon cap triangles + 2*n_lon*(n_lat-2) quads' triangles."""
+    pos, nrm, idx = [], [], []
+    for i in range(n_lat + 1):
+        th = math.pi * i / n_lat
+        for j in range(n_lon):
+            ph = 2 * math.pi * j / n_lon
+            n = (math.sin(th) * math.cos(ph), math.cos(th), math.sin(th) * math.sin(ph))
+            pos.append(tuple(center[k] + radius * n[k] for k in range(3)))
+            nrm.append(n)
+    vid = lambda i, j: i * n_lon + (j % n_lon)
+    for i in range(n_lat):
+        for j in range(n_lon):
+            a, b_, c, d = vid(i, j), vid(i, j + 1), vid(i + 1, j + 1), vid(i + 1, j)
+            if i == 0:
+                idx.append((a, c, d))
+            elif i == n_lat - 1:
+                idx.append((a, b_, d))
+            else:
+                idx.append((a, b_, c))
+                idx.append((a, c, d))
+    return pos, nrm, idx
+
+
+def icosphere(center, radius, subdiv=1):
+    t = (1.0 + 5 ** 0.5) / 2.0
+    v = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0), (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+         (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    v = [tuple(np.asarray(p) / np.linalg.norm(p)) for p in v]
+    f = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6),
+         (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11), (6, 2, 10),
+         (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache, nf = {}, []
+
+        def mid(a, b):
+            key = (min(a, b), max(a, b))
+            if key not in cache:
+                m = np.add(v[a], v[b]) / 2.0
+                v.append(tuple(m / np.linalg.norm(m)))
+                cache[key] = len(v) - 1
+            return cache[key]
+        for a, b_, c in f:
+            ab, bc, ca = mid(a, b_), mid(b_, c), mid(c, a)
+            nf += [(a, ab, ca), (b_, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        f = nf
+    pos = [tuple(center[k] + radius * p[k] for k in range(3)) for p in v]
+    return pos, v, f
+
+
+def sphere_grid(grid=64, seed=0x5EED0002, spacing=2.0, light_radiance=10.0):
+    """Streaming-BDPT workload: grid x grid 64-triangle spheres on a ground plane + 2 sphere lights.
+
+    64 x 64 spheres x 64 triangles = 262,144 sphere triangles (the stated total;
+    a 320-triangle subdivision-2 icosphere would give 1,310,720).  Radius
+    U(0.3, 1.0), jitter U(-0.5, 0.5), per-object BRDF parameters uniform random
+    (metallic, roughness U(0.05, 1), specular, sheen, clearcoat).  The lights are
+    emissive 80-triangle icospheres, because the reference estimator only knows
+    emissive triangles.
+    """
+    rng = np.random.default_rng(seed)
+    b = _Builder()
+    half = grid * spacing / 2.0
+    ground = b.add_material(material(diffuse=(0.5, 0.5, 0.5), specular=0.2, roughness=0.8))
+    b.add_quad((-half - 2, 0, half + 2), (half + 2, 0, half + 2), (half + 2, 0, -half - 2), (-half - 2, 0, -half - 2),
+               (0, 1, 0), ground)
+    radius = rng.uniform(0.3, 1.0, grid * grid)
+    jitter = rng.uniform(-0.5, 0.5, (grid * grid, 2))
+    par = rng.uniform(0.0, 1.0, (grid * grid, 6))
+    pos_all, nrm_all, idx_all, mat_all = [], [], [], []
+    base = 0
+    unit_pos, unit_nrm, unit_idx = uv_sphere((0.0, 0.0, 0.0), 1.0)
+    unit_pos = np.asarray(unit_pos, np.float64)
+    unit_nrm = np.asarray(unit_nrm, np.float32)
+    unit_idx = np.asarray(unit_idx, np.uint32)
+    for k in range(grid * grid):
+        gx, gz = k % grid, k // grid
+        c = np.array([-half + spacing * (gx + 0.5) + jitter[k, 0], radius[k], -half + spacing * (gz + 0.5) + jitter[k, 1]])
+        m = b.add_material(material(diffuse=tuple(0.2 + 0.6 * par[k, :3]), metallic=par[k, 3],
+                                    roughness=0.05 + 0.95 * par[k, 4], specular=par[k, 5],
+                                    sheen=rng.uniform(), clearcoat=rng.uniform()))
+        pos_all.append(c + radius[k] * unit_pos)
+        nrm_all.append(unit_nrm)
+        idx_all.append(unit_idx + (b.nv + base))
+        mat_all.append(np.full(len(unit_idx), m, np.uint32))
+        base += len(unit_pos)
+    scene = b.build(None)
+    lights = _Builder()
+    lights.nv = scene["pos"].shape[0] + base
+    lm = len(b.mats)
+    lmat = material(diffuse=(0.0, 0.0, 0.0), emissive=(light_radiance,) * 3)
+    for center in ((-half / 2, 24.0, -half / 3), (half / 2, 30.0, half / 3)):
+        p, n, f = icosphere(center, 4.0, 1)
+        lights.add_mesh(p, n, f, lm)
+    mats = np.concatenate([scene["mats"], np.asarray([lmat], MATERIAL_DTYPE)])
+    cam = dict(pos=(0.0, 38.0, half + 30.0), dir=tuple(np.array([0.0, -38.0, -(half + 30.0)]) /
+                                                      np.linalg.norm([0.0, -38.0, -(half + 30.0)])),
+               up=(0.0, 1.0, 0.0), fov=math.radians(45.0))
+    return dict(
+        pos=np.concatenate([scene["pos"], np.concatenate(pos_all).astype(np.float32),
+                            np.asarray(lights.pos, np.float32)]),
+        nrm=np.concatenate([scene["nrm"], np.concatenate(nrm_all), np.asarray(lights.nrm, np.float32)]),
+        idx=np.concatenate([scene["idx"], np.concatenate(idx_all), np.asarray(lights.idx, np.uint32)]),
+        mat=np.concatenate([scene["mat"], np.concatenate(mat_all), np.asarray(lights.mat, np.uint32)]),
+        mats=mats, camera=cam)
+
+
+def read_obj(text: str):
+    """Positions and faces of an OBJ file; polygons are fan-triangulated
+    (assimp's aiProcess_Triangulate order for convex faces)."""
+    pos, idx = [], []
+    for line in text.splitlines():
+        parts = line.split()
+        if not parts:
+            continue
+        if parts[0] == "v":
+            pos.append(tuple(float(x) for x in parts[1:4]))
+        elif parts[0] == "f":
+            vs = [int(p.split("/")[0]) for p in parts[1:]]
+            vs = [v - 1 if v > 0 else len(pos) + v for v in vs]
+            for k in range(1, len(vs) - 1):
+                idx.append((vs[0], vs[k], vs[k + 1]))
+    return np.asarray(pos, np.float32).reshape(-1, 3), np.asarray(idx, np.uint32).reshape(-1, 3)
+
+
+def mesh_scene(pos, idx, camera=None):
+    """A scene around a bare mesh: flat normals, one grey material."""
+    pos = np.asarray(pos, np.float32)
+    idx = np.asarray(idx, np.uint32)
+    p = pos[idx]  # nt x 3 x 3
+    n = np.cross(p[:, 1] - p[:, 0], p[:, 2] - p[:, 0])
+    n /= np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+    # unshared vertices so flat normals survive
+    mats = np.asarray([material()], MATERIAL_DTYPE)
+    return dict(pos=p.reshape(-1, 3).astype(np.float32), nrm=np.repeat(n, 3, axis=0).astype(np.float32),
+                idx=np.arange(len(idx) * 3, dtype=np.uint32).reshape(-1, 3),
+                mat=np.zeros(len(idx), np.uint32), mats=mats, camera=camera)

@@ -1,0 +1,198 @@
+/* cpt.h -- C ABI of libcpt_hip.so, the MI355X (gfx950) streaming-BDPT hot path.
+ *
+ * Plain C types only: opaque handles, plain pointers and sizes.  Every entry
+ * point returns 0 on success and a negative CPT_E_* code on failure, never
+ * exits, and records a message readable through cpt_last_error() (thread-local).
+ * The caller owns every host array; the library owns device memory and one
+ * HIP stream per renderer.  Handles are not shared across threads.
+ *
+ * Reference interfaces each group replaces are cited per declaration, paths
+ * relative to the namopth/CudaPT checkout.
+ */
+#ifndef CPT_H
+#define CPT_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  CPT_OK = 0,
+  CPT_E_ARG = -1,      /* bad argument (null handle, size, mode, index) */
+  CPT_E_STATE = -2,    /* call out of order (no mesh, no BVH, not uploaded) */
+  CPT_E_HIP = -3,      /* a HIP runtime call failed */
+  CPT_E_NOMEM = -4,    /* device or host allocation failed */
+  CPT_E_LIMIT = -5     /* a size exceeds a documented maximum */
+};
+
+typedef struct cpt_scene cpt_scene;
+typedef struct cpt_renderer cpt_renderer;
+
+/* RTMaterial, field for field (include/raytracer.h:82-100), 80 bytes. */
+typedef struct cpt_material {
+  float diffuse[3];
+  float emissive[3];
+  int32_t diffuse_tex_id, normal_tex_id, emissive_tex_id, mat_type;
+  float transparency, subsurface, specular, metallic, roughness, anisotropic;
+  float sheen, sheen_tint, clearcoat, clearcoat_gloss;
+} cpt_material;
+
+/* One TracePrimitive query (src/cudaRTCommon.cu:66; include/cudaRTCommon.h:61):
+ * CURay origin/direction plus the maxDist and rayEpsilon arguments. 32 bytes. */
+typedef struct cpt_ray {
+  float ox, oy, oz, max_dist;
+  float dx, dy, dz, eps;
+} cpt_ray;
+
+/* TracePrimitiveResult (include/cudaRTCommon.h:52-59) with the triangle given
+ * by its ORIGINAL index (the reference reports the BVH-reordered index).
+ * Miss: tri = 0xFFFFFFFF, t = 1e20, u = v = 0.  w = (1 - u) - v. 16 bytes. */
+typedef struct cpt_hit {
+  uint32_t tri;
+  float t, u, v;
+} cpt_hit;
+
+/* Last error message of the calling thread ("" when none). */
+const char* cpt_last_error(void);
+
+/* Number of visible HIP devices (0 when the runtime finds none). */
+int cpt_device_count(int* n);
+
+/* Device memory through the library's own HIP runtime, for callers that hold
+ * rays and hits in HBM without a HIP runtime of their own (no reference
+ * counterpart: the reference allocates only inside Render). */
+int cpt_device_alloc(void** ptr, uint64_t bytes);
+int cpt_device_free(void* ptr);
+int cpt_memcpy_htod(void* dst, const void* src, uint64_t bytes);
+int cpt_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+int cpt_device_synchronize(void);
+
+/* ---- scene (RTScene, include/raytracer.h:228-290) ---------------------- */
+
+/* RTScene::RTScene (include/raytracer.h:247). */
+int cpt_scene_create(cpt_scene** out);
+
+/* Mesh arrays replacing RTScene::m_pVertices / m_pTriangles as filled by
+ * RTScene::AddModel (src/raytracer.cpp:238-266).  nrm3 may be NULL (zero
+ * normals); uv2 may be NULL.  Invalidates any BVH and device copy. */
+int cpt_scene_set_mesh(cpt_scene* s, const float* pos3, const float* nrm3, const float* uv2, uint32_t nv,
+                       const uint32_t* idx3, const uint32_t* mat, uint32_t nt);
+
+/* RTScene::m_pMaterials (include/raytracer.h:275). */
+int cpt_scene_set_materials(cpt_scene* s, const cpt_material* m, uint32_t n);
+
+/* updateAllSceneMaterialsCudaMem (include/cudaRTCommon.h:95,
+ * src/cudaRTCommon.cu:580-599) + updateLightTriCudaMem
+ * (src/cudaRTBDPTStream.cu:57-77): pushes the current materials to the device
+ * and restarts accumulation of every renderer drawing this scene, as the
+ * reference's material dirty flag does (src/cudaRTBDPTStream.cu:845-850). */
+int cpt_scene_update_materials(cpt_scene* s);
+
+/* NPBVHHelper::CreateBVH + CompactBVH::InitialCompactBVH
+ * (include/bvhhelper.h:43-54, src/bvhhelper.cpp:55-311): binned SAH, 16 bins,
+ * depth <= max_depth (reference default 32), then the triangle reorder of
+ * src/raytracer.cpp:267-279 and the 4-wide device re-layout. */
+int cpt_bvh_build_sah(cpt_scene* s, uint32_t max_depth);
+
+/* The CompactBVH arrays: bounds6 = node_n x {min xyz, max xyz}, off2 = node_n x
+ * {offset-or-triStart, triN}, reorder = nt leaf-order -> original triangle ids.
+ * Pass NULL arrays to query node_n only. */
+int cpt_bvh_export(cpt_scene* s, float* bounds6, uint32_t* off2, uint32_t* reorder, uint32_t* node_n);
+
+/* Statistics of the 4-wide device BVH: node count, depth, leaf count. */
+int cpt_bvh4_info(cpt_scene* s, uint32_t* node_n, uint32_t* depth, uint32_t* leaf_n);
+
+/* initAllSceneCudaMem (include/cudaRTCommon.h:94, src/cudaRTCommon.cu:472-578):
+ * device copies of nodes, triangle intersection data, vertices, materials. */
+int cpt_scene_upload(cpt_scene* s);
+
+/* freeAllBVHCudaMem (include/cudaRTCommon.h:92) + destruction. */
+int cpt_scene_destroy(cpt_scene* s);
+
+/* ---- traversal (TracePrimitive, src/cudaRTCommon.cu:66-127) ------------ */
+
+/* Batched closest hit over host arrays (copies in and out). cull != 0 enables
+ * the back-face test of IntersectTri (src/cudaRTCommon.cu:44). */
+int cpt_trace_closest(cpt_scene* s, const cpt_ray* rays, uint32_t n, uint32_t cull, cpt_hit* hits);
+
+/* Same over device arrays already resident in HBM, launched on `stream`
+ * (a hipStream_t, NULL = the scene's stream).  layout 0 = 4-wide nodes,
+ * 1 = the reference's binary CompactBVH layout. */
+int cpt_trace_closest_device(cpt_scene* s, const cpt_ray* d_rays, uint32_t n, uint32_t cull, cpt_hit* d_hits,
+                             void* stream, uint32_t layout);
+
+/* Occlusion form used by the light-vertex connection
+ * (src/cudaRTBDPTStream.cu:685-686): d_occluded[i] = 1 when any triangle lies
+ * at 0 <= t < max_dist. Device arrays. */
+int cpt_trace_occluded_device(cpt_scene* s, const cpt_ray* d_rays, uint32_t n, uint32_t cull, uint32_t* d_occluded,
+                              void* stream);
+
+/* Node visits and triangle tests of every traversal since the last reset. */
+int cpt_trace_counters(cpt_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* rays, int reset);
+
+/* ---- renderer (RTRenderer / <ns>::Render, include/raytracer.h:292-323,
+ *      src/raytracer.cpp:19-31,574-625) ---------------------------------- */
+
+enum {
+  CPT_MODE_BDPT_STREAM = 0,       /* cudaRTBDPTStream (src/cudaRTBDPTStream.cu) */
+  CPT_MODE_BDPT_ADAPTIVE_PLL = 1  /* cudaRTBDPTStreamAdapBilatAtomic (src/cudaRTBDPTStreamAdapBilatAtomic.cu) */
+};
+
+/* RTRenderer::Init(w, h) + SetRendererMode(mode) (src/raytracer.cpp:574-608). */
+int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t height);
+
+/* One <ns>::Render call (src/cudaRTBDPTStream.cu:816-951): one eye path per
+ * pixel, light-vertex cache rebuilt on every third frame, running mean updated.
+ * Accumulation restarts when (pos, dir, up) differ from the previous call.
+ * result (optional, width*height*3 floats, row y = 0 at the bottom) receives
+ * the running mean, as the reference's D2H copy (:949). */
+int cpt_render_iteration(cpt_renderer* r, cpt_scene* s, const float cam_pos[3], const float cam_dir[3],
+                         const float cam_up[3], float fov, float* result);
+
+/* n_frames consecutive Render calls with one host synchronisation at the end;
+ * identical results to n_frames calls of cpt_render_iteration. */
+int cpt_render_frames(cpt_renderer* r, cpt_scene* s, const float cam_pos[3], const float cam_dir[3],
+                      const float cam_up[3], float fov, uint32_t n_frames, float* result);
+
+/* <ns>::GetAttribsN / GetAttribute (src/raytracer.cpp:26-27,
+ * include/raytracer.h:32-44).  type: 0 uint, 1 int, 2 bool, 3 float, 4 enum. */
+int cpt_renderer_attrib_count(cpt_renderer* r, uint32_t* n);
+int cpt_renderer_get_attrib(cpt_renderer* r, uint32_t idx, char* name, uint32_t name_cap, int* type, double* value);
+int cpt_renderer_set_attrib(cpt_renderer* r, uint32_t idx, double value);
+
+/* Restart accumulation (frame counter to 0), as a camera change does. */
+int cpt_renderer_reset(cpt_renderer* r);
+
+enum {
+  CPT_BUF_RESULT = 0,     /* float x3 per pixel: running mean */
+  CPT_BUF_SAMPLE_N = 1,   /* uint32 per pixel: accumulated sample count */
+  CPT_BUF_GBUF_POS = 2,   /* float x3: first-hit position (adaptive mode) */
+  CPT_BUF_GBUF_NORM = 3,  /* float x3: first-hit normal */
+  CPT_BUF_GBUF_DIFF = 4,  /* float x3: first-hit albedo */
+  CPT_BUF_FILTERED = 5,   /* float x3: cross-bilateral output */
+  CPT_BUF_ERROR = 6       /* float x1: per-pixel error estimate */
+};
+int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out);
+
+/* Work and time counters since the last reset: rays traced (extension +
+ * shadow), eye paths completed, frames, and HIP-event milliseconds of the
+ * closest-hit kernel, all kernels, and the light pass. */
+typedef struct cpt_stats {
+  uint64_t extension_rays, shadow_rays, eye_paths, frames;
+  uint64_t extend_launches, extend_nodes, extend_tris;
+  double extend_ms, shadow_ms, shade_ms, light_ms, total_ms;
+} cpt_stats;
+int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset);
+
+/* Enables per-launch HIP-event timing of every kernel (off by default). */
+int cpt_renderer_set_profiling(cpt_renderer* r, int on);
+
+/* <ns>::CleanMem (src/raytracer.cpp:25) + destruction. */
+int cpt_renderer_destroy(cpt_renderer* r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CPT_H */

@@ -1,0 +1,222 @@
+"""ctypes front end of the CPU oracle (oracle/oracle.cpp).
+
+Test infrastructure only: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+u32p = np.ctypeslib.ndpointer(np.uint32, flags="C_CONTIGUOUS")
+f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        vp, c32, c64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64
+        L.orc_scene_create.restype = vp
+        L.orc_scene_create.argtypes = [f32p, f32p, c32, u32p, u32p, c32, vp, c32, c32]
+        L.orc_scene_destroy.argtypes = [vp]
+        L.orc_scene_bvh.restype = c32
+        L.orc_scene_bvh.argtypes = [vp, vp, vp, vp, c32]
+        L.orc_scene_light_tris.restype = c32
+        L.orc_scene_light_tris.argtypes = [vp, vp, c32]
+        L.orc_trace.argtypes = [vp, f32p, c32, c32, c32, c32, vp, vp]
+        L.orc_renderer_create.restype = vp
+        L.orc_renderer_create.argtypes = [vp, c32]
+        L.orc_renderer_destroy.argtypes = [vp]
+        L.orc_render_params_size.restype = c32
+        L.orc_render.argtypes = [vp, vp, c32, c32, vp, vp, vp, vp]
+        L.orc_render_result64.argtypes = [vp, f64p]
+        L.orc_render_counts.argtypes = [vp, np.ctypeslib.ndpointer(np.uint64, flags="C_CONTIGUOUS")]
+        L.orc_max_threads.restype = ctypes.c_int
+        L.orc_light_vertices.restype = c32
+        L.orc_light_vertices.argtypes = [vp, f32p, c32]
+        L.orc_pcg32_stream.restype = c32
+        L.orc_pcg32_stream.argtypes = [c64, c64, c32, u32p]
+        L.orc_stream_key.restype = c64
+        L.orc_stream_key.argtypes = [c64, c32, c32, c32]
+        L.orc_bilateral.argtypes = [f32p, f32p, f32p, vp, c32, c32, ctypes.c_float, ctypes.c_float,
+                                    ctypes.c_float, ctypes.c_float, c32, c32, f32p]
+        L.orc_gaussian_table.argtypes = [ctypes.c_float, c32, f32p]
+        L.orc_square_error.argtypes = [f32p, f32p, u32p, c32, c32, f32p]
+        L.orc_empirical_error.argtypes = [f32p, f32p, u32p, c32, f32p]
+        L.orc_alloc_proportional.argtypes = [f32p, c32, ctypes.c_float, c32, u32p]
+        L.orc_det_eps.restype = ctypes.c_float
+        _lib = L
+    return _lib
+
+
+HIT_DTYPE = np.dtype([("tri", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
+
+RENDER_PARAMS_DTYPE = np.dtype([
+    ("width", "<u4"), ("height", "<u4"), ("cam_pos", "<f4", 3), ("cam_dir", "<f4", 3), ("cam_up", "<f4", 3),
+    ("fov", "<f4"), ("eye_depth", "<u4"), ("light_depth", "<u4"), ("light_paths", "<u4"), ("flags", "<u4"),
+    ("seed", "<u8"), ("row0", "<u4"), ("rows", "<u4")])
+
+
+def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:
+    out = np.zeros(n, np.uint32)
+    lib().orc_pcg32_stream(initstate, initseq, n, out)
+    return out
+
+
+class Scene:
+    """Oracle scene: literal bvhhelper build + BVH-leaf-order triangle data."""
+
+    def __init__(self, sc: dict, max_depth: int = 32):
+        L = lib()
+        self.pos = np.ascontiguousarray(sc["pos"], np.float32)
+        self.nrm = np.ascontiguousarray(sc["nrm"], np.float32)
+        self.idx = np.ascontiguousarray(sc["idx"], np.uint32)
+        self.mat = np.ascontiguousarray(sc["mat"], np.uint32)
+        self.mats = np.ascontiguousarray(sc["mats"])
+        self.h = L.orc_scene_create(self.pos.ravel(), self.nrm.ravel(), len(self.pos), self.idx.ravel(), self.mat,
+                                    len(self.idx), self.mats.ctypes.data, len(self.mats), max_depth)
+        if not self.h:
+            raise RuntimeError("oracle: BVH build failed")
+        self.nt = len(self.idx)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_scene_destroy(self.h)
+            self.h = None
+
+    def bvh(self):
+        L = lib()
+        n = L.orc_scene_bvh(self.h, None, None, None, 0)
+        bounds = np.zeros((n, 6), np.float32)
+        off = np.zeros((n, 2), np.uint32)
+        reorder = np.zeros(self.nt, np.uint32)
+        L.orc_scene_bvh(self.h, bounds.ctypes.data, off.ctypes.data, reorder.ctypes.data, n)
+        return bounds, off, reorder
+
+    def light_tris(self):
+        n = lib().orc_scene_light_tris(self.h, None, 0)
+        out = np.zeros(max(n, 1), np.uint32)
+        lib().orc_scene_light_tris(self.h, out.ctypes.data, n)
+        return out[:n]
+
+    def trace(self, rays: np.ndarray, mode: str = "brute", prec: int = 32, cull: bool = False):
+        """rays: n x 8 float32 {ox,oy,oz,maxDist,dx,dy,dz,eps}. Returns (hits, stats)."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        hits = np.zeros(len(rays), HIT_DTYPE)
+        stats = np.zeros(2, np.uint32)
+        m = {"brute": 0, "defined": 1, "reference": 2}[mode]
+        lib().orc_trace(self.h, rays.ravel(), len(rays), int(cull), m, prec, hits.ctypes.data, stats.ctypes.data)
+        return hits, stats
+
+
+def max_threads() -> int:
+    return int(lib().orc_max_threads())
+
+
+def render_params(width, height, camera, eye_depth=5, light_depth=5, light_paths=128, seed=0x5EED0001, flags=0,
+                  row0=0, rows=0):
+    p = np.zeros((), RENDER_PARAMS_DTYPE)
+    p["width"], p["height"] = width, height
+    p["cam_pos"], p["cam_dir"], p["cam_up"] = camera["pos"], camera["dir"], camera["up"]
+    p["fov"] = camera["fov"]
+    p["eye_depth"], p["light_depth"], p["light_paths"] = eye_depth, light_depth, light_paths
+    p["seed"] = seed
+    p["flags"] = flags
+    p["row0"], p["rows"] = row0, rows
+    return p
+
+
+class Renderer:
+    """Stateful reference-estimator run (running mean persists across calls)."""
+
+    def __init__(self, params, prec: int = 32):
+        assert lib().orc_render_params_size() == RENDER_PARAMS_DTYPE.itemsize
+        self.p = np.ascontiguousarray(params)
+        self.prec = prec
+        self.h = lib().orc_renderer_create(self.p.ctypes.data, prec)
+        self.W, self.H = int(params["width"]), int(params["height"])
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().orc_renderer_destroy(self.h)
+            self.h = None
+
+    def render(self, scene: Scene, frame0: int, nframes: int, gbuf: bool = False):
+        n = self.W * self.H
+        res = np.zeros((self.H, self.W, 3), np.float32)
+        N = np.zeros((self.H, self.W), np.uint32)
+        g = np.zeros((self.H, self.W, 9), np.float32) if gbuf else None
+        lvn = np.zeros(1, np.uint32)
+        lib().orc_render(scene.h, self.h, frame0, nframes, res.ctypes.data, N.ctypes.data,
+                         g.ctypes.data if gbuf else None, lvn.ctypes.data)
+        self.light_vertex_count = int(lvn[0])
+        return (res, N, g) if gbuf else (res, N)
+
+    def ray_counts(self):
+        out = np.zeros(2, np.uint64)
+        lib().orc_render_counts(self.h, out)
+        return int(out[0]), int(out[1])
+
+    def result64(self):
+        out = np.zeros(self.W * self.H * 3, np.float64)
+        lib().orc_render_result64(self.h, out)
+        return out.reshape(self.H, self.W, 3)
+
+    def light_vertices(self):
+        out = np.zeros((2048, 21), np.float32)
+        n = lib().orc_light_vertices(self.h, out.ravel(), 2048)
+        return out[:n]
+
+
+def bilateral(color, pos, norm, emit, sc, sp, sn, se, radius, prec=32):
+    H, W = color.shape[:2]
+    out = np.zeros((H, W, 3), np.float32)
+    e = np.ascontiguousarray(emit, np.float32) if emit is not None else None
+    lib().orc_bilateral(np.ascontiguousarray(color, np.float32).ravel(), np.ascontiguousarray(pos, np.float32).ravel(),
+                        np.ascontiguousarray(norm, np.float32).ravel(), e.ctypes.data if e is not None else None,
+                        W, H, sc, sp, sn, se, radius, prec, out.ravel())
+    return out
+
+
+def gaussian_table(sigma, radius):
+    out = np.zeros(2 * radius + 1, np.float32)
+    lib().orc_gaussian_table(sigma, radius, out)
+    return out
+
+
+def square_error(filtered, raw, N, nd):
+    n = N.size
+    out = np.zeros(n, np.float32)
+    lib().orc_square_error(np.ascontiguousarray(filtered, np.float32).ravel(),
+                           np.ascontiguousarray(raw, np.float32).ravel(),
+                           np.ascontiguousarray(N, np.uint32).ravel(), nd, n, out)
+    return out.reshape(N.shape)
+
+
+def empirical_error(mean, sq, N):
+    n = N.size
+    out = np.zeros(n, np.float32)
+    lib().orc_empirical_error(np.ascontiguousarray(mean, np.float32).ravel(), np.ascontiguousarray(sq, np.float32).ravel(),
+                              np.ascontiguousarray(N, np.uint32).ravel(), n, out)
+    return out.reshape(N.shape)
+
+
+def alloc_proportional(err, total, gen):
+    out = np.zeros(err.size, np.uint32)
+    lib().orc_alloc_proportional(np.ascontiguousarray(err, np.float32).ravel(), err.size, total, gen, out)
+    return out.reshape(err.shape)

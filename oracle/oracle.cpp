@@ -1,0 +1,1033 @@
+// oracle/oracle.cpp -- CPU restatement of CudaPT's streaming-BDPT hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  Nothing in the product library links, loads or
+// calls this file; only tests/, __graft_entry__.smoke() and bench.py's
+// cpu_baseline leg load liboracle.so, and only as the checker (or as the timed
+// CPU baseline).  The product path fails loudly when its HIP library is absent.
+//
+// What each function restates (paths relative to the reference checkout):
+//   pcg32_*            PCG32, O'Neill 2014 (pcg-c-basic), replaces cuRAND XORWOW
+//                      (cudaRTBDPTStream.cu:552,633) -- pinned by the published
+//                      pcg32-demo known-answer vector (tests/golden).
+//   splitmix64         Steele/Lea/Flood 2014 finaliser -- pinned by its KAT.
+//   bvh_build_literal  NPBVHHelper::CreateBVH / InitialBVHNode /
+//                      InitialCompactBVH (src/bvhhelper.cpp:33-311), kept in
+//                      the reference's per-bin loop form -- pinned by
+//                      hand-worked 2..8 triangle cases and structural invariants.
+//   intersect_tri      CURay::IntersectTri (src/cudaRTCommon.cu:42-64) with the
+//                      device arithmetic of include/cudahelper.h:127-198 --
+//                      pinned by closed-form triangles.
+//   intersect_aabb     CURay::IntersectAABB (src/cudaRTCommon.cu:24-40).
+//   trace_refbin       TracePrimitive (src/cudaRTCommon.cu:66-127): right-first
+//                      DFS, 32-entry stack dropping pushes at 29, 2048-pop cap,
+//                      strict '<' first-found minimum.
+//   trace_brute        All-triangle minimum of (t, original triangle id): the
+//                      tie-defined closest hit the product promises.
+//   trace_defined      Same answer as trace_brute through the binary BVH with
+//                      conservative box culling -- pinned against trace_brute.
+//   render_frames      cudaRTBDPTStream::Render (src/cudaRTBDPTStream.cu:
+//                      171-951): light-vertex cache every 3rd frame, eye bounce,
+//                      light-vertex connection (N += 4), running-mean apply.
+//                      Pinned by the emitter-only and all-black scenes and the
+//                      hand-checked single-connection case; the FP32 GPU run is
+//                      held to relative MSE 1e-4 of the FP64 run of this code.
+//   bilateral_posnorm  cudaBilateralFilter::updateGaussian and
+//                      bilaterial_posnorm(emit)_kernel
+//                      (src/cudaBilateralFilter.cu:5-142), including the
+//                      mixed int/uint clamp that sends low-side taps to the
+//                      high edge -- pinned by radius-0 identity, constant-image
+//                      fixed point and the large-sigma Gaussian closed form.
+//   square_error       pt_calculateSquareError_kernel
+//                      (src/cudaRTBDPTStreamAdapBilatAtomic.cu:1143-1156).
+//   empirical_error    pt_calculateEmpiricalSquareError_kernel
+//                      (src/cudaRTBDPTStreamAdapEmp.cu:949-960).
+//   alloc_proportional pt_genAdapPathQueue_kernel sample counts
+//                      (src/cudaRTBDPTStreamAdapBilatAtomic.cu:817-842).
+//
+// Precision: every numeric routine is a template over R in {float, double}.
+// R = float reproduces the device operation sequence (explicit fma nesting in
+// dot/cross, correctly rounded divide, round-down sqrt in length()); R = double
+// is the FP64 run the FP32 GPU radiance is judged against.  Random uniforms are
+// 24-bit ((x >> 8) + 1) * 2^-24, exact in both precisions.
+//
+// Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp).
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <cstdio>
+#include <vector>
+#include <algorithm>
+#include <limits>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef uint32_t u32;
+typedef int32_t i32;
+typedef uint64_t u64;
+
+namespace orc {
+
+// ---------------------------------------------------------------- constants
+// include/cudahelper.h:27-30
+static const double kEpsilon = 1e-9;     // M_EPSILON
+static const double kFltEps = 1e-5;      // M_FLT_EPSILON
+static const double kBiasEps = 1e-4;     // M_FLT_BIAS_EPSILON
+static const double kInf = 1e20;         // M_INF
+static const double kPi = 3.14159265358979323846;
+
+// ---------------------------------------------------------------- RNG
+struct Pcg32 { u64 state, inc; };
+static inline u32 pcg32_next(Pcg32& r) {
+  u64 old = r.state;
+  r.state = old * 6364136223846793005ULL + r.inc;
+  u32 xs = (u32)(((old >> 18u) ^ old) >> 27u);
+  u32 rot = (u32)(old >> 59u);
+  return (xs >> rot) | (xs << ((-rot) & 31u));
+}
+static inline void pcg32_seed(Pcg32& r, u64 initstate, u64 initseq) {
+  r.state = 0u;
+  r.inc = (initseq << 1u) | 1u;
+  pcg32_next(r);
+  r.state += initstate;
+  pcg32_next(r);
+}
+static inline u64 splitmix64_mix(u64 z) {
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+// Stream key for (seed, frame, domain, sample): the counter-based analogue of
+// curand_init(WangHash(frameN) + index, 0, 0) (cudaRTBDPTStream.cu:552,633).
+static inline u64 stream_key(u64 seed, u32 frame, u32 domain, u32 sample) {
+  return splitmix64_mix(seed + 0x9E3779B97F4A7C15ULL * (u64)(frame + 1) +
+                        ((u64)sample << 32) + ((u64)domain << 60));
+}
+// Uniform in (0, 1]: same support as curand_uniform, exact in FP32 and FP64.
+template <class R> static inline R uniform(Pcg32& r) {
+  return (R)((pcg32_next(r) >> 8) + 1u) * (R)(1.0 / 16777216.0);
+}
+enum { DOMAIN_EYE = 1, DOMAIN_LIGHT = 2 };
+
+// ---------------------------------------------------------------- vectors
+template <class R> struct V3 { R x, y, z; };
+template <class R> static inline V3<R> mk(R x, R y, R z) { V3<R> v; v.x = x; v.y = y; v.z = z; return v; }
+template <class R> static inline R fm(R a, R b, R c) { return std::fma(a, b, c); }
+template <class R> static inline V3<R> operator+(V3<R> a, V3<R> b) { return mk(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <class R> static inline V3<R> operator-(V3<R> a, V3<R> b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <class R> static inline V3<R> operator*(V3<R> a, R s) { return mk(a.x * s, a.y * s, a.z * s); }
+template <class R> static inline V3<R> operator*(R s, V3<R> a) { return a * s; }
+template <class R> static inline V3<R> operator/(V3<R> a, R s) { return mk(a.x / s, a.y / s, a.z / s); }
+template <class R> static inline V3<R> vmul(V3<R> a, V3<R> b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
+template <class R> static inline V3<R> vdiv(V3<R> a, V3<R> b) { return mk(a.x / b.x, a.y / b.y, a.z / b.z); }
+template <class R> static inline V3<R> vmax(V3<R> a, V3<R> b) { return mk(std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)); }
+template <class R> static inline V3<R> vmin(V3<R> a, V3<R> b) { return mk(std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)); }
+// cudahelper.h:179-198 -- explicit fma nesting of the device dot/cross.
+template <class R> static inline R dot(V3<R> a, V3<R> b) { return fm(a.x, b.x, fm(a.y, b.y, a.z * b.z)); }
+template <class R> static inline V3<R> cross(V3<R> a, V3<R> b) {
+  return mk(fm(a.y, b.z, -(a.z * b.y)), fm(a.z, b.x, -(a.x * b.z)), fm(a.x, b.y, -(a.y * b.x)));
+}
+// cudahelper.h:292-299: __fsqrt_rd of the (contracted) sum of squares.
+template <class R> static inline R sqrt_rd(R x) {
+  R s = std::sqrt(x);
+  if (s > (R)0 && std::fma(s, s, -x) > (R)0) s = std::nextafter(s, (R)0);
+  return s;
+}
+template <class R> static inline R length(V3<R> a) { return sqrt_rd(fm(a.z, a.z, fm(a.y, a.y, a.x * a.x))); }
+template <class R> static inline V3<R> normalize(V3<R> a) { R l = length(a); return mk(a.x / l, a.y / l, a.z / l); }
+template <class R> static inline R lerp(R x, R y, R d) { return x + (y - x) * d; }
+template <class R> static inline V3<R> vlerp(V3<R> x, V3<R> y, R d) { return x + (y - x) * d; }
+template <class R> static inline R saturate(R x) { return x < (R)0 ? (R)0 : (x > (R)1 ? (R)1 : x); }
+template <class R> static inline V3<R> cvt(const float* p) { return mk((R)p[0], (R)p[1], (R)p[2]); }
+template <class R> static inline bool isinf3(V3<R> a) { return std::isinf(a.x) || std::isinf(a.y) || std::isinf(a.z); }
+
+// smallest float >= 1e-9: for a float x, x < kDetEpsF  <=>  (double)x < 1e-9.
+static float det_eps_f() {
+  float f = (float)kEpsilon;
+  if ((double)f < kEpsilon) f = std::nextafter(f, 1.0f);
+  return f;
+}
+template <class R> static inline R det_eps();
+template <> inline float det_eps<float>() { static const float v = det_eps_f(); return v; }
+template <> inline double det_eps<double>() { return kEpsilon; }
+
+// ---------------------------------------------------------------- scene
+struct Material {            // RTMaterial field order (include/raytracer.h:82-100)
+  float diffuse[3];
+  float emissive[3];
+  i32 diffuseTexId, normalTexId, emissiveTexId, matType;
+  float transparency, subsurface, specular, metallic, roughness, anisotropic;
+  float sheen, sheenTint, clearcoat, clearcoatGloss;
+};
+static_assert(sizeof(Material) == 80, "RTMaterial is 80 bytes");
+
+struct Scene {
+  u32 nv = 0, nt = 0;
+  std::vector<float> pos, nrm;       // 3 per vertex
+  std::vector<u32> tri;              // 3 per triangle, BVH leaf order
+  std::vector<u32> tmat;             // material per triangle, leaf order
+  std::vector<u32> orig;             // leaf order -> original triangle id
+  std::vector<Material> mats;
+  std::vector<float> p0, e0, e1;     // 3 per leaf-order triangle (float, host-subtracted)
+  // compact binary BVH (bvhhelper.h:45-54)
+  u32 nodeN = 0;
+  std::vector<float> bmin, bmax;     // 3 per node
+  std::vector<u32> offOrTSTN;        // 2 per node
+  std::vector<u32> lightTri;         // leaf-order emissive triangles
+};
+
+// ---------------------------------------------------------------- BVH build
+// src/bvhhelper.cpp, restated literally (float host arithmetic, no contraction).
+struct F3 { float x, y, z; float& operator[](int i) { return (&x)[i]; } float operator[](int i) const { return (&x)[i]; } };
+static inline F3 f3(float x, float y, float z) { F3 r; r.x = x; r.y = y; r.z = z; return r; }
+static inline F3 f3min(F3 a, F3 b) { return f3(std::fmin(a.x, b.x), std::fmin(a.y, b.y), std::fmin(a.z, b.z)); }
+static inline F3 f3max(F3 a, F3 b) { return f3(std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)); }
+struct Box { F3 mn, mx; };
+static inline Box empty_box() { Box b; b.mn = f3((float)kInf, (float)kInf, (float)kInf); b.mx = f3((float)-kInf, (float)-kInf, (float)-kInf); return b; }
+static inline Box merge(Box a, Box b) { Box r; r.mn = f3min(a.mn, b.mn); r.mx = f3max(a.mx, b.mx); return r; }
+static inline Box merge(Box a, F3 p) { Box r; r.mn = f3min(a.mn, p); r.mx = f3max(a.mx, p); return r; }
+// bvhhelper.cpp:33-39
+static float surface_area(const Box& b) {
+  F3 d = f3(b.mx.x - b.mn.x, b.mx.y - b.mn.y, b.mx.z - b.mn.z);
+  float len = std::sqrt(d.x * d.x + d.y * d.y + d.z * d.z);
+  if (d.x < 0 || d.y < 0 || d.z < 0 || (double)len < kEpsilon) return 0;
+  return d.x * d.y + d.x * d.z + d.y * d.z;
+}
+struct BNode { BNode* c[2] = {nullptr, nullptr}; Box bound = empty_box(); u32 triStart = 0, triN = 0, desN = 0; };
+struct BTri { F3 centroid; Box bound; };
+static const int kBins = 16;
+
+static void free_tree(BNode* n) { if (!n) return; free_tree(n->c[0]); free_tree(n->c[1]); delete n; }
+
+// bvhhelper.cpp:55-237
+static void build_node(BNode* node, const BTri* tris, u32* order, u32 triStart, u32 triN, u32 depthBudget) {
+  if (triN <= 0) return;
+  Box cb = empty_box();
+  for (u32 i = 0; i < triN; i++) {
+    u32 ind = order[triStart + i];
+    node->bound = merge(node->bound, tris[ind].bound);
+    cb = merge(cb, tris[ind].centroid);
+  }
+  F3 dia = f3(cb.mx.x - cb.mn.x, cb.mx.y - cb.mn.y, cb.mx.z - cb.mn.z);
+  if (dia.x < 0 || dia.y < 0 || dia.z < 0) return;
+  float dlen = std::sqrt(dia.x * dia.x + dia.y * dia.y + dia.z * dia.z);
+  if (triN == 1 || depthBudget <= 1 || (double)dlen < kEpsilon) {
+    node->triStart = triStart; node->triN = triN; node->desN = 0; return;
+  }
+  u32 axis = 0;
+  if (std::fabs(dia.y) > std::fabs(dia.x)) axis = 1;
+  if (std::fabs(dia.z) > std::fabs(dia.y) && std::fabs(dia.z) > std::fabs(dia.x)) axis = 2;
+  float len = std::fabs(dia[axis]);
+  // bins, literal per-bin scan (bvhhelper.cpp:86-111)
+  u32 binN[kBins]; Box binB[kBins];
+  for (int i = 0; i < kBins; i++) {
+    binN[i] = 0; binB[i] = empty_box();
+    float s = cb.mn[axis] + (float)i * len / (float)kBins;
+    float e = cb.mn[axis] + (float)(i + 1) * len / (float)kBins;
+    if (i + 1 == kBins) e = (float)((double)cb.mx[axis] + kBiasEps);
+    for (u32 j = 0; j < triN; j++) {
+      u32 ind = order[triStart + j];
+      float p = tris[ind].centroid[axis];
+      if (p >= s && p < e) { binN[i]++; binB[i] = merge(binB[i], tris[ind].centroid); }
+    }
+  }
+  // costs (bvhhelper.cpp:120-164): heu[k] = left bins 0..k, right bins k+1..15
+  float nodeSA = surface_area(node->bound);
+  float heu[kBins];
+  for (int pass = 0; pass < 2; pass++) {
+    u32 start = pass == 0 ? 0 : kBins, end = kBins / 2, step = pass == 0 ? 1u : (u32)-1;
+    u32 mainN = 0; Box mainB = empty_box();
+    for (u32 j = start; j != end; j += step) {
+      if (j < (u32)kBins) { mainN += binN[j]; mainB = merge(mainB, binB[j]); }
+      u32 opN = 0; Box opB = empty_box();
+      for (u32 k = j + step; k < (u32)kBins; k += step) { opN += binN[k]; opB = merge(opB, binB[k]); }
+      float c = 0.125f + ((float)mainN * surface_area(mainB) + (float)opN * surface_area(opB)) / nodeSA;
+      if (pass == 0) heu[j] = c; else heu[j - 1] = c;
+    }
+  }
+  float minCost = (float)kInf, minBin = (float)(kBins - 1);
+  for (int i = 0; i < kBins; i++) if (heu[i] < minCost) { minCost = heu[i]; minBin = (float)i; }
+  // partition (bvhhelper.cpp:190-211)
+  float split = cb.mn[axis] + (minBin + 1) * len / (float)kBins;
+  u32 left = triStart, right = triStart + triN - 1, mid;
+  while (left < right) {
+    while (tris[order[left]].centroid[axis] <= split && left < right) left++;
+    while (tris[order[right]].centroid[axis] > split && left < right) right--;
+    if (left < right) { std::swap(order[left], order[right]); left++; right--; }
+  }
+  mid = (tris[order[left]].centroid[axis] <= split) ? left : left - 1;
+  BNode* l = new BNode(); BNode* r = new BNode();
+  node->c[0] = l; node->c[1] = r;
+  build_node(l, tris, order, triStart, mid - triStart + 1, depthBudget - 1);
+  build_node(r, tris, order, mid + 1, triN - (mid - triStart + 1), depthBudget - 1);
+  node->desN = 2 + l->desN + r->desN;
+}
+
+// bvhhelper.cpp:285-298
+static u32 compact_node(const BNode* n, Scene& s, u32 cur) {
+  s.bmin[cur * 3 + 0] = n->bound.mn.x; s.bmin[cur * 3 + 1] = n->bound.mn.y; s.bmin[cur * 3 + 2] = n->bound.mn.z;
+  s.bmax[cur * 3 + 0] = n->bound.mx.x; s.bmax[cur * 3 + 1] = n->bound.mx.y; s.bmax[cur * 3 + 2] = n->bound.mx.z;
+  if (!n->c[0] || !n->c[1]) { s.offOrTSTN[cur * 2] = n->triStart; s.offOrTSTN[cur * 2 + 1] = n->triN; return 1; }
+  s.offOrTSTN[cur * 2 + 1] = 0;
+  u32 first = compact_node(n->c[0], s, cur + 1);
+  s.offOrTSTN[cur * 2] = first + 1;
+  return compact_node(n->c[1], s, cur + first + 1) + first + 1;
+}
+
+// bvhhelper.cpp:239-283 + raytracer.cpp:255-281 (reorder, intersection data)
+static bool build_scene_bvh(Scene& s, const float* pos, const u32* idx, u32 maxDepth, std::vector<u32>& order) {
+  u32 nt = s.nt;
+  std::vector<BTri> tris(nt);
+  order.resize(nt);
+  for (u32 i = 0; i < nt; i++) {
+    order[i] = i;
+    F3 a = f3(pos[idx[i * 3] * 3], pos[idx[i * 3] * 3 + 1], pos[idx[i * 3] * 3 + 2]);
+    F3 b = f3(pos[idx[i * 3 + 1] * 3], pos[idx[i * 3 + 1] * 3 + 1], pos[idx[i * 3 + 1] * 3 + 2]);
+    F3 c = f3(pos[idx[i * 3 + 2] * 3], pos[idx[i * 3 + 2] * 3 + 1], pos[idx[i * 3 + 2] * 3 + 2]);
+    tris[i].centroid = f3(((a.x + b.x) + c.x) / 3.f, ((a.y + b.y) + c.y) / 3.f, ((a.z + b.z) + c.z) / 3.f);
+    tris[i].bound.mn = f3min(c, f3min(b, a));
+    tris[i].bound.mx = f3max(c, f3max(b, a));
+  }
+  if (nt == 0) return false;
+  BNode* root = new BNode();
+  build_node(root, tris.data(), order.data(), 0, nt, maxDepth);
+  s.nodeN = root->desN + 1;
+  s.bmin.assign(s.nodeN * 3, 0.f); s.bmax.assign(s.nodeN * 3, 0.f); s.offOrTSTN.assign(s.nodeN * 2, 0u);
+  bool ok = compact_node(root, s, 0) == s.nodeN;
+  free_tree(root);
+  return ok;
+}
+
+// ---------------------------------------------------------------- intersection
+// cudaRTCommon.cu:42-64 (IntersectTri) with the device arithmetic.
+template <class R>
+static inline R intersect_tri(V3<R> o, V3<R> d, V3<R> p0, V3<R> e0, V3<R> e1, R eps, bool cull, R& u, R& v) {
+  if (cull && dot(cross(e0, e1), d) > (R)0) return (R)kInf;
+  V3<R> de2 = cross(d, e1);
+  R div = dot(de2, e0);
+  if (std::fabs(div) < det_eps<R>()) return (R)kInf;
+  div = (R)1 / div;
+  V3<R> t = (o + d * eps) - p0;
+  V3<R> te1 = cross(t, e0);
+  R rT = dot(te1, e1) * div;
+  if (rT < (R)0) return (R)kInf;
+  u = dot(de2, t) * div;
+  if (u < (R)0 || u > (R)1) return (R)kInf;
+  v = dot(te1, d) * div;
+  if (v < (R)0 || (u + v) > (R)1) return (R)kInf;
+  return rT;
+}
+// cudahelper.h:119-125
+template <class R> static inline R escape_zero(R v, R e) { if (std::fabs(v) < e) v = (v > (R)0) ? v + e : v - e; return v; }
+// cudaRTCommon.cu:24-40
+template <class R>
+static inline R intersect_aabb(V3<R> o, V3<R> d, V3<R> mn, V3<R> mx) {
+  R e = (R)(float)kEpsilon;
+  V3<R> rd = mk((R)1 / escape_zero(d.x, e), (R)1 / escape_zero(d.y, e), (R)1 / escape_zero(d.z, e));
+  V3<R> t0 = vmul(mn - o, rd), t1 = vmul(mx - o, rd);
+  V3<R> lo = vmin(t0, t1), hi = vmax(t0, t1);
+  R minmax = std::fmin(std::fmin(hi.x, hi.y), hi.z);
+  R maxmin = std::fmax(std::fmax(lo.x, lo.y), lo.z);
+  if (minmax >= maxmin && (minmax > (R)0 || maxmin > (R)0)) return ((double)maxmin > kEpsilon) ? maxmin : (R)0;
+  return (R)kInf;
+}
+
+struct Hit { u32 tri; float t, u, v; };   // tri = original id, 0xFFFFFFFF on miss
+
+// Conservative slab test for the defined traversal: exact-form slab distances
+// with the far distance widened by 1 + 2*gamma(3) (Ize, "Robust BVH Ray
+// Traversal", JCGT 2013), so a box that the ray touches is never culled.
+template <class R>
+static inline R conservative_aabb(V3<R> o, V3<R> d, V3<R> mn, V3<R> mx) {
+  R e = (R)(float)kEpsilon;
+  V3<R> rd = mk((R)1 / escape_zero(d.x, e), (R)1 / escape_zero(d.y, e), (R)1 / escape_zero(d.z, e));
+  V3<R> t0 = vmul(mn - o, rd), t1 = vmul(mx - o, rd);
+  V3<R> lo = vmin(t0, t1), hi = vmax(t0, t1);
+  R tnear = std::fmax(std::fmax(lo.x, lo.y), std::fmax(lo.z, (R)0));
+  R tfar = std::fmin(std::fmin(hi.x, hi.y), hi.z) * (R)1.0000004;
+  return tnear <= tfar ? tnear : std::numeric_limits<R>::infinity();
+}
+
+template <class R> struct HitR { u32 leaf; u32 orig; R t, u, v; };
+
+// Tie-defined acceptance: strictly closer, or equal t with a smaller original id.
+template <class R> static inline bool better(R t, u32 id, const HitR<R>& h) {
+  if (t < h.t) return true;
+  return t == h.t && h.leaf != 0xFFFFFFFFu && id < h.orig;
+}
+
+// Conservative box slack: covers the eps origin offset of IntersectTri and the
+// rounding of the slab test, so culling never drops the tie-defined minimum.
+template <class R> static inline R cull_slack(R best, R eps) { return eps + best * (R)1e-4 + (R)1e-5; }
+
+template <class R>
+static bool trace_brute(const Scene& s, V3<R> o, V3<R> d, R maxDist, R eps, bool cull, HitR<R>& h) {
+  h.leaf = 0xFFFFFFFFu; h.orig = 0xFFFFFFFFu; h.t = maxDist; h.u = h.v = 0;
+  for (u32 i = 0; i < s.nt; i++) {
+    R u = 0, v = 0;
+    R t = intersect_tri(o, d, cvt<R>(&s.p0[i * 3]), cvt<R>(&s.e0[i * 3]), cvt<R>(&s.e1[i * 3]), eps, cull, u, v);
+    if (t >= (R)0 && better(t, s.orig[i], h)) { h.leaf = i; h.orig = s.orig[i]; h.t = t; h.u = u; h.v = v; }
+  }
+  return h.leaf != 0xFFFFFFFFu;
+}
+
+template <class R>
+static bool trace_defined(const Scene& s, V3<R> o, V3<R> d, R maxDist, R eps, bool cull, HitR<R>& h, bool any_hit = false) {
+  h.leaf = 0xFFFFFFFFu; h.orig = 0xFFFFFFFFu; h.t = maxDist; h.u = h.v = 0;
+  if (s.nodeN == 0) return false;
+  u32 stack[128]; int sp = 0; stack[0] = 0;
+  while (sp >= 0) {
+    u32 n = stack[sp--];
+    R tb = conservative_aabb(o, d, cvt<R>(&s.bmin[n * 3]), cvt<R>(&s.bmax[n * 3]));
+    if (!(tb <= h.t + cull_slack(h.t, eps))) continue;
+    u32 a = s.offOrTSTN[n * 2], cnt = s.offOrTSTN[n * 2 + 1];
+    if (cnt == 0) {
+      if (s.bmin[n * 3] > s.bmax[n * 3]) continue;       // empty leaf of a degenerate split
+      stack[++sp] = n + 1; stack[++sp] = n + a;
+    } else {
+      for (u32 i = a; i < a + cnt; i++) {
+        R u = 0, v = 0;
+        R t = intersect_tri(o, d, cvt<R>(&s.p0[i * 3]), cvt<R>(&s.e0[i * 3]), cvt<R>(&s.e1[i * 3]), eps, cull, u, v);
+        if (t >= (R)0 && better(t, s.orig[i], h)) {
+          h.leaf = i; h.orig = s.orig[i]; h.t = t; h.u = u; h.v = v;
+          if (any_hit) return true;
+        }
+      }
+    }
+  }
+  return h.leaf != 0xFFFFFFFFu;
+}
+
+// cudaRTCommon.cu:66-127, literal (reference tie and cap behaviour).
+struct RefStats { u32 stack_drops, cap_hits; };
+template <class R>
+static bool trace_refbin(const Scene& s, V3<R> o, V3<R> d, R maxDist, R eps, bool cull, HitR<R>& h, RefStats& st) {
+  R best = maxDist; u32 id = 0; R hu = 0, hv = 0;
+  u32 stack[32]; i32 sp = 0; stack[0] = 0; u32 traced = 0;
+  while (sp >= 0 && traced++ < 2048u) {
+    u32 n = stack[sp--];
+    R tb = intersect_aabb(o, d, cvt<R>(&s.bmin[n * 3]), cvt<R>(&s.bmax[n * 3]));
+    if (tb >= (R)0 && tb < best) {
+      u32 a = s.offOrTSTN[n * 2], cnt = s.offOrTSTN[n * 2 + 1];
+      if (cnt == 0) {
+        if (sp < 32 - 3) { stack[++sp] = n + 1; stack[++sp] = n + a; }
+        else st.stack_drops++;
+      } else {
+        for (u32 i = a; i < a + cnt; i++) {
+          R u = 0, v = 0;
+          R t = intersect_tri(o, d, cvt<R>(&s.p0[i * 3]), cvt<R>(&s.e0[i * 3]), cvt<R>(&s.e1[i * 3]), eps, cull, u, v);
+          if (t >= (R)0 && t < best) { best = t; id = i; hu = u; hv = v; }
+        }
+      }
+    }
+  }
+  if (sp >= 0) st.cap_hits++;
+  if (best < maxDist) { h.leaf = id; h.orig = s.orig[id]; h.t = best; h.u = hu; h.v = hv; return true; }
+  h.leaf = 0xFFFFFFFFu; h.orig = 0xFFFFFFFFu; h.t = maxDist; h.u = h.v = 0;
+  return false;
+}
+
+// ---------------------------------------------------------------- shading
+// cudaRTCommon.cu:284-365
+template <class R> static inline bool prob_rand(Pcg32& rng, R prob) {
+  if ((double)prob < kBiasEps) return false;
+  if ((double)prob + kBiasEps > 1.0) return true;
+  return uniform<R>(rng) < prob;
+}
+template <class R> static inline R vis_smith(R rough, R nov, R nol) {
+  R k = (rough * rough) / (R)2;
+  return (nov / (nov * ((R)1 - k) + k)) * (nol / (nol * ((R)1 - k) + k));
+}
+template <class R> static inline R d_ggx(R rough, R noh) {
+  R m = rough * rough, m2 = m * m;
+  R d = (noh * m2 - noh) * noh + (R)1;
+  return (R)((double)m2 / (kPi * (double)d * (double)d));
+}
+template <class R> static inline V3<R> f_schlick(V3<R> sc, R voh) {
+  R fc = std::pow((R)1 - voh, (R)5);
+  R first = (R)saturate(50.0 * (double)sc.z) * fc;
+  return mk(first, first, first) + sc * ((R)1 - fc);
+}
+template <class R> static inline void basis(V3<R> w, V3<R>& u, V3<R>& v) {
+  V3<R> a = (double)std::fabs(w.x) > 0.1 ? mk((R)0, (R)1, (R)0) : mk((R)1, (R)0, (R)0);
+  u = normalize(cross(a, w));
+  v = cross(w, u);
+  u = cross(v, w);
+}
+template <class R> static inline V3<R> sample_ggx(R xi0, R xi1, R rough, V3<R> n) {
+  R a = rough * rough;
+  R phi = (R)(2.0 * kPi * (double)xi0);
+  R ct = std::sqrt(((R)1 - xi1) / ((R)1 + (a * a - (R)1) * xi1));
+  R st = std::sqrt((R)1 - ct * ct);
+  V3<R> h = mk(st * std::cos(phi), st * std::sin(phi), ct);
+  V3<R> u, v; basis(n, u, v);
+  return (u * h.x + v * h.y) + n * h.z;
+}
+template <class R> static inline V3<R> lambert(V3<R> diff) { return diff * (R)(1.0 / kPi); }
+// cudaRTBDPTStream.cu:642-663
+template <class R>
+static inline V3<R> shading(V3<R> outDir, V3<R> inDir, V3<R> irrad, V3<R> n, V3<R> diff, R metallic, R rough, R spec, R dw, R sw) {
+  if (dot(n, inDir) <= (R)0) return mk((R)0, (R)0, (R)0);
+  V3<R> h = normalize(outDir + inDir);
+  R voh = dot(outDir, h), nov = dot(n, outDir), noh = dot(n, h), nol = dot(n, inDir);
+  V3<R> f0 = vlerp(mk((R)1, (R)1, (R)1) * ((R)0.08 * spec), diff, metallic);
+  V3<R> F = f_schlick(f0, voh);
+  R dd = d_ggx(rough, noh), vv = vis_smith(rough, nov, nol);
+  V3<R> specI = F * (dd * vv);
+  V3<R> diffI = vmul(mk((R)1, (R)1, (R)1) - F, lambert(diff));
+  return vmul(irrad * nol, specI * sw + diffI * dw);
+}
+
+// ---------------------------------------------------------------- BDPT
+struct RenderParams {
+  u32 width, height;
+  float cam_pos[3], cam_dir[3], cam_up[3], fov;
+  u32 eye_depth, light_depth, light_paths;
+  u32 flags;      // bit 0: literal shadow endpoint dist - eps (ill-conditioned, see shadow_max)
+  u64 seed;
+  u32 row0, rows; // rows [row0, row0 + rows) of the frame are traced (rows = 0: all)
+};
+
+// Shadow-segment end for a connection.  The reference passes dist - eps
+// (cudaRTBDPTStream.cu:685-686) but IntersectTri also measures t from the
+// origin shifted by eps, so the target vertex's own surface lands on maxDist to
+// within rounding and visibility becomes a coin flip of the arithmetic.  The
+// defined endpoint stops one eps before the target in unshifted distance.
+static inline double shadow_max(double dist, u32 flags) { return (flags & 1u) ? dist - kBiasEps : dist - 2.0 * kBiasEps; }
+
+template <class R> struct LightVertex {
+  V3<R> pos, norm, irrad, irradDir, diff, emissive;
+  R specular, metallic, roughness;
+};
+
+// Shading inputs at a hit (cudaRTBDPTStream.cu:374-401).
+template <class R> struct HitShade {
+  V3<R> norm, pos, diff, emissive;
+  R trans, specular, metallic, roughness;
+};
+
+template <class R>
+static HitShade<R> fetch_hit(const Scene& s, u32 leaf, R w, R u, R v) {
+  HitShade<R> hs;
+  const u32* t = &s.tri[leaf * 3];
+  const Material& m = s.mats[s.tmat[leaf]];
+  V3<R> n0 = cvt<R>(&s.nrm[t[0] * 3]), n1 = cvt<R>(&s.nrm[t[1] * 3]), n2 = cvt<R>(&s.nrm[t[2] * 3]);
+  V3<R> q0 = cvt<R>(&s.pos[t[0] * 3]), q1 = cvt<R>(&s.pos[t[1] * 3]), q2 = cvt<R>(&s.pos[t[2] * 3]);
+  hs.norm = (n0 * w + n1 * u) + n2 * v;
+  hs.pos = (q0 * w + q1 * u) + q2 * v;
+  hs.diff = cvt<R>(m.diffuse); hs.emissive = cvt<R>(m.emissive);
+  hs.trans = m.transparency; hs.specular = m.specular; hs.metallic = m.metallic; hs.roughness = m.roughness;
+  return hs;
+}
+
+enum { RT_EYE = 0, RT_DIFF = 1, RT_SPEC = 2, RT_LIGHT = 3 };
+
+// One BRDF sampling step shared by eye and light bounces
+// (cudaRTBDPTStream.cu:228-329 and :402-502).
+template <class R> struct Bounce { V3<R> orig, dir, mul; int type; };
+template <class R>
+static Bounce<R> sample_bounce(Pcg32& rng, V3<R> ro, V3<R> rd, R t, const HitShade<R>& hs, V3<R> nl, int curType) {
+  R x0 = uniform<R>(rng), x1 = uniform<R>(rng);
+  V3<R> hDir = sample_ggx(x0, x1, hs.roughness, nl);
+  V3<R> ndir = rd * (R)-1;
+  R voH = dot(ndir, hDir);
+  V3<R> f0 = vlerp(mk((R)0.08 * hs.specular, (R)0.08 * hs.specular, (R)0.08 * hs.specular), hs.diff, hs.metallic);
+  V3<R> F = f_schlick(f0, voH);
+  R reflProb = lerp(length(F), (R)1, hs.metallic);
+  R refrProb = hs.trans;
+  V3<R> reflDir = mk((R)0, (R)0, (R)0), refrDir = mk((R)0, (R)0, (R)0);
+  Bounce<R> b; b.orig = ro; b.dir = rd; b.type = curType; b.mul = mk((R)0, (R)0, (R)0);
+  if (refrProb > (R)0) {
+    bool into = dot(nl, hs.norm) > (R)0;
+    R nt = hs.specular * (R)0.8 + (R)1, nc = (R)1;
+    R nnt = into ? nc / nt : nt / nc;
+    R ddn = dot(hDir, rd);
+    R cos2t = (R)1 - nnt * nnt * ((R)1 - ddn * ddn);
+    refrDir = normalize(rd * nnt - hDir * (ddn * nnt + std::sqrt(cos2t)));
+  }
+  if (reflProb > (R)0) {
+    reflDir = normalize(rd - (hDir * (R)2) * dot(hDir, rd));
+    if (dot(reflDir, nl) < (R)0) reflProb = (R)0;
+  }
+  if (prob_rand(rng, reflProb)) {
+    b.orig = ro + rd * (R)((double)t - kBiasEps); b.dir = reflDir;
+    R VoH = dot(ndir, hDir), NoV = dot(nl, ndir), NoH = dot(nl, hDir), NoL = dot(nl, reflDir);
+    R G = vis_smith(hs.roughness, NoV, NoL);
+    b.mul = (F * G * VoH) / (NoV * NoH * reflProb);
+    b.type = RT_SPEC;
+  } else if (prob_rand(rng, refrProb)) {
+    b.orig = (ro + rd * (R)((double)t + kBiasEps)) + refrDir * (R)kBiasEps; b.dir = refrDir;
+    R cosine = dot(nl * (R)-1, refrDir);
+    b.mul = hs.diff * cosine / (refrProb * ((R)1 - reflProb));
+    b.type = RT_SPEC;
+  } else {
+    V3<R> u, v; basis(nl, u, v);
+    R r1 = (R)(2.0 * kPi * (double)uniform<R>(rng));
+    R r2cos = std::sqrt(uniform<R>(rng));
+    R r2sin = (R)1 - r2cos * r2cos;
+    V3<R> dd = normalize((nl * r2cos + (u * r2sin) * std::cos(r1)) + (v * r2sin) * std::sin(r1));
+    b.orig = (ro + rd * t) + dd * (R)kBiasEps; b.dir = dd;
+    b.mul = lambert(hs.diff) * (R)kPi / (((R)1 - refrProb) * ((R)1 - reflProb));
+    b.type = RT_DIFF;
+  }
+  return b;
+}
+
+template <class R> struct Camera { V3<R> pos, dir, up, right; R fov; };
+
+template <class R>
+static Camera<R> make_camera(const RenderParams& p) {
+  // cudaRTBDPTStream.cu:823-824 (host float Vec3 math)
+  F3 d = f3(p.cam_dir[0], p.cam_dir[1], p.cam_dir[2]), up = f3(p.cam_up[0], p.cam_up[1], p.cam_up[2]);
+  F3 r = f3(d.y * up.z - d.z * up.y, d.z * up.x - d.x * up.z, d.x * up.y - d.y * up.x);
+  float rl = std::sqrt(r.x * r.x + r.y * r.y + r.z * r.z); r = f3(r.x / rl, r.y / rl, r.z / rl);
+  F3 u2 = f3(r.y * d.z - r.z * d.y, r.z * d.x - r.x * d.z, r.x * d.y - r.y * d.x);
+  float ul = std::sqrt(u2.x * u2.x + u2.y * u2.y + u2.z * u2.z); u2 = f3(u2.x / ul, u2.y / ul, u2.z / ul);
+  Camera<R> c;
+  c.pos = mk((R)p.cam_pos[0], (R)p.cam_pos[1], (R)p.cam_pos[2]);
+  c.dir = mk((R)d.x, (R)d.y, (R)d.z); c.up = mk((R)u2.x, (R)u2.y, (R)u2.z); c.right = mk((R)r.x, (R)r.y, (R)r.z);
+  c.fov = (R)p.fov;
+  return c;
+}
+
+// cudaRTBDPTStream.cu:629-637
+template <class R>
+static V3<R> eye_dir(const Camera<R>& c, u32 x, u32 y, R W, R H, Pcg32& rng) {
+  R tf = std::tan(c.fov * (R)0.5);
+  R u = ((((R)2 * ((R)x + (R)0.5)) / W - (R)1) * tf * W) / H;
+  R v = (((R)2 * ((R)y + (R)0.5)) / H - (R)1) * tf;
+  R au = u + ((uniform<R>(rng) - (R)0.5) / H) * tf;
+  R av = v + ((uniform<R>(rng) - (R)0.5) / H) * tf;
+  return normalize((c.right * au + c.up * av) + c.dir);
+}
+
+template <class R> struct FrameState {
+  std::vector<R> result;      // W*H*3 running mean
+  std::vector<u32> N;         // W*H sample counts
+  u64 ext_rays = 0, shadow_rays = 0;
+  std::vector<LightVertex<R>> lv;
+  u32 lvN = 0;
+};
+
+// Light-vertex cache pass (cudaRTBDPTStream.cu:180-363,545-617,875-906).
+template <class R>
+static void light_pass(const Scene& s, const RenderParams& p, u32 frame, FrameState<R>& fs) {
+  u32 L = p.light_paths;
+  fs.lv.assign((size_t)L * (p.light_depth + 1), LightVertex<R>());
+  fs.lvN = 0;
+  if (s.lightTri.empty()) return;
+  struct LP { Pcg32 rng; V3<R> orig, dir, sample, mul; int type; u32 depth, origin; bool term; };
+  std::vector<LP> lp(L);
+  u32 nlt = (u32)s.lightTri.size();
+  for (u32 x = 0; x < L; x++) {
+    LP& P = lp[x];
+    pcg32_seed(P.rng, stream_key(p.seed, frame, DOMAIN_LIGHT, 0), x);
+    u32 id = (u32)(uniform<R>(P.rng) * (R)nlt); if (id >= nlt) id = nlt - 1;
+    R lw = uniform<R>(P.rng), lu = uniform<R>(P.rng);
+    if (lw + lu > (R)1) { lw = (R)1 - lw; lu = (R)1 - lu; }
+    R lvv = ((R)1 - lw) - lu;
+    HitShade<R> hs = fetch_hit<R>(s, s.lightTri[id], lw, lu, lvv);
+    V3<R> u, v; basis(hs.norm, u, v);
+    R r1 = (R)(2.0 * kPi * (double)uniform<R>(P.rng));
+    R r2cos = std::sqrt(uniform<R>(P.rng));
+    R r2sin = (R)1 - r2cos * r2cos;
+    V3<R> dd = normalize((hs.norm * r2cos + (u * r2sin) * std::cos(r1)) + (v * r2sin) * std::sin(r1));
+    P.orig = hs.pos + hs.norm * (R)kBiasEps; P.dir = dd; P.sample = hs.emissive; P.mul = mk((R)1, (R)1, (R)1);
+    P.type = RT_LIGHT; P.depth = 0; P.origin = x; P.term = false;
+    LightVertex<R>& V = fs.lv[x];
+    V.irrad = hs.emissive; V.irradDir = mk((R)0, (R)0, (R)0); V.norm = hs.norm; V.pos = hs.pos;
+    V.diff = hs.diff; V.emissive = hs.emissive; V.specular = hs.specular; V.metallic = hs.metallic; V.roughness = hs.roughness;
+  }
+  u32 cur = L;
+  for (;;) {
+    // stream position of each live path = rank among live paths (stable compaction)
+    u32 rank = 0;
+    for (u32 x = 0; x < L; x++) {
+      LP& P = lp[x];
+      if (P.term) continue;
+      u32 slot = cur + rank++;
+      HitR<R> h;
+      bool hit = trace_defined<R>(s, P.orig, P.dir, (R)kInf, (R)kBiasEps, false, h);
+      if (!hit) { fs.lv[slot] = fs.lv[P.origin]; P.term = true; continue; }
+      R w = ((R)1 - h.u) - h.v;
+      HitShade<R> hs = fetch_hit<R>(s, h.leaf, w, h.u, h.v);
+      V3<R> nl = dot(hs.norm, P.dir) < (R)0 ? hs.norm : hs.norm * (R)-1;
+      LightVertex<R>& V = fs.lv[slot];
+      V.irrad = P.sample; V.irradDir = P.dir * (R)-1; V.norm = nl; V.pos = hs.pos;
+      V.diff = hs.diff; V.emissive = hs.emissive; V.specular = hs.specular; V.metallic = hs.metallic; V.roughness = hs.roughness;
+      Bounce<R> b = sample_bounce<R>(P.rng, P.orig, P.dir, h.t, hs, nl, P.type);
+      if (b.type != RT_DIFF) V.irrad = mk((R)0, (R)0, (R)0);
+      if (dot(b.dir, nl) < (R)0) V.norm = V.norm * (R)-1;
+      P.sample = hs.emissive + vmul(P.sample, b.mul);
+      R contrib = length(P.mul) * length(b.mul);
+      if (length(hs.emissive) > (R)0) contrib = (R)0;
+      if (uniform<R>(P.rng) > contrib || P.depth + 1 >= p.light_depth) P.term = true;
+      else { P.mul = vmul(P.mul, b.mul); P.dir = b.dir; P.orig = b.orig; P.type = b.type; P.depth++; }
+    }
+    if (rank == 0) break;
+    cur += rank;
+  }
+  fs.lvN = cur;
+}
+
+template <class R> struct PathResult { V3<R> accum; u32 N; V3<R> gpos, gnorm, gdiff; u32 ext_rays, shadow_rays; };
+
+// One eye path (cudaRTBDPTStream.cu:365-543,665-723).
+template <class R>
+static PathResult<R> eye_path(const Scene& s, const RenderParams& p, const Camera<R>& cam, const FrameState<R>& fs,
+                              u32 frame, u32 x, u32 y, u32 sample, int variant) {
+  u32 W = p.width, H = p.height, pix = y * W + x;
+  Pcg32 rng; pcg32_seed(rng, stream_key(p.seed, frame, DOMAIN_EYE, sample), pix);
+  V3<R> dir = eye_dir(cam, x, y, (R)W, (R)H, rng);
+  V3<R> orig = cam.pos, mul = mk((R)1, (R)1, (R)1), psample = mk((R)0, (R)0, (R)0);
+  PathResult<R> pr; pr.accum = mk((R)0, (R)0, (R)0); pr.N = 0;
+  pr.gpos = pr.gnorm = pr.gdiff = mk((R)0, (R)0, (R)0);
+  pr.ext_rays = pr.shadow_rays = 0;
+  (void)variant;
+  for (u32 depth = 0;; ) {
+    HitR<R> h;
+    pr.ext_rays++;
+    if (!trace_defined<R>(s, orig, dir, (R)kInf, (R)kBiasEps, false, h)) {
+      pr.accum = pr.accum + psample; pr.N++; break;
+    }
+    R w = ((R)1 - h.u) - h.v;
+    HitShade<R> hs = fetch_hit<R>(s, h.leaf, w, h.u, h.v);
+    V3<R> nl = dot(hs.norm, dir) < (R)0 ? hs.norm : hs.norm * (R)-1;
+    if (depth == 0) { pr.gpos = hs.pos; pr.gnorm = nl; pr.gdiff = hs.diff; }
+    Bounce<R> b = sample_bounce<R>(rng, orig, dir, h.t, hs, nl, RT_EYE);
+    psample = psample + vmul(hs.emissive, mul);
+    V3<R> inMul = mul, inDir = dir * (R)-1;
+    R contrib = length(mul) * length(b.mul);
+    if (length(hs.emissive) > (R)0) contrib = (R)0;
+    if (uniform<R>(rng) > contrib || depth + 1 >= p.eye_depth) {
+      pr.accum = pr.accum + psample; pr.N++; break;
+    }
+    mul = vmul(mul, b.mul); dir = b.dir; depth++;
+    orig = b.orig;
+    if (b.type == RT_DIFF && fs.lvN > 0) {
+      // pt_connectEyeLightPath_kernel + GetLightFromRandLightVertices
+      V3<R> pos = orig + nl * (R)kBiasEps;
+      u32 li = (u32)(uniform<R>(rng) * (R)fs.lvN); if (li >= fs.lvN) li = fs.lvN - 1;
+      const LightVertex<R>& L = fs.lv[li];
+      V3<R> irrad = mk((R)0, (R)0, (R)0), irradDir = mk((R)0, (R)0, (R)0);
+      V3<R> toL = normalize(L.pos - pos);
+      R dist = length(L.pos - pos);
+      if (length(L.irrad) > (R)0 && dot(nl, toL) > (R)0) {
+        HitR<R> sh;
+        pr.shadow_rays++;
+        if (!trace_defined<R>(s, pos, toL, (R)shadow_max((double)dist, p.flags), (R)kBiasEps, false, sh, true)) {
+          if ((double)dist > kFltEps) {
+            irrad = shading(toL * (R)-1, L.irradDir, L.irrad, L.norm, L.diff, L.metallic, L.roughness, L.specular, (R)1, (R)1) + L.emissive;
+            irradDir = toL;
+          } else { irrad = L.irrad; irradDir = L.irradDir * (R)-1; }
+        }
+      }
+      V3<R> c = vmax(mk((R)0, (R)0, (R)0),
+                     shading(inDir, irradDir, irrad, nl, hs.diff, hs.metallic, hs.roughness, hs.specular, (R)1 - hs.trans, (R)1));
+      if (length(c) > (R)0) { pr.accum = pr.accum + vmul(c, inMul); pr.N += 4; }
+    }
+  }
+  return pr;
+}
+
+// pt_applyPathQueueResult_kernel (cudaRTBDPTStream.cu:741-778)
+template <class R>
+static inline void apply_sample(R* res, u32& Nold, V3<R> accum, u32 n) {
+  if (n == 0) return;
+  u32 nn = Nold + n;
+  if (!(nn > Nold) || isinf3(accum)) return;
+  R ri = (R)1 / (R)nn, oi = (R)Nold * ri;
+  res[0] = std::fmax(ri * accum.x + oi * res[0], (R)0);
+  res[1] = std::fmax(ri * accum.y + oi * res[1], (R)0);
+  res[2] = std::fmax(ri * accum.z + oi * res[2], (R)0);
+  Nold = nn;
+}
+
+template <class R>
+static void render_frames(const Scene& s, const RenderParams& p, u32 frame0, u32 nframes, FrameState<R>& fs,
+                          R* gbuf /* optional W*H*9: pos, norm, diff of the last frame */) {
+  Camera<R> cam = make_camera<R>(p);
+  u32 W = p.width, H = p.height;
+  for (u32 f = frame0; f < frame0 + nframes; f++) {
+    if (f % 3 == 0) light_pass<R>(s, p, f, fs);
+    if (f == 0) std::fill(fs.N.begin(), fs.N.end(), 0u);
+    u64 er = 0, sr = 0;
+    const u32 y_end = std::min<u32>(H, p.rows ? p.row0 + p.rows : H);
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : er, sr)
+    for (i32 y = (i32)std::min(p.row0, H); y < (i32)y_end; y++) {
+      for (u32 x = 0; x < W; x++) {
+        PathResult<R> pr = eye_path<R>(s, p, cam, fs, f, x, (u32)y, 0, 0);
+        er += pr.ext_rays; sr += pr.shadow_rays;
+        u32 pix = (u32)y * W + x;
+        apply_sample(&fs.result[pix * 3], fs.N[pix], pr.accum, pr.N);
+        if (gbuf) {
+          R* g = gbuf + (size_t)pix * 9;
+          g[0] = pr.gpos.x; g[1] = pr.gpos.y; g[2] = pr.gpos.z;
+          g[3] = pr.gnorm.x; g[4] = pr.gnorm.y; g[5] = pr.gnorm.z;
+          g[6] = pr.gdiff.x; g[7] = pr.gdiff.y; g[8] = pr.gdiff.z;
+        }
+      }
+    }
+    fs.ext_rays += er; fs.shadow_rays += sr;
+  }
+}
+
+// ---------------------------------------------------------------- filter
+// cudaBilateralFilter.cu:5-24
+static void gaussian_table(float sigma, u32 radius, std::vector<float>& g) {
+  g.assign(2 * radius + 1, 0.f);
+  for (u32 i = 0; i < 2 * radius + 1; i++) {
+    int x = (int)i - (int)radius;
+    g[i] = std::exp(-(float)(x * x) / (2 * sigma * sigma));
+  }
+}
+// The reference clamps with CUDA's mixed int/unsigned min/max: (x + j) is
+// unsigned, so a negative coordinate wraps and clamps to the HIGH edge.
+static inline u32 ref_clamp(u32 x, int j, u32 n) {
+  u32 v = x + (u32)j;          // unsigned wrap as in the reference
+  return v < n - 1 ? v : n - 1;
+}
+template <class R>
+static inline R euclid(const float* a, const float* b, R d) {
+  R m = ((R)(b[0] - a[0]) * (R)(b[0] - a[0]) + (R)(b[1] - a[1]) * (R)(b[1] - a[1])) + (R)(b[2] - a[2]) * (R)(b[2] - a[2]);
+  return std::exp(-m / ((R)2 * d * d));
+}
+// bilaterial_posnorm_kernel / _posnormemit_kernel (cudaBilateralFilter.cu:66-142)
+template <class R>
+static void bilateral(const float* color, const float* pos, const float* norm, const float* emit, u32 W, u32 H,
+                      float sc, float sp, float sn, float se, u32 radius, const std::vector<float>& g, float* out) {
+#pragma omp parallel for schedule(static)
+  for (i32 yy = 0; yy < (i32)H; yy++) {
+    u32 y = (u32)yy;
+    for (u32 x = 0; x < W; x++) {
+      u32 ind = x + y * W;
+      R sum = 0, t0 = 0, t1 = 0, t2 = 0;
+      for (int i = -(int)radius; i <= (int)radius; i++) {
+        for (int j = -(int)radius; j <= (int)radius; j++) {
+          u32 c = ref_clamp(x, j, W) + ref_clamp(y, i, H) * W;
+          R f = (R)g[i + radius] * (R)g[j + radius];
+          f = f * euclid<R>(&color[c * 3], &color[ind * 3], (R)sc);
+          f = f * euclid<R>(&pos[c * 3], &pos[ind * 3], (R)sp);
+          f = f * euclid<R>(&norm[c * 3], &norm[ind * 3], (R)sn);
+          if (emit) f = f * euclid<R>(&emit[c * 3], &emit[ind * 3], (R)se);
+          t0 += f * (R)color[c * 3]; t1 += f * (R)color[c * 3 + 1]; t2 += f * (R)color[c * 3 + 2];
+          sum += f;
+        }
+      }
+      out[ind * 3] = (float)(t0 / sum); out[ind * 3 + 1] = (float)(t1 / sum); out[ind * 3 + 2] = (float)(t2 / sum);
+    }
+  }
+}
+
+}  // namespace orc
+
+// ====================================================================== C ABI
+using namespace orc;
+
+extern "C" {
+
+u32 orc_pcg32_stream(u64 initstate, u64 initseq, u32 n, u32* out) {
+  Pcg32 r; pcg32_seed(r, initstate, initseq);
+  for (u32 i = 0; i < n; i++) out[i] = pcg32_next(r);
+  return n;
+}
+u64 orc_splitmix64_next(u64* state) {
+  *state += 0x9E3779B97F4A7C15ULL;
+  return splitmix64_mix(*state);
+}
+u64 orc_stream_key(u64 seed, u32 frame, u32 domain, u32 sample) { return stream_key(seed, frame, domain, sample); }
+float orc_det_eps() { return det_eps<float>(); }
+
+void* orc_scene_create(const float* pos3, const float* nrm3, u32 nv, const u32* idx3, const u32* mat, u32 nt,
+                       const void* materials, u32 nmat, u32 max_depth) {
+  Scene* s = new Scene();
+  s->nv = nv; s->nt = nt;
+  s->pos.assign(pos3, pos3 + (size_t)nv * 3);
+  if (nrm3) s->nrm.assign(nrm3, nrm3 + (size_t)nv * 3); else s->nrm.assign((size_t)nv * 3, 0.f);
+  s->mats.resize(nmat);
+  if (nmat) std::memcpy(s->mats.data(), materials, sizeof(Material) * nmat);
+  std::vector<u32> order;
+  if (!build_scene_bvh(*s, pos3, idx3, max_depth, order)) { delete s; return nullptr; }
+  s->tri.resize((size_t)nt * 3); s->tmat.resize(nt); s->orig.resize(nt);
+  s->p0.resize((size_t)nt * 3); s->e0.resize((size_t)nt * 3); s->e1.resize((size_t)nt * 3);
+  for (u32 i = 0; i < nt; i++) {
+    u32 o = order[i];
+    s->orig[i] = o;
+    for (int k = 0; k < 3; k++) s->tri[i * 3 + k] = idx3[o * 3 + k];
+    s->tmat[i] = mat ? mat[o] : 0;
+    const float* a = &pos3[idx3[o * 3] * 3];
+    const float* b = &pos3[idx3[o * 3 + 1] * 3];
+    const float* c = &pos3[idx3[o * 3 + 2] * 3];
+    for (int k = 0; k < 3; k++) { s->p0[i * 3 + k] = a[k]; s->e0[i * 3 + k] = b[k] - a[k]; s->e1[i * 3 + k] = c[k] - a[k]; }
+  }
+  // updateLightTriCudaMem (cudaRTBDPTStream.cu:57-77): host float length of emissive
+  for (u32 i = 0; i < nt; i++) {
+    const float* e = s->mats.empty() ? nullptr : s->mats[s->tmat[i]].emissive;
+    if (e && std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) > 0.f) s->lightTri.push_back(i);
+  }
+  return s;
+}
+void orc_scene_destroy(void* h) { delete (Scene*)h; }
+
+// Exports the compact BVH: bounds6 (min xyz, max xyz), offOrTSTN2, reorder, nodeN.
+u32 orc_scene_bvh(void* h, float* bounds6, u32* off2, u32* reorder, u32 cap) {
+  Scene* s = (Scene*)h;
+  if (bounds6 && cap >= s->nodeN) {
+    for (u32 i = 0; i < s->nodeN; i++) {
+      for (int k = 0; k < 3; k++) { bounds6[i * 6 + k] = s->bmin[i * 3 + k]; bounds6[i * 6 + 3 + k] = s->bmax[i * 3 + k]; }
+      off2[i * 2] = s->offOrTSTN[i * 2]; off2[i * 2 + 1] = s->offOrTSTN[i * 2 + 1];
+    }
+  }
+  if (reorder) for (u32 i = 0; i < s->nt; i++) reorder[i] = s->orig[i];
+  return s->nodeN;
+}
+u32 orc_scene_light_tris(void* h, u32* out, u32 cap) {
+  Scene* s = (Scene*)h;
+  for (u32 i = 0; i < s->lightTri.size() && i < cap; i++) out[i] = s->lightTri[i];
+  return (u32)s->lightTri.size();
+}
+
+// rays: n x 8 floats {ox,oy,oz,maxDist, dx,dy,dz,eps}; hits: n x {u32 tri, f32 t,u,v}.
+// mode 0 = brute, 1 = defined (conservative BVH), 2 = reference TracePrimitive.
+// prec 32 or 64. stats[2] = stack drops, cap hits (mode 2).
+void orc_trace(void* h, const float* rays, u32 n, u32 cull, u32 mode, u32 prec, void* hits, u32* stats) {
+  Scene* s = (Scene*)h;
+  Hit* out = (Hit*)hits;
+  u32 drops = 0, caps = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : drops, caps)
+  for (i32 i = 0; i < (i32)n; i++) {
+    const float* r = rays + (size_t)i * 8;
+    RefStats st = {0, 0};
+    Hit o;
+    if (prec == 64) {
+      HitR<double> hh; V3<double> org = cvt<double>(r), dir = cvt<double>(r + 4);
+      bool ok = mode == 0 ? trace_brute<double>(*s, org, dir, r[3], r[7], cull != 0, hh)
+              : mode == 1 ? trace_defined<double>(*s, org, dir, r[3], r[7], cull != 0, hh)
+                          : trace_refbin<double>(*s, org, dir, r[3], r[7], cull != 0, hh, st);
+      o.tri = ok ? hh.orig : 0xFFFFFFFFu; o.t = ok ? (float)hh.t : (float)kInf; o.u = ok ? (float)hh.u : 0; o.v = ok ? (float)hh.v : 0;
+    } else {
+      HitR<float> hh; V3<float> org = cvt<float>(r), dir = cvt<float>(r + 4);
+      bool ok = mode == 0 ? trace_brute<float>(*s, org, dir, r[3], r[7], cull != 0, hh)
+              : mode == 1 ? trace_defined<float>(*s, org, dir, r[3], r[7], cull != 0, hh)
+                          : trace_refbin<float>(*s, org, dir, r[3], r[7], cull != 0, hh, st);
+      o.tri = ok ? hh.orig : 0xFFFFFFFFu; o.t = ok ? hh.t : (float)kInf; o.u = ok ? hh.u : 0; o.v = ok ? hh.v : 0;
+    }
+    out[i] = o;
+    drops += st.stack_drops; caps += st.cap_hits;
+  }
+  if (stats) { stats[0] = drops; stats[1] = caps; }
+}
+
+// Stateful renderer: the running mean persists across calls like the
+// reference's namespace globals.  prec 32 or 64.
+struct OrcRenderer { RenderParams p; u32 prec; FrameState<float> f32; FrameState<double> f64; };
+
+void* orc_renderer_create(const void* params, u32 prec) {
+  OrcRenderer* r = new OrcRenderer();
+  std::memcpy(&r->p, params, sizeof(RenderParams));
+  r->prec = prec;
+  size_t n = (size_t)r->p.width * r->p.height;
+  r->f32.result.assign(n * 3, 0.f); r->f32.N.assign(n, 0u);
+  r->f64.result.assign(n * 3, 0.0); r->f64.N.assign(n, 0u);
+  return r;
+}
+void orc_renderer_destroy(void* h) { delete (OrcRenderer*)h; }
+u32 orc_render_params_size() { return (u32)sizeof(RenderParams); }
+
+// Renders frames [frame0, frame0+n). result (W*H*3, float) and N (W*H) out;
+// gbuf (W*H*9 float, optional) receives the first-hit pos/norm/diff of the last frame.
+void orc_render(void* scene, void* rh, u32 frame0, u32 nframes, float* result, u32* N, float* gbuf, u32* lvN) {
+  Scene* s = (Scene*)scene;
+  OrcRenderer* r = (OrcRenderer*)rh;
+  size_t n = (size_t)r->p.width * r->p.height;
+  if (r->prec == 64) {
+    std::vector<double> g; if (gbuf) g.assign(n * 9, 0.0);
+    render_frames<double>(*s, r->p, frame0, nframes, r->f64, gbuf ? g.data() : nullptr);
+    if (result) for (size_t i = 0; i < n * 3; i++) result[i] = (float)r->f64.result[i];
+    if (N) std::memcpy(N, r->f64.N.data(), n * 4);
+    if (gbuf) for (size_t i = 0; i < n * 9; i++) gbuf[i] = (float)g[i];
+    if (lvN) *lvN = r->f64.lvN;
+  } else {
+    std::vector<float> g; if (gbuf) g.assign(n * 9, 0.f);
+    render_frames<float>(*s, r->p, frame0, nframes, r->f32, gbuf ? g.data() : nullptr);
+    if (result) std::memcpy(result, r->f32.result.data(), n * 12);
+    if (N) std::memcpy(N, r->f32.N.data(), n * 4);
+    if (gbuf) std::memcpy(gbuf, g.data(), n * 36);
+    if (lvN) *lvN = r->f32.lvN;
+  }
+}
+
+// Double-precision result of the running mean (for the FP64 gate).
+void orc_render_counts(void* rh, u64* out) {
+  OrcRenderer* r = (OrcRenderer*)rh;
+  out[0] = r->prec == 64 ? r->f64.ext_rays : r->f32.ext_rays;
+  out[1] = r->prec == 64 ? r->f64.shadow_rays : r->f32.shadow_rays;
+}
+int orc_max_threads() {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_render_result64(void* rh, double* out) {
+  OrcRenderer* r = (OrcRenderer*)rh;
+  std::memcpy(out, r->f64.result.data(), r->f64.result.size() * 8);
+}
+
+// Light-vertex cache of the last light pass: n x 21 floats
+// {pos3, norm3, irrad3, irradDir3, diff3, emissive3, spec, metal, rough}.
+u32 orc_light_vertices(void* rh, float* out, u32 cap) {
+  OrcRenderer* r = (OrcRenderer*)rh;
+  auto dump = [&](auto& fs) {
+    for (u32 i = 0; i < fs.lvN && i < cap; i++) {
+      auto& v = fs.lv[i];
+      float* o = out + (size_t)i * 21;
+      auto put = [&](int k, auto a) { o[k] = (float)a.x; o[k + 1] = (float)a.y; o[k + 2] = (float)a.z; };
+      put(0, v.pos); put(3, v.norm); put(6, v.irrad); put(9, v.irradDir); put(12, v.diff); put(15, v.emissive);
+      o[18] = (float)v.specular; o[19] = (float)v.metallic; o[20] = (float)v.roughness;
+    }
+    return fs.lvN;
+  };
+  return r->prec == 64 ? dump(r->f64) : dump(r->f32);
+}
+
+void orc_bilateral(const float* color, const float* pos, const float* norm, const float* emit, u32 W, u32 H,
+                   float sc, float sp, float sn, float se, u32 radius, u32 prec, float* out) {
+  std::vector<float> g; gaussian_table(sc, radius, g);
+  if (prec == 64) bilateral<double>(color, pos, norm, emit, W, H, sc, sp, sn, se, radius, g, out);
+  else bilateral<float>(color, pos, norm, emit, W, H, sc, sp, sn, se, radius, g, out);
+}
+void orc_gaussian_table(float sigma, u32 radius, float* out) {
+  std::vector<float> g; gaussian_table(sigma, radius, g);
+  std::memcpy(out, g.data(), g.size() * 4);
+}
+
+// pt_calculateSquareError_kernel (AdapBilatAtomic.cu:1143-1156)
+void orc_square_error(const float* filtered, const float* raw, const u32* N, u32 nd, u32 n, float* out) {
+  for (u32 x = 0; x < n; x++) {
+    u32 sn = N[x];
+    float red = (float)nd / (float)(nd + sn);
+    if (nd + sn < sn) red = 0;
+    float d0 = filtered[x * 3] - raw[x * 3], d1 = filtered[x * 3 + 1] - raw[x * 3 + 1], d2 = filtered[x * 3 + 2] - raw[x * 3 + 2];
+    out[x] = red * ((d0 * d0 + d1 * d1) + d2 * d2) / 3.f;
+  }
+}
+// pt_calculateEmpiricalSquareError_kernel (AdapEmp.cu:949-960)
+void orc_empirical_error(const float* mean, const float* sq, const u32* N, u32 n, float* out) {
+  for (u32 x = 0; x < n; x++) {
+    u32 sn = N[x];
+    float e = ((sq[x * 3] - mean[x * 3] * mean[x * 3]) + (sq[x * 3 + 1] - mean[x * 3 + 1] * mean[x * 3 + 1])) +
+              (sq[x * 3 + 2] - mean[x * 3 + 2] * mean[x * 3 + 2]);
+    e = e * (float)sn / ((float)(sn - 1) * 3.f);
+    out[x] = std::fmin(e, 1.0f);
+  }
+}
+// pt_genAdapPathQueue_kernel counts (AdapBilatAtomic.cu:830-831): floor(gen * e/sum)
+void orc_alloc_proportional(const float* err, u32 n, float sum, u32 gen, u32* counts) {
+  for (u32 x = 0; x < n; x++) { float r = err[x] / sum; counts[x] = (u32)((float)gen * r); }
+}
+
+}  // extern "C"

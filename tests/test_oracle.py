@@ -1,0 +1,335 @@
+"""CPU checks that pin the oracle: known-answer vectors, hand-worked BVH
+cases, closed-form intersections, estimator special cases and filter
+identities.  No GPU needed."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from cudapt_amd import scenes
+from tests.helpers import camera_rays, hits_equal, random_rays, rays_array, rel_mse, vertex_rays
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+# ---------------------------------------------------------------- RNG
+def test_pcg32_known_answer():
+    kat = json.load(open(os.path.join(GOLDEN, "kat.json")))["pcg32_demo"]
+    out = oracle.pcg32(kat["initstate"], kat["initseq"], len(kat["outputs"]))
+    assert [hex(x) for x in out] == [hex(int(x, 16)) for x in kat["outputs"]]
+
+
+def test_splitmix64_known_answer():
+    import ctypes
+    kat = json.load(open(os.path.join(GOLDEN, "kat.json")))["splitmix64"]
+    L = oracle.lib()
+    L.orc_splitmix64_next.restype = ctypes.c_uint64
+    st = ctypes.c_uint64(0)
+    got = [L.orc_splitmix64_next(ctypes.byref(st)) for _ in kat["outputs"]]
+    assert got == [int(x, 16) for x in kat["outputs"]]
+
+
+def test_uniforms_exact_and_in_range():
+    # (x >> 8) + 1 scaled by 2^-24: 24-bit values exact in FP32, support (0, 1].
+    x = oracle.pcg32(7, 11, 10000).astype(np.uint64)
+    u = ((x >> 8) + 1).astype(np.float64) / 16777216.0
+    assert (u > 0).all() and (u <= 1).all()
+    assert np.array_equal(u.astype(np.float32).astype(np.float64), u)
+
+
+# ---------------------------------------------------------------- BVH (bvhhelper.cpp:55-311)
+def _tri_scene(tris):
+    tris = np.asarray(tris, np.float32)
+    pos = tris.reshape(-1, 3)
+    idx = np.arange(len(pos), dtype=np.uint32).reshape(-1, 3)
+    return scenes.mesh_scene(pos, idx)
+
+
+def _tri_at(x):
+    # area-carrying triangle whose centroid is (x, 1/3, 1/3)
+    return [[x - 0.5, 0, 0], [x + 0.5, 0, 0], [x, 1, 1]]
+
+
+def test_bvh_two_triangles_hand_worked():
+    # Centroids at x = 0 and x = 3 on a line: every bin's centroid box has
+    # zero surface area, so every split costs 0.125 + 0 / SA(node); strict '<'
+    # keeps bin 0 and the split is 0 + 1 * 3 / 16 = 0.1875.
+    a, b = _tri_at(0.0), _tri_at(3.0)
+    for order, reorder in (([a, b], [0, 1]), ([b, a], [1, 0])):
+        bounds, off, ro = oracle.Scene(_tri_scene(order)).bvh()
+        assert off.tolist() == [[2, 0], [0, 1], [1, 1]]
+        assert ro.tolist() == reorder
+        np.testing.assert_array_equal(bounds[0], [-0.5, 0, 0, 3.5, 1, 1])
+
+
+def test_bvh_collinear_triangles_chain_to_depth_budget():
+    # Zero-area (collinear) triangles give a node surface area of 0, every
+    # cost is 0/0 = NaN, no bin wins the strict '<', bin 15 splits at the
+    # centroid maximum, everything goes left, and the node chains down to the
+    # depth budget with empty right children (bvhhelper.cpp:155-233).
+    a = [[-0.5, 0, 0], [0.5, 0, 0], [0, 0, 0]]
+    b = [[2.5, 0, 0], [3.5, 0, 0], [3, 0, 0]]
+    bounds, off, ro = oracle.Scene(_tri_scene([a, b]), max_depth=6).bvh()
+    leaves = off[off[:, 1] > 0]
+    empties = (off[:, 1] == 0) & (bounds[:, 0] > bounds[:, 3])
+    assert leaves.tolist() == [[0, 2]] and empties.sum() == 5 and len(off) == 11
+
+
+def test_bvh_depth_budget_and_degenerate_centroids():
+    tris = [[[i, 0, 0], [i + 1, 0, 0], [i, 1, 0]] for i in range(6)]
+    _, off, _ = oracle.Scene(_tri_scene(tris), max_depth=1).bvh()
+    assert off.tolist() == [[0, 6]]                       # depthBudget <= 1: one leaf
+    same = [[[0, 0, 0], [1, 0, 0], [0, 1, 0]]] * 5
+    _, off, ro = oracle.Scene(_tri_scene(same)).bvh()
+    assert off.tolist() == [[0, 5]] and ro.tolist() == [0, 1, 2, 3, 4]   # centroid box < 1e-9
+
+
+def test_bvh_four_triangles_hand_worked():
+    # Centroids x = 0, 1, 2, 10: bins of width 10/16; left {0,1,2} has a
+    # centroid SA of 0 (a line), as does right {10}; k = 0 gives left {0}
+    # (0.125 + 0) and is the first minimum.
+    tris = [_tri_at(x) for x in (0.0, 1.0, 2.0, 10.0)]
+    b, off, ro = oracle.Scene(_tri_scene(tris)).bvh()
+    # root -> leaf {0} and subtree {1,2,10}
+    assert off[0].tolist() == [2, 0] and off[1].tolist() == [0, 1]
+    assert ro[0] == 0
+
+
+def _check_invariants(sc, bounds, off, ro, max_depth=32):
+    nt = len(sc["idx"])
+    seen = np.zeros(nt, int)
+    p = sc["pos"][sc["idx"]]
+    tmin, tmax = p.min(axis=1), p.max(axis=1)
+    stack = [(0, 0)]
+    while stack:
+        n, d = stack.pop()
+        assert d < max_depth
+        if off[n, 1] == 0 and bounds[n, 0] <= bounds[n, 3]:
+            for c in (n + 1, n + off[n, 0]):
+                assert (bounds[c, :3] >= bounds[n, :3]).all() and (bounds[c, 3:] <= bounds[n, 3:]).all()
+                stack.append((c, d + 1))
+        elif off[n, 1] > 0:
+            t = ro[off[n, 0]:off[n, 0] + off[n, 1]]
+            seen[t] += 1
+            np.testing.assert_array_equal(bounds[n, :3], tmin[t].min(axis=0))
+            np.testing.assert_array_equal(bounds[n, 3:], tmax[t].max(axis=0))
+    assert (seen == 1).all()
+    assert sorted(ro.tolist()) == list(range(nt))
+
+
+@pytest.mark.parametrize("name", ["test_cube", "test_sphere", "cornell_box_modded", "test_prism", "test_spheres"])
+def test_bvh_invariants_reference_meshes(name):
+    m = np.load(os.path.join(GOLDEN, "meshes.npz"))
+    sc = scenes.mesh_scene(m[name + "_pos"], m[name + "_idx"])
+    bounds, off, ro = oracle.Scene(sc).bvh()
+    _check_invariants(sc, bounds, off, ro)
+
+
+def test_bvh_invariants_cornell_and_random():
+    sc = scenes.cornell()
+    _check_invariants(sc, *oracle.Scene(sc).bvh())
+    rng = np.random.default_rng(3)
+    pos = rng.normal(size=(3000, 3)).astype(np.float32)
+    sc = scenes.mesh_scene(pos, np.arange(3000, dtype=np.uint32).reshape(-1, 3))
+    _check_invariants(sc, *oracle.Scene(sc).bvh())
+
+
+# ---------------------------------------------------------------- intersection (cudaRTCommon.cu:42-64)
+def test_intersect_closed_form():
+    sc = _tri_scene([[[0, 0, 0], [1, 0, 0], [0, 1, 0]]])
+    S = oracle.Scene(sc)
+    rays = rays_array(np.array([[0.25, 0.25, 1.0]], np.float32), np.array([[0, 0, -1.0]], np.float32), eps=0.0)
+    for mode in ("brute", "defined", "reference"):
+        h, _ = S.trace(rays, mode)
+        assert (h["tri"][0], h["t"][0], h["u"][0], h["v"][0]) == (0, 1.0, 0.25, 0.25)
+    # eps shifts the origin along the ray: t = 1 - eps (IntersectTri's T = (o + eps d) - p0)
+    rays[0, 7] = 0.5
+    h, _ = S.trace(rays, "brute")
+    assert h["t"][0] == 0.5
+    # back-face culling: the triangle normal is +z, a ray travelling +z sees the back
+    up = rays_array(np.array([[0.25, 0.25, -1.0]], np.float32), np.array([[0, 0, 1.0]], np.float32), eps=0.0)
+    assert S.trace(up, "brute", cull=False)[0]["tri"][0] == 0
+    assert S.trace(up, "brute", cull=True)[0]["tri"][0] == 0xFFFFFFFF
+    # parallel ray: |det| < 1e-9 -> miss; behind the origin -> miss; edge u + v = 1 -> hit
+    par = rays_array(np.array([[0.25, 0.25, 0.0]], np.float32), np.array([[1.0, 0, 0]], np.float32), eps=0.0)
+    assert S.trace(par, "brute")[0]["tri"][0] == 0xFFFFFFFF
+    away = rays_array(np.array([[0.25, 0.25, 1.0]], np.float32), np.array([[0, 0, 1.0]], np.float32), eps=0.0)
+    assert S.trace(away, "brute")[0]["tri"][0] == 0xFFFFFFFF
+    edge = rays_array(np.array([[0.5, 0.5, 1.0]], np.float32), np.array([[0, 0, -1.0]], np.float32), eps=0.0)
+    h = S.trace(edge, "brute")[0]
+    assert h["tri"][0] == 0 and h["u"][0] + h["v"][0] == 1.0
+    # max distance is exclusive
+    short = rays.copy()
+    short[0, 7] = 0.0
+    short[0, 3] = 1.0
+    assert S.trace(short, "brute")[0]["tri"][0] == 0xFFFFFFFF
+
+
+def test_tie_break_smallest_original_id():
+    # two coincident triangles: equal t, the smaller original id wins
+    t = [[0, 0, 0], [1, 0, 0], [0, 1, 0]]
+    S = oracle.Scene(_tri_scene([t, t, t]))
+    rays = rays_array(np.array([[0.2, 0.3, 1.0]], np.float32), np.array([[0, 0, -1.0]], np.float32))
+    for mode in ("brute", "defined"):
+        assert S.trace(rays, mode)[0]["tri"][0] == 0
+
+
+@pytest.mark.parametrize("name", ["test_sphere", "cornell_box_with_sphere", "test_spheres"])
+def test_defined_traversal_equals_brute_force(name):
+    m = np.load(os.path.join(GOLDEN, "meshes.npz"))
+    sc = scenes.mesh_scene(m[name + "_pos"], m[name + "_idx"])
+    S = oracle.Scene(sc)
+    lo, hi = sc["pos"].min(axis=0), sc["pos"].max(axis=0)
+    rays = np.concatenate([random_rays(4000, lo, hi, seed=1), vertex_rays(sc, 2000, seed=2)])
+    a, _ = S.trace(rays, "brute")
+    b, _ = S.trace(rays, "defined")
+    assert hits_equal(a, b).all()
+
+
+def test_reference_traversal_agrees_off_ties():
+    """TracePrimitive's own order (strict '<', first found) agrees with the
+    tie-defined answer except on equal-t ties and eps-near misses."""
+    sc = scenes.cornell()
+    S = oracle.Scene(sc)
+    rays = np.concatenate([camera_rays(sc["camera"], 128, 128), random_rays(20000, [-0.9, 0.1, -0.9], [0.9, 1.9, 0.9])])
+    a, _ = S.trace(rays, "brute")
+    r, stats = S.trace(rays, "reference")
+    assert stats.tolist() == [0, 0]                       # no stack drop, no 2048 cap
+    agree = hits_equal(a, r)
+    assert agree.mean() > 0.99
+    diff = ~agree
+    # every disagreement is a seam: a different triangle at the same or a
+    # rounding-close distance, never a miss against a hit
+    assert (r["tri"][diff] != 0xFFFFFFFF).all() and (a["tri"][diff] != 0xFFFFFFFF).all()
+    assert np.all(np.abs(a["t"][diff] - r["t"][diff]) <= 1e-5 * np.abs(a["t"][diff]))
+
+
+# ---------------------------------------------------------------- estimator (cudaRTBDPTStream.cu)
+def test_emitter_only_scene_is_exact():
+    b = scenes._Builder()
+    m = b.add_material(scenes.material(diffuse=(0, 0, 0), emissive=(2.5, 1.5, 0.5)))
+    b.add_quad((-50, -50, -1), (50, -50, -1), (50, 50, -1), (-50, 50, -1), (0, 0, 1), m)
+    cam = dict(pos=(0.0, 0.0, 0.0), dir=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fov=0.5)
+    sc = b.build(cam)
+    O = oracle.Scene(sc)
+    for prec in (32, 64):
+        R = oracle.Renderer(oracle.render_params(16, 16, cam), prec)
+        img, N = R.render(O, 0, 3)
+        assert np.allclose(img, [2.5, 1.5, 0.5], atol=1e-6) and (N == 3).all()
+
+
+def test_black_scene_is_zero():
+    sc = scenes.cornell(light_radiance=0.0)
+    O = oracle.Scene(sc)
+    img, N = oracle.Renderer(oracle.render_params(16, 16, sc["camera"]), 64).render(O, 0, 2)
+    assert (img == 0).all() and (N == 2).all()
+
+
+def test_single_connection_weight_by_hand():
+    """One diffuse floor under a one-vertex light cache: with the light paths
+    all leaving the scene, every cached vertex is an emission point, and the
+    connection adds GetShadingResult(...) * pathInMulTerm with N += 4."""
+    b = scenes._Builder()
+    floor = b.add_material(scenes.material(diffuse=(0.5, 0.5, 0.5)))
+    lmat = b.add_material(scenes.material(diffuse=(0, 0, 0), emissive=(4.0, 4.0, 4.0)))
+    b.add_quad((-1, 0, 1), (1, 0, 1), (1, 0, -1), (-1, 0, -1), (0, 1, 0), floor)
+    # light faces up: its light paths all leave the scene
+    b.add_quad((-0.01, 1, 0.01), (0.01, 1, 0.01), (0.01, 1, -0.01), (-0.01, 1, -0.01), (0, 1, 0), lmat)
+    cam = dict(pos=(0.0, 0.5, 0.0), dir=(0.0, -1.0, 0.0), up=(0.0, 0.0, -1.0), fov=0.2)
+    sc = b.build(cam)
+    O = oracle.Scene(sc)
+    R = oracle.Renderer(oracle.render_params(4, 4, cam, eye_depth=2), 64)
+    img, N = R.render(O, 0, 1)
+    lv = R.light_vertices()
+    # every light path leaves through the open side: the cache holds emission points only
+    assert (lv[:, 9:12] == 0).all()
+    # N counts 1 per path + 4 per successful connection; depth 2 allows one connection
+    assert set(np.unique(N).tolist()) <= {1, 5}
+    # centre pixel: floor point ~ (0, 0, 0), light vertex ~ (0, 1, 0) above it.
+    # GetShadingResult(in = -d, toL = +y, irrad = 4, n = +y, diff .5, metal 0, rough 1, spec 0):
+    # f0 = 0, F = 0 -> diffuse term only: irrad * noL * (1 - F) * diff / pi = 4 * 1 * 0.5 / pi
+    # pathInMulTerm = 1 at the first hit -> accum = 2/pi; N = 5 -> result = (2/pi) / 5.
+    connected = N == 5
+    assert connected.any()
+    # off-centre points see the light at noL >= 0.996
+    np.testing.assert_allclose(img[connected], (2 / math.pi) / 5, rtol=5e-3)
+
+
+def test_fp32_matches_fp64_within_gate():
+    sc = scenes.cornell()
+    O = oracle.Scene(sc)
+    p = oracle.render_params(96, 96, sc["camera"])
+    a, _ = oracle.Renderer(p, 32).render(O, 0, 8)
+    R64 = oracle.Renderer(p, 64)
+    R64.render(O, 0, 8)
+    assert rel_mse(a, R64.result64()) <= 1e-4
+
+
+def test_literal_shadow_endpoint_is_ill_conditioned():
+    """The reference's dist - eps endpoint leaves visibility to rounding: FP32
+    and FP64 runs then disagree far beyond the gate (recorded deviation)."""
+    sc = scenes.cornell()
+    O = oracle.Scene(sc)
+    p = oracle.render_params(64, 64, sc["camera"], flags=1)
+    a, na = oracle.Renderer(p, 32).render(O, 0, 8)
+    R64 = oracle.Renderer(p, 64)
+    _, nb = R64.render(O, 0, 8)
+    assert (na != nb).mean() > 0.2
+
+
+# ---------------------------------------------------------------- filter (cudaBilateralFilter.cu)
+def _ref_clamp(x, j, n):
+    v = (x + j) % (1 << 32)
+    return v if v < n - 1 else n - 1
+
+
+def test_gaussian_table():
+    g = oracle.gaussian_table(2.0, 3)
+    np.testing.assert_allclose(g, np.exp(-np.arange(-3, 4) ** 2 / 8.0), rtol=1e-6)
+
+
+def test_bilateral_identity_fixed_point_and_gaussian_limit():
+    rng = np.random.default_rng(0)
+    H, W = 12, 17
+    col = rng.uniform(size=(H, W, 3)).astype(np.float32)
+    pos = rng.uniform(size=(H, W, 3)).astype(np.float32)
+    nrm = rng.uniform(size=(H, W, 3)).astype(np.float32)
+    out = oracle.bilateral(col, pos, nrm, None, 0.5, 1.0, 0.25, 0.1, 0)
+    np.testing.assert_array_equal(out, col)                          # radius 0
+    const = np.full_like(col, 0.37)
+    out = oracle.bilateral(const, pos, nrm, None, 0.5, 1.0, 0.25, 0.1, 3)
+    np.testing.assert_allclose(out, const, rtol=1e-6)                # constant image
+    # huge range sigmas: a separable Gaussian (spatial sigma = colour sigma = 2)
+    # with the reference's low-side-to-high-edge clamp
+    r, s = 3, 1e4
+    out = oracle.bilateral(col, pos, nrm, None, s, s, s, s, r, prec=64)
+    g = np.exp(-np.arange(-r, r + 1) ** 2 / (2 * s * s))
+    ref = np.zeros_like(col, dtype=np.float64)
+    wsum = 0
+    for i in range(-r, r + 1):
+        for j in range(-r, r + 1):
+            ys = [_ref_clamp(y, i, H) for y in range(H)]
+            xs = [_ref_clamp(x, j, W) for x in range(W)]
+            ref += g[i + r] * g[j + r] * col[np.ix_(ys, xs)]
+            wsum += g[i + r] * g[j + r]
+    np.testing.assert_allclose(out, ref / wsum, rtol=1e-5)
+    # the quirk itself: a tap left of column 0 reads the last column
+    assert _ref_clamp(0, -1, W) == W - 1 and _ref_clamp(5, -2, W) == 3
+
+
+def test_error_estimators():
+    f = np.array([[1.0, 2.0, 3.0]], np.float32)
+    r = np.array([[0.0, 0.0, 0.0]], np.float32)
+    N = np.array([5], np.uint32)
+    e = oracle.square_error(f, r, N, 5)
+    np.testing.assert_allclose(e, 0.5 * 14 / 3, rtol=1e-6)           # nd/(nd+n) * |d|^2 / 3
+    mean = np.array([[0.5, 0.5, 0.5]], np.float32)
+    sq = np.array([[0.5, 0.5, 0.5]], np.float32)                      # Bernoulli(1/2): E[x^2] = 1/2
+    v = oracle.empirical_error(mean, sq, np.array([4], np.uint32))
+    np.testing.assert_allclose(v, 0.25 * 4 / 3, rtol=1e-6)           # unbiased variance, n/(n-1)
+    err = np.random.default_rng(1).uniform(size=1000).astype(np.float32)
+    c = oracle.alloc_proportional(err, float(err.sum()), 1000)
+    assert c.sum() <= 1000 and c.sum() >= 1000 - 1000
