@@ -19,7 +19,8 @@ _lib = None
 
 CPT_MODE_BDPT_STREAM = 0
 CPT_MODE_BDPT_ADAPTIVE_PLL = 1
-CPT_BUF_RESULT, CPT_BUF_SAMPLE_N = 0, 1
+(CPT_BUF_RESULT, CPT_BUF_SAMPLE_N, CPT_BUF_GBUF_POS, CPT_BUF_GBUF_NORM, CPT_BUF_GBUF_DIFF, CPT_BUF_FILTERED,
+ CPT_BUF_ERROR, CPT_BUF_SAMPLE_COUNTS) = range(8)
 
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("max_dist", "<f4"), ("d", "<f4", 3), ("eps", "<f4")])
 HIT_DTYPE = np.dtype([("tri", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
@@ -33,7 +34,8 @@ class CptError(RuntimeError):
 class _Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("extension_rays", "shadow_rays", "eye_paths", "frames",
                                               "extend_launches", "extend_nodes", "extend_tris")] + \
-               [(n, ctypes.c_double) for n in ("extend_ms", "shadow_ms", "shade_ms", "light_ms", "total_ms")]
+               [(n, ctypes.c_double) for n in ("extend_ms", "shadow_ms", "shade_ms", "light_ms", "total_ms",
+                                               "filter_ms", "error_ms", "gen_ms", "accum_ms")]
 
 
 def lib():
@@ -65,6 +67,8 @@ def lib():
             "cpt_trace_closest_device": [vp, vp, c32, c32, vp, vp, c32],
             "cpt_trace_occluded_device": [vp, vp, c32, c32, vp, vp],
             "cpt_trace_counters": [vp, vp, vp, vp, i32],
+            "cpt_filter_cross_bilateral_device": [vp, vp, vp, vp, c32, c32, f32, f32, f32, f32, c32, vp, c32, vp,
+                                                  vp, vp],
             "cpt_renderer_create": [vp, i32, c32, c32],
             "cpt_render_iteration": [vp, vp, vp, vp, vp, f32, vp],
             "cpt_render_frames": [vp, vp, vp, vp, vp, f32, c32, vp],
@@ -254,7 +258,10 @@ class Renderer:
 
     def set_attrib(self, name, value):
         key = {"eye_depth": "Eye Depth", "light_depth": "Light Depth", "light_paths": "Light Paths",
-               "seed": "Seed", "literal_shadow": "Literal Shadow Endpoint"}.get(name, name)
+               "seed": "Seed", "literal_shadow": "Literal Shadow Endpoint", "warmup": "Warm-up Frames",
+               "nd": "Desired Max Sampling", "sigma_color": "Filter Color EuD", "sigma_pos": "Filter Pos EuD",
+               "sigma_norm": "Filter Norm EuD", "sigma_diff": "Filter Diff EuD", "radius": "Filter Radius",
+               "filter_diff": "Filter Diffuse Flag"}.get(name, name)
         idx = self.attribs()[key][0]
         _check(lib().cpt_renderer_set_attrib(self.h, idx, float(value)), "cpt_renderer_set_attrib")
 
@@ -271,6 +278,17 @@ class Renderer:
     def sample_counts(self):
         out = np.zeros((self.H, self.W), np.uint32)
         _check(lib().cpt_renderer_read_buffer(self.h, CPT_BUF_SAMPLE_N, _ptr(out)), "cpt_renderer_read_buffer")
+        return out
+
+    def read(self, which: int):
+        """A renderer buffer: CPT_BUF_* (result, counts, G-buffer, filtered, error, per-frame sample counts)."""
+        if which in (CPT_BUF_SAMPLE_N, CPT_BUF_SAMPLE_COUNTS):
+            out = np.zeros((self.H, self.W), np.uint32)
+        elif which == CPT_BUF_ERROR:
+            out = np.zeros((self.H, self.W), np.float32)
+        else:
+            out = np.zeros((self.H, self.W, 3), np.float32)
+        _check(lib().cpt_renderer_read_buffer(self.h, which, _ptr(out)), "cpt_renderer_read_buffer")
         return out
 
     def reset(self):
@@ -292,3 +310,20 @@ def make_rays(o, d, max_dist=1e20, eps=1e-4) -> np.ndarray:
     r["o"], r["d"] = o, d
     r["max_dist"], r["eps"] = max_dist, eps
     return r
+
+
+def filter_cross_bilateral(color, pos, norm, diff, sample_n, sigma_color, sigma_pos, sigma_norm, sigma_diff, radius,
+                           nd=5):
+    """Device cross-bilateral + weighted squared residual on host images (copies in and out)."""
+    H, W = color.shape[:2]
+    bufs = [DeviceBuffer.from_array(np.ascontiguousarray(a, np.float32)) for a in (color, pos, norm)]
+    db = DeviceBuffer.from_array(np.ascontiguousarray(diff, np.float32)) if diff is not None else None
+    dn = DeviceBuffer.from_array(np.ascontiguousarray(sample_n, np.uint32))
+    df = DeviceBuffer(W * H * 12)
+    de = DeviceBuffer(W * H * 4)
+    vp = ctypes.c_void_p
+    _check(lib().cpt_filter_cross_bilateral_device(vp(bufs[0].ptr), vp(bufs[1].ptr), vp(bufs[2].ptr),
+                                                   vp(db.ptr) if db else None, W, H, sigma_color, sigma_pos,
+                                                   sigma_norm, sigma_diff, radius, vp(dn.ptr), nd, vp(df.ptr),
+                                                   vp(de.ptr), None), "cpt_filter_cross_bilateral_device")
+    return df.download(np.float32, W * H * 3).reshape(H, W, 3), de.download(np.float32, W * H).reshape(H, W)

@@ -352,6 +352,37 @@ int cpt_trace_closest(cpt_scene* s, const cpt_ray* rays, uint32_t n, uint32_t cu
   return CPT_OK;
 }
 
+int cpt_filter_cross_bilateral_device(const float* d_color, const float* d_pos, const float* d_norm,
+                                      const float* d_diff, uint32_t width, uint32_t height, float sigma_color,
+                                      float sigma_pos, float sigma_norm, float sigma_diff, uint32_t radius,
+                                      const uint32_t* d_sample_n, uint32_t nd, float* d_filtered, float* d_err,
+                                      void* stream) {
+  if (!d_color || !d_pos || !d_norm || !d_sample_n || !d_filtered || !d_err)
+    return fail(CPT_E_ARG, "cpt_filter_cross_bilateral_device: null argument");
+  if (width == 0 || height == 0) return fail(CPT_E_ARG, "cpt_filter_cross_bilateral_device: empty image");
+  if (radius > (u32)kMaxFilterRadius) return fail(CPT_E_LIMIT, "cpt_filter_cross_bilateral_device: radius %u > 31", radius);
+  hipStream_t st = (hipStream_t)stream;
+  float g[64] = {0};
+  for (u32 i = 0; i < 2 * radius + 1; i++) {
+    int x = (int)i - (int)radius;
+    g[i] = std::exp(-(float)(x * x) / (2 * sigma_color * sigma_color));
+  }
+  float* d_g = nullptr;
+  HIP_TRY(hipMalloc((void**)&d_g, sizeof g));
+  HIP_TRY(hipMemcpyAsync(d_g, g, sizeof g, hipMemcpyHostToDevice, st));
+  FilterArgs f;
+  f.width = width; f.height = height; f.radius = radius; f.nd = nd;
+  f.kc = 2.f * sigma_color * sigma_color; f.kp = 2.f * sigma_pos * sigma_pos;
+  f.kn = 2.f * sigma_norm * sigma_norm; f.kd = 2.f * sigma_diff * sigma_diff;
+  f.use_diff = d_diff != nullptr;
+  hipError_t e = launch_filter_error(f, d_color, d_pos, d_norm, d_diff, d_g, d_sample_n, d_filtered, d_err, st);
+  hipError_t e2 = hipStreamSynchronize(st);
+  (void)hipFree(d_g);
+  HIP_TRY(e);
+  HIP_TRY(e2);
+  return CPT_OK;
+}
+
 int cpt_trace_counters(cpt_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* rays, int reset) {
   if (!s) return fail(CPT_E_ARG, "cpt_trace_counters: null scene");
   if (!s->d_stats.p) return fail(CPT_E_STATE, "cpt_trace_counters: scene not uploaded");
@@ -379,24 +410,33 @@ struct Attrib {
 
 struct PathSet {
   DevArray<float4> ray_o, ray_d, mul, smp, acc;
-  DevArray<u32> pix, depth;
+  DevArray<u32> pix, sid, depth;
   DevArray<uint64_t> rng;
   hipError_t alloc(size_t n) {
     hipError_t e;
     if ((e = ray_o.alloc(n)) || (e = ray_d.alloc(n)) || (e = mul.alloc(n)) || (e = smp.alloc(n)) ||
-        (e = acc.alloc(n)) || (e = pix.alloc(n)) || (e = depth.alloc(n)) || (e = rng.alloc(n)))
+        (e = acc.alloc(n)) || (e = pix.alloc(n)) || (e = sid.alloc(n)) || (e = depth.alloc(n)) ||
+        (e = rng.alloc(n)))
       return e;
     return hipSuccess;
   }
   PathBuf view() {
     PathBuf b;
-    b.ray_o = ray_o.p; b.ray_d = ray_d.p; b.pix = pix.p; b.depth = depth.p;
+    b.ray_o = ray_o.p; b.ray_d = ray_d.p; b.pix = pix.p; b.sid = sid.p; b.depth = depth.p;
     b.mul = mul.p; b.smp = smp.p; b.acc = acc.p; b.rng = rng.p;
     return b;
   }
 };
 
-enum { ATTR_EYE_DEPTH = 0, ATTR_LIGHT_DEPTH, ATTR_LIGHT_PATHS, ATTR_SEED, ATTR_LITERAL_SHADOW, ATTR_N };
+// Attribute names: the reference's own list for the adaptive renderer
+// (src/cudaRTBDPTStreamAdapBilatAtomic.cu:53-71, same order), then this
+// build's estimator parameters.
+const char* const kEyeDepth = "Eye Depth";
+const char* const kLightDepth = "Light Depth";
+const char* const kLightPaths = "Light Paths";
+const char* const kSeed = "Seed";
+const char* const kLiteralShadow = "Literal Shadow Endpoint";
+const char* const kWarmup = "Warm-up Frames";
 
 }  // namespace
 
@@ -423,7 +463,7 @@ struct cpt_renderer {
   struct Ev { hipEvent_t a, b; int kind; };
   std::vector<Ev> events;          // recorded pairs awaiting readback
   std::vector<hipEvent_t> pool;    // reusable events
-  double ms[5] = {0, 0, 0, 0, 0};  // extend, shadow, shade, light, total
+  double ms[9] = {0};  // extend, shadow, shade, light, total, filter, error sum, generation, accumulation
   ~cpt_renderer() {
     for (auto& e : events) { pool.push_back(e.a); pool.push_back(e.b); }
     for (auto e : pool) (void)hipEventDestroy(e);
@@ -433,19 +473,66 @@ struct cpt_renderer {
     if (!pool.empty()) { *e = pool.back(); pool.pop_back(); return hipSuccess; }
     return hipEventCreate(e);
   }
-  double attr(int i) const { return attribs[i].value; }
+  double attr(const char* name) const {
+    for (const auto& a : attribs)
+      if (std::strcmp(a.name, name) == 0) return a.value;
+    return 0.0;
+  }
+  void set_attr(const char* name, double v) {
+    for (auto& a : attribs)
+      if (std::strcmp(a.name, name) == 0) a.value = v;
+  }
+  // adaptive mode
+  DevArray<float> gpos, gnrm, gdif, filtered, err, gauss, err_sum;
+  DevArray<u32> counts, offsets, block_tot, total;
+  DevArray<double> block_sums;
+  DevArray<float4> rec_acc, rec_pos, rec_nrm, rec_dif;
+  float gauss_sigma = -1.f;
+  u32 gauss_radius = 0;
+  u32 cap = 0;
+  SampleRec rec() {
+    SampleRec r;
+    r.acc = rec_acc.p; r.gpos = rec_pos.p; r.gnrm = rec_nrm.p; r.gdif = rec_dif.p;
+    return r;
+  }
 };
 
 namespace {
 
 int renderer_alloc(cpt_renderer* r) {
   size_t n = (size_t)r->W * r->H;
-  HIP_TRY(r->A.alloc(n));
-  HIP_TRY(r->B.alloc(n));
-  HIP_TRY(r->hits.alloc(n));
-  HIP_TRY(r->sh_o.alloc(n));
-  HIP_TRY(r->sh_d.alloc(n));
-  HIP_TRY(r->sh_c.alloc(n));
+  // adaptive frames draw floor(W*H*e_i/sum e) samples: at most W*H plus float
+  // rounding of the ratios, bounded by the margin (and clamped on the device)
+  size_t cap = r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL ? n + 1024 : n;
+  r->cap = (u32)cap;
+  HIP_TRY(r->A.alloc(cap));
+  HIP_TRY(r->B.alloc(cap));
+  HIP_TRY(r->hits.alloc(cap));
+  HIP_TRY(r->sh_o.alloc(cap));
+  HIP_TRY(r->sh_d.alloc(cap));
+  HIP_TRY(r->sh_c.alloc(cap));
+  if (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL) {
+    HIP_TRY(r->gpos.alloc(n * 3));
+    HIP_TRY(r->gnrm.alloc(n * 3));
+    HIP_TRY(r->gdif.alloc(n * 3));
+    HIP_TRY(r->filtered.alloc(n * 3));
+    HIP_TRY(r->err.alloc(n));
+    HIP_TRY(r->gauss.alloc(64));
+    HIP_TRY(r->err_sum.alloc(1));
+    HIP_TRY(r->counts.alloc(n));
+    HIP_TRY(r->offsets.alloc(n));
+    HIP_TRY(r->block_tot.alloc(n / 1024 + 2));
+    HIP_TRY(r->total.alloc(1));
+    HIP_TRY(r->block_sums.alloc(n / 256 + 2));
+    HIP_TRY(r->rec_acc.alloc(cap));
+    HIP_TRY(r->rec_pos.alloc(cap));
+    HIP_TRY(r->rec_nrm.alloc(cap));
+    HIP_TRY(r->rec_dif.alloc(cap));
+    for (auto* a : {&r->gpos, &r->gnrm, &r->gdif, &r->filtered})
+      HIP_TRY(hipMemsetAsync(a->p, 0, sizeof(float) * n * 3, r->stream));
+    HIP_TRY(hipMemsetAsync(r->err.p, 0, sizeof(float) * n, r->stream));
+    HIP_TRY(hipMemsetAsync(r->counts.p, 0, sizeof(u32) * n, r->stream));
+  }
   HIP_TRY(r->fc.alloc(1));
   HIP_TRY(r->stats.alloc(1));
   HIP_TRY(r->result.alloc(n * 3));
@@ -460,7 +547,7 @@ int renderer_alloc(cpt_renderer* r) {
 }
 
 int lv_alloc(cpt_renderer* r) {
-  size_t need = (size_t)r->attr(ATTR_LIGHT_PATHS) * ((size_t)r->attr(ATTR_LIGHT_DEPTH) + 1);
+  size_t need = (size_t)r->attr(kLightPaths) * ((size_t)r->attr(kLightDepth) + 1);
   HIP_TRY(r->lv.alloc(need));
   return CPT_OK;
 }
@@ -492,7 +579,104 @@ void camera_basis(const float* d, const float* up, float* right, float* up2) {
   up2[0] = ux / ul; up2[1] = uy / ul; up2[2] = uz / ul;
 }
 
+// Gaussian table of cudaBilateralFilter::updateGaussian (cudaBilateralFilter.cu:5-24),
+// spatial sigma = colour sigma as the reference passes it (AdapBilatAtomic.cu:1327).
+int update_gauss(cpt_renderer* r, float sigma, u32 radius) {
+  if (sigma == r->gauss_sigma && radius == r->gauss_radius) return CPT_OK;
+  float g[64] = {0};
+  for (u32 i = 0; i < 2 * radius + 1; i++) {
+    int x = (int)i - (int)radius;
+    g[i] = std::exp(-(float)(x * x) / (2 * sigma * sigma));
+  }
+  HIP_TRY(hipMemcpyAsync(r->gauss.p, g, sizeof g, hipMemcpyHostToDevice, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  r->gauss_sigma = sigma;
+  r->gauss_radius = radius;
+  return CPT_OK;
+}
+
+int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 max_n, const Accum& ac) {
+  hipEvent_t ev = nullptr;
+  PathSet* cur = &r->A;
+  PathSet* nxt = &r->B;
+  ShadowBuf sb;
+  sb.ray_o = r->sh_o.p; sb.ray_d = r->sh_d.p; sb.contrib = r->sh_c.p;
+  for (u32 b = 0; b < fa.eye_depth; b++) {
+    HIP_TRY(timed_begin(r, 0, &ev));
+    HIP_TRY(launch_extend(sc, cur->view(), r->fc.p, (int)b, max_n, r->hits.p, r->stats.p, r->stream));
+    HIP_TRY(timed_end(r, 0, ev));
+    HIP_TRY(timed_begin(r, 2, &ev));
+    HIP_TRY(launch_shade(sc, fa, cur->view(), nxt->view(), r->hits.p, r->fc.p, (int)b, max_n, sb, r->lv.p, ac,
+                         r->stream));
+    HIP_TRY(timed_end(r, 2, ev));
+    if (b + 1 < fa.eye_depth) {
+      HIP_TRY(timed_begin(r, 1, &ev));
+      HIP_TRY(launch_shadow(sc, sb, nxt->view(), r->fc.p, (int)b, max_n, r->stream));
+      HIP_TRY(timed_end(r, 1, ev));
+    }
+    std::swap(cur, nxt);
+  }
+  return CPT_OK;
+}
+
+// One frame of cudaRTBDPTStreamAdapBilatAtomic::Render (AdapBilatAtomic.cu:1338-1538).
+int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
+  DevScene sc = s->dev();
+  const u32 n = r->W * r->H;
+  hipEvent_t ev = nullptr;
+  if (fa.frame % 3 == 0) {
+    HIP_TRY(timed_begin(r, 3, &ev));
+    HIP_TRY(launch_light_pass(sc, fa, r->lv.p, r->fc.p, r->stream));
+    HIP_TRY(timed_end(r, 3, ev));
+  }
+  Accum ac;
+  ac.result = r->result.p; ac.sample_n = r->sample_n.p;
+  ac.gpos = r->gpos.p; ac.gnrm = r->gnrm.p; ac.gdif = r->gdif.p;
+  ac.rec = r->rec();
+  ac.variant = 1;
+  if (fa.frame < (u32)r->attr(kWarmup)) {          // plain 1 spp frames with G-buffer (:1372-1406)
+    ac.apply = 1;
+    HIP_TRY(launch_frame_begin(r->fc.p, n, r->stream));
+    HIP_TRY(launch_gen_eye_rec(fa, r->A.view(), ac.rec, r->stream));
+    int rc = bounce_loop(r, sc, fa, n, ac);
+    if (rc) return rc;
+  } else {
+    float sc_ = (float)r->attr("Filter Color EuD"), sp_ = (float)r->attr("Filter Pos EuD");
+    float sn_ = (float)r->attr("Filter Norm EuD"), sd_ = (float)r->attr("Filter Diff EuD");
+    u32 radius = (u32)r->attr("Filter Radius");
+    int rc = update_gauss(r, sc_, radius);
+    if (rc) return rc;
+    FilterArgs f;
+    f.width = r->W; f.height = r->H; f.radius = radius; f.nd = (u32)r->attr("Desired Max Sampling");
+    f.kc = 2.f * sc_ * sc_; f.kp = 2.f * sp_ * sp_; f.kn = 2.f * sn_ * sn_; f.kd = 2.f * sd_ * sd_;
+    f.use_diff = r->attr("Filter Diffuse Flag") != 0;
+    HIP_TRY(timed_begin(r, 5, &ev));
+    HIP_TRY(launch_filter_error(f, r->result.p, r->gpos.p, r->gnrm.p, r->gdif.p, r->gauss.p, r->sample_n.p,
+                                r->filtered.p, r->err.p, r->stream));
+    HIP_TRY(timed_end(r, 5, ev));
+    HIP_TRY(timed_begin(r, 6, &ev));
+    HIP_TRY(launch_error_sum(r->err.p, n, r->block_sums.p, r->err_sum.p, r->stream));
+    HIP_TRY(timed_end(r, 6, ev));
+    HIP_TRY(timed_begin(r, 7, &ev));
+    HIP_TRY(launch_alloc_scan(r->err.p, r->err_sum.p, n, n, r->counts.p, r->offsets.p, r->block_tot.p, r->total.p,
+                              r->cap, r->stream));
+    HIP_TRY(launch_frame_begin_dev(r->fc.p, r->total.p, r->cap, r->stream));
+    HIP_TRY(launch_gen_adaptive(fa, r->offsets.p, r->counts.p, r->total.p, r->cap, r->A.view(), ac.rec, r->stream));
+    HIP_TRY(timed_end(r, 7, ev));
+    ac.apply = 2;
+    rc = bounce_loop(r, sc, fa, r->cap, ac);
+    if (rc) return rc;
+    HIP_TRY(timed_begin(r, 8, &ev));
+    HIP_TRY(launch_accum_adaptive(r->offsets.p, r->counts.p, n, r->cap, ac.rec, r->result.p, r->sample_n.p, r->gpos.p,
+                                  r->gnrm.p, r->gdif.p, r->stream));
+    HIP_TRY(timed_end(r, 8, ev));
+  }
+  HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, n, fa.eye_depth, r->stream));
+  return CPT_OK;
+}
+
 int render_one_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
+  if (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL) return render_adaptive_frame(r, s, fa);
   DevScene sc = s->dev();
   const u32 n = r->W * r->H;
   hipEvent_t ev = nullptr;
@@ -503,25 +687,14 @@ int render_one_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   }
   HIP_TRY(launch_frame_begin(r->fc.p, n, r->stream));
   HIP_TRY(launch_gen_eye(fa, r->A.view(), r->stream));
-  PathSet* cur = &r->A;
-  PathSet* nxt = &r->B;
-  ShadowBuf sb;
-  sb.ray_o = r->sh_o.p; sb.ray_d = r->sh_d.p; sb.contrib = r->sh_c.p;
-  for (u32 b = 0; b < fa.eye_depth; b++) {
-    HIP_TRY(timed_begin(r, 0, &ev));
-    HIP_TRY(launch_extend(sc, cur->view(), r->fc.p, (int)b, n, r->hits.p, r->stats.p, r->stream));
-    HIP_TRY(timed_end(r, 0, ev));
-    HIP_TRY(timed_begin(r, 2, &ev));
-    HIP_TRY(launch_shade(sc, fa, cur->view(), nxt->view(), r->hits.p, r->fc.p, (int)b, n, sb, r->lv.p, r->result.p,
-                         r->sample_n.p, r->stream));
-    HIP_TRY(timed_end(r, 2, ev));
-    if (b + 1 < fa.eye_depth) {
-      HIP_TRY(timed_begin(r, 1, &ev));
-      HIP_TRY(launch_shadow(sc, sb, nxt->view(), r->fc.p, (int)b, n, r->stream));
-      HIP_TRY(timed_end(r, 1, ev));
-    }
-    std::swap(cur, nxt);
-  }
+  Accum ac;
+  std::memset(&ac, 0, sizeof ac);
+  ac.result = r->result.p;
+  ac.sample_n = r->sample_n.p;
+  ac.apply = 0;
+  ac.variant = 0;
+  int rc = bounce_loop(r, sc, fa, n, ac);
+  if (rc) return rc;
   HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, n, fa.eye_depth, r->stream));
   return CPT_OK;
 }
@@ -529,14 +702,18 @@ int render_one_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
 int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* dir, const float* up, float fov,
                   uint32_t nframes, float* result) {
   if (!r || !s || !pos || !dir || !up) return fail(CPT_E_ARG, "render: null argument");
-  if (r->mode != CPT_MODE_BDPT_STREAM) return fail(CPT_E_ARG, "render: mode %d not available", r->mode);
   if (!s->has_bvh) return fail(CPT_E_STATE, "render: scene has no BVH");
   if (!s->uploaded) {                              // GetIsCudaDirty -> initAllSceneCudaMem (:830-838)
     int rc = cpt_scene_upload(s);
     if (rc) return rc;
   }
-  u32 eye_depth = (u32)r->attr(ATTR_EYE_DEPTH), light_depth = (u32)r->attr(ATTR_LIGHT_DEPTH);
-  u32 light_paths = (u32)r->attr(ATTR_LIGHT_PATHS);
+  u32 eye_depth = (u32)r->attr(kEyeDepth), light_depth = (u32)r->attr(kLightDepth);
+  u32 light_paths = (u32)r->attr(kLightPaths);
+  if (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL) {
+    u32 radius = (u32)r->attr("Filter Radius");
+    if (radius > (u32)kMaxFilterRadius) return fail(CPT_E_LIMIT, "render: filter radius %u exceeds 31", radius);
+    if (r->attr(kWarmup) < 1) return fail(CPT_E_LIMIT, "render: warm-up frames must be >= 1");
+  }
   if (eye_depth < 1 || eye_depth > (u32)kMaxDepth) return fail(CPT_E_LIMIT, "render: eye depth %u not in 1..15", eye_depth);
   if (light_depth < 1 || light_depth > 64) return fail(CPT_E_LIMIT, "render: light depth %u not in 1..64", light_depth);
   if (light_paths < 1 || light_paths > (u32)kBlock) return fail(CPT_E_LIMIT, "render: light paths %u not in 1..256", light_paths);
@@ -564,8 +741,9 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   fa.eye_depth = eye_depth;
   fa.light_depth = light_depth;
   fa.light_paths = light_paths;
-  fa.flags = r->attr(ATTR_LITERAL_SHADOW) != 0 ? 1u : 0u;
-  uint64_t seed = (uint64_t)r->attr(ATTR_SEED);
+  fa.flags = r->attr(kLiteralShadow) != 0 ? 1u : 0u;
+  uint64_t seed = (uint64_t)r->attr(kSeed);
+  fa.seed = seed;
   hipEvent_t ev_all = nullptr;
   HIP_TRY(timed_begin(r, 4, &ev_all));
   for (u32 f = 0; f < nframes; f++) {
@@ -598,8 +776,23 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   r->mode = mode;
   r->W = width;
   r->H = height;
-  r->attribs = {{"Eye Depth", 0, 5}, {"Light Depth", 0, 5}, {"Light Paths", 0, 128},
-                {"Seed", 0, (double)0x5EED0001u}, {"Literal Shadow Endpoint", 2, 0}};
+  if (mode == CPT_MODE_BDPT_ADAPTIVE_PLL) {
+    // src/cudaRTBDPTStreamAdapBilatAtomic.cu:26-71, indices 0-14 as the reference numbers them
+    r->attribs = {{"Adaptive Mode", 4, 0},          {"Desired Max Sampling", 0, 5},
+                  {"Min Trace Probability", 3, 0},  {"Debug Mode", 4, 0},
+                  {"Filter Color EuD", 3, 50},      {"Filter Pos EuD", 3, 1},
+                  {"Filter Norm EuD", 3, 0.25},     {"Filter Diff EuD", 3, 0.1},
+                  {"Filter Radius", 0, 5},          {"Filter Diffuse Flag", 2, 0},
+                  {"Avg Filter Time", 3, 0},        {"Avg MSE Time", 3, 0},
+                  {"Avg Gen Time", 3, 0},           {"Avg Trace Time", 3, 0},
+                  {"Avg Accum Time", 3, 0}};
+  }
+  r->attribs.push_back({kEyeDepth, 0, 5});
+  r->attribs.push_back({kLightDepth, 0, 5});
+  r->attribs.push_back({kLightPaths, 0, 128});
+  r->attribs.push_back({kSeed, 0, (double)0x5EED0001u});
+  r->attribs.push_back({kLiteralShadow, 2, 0});
+  if (mode == CPT_MODE_BDPT_ADAPTIVE_PLL) r->attribs.push_back({kWarmup, 0, 5});
   int rc = ensure_stream(r->stream);
   if (rc) { delete r; return rc; }
   *out = r;
@@ -659,6 +852,23 @@ int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out) {
     case CPT_BUF_SAMPLE_N:
       HIP_TRY(hipMemcpyAsync(out, r->sample_n.p, n * 4, hipMemcpyDeviceToHost, r->stream));
       break;
+    case CPT_BUF_GBUF_POS:
+    case CPT_BUF_GBUF_NORM:
+    case CPT_BUF_GBUF_DIFF:
+    case CPT_BUF_FILTERED:
+    case CPT_BUF_ERROR:
+    case CPT_BUF_SAMPLE_COUNTS: {
+      if (r->mode != CPT_MODE_BDPT_ADAPTIVE_PLL)
+        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive mode", which);
+      const void* src = which == CPT_BUF_GBUF_POS ? (const void*)r->gpos.p
+                      : which == CPT_BUF_GBUF_NORM ? (const void*)r->gnrm.p
+                      : which == CPT_BUF_GBUF_DIFF ? (const void*)r->gdif.p
+                      : which == CPT_BUF_FILTERED ? (const void*)r->filtered.p
+                      : which == CPT_BUF_ERROR ? (const void*)r->err.p : (const void*)r->counts.p;
+      size_t bytes = (which == CPT_BUF_ERROR || which == CPT_BUF_SAMPLE_COUNTS) ? n * 4 : n * 12;
+      HIP_TRY(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, r->stream));
+      break;
+    }
     default:
       return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d not produced by this mode", which);
   }
@@ -687,12 +897,16 @@ int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset) {
   out->frames = h.frames;
   out->extend_nodes = h.ext_nodes;
   out->extend_tris = h.ext_tris;
-  out->extend_launches = h.frames * (uint64_t)r->attr(ATTR_EYE_DEPTH);
+  out->extend_launches = h.frames * (uint64_t)r->attr(kEyeDepth);
   out->extend_ms = r->ms[0];
   out->shadow_ms = r->ms[1];
   out->shade_ms = r->ms[2];
   out->light_ms = r->ms[3];
   out->total_ms = r->ms[4];
+  out->filter_ms = r->ms[5];
+  out->error_ms = r->ms[6];
+  out->gen_ms = r->ms[7];
+  out->accum_ms = r->ms[8];
   if (reset) {
     HIP_TRY(hipMemset(r->stats.p, 0, sizeof(DevStats)));
     for (double& m : r->ms) m = 0;

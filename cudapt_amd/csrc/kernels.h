@@ -36,11 +36,22 @@ struct PathBuf {
   float4* ray_o;     // origin xyz, w = max distance (1e20)
   float4* ray_d;     // direction xyz, w = ray epsilon (1e-4)
   uint32_t* pix;     // pixel index
+  uint32_t* sid;     // sample record index (adaptive mode; pixel index otherwise)
   uint32_t* depth;   // bounce count
   float4* mul;       // path throughput (pathOutMulTerm)
   float4* smp;       // emitted radiance gathered along the path (pathSample)
   float4* acc;       // xyz accumulated sample, w = sample count bits
   uint64_t* rng;     // PCG32 state
+};
+
+// Per-sample records of one frame (adaptive mode): accumulated radiance and
+// sample count, and the first-hit G-buffer (pathDirectPos / Norm / Diffuse,
+// src/cudaRTBDPTStreamAdapBilatAtomic.cu:172-175,650-655).
+struct SampleRec {
+  float4* acc;       // xyz radiance, w = N bits
+  float4* gpos;
+  float4* gnrm;
+  float4* gdif;
 };
 
 // Shadow (connection) rays: segment + the contribution to add when visible.
@@ -66,6 +77,7 @@ struct FrameArgs {
   uint64_t light_key;    // PCG32 initstate for light paths
   uint32_t eye_depth, light_depth, light_paths;
   uint32_t flags;        // bit 0: literal shadow endpoint dist - eps
+  uint64_t seed;         // stream seed: per-sample keys of adaptive frames are derived on the device
 };
 
 // Device counters of one frame (zeroed by frame_begin).
@@ -84,6 +96,7 @@ struct DevStats {
 };
 
 constexpr int kMaxDepth = 15;
+constexpr int kMaxFilterRadius = 31;   // GAUSSIANCOST_N = 64 entries (include/cudaBilateralFilter.h:3)
 constexpr int kBlock = 256;
 
 // ---- launches -------------------------------------------------------------
@@ -94,9 +107,41 @@ hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, LightVert*
 hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st);
 hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, uint32_t max_n,
                          float4* hits, DevStats* stats, hipStream_t st);
+// apply: 0 = plain running mean at termination, 1 = running mean + G-buffer
+// (adaptive warm-up), 2 = write the sample record only (adaptive frame).
+struct Accum {
+  float* result;       // W*H*3
+  uint32_t* sample_n;  // W*H
+  float* gpos;         // W*H*3 running means (adaptive)
+  float* gnrm;
+  float* gdif;
+  SampleRec rec;
+  int apply;
+  int variant;         // 1: adaptive-file eye bounce (reflection offset, G-buffer)
+};
 hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, PathBuf nxt, const float4* hits,
                         FrameCounters* fc, int bounce, uint32_t max_n, ShadowBuf sb, const LightVert* lv,
-                        float* result, uint32_t* sample_n, hipStream_t st);
+                        const Accum& ac, hipStream_t st);
+hipError_t launch_frame_begin_dev(FrameCounters* fc, const uint32_t* total, uint32_t cap, hipStream_t st);
+hipError_t launch_gen_eye_rec(const FrameArgs& fa, PathBuf pb, SampleRec rec, hipStream_t st);
+
+// ---- adaptive sampling (cudaRTBDPTStreamAdapBilatAtomic.cu:1407-1538) ----
+struct FilterArgs {
+  uint32_t width, height, radius, nd;
+  float kc, kp, kn, kd;   // 2 sigma^2 per term
+  int use_diff;
+};
+hipError_t launch_filter_error(const FilterArgs& fa, const float* color, const float* gpos, const float* gnrm,
+                               const float* gdif, const float* gauss, const uint32_t* sample_n, float* filtered,
+                               float* err, hipStream_t st);
+hipError_t launch_error_sum(const float* err, uint32_t n, double* block_sums, float* sum_out, hipStream_t st);
+hipError_t launch_alloc_scan(const float* err, const float* sum, uint32_t n, uint32_t gen, uint32_t* counts,
+                             uint32_t* offsets, uint32_t* block_tot, uint32_t* total, uint32_t cap, hipStream_t st);
+hipError_t launch_gen_adaptive(const FrameArgs& fa, const uint32_t* offsets, const uint32_t* counts,
+                               const uint32_t* total, uint32_t cap, PathBuf pb, SampleRec rec, hipStream_t st);
+hipError_t launch_accum_adaptive(const uint32_t* offsets, const uint32_t* counts, uint32_t n, uint32_t cap,
+                                 SampleRec rec, float* result, uint32_t* sample_n, float* gpos, float* gnrm,
+                                 float* gdif, hipStream_t st);
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
                          uint32_t max_n, hipStream_t st);
 hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t n_paths, uint32_t depth, hipStream_t st);

@@ -197,6 +197,12 @@ __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
   return v;
 }
 
+// Host-side guards: a launch never receives a null buffer it will index.
+static bool pathbuf_ok(const PathBuf& b) {
+  return b.ray_o && b.ray_d && b.pix && b.sid && b.depth && b.mul && b.smp && b.acc && b.rng;
+}
+static bool rec_ok(const SampleRec& r) { return r.acc && r.gpos && r.gnrm && r.gdif; }
+
 // ---------------------------------------------------------------- trace API
 // out mode 0: cpt_hit {orig id, t, u, v}; 1: {t, leaf bits, u, v}; 2: occluded flag.
 template <int LAYOUT, bool ANY, int OUT>
@@ -342,7 +348,8 @@ struct Bounce {
   int type;
 };
 // BRDF sample of one bounce (src/cudaRTBDPTStream.cu:228-329 and :402-502).
-__device__ Bounce sample_bounce(uint64_t& st, uint64_t inc, f3 ro, f3 rd, float t, const Surf& s, f3 nl, int cur) {
+__device__ Bounce sample_bounce(uint64_t& st, uint64_t inc, f3 ro, f3 rd, float t, const Surf& s, f3 nl, int cur,
+                                bool refl_offset = false) {
   float x0 = uniform01(st, inc), x1 = uniform01(st, inc);
   f3 hDir = sample_ggx(x0, x1, s.rough, nl);
   f3 ndir = rd * -1.f;
@@ -368,6 +375,8 @@ __device__ Bounce sample_bounce(uint64_t& st, uint64_t inc, f3 ro, f3 rd, float 
   }
   if (prob_rand(st, inc, reflProb)) {
     b.orig = ro + rd * (float)((double)t - kBiasEps);
+    // the adaptive files also step off along the reflection (AdapBilatAtomic.cu:584)
+    if (refl_offset) b.orig = b.orig + reflDir * (float)kBiasEps;
     b.dir = reflDir;
     float VoH = dot(ndir, hDir), NoV = dot(nl, ndir), NoH = dot(nl, hDir), NoL = dot(nl, reflDir);
     float G = vis_smith(s.rough, NoV, NoL);
@@ -407,6 +416,15 @@ hipError_t launch_frame_begin(FrameCounters* fc, uint32_t n_paths, hipStream_t s
   k_frame_begin<<<1, 64, 0, st>>>(fc, n_paths);
   return hipGetLastError();
 }
+__global__ void k_frame_begin_dev(FrameCounters* fc, const uint32_t* total, uint32_t cap) {
+  uint32_t i = threadIdx.x;
+  uint32_t t = *total;
+  if (i < 16) { fc->active[i] = i == 0 ? (t < cap ? t : cap) : 0u; fc->shadow[i] = 0u; }
+}
+hipError_t launch_frame_begin_dev(FrameCounters* fc, const uint32_t* total, uint32_t cap, hipStream_t st) {
+  k_frame_begin_dev<<<1, 64, 0, st>>>(fc, total, cap);
+  return hipGetLastError();
+}
 
 __global__ void k_frame_end(FrameCounters* fc, DevStats* stats, uint32_t n_paths, uint32_t depth) {
   if (threadIdx.x == 0) {
@@ -414,7 +432,8 @@ __global__ void k_frame_end(FrameCounters* fc, DevStats* stats, uint32_t n_paths
     for (uint32_t b = 0; b < depth && b < 16; b++) { e += fc->active[b]; s += fc->shadow[b]; }
     stats->ext_rays += e;
     stats->shadow_rays += s;
-    stats->eye_paths += n_paths;
+    stats->eye_paths += fc->active[0];
+    (void)n_paths;
     stats->frames += 1;
   }
 }
@@ -518,46 +537,70 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(DevScene sc, FrameArgs fa
 }
 hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, LightVert* lv, FrameCounters* fc,
                              hipStream_t st) {
+  if (!lv || !fc || fa.light_paths > (uint32_t)kBlock || (sc.n_light_tris && !sc.light_tris)) return hipErrorInvalidValue;
   k_light_pass<<<1, kBlock, 0, st>>>(sc, fa, lv, fc);
   return hipGetLastError();
 }
 
-// pt_genPathQueue_kernel (src/cudaRTBDPTStream.cu:619-640); slot = pixel.
-__global__ __launch_bounds__(kBlock) void k_gen_eye(FrameArgs fa, PathBuf pb) {
-  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t n = fa.width * fa.height;
-  if (i >= n) return;
-  uint32_t y = i / fa.width, x = i - y * fa.width;
+// Eye ray of pixel (x, y) drawn from PCG32 stream `key`, pixel as sequence
+// (pt_genPathQueue_kernel, src/cudaRTBDPTStream.cu:619-640; the adaptive queue
+// of AdapBilatAtomic.cu:826-840 uses the same pinhole).
+__device__ __forceinline__ void write_eye_path(const FrameArgs& fa, PathBuf& pb, uint32_t slot, uint32_t pix,
+                                               uint32_t sid, uint64_t key) {
+  uint32_t y = pix / fa.width, x = pix - y * fa.width;
   float W = (float)fa.width, H = (float)fa.height;
   float tf = tanf(fa.fov * 0.5f);
   float u = ((((2.f * ((float)x + 0.5f)) / W - 1.f) * tf) * W) / H;
   float v = ((2.f * ((float)y + 0.5f)) / H - 1.f) * tf;
-  uint64_t inc = ((uint64_t)i << 1) | 1u, st;
-  pcg32_seed(st, inc, fa.eye_key);
+  uint64_t inc = ((uint64_t)pix << 1) | 1u, st;
+  pcg32_seed(st, inc, key);
   float au = u + ((uniform01(st, inc) - 0.5f) / H) * tf;
   float av = v + ((uniform01(st, inc) - 0.5f) / H) * tf;
   f3 right = mk3(fa.cam_right[0], fa.cam_right[1], fa.cam_right[2]);
   f3 up = mk3(fa.cam_up[0], fa.cam_up[1], fa.cam_up[2]);
   f3 cd = mk3(fa.cam_dir[0], fa.cam_dir[1], fa.cam_dir[2]);
   f3 d = normalize((right * au + up * av) + cd);
-  pb.ray_o[i] = make_float4(fa.cam_pos[0], fa.cam_pos[1], fa.cam_pos[2], kInfF);
-  pb.ray_d[i] = make_float4(d.x, d.y, d.z, (float)kBiasEps);
-  pb.pix[i] = i;
-  pb.depth[i] = 0;
-  pb.mul[i] = make_float4(1.f, 1.f, 1.f, 0.f);
-  pb.smp[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  pb.acc[i] = make_float4(0.f, 0.f, 0.f, 0.f);
-  pb.rng[i] = st;
+  pb.ray_o[slot] = make_float4(fa.cam_pos[0], fa.cam_pos[1], fa.cam_pos[2], kInfF);
+  pb.ray_d[slot] = make_float4(d.x, d.y, d.z, (float)kBiasEps);
+  pb.pix[slot] = pix;
+  pb.sid[slot] = sid;
+  pb.depth[slot] = 0;
+  pb.mul[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
+  pb.smp[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.acc[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
+  pb.rng[slot] = st;
+}
+
+__device__ __forceinline__ void zero_rec(SampleRec& rec, uint32_t q) {
+  const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+  rec.acc[q] = z; rec.gpos[q] = z; rec.gnrm[q] = z; rec.gdif[q] = z;
+}
+
+// One path per pixel, slot = pixel.
+__global__ __launch_bounds__(kBlock) void k_gen_eye(FrameArgs fa, PathBuf pb, SampleRec rec) {
+  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= fa.width * fa.height) return;
+  write_eye_path(fa, pb, i, i, i, fa.eye_key);
+  if (rec.acc) zero_rec(rec, i);
 }
 hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st) {
+  if (!pathbuf_ok(pb)) return hipErrorInvalidValue;
   uint32_t n = fa.width * fa.height;
-  k_gen_eye<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(fa, pb);
+  SampleRec none = {nullptr, nullptr, nullptr, nullptr};
+  k_gen_eye<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(fa, pb, none);
+  return hipGetLastError();
+}
+hipError_t launch_gen_eye_rec(const FrameArgs& fa, PathBuf pb, SampleRec rec, hipStream_t st) {
+  if (!pathbuf_ok(pb) || !rec_ok(rec)) return hipErrorInvalidValue;
+  uint32_t n = fa.width * fa.height;
+  k_gen_eye<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(fa, pb, rec);
   return hipGetLastError();
 }
 
 hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, uint32_t max_n,
                          float4* hits, DevStats* stats, hipStream_t st) {
   if (max_n == 0) return hipSuccess;
+  if (!pathbuf_ok(pb) || !fc || !hits || !sc.nodes4 || !sc.tris) return hipErrorInvalidValue;
   k_trace<0, false, 1><<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, pb.ray_o, pb.ray_d, 1,
                                                                             &fc->active[bounce], max_n, 0, hits, stats);
   return hipGetLastError();
@@ -577,6 +620,32 @@ __device__ __forceinline__ void apply_pixel(float* result, uint32_t* sample_n, u
   r[1] = fmaxf(ri * acc.y + oi * r[1], 0.f);
   r[2] = fmaxf(ri * acc.z + oi * r[2], 0.f);
   sample_n[pix] = nn;
+}
+
+// Same plus the count-weighted first-hit G-buffer running means of the
+// adaptive file (src/cudaRTBDPTStreamAdapBilatAtomic.cu:999-1036).
+__device__ __forceinline__ void apply_pixel_gbuf(const Accum& ac, uint32_t pix, uint32_t frame, f3 acc, uint32_t n,
+                                                 uint32_t sid) {
+  uint32_t nold = frame == 0 ? 0u : ac.sample_n[pix];
+  if (frame == 0) ac.sample_n[pix] = 0u;
+  uint32_t nn = nold + n;
+  if (!(nn > nold) || isinf3(acc)) return;
+  float ri = 1.f / (float)nn, oi = (float)nold * ri;
+  float* r = ac.result + (size_t)pix * 3;
+  r[0] = fmaxf(ri * acc.x + oi * r[0], 0.f);
+  r[1] = fmaxf(ri * acc.y + oi * r[1], 0.f);
+  r[2] = fmaxf(ri * acc.z + oi * r[2], 0.f);
+  ac.sample_n[pix] = nn;
+  const float nf = (float)n;
+  float4 g[3] = {ac.rec.gpos[sid], ac.rec.gnrm[sid], ac.rec.gdif[sid]};
+  float* dst[3] = {ac.gpos + (size_t)pix * 3, ac.gnrm + (size_t)pix * 3, ac.gdif + (size_t)pix * 3};
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    f3 w = ld3(g[k]) * nf;
+    dst[k][0] = ri * w.x + oi * dst[k][0];
+    dst[k][1] = ri * w.y + oi * dst[k][1];
+    dst[k][2] = ri * w.z + oi * dst[k][2];
+  }
 }
 
 // Append with one atomic per workgroup: returns this lane's slot when pred.
@@ -604,7 +673,7 @@ __device__ __forceinline__ void block_append2(bool p0, bool p1, uint32_t* ctr0, 
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, PathBuf cur, PathBuf nxt,
                                                   const float4* __restrict__ hits, FrameCounters* __restrict__ fc,
                                                   int bounce, ShadowBuf sb, const LightVert* __restrict__ lv,
-                                                  float* __restrict__ result, uint32_t* __restrict__ sample_n) {
+                                                  Accum ac) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t count = fc->active[bounce];
   if (blockIdx.x * kBlock >= count) return;               // whole workgroup idle
@@ -612,11 +681,12 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
   bool survive = false, emit = false;
   float4 o_out = make_float4(0.f, 0.f, 0.f, 0.f), d_out = o_out, mul_out = o_out, smp_out = o_out, acc_out = o_out;
   float4 so = o_out, sd = o_out, sc4 = o_out;
-  uint32_t pix = 0, depth = 0;
+  uint32_t pix = 0, sid = 0, depth = 0;
   uint64_t st = 0;
   if (i < count) {
     float4 ro = cur.ray_o[i], rd = cur.ray_d[i];
     pix = cur.pix[i];
+    sid = cur.sid[i];
     depth = cur.depth[i];
     float4 m4 = cur.mul[i], s4 = cur.smp[i], a4 = cur.acc[i];
     st = cur.rng[i];
@@ -630,9 +700,15 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
       acc = acc + psample; N++; term = true;
     } else {
       float w = (1.f - h.z) - h.w;
-      Surf s = fetch_surf(sc, leaf, w, h.z, h.w, false);
+      const bool gbuf = ac.variant == 1 && depth == 0;
+      Surf s = fetch_surf(sc, leaf, w, h.z, h.w, gbuf);
       f3 nl = dot(s.norm, dir) < 0.f ? s.norm : s.norm * -1.f;
-      Bounce b = sample_bounce(st, inc, orig, dir, h.x, s, nl, RT_EYE);
+      if (gbuf) {                                          // AdapBilatAtomic.cu:650-655
+        ac.rec.gpos[sid] = make_float4(s.pos.x, s.pos.y, s.pos.z, 0.f);
+        ac.rec.gnrm[sid] = make_float4(nl.x, nl.y, nl.z, 0.f);
+        ac.rec.gdif[sid] = make_float4(s.diff.x, s.diff.y, s.diff.z, 0.f);
+      }
+      Bounce b = sample_bounce(st, inc, orig, dir, h.x, s, nl, RT_EYE, ac.variant == 1);
       psample = psample + vmul(s.emissive, mul);
       f3 inMul = mul, inDir = dir * -1.f;
       float contrib = length(mul) * length(b.mul);
@@ -678,7 +754,9 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
       }
     }
     if (term) {
-      apply_pixel(result, sample_n, pix, fa.frame, acc, N);
+      if (ac.apply == 0) apply_pixel(ac.result, ac.sample_n, pix, fa.frame, acc, N);
+      else if (ac.apply == 1) apply_pixel_gbuf(ac, pix, fa.frame, acc, N, sid);
+      else ac.rec.acc[sid] = make_float4(acc.x, acc.y, acc.z, __uint_as_float(N));
     } else {
       survive = true;
       o_out = make_float4(orig.x, orig.y, orig.z, kInfF);
@@ -694,6 +772,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
     nxt.ray_o[slot] = o_out;
     nxt.ray_d[slot] = d_out;
     nxt.pix[slot] = pix;
+    nxt.sid[slot] = sid;
     nxt.depth[slot] = depth;
     nxt.mul[slot] = mul_out;
     nxt.smp[slot] = smp_out;
@@ -709,10 +788,14 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
 }
 hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, PathBuf nxt, const float4* hits,
                         FrameCounters* fc, int bounce, uint32_t max_n, ShadowBuf sb, const LightVert* lv,
-                        float* result, uint32_t* sample_n, hipStream_t st) {
+                        const Accum& ac, hipStream_t st) {
   if (max_n == 0) return hipSuccess;
-  k_shade<<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, fa, cur, nxt, hits, fc, bounce, sb, lv, result,
-                                                             sample_n);
+  if (!pathbuf_ok(cur) || !pathbuf_ok(nxt) || !hits || !fc || !lv || !sb.ray_o || !sb.ray_d || !sb.contrib ||
+      !ac.result || !ac.sample_n)
+    return hipErrorInvalidValue;
+  if (ac.apply != 0 && (!rec_ok(ac.rec) || !ac.gpos || !ac.gnrm || !ac.gdif)) return hipErrorInvalidValue;
+  if (ac.variant == 1 && !rec_ok(ac.rec)) return hipErrorInvalidValue;
+  k_shade<<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, fa, cur, nxt, hits, fc, bounce, sb, lv, ac);
   return hipGetLastError();
 }
 
@@ -738,7 +821,343 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, ShadowBuf sb, Pa
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
                          uint32_t max_n, hipStream_t st) {
   if (max_n == 0) return hipSuccess;
+  if (!pathbuf_ok(nxt) || !sb.ray_o || !sb.ray_d || !sb.contrib || !fc) return hipErrorInvalidValue;
   k_shadow<<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, sb, nxt, fc, bounce);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- adaptive sampling
+// splitmix64 finaliser keyed by (seed, frame, domain, sample); the host uses
+// the same function for sample 0 of each frame.
+__device__ __forceinline__ uint64_t stream_key(uint64_t seed, uint32_t frame, uint32_t domain, uint32_t sample) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(frame + 1) + ((uint64_t)sample << 32) + ((uint64_t)domain << 60);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+// e^x: Cody-Waite reduction + degree-7 Taylor polynomial, 2^k by ldexp
+// (Cody & Waite 1980).  The host oracle evaluates the identical fma sequence.
+__device__ __forceinline__ float cpt_expf(float x) {
+  if (x < -103.0f) return 0.f;
+  if (x > 88.0f) return __builtin_huge_valf();
+  float k = rintf(x * 1.44269504f);
+  float r = fmaf(k, -0.693145751953125f, x);
+  r = fmaf(k, -1.42860676533018e-06f, r);
+  float p = 1.f / 5040.f;
+  p = fmaf(p, r, 1.f / 720.f);
+  p = fmaf(p, r, 1.f / 120.f);
+  p = fmaf(p, r, 1.f / 24.f);
+  p = fmaf(p, r, 1.f / 6.f);
+  p = fmaf(p, r, 0.5f);
+  p = fmaf(p, r, 1.f);
+  p = fmaf(p, r, 1.f);
+  return ldexpf(p, (int)k);
+}
+__device__ __forceinline__ float sqdist3(const float* a, const float* b) {
+  float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return (dx * dx + dy * dy) + dz * dz;
+}
+// Coordinate as the reference clamps it: (x + j) is unsigned, so a negative
+// coordinate wraps and lands on the high edge (cudaBilateralFilter.cu:85).
+__device__ __forceinline__ uint32_t ref_clamp_i(int v, uint32_t n) {
+  return (v < 0 || (uint32_t)v >= n) ? n - 1 : (uint32_t)v;
+}
+
+constexpr int kTile = 16;
+constexpr int kLdsRadius = 9;     // LDS tile path up to this radius, global reads beyond
+
+// Cross-bilateral over colour/position/normal(/albedo) + the weighted squared
+// residual of pt_calculateSquareError_kernel, one 16x16 tile per workgroup.
+// The (16+2r)^2 input halo is staged in LDS once; every tap then reads LDS.
+template <bool DIFF, bool LDS>
+__global__ __launch_bounds__(kTile * kTile) void k_filter_error(FilterArgs fa, const float* __restrict__ color,
+                                                                const float* __restrict__ gpos,
+                                                                const float* __restrict__ gnrm,
+                                                                const float* __restrict__ gdif,
+                                                                const float* __restrict__ gauss,
+                                                                const uint32_t* __restrict__ sample_n,
+                                                                float* __restrict__ filtered, float* __restrict__ err) {
+  extern __shared__ float sm[];
+  const int R = (int)fa.radius, TW = kTile + 2 * R, TA = TW * TW;
+  const uint32_t W = fa.width, H = fa.height;
+  const int tx = threadIdx.x, ty = threadIdx.y, tid = ty * kTile + tx;
+  const int bx = blockIdx.x * kTile, by = blockIdx.y * kTile;
+  float* sg = sm;                                  // 2R+1 spatial weights
+  float* sc = sg + 2 * kMaxFilterRadius + 2;       // 3 floats per halo pixel, per plane
+  float* sp = sc + 3 * (LDS ? TA : 0);
+  float* sn = sp + 3 * (LDS ? TA : 0);
+  float* sd = sn + 3 * (LDS ? TA : 0);
+  for (int k = tid; k < 2 * R + 1; k += kTile * kTile) sg[k] = gauss[k];
+  if (LDS) {
+    for (int k = tid; k < TA; k += kTile * kTile) {
+      int hx = k % TW, hy = k / TW;
+      size_t g = ((size_t)ref_clamp_i(by + hy - R, H) * W + ref_clamp_i(bx + hx - R, W)) * 3;
+#pragma unroll
+      for (int c = 0; c < 3; c++) {
+        sc[k * 3 + c] = color[g + c];
+        sp[k * 3 + c] = gpos[g + c];
+        sn[k * 3 + c] = gnrm[g + c];
+        if (DIFF) sd[k * 3 + c] = gdif[g + c];
+      }
+    }
+  }
+  __syncthreads();
+  const uint32_t x = bx + tx, y = by + ty;
+  if (x >= W || y >= H) return;
+  const size_t ind = (size_t)y * W + x;
+  float cc[3], cp[3], cn[3], cd[3];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    cc[c] = color[ind * 3 + c];
+    cp[c] = gpos[ind * 3 + c];
+    cn[c] = gnrm[ind * 3 + c];
+    cd[c] = DIFF ? gdif[ind * 3 + c] : 0.f;
+  }
+  float sum = 0.f, t0 = 0.f, t1 = 0.f, t2 = 0.f;
+  for (int i = -R; i <= R; i++) {
+    for (int j = -R; j <= R; j++) {
+      const float *qc, *qp, *qn, *qd = nullptr;
+      if (LDS) {
+        int k = (ty + R + i) * TW + (tx + R + j);
+        qc = sc + k * 3; qp = sp + k * 3; qn = sn + k * 3;
+        if (DIFF) qd = sd + k * 3;
+      } else {
+        size_t g = ((size_t)ref_clamp_i((int)y + i, H) * W + ref_clamp_i((int)x + j, W)) * 3;
+        qc = color + g; qp = gpos + g; qn = gnrm + g;
+        if (DIFF) qd = gdif + g;
+      }
+      float a = sqdist3(qc, cc) / fa.kc + sqdist3(qp, cp) / fa.kp;
+      a = a + sqdist3(qn, cn) / fa.kn;
+      if (DIFF) a = a + sqdist3(qd, cd) / fa.kd;
+      float f = (sg[i + R] * sg[j + R]) * cpt_expf(-a);
+      t0 = t0 + f * qc[0];
+      t1 = t1 + f * qc[1];
+      t2 = t2 + f * qc[2];
+      sum = sum + f;
+    }
+  }
+  float o0 = t0 / sum, o1 = t1 / sum, o2 = t2 / sum;
+  filtered[ind * 3] = o0;
+  filtered[ind * 3 + 1] = o1;
+  filtered[ind * 3 + 2] = o2;
+  // pt_calculateSquareError_kernel (AdapBilatAtomic.cu:1143-1156)
+  uint32_t sn_ = sample_n[ind];
+  float red = (float)fa.nd / (float)(fa.nd + sn_);
+  if (fa.nd + sn_ < sn_) red = 0.f;
+  float d0 = o0 - cc[0], d1 = o1 - cc[1], d2 = o2 - cc[2];
+  err[ind] = red * ((d0 * d0 + d1 * d1) + d2 * d2) / 3.f;
+}
+
+hipError_t launch_filter_error(const FilterArgs& fa, const float* color, const float* gpos, const float* gnrm,
+                               const float* gdif, const float* gauss, const uint32_t* sample_n, float* filtered,
+                               float* err, hipStream_t st) {
+  if (!color || !gpos || !gnrm || !gauss || !sample_n || !filtered || !err || (fa.use_diff && !gdif) ||
+      (int)fa.radius > kMaxFilterRadius || fa.width == 0 || fa.height == 0)
+    return hipErrorInvalidValue;
+  dim3 block(kTile, kTile), grid((fa.width + kTile - 1) / kTile, (fa.height + kTile - 1) / kTile);
+  const bool lds = (int)fa.radius <= kLdsRadius;
+  const int TW = kTile + 2 * (int)fa.radius;
+  size_t bytes = sizeof(float) * (2 * kMaxFilterRadius + 2);
+  if (lds) bytes += sizeof(float) * 3 * TW * TW * (fa.use_diff ? 4 : 3);
+  if (fa.use_diff) {
+    if (lds) k_filter_error<true, true><<<grid, block, bytes, st>>>(fa, color, gpos, gnrm, gdif, gauss, sample_n, filtered, err);
+    else k_filter_error<true, false><<<grid, block, bytes, st>>>(fa, color, gpos, gnrm, gdif, gauss, sample_n, filtered, err);
+  } else {
+    if (lds) k_filter_error<false, true><<<grid, block, bytes, st>>>(fa, color, gpos, gnrm, gdif, gauss, sample_n, filtered, err);
+    else k_filter_error<false, false><<<grid, block, bytes, st>>>(fa, color, gpos, gnrm, gdif, gauss, sample_n, filtered, err);
+  }
+  return hipGetLastError();
+}
+
+// Fixed-order double sum: pairwise tree inside 256-element blocks, then the
+// block sums in order (stands in for thrust::reduce, AdapBilatAtomic.cu:1444).
+__global__ __launch_bounds__(kBlock) void k_err_block_sums(const float* __restrict__ e, uint32_t n,
+                                                           double* __restrict__ bs) {
+  __shared__ double s[kBlock];
+  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  s[threadIdx.x] = i < n ? (double)e[i] : 0.0;
+  __syncthreads();
+  for (int stp = kBlock / 2; stp >= 1; stp >>= 1) {
+    if ((int)threadIdx.x < stp) s[threadIdx.x] += s[threadIdx.x + stp];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) bs[blockIdx.x] = s[0];
+}
+__global__ void k_err_final(const double* __restrict__ bs, uint32_t nb, float* __restrict__ sum) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) {
+    double t = 0.0;
+    for (uint32_t b = 0; b < nb; b++) t += bs[b];
+    *sum = (float)t;
+  }
+}
+hipError_t launch_error_sum(const float* err, uint32_t n, double* block_sums, float* sum_out, hipStream_t st) {
+  uint32_t nb = (n + kBlock - 1) / kBlock;
+  k_err_block_sums<<<nb, kBlock, 0, st>>>(err, n, block_sums);
+  k_err_final<<<1, 64, 0, st>>>(block_sums, nb, sum_out);
+  return hipGetLastError();
+}
+
+// pt_genAdapPathQueue_kernel's per-pixel count (AdapBilatAtomic.cu:830-831).
+__device__ __forceinline__ uint32_t prop_count(float e, float sum, uint32_t gen) {
+  float r = (float)gen * (e / sum);
+  if (!(r >= 0.f) || !(r < 4294967040.f)) return 0u;
+  return (uint32_t)r;
+}
+
+constexpr int kScanPer = 4;                       // elements per thread in the scan
+constexpr int kScanBlock = kBlock * kScanPer;     // 1024 elements per workgroup
+
+// Counts and workgroup-local exclusive offsets.
+__global__ __launch_bounds__(kBlock) void k_scan_local(const float* __restrict__ err, const float* __restrict__ sum,
+                                                       uint32_t n, uint32_t gen, uint32_t* __restrict__ counts,
+                                                       uint32_t* __restrict__ offsets, uint32_t* __restrict__ btot) {
+  __shared__ uint32_t s[kBlock];
+  const float sm = *sum;
+  const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
+  uint32_t c[kScanPer], tsum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; k++) {
+    uint32_t i = base + k;
+    c[k] = i < n ? prop_count(err[i], sm, gen) : 0u;
+    tsum += c[k];
+  }
+  s[threadIdx.x] = tsum;
+  __syncthreads();
+  for (int o = 1; o < kBlock; o <<= 1) {          // Hillis-Steele inclusive scan of thread sums
+    uint32_t v = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - tsum;
+#pragma unroll
+  for (int k = 0; k < kScanPer; k++) {
+    uint32_t i = base + k;
+    if (i < n) { counts[i] = c[k]; offsets[i] = run; }
+    run += c[k];
+  }
+  if (threadIdx.x == kBlock - 1) btot[blockIdx.x] = s[kBlock - 1];
+}
+// Exclusive scan of the workgroup totals (one workgroup, chunks of 256).
+__global__ __launch_bounds__(kBlock) void k_scan_totals(uint32_t* __restrict__ btot, uint32_t nb,
+                                                        uint32_t* __restrict__ total) {
+  __shared__ uint32_t s[kBlock];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t c0 = 0; c0 < nb; c0 += kBlock) {
+    uint32_t i = c0 + threadIdx.x;
+    uint32_t v = i < nb ? btot[i] : 0u;
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int o = 1; o < kBlock; o <<= 1) {
+      uint32_t w = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0u;
+      __syncthreads();
+      s[threadIdx.x] += w;
+      __syncthreads();
+    }
+    if (i < nb) btot[i] = carry + s[threadIdx.x] - v;
+    __syncthreads();
+    if (threadIdx.x == 0) carry += s[kBlock - 1];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *total = carry;
+}
+__global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* __restrict__ offsets, const uint32_t* __restrict__ btot,
+                                                     uint32_t n) {
+  uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) offsets[i] += btot[i / kScanBlock];
+}
+hipError_t launch_alloc_scan(const float* err, const float* sum, uint32_t n, uint32_t gen, uint32_t* counts,
+                             uint32_t* offsets, uint32_t* block_tot, uint32_t* total, uint32_t cap, hipStream_t st) {
+  (void)cap;
+  uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+  k_scan_local<<<nb, kBlock, 0, st>>>(err, sum, n, gen, counts, offsets, block_tot);
+  k_scan_totals<<<1, kBlock, 0, st>>>(block_tot, nb, total);
+  k_scan_add<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(offsets, block_tot, n);
+  return hipGetLastError();
+}
+
+// One eye path per queue slot: the pixel is the last one whose offset is
+// <= slot (binary search), the sample index is slot - offset.
+__global__ __launch_bounds__(kBlock) void k_gen_adaptive(FrameArgs fa, const uint32_t* __restrict__ offsets,
+                                                         const uint32_t* __restrict__ counts,
+                                                         const uint32_t* __restrict__ total, uint32_t cap,
+                                                         PathBuf pb, SampleRec rec) {
+  const uint32_t q = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t t = *total;
+  if (t > cap) t = cap;
+  if (q >= t) return;
+  const uint32_t n = fa.width * fa.height;
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (offsets[mid] <= q) lo = mid; else hi = mid;
+  }
+  const uint32_t k = q - offsets[lo];
+  (void)counts;
+  write_eye_path(fa, pb, q, lo, q, stream_key(fa.seed, fa.frame, 1u, k));
+  zero_rec(rec, q);
+}
+hipError_t launch_gen_adaptive(const FrameArgs& fa, const uint32_t* offsets, const uint32_t* counts,
+                               const uint32_t* total, uint32_t cap, PathBuf pb, SampleRec rec, hipStream_t st) {
+  if (!pathbuf_ok(pb) || !rec_ok(rec) || !offsets || !total) return hipErrorInvalidValue;
+  k_gen_adaptive<<<(cap + kBlock - 1) / kBlock, kBlock, 0, st>>>(fa, offsets, counts, total, cap, pb, rec);
+  return hipGetLastError();
+}
+
+// pt_atomicApplyPathQueueResult_kernel + pt_accumTempResultToResult_kernel
+// (AdapBilatAtomic.cu:1041-1141) with the per-pixel sums taken over the
+// pixel's samples in queue order instead of float atomics.
+__global__ __launch_bounds__(kBlock) void k_accum_adaptive(const uint32_t* __restrict__ offsets,
+                                                           const uint32_t* __restrict__ counts, uint32_t n,
+                                                           uint32_t cap, SampleRec rec, float* __restrict__ result,
+                                                           uint32_t* __restrict__ sample_n, float* __restrict__ gpos,
+                                                           float* __restrict__ gnrm, float* __restrict__ gdif) {
+  const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
+  if (pix >= n) return;
+  uint32_t q0 = offsets[pix], q1 = q0 + counts[pix];
+  if (q1 > cap) q1 = cap;
+  f3 ts = mk3(0.f, 0.f, 0.f), tp = ts, tn = ts, td = ts;
+  uint32_t tN = 0;
+  for (uint32_t q = q0; q < q1; q++) {
+    float4 a = rec.acc[q];
+    uint32_t N = __float_as_uint(a.w);
+    f3 acc = ld3(a);
+    if (N == 0 || isinf3(acc)) continue;
+    float nf = (float)N;
+    ts = ts + acc;
+    tN += N;
+    tp = tp + ld3(rec.gpos[q]) * nf;
+    tn = tn + ld3(rec.gnrm[q]) * nf;
+    td = td + ld3(rec.gdif[q]) * nf;
+  }
+  uint32_t nold = sample_n[pix], nn = tN + nold;
+  if (nn > nold) {
+    float ri = 1.f / (float)nn, oi = (float)nold * ri;
+    float* r = result + (size_t)pix * 3;
+    r[0] = fmaxf(ri * ts.x + oi * r[0], 0.f);
+    r[1] = fmaxf(ri * ts.y + oi * r[1], 0.f);
+    r[2] = fmaxf(ri * ts.z + oi * r[2], 0.f);
+    sample_n[pix] = nold + tN;
+    f3 vals[3] = {tp, tn, td};
+    float* dst[3] = {gpos + (size_t)pix * 3, gnrm + (size_t)pix * 3, gdif + (size_t)pix * 3};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      dst[k][0] = ri * vals[k].x + oi * dst[k][0];
+      dst[k][1] = ri * vals[k].y + oi * dst[k][1];
+      dst[k][2] = ri * vals[k].z + oi * dst[k][2];
+    }
+  }
+}
+hipError_t launch_accum_adaptive(const uint32_t* offsets, const uint32_t* counts, uint32_t n, uint32_t cap,
+                                 SampleRec rec, float* result, uint32_t* sample_n, float* gpos, float* gnrm,
+                                 float* gdif, hipStream_t st) {
+  if (!rec_ok(rec) || !offsets || !counts || !result || !sample_n || !gpos || !gnrm || !gdif)
+    return hipErrorInvalidValue;
+  k_accum_adaptive<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(offsets, counts, n, cap, rec, result, sample_n, gpos,
+                                                                  gnrm, gdif);
   return hipGetLastError();
 }
 

@@ -129,6 +129,18 @@ int cpt_trace_closest_device(cpt_scene* s, const cpt_ray* d_rays, uint32_t n, ui
 int cpt_trace_occluded_device(cpt_scene* s, const cpt_ray* d_rays, uint32_t n, uint32_t cull, uint32_t* d_occluded,
                               void* stream);
 
+/* Cross-bilateral filter over device images plus the weighted squared
+ * residual: cudaBilateralFilter::updateGaussian + bilaterial_posnorm_kernel /
+ * bilaterial_posnormemit_kernel (src/cudaBilateralFilter.cu:5-142) and
+ * pt_calculateSquareError_kernel (src/cudaRTBDPTStreamAdapBilatAtomic.cu:1143-1156).
+ * Images are width*height x 3 floats; d_diff may be NULL (no albedo term).
+ * The spatial Gaussian uses sigma_color, as the reference's callers pass it. */
+int cpt_filter_cross_bilateral_device(const float* d_color, const float* d_pos, const float* d_norm,
+                                      const float* d_diff, uint32_t width, uint32_t height, float sigma_color,
+                                      float sigma_pos, float sigma_norm, float sigma_diff, uint32_t radius,
+                                      const uint32_t* d_sample_n, uint32_t nd, float* d_filtered, float* d_err,
+                                      void* stream);
+
 /* Node visits and triangle tests of every traversal since the last reset. */
 int cpt_trace_counters(cpt_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* rays, int reset);
 
@@ -172,7 +184,8 @@ enum {
   CPT_BUF_GBUF_NORM = 3,  /* float x3: first-hit normal */
   CPT_BUF_GBUF_DIFF = 4,  /* float x3: first-hit albedo */
   CPT_BUF_FILTERED = 5,   /* float x3: cross-bilateral output */
-  CPT_BUF_ERROR = 6       /* float x1: per-pixel error estimate */
+  CPT_BUF_ERROR = 6,      /* float x1: per-pixel error estimate */
+  CPT_BUF_SAMPLE_COUNTS = 7  /* uint32: samples drawn per pixel in the last adaptive frame */
 };
 int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out);
 
@@ -183,6 +196,7 @@ typedef struct cpt_stats {
   uint64_t extension_rays, shadow_rays, eye_paths, frames;
   uint64_t extend_launches, extend_nodes, extend_tris;
   double extend_ms, shadow_ms, shade_ms, light_ms, total_ms;
+  double filter_ms, error_ms, gen_ms, accum_ms;   /* adaptive phases (PERFBREAKDOWN, AdapBilatAtomic.cu:1409-1536) */
 } cpt_stats;
 int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset);
 

@@ -60,6 +60,13 @@ def lib():
         L.orc_empirical_error.argtypes = [f32p, f32p, u32p, c32, f32p]
         L.orc_alloc_proportional.argtypes = [f32p, c32, ctypes.c_float, c32, u32p]
         L.orc_det_eps.restype = ctypes.c_float
+        L.orc_adaptive_state.argtypes = [vp, c32, vp, vp]
+        L.orc_bilateral_defined.argtypes = [f32p, f32p, f32p, vp, c32, c32, ctypes.c_float, ctypes.c_float,
+                                            ctypes.c_float, ctypes.c_float, c32, f32p]
+        L.orc_expf.restype = ctypes.c_float
+        L.orc_expf.argtypes = [ctypes.c_float]
+        L.orc_tree_sum.restype = ctypes.c_double
+        L.orc_tree_sum.argtypes = [f32p, c32]
         _lib = L
     return _lib
 
@@ -69,7 +76,9 @@ HIT_DTYPE = np.dtype([("tri", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
 RENDER_PARAMS_DTYPE = np.dtype([
     ("width", "<u4"), ("height", "<u4"), ("cam_pos", "<f4", 3), ("cam_dir", "<f4", 3), ("cam_up", "<f4", 3),
     ("fov", "<f4"), ("eye_depth", "<u4"), ("light_depth", "<u4"), ("light_paths", "<u4"), ("flags", "<u4"),
-    ("seed", "<u8"), ("row0", "<u4"), ("rows", "<u4")])
+    ("seed", "<u8"), ("row0", "<u4"), ("rows", "<u4"),
+    ("mode", "<u4"), ("nd", "<u4"), ("sc", "<f4"), ("sp", "<f4"), ("sn", "<f4"), ("sd", "<f4"),
+    ("radius", "<u4"), ("filter_diff", "<u4"), ("warmup", "<u4"), ("pad2", "<u4")])
 
 
 def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:
@@ -129,7 +138,10 @@ def max_threads() -> int:
 
 
 def render_params(width, height, camera, eye_depth=5, light_depth=5, light_paths=128, seed=0x5EED0001, flags=0,
-                  row0=0, rows=0):
+                  row0=0, rows=0, mode=0, nd=5, sc=50.0, sp=1.0, sn=0.25, sd=0.1, radius=5, filter_diff=0,
+                  warmup=5):
+    """Reference-estimator parameters; mode 1 = adaptive proportional allocation with the
+    reference's defaults (AdapBilatAtomic.cu:28-38)."""
     p = np.zeros((), RENDER_PARAMS_DTYPE)
     p["width"], p["height"] = width, height
     p["cam_pos"], p["cam_dir"], p["cam_up"] = camera["pos"], camera["dir"], camera["up"]
@@ -138,6 +150,8 @@ def render_params(width, height, camera, eye_depth=5, light_depth=5, light_paths
     p["seed"] = seed
     p["flags"] = flags
     p["row0"], p["rows"] = row0, rows
+    p["mode"], p["nd"], p["sc"], p["sp"], p["sn"], p["sd"] = mode, nd, sc, sp, sn, sd
+    p["radius"], p["filter_diff"], p["warmup"] = radius, filter_diff, warmup
     return p
 
 
@@ -176,6 +190,19 @@ class Renderer:
         out = np.zeros(self.W * self.H * 3, np.float64)
         lib().orc_render_result64(self.h, out)
         return out.reshape(self.H, self.W, 3)
+
+    def adaptive_state(self):
+        n = self.W * self.H
+        out = {}
+        for k, name, shape, dt in ((0, "gpos", (self.H, self.W, 3), np.float32), (1, "gnorm", (self.H, self.W, 3), np.float32),
+                                   (2, "gdiff", (self.H, self.W, 3), np.float32), (3, "filtered", (self.H, self.W, 3), np.float32),
+                                   (4, "err", (self.H, self.W), np.float32), (5, "counts", (self.H, self.W), np.uint32)):
+            a = np.zeros(shape, dt)
+            s = np.zeros(1, np.float32)
+            lib().orc_adaptive_state(self.h, k, a.ctypes.data, s.ctypes.data)
+            out[name] = a
+            out["err_sum"] = float(s[0])
+        return out
 
     def light_vertices(self):
         out = np.zeros((2048, 21), np.float32)
@@ -220,3 +247,23 @@ def alloc_proportional(err, total, gen):
     out = np.zeros(err.size, np.uint32)
     lib().orc_alloc_proportional(np.ascontiguousarray(err, np.float32).ravel(), err.size, total, gen, out)
     return out.reshape(err.shape)
+
+
+def bilateral_defined(color, pos, norm, emit, sc, sp, sn, se, radius):
+    H, W = color.shape[:2]
+    out = np.zeros((H, W, 3), np.float32)
+    e = np.ascontiguousarray(emit, np.float32) if emit is not None else None
+    lib().orc_bilateral_defined(np.ascontiguousarray(color, np.float32).ravel(),
+                                np.ascontiguousarray(pos, np.float32).ravel(),
+                                np.ascontiguousarray(norm, np.float32).ravel(), e.ctypes.data if e is not None else None,
+                                W, H, sc, sp, sn, se, radius, out.ravel())
+    return out
+
+
+def expf(x: float) -> float:
+    return float(lib().orc_expf(x))
+
+
+def tree_sum(e) -> float:
+    e = np.ascontiguousarray(e, np.float32).ravel()
+    return float(lib().orc_tree_sum(e, e.size))

@@ -487,6 +487,14 @@ struct RenderParams {
   u32 flags;      // bit 0: literal shadow endpoint dist - eps (ill-conditioned, see shadow_max)
   u64 seed;
   u32 row0, rows; // rows [row0, row0 + rows) of the frame are traced (rows = 0: all)
+  // adaptive mode (cudaRTBDPTStreamAdapBilatAtomic.cu:26-38)
+  u32 mode;        // 0 plain (cudaRTBDPTStream), 1 adaptive proportional (AdapBilatAtomic)
+  u32 nd;          // DesiredSamplingN
+  float sc, sp, sn, sd;   // filter colour / position / normal / diffuse sigmas
+  u32 radius;      // filter radius
+  u32 filter_diff; // Filter Diff Flag
+  u32 warmup;      // plain frames before adaptive sampling starts (5)
+  u32 pad2;
 };
 
 // Shadow-segment end for a connection.  The reference passes dist - eps
@@ -527,7 +535,8 @@ enum { RT_EYE = 0, RT_DIFF = 1, RT_SPEC = 2, RT_LIGHT = 3 };
 // (cudaRTBDPTStream.cu:228-329 and :402-502).
 template <class R> struct Bounce { V3<R> orig, dir, mul; int type; };
 template <class R>
-static Bounce<R> sample_bounce(Pcg32& rng, V3<R> ro, V3<R> rd, R t, const HitShade<R>& hs, V3<R> nl, int curType) {
+static Bounce<R> sample_bounce(Pcg32& rng, V3<R> ro, V3<R> rd, R t, const HitShade<R>& hs, V3<R> nl, int curType,
+                               bool refl_offset = false) {
   R x0 = uniform<R>(rng), x1 = uniform<R>(rng);
   V3<R> hDir = sample_ggx(x0, x1, hs.roughness, nl);
   V3<R> ndir = rd * (R)-1;
@@ -552,6 +561,8 @@ static Bounce<R> sample_bounce(Pcg32& rng, V3<R> ro, V3<R> rd, R t, const HitSha
   }
   if (prob_rand(rng, reflProb)) {
     b.orig = ro + rd * (R)((double)t - kBiasEps); b.dir = reflDir;
+    // the adaptive files also step off along the reflection (AdapBilatAtomic.cu:584)
+    if (refl_offset) b.orig = b.orig + reflDir * (R)kBiasEps;
     R VoH = dot(ndir, hDir), NoV = dot(nl, ndir), NoH = dot(nl, hDir), NoL = dot(nl, reflDir);
     R G = vis_smith(hs.roughness, NoV, NoL);
     b.mul = (F * G * VoH) / (NoV * NoH * reflProb);
@@ -605,6 +616,10 @@ static V3<R> eye_dir(const Camera<R>& c, u32 x, u32 y, R W, R H, Pcg32& rng) {
 template <class R> struct FrameState {
   std::vector<R> result;      // W*H*3 running mean
   std::vector<u32> N;         // W*H sample counts
+  std::vector<R> gpos, gnorm, gdiff;   // W*H*3 first-hit running means (adaptive mode)
+  std::vector<float> filtered, err;    // last adaptive frame: filter output, error map
+  std::vector<u32> counts;             // last adaptive frame: samples per pixel
+  float err_sum = 0.f;
   u64 ext_rays = 0, shadow_rays = 0;
   std::vector<LightVertex<R>> lv;
   u32 lvN = 0;
@@ -684,7 +699,6 @@ static PathResult<R> eye_path(const Scene& s, const RenderParams& p, const Camer
   PathResult<R> pr; pr.accum = mk((R)0, (R)0, (R)0); pr.N = 0;
   pr.gpos = pr.gnorm = pr.gdiff = mk((R)0, (R)0, (R)0);
   pr.ext_rays = pr.shadow_rays = 0;
-  (void)variant;
   for (u32 depth = 0;; ) {
     HitR<R> h;
     pr.ext_rays++;
@@ -695,7 +709,7 @@ static PathResult<R> eye_path(const Scene& s, const RenderParams& p, const Camer
     HitShade<R> hs = fetch_hit<R>(s, h.leaf, w, h.u, h.v);
     V3<R> nl = dot(hs.norm, dir) < (R)0 ? hs.norm : hs.norm * (R)-1;
     if (depth == 0) { pr.gpos = hs.pos; pr.gnorm = nl; pr.gdiff = hs.diff; }
-    Bounce<R> b = sample_bounce<R>(rng, orig, dir, h.t, hs, nl, RT_EYE);
+    Bounce<R> b = sample_bounce<R>(rng, orig, dir, h.t, hs, nl, RT_EYE, variant == 1);
     psample = psample + vmul(hs.emissive, mul);
     V3<R> inMul = mul, inDir = dir * (R)-1;
     R contrib = length(mul) * length(b.mul);
@@ -816,6 +830,198 @@ static void bilateral(const float* color, const float* pos, const float* norm, c
         }
       }
       out[ind * 3] = (float)(t0 / sum); out[ind * 3 + 1] = (float)(t1 / sum); out[ind * 3 + 2] = (float)(t2 / sum);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- defined filter arithmetic
+// e^x by Cody-Waite argument reduction (Cody & Waite, "Software Manual for the
+// Elementary Functions", 1980) and a degree-7 Taylor polynomial on
+// |r| <= ln2/2, scaled by 2^k with ldexp.  The device kernel evaluates the same
+// fma sequence, so filter weights agree bit for bit between the two.
+static inline float cpt_expf(float x) {
+  if (x < -103.0f) return 0.f;
+  if (x > 88.0f) return std::numeric_limits<float>::infinity();
+  float k = std::nearbyint(x * 1.44269504f);
+  float r = std::fma(k, -0.693145751953125f, x);
+  r = std::fma(k, -1.42860676533018e-06f, r);
+  float p = 1.f / 5040.f;
+  p = std::fma(p, r, 1.f / 720.f);
+  p = std::fma(p, r, 1.f / 120.f);
+  p = std::fma(p, r, 1.f / 24.f);
+  p = std::fma(p, r, 1.f / 6.f);
+  p = std::fma(p, r, 0.5f);
+  p = std::fma(p, r, 1.f);
+  p = std::fma(p, r, 1.f);
+  return std::ldexp(p, (int)k);
+}
+static inline float sqdist(const float* a, const float* b) {
+  float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return (dx * dx + dy * dy) + dz * dz;
+}
+// Cross-bilateral with one exponential per tap:
+// w = (G[i] G[j]) * exp(-(|dc|^2/2sc^2 + |dp|^2/2sp^2 + |dn|^2/2sn^2 [+ |dd|^2/2sd^2])),
+// the product of the reference's per-term exponentials (cudaBilateralFilter.cu:89-92)
+// folded into one; FP64 runs keep the literal product (bilateral<double>).
+static void bilateral_defined(const float* color, const float* pos, const float* norm, const float* emit, u32 W, u32 H,
+                              float sc, float sp, float sn, float se, u32 radius, const std::vector<float>& g,
+                              float* out) {
+  const float kc = 2.f * sc * sc, kp = 2.f * sp * sp, kn = 2.f * sn * sn, ke = 2.f * se * se;
+#pragma omp parallel for schedule(static)
+  for (i32 yy = 0; yy < (i32)H; yy++) {
+    u32 y = (u32)yy;
+    for (u32 x = 0; x < W; x++) {
+      u32 ind = x + y * W;
+      float sum = 0.f, t0 = 0.f, t1 = 0.f, t2 = 0.f;
+      for (int i = -(int)radius; i <= (int)radius; i++) {
+        for (int j = -(int)radius; j <= (int)radius; j++) {
+          u32 c = ref_clamp(x, j, W) + ref_clamp(y, i, H) * W;
+          float a = sqdist(&color[c * 3], &color[ind * 3]) / kc + sqdist(&pos[c * 3], &pos[ind * 3]) / kp;
+          a = a + sqdist(&norm[c * 3], &norm[ind * 3]) / kn;
+          if (emit) a = a + sqdist(&emit[c * 3], &emit[ind * 3]) / ke;
+          float f = (g[i + radius] * g[j + radius]) * cpt_expf(-a);
+          t0 = t0 + f * color[c * 3]; t1 = t1 + f * color[c * 3 + 1]; t2 = t2 + f * color[c * 3 + 2];
+          sum = sum + f;
+        }
+      }
+      out[ind * 3] = t0 / sum; out[ind * 3 + 1] = t1 / sum; out[ind * 3 + 2] = t2 / sum;
+    }
+  }
+}
+
+// Fixed-order double sum: pairwise tree inside 256-element blocks, blocks in
+// order.  Stands in for thrust::reduce (AdapBilatAtomic.cu:1444), whose float
+// order is unspecified; the device reduction uses the same tree.
+static double tree_sum(const float* e, size_t n) {
+  double total = 0.0;
+  double s[256];
+  for (size_t b = 0; b < n; b += 256) {
+    for (int t = 0; t < 256; t++) s[t] = (b + t < n) ? (double)e[b + t] : 0.0;
+    for (int st = 128; st >= 1; st >>= 1)
+      for (int t = 0; t < st; t++) s[t] += s[t + st];
+    total += s[0];
+  }
+  return total;
+}
+
+// pt_genAdapPathQueue_kernel count (AdapBilatAtomic.cu:830-831)
+static inline u32 prop_count(float e, float sum, u32 gen) {
+  float r = (float)gen * (e / sum);
+  if (!(r >= 0.f) || !(r < 4294967040.f)) return 0;
+  return (u32)r;
+}
+
+// pt_applyPathQueueResult_kernel of the adaptive file (AdapBilatAtomic.cu:984-1039):
+// running means of radiance and of the count-weighted first-hit G-buffer.
+template <class R>
+static inline void apply_sample_gbuf(FrameState<R>& fs, u32 pix, const PathResult<R>& pr) {
+  u32& Nold = fs.N[pix];
+  u32 nn = Nold + pr.N;
+  if (pr.N == 0 || !(nn > Nold) || isinf3(pr.accum)) return;
+  R ri = (R)1 / (R)nn, oi = (R)Nold * ri;
+  R* res = &fs.result[pix * 3];
+  res[0] = std::fmax(ri * pr.accum.x + oi * res[0], (R)0);
+  res[1] = std::fmax(ri * pr.accum.y + oi * res[1], (R)0);
+  res[2] = std::fmax(ri * pr.accum.z + oi * res[2], (R)0);
+  Nold = nn;
+  R n = (R)pr.N;
+  auto upd = [&](std::vector<R>& buf, V3<R> v) {
+    V3<R> w = v * n;
+    R* q = &buf[pix * 3];
+    q[0] = ri * w.x + oi * q[0]; q[1] = ri * w.y + oi * q[1]; q[2] = ri * w.z + oi * q[2];
+  };
+  upd(fs.gpos, pr.gpos); upd(fs.gnorm, pr.gnorm); upd(fs.gdiff, pr.gdiff);
+}
+
+// One adaptive frame after warm-up (AdapBilatAtomic.cu:1407-1538).
+template <class R>
+static void adaptive_frame(const Scene& s, const RenderParams& p, const Camera<R>& cam, u32 f, FrameState<R>& fs) {
+  const u32 W = p.width, H = p.height, n = W * H;
+  std::vector<float> col(n * 3), pos(n * 3), nrm(n * 3), dif(n * 3);
+  for (size_t i = 0; i < (size_t)n * 3; i++) {
+    col[i] = (float)fs.result[i]; pos[i] = (float)fs.gpos[i]; nrm[i] = (float)fs.gnorm[i]; dif[i] = (float)fs.gdiff[i];
+  }
+  std::vector<float> g;
+  gaussian_table(p.sc, p.radius, g);                       // spatial sigma = colour sigma (:1327)
+  fs.filtered.assign(n * 3, 0.f);
+  fs.err.assign(n, 0.f);
+  fs.counts.assign(n, 0u);
+  const float* emit = p.filter_diff ? dif.data() : nullptr;
+  if (sizeof(R) == 4) {
+    bilateral_defined(col.data(), pos.data(), nrm.data(), emit, W, H, p.sc, p.sp, p.sn, p.sd, p.radius, g,
+                      fs.filtered.data());
+  } else {
+    bilateral<double>(col.data(), pos.data(), nrm.data(), emit, W, H, p.sc, p.sp, p.sn, p.sd, p.radius, g,
+                      fs.filtered.data());
+  }
+  // pt_calculateSquareError_kernel (:1143-1156)
+  for (u32 x = 0; x < n; x++) {
+    u32 sn = fs.N[x];
+    float red = (float)p.nd / (float)(p.nd + sn);
+    if (p.nd + sn < sn) red = 0;
+    const float* a = &fs.filtered[x * 3];
+    const float* b = &col[x * 3];
+    float d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+    fs.err[x] = red * ((d0 * d0 + d1 * d1) + d2 * d2) / 3.f;
+  }
+  float sum = (float)tree_sum(fs.err.data(), n);
+  fs.err_sum = sum;
+  for (u32 x = 0; x < n; x++) fs.counts[x] = prop_count(fs.err[x], sum, n);
+  u64 er = 0, sr = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : er, sr)
+  for (i32 ii = 0; ii < (i32)n; ii++) {
+    u32 pix = (u32)ii, y = pix / W, x = pix - y * W;
+    V3<R> ts = mk((R)0, (R)0, (R)0), tp = ts, tn = ts, td = ts;
+    u32 tN = 0;
+    for (u32 k = 0; k < fs.counts[pix]; k++) {
+      PathResult<R> pr = eye_path<R>(s, p, cam, fs, f, x, y, k, 1);
+      er += pr.ext_rays; sr += pr.shadow_rays;
+      if (pr.N == 0 || isinf3(pr.accum)) continue;        // pt_atomicApplyPathQueueResult_kernel (:1056-1061)
+      R nf = (R)pr.N;
+      ts = ts + pr.accum; tN += pr.N;
+      tp = tp + pr.gpos * nf; tn = tn + pr.gnorm * nf; td = td + pr.gdiff * nf;
+    }
+    // pt_accumTempResultToResult_kernel (:1117-1140)
+    u32 Nold = fs.N[pix], nn = tN + Nold;
+    if (nn > Nold) {
+      R ri = (R)1 / (R)nn, oi = (R)Nold * ri;
+      R* res = &fs.result[pix * 3];
+      res[0] = std::fmax(ri * ts.x + oi * res[0], (R)0);
+      res[1] = std::fmax(ri * ts.y + oi * res[1], (R)0);
+      res[2] = std::fmax(ri * ts.z + oi * res[2], (R)0);
+      fs.N[pix] = Nold + tN;
+      auto upd = [&](std::vector<R>& buf, V3<R> v) {
+        R* q = &buf[pix * 3];
+        q[0] = ri * v.x + oi * q[0]; q[1] = ri * v.y + oi * q[1]; q[2] = ri * v.z + oi * q[2];
+      };
+      upd(fs.gpos, tp); upd(fs.gnorm, tn); upd(fs.gdiff, td);
+    }
+  }
+  fs.ext_rays += er; fs.shadow_rays += sr;
+}
+
+template <class R>
+static void render_adaptive(const Scene& s, const RenderParams& p, u32 frame0, u32 nframes, FrameState<R>& fs) {
+  Camera<R> cam = make_camera<R>(p);
+  const u32 W = p.width, H = p.height, n = W * H;
+  if (fs.gpos.size() != (size_t)n * 3) {
+    fs.gpos.assign(n * 3, (R)0); fs.gnorm.assign(n * 3, (R)0); fs.gdiff.assign(n * 3, (R)0);
+  }
+  for (u32 f = frame0; f < frame0 + nframes; f++) {
+    if (f % 3 == 0) light_pass<R>(s, p, f, fs);
+    if (f < p.warmup) {
+      if (f == 0) std::fill(fs.N.begin(), fs.N.end(), 0u);
+      u64 er = 0, sr = 0;
+#pragma omp parallel for schedule(dynamic, 64) reduction(+ : er, sr)
+      for (i32 ii = 0; ii < (i32)n; ii++) {
+        u32 pix = (u32)ii, y = pix / W, x = pix - y * W;
+        PathResult<R> pr = eye_path<R>(s, p, cam, fs, f, x, y, 0, 1);
+        er += pr.ext_rays; sr += pr.shadow_rays;
+        apply_sample_gbuf(fs, pix, pr);
+      }
+      fs.ext_rays += er; fs.shadow_rays += sr;
+    } else {
+      adaptive_frame<R>(s, p, cam, f, fs);
     }
   }
 }
@@ -943,20 +1149,51 @@ void orc_render(void* scene, void* rh, u32 frame0, u32 nframes, float* result, u
   size_t n = (size_t)r->p.width * r->p.height;
   if (r->prec == 64) {
     std::vector<double> g; if (gbuf) g.assign(n * 9, 0.0);
-    render_frames<double>(*s, r->p, frame0, nframes, r->f64, gbuf ? g.data() : nullptr);
+    if (r->p.mode == 1) render_adaptive<double>(*s, r->p, frame0, nframes, r->f64);
+    else render_frames<double>(*s, r->p, frame0, nframes, r->f64, gbuf ? g.data() : nullptr);
     if (result) for (size_t i = 0; i < n * 3; i++) result[i] = (float)r->f64.result[i];
     if (N) std::memcpy(N, r->f64.N.data(), n * 4);
     if (gbuf) for (size_t i = 0; i < n * 9; i++) gbuf[i] = (float)g[i];
     if (lvN) *lvN = r->f64.lvN;
   } else {
     std::vector<float> g; if (gbuf) g.assign(n * 9, 0.f);
-    render_frames<float>(*s, r->p, frame0, nframes, r->f32, gbuf ? g.data() : nullptr);
+    if (r->p.mode == 1) render_adaptive<float>(*s, r->p, frame0, nframes, r->f32);
+    else render_frames<float>(*s, r->p, frame0, nframes, r->f32, gbuf ? g.data() : nullptr);
     if (result) std::memcpy(result, r->f32.result.data(), n * 12);
     if (N) std::memcpy(N, r->f32.N.data(), n * 4);
     if (gbuf) std::memcpy(gbuf, g.data(), n * 36);
     if (lvN) *lvN = r->f32.lvN;
   }
 }
+
+// Adaptive-mode state: which = 0 G-buffer pos, 1 norm, 2 diff (W*H*3 floats),
+// 3 filtered (W*H*3), 4 error map (W*H), 5 per-pixel sample counts (W*H u32).
+// Returns the error sum of the last adaptive frame through *sum when non-null.
+void orc_adaptive_state(void* rh, u32 which, void* out, float* sum) {
+  OrcRenderer* r = (OrcRenderer*)rh;
+  auto dump = [&](auto& fs) {
+    size_t n = (size_t)r->p.width * r->p.height;
+    float* o = (float*)out;
+    auto cp = [&](auto& v) { for (size_t i = 0; i < v.size(); i++) o[i] = (float)v[i]; };
+    if (which == 0) cp(fs.gpos);
+    else if (which == 1) cp(fs.gnorm);
+    else if (which == 2) cp(fs.gdiff);
+    else if (which == 3) { if (fs.filtered.size()) std::memcpy(out, fs.filtered.data(), n * 12); }
+    else if (which == 4) { if (fs.err.size()) std::memcpy(out, fs.err.data(), n * 4); }
+    else if (which == 5) { if (fs.counts.size()) std::memcpy(out, fs.counts.data(), n * 4); }
+    if (sum) *sum = fs.err_sum;
+  };
+  if (r->prec == 64) dump(r->f64); else dump(r->f32);
+}
+
+// Defined cross-bilateral (FP32, one exponential per tap) for tests.
+void orc_bilateral_defined(const float* color, const float* pos, const float* norm, const float* emit, u32 W, u32 H,
+                           float sc, float sp, float sn, float se, u32 radius, float* out) {
+  std::vector<float> g; gaussian_table(sc, radius, g);
+  bilateral_defined(color, pos, norm, emit, W, H, sc, sp, sn, se, radius, g, out);
+}
+float orc_expf(float x) { return cpt_expf(x); }
+double orc_tree_sum(const float* e, u32 n) { return tree_sum(e, n); }
 
 // Double-precision result of the running mean (for the FP64 gate).
 void orc_render_counts(void* rh, u64* out) {
