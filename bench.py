@@ -18,6 +18,7 @@ python bench.py [--gpus 1] [--steps K] [--warmup W]
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import platform
@@ -65,6 +66,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--frames", type=int, default=SPP)
+    ap.add_argument("--in-flight", type=int, default=3, help="frames traced as one wavefront (multiple of 3)")
+    ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
+                    help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (one-GPU build; "
@@ -74,12 +78,21 @@ def main():
     import cudapt_amd as ca
     from cudapt_amd import scenes
 
+    if args.copy_calib:
+        nbytes = args.copy_calib << 20
+        src, dst = ca.DeviceBuffer(nbytes), ca.DeviceBuffer(nbytes)
+        ca._check(ca.lib().cpt_memcpy_dtod(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes),
+                  "cpt_memcpy_dtod")
+        ca.synchronize()
+        log(f"calibration copy: {nbytes} bytes device to device")
+        del src, dst
     t0 = time.perf_counter()
     sc = scenes.sphere_grid()
     S = ca.Scene(sc)
     info = S.bvh4_info()
     log(f"scene: {len(sc['idx'])} triangles, bvh4 {info}, built+uploaded in {time.perf_counter() - t0:.1f} s")
     R = ca.Renderer(WIDTH, HEIGHT, eye_depth=EYE_DEPTH, light_depth=LIGHT_DEPTH)
+    R.set_attrib("Frames In Flight", args.in_flight)
     cam = sc["camera"]
     for w in range(args.warmup):
         R.render(S, cam, frames=args.frames, result=False)
@@ -120,7 +133,7 @@ def main():
         "config": {"workload": "streaming BDPT, sphere grid 262,144 tris + 2 sphere lights, 1280x720, "
                                "depth 8, 64 spp per step, light-vertex cache every 3rd frame",
                    "triangles": int(len(sc["idx"])), "width": WIDTH, "height": HEIGHT, "spp": args.frames,
-                   "eye_depth": EYE_DEPTH, "light_depth": LIGHT_DEPTH},
+                   "eye_depth": EYE_DEPTH, "light_depth": LIGHT_DEPTH, "frames_in_flight": args.in_flight},
         "samples_per_s": round(st["eye_paths"] / wall, 1),
         "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},
         "kernel_ms": {"extend": round(st["extend_ms"], 3), "shade": round(st["shade_ms"], 3),

@@ -53,6 +53,7 @@ def lib():
             "cpt_device_free": [vp],
             "cpt_memcpy_htod": [vp, vp, ctypes.c_uint64],
             "cpt_memcpy_dtoh": [vp, vp, ctypes.c_uint64],
+            "cpt_memcpy_dtod": [vp, vp, ctypes.c_uint64],
             "cpt_device_synchronize": [],
             "cpt_scene_create": [vp],
             "cpt_scene_set_mesh": [vp, vp, vp, vp, c32, vp, vp, c32],

@@ -61,16 +61,7 @@ struct DevArray {
   }
 };
 
-// splitmix64 finaliser (Steele, Lea, Flood 2014) keyed by (seed, frame,
-// domain, sample): the per-frame stream key that takes the place of
-// curand_init(WangHash(frameN) + index, 0, 0) (src/cudaRTBDPTStream.cu:552,633).
-uint64_t stream_key(uint64_t seed, uint32_t frame, uint32_t domain, uint32_t sample) {
-  uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(frame + 1) + ((uint64_t)sample << 32) + ((uint64_t)domain << 60);
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  return z ^ (z >> 31);
-}
-enum { DOMAIN_EYE = 1, DOMAIN_LIGHT = 2 };
+
 
 }  // namespace
 
@@ -171,6 +162,11 @@ int cpt_memcpy_htod(void* dst, const void* src, uint64_t bytes) {
 int cpt_memcpy_dtoh(void* dst, const void* src, uint64_t bytes) {
   if (bytes && (!dst || !src)) return fail(CPT_E_ARG, "cpt_memcpy_dtoh: null pointer");
   if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return CPT_OK;
+}
+int cpt_memcpy_dtod(void* dst, const void* src, uint64_t bytes) {
+  if (bytes && (!dst || !src)) return fail(CPT_E_ARG, "cpt_memcpy_dtod: null pointer");
+  if (bytes) HIP_TRY(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice));
   return CPT_OK;
 }
 int cpt_device_synchronize(void) {
@@ -437,6 +433,8 @@ const char* const kLightPaths = "Light Paths";
 const char* const kSeed = "Seed";
 const char* const kLiteralShadow = "Literal Shadow Endpoint";
 const char* const kWarmup = "Warm-up Frames";
+const char* const kInFlight = "Frames In Flight";
+constexpr u32 kMaxInFlight = 90;   // 30 three-frame light-cache groups + partial ones at either end
 
 }  // namespace
 
@@ -455,6 +453,7 @@ struct cpt_renderer {
   DevArray<u32> sample_n;
   bool allocated = false;
   u32 frame = 0;
+  bool started = false;
   bool have_cam = false;
   float cam[9] = {0};
   const cpt_scene* scene = nullptr;
@@ -490,6 +489,8 @@ struct cpt_renderer {
   float gauss_sigma = -1.f;
   u32 gauss_radius = 0;
   u32 cap = 0;
+  long long slot_group[kLightSlots];   // three-frame group whose light cache each slot holds (-1: none)
+  void forget_light_caches() { for (auto& g : slot_group) g = -1; }
   SampleRec rec() {
     SampleRec r;
     r.acc = rec_acc.p; r.gpos = rec_pos.p; r.gnrm = rec_nrm.p; r.gdif = rec_dif.p;
@@ -502,9 +503,12 @@ namespace {
 int renderer_alloc(cpt_renderer* r) {
   size_t n = (size_t)r->W * r->H;
   // adaptive frames draw floor(W*H*e_i/sum e) samples: at most W*H plus float
-  // rounding of the ratios, bounded by the margin (and clamped on the device)
-  size_t cap = r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL ? n + 1024 : n;
+  // rounding of the ratios, bounded by the margin (and clamped on the device);
+  // plain frames trace "Frames In Flight" frames as one wavefront
+  size_t cap = r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL ? n + 1024 : n * (size_t)r->attr(kInFlight);
+  if (cap > 0xFFFFFFFFull / 2) return fail(CPT_E_LIMIT, "render: %zu path slots exceed 2^31", cap);
   r->cap = (u32)cap;
+  r->forget_light_caches();
   HIP_TRY(r->A.alloc(cap));
   HIP_TRY(r->B.alloc(cap));
   HIP_TRY(r->hits.alloc(cap));
@@ -524,7 +528,6 @@ int renderer_alloc(cpt_renderer* r) {
     HIP_TRY(r->block_tot.alloc(n / 1024 + 2));
     HIP_TRY(r->total.alloc(1));
     HIP_TRY(r->block_sums.alloc(n / 256 + 2));
-    HIP_TRY(r->rec_acc.alloc(cap));
     HIP_TRY(r->rec_pos.alloc(cap));
     HIP_TRY(r->rec_nrm.alloc(cap));
     HIP_TRY(r->rec_dif.alloc(cap));
@@ -533,6 +536,7 @@ int renderer_alloc(cpt_renderer* r) {
     HIP_TRY(hipMemsetAsync(r->err.p, 0, sizeof(float) * n, r->stream));
     HIP_TRY(hipMemsetAsync(r->counts.p, 0, sizeof(u32) * n, r->stream));
   }
+  HIP_TRY(r->rec_acc.alloc(cap));
   HIP_TRY(r->fc.alloc(1));
   HIP_TRY(r->stats.alloc(1));
   HIP_TRY(r->result.alloc(n * 3));
@@ -547,9 +551,28 @@ int renderer_alloc(cpt_renderer* r) {
 }
 
 int lv_alloc(cpt_renderer* r) {
-  size_t need = (size_t)r->attr(kLightPaths) * ((size_t)r->attr(kLightDepth) + 1);
+  size_t need = (size_t)r->attr(kLightPaths) * ((size_t)r->attr(kLightDepth) + 1) * kLightSlots;
+  if (need > r->lv.n) r->forget_light_caches();
   HIP_TRY(r->lv.alloc(need));
   return CPT_OK;
+}
+
+// Light caches for frames [f0, f0 + nf): one per three-frame group, in slot
+// group % kLightSlots, rebuilt only when the slot holds another group.  A
+// cache depends only on (seed, group), so a rebuild reproduces it exactly.
+LightJobs light_jobs(cpt_renderer* r, u32 f0, u32 nf) {
+  LightJobs j;
+  std::memset(&j, 0, sizeof j);
+  for (u32 g = f0 / 3; g <= (f0 + nf - 1) / 3; g++) {
+    u32 slot = g % kLightSlots;
+    if (r->slot_group[slot] != (long long)g) {
+      j.slot[j.n] = slot;
+      j.frame[j.n] = 3 * g;
+      j.n++;
+      r->slot_group[slot] = g;
+    }
+  }
+  return j;
 }
 
 hipError_t timed_begin(cpt_renderer* r, int kind, hipEvent_t* a) {
@@ -624,9 +647,10 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   DevScene sc = s->dev();
   const u32 n = r->W * r->H;
   hipEvent_t ev = nullptr;
-  if (fa.frame % 3 == 0) {
+  LightJobs jobs = light_jobs(r, fa.frame0, 1);
+  if (jobs.n) {
     HIP_TRY(timed_begin(r, 3, &ev));
-    HIP_TRY(launch_light_pass(sc, fa, r->lv.p, r->fc.p, r->stream));
+    HIP_TRY(launch_light_pass(sc, fa, jobs, r->lv.p, r->fc.p, r->stream));
     HIP_TRY(timed_end(r, 3, ev));
   }
   Accum ac;
@@ -634,7 +658,7 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   ac.gpos = r->gpos.p; ac.gnrm = r->gnrm.p; ac.gdif = r->gdif.p;
   ac.rec = r->rec();
   ac.variant = 1;
-  if (fa.frame < (u32)r->attr(kWarmup)) {          // plain 1 spp frames with G-buffer (:1372-1406)
+  if (fa.frame0 < (u32)r->attr(kWarmup)) {         // plain 1 spp frames with G-buffer (:1372-1406)
     ac.apply = 1;
     HIP_TRY(launch_frame_begin(r->fc.p, n, r->stream));
     HIP_TRY(launch_gen_eye_rec(fa, r->A.view(), ac.rec, r->stream));
@@ -671,18 +695,22 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
                                   r->gnrm.p, r->gdif.p, r->stream));
     HIP_TRY(timed_end(r, 8, ev));
   }
-  HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, n, fa.eye_depth, r->stream));
+  HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, 1, fa.eye_depth, r->stream));
   return CPT_OK;
 }
 
-int render_one_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
+// fa.nframes frames of cudaRTBDPTStream::Render traced as one wavefront
+// (src/cudaRTBDPTStream.cu:874-946), then applied to the running mean in
+// frame order: identical to fa.nframes sequential calls.
+int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   if (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL) return render_adaptive_frame(r, s, fa);
   DevScene sc = s->dev();
-  const u32 n = r->W * r->H;
+  const u32 n = r->W * r->H * fa.nframes;
   hipEvent_t ev = nullptr;
-  if (fa.frame % 3 == 0) {                        // light paths every third frame (:875)
+  LightJobs jobs = light_jobs(r, fa.frame0, fa.nframes);
+  if (jobs.n) {
     HIP_TRY(timed_begin(r, 3, &ev));
-    HIP_TRY(launch_light_pass(sc, fa, r->lv.p, r->fc.p, r->stream));
+    HIP_TRY(launch_light_pass(sc, fa, jobs, r->lv.p, r->fc.p, r->stream));
     HIP_TRY(timed_end(r, 3, ev));
   }
   HIP_TRY(launch_frame_begin(r->fc.p, n, r->stream));
@@ -691,11 +719,15 @@ int render_one_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   std::memset(&ac, 0, sizeof ac);
   ac.result = r->result.p;
   ac.sample_n = r->sample_n.p;
-  ac.apply = 0;
+  ac.rec.acc = r->rec_acc.p;
+  ac.apply = 2;
   ac.variant = 0;
   int rc = bounce_loop(r, sc, fa, n, ac);
   if (rc) return rc;
-  HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, n, fa.eye_depth, r->stream));
+  HIP_TRY(timed_begin(r, 8, &ev));
+  HIP_TRY(launch_apply_frames(fa, r->rec_acc.p, r->result.p, r->sample_n.p, r->stream));
+  HIP_TRY(timed_end(r, 8, ev));
+  HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, fa.nframes, fa.eye_depth, r->stream));
   return CPT_OK;
 }
 
@@ -717,7 +749,10 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   if (eye_depth < 1 || eye_depth > (u32)kMaxDepth) return fail(CPT_E_LIMIT, "render: eye depth %u not in 1..15", eye_depth);
   if (light_depth < 1 || light_depth > 64) return fail(CPT_E_LIMIT, "render: light depth %u not in 1..64", light_depth);
   if (light_paths < 1 || light_paths > (u32)kBlock) return fail(CPT_E_LIMIT, "render: light paths %u not in 1..256", light_paths);
-  if (!r->allocated) {
+  u32 in_flight = r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL ? 1u : (u32)r->attr(kInFlight);
+  if (in_flight < 1 || in_flight > kMaxInFlight)
+    return fail(CPT_E_LIMIT, "render: frames in flight %u not in 1..%u", in_flight, kMaxInFlight);
+  if (!r->allocated || (size_t)r->cap < (size_t)r->W * r->H * in_flight) {
     int rc = renderer_alloc(r);
     if (rc) return rc;
   }
@@ -726,6 +761,7 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   float cam[9] = {pos[0], pos[1], pos[2], dir[0], dir[1], dir[2], up[0], up[1], up[2]};
   bool reset = !r->have_cam || std::memcmp(cam, r->cam, sizeof cam) != 0 || r->scene != s ||
                r->scene_mesh_version != s->mesh_version || r->scene_mat_version != s->mat_version;
+  if (reset) r->forget_light_caches();
   std::memcpy(r->cam, cam, sizeof cam);
   r->have_cam = true;
   r->scene = s;
@@ -746,14 +782,18 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   fa.seed = seed;
   hipEvent_t ev_all = nullptr;
   HIP_TRY(timed_begin(r, 4, &ev_all));
-  for (u32 f = 0; f < nframes; f++) {
-    if (reset && f == 0) r->frame = 0;
-    else r->frame++;
-    fa.frame = r->frame;
-    fa.eye_key = stream_key(seed, fa.frame, DOMAIN_EYE, 0);
-    fa.light_key = stream_key(seed, fa.frame, DOMAIN_LIGHT, 0);
-    rc = render_one_frame(r, s, fa);
+  fa.lv_stride = light_paths * (light_depth + 1);
+  u32 next = (reset || !r->started) ? 0u : r->frame + 1;   // frame counter (:826-828)
+  for (u32 done = 0; done < nframes;) {
+    u32 k = std::min(in_flight, nframes - done);
+    fa.frame0 = next;
+    fa.nframes = k;
+    rc = render_batch(r, s, fa);
     if (rc) return rc;
+    r->frame = next + k - 1;
+    r->started = true;
+    next += k;
+    done += k;
   }
   HIP_TRY(timed_end(r, 4, ev_all));
   if (result)
@@ -793,6 +833,7 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   r->attribs.push_back({kSeed, 0, (double)0x5EED0001u});
   r->attribs.push_back({kLiteralShadow, 2, 0});
   if (mode == CPT_MODE_BDPT_ADAPTIVE_PLL) r->attribs.push_back({kWarmup, 0, 5});
+  else r->attribs.push_back({kInFlight, 0, 3});
   int rc = ensure_stream(r->stream);
   if (rc) { delete r; return rc; }
   *out = r;

@@ -37,7 +37,7 @@ struct PathBuf {
   float4* ray_d;     // direction xyz, w = ray epsilon (1e-4)
   uint32_t* pix;     // pixel index
   uint32_t* sid;     // sample record index (adaptive mode; pixel index otherwise)
-  uint32_t* depth;   // bounce count
+  uint32_t* depth;   // bounce count (low 8 bits) | frame offset within the batch << 8
   float4* mul;       // path throughput (pathOutMulTerm)
   float4* smp;       // emitted radiance gathered along the path (pathSample)
   float4* acc;       // xyz accumulated sample, w = sample count bits
@@ -72,20 +72,30 @@ struct FrameArgs {
   uint32_t width, height;
   float cam_pos[3], cam_dir[3], cam_up[3], cam_right[3];
   float fov;
-  uint32_t frame;        // frame counter since the last reset
-  uint64_t eye_key;      // PCG32 initstate for eye paths of this frame
-  uint64_t light_key;    // PCG32 initstate for light paths
   uint32_t eye_depth, light_depth, light_paths;
   uint32_t flags;        // bit 0: literal shadow endpoint dist - eps
-  uint64_t seed;         // stream seed: per-sample keys of adaptive frames are derived on the device
+  uint64_t seed;         // stream seed: per-path keys are derived on the device
+  uint32_t frame0;       // first frame of the batch (frame == frame0 for single-frame work)
+  uint32_t nframes;      // frames traced together as one wavefront
+  uint32_t lv_stride;    // light vertices per cache slot
+  uint32_t pad0;
+};
+
+// Light-vertex cache slots to (re)build: slot s holds the cache of the
+// three-frame group starting at frame[i] (cudaRTBDPTStream.cu:875).
+constexpr int kLightSlots = 32;
+struct LightJobs {
+  uint32_t n;
+  uint32_t slot[kLightSlots];
+  uint32_t frame[kLightSlots];
 };
 
 // Device counters of one frame (zeroed by frame_begin).
 struct FrameCounters {
-  uint32_t active[16];   // live paths entering bounce b
-  uint32_t shadow[16];   // shadow rays emitted at bounce b
-  uint32_t light_n;      // light-vertex cache size
-  uint32_t pad[31];
+  uint32_t active[16];            // live paths entering bounce b
+  uint32_t shadow[16];            // shadow rays emitted at bounce b
+  uint32_t light_n[kLightSlots];  // light-vertex cache size per slot
+  uint32_t pad[16];
 };
 
 // Cumulative device statistics.
@@ -103,8 +113,13 @@ constexpr int kBlock = 256;
 hipError_t launch_trace(const DevScene& sc, const float4* rays, uint32_t n, const uint32_t* count, uint32_t cull,
                         float4* hits_out, int out_orig, int any_hit, uint32_t layout, DevStats* stats, hipStream_t st);
 hipError_t launch_frame_begin(FrameCounters* fc, uint32_t n_paths, hipStream_t st);
-hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, LightVert* lv, FrameCounters* fc, hipStream_t st);
+hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, const LightJobs& jobs, LightVert* lv,
+                             FrameCounters* fc, hipStream_t st);
+// nframes x W*H eye paths, slot = frame offset * W*H + pixel.
 hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st);
+// Running-mean update of every pixel from the batch's per-frame records, frames in order.
+hipError_t launch_apply_frames(const FrameArgs& fa, const float4* rec_acc, float* result, uint32_t* sample_n,
+                               hipStream_t st);
 hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, uint32_t max_n,
                          float4* hits, DevStats* stats, hipStream_t st);
 // apply: 0 = plain running mean at termination, 1 = running mean + G-buffer
@@ -144,6 +159,6 @@ hipError_t launch_accum_adaptive(const uint32_t* offsets, const uint32_t* counts
                                  float* gdif, hipStream_t st);
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
                          uint32_t max_n, hipStream_t st);
-hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t n_paths, uint32_t depth, hipStream_t st);
+hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes, uint32_t depth, hipStream_t st);
 
 }  // namespace cpt

@@ -44,6 +44,18 @@ struct TraceHit {
   uint32_t leaf, orig;
 };
 
+// splitmix64 finaliser (Steele, Lea, Flood 2014) keyed by (seed, frame, domain,
+// sample): the per-path PCG32 initstate that takes the place of
+// curand_init(WangHash(frameN) + index, 0, 0) (src/cudaRTBDPTStream.cu:552,633).
+// Domains: 1 eye paths, 2 light paths.
+__device__ __forceinline__ uint64_t stream_key(uint64_t seed, uint32_t frame, uint32_t domain, uint32_t sample) {
+  uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(frame + 1) + ((uint64_t)sample << 32) + ((uint64_t)domain << 60);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
+  return z ^ (z >> 31);
+}
+
+
 __device__ __forceinline__ float escape_zero(float v) {
   const float e = 1e-9f;
   if (fabsf(v) < e) v = (v > 0.f) ? v + e : v - e;
@@ -426,19 +438,18 @@ hipError_t launch_frame_begin_dev(FrameCounters* fc, const uint32_t* total, uint
   return hipGetLastError();
 }
 
-__global__ void k_frame_end(FrameCounters* fc, DevStats* stats, uint32_t n_paths, uint32_t depth) {
+__global__ void k_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes, uint32_t depth) {
   if (threadIdx.x == 0) {
     unsigned long long e = 0, s = 0;
     for (uint32_t b = 0; b < depth && b < 16; b++) { e += fc->active[b]; s += fc->shadow[b]; }
     stats->ext_rays += e;
     stats->shadow_rays += s;
     stats->eye_paths += fc->active[0];
-    (void)n_paths;
-    stats->frames += 1;
+    stats->frames += nframes;
   }
 }
-hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t n_paths, uint32_t depth, hipStream_t st) {
-  k_frame_end<<<1, 64, 0, st>>>(fc, stats, n_paths, depth);
+hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes, uint32_t depth, hipStream_t st) {
+  k_frame_end<<<1, 64, 0, st>>>(fc, stats, nframes, depth);
   return hipGetLastError();
 }
 
@@ -453,12 +464,16 @@ __device__ __forceinline__ void store_lv(LightVert& V, f3 pos, f3 norm, f3 irrad
   V.pad[0] = V.pad[1] = V.pad[2] = 0.f;
 }
 
-// Light-vertex cache, one workgroup (light_paths <= kBlock).
-__global__ __launch_bounds__(kBlock) void k_light_pass(DevScene sc, FrameArgs fa, LightVert* __restrict__ lv,
+// Light-vertex caches, one workgroup per cache slot to build (light_paths <= kBlock).
+__global__ __launch_bounds__(kBlock) void k_light_pass(DevScene sc, FrameArgs fa, LightJobs jobs,
+                                                       LightVert* __restrict__ lv_all,
                                                        FrameCounters* __restrict__ fc) {
   __shared__ uint64_t lstk[CPT_LDS_STACK * kBlock];
   __shared__ uint32_t wcnt[kBlock / 64];
   const uint32_t x = threadIdx.x, L = fa.light_paths;
+  const uint32_t slot = jobs.slot[blockIdx.x];
+  LightVert* __restrict__ lv = lv_all + (size_t)slot * fa.lv_stride;
+  const uint64_t light_key = stream_key(fa.seed, jobs.frame[blockIdx.x], 2u, 0u);
   const uint64_t inc = ((uint64_t)x << 1) | 1u;
   uint64_t st = 0;
   bool live = false;
@@ -466,11 +481,11 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(DevScene sc, FrameArgs fa
   int type = RT_LIGHT;
   uint32_t depth = 0;
   if (sc.n_light_tris == 0) {
-    if (x == 0) fc->light_n = 0;
+    if (x == 0) fc->light_n[slot] = 0;
     return;
   }
   if (x < L) {
-    pcg32_seed(st, inc, fa.light_key);
+    pcg32_seed(st, inc, light_key);
     uint32_t id = (uint32_t)(uniform01(st, inc) * (float)sc.n_light_tris);
     if (id >= sc.n_light_tris) id = sc.n_light_tris - 1;
     float lw = uniform01(st, inc), lu = uniform01(st, inc);
@@ -533,13 +548,34 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(DevScene sc, FrameArgs fa
     cur += total;
     __syncthreads();
   }
-  if (x == 0) fc->light_n = cur;
+  if (x == 0) fc->light_n[slot] = cur;
 }
-hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, LightVert* lv, FrameCounters* fc,
-                             hipStream_t st) {
-  if (!lv || !fc || fa.light_paths > (uint32_t)kBlock || (sc.n_light_tris && !sc.light_tris)) return hipErrorInvalidValue;
-  k_light_pass<<<1, kBlock, 0, st>>>(sc, fa, lv, fc);
+hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, const LightJobs& jobs, LightVert* lv,
+                             FrameCounters* fc, hipStream_t st) {
+  if (jobs.n == 0) return hipSuccess;
+  if (!lv || !fc || fa.light_paths > (uint32_t)kBlock || (sc.n_light_tris && !sc.light_tris) ||
+      jobs.n > (uint32_t)kLightSlots)
+    return hipErrorInvalidValue;
+  for (uint32_t j = 0; j < jobs.n; j++)
+    if (jobs.slot[j] >= (uint32_t)kLightSlots) return hipErrorInvalidValue;
+  k_light_pass<<<jobs.n, kBlock, 0, st>>>(sc, fa, jobs, lv, fc);
   return hipGetLastError();
+}
+
+// Running-mean update of pt_applyPathQueueResult_kernel
+// (src/cudaRTBDPTStream.cu:754-776); one path per pixel per frame.
+__device__ __forceinline__ void apply_pixel(float* result, uint32_t* sample_n, uint32_t pix, uint32_t frame, f3 acc,
+                                            uint32_t n) {
+  uint32_t nold = frame == 0 ? 0u : sample_n[pix];
+  if (frame == 0) sample_n[pix] = 0u;
+  uint32_t nn = nold + n;
+  if (!(nn > nold) || isinf3(acc)) return;
+  float ri = 1.f / (float)nn, oi = (float)nold * ri;
+  float* r = result + (size_t)pix * 3;
+  r[0] = fmaxf(ri * acc.x + oi * r[0], 0.f);
+  r[1] = fmaxf(ri * acc.y + oi * r[1], 0.f);
+  r[2] = fmaxf(ri * acc.z + oi * r[2], 0.f);
+  sample_n[pix] = nn;
 }
 
 // Eye ray of pixel (x, y) drawn from PCG32 stream `key`, pixel as sequence
@@ -576,24 +612,47 @@ __device__ __forceinline__ void zero_rec(SampleRec& rec, uint32_t q) {
   rec.acc[q] = z; rec.gpos[q] = z; rec.gnrm[q] = z; rec.gdif[q] = z;
 }
 
-// One path per pixel, slot = pixel.
+// One path per pixel and frame of the batch, slot = frame offset * W*H + pixel.
 __global__ __launch_bounds__(kBlock) void k_gen_eye(FrameArgs fa, PathBuf pb, SampleRec rec) {
+  const uint32_t wh = fa.width * fa.height;
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= fa.width * fa.height) return;
-  write_eye_path(fa, pb, i, i, i, fa.eye_key);
+  if (i >= wh * fa.nframes) return;
+  uint32_t k = i / wh, pix = i - k * wh;
+  write_eye_path(fa, pb, i, pix, i, stream_key(fa.seed, fa.frame0 + k, 1u, 0u));
+  pb.depth[i] = k << 8;
   if (rec.acc) zero_rec(rec, i);
 }
 hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st) {
   if (!pathbuf_ok(pb)) return hipErrorInvalidValue;
-  uint32_t n = fa.width * fa.height;
+  uint32_t n = fa.width * fa.height * fa.nframes;
   SampleRec none = {nullptr, nullptr, nullptr, nullptr};
   k_gen_eye<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(fa, pb, none);
   return hipGetLastError();
 }
 hipError_t launch_gen_eye_rec(const FrameArgs& fa, PathBuf pb, SampleRec rec, hipStream_t st) {
   if (!pathbuf_ok(pb) || !rec_ok(rec)) return hipErrorInvalidValue;
-  uint32_t n = fa.width * fa.height;
+  uint32_t n = fa.width * fa.height * fa.nframes;
   k_gen_eye<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(fa, pb, rec);
+  return hipGetLastError();
+}
+
+// Frames of the batch applied in order: pt_applyPathQueueResult_kernel's
+// running mean (src/cudaRTBDPTStream.cu:741-778) once per frame.
+__global__ __launch_bounds__(kBlock) void k_apply_frames(FrameArgs fa, const float4* __restrict__ rec_acc,
+                                                         float* __restrict__ result, uint32_t* __restrict__ sample_n) {
+  const uint32_t wh = fa.width * fa.height;
+  const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
+  if (pix >= wh) return;
+  for (uint32_t k = 0; k < fa.nframes; k++) {
+    float4 a = rec_acc[(size_t)k * wh + pix];
+    apply_pixel(result, sample_n, pix, fa.frame0 + k, ld3(a), __float_as_uint(a.w));
+  }
+}
+hipError_t launch_apply_frames(const FrameArgs& fa, const float4* rec_acc, float* result, uint32_t* sample_n,
+                               hipStream_t st) {
+  if (!rec_acc || !result || !sample_n) return hipErrorInvalidValue;
+  uint32_t wh = fa.width * fa.height;
+  k_apply_frames<<<(wh + kBlock - 1) / kBlock, kBlock, 0, st>>>(fa, rec_acc, result, sample_n);
   return hipGetLastError();
 }
 
@@ -604,22 +663,6 @@ hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc
   k_trace<0, false, 1><<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, pb.ray_o, pb.ray_d, 1,
                                                                             &fc->active[bounce], max_n, 0, hits, stats);
   return hipGetLastError();
-}
-
-// Running-mean update of pt_applyPathQueueResult_kernel
-// (src/cudaRTBDPTStream.cu:754-776); one path per pixel per frame.
-__device__ __forceinline__ void apply_pixel(float* result, uint32_t* sample_n, uint32_t pix, uint32_t frame, f3 acc,
-                                            uint32_t n) {
-  uint32_t nold = frame == 0 ? 0u : sample_n[pix];
-  if (frame == 0) sample_n[pix] = 0u;
-  uint32_t nn = nold + n;
-  if (!(nn > nold) || isinf3(acc)) return;
-  float ri = 1.f / (float)nn, oi = (float)nold * ri;
-  float* r = result + (size_t)pix * 3;
-  r[0] = fmaxf(ri * acc.x + oi * r[0], 0.f);
-  r[1] = fmaxf(ri * acc.y + oi * r[1], 0.f);
-  r[2] = fmaxf(ri * acc.z + oi * r[2], 0.f);
-  sample_n[pix] = nn;
 }
 
 // Same plus the count-weighted first-hit G-buffer running means of the
@@ -677,17 +720,22 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t count = fc->active[bounce];
   if (blockIdx.x * kBlock >= count) return;               // whole workgroup idle
-  const uint32_t lvN = fc->light_n;
   bool survive = false, emit = false;
   float4 o_out = make_float4(0.f, 0.f, 0.f, 0.f), d_out = o_out, mul_out = o_out, smp_out = o_out, acc_out = o_out;
   float4 so = o_out, sd = o_out, sc4 = o_out;
-  uint32_t pix = 0, sid = 0, depth = 0;
+  uint32_t pix = 0, sid = 0, depth = 0, fk = 0;
   uint64_t st = 0;
   if (i < count) {
     float4 ro = cur.ray_o[i], rd = cur.ray_d[i];
     pix = cur.pix[i];
     sid = cur.sid[i];
-    depth = cur.depth[i];
+    const uint32_t dk = cur.depth[i];
+    depth = dk & 0xFFu;
+    fk = dk >> 8;
+    const uint32_t frame = fa.frame0 + fk;
+    const uint32_t slot = (frame / 3u) % (uint32_t)kLightSlots;
+    const uint32_t lvN = fc->light_n[slot];
+    const LightVert* __restrict__ lvb = lv + (size_t)slot * fa.lv_stride;
     float4 m4 = cur.mul[i], s4 = cur.smp[i], a4 = cur.acc[i];
     st = cur.rng[i];
     const uint64_t inc = ((uint64_t)pix << 1) | 1u;
@@ -724,7 +772,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
           f3 pos = orig + nl * (float)kBiasEps;
           uint32_t li = (uint32_t)(uniform01(st, inc) * (float)lvN);
           if (li >= lvN) li = lvN - 1;
-          const LightVert& L = lv[li];
+          const LightVert& L = lvb[li];
           f3 lpos = mk3(L.pos[0], L.pos[1], L.pos[2]);
           f3 lirr = mk3(L.irrad[0], L.irrad[1], L.irrad[2]);
           f3 toL = normalize(lpos - pos);
@@ -754,8 +802,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
       }
     }
     if (term) {
-      if (ac.apply == 0) apply_pixel(ac.result, ac.sample_n, pix, fa.frame, acc, N);
-      else if (ac.apply == 1) apply_pixel_gbuf(ac, pix, fa.frame, acc, N, sid);
+      if (ac.apply == 0) apply_pixel(ac.result, ac.sample_n, pix, fa.frame0 + fk, acc, N);
+      else if (ac.apply == 1) apply_pixel_gbuf(ac, pix, fa.frame0 + fk, acc, N, sid);
       else ac.rec.acc[sid] = make_float4(acc.x, acc.y, acc.z, __uint_as_float(N));
     } else {
       survive = true;
@@ -773,7 +821,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
     nxt.ray_d[slot] = d_out;
     nxt.pix[slot] = pix;
     nxt.sid[slot] = sid;
-    nxt.depth[slot] = depth;
+    nxt.depth[slot] = depth | (fk << 8);
     nxt.mul[slot] = mul_out;
     nxt.smp[slot] = smp_out;
     nxt.acc[slot] = acc_out;
@@ -793,7 +841,8 @@ hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, Pa
   if (!pathbuf_ok(cur) || !pathbuf_ok(nxt) || !hits || !fc || !lv || !sb.ray_o || !sb.ray_d || !sb.contrib ||
       !ac.result || !ac.sample_n)
     return hipErrorInvalidValue;
-  if (ac.apply != 0 && (!rec_ok(ac.rec) || !ac.gpos || !ac.gnrm || !ac.gdif)) return hipErrorInvalidValue;
+  if (ac.apply == 1 && (!rec_ok(ac.rec) || !ac.gpos || !ac.gnrm || !ac.gdif)) return hipErrorInvalidValue;
+  if (ac.apply == 2 && !ac.rec.acc) return hipErrorInvalidValue;
   if (ac.variant == 1 && !rec_ok(ac.rec)) return hipErrorInvalidValue;
   k_shade<<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, fa, cur, nxt, hits, fc, bounce, sb, lv, ac);
   return hipGetLastError();
@@ -827,15 +876,6 @@ hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const Fr
 }
 
 // ---------------------------------------------------------------- adaptive sampling
-// splitmix64 finaliser keyed by (seed, frame, domain, sample); the host uses
-// the same function for sample 0 of each frame.
-__device__ __forceinline__ uint64_t stream_key(uint64_t seed, uint32_t frame, uint32_t domain, uint32_t sample) {
-  uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (uint64_t)(frame + 1) + ((uint64_t)sample << 32) + ((uint64_t)domain << 60);
-  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ULL;
-  z = (z ^ (z >> 27)) * 0x94d049bb133111ebULL;
-  return z ^ (z >> 31);
-}
-
 // e^x: Cody-Waite reduction + degree-7 Taylor polynomial, 2^k by ldexp
 // (Cody & Waite 1980).  The host oracle evaluates the identical fma sequence.
 __device__ __forceinline__ float cpt_expf(float x) {
@@ -1097,7 +1137,7 @@ __global__ __launch_bounds__(kBlock) void k_gen_adaptive(FrameArgs fa, const uin
   }
   const uint32_t k = q - offsets[lo];
   (void)counts;
-  write_eye_path(fa, pb, q, lo, q, stream_key(fa.seed, fa.frame, 1u, k));
+  write_eye_path(fa, pb, q, lo, q, stream_key(fa.seed, fa.frame0, 1u, k));
   zero_rec(rec, q);
 }
 hipError_t launch_gen_adaptive(const FrameArgs& fa, const uint32_t* offsets, const uint32_t* counts,

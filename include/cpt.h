@@ -67,6 +67,7 @@ int cpt_device_alloc(void** ptr, uint64_t bytes);
 int cpt_device_free(void* ptr);
 int cpt_memcpy_htod(void* dst, const void* src, uint64_t bytes);
 int cpt_memcpy_dtoh(void* dst, const void* src, uint64_t bytes);
+int cpt_memcpy_dtod(void* dst, const void* src, uint64_t bytes);
 int cpt_device_synchronize(void);
 
 /* ---- scene (RTScene, include/raytracer.h:228-290) ---------------------- */

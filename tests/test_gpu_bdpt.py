@@ -66,6 +66,22 @@ def test_batched_frames_equal_sequential(cpt, cornell_runs):
     assert np.array_equal(R.sample_counts(), R2.sample_counts())
 
 
+@pytest.mark.parametrize("in_flight", [1, 3, 6])
+def test_frames_in_flight_bit_identical(cpt, cornell_runs, in_flight):
+    """Frames traced as one wavefront equal frames traced one at a time."""
+    sc, S = cornell_runs["sc"], cornell_runs["S"]
+    ref = cpt.Renderer(64, 48)
+    ref.set_attrib("Frames In Flight", 1)
+    for _ in range(8):
+        a = ref.render(S, sc["camera"], frames=1)
+    R = cpt.Renderer(64, 48)
+    R.set_attrib("Frames In Flight", in_flight)
+    b = R.render(S, sc["camera"], frames=5)
+    b = R.render(S, sc["camera"], frames=3)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    assert np.array_equal(ref.sample_counts(), R.sample_counts())
+
+
 def test_camera_change_restarts_accumulation(cpt, cornell_runs):
     sc, S = cornell_runs["sc"], cornell_runs["S"]
     cam2 = dict(sc["camera"], pos=(0.0, 1.0, 3.3))
