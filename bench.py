@@ -59,6 +59,52 @@ def cpu_baseline(sc, seconds_target=10.0, max_frames=8):
                       f"{platform.processor() or platform.machine()})"}
 
 
+EXT_KERNEL = "k_trace<0, false, 1"
+
+
+def hbm_traffic(args):
+    """HBM bytes per extension launch from two rocprofv3 --pmc passes (FETCH_SIZE,
+    then WRITE_SIZE, KiB per dispatch) over one step of the same workload, each
+    scaled by what the counter reads for a 256 MiB device-to-device copy."""
+    import csv
+    import shutil
+    import subprocess
+    import tempfile
+    prof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(prof):
+        return None
+    calib = 256 << 20
+    out = {}
+    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+        d = tempfile.mkdtemp(prefix="cpt_pmc_")
+        cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t",
+               "--", sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
+               "--no-traffic", "--frames", str(args.frames), "--in-flight", str(args.in_flight),
+               "--lds-stack", str(args.lds_stack), "--copy-calib", str(calib >> 20)]
+        log(f"traffic pass: {counter}")
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        files = [os.path.join(root, f) for root, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+        if r.returncode != 0 or not files:
+            log(f"traffic pass {counter} failed (rc {r.returncode}); traffic reported as null")
+            return None
+        ext, copy = [], []
+        for row in csv.DictReader(open(files[0])):
+            v = float(row["Counter_Value"]) * 1024.0
+            if EXT_KERNEL in row["Kernel_Name"]:
+                ext.append(v)
+            elif "copyBuffer" in row["Kernel_Name"]:
+                copy.append(v)
+        shutil.rmtree(d, ignore_errors=True)
+        if not ext or not copy:
+            return None
+        factor = calib / max(copy)
+        out[counter] = (sum(ext) / len(ext), factor)
+    fetch, ffac = out["FETCH_SIZE"]
+    write, wfac = out["WRITE_SIZE"]
+    return {"bytes_per_launch": int(fetch * ffac + write * wfac), "fetch_bytes": int(fetch * ffac),
+            "write_bytes": int(write * wfac), "fetch_scale": round(ffac, 3), "write_scale": round(wfac, 3)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,7 +112,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--frames", type=int, default=SPP)
-    ap.add_argument("--in-flight", type=int, default=3, help="frames traced as one wavefront (multiple of 3)")
+    ap.add_argument("--in-flight", type=int, default=64, help="frames traced as one wavefront")
+    ap.add_argument("--lds-stack", type=int, default=806, help="traversal variant: LDS stack entries (8/16/24; "
+                    "806/807/808/1606 = 8 or 16 entries with >= 6, 7 or 8 waves per SIMD)")
+    ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -93,6 +142,8 @@ def main():
     log(f"scene: {len(sc['idx'])} triangles, bvh4 {info}, built+uploaded in {time.perf_counter() - t0:.1f} s")
     R = ca.Renderer(WIDTH, HEIGHT, eye_depth=EYE_DEPTH, light_depth=LIGHT_DEPTH)
     R.set_attrib("Frames In Flight", args.in_flight)
+    R.set_attrib("Traversal LDS Stack", args.lds_stack)
+    R.set_attrib("Shadow LDS Stack", args.lds_stack)
     cam = sc["camera"]
     for w in range(args.warmup):
         R.render(S, cam, frames=args.frames, result=False)
@@ -133,7 +184,8 @@ def main():
         "config": {"workload": "streaming BDPT, sphere grid 262,144 tris + 2 sphere lights, 1280x720, "
                                "depth 8, 64 spp per step, light-vertex cache every 3rd frame",
                    "triangles": int(len(sc["idx"])), "width": WIDTH, "height": HEIGHT, "spp": args.frames,
-                   "eye_depth": EYE_DEPTH, "light_depth": LIGHT_DEPTH, "frames_in_flight": args.in_flight},
+                   "eye_depth": EYE_DEPTH, "light_depth": LIGHT_DEPTH, "frames_in_flight": args.in_flight,
+                   "traversal_variant": args.lds_stack},
         "samples_per_s": round(st["eye_paths"] / wall, 1),
         "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},
         "kernel_ms": {"extend": round(st["extend_ms"], 3), "shade": round(st["shade_ms"], 3),
@@ -148,6 +200,9 @@ def main():
                      "tris_per_ray": round(st["extend_tris"] / max(st["extension_rays"], 1), 2)},
         "time_to_equal_mse": None,
     }
+    del R, S
+    if not args.no_traffic:
+        res["roofline"]["traffic"] = hbm_traffic(args)
     if not args.no_cpu:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(sc)

@@ -120,8 +120,9 @@ hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st);
 // Running-mean update of every pixel from the batch's per-frame records, frames in order.
 hipError_t launch_apply_frames(const FrameArgs& fa, const float4* rec_acc, float* result, uint32_t* sample_n,
                                hipStream_t st);
+// lds_stack: traversal stack entries kept in LDS (8, 16 or 24; the rest in private memory)
 hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, uint32_t max_n,
-                         float4* hits, DevStats* stats, hipStream_t st);
+                         float4* hits, DevStats* stats, int lds_stack, hipStream_t st);
 // apply: 0 = plain running mean at termination, 1 = running mean + G-buffer
 // (adaptive warm-up), 2 = write the sample record only (adaptive frame).
 struct Accum {
@@ -158,7 +159,7 @@ hipError_t launch_accum_adaptive(const uint32_t* offsets, const uint32_t* counts
                                  SampleRec rec, float* result, uint32_t* sample_n, float* gpos, float* gnrm,
                                  float* gdif, hipStream_t st);
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
-                         uint32_t max_n, hipStream_t st);
+                         uint32_t max_n, int lds_stack, hipStream_t st);
 hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes, uint32_t depth, hipStream_t st);
 
 }  // namespace cpt

@@ -31,8 +31,11 @@
 
 namespace cpt {
 
-#define CPT_LDS_STACK 16
-#define CPT_PRIV_STACK 80
+// Stack entries per thread in total; the top STK live in LDS, the rest in
+// private memory.  A 4-wide level pushes at most 3 entries and the depth is
+// <= 31 (checked at build), so 96 entries never overflow.
+#define CPT_STACK_TOTAL 96
+#define CPT_LDS_STACK 16   // default LDS share (k_light_pass, shadow)
 
 // Smallest float >= 1e-9: float |det| < it  <=>  (double)|det| < M_EPSILON.
 __device__ constexpr float kDetEps = 0x1.12e0c0p-30f;
@@ -102,14 +105,14 @@ __device__ __forceinline__ void cswap(uint64_t& a, uint64_t& b) {
   a = lo; b = hi;
 }
 
-// lstk points at this thread's column of a [CPT_LDS_STACK][kBlock] u64 array.
-template <bool ANY>
+// lstk points at this thread's column of a [STK][kBlock] u64 array in LDS.
+template <bool ANY, int STK = CPT_LDS_STACK>
 __device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps, bool cull, uint64_t* lstk,
                           TraceHit& h, uint32_t& nn, uint32_t& nt) {
   const f3 idir = mk3(1.f / escape_zero(d.x), 1.f / escape_zero(d.y), 1.f / escape_zero(d.z));
   const f3 o2 = o + d * eps;
   h.t = maxd; h.u = h.v = 0.f; h.leaf = kInvalid; h.orig = kInvalid;
-  uint64_t pstk[CPT_PRIV_STACK];
+  uint64_t pstk[CPT_STACK_TOTAL - STK];
   int sp = 0;
   uint32_t node = 0;
   for (;;) {
@@ -148,11 +151,9 @@ __device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps,
       node = (uint32_t)key[0];
 #pragma unroll
       for (int c = 3; c >= 1; c--) {
-        // 3 pushes per level and a 4-wide depth <= 31 (checked at build) keep
-        // sp below CPT_LDS_STACK + CPT_PRIV_STACK = 96.
-        if (key[c] != ~0ull && sp < CPT_LDS_STACK + CPT_PRIV_STACK) {
-          if (sp < CPT_LDS_STACK) lstk[sp * kBlock] = key[c];
-          else pstk[sp - CPT_LDS_STACK] = key[c];
+        if (key[c] != ~0ull && sp < CPT_STACK_TOTAL) {
+          if (sp < STK) lstk[sp * kBlock] = key[c];
+          else pstk[sp - STK] = key[c];
           sp++;
         }
       }
@@ -162,7 +163,7 @@ __device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps,
     bool found = false;
     while (sp > 0) {
       sp--;
-      uint64_t e = sp < CPT_LDS_STACK ? lstk[sp * kBlock] : pstk[sp - CPT_LDS_STACK];
+      uint64_t e = sp < STK ? lstk[sp * kBlock] : pstk[sp - STK];
       float tn = __uint_as_float((uint32_t)(e >> 32));
       if (tn <= h.t + cull_slack(h.t, eps)) { node = (uint32_t)e; found = true; break; }
     }
@@ -217,19 +218,21 @@ static bool rec_ok(const SampleRec& r) { return r.acc && r.gpos && r.gnrm && r.g
 
 // ---------------------------------------------------------------- trace API
 // out mode 0: cpt_hit {orig id, t, u, v}; 1: {t, leaf bits, u, v}; 2: occluded flag.
-template <int LAYOUT, bool ANY, int OUT>
-__global__ __launch_bounds__(kBlock) void k_trace(DevScene sc, const float4* __restrict__ ro,
+// MINW: minimum resident waves per SIMD requested from the register allocator
+// (0 = compiler default).
+template <int LAYOUT, bool ANY, int OUT, int STK = CPT_LDS_STACK, int MINW = 0>
+__global__ __launch_bounds__(kBlock, MINW) void k_trace(DevScene sc, const float4* __restrict__ ro,
                                                   const float4* __restrict__ rd, uint32_t stride,
                                                   const uint32_t* __restrict__ countp, uint32_t n, uint32_t cull,
                                                   float4* __restrict__ out, DevStats* __restrict__ stats) {
-  __shared__ uint64_t lstk[CPT_LDS_STACK * kBlock];
+  __shared__ uint64_t lstk[STK * kBlock];
   uint32_t count = countp ? *countp : n;
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   uint32_t nn = 0, nt = 0;
   if (i < count) {
     float4 a = ro[(size_t)i * stride], b = rd[(size_t)i * stride];
     TraceHit h;
-    bool hit = LAYOUT == 0 ? traverse4<ANY>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, lstk + threadIdx.x, h, nn, nt)
+    bool hit = LAYOUT == 0 ? traverse4<ANY, STK>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, lstk + threadIdx.x, h, nn, nt)
                            : traverse_bin<ANY>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, h, nn, nt);
     if (OUT == 0) {
       float4 r;
@@ -657,11 +660,22 @@ hipError_t launch_apply_frames(const FrameArgs& fa, const float4* rec_acc, float
 }
 
 hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, uint32_t max_n,
-                         float4* hits, DevStats* stats, hipStream_t st) {
+                         float4* hits, DevStats* stats, int lds_stack, hipStream_t st) {
   if (max_n == 0) return hipSuccess;
   if (!pathbuf_ok(pb) || !fc || !hits || !sc.nodes4 || !sc.tris) return hipErrorInvalidValue;
-  k_trace<0, false, 1><<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, pb.ray_o, pb.ray_d, 1,
-                                                                            &fc->active[bounce], max_n, 0, hits, stats);
+  dim3 grid((max_n + kBlock - 1) / kBlock), block(kBlock);
+  const uint32_t* cnt = &fc->active[bounce];
+#define CPT_EXT(S, W) k_trace<0, false, 1, S, W><<<grid, block, 0, st>>>(sc, pb.ray_o, pb.ray_d, 1, cnt, max_n, 0, hits, stats)
+  switch (lds_stack) {
+    case 8: CPT_EXT(8, 0); break;
+    case 24: CPT_EXT(24, 0); break;
+    case 806: CPT_EXT(8, 6); break;      // 8 LDS entries, >= 6 waves per SIMD
+    case 807: CPT_EXT(8, 7); break;      // 8 LDS entries, >= 7 waves per SIMD
+    case 808: CPT_EXT(8, 8); break;      // 8 LDS entries, >= 8 waves per SIMD
+    case 1606: CPT_EXT(16, 6); break;
+    default: CPT_EXT(16, 0); break;
+  }
+#undef CPT_EXT
   return hipGetLastError();
 }
 
@@ -848,16 +862,17 @@ hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, Pa
   return hipGetLastError();
 }
 
-__global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, ShadowBuf sb, PathBuf nxt,
+template <int STK, int MINW = 0>
+__global__ __launch_bounds__(kBlock, MINW) void k_shadow(DevScene sc, ShadowBuf sb, PathBuf nxt,
                                                    const FrameCounters* __restrict__ fc, int bounce) {
-  __shared__ uint64_t lstk[CPT_LDS_STACK * kBlock];
+  __shared__ uint64_t lstk[STK * kBlock];
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t count = fc->shadow[bounce];
   if (i >= count) return;
   float4 a = sb.ray_o[i], b = sb.ray_d[i], c = sb.contrib[i];
   TraceHit h;
   uint32_t nn = 0, nt = 0;
-  if (!traverse4<true>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt)) {
+  if (!traverse4<true, STK>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt)) {
     uint32_t slot = __float_as_uint(c.w);
     float4 acc = nxt.acc[slot];
     acc.x = acc.x + c.x;
@@ -868,10 +883,19 @@ __global__ __launch_bounds__(kBlock) void k_shadow(DevScene sc, ShadowBuf sb, Pa
   }
 }
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
-                         uint32_t max_n, hipStream_t st) {
+                         uint32_t max_n, int lds_stack, hipStream_t st) {
   if (max_n == 0) return hipSuccess;
   if (!pathbuf_ok(nxt) || !sb.ray_o || !sb.ray_d || !sb.contrib || !fc) return hipErrorInvalidValue;
-  k_shadow<<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, sb, nxt, fc, bounce);
+  dim3 grid((max_n + kBlock - 1) / kBlock), block(kBlock);
+  switch (lds_stack) {
+    case 8: k_shadow<8><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
+    case 24: k_shadow<24><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
+    case 806: k_shadow<8, 6><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
+    case 807: k_shadow<8, 7><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
+    case 808: k_shadow<8, 8><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
+    case 1606: k_shadow<16, 6><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
+    default: k_shadow<16><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
+  }
   return hipGetLastError();
 }
 
