@@ -20,7 +20,8 @@ _lib = None
 CPT_MODE_BDPT_STREAM = 0
 CPT_MODE_BDPT_ADAPTIVE_PLL = 1
 (CPT_BUF_RESULT, CPT_BUF_SAMPLE_N, CPT_BUF_GBUF_POS, CPT_BUF_GBUF_NORM, CPT_BUF_GBUF_DIFF, CPT_BUF_FILTERED,
- CPT_BUF_ERROR, CPT_BUF_SAMPLE_COUNTS) = range(8)
+ CPT_BUF_ERROR, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_ALLOC_ERROR) = range(9)
+MLP_INPUTS, MLP_HIDDEN, MLP_PARAMS = 9, 64, 4865
 
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("max_dist", "<f4"), ("d", "<f4", 3), ("eps", "<f4")])
 HIT_DTYPE = np.dtype([("tri", "<u4"), ("t", "<f4"), ("u", "<f4"), ("v", "<f4")])
@@ -35,7 +36,7 @@ class _Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("extension_rays", "shadow_rays", "eye_paths", "frames",
                                               "extend_launches", "extend_nodes", "extend_tris")] + \
                [(n, ctypes.c_double) for n in ("extend_ms", "shadow_ms", "shade_ms", "light_ms", "total_ms",
-                                               "filter_ms", "error_ms", "gen_ms", "accum_ms")]
+                                               "filter_ms", "error_ms", "gen_ms", "accum_ms", "mlp_ms")]
 
 
 def lib():
@@ -81,6 +82,16 @@ def lib():
             "cpt_renderer_stats": [vp, vp, i32],
             "cpt_renderer_set_profiling": [vp, i32],
             "cpt_renderer_destroy": [vp],
+            "cpt_mlp_create": [vp, ctypes.c_uint64],
+            "cpt_mlp_destroy": [vp],
+            "cpt_mlp_set_params": [vp, vp, c32],
+            "cpt_mlp_get_params": [vp, vp, c32],
+            "cpt_mlp_get_grad": [vp, vp, c32],
+            "cpt_mlp_train_step": [vp, vp, vp, c32, f32, i32, vp, vp],
+            "cpt_mlp_infer": [vp, vp, c32, vp, vp],
+            "cpt_renderer_mlp_features": [vp, vp],
+            "cpt_renderer_mlp_target": [vp, vp, vp],
+            "cpt_renderer_set_mlp": [vp, vp],
         }
         for name, args in sigs.items():
             fn = getattr(L, name)
@@ -285,7 +296,7 @@ class Renderer:
         """A renderer buffer: CPT_BUF_* (result, counts, G-buffer, filtered, error, per-frame sample counts)."""
         if which in (CPT_BUF_SAMPLE_N, CPT_BUF_SAMPLE_COUNTS):
             out = np.zeros((self.H, self.W), np.uint32)
-        elif which == CPT_BUF_ERROR:
+        elif which in (CPT_BUF_ERROR, CPT_BUF_ALLOC_ERROR):
             out = np.zeros((self.H, self.W), np.float32)
         else:
             out = np.zeros((self.H, self.W, 3), np.float32)
@@ -295,6 +306,24 @@ class Renderer:
     def reset(self):
         _check(lib().cpt_renderer_reset(self.h), "cpt_renderer_reset")
 
+    def mlp_features(self, d_x: "DeviceBuffer"):
+        """Writes the W*H x 9 model inputs of the current adaptive state into d_x."""
+        if d_x.nbytes < 36 * self.W * self.H:
+            raise CptError("feature buffer smaller than W*H x 9 floats")
+        _check(lib().cpt_renderer_mlp_features(self.h, ctypes.c_void_p(d_x.ptr)), "cpt_renderer_mlp_features")
+
+    def mlp_target(self, d_ref: "DeviceBuffer", d_t: "DeviceBuffer"):
+        """Writes the W*H training targets against the converged image in d_ref into d_t."""
+        if d_ref.nbytes < 12 * self.W * self.H or d_t.nbytes < 4 * self.W * self.H:
+            raise CptError("reference or target buffer smaller than the image")
+        _check(lib().cpt_renderer_mlp_target(self.h, ctypes.c_void_p(d_ref.ptr), ctypes.c_void_p(d_t.ptr)),
+               "cpt_renderer_mlp_target")
+
+    def set_mlp(self, model: "Mlp | None"):
+        """Drives the adaptive allocation with `model` (None: the filter residual)."""
+        _check(lib().cpt_renderer_set_mlp(self.h, model.h if model is not None else None), "cpt_renderer_set_mlp")
+        self._mlp = model                 # the model must outlive the attachment
+
     def set_profiling(self, on: bool):
         _check(lib().cpt_renderer_set_profiling(self.h, int(on)), "cpt_renderer_set_profiling")
 
@@ -302,6 +331,65 @@ class Renderer:
         s = _Stats()
         _check(lib().cpt_renderer_stats(self.h, ctypes.byref(s), int(reset)), "cpt_renderer_stats")
         return {n: getattr(s, n) for n, _ in s._fields_}
+
+
+class Mlp:
+    """Per-pixel error model 9 -> 64 -> 64 -> 1 (BF16 MFMA, FP32 master weights, Adam)."""
+
+    def __init__(self, seed: int = 0xA22):
+        h = ctypes.c_void_p()
+        _check(lib().cpt_mlp_create(ctypes.byref(h), seed), "cpt_mlp_create")
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cpt_mlp_destroy(self.h)
+            self.h = None
+
+    def params(self) -> np.ndarray:
+        out = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib().cpt_mlp_get_params(self.h, _ptr(out), MLP_PARAMS), "cpt_mlp_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, np.float32)
+        _check(lib().cpt_mlp_set_params(self.h, _ptr(p), p.size), "cpt_mlp_set_params")
+
+    def grad(self) -> np.ndarray:
+        out = np.zeros(MLP_PARAMS, np.float32)
+        _check(lib().cpt_mlp_get_grad(self.h, _ptr(out), MLP_PARAMS), "cpt_mlp_get_grad")
+        return out
+
+    def train_step_device(self, d_x: "DeviceBuffer", d_t: "DeviceBuffer", n: int, lr: float = 1e-3,
+                          apply: bool = True, want_loss: bool = True, x_offset: int = 0):
+        """One step on n samples starting at sample x_offset of device arrays d_x (n x 9) and d_t."""
+        if n < 1 or x_offset < 0 or 36 * (x_offset + n) > d_x.nbytes or 4 * (x_offset + n) > d_t.nbytes:
+            raise CptError(f"train batch [{x_offset}, {x_offset + n}) exceeds the device arrays")
+        loss = ctypes.c_float()
+        vp = ctypes.c_void_p
+        _check(lib().cpt_mlp_train_step(self.h, vp(d_x.ptr + 36 * x_offset), vp(d_t.ptr + 4 * x_offset), n, lr,
+                                        int(apply), ctypes.byref(loss) if want_loss else None, None),
+               "cpt_mlp_train_step")
+        return loss.value if want_loss else None
+
+    def train_step(self, x, t, lr: float = 1e-3, apply: bool = True) -> float:
+        x = np.ascontiguousarray(x, np.float32)
+        t = np.ascontiguousarray(t, np.float32)
+        dx, dt = DeviceBuffer.from_array(x), DeviceBuffer.from_array(t)
+        return self.train_step_device(dx, dt, len(t), lr, apply)
+
+    def infer_device(self, d_x: "DeviceBuffer", n: int, d_y: "DeviceBuffer"):
+        if 36 * n > d_x.nbytes or 4 * n > d_y.nbytes:
+            raise CptError(f"{n} samples exceed the device arrays")
+        _check(lib().cpt_mlp_infer(self.h, ctypes.c_void_p(d_x.ptr), n, ctypes.c_void_p(d_y.ptr), None),
+               "cpt_mlp_infer")
+
+    def infer(self, x) -> np.ndarray:
+        x = np.ascontiguousarray(x, np.float32)
+        n = len(x)
+        dx, dy = DeviceBuffer.from_array(x), DeviceBuffer(max(n, 1) * 4)
+        self.infer_device(dx, n, dy)
+        return dy.download(np.float32, n)
 
 
 def make_rays(o, d, max_dist=1e20, eps=1e-4) -> np.ndarray:

@@ -37,7 +37,7 @@ def build(verbose: bool = False) -> str:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
     objs = []
-    for src in ("kernels.hip", "capi.cpp", "bvh.cpp"):
+    for src in ("kernels.hip", "mlp.hip", "capi.cpp", "bvh.cpp"):
         obj = os.path.join(objdir, src + ".o")
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
         cmd = [hipcc, *HIP_FLAGS, *lang, "-c", os.path.join(CSRC, src), "-o", obj]

@@ -12,6 +12,7 @@
 #include "../../include/cpt.h"
 #include "cpt_internal.h"
 #include "kernels.h"
+#include "mlp.h"
 
 using namespace cpt;
 
@@ -440,6 +441,25 @@ constexpr u32 kMaxInFlight = 90;   // 30 three-frame light-cache groups + partia
 
 }  // namespace
 
+// ---- learned per-pixel error model ---------------------------------------
+
+struct cpt_mlp {
+  hipStream_t stream = nullptr;
+  DevArray<float> w, m, v, grad, loss, b1, b2, w3, b3, part, loss_part;
+  DevArray<uint16_t> w1t, w2, w2t;
+  int step = 0;
+  ~cpt_mlp() {
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  MlpState state() {
+    MlpState s;
+    s.w = w.p; s.m = m.p; s.v = v.p; s.grad = grad.p; s.loss = loss.p;
+    s.dev.w1t = w1t.p; s.dev.w2 = w2.p; s.dev.w2t = w2t.p;
+    s.dev.b1 = b1.p; s.dev.b2 = b2.p; s.dev.w3 = w3.p; s.dev.b3 = b3.p;
+    return s;
+  }
+};
+
 struct cpt_renderer {
   int mode = 0;
   u32 W = 0, H = 0;
@@ -465,7 +485,7 @@ struct cpt_renderer {
   struct Ev { hipEvent_t a, b; int kind; };
   std::vector<Ev> events;          // recorded pairs awaiting readback
   std::vector<hipEvent_t> pool;    // reusable events
-  double ms[9] = {0};  // extend, shadow, shade, light, total, filter, error sum, generation, accumulation
+  double ms[10] = {0};  // extend, shadow, shade, light, total, filter, error sum, generation, accumulation, model
   ~cpt_renderer() {
     for (auto& e : events) { pool.push_back(e.a); pool.push_back(e.b); }
     for (auto e : pool) (void)hipEventDestroy(e);
@@ -491,6 +511,8 @@ struct cpt_renderer {
   DevArray<float4> rec_acc, rec_pos, rec_nrm, rec_dif;
   float gauss_sigma = -1.f;
   u32 gauss_radius = 0;
+  cpt_mlp* mlp = nullptr;              // learned error model (adaptive mode)
+  DevArray<float> mlp_x, mlp_y, mlp_e;
   u32 cap = 0;
   long long slot_group[kLightSlots];   // three-frame group whose light cache each slot holds (-1: none)
   void forget_light_caches() { for (auto& g : slot_group) g = -1; }
@@ -648,6 +670,24 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
 }
 
 // One frame of cudaRTBDPTStreamAdapBilatAtomic::Render (AdapBilatAtomic.cu:1338-1538).
+// Cross-bilateral filter of the running mean and the weighted squared residual
+// (src/cudaRTBDPTStreamAdapBilatAtomic.cu:1409-1430) into r->filtered / r->err.
+int run_filter(cpt_renderer* r) {
+  float sc_ = (float)r->attr("Filter Color EuD"), sp_ = (float)r->attr("Filter Pos EuD");
+  float sn_ = (float)r->attr("Filter Norm EuD"), sd_ = (float)r->attr("Filter Diff EuD");
+  u32 radius = (u32)r->attr("Filter Radius");
+  if (radius > (u32)kMaxFilterRadius) return fail(CPT_E_LIMIT, "render: filter radius %u exceeds 31", radius);
+  int rc = update_gauss(r, sc_, radius);
+  if (rc) return rc;
+  FilterArgs f;
+  f.width = r->W; f.height = r->H; f.radius = radius; f.nd = (u32)r->attr("Desired Max Sampling");
+  f.kc = 2.f * sc_ * sc_; f.kp = 2.f * sp_ * sp_; f.kn = 2.f * sn_ * sn_; f.kd = 2.f * sd_ * sd_;
+  f.use_diff = r->attr("Filter Diffuse Flag") != 0;
+  HIP_TRY(launch_filter_error(f, r->result.p, r->gpos.p, r->gnrm.p, r->gdif.p, r->gauss.p, r->sample_n.p,
+                              r->filtered.p, r->err.p, r->stream));
+  return CPT_OK;
+}
+
 int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   DevScene sc = s->dev();
   const u32 n = r->W * r->H;
@@ -670,24 +710,29 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
     int rc = bounce_loop(r, sc, fa, n, ac);
     if (rc) return rc;
   } else {
-    float sc_ = (float)r->attr("Filter Color EuD"), sp_ = (float)r->attr("Filter Pos EuD");
-    float sn_ = (float)r->attr("Filter Norm EuD"), sd_ = (float)r->attr("Filter Diff EuD");
-    u32 radius = (u32)r->attr("Filter Radius");
-    int rc = update_gauss(r, sc_, radius);
-    if (rc) return rc;
-    FilterArgs f;
-    f.width = r->W; f.height = r->H; f.radius = radius; f.nd = (u32)r->attr("Desired Max Sampling");
-    f.kc = 2.f * sc_ * sc_; f.kp = 2.f * sp_ * sp_; f.kn = 2.f * sn_ * sn_; f.kd = 2.f * sd_ * sd_;
-    f.use_diff = r->attr("Filter Diffuse Flag") != 0;
     HIP_TRY(timed_begin(r, 5, &ev));
-    HIP_TRY(launch_filter_error(f, r->result.p, r->gpos.p, r->gnrm.p, r->gdif.p, r->gauss.p, r->sample_n.p,
-                                r->filtered.p, r->err.p, r->stream));
+    int rc = run_filter(r);
+    if (rc) return rc;
     HIP_TRY(timed_end(r, 5, ev));
+    const u32 nd = (u32)r->attr("Desired Max Sampling");
+    const float* alloc_err = r->err.p;
+    if (r->mlp) {                                  // learned estimate replaces the filter residual
+      HIP_TRY(r->mlp_x.alloc((size_t)n * kMlpIn));
+      HIP_TRY(r->mlp_y.alloc(n));
+      HIP_TRY(r->mlp_e.alloc(n));
+      HIP_TRY(timed_begin(r, 9, &ev));
+      HIP_TRY(launch_mlp_features(r->result.p, r->filtered.p, r->err.p, r->gpos.p, r->gnrm.p, r->gdif.p,
+                                  r->sample_n.p, r->cam, n, r->mlp_x.p, r->stream));
+      HIP_TRY(launch_mlp_infer(r->mlp->state(), r->mlp_x.p, n, r->mlp_y.p, r->stream));
+      HIP_TRY(launch_mlp_err(r->mlp_y.p, r->sample_n.p, nd, n, r->mlp_e.p, r->stream));
+      HIP_TRY(timed_end(r, 9, ev));
+      alloc_err = r->mlp_e.p;
+    }
     HIP_TRY(timed_begin(r, 6, &ev));
-    HIP_TRY(launch_error_sum(r->err.p, n, r->block_sums.p, r->err_sum.p, r->stream));
+    HIP_TRY(launch_error_sum(alloc_err, n, r->block_sums.p, r->err_sum.p, r->stream));
     HIP_TRY(timed_end(r, 6, ev));
     HIP_TRY(timed_begin(r, 7, &ev));
-    HIP_TRY(launch_alloc_scan(r->err.p, r->err_sum.p, n, n, r->counts.p, r->offsets.p, r->block_tot.p, r->total.p,
+    HIP_TRY(launch_alloc_scan(alloc_err, r->err_sum.p, n, n, r->counts.p, r->offsets.p, r->block_tot.p, r->total.p,
                               r->cap, r->stream));
     HIP_TRY(launch_frame_begin_dev(r->fc.p, r->total.p, r->cap, r->stream));
     HIP_TRY(launch_gen_adaptive(fa, r->offsets.p, r->counts.p, r->total.p, r->cap, r->A.view(), ac.rec, r->stream));
@@ -919,6 +964,13 @@ int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out) {
       HIP_TRY(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, r->stream));
       break;
     }
+    case CPT_BUF_ALLOC_ERROR: {
+      if (r->mode != CPT_MODE_BDPT_ADAPTIVE_PLL)
+        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive mode", which);
+      const float* src = r->mlp && r->mlp_e.p ? r->mlp_e.p : r->err.p;
+      HIP_TRY(hipMemcpyAsync(out, src, n * 4, hipMemcpyDeviceToHost, r->stream));
+      break;
+    }
     default:
       return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d not produced by this mode", which);
   }
@@ -957,6 +1009,7 @@ int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset) {
   out->error_ms = r->ms[6];
   out->gen_ms = r->ms[7];
   out->accum_ms = r->ms[8];
+  out->mlp_ms = r->ms[9];
   if (reset) {
     HIP_TRY(hipMemset(r->stats.p, 0, sizeof(DevStats)));
     for (double& m : r->ms) m = 0;
@@ -975,6 +1028,164 @@ int cpt_renderer_destroy(cpt_renderer* r) {
   if (!r) return fail(CPT_E_ARG, "cpt_renderer_destroy: null renderer");
   if (r->stream) (void)hipStreamSynchronize(r->stream);
   delete r;
+  return CPT_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Host PCG32 (O'Neill 2014), the generator of the device kernels.
+struct HostPcg {
+  uint64_t state = 0, inc = 0;
+  HostPcg(uint64_t initstate, uint64_t initseq) {
+    inc = (initseq << 1u) | 1u;
+    next();
+    state += initstate;
+    next();
+  }
+  uint32_t next() {
+    uint64_t old = state;
+    state = old * 6364136223846793005ULL + inc;
+    uint32_t xs = (uint32_t)(((old >> 18u) ^ old) >> 27u);
+    uint32_t rot = (uint32_t)(old >> 59u);
+    return (xs >> rot) | (xs << ((-rot) & 31));
+  }
+  float uniform01() { return (float)((next() >> 8) + 1u) * (1.0f / 16777216.0f); }
+};
+
+int mlp_upload(cpt_mlp* m, const float* params) {
+  hipStream_t st = m->stream;
+  HIP_TRY(hipMemcpyAsync(m->w.p, params, sizeof(float) * kMlpParams, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemsetAsync(m->m.p, 0, sizeof(float) * kMlpParams, st));
+  HIP_TRY(hipMemsetAsync(m->v.p, 0, sizeof(float) * kMlpParams, st));
+  HIP_TRY(hipMemsetAsync(m->grad.p, 0, sizeof(float) * kMlpParams, st));
+  HIP_TRY(launch_mlp_pack(m->state(), st));
+  HIP_TRY(hipStreamSynchronize(st));
+  m->step = 0;
+  return CPT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int cpt_mlp_create(cpt_mlp** out, uint64_t seed) {
+  if (!out) return fail(CPT_E_ARG, "cpt_mlp_create: null output");
+  *out = nullptr;
+  auto* m = new (std::nothrow) cpt_mlp();
+  if (!m) return fail(CPT_E_NOMEM, "cpt_mlp_create: out of host memory");
+  auto bad = [&](int rc) { delete m; return rc; };
+  if (int rc = ensure_stream(m->stream)) return bad(rc);
+  for (auto* a : {&m->w, &m->m, &m->v, &m->grad})
+    if (a->alloc(kMlpParams) != hipSuccess) return bad(fail(CPT_E_NOMEM, "cpt_mlp_create: device allocation"));
+  if (m->loss.alloc(1) || m->b1.alloc(64) || m->b2.alloc(64) || m->w3.alloc(64) || m->b3.alloc(1) ||
+      m->w1t.alloc(64 * 32) || m->w2.alloc(64 * 64) || m->w2t.alloc(64 * 64))
+    return bad(fail(CPT_E_NOMEM, "cpt_mlp_create: device allocation"));
+  // feature columns 9..31 of W1^T stay zero
+  if (hipMemset(m->w1t.p, 0, sizeof(uint16_t) * 64 * 32) != hipSuccess)
+    return bad(fail(CPT_E_HIP, "cpt_mlp_create: hipMemset"));
+  std::vector<float> p(kMlpParams, 0.f);
+  HostPcg rng(seed, 0);
+  auto glorot = [&](int off, int fin, int fout) {
+    float a = std::sqrt(6.0f / (float)(fin + fout));
+    for (int i = 0; i < fin * fout; i++) p[off + i] = (2.0f * rng.uniform01() - 1.0f) * a;
+  };
+  glorot(kMlpOffW1, kMlpIn, kMlpHidden);
+  glorot(kMlpOffW2, kMlpHidden, kMlpHidden);
+  glorot(kMlpOffW3, kMlpHidden, 1);
+  if (int rc = mlp_upload(m, p.data())) return bad(rc);
+  *out = m;
+  return CPT_OK;
+}
+
+int cpt_mlp_destroy(cpt_mlp* m) {
+  if (!m) return fail(CPT_E_ARG, "cpt_mlp_destroy: null model");
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  delete m;
+  return CPT_OK;
+}
+
+int cpt_mlp_set_params(cpt_mlp* m, const float* params, uint32_t n) {
+  if (!m || !params) return fail(CPT_E_ARG, "cpt_mlp_set_params: null argument");
+  if (n != (uint32_t)kMlpParams) return fail(CPT_E_ARG, "cpt_mlp_set_params: %u parameters, expected 4865", n);
+  return mlp_upload(m, params);
+}
+
+int cpt_mlp_get_params(cpt_mlp* m, float* params, uint32_t n) {
+  if (!m || !params) return fail(CPT_E_ARG, "cpt_mlp_get_params: null argument");
+  if (n != (uint32_t)kMlpParams) return fail(CPT_E_ARG, "cpt_mlp_get_params: %u parameters, expected 4865", n);
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  HIP_TRY(hipMemcpy(params, m->w.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  return CPT_OK;
+}
+
+int cpt_mlp_get_grad(cpt_mlp* m, float* grad, uint32_t n) {
+  if (!m || !grad) return fail(CPT_E_ARG, "cpt_mlp_get_grad: null argument");
+  if (n != (uint32_t)kMlpParams) return fail(CPT_E_ARG, "cpt_mlp_get_grad: %u parameters, expected 4865", n);
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  HIP_TRY(hipMemcpy(grad, m->grad.p, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost));
+  return CPT_OK;
+}
+
+int cpt_mlp_train_step(cpt_mlp* m, const float* d_x, const float* d_target, uint32_t n, float lr, int apply,
+                       float* loss, void* stream) {
+  if (!m || !d_x || !d_target) return fail(CPT_E_ARG, "cpt_mlp_train_step: null argument");
+  if (n == 0) return fail(CPT_E_ARG, "cpt_mlp_train_step: empty batch");
+  if (n > (1u << 24)) return fail(CPT_E_LIMIT, "cpt_mlp_train_step: batch %u exceeds 2^24", n);
+  if (!(lr >= 0.f)) return fail(CPT_E_ARG, "cpt_mlp_train_step: learning rate must be >= 0");
+  hipStream_t st = stream ? (hipStream_t)stream : m->stream;
+  u32 nwg = (n + 63) / 64;
+  HIP_TRY(m->part.alloc((size_t)nwg * kMlpParams));
+  HIP_TRY(m->loss_part.alloc(nwg));
+  MlpState s = m->state();
+  HIP_TRY(launch_mlp_train(s, d_x, d_target, n, m->part.p, m->loss_part.p, st));
+  int step = apply ? m->step + 1 : (m->step > 0 ? m->step : 1);
+  HIP_TRY(launch_mlp_adam(s, m->part.p, nwg, m->loss_part.p, lr, step, apply != 0, st));
+  if (apply) m->step = step;
+  if (loss) {
+    HIP_TRY(hipStreamSynchronize(st));
+    float l = 0.f;
+    HIP_TRY(hipMemcpy(&l, m->loss.p, sizeof l, hipMemcpyDeviceToHost));
+    *loss = l / (float)n;
+  }
+  return CPT_OK;
+}
+
+int cpt_mlp_infer(cpt_mlp* m, const float* d_x, uint32_t n, float* d_y, void* stream) {
+  if (!m || ((!d_x || !d_y) && n)) return fail(CPT_E_ARG, "cpt_mlp_infer: null argument");
+  hipStream_t st = stream ? (hipStream_t)stream : m->stream;
+  HIP_TRY(launch_mlp_infer(m->state(), d_x, n, d_y, st));
+  if (!stream) HIP_TRY(hipStreamSynchronize(st));
+  return CPT_OK;
+}
+
+int cpt_renderer_mlp_features(cpt_renderer* r, float* d_x) {
+  if (!r || !d_x) return fail(CPT_E_ARG, "cpt_renderer_mlp_features: null argument");
+  if (r->mode != CPT_MODE_BDPT_ADAPTIVE_PLL || !r->allocated || !r->have_cam)
+    return fail(CPT_E_STATE, "cpt_renderer_mlp_features: needs an adaptive renderer that has rendered");
+  // the filter state the next adaptive frame allocates from
+  int rc = run_filter(r);
+  if (rc) return rc;
+  HIP_TRY(launch_mlp_features(r->result.p, r->filtered.p, r->err.p, r->gpos.p, r->gnrm.p, r->gdif.p, r->sample_n.p,
+                              r->cam, r->W * r->H, d_x, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return CPT_OK;
+}
+
+int cpt_renderer_mlp_target(cpt_renderer* r, const float* d_ref, float* d_t) {
+  if (!r || !d_ref || !d_t) return fail(CPT_E_ARG, "cpt_renderer_mlp_target: null argument");
+  if (!r->allocated) return fail(CPT_E_STATE, "cpt_renderer_mlp_target: renderer has not rendered");
+  HIP_TRY(launch_mlp_target(r->result.p, d_ref, r->sample_n.p, r->W * r->H, d_t, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return CPT_OK;
+}
+
+int cpt_renderer_set_mlp(cpt_renderer* r, cpt_mlp* m) {
+  if (!r) return fail(CPT_E_ARG, "cpt_renderer_set_mlp: null renderer");
+  if (m && r->mode != CPT_MODE_BDPT_ADAPTIVE_PLL)
+    return fail(CPT_E_STATE, "cpt_renderer_set_mlp: the model drives the adaptive renderer only");
+  r->mlp = m;
   return CPT_OK;
 }
 

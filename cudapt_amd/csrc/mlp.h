@@ -1,0 +1,66 @@
+// Per-pixel error MLP 9 -> 64 -> 64 -> 1: parameter layout and launch interface
+// of mlp.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace cpt {
+
+// Flat FP32 parameter vector (row-major, input index first):
+//   W1[9][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3
+constexpr int kMlpIn = 9;
+constexpr int kMlpHidden = 64;
+constexpr int kMlpOffW1 = 0;
+constexpr int kMlpOffB1 = kMlpOffW1 + kMlpIn * kMlpHidden;      // 576
+constexpr int kMlpOffW2 = kMlpOffB1 + kMlpHidden;                // 640
+constexpr int kMlpOffB2 = kMlpOffW2 + kMlpHidden * kMlpHidden;   // 4736
+constexpr int kMlpOffW3 = kMlpOffB2 + kMlpHidden;                // 4800
+constexpr int kMlpOffB3 = kMlpOffW3 + kMlpHidden;                // 4864
+constexpr int kMlpParams = kMlpOffB3 + 1;                        // 4865
+static_assert(kMlpParams == 4865, "parameter count");
+
+// Operands in the layouts the MFMA tiles read: BF16 matrices, FP32 vectors.
+struct MlpDevWeights {
+  uint16_t* w1t;  // [64][32]  W1^T, feature columns 9..31 zero
+  uint16_t* w2;   // [64][64]  W2
+  uint16_t* w2t;  // [64][64]  W2^T
+  float* b1;      // [64]
+  float* b2;      // [64]
+  float* w3;      // [64]
+  float* b3;      // [1]
+};
+
+struct MlpState {
+  float* w;       // FP32 master parameters
+  float* m;       // Adam first moment
+  float* v;       // Adam second moment
+  float* grad;    // last summed gradient
+  float* loss;    // last batch loss (sum of squared residuals)
+  MlpDevWeights dev;
+};
+
+// Forward + backward of n samples (x: n x 9 row-major, t: n targets); writes
+// one gradient vector per 64-sample workgroup into `part` (ceil(n/64) x 4865)
+// and one loss partial per workgroup.  The loss is mean squared error.
+hipError_t launch_mlp_train(const MlpState& st, const float* x, const float* t, uint32_t n, float* part,
+                            float* loss_part, hipStream_t s);
+// Sums the partials in workgroup order into st.grad / st.loss, and when
+// `apply` is set performs Adam step `step` (1-based) and refreshes the operands.
+hipError_t launch_mlp_adam(const MlpState& st, const float* part, uint32_t nwg, const float* loss_part, float lr,
+                           int step, int apply, hipStream_t s);
+hipError_t launch_mlp_infer(const MlpState& st, const float* x, uint32_t n, float* y, hipStream_t s);
+// Rebuilds the operand pack from st.w.
+hipError_t launch_mlp_pack(const MlpState& st, hipStream_t s);
+size_t mlp_lds_bytes();
+
+// Renderer-side kernels (images W*H x 3 FP32, n = W*H pixels):
+// the 9 features of each pixel, the log per-sample-variance training target
+// against a converged image, and the allocation signal from model outputs y.
+hipError_t launch_mlp_features(const float* result, const float* filtered, const float* err, const float* gpos,
+                               const float* gnrm, const float* gdif, const uint32_t* sample_n, const float* cam_pos,
+                               uint32_t n, float* x, hipStream_t s);
+hipError_t launch_mlp_target(const float* result, const float* ref, const uint32_t* sample_n, uint32_t n, float* t,
+                             hipStream_t s);
+hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd, uint32_t n, float* e, hipStream_t s);
+
+}  // namespace cpt

@@ -1,0 +1,407 @@
+// Per-pixel error MLP 9 -> 64 -> 64 -> 1 (GELU), BF16 operands on gfx950 MFMA
+// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation, FP32 master weights and
+// Adam.  No reference counterpart: CudaPT links FANN but never calls it; the
+// network replaces the filter-residual error of the adaptive renderer
+// (src/cudaRTBDPTStreamAdapBilatAtomic.cu:1143-1156) as the allocation signal.
+//
+// One workgroup = one wave = 64 pixels.  Layer matrices are 64 x 64 and are
+// computed as 4 x 4 tiles of 16 x 16; each tile is ceil(K/32) MFMAs.
+//   forward   Z1 = X W1 + b1, H1 = gelu(Z1), Z2 = H1 W2 + b2, H2 = gelu(Z2), y = H2 w3 + b3
+//   backward  dy = 2 (y - t) / B, dZ2 = dy w3^T * gelu'(Z2), dH1 = dZ2 W2^T,
+//             dZ1 = dH1 * gelu'(Z1), dW2 = H1^T dZ2, dW1 = X^T dZ1 (all MFMA)
+// Operands sit in LDS in the orientation each MFMA reads contiguously (an
+// activation is stored once per orientation it is read in).  Per-wave weight
+// gradients go to a partials array; k_mlp_adam sums them in workgroup order
+// (deterministic) and applies Adam (Kingma & Ba 2015).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mlp.h"
+
+namespace cpt {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// float -> bf16 with round-to-nearest-even (finite inputs).
+__device__ __forceinline__ uint16_t f2bf(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7FFFu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// GELU, tanh form (Hendrycks & Gimpel 2016), and its derivative.
+__device__ __forceinline__ void gelu_fwd(float x, float& g, float& dg) {
+  const float c = 0.7978845608028654f, a = 0.044715f;
+  float x2 = x * x;
+  float th = tanhf(c * (x + a * x2 * x));
+  g = 0.5f * x * (1.f + th);
+  dg = 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * c * (1.f + 3.f * a * x2);
+}
+
+// 16x16 tile of A (rows m0.., K contiguous, leading dim lda) times B^T given as
+// Bt (rows n0.., K contiguous, leading dim ldb), K a multiple of 32.
+// Lane l holds D[m0 + 4(l/16) + e][n0 + l%16], e = 0..3.
+__device__ __forceinline__ f32x4 mfma_tile(const uint16_t* A, int lda, int m0, const uint16_t* Bt, int ldb, int n0,
+                                           int K) {
+  const int l = threadIdx.x & 63;
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  const int r = l & 15, kq = (l >> 4) * 8;
+  for (int k0 = 0; k0 < K; k0 += 32) {
+    bf16x8 a = *(const bf16x8*)(A + (m0 + r) * lda + k0 + kq);
+    bf16x8 b = *(const bf16x8*)(Bt + (n0 + r) * ldb + k0 + kq);
+    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+  }
+  return acc;
+}
+
+// LDS of the forward pass (inference): operands of the wave's 64 pixels.
+struct MlpFwdLds {
+  alignas(16) uint16_t w1t[64 * 32];   // W1^T  [hidden][feature], features >= 9 zero
+  alignas(16) uint16_t w2t[64 * 64];   // W2^T  [out][in]
+  float b1[64], b2[64], w3[64], b3;
+  alignas(16) uint16_t x[64 * 32];     // X     [pixel][feature]
+  alignas(16) uint16_t h1[64 * 64];    // H1    [pixel][hidden]
+  float ys[64];                        // y per pixel
+};
+// Training adds the other operand orientations and the backward buffers.
+struct MlpLds : MlpFwdLds {
+  alignas(16) uint16_t w2[64 * 64];    // W2    [in][out]
+  alignas(16) uint16_t xt[32 * 64];    // X^T   [feature][pixel]
+  alignas(16) uint16_t h1t[64 * 64];   // H1^T  [hidden][pixel]
+  alignas(16) uint16_t dz2[64 * 64];   // dZ2   [pixel][out]
+  alignas(16) uint16_t dz2t[64 * 64];  // dZ2^T [out][pixel]
+  alignas(16) uint16_t dz1t[64 * 64];  // dZ1^T [hidden][pixel]
+  float h2[64 * 65];                   // H2    [pixel][hidden], FP32, padded rows
+  float dy[64];
+  // GELU derivatives at Z1, Z2 in the MFMA output layout: [tile * 4 + element][lane]
+  float dg1[64 * 64];
+  float dg2[64 * 64];
+};
+
+template <bool TRAIN, class L>
+__device__ __forceinline__ void load_weights(L& s, const MlpDevWeights& w) {
+  const int l = threadIdx.x;
+  for (int i = l; i < 64 * 32; i += 64) s.w1t[i] = w.w1t[i];
+  for (int i = l; i < 64 * 64; i += 64) {
+    s.w2t[i] = w.w2t[i];
+    if constexpr (TRAIN) s.w2[i] = w.w2[i];
+  }
+  s.b1[l] = w.b1[l]; s.b2[l] = w.b2[l]; s.w3[l] = w.w3[l];
+  if (l == 0) s.b3 = w.b3[0];
+}
+
+// Forward of the wave's 64 pixels; returns y of the lane's pixel.  With TRAIN
+// the transposed operands, H2 and the GELU derivatives stay in LDS for the
+// backward pass.
+template <bool TRAIN, class L>
+__device__ __forceinline__ float forward(L& s, const float* __restrict__ x, uint32_t base, uint32_t n) {
+  const int l = threadIdx.x;
+  const int col = l & 15, rq = (l >> 4) * 4;
+  // X as bf16; rows past n are zero
+  for (int f = 0; f < 32; f++) {
+    float v = (f < kMlpIn && base + l < n) ? x[(size_t)(base + l) * kMlpIn + f] : 0.f;
+    uint16_t h = f2bf(v);
+    s.x[l * 32 + f] = h;
+    if constexpr (TRAIN) s.xt[f * 64 + l] = h;
+  }
+  __syncthreads();
+  // layer 1
+  for (int t = 0; t < 16; t++) {
+    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+    f32x4 z = mfma_tile(s.x, 32, m0, s.w1t, 32, n0, 32);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int px = m0 + rq + e, hd = n0 + col;
+      float g, dg;
+      gelu_fwd(z[e] + s.b1[hd], g, dg);
+      uint16_t hb = f2bf(g);
+      s.h1[px * 64 + hd] = hb;
+      if constexpr (TRAIN) {
+        s.dg1[(t * 4 + e) * 64 + l] = dg;
+        s.h1t[hd * 64 + px] = hb;
+      }
+    }
+  }
+  __syncthreads();
+  // layer 2, and layer 3 as a reduction over the hidden units: per row block,
+  // each lane sums its 4 column tiles, then the 16 lanes of a column group add
+  // up by butterfly.
+  for (int mb = 0; mb < 4; mb++) {
+    float part[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int t = mb * 4 + c, m0 = mb * 16, n0 = c * 16;
+      f32x4 z = mfma_tile(s.h1, 64, m0, s.w2t, 64, n0, 64);
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int px = m0 + rq + e, hd = n0 + col;
+        float g, dg;
+        gelu_fwd(z[e] + s.b2[hd], g, dg);
+        part[e] = fmaf(g, s.w3[hd], part[e]);
+        if constexpr (TRAIN) {
+          s.dg2[(t * 4 + e) * 64 + l] = dg;
+          s.h2[px * 65 + hd] = g;
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      float v = part[e];
+      for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
+      if (col == e) s.ys[mb * 16 + rq + e] = v;
+    }
+  }
+  __syncthreads();
+  return s.ys[l] + s.b3;
+}
+
+__global__ __launch_bounds__(64) void k_mlp_train(MlpDevWeights w, const float* __restrict__ x,
+                                                  const float* __restrict__ target, uint32_t n, float inv_batch,
+                                                  float* __restrict__ part, float* __restrict__ loss_part) {
+  __shared__ MlpLds s;
+  const int l = threadIdx.x;
+  const uint32_t base = blockIdx.x * 64;
+  const int col = l & 15, rq = (l >> 4) * 4;
+  load_weights<true>(s, w);
+  const float y = forward<true>(s, x, base, n);
+  const bool valid = base + l < n;
+  const float d = valid ? (y - target[base + l]) : 0.f;
+  float* P = part + (size_t)blockIdx.x * kMlpParams;
+  // loss partial (sum of squares, workgroup order)
+  float sq = d * d;
+  for (int o = 32; o >= 1; o >>= 1) sq += __shfl_xor(sq, o, 64);
+  if (l == 0) loss_part[blockIdx.x] = sq;
+  s.dy[l] = 2.f * d * inv_batch;
+  __syncthreads();
+  // dW3 (lane = hidden unit), db3
+  {
+    float g3 = 0.f;
+    for (int px = 0; px < 64; px++) g3 = fmaf(s.h2[px * 65 + l], s.dy[px], g3);
+    P[kMlpOffW3 + l] = g3;
+    float db3 = s.dy[l];
+    for (int o = 32; o >= 1; o >>= 1) db3 += __shfl_xor(db3, o, 64);
+    if (l == 0) P[kMlpOffB3] = db3;
+  }
+  // dZ2 = dy w3^T * gelu'(Z2), both orientations; db2 = column sums
+  float db2[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int t = 0; t < 16; t++) {
+    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int px = m0 + rq + e, hd = n0 + col;
+      float g = s.dy[px] * s.w3[hd] * s.dg2[(t * 4 + e) * 64 + l];
+      db2[t & 3] += g;
+      uint16_t hb = f2bf(g);
+      s.dz2[px * 64 + hd] = hb;
+      s.dz2t[hd * 64 + px] = hb;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    float v = db2[c];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (l < 16) P[kMlpOffB2 + c * 16 + l] = v;
+  }
+  __syncthreads();
+  // dH1 = dZ2 W2^T -> dZ1 = dH1 * gelu'(Z1) (transposed), db1
+  float db1[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll 4
+  for (int t = 0; t < 16; t++) {
+    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+    f32x4 dh = mfma_tile(s.dz2, 64, m0, s.w2, 64, n0, 64);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int px = m0 + rq + e, hd = n0 + col;
+      float g = dh[e] * s.dg1[(t * 4 + e) * 64 + l];
+      db1[t & 3] += g;
+      s.dz1t[hd * 64 + px] = f2bf(g);
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    float v = db1[c];
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (l < 16) P[kMlpOffB1 + c * 16 + l] = v;
+  }
+  __syncthreads();
+  // dW2[in][out] = H1^T dZ2 : A = H1^T [in][px], Bt = dZ2^T [out][px]
+  for (int t = 0; t < 16; t++) {
+    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+    f32x4 g = mfma_tile(s.h1t, 64, m0, s.dz2t, 64, n0, 64);
+#pragma unroll
+    for (int e = 0; e < 4; e++) P[kMlpOffW2 + (m0 + rq + e) * 64 + n0 + col] = g[e];
+  }
+  // dW1[feature][hidden] = X^T dZ1 : A = X^T [feature][px] (rows 0..15), Bt = dZ1^T [hidden][px]
+  for (int c = 0; c < 4; c++) {
+    f32x4 g = mfma_tile(s.xt, 64, 0, s.dz1t, 64, c * 16, 64);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      const int f = rq + e;
+      if (f < kMlpIn) P[kMlpOffW1 + f * 64 + c * 16 + col] = g[e];
+    }
+  }
+}
+
+__global__ __launch_bounds__(64) void k_mlp_infer(MlpDevWeights w, const float* __restrict__ x, uint32_t n,
+                                                  float* __restrict__ y) {
+  __shared__ MlpFwdLds s;
+  const uint32_t base = blockIdx.x * 64;
+  load_weights<false>(s, w);
+  const float out = forward<false>(s, x, base, n);
+  if (base + threadIdx.x < n) y[base + threadIdx.x] = out;
+}
+
+// Writes parameter p (value w) into the BF16/FP32 operand pack.
+__device__ __forceinline__ void mlp_pack_one(MlpDevWeights d, uint32_t p, float w) {
+  if (p < (uint32_t)kMlpOffB1) {                       // W1[f][h] -> W1^T[h][f]
+    uint32_t f = p / 64, h = p % 64;
+    d.w1t[h * 32 + f] = f2bf(w);
+  } else if (p < (uint32_t)kMlpOffW2) {
+    d.b1[p - kMlpOffB1] = w;
+  } else if (p < (uint32_t)kMlpOffB2) {                // W2[i][o]
+    uint32_t q = p - kMlpOffW2, i = q / 64, o = q % 64;
+    uint16_t h = f2bf(w);
+    d.w2[i * 64 + o] = h;
+    d.w2t[o * 64 + i] = h;
+  } else if (p < (uint32_t)kMlpOffW3) {
+    d.b2[p - kMlpOffB2] = w;
+  } else if (p < (uint32_t)kMlpOffB3) {
+    d.w3[p - kMlpOffW3] = w;
+  } else {
+    d.b3[0] = w;
+  }
+}
+
+// Sum of the per-workgroup gradients (workgroup order), Adam, and the BF16
+// operand copies in the layouts the MFMA tiles read.
+__global__ __launch_bounds__(256) void k_mlp_adam(MlpState st, const float* __restrict__ part, uint32_t nwg,
+                                                  const float* __restrict__ loss_part, float lr, float beta1,
+                                                  float beta2, float eps, float bc1, float bc2, int apply) {
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p == 0) {
+    float L = 0.f;
+    for (uint32_t i = 0; i < nwg; i++) L += loss_part[i];
+    st.loss[0] = L;
+  }
+  if (p >= (uint32_t)kMlpParams) return;
+  float g = 0.f;
+  for (uint32_t i = 0; i < nwg; i++) g += part[(size_t)i * kMlpParams + p];
+  st.grad[p] = g;
+  float w = st.w[p];
+  if (apply) {
+    float m = beta1 * st.m[p] + (1.f - beta1) * g;
+    float v = beta2 * st.v[p] + (1.f - beta2) * g * g;
+    st.m[p] = m;
+    st.v[p] = v;
+    w = w - lr * (m / bc1) / (sqrtf(v / bc2) + eps);
+    st.w[p] = w;
+  }
+  mlp_pack_one(st.dev, p, w);
+}
+
+hipError_t launch_mlp_train(const MlpState& st, const float* x, const float* t, uint32_t n, float* part,
+                            float* loss_part, hipStream_t s) {
+  if (!x || !t || !part || !loss_part || n == 0) return hipErrorInvalidValue;
+  uint32_t nwg = (n + 63) / 64;
+  k_mlp_train<<<nwg, 64, 0, s>>>(st.dev, x, t, n, 1.f / (float)n, part, loss_part);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_adam(const MlpState& st, const float* part, uint32_t nwg, const float* loss_part, float lr,
+                           int step, int apply, hipStream_t s) {
+  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+  float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
+  k_mlp_adam<<<(kMlpParams + 255) / 256, 256, 0, s>>>(st, part, nwg, loss_part, lr, b1, b2, eps, bc1, bc2, apply);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_infer(const MlpState& st, const float* x, uint32_t n, float* y, hipStream_t s) {
+  if (!x || !y) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  k_mlp_infer<<<(n + 63) / 64, 64, 0, s>>>(st.dev, x, n, y);
+  return hipGetLastError();
+}
+__global__ void k_mlp_pack(MlpState st) {
+  uint32_t p = blockIdx.x * 256 + threadIdx.x;
+  if (p < (uint32_t)kMlpParams) mlp_pack_one(st.dev, p, st.w[p]);
+}
+hipError_t launch_mlp_pack(const MlpState& st, hipStream_t s) {
+  k_mlp_pack<<<(kMlpParams + 255) / 256, 256, 0, s>>>(st);
+  return hipGetLastError();
+}
+
+// ---- renderer-side feature, target and allocation-signal kernels ----------
+
+__device__ __forceinline__ float luminance(const float* c) { return 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2]; }
+
+__global__ __launch_bounds__(256) void k_mlp_features(const float* __restrict__ result, const float* __restrict__ filtered,
+                                                      const float* __restrict__ err, const float* __restrict__ gpos,
+                                                      const float* __restrict__ gnrm, const float* __restrict__ gdif,
+                                                      const uint32_t* __restrict__ sample_n, float cx, float cy,
+                                                      float cz, uint32_t n, float* __restrict__ x) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  float* o = x + (size_t)i * kMlpIn;
+  o[0] = log1pf(luminance(result + 3 * (size_t)i));
+  o[1] = log1pf(luminance(filtered + 3 * (size_t)i));
+  o[2] = logf(err[i] + 1e-12f) * 0.1f;
+  o[3] = luminance(gdif + 3 * (size_t)i);
+  o[4] = gnrm[3 * (size_t)i];
+  o[5] = gnrm[3 * (size_t)i + 1];
+  o[6] = gnrm[3 * (size_t)i + 2];
+  float dx = gpos[3 * (size_t)i] - cx, dy = gpos[3 * (size_t)i + 1] - cy, dz = gpos[3 * (size_t)i + 2] - cz;
+  o[7] = log1pf(sqrtf(dx * dx + dy * dy + dz * dz)) * 0.2f;
+  o[8] = logf((float)max(sample_n[i], 1u)) * 0.2f;
+}
+
+__global__ __launch_bounds__(256) void k_mlp_target(const float* __restrict__ result, const float* __restrict__ ref,
+                                                    const uint32_t* __restrict__ sample_n, uint32_t n,
+                                                    float* __restrict__ t) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const float* r = result + 3 * (size_t)i;
+  const float* q = ref + 3 * (size_t)i;
+  float d0 = r[0] - q[0], d1 = r[1] - q[1], d2 = r[2] - q[2];
+  float se = (d0 * d0 + d1 * d1 + d2 * d2) / 3.f;
+  t[i] = logf((float)sample_n[i] * se + 1e-12f);
+}
+
+__global__ __launch_bounds__(256) void k_mlp_err(const float* __restrict__ y, const uint32_t* __restrict__ sample_n,
+                                                 uint32_t nd, uint32_t n, float* __restrict__ e) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t N = max(sample_n[i], 1u);
+  const float v = expf(fminf(fmaxf(y[i], -60.f), 60.f));
+  e[i] = (float)nd / (float)(nd + N) * v / (float)N;
+}
+
+hipError_t launch_mlp_features(const float* result, const float* filtered, const float* err, const float* gpos,
+                               const float* gnrm, const float* gdif, const uint32_t* sample_n, const float* cam_pos,
+                               uint32_t n, float* x, hipStream_t s) {
+  if (!result || !filtered || !err || !gpos || !gnrm || !gdif || !sample_n || !cam_pos || !x)
+    return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  k_mlp_features<<<(n + 255) / 256, 256, 0, s>>>(result, filtered, err, gpos, gnrm, gdif, sample_n, cam_pos[0],
+                                                 cam_pos[1], cam_pos[2], n, x);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_target(const float* result, const float* ref, const uint32_t* sample_n, uint32_t n, float* t,
+                             hipStream_t s) {
+  if (!result || !ref || !sample_n || !t) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  k_mlp_target<<<(n + 255) / 256, 256, 0, s>>>(result, ref, sample_n, n, t);
+  return hipGetLastError();
+}
+hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd, uint32_t n, float* e, hipStream_t s) {
+  if (!y || !sample_n || !e) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  k_mlp_err<<<(n + 255) / 256, 256, 0, s>>>(y, sample_n, nd, n, e);
+  return hipGetLastError();
+}
+
+size_t mlp_lds_bytes() { return sizeof(MlpLds); }
+
+}  // namespace cpt

@@ -186,7 +186,9 @@ enum {
   CPT_BUF_GBUF_DIFF = 4,  /* float x3: first-hit albedo */
   CPT_BUF_FILTERED = 5,   /* float x3: cross-bilateral output */
   CPT_BUF_ERROR = 6,      /* float x1: per-pixel error estimate */
-  CPT_BUF_SAMPLE_COUNTS = 7  /* uint32: samples drawn per pixel in the last adaptive frame */
+  CPT_BUF_SAMPLE_COUNTS = 7, /* uint32: samples drawn per pixel in the last adaptive frame */
+  CPT_BUF_ALLOC_ERROR = 8    /* float x1: the error signal the last allocation used (the learned
+                                estimate when a model is attached, else CPT_BUF_ERROR) */
 };
 int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out);
 
@@ -198,6 +200,7 @@ typedef struct cpt_stats {
   uint64_t extend_launches, extend_nodes, extend_tris;
   double extend_ms, shadow_ms, shade_ms, light_ms, total_ms;
   double filter_ms, error_ms, gen_ms, accum_ms;   /* adaptive phases (PERFBREAKDOWN, AdapBilatAtomic.cu:1409-1536) */
+  double mlp_ms;                                  /* learned error model: features + inference */
 } cpt_stats;
 int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset);
 
@@ -206,6 +209,57 @@ int cpt_renderer_set_profiling(cpt_renderer* r, int on);
 
 /* <ns>::CleanMem (src/raytracer.cpp:25) + destruction. */
 int cpt_renderer_destroy(cpt_renderer* r);
+
+/* ---- Learned per-pixel error model ----------------------------------------
+ * No counterpart in the reference: CudaPT includes and links FANN
+ * (src/cudaRTBDPTStreamBilat.cu:11-12, CudaPT.vcxproj:129) but calls no fann_*
+ * function.  The model replaces the filter-residual error estimate of the
+ * adaptive renderer (src/cudaRTBDPTStreamAdapBilatAtomic.cu:1143-1156) as the
+ * signal the per-pixel sample allocation is proportional to.
+ *
+ * Network 9 -> 64 -> 64 -> 1, GELU (tanh form) after both hidden layers,
+ * BF16 matrix operands with FP32 accumulation, FP32 master weights, Adam
+ * (beta1 0.9, beta2 0.999, eps 1e-8).  Parameters are one flat FP32 vector of
+ * CPT_MLP_PARAMS entries, row-major with the input index first:
+ *   W1[9][64] | b1[64] | W2[64][64] | b2[64] | w3[64] | b3. */
+typedef struct cpt_mlp cpt_mlp;
+enum { CPT_MLP_INPUTS = 9, CPT_MLP_HIDDEN = 64, CPT_MLP_PARAMS = 4865 };
+
+/* Creates a model with Glorot-uniform weights drawn from PCG32 stream `seed`
+ * (W1, W2, w3 in that order, each entry (2u - 1) * sqrt(6 / (fan_in + fan_out))
+ * with u = uniform01 of the stream) and zero biases. */
+int cpt_mlp_create(cpt_mlp** out, uint64_t seed);
+int cpt_mlp_destroy(cpt_mlp* m);
+/* Host copies of the parameters (resets the Adam moments and step). */
+int cpt_mlp_set_params(cpt_mlp* m, const float* params, uint32_t n);
+int cpt_mlp_get_params(cpt_mlp* m, float* params, uint32_t n);
+/* Gradient of the last cpt_mlp_train_step (mean squared error). */
+int cpt_mlp_get_grad(cpt_mlp* m, float* grad, uint32_t n);
+/* One step over a device batch: x is n x 9 row-major FP32, target n FP32.
+ * Computes loss = mean (y - target)^2 and its gradient; with apply != 0 takes
+ * one Adam step of rate lr.  *loss (host, may be NULL) receives the loss and
+ * makes the call synchronous.  stream NULL = the model's own stream. */
+int cpt_mlp_train_step(cpt_mlp* m, const float* d_x, const float* d_target, uint32_t n, float lr, int apply,
+                       float* loss, void* stream);
+/* Inference of n samples into d_y (device FP32). */
+int cpt_mlp_infer(cpt_mlp* m, const float* d_x, uint32_t n, float* d_y, void* stream);
+
+/* Per-pixel features of an adaptive renderer's current state (after at least
+ * one adaptive frame), W*H x 9 FP32 into device memory, with L = Rec.709
+ * luminance, r = running mean, f = filtered image, e = filter-residual error,
+ * a = first-hit albedo, n = first-hit normal, p = first-hit position, c = camera:
+ *   log1p(L(r)), log1p(L(f)), log(e + 1e-12) / 10, L(a), n.x, n.y, n.z,
+ *   log1p(|p - c|) / 5, log(N) / 5. */
+int cpt_renderer_mlp_features(cpt_renderer* r, float* d_x);
+/* Training target of each pixel against a converged image ref (device,
+ * W*H x 3 FP32): log(N * |r - ref|^2 / 3 + 1e-12), an estimate of the log
+ * per-sample variance. */
+int cpt_renderer_mlp_target(cpt_renderer* r, const float* d_ref, float* d_t);
+/* Attaches a model (NULL detaches): each adaptive frame then allocates its
+ * samples proportional to e' = nd / (nd + N) * exp(y) / N, y the model's output
+ * on the features above, in place of the filter-residual estimate.  The model
+ * must outlive the attachment. */
+int cpt_renderer_set_mlp(cpt_renderer* r, cpt_mlp* m);
 
 #ifdef __cplusplus
 }
