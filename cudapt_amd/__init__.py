@@ -90,7 +90,7 @@ def lib():
             "cpt_mlp_train_step": [vp, vp, vp, c32, f32, i32, vp, vp],
             "cpt_mlp_infer": [vp, vp, c32, vp, vp],
             "cpt_renderer_mlp_features": [vp, vp],
-            "cpt_renderer_mlp_target": [vp, vp, vp],
+            "cpt_renderer_mlp_target": [vp, vp, f32, vp],
             "cpt_renderer_set_mlp": [vp, vp],
         }
         for name, args in sigs.items():
@@ -273,7 +273,8 @@ class Renderer:
                "seed": "Seed", "literal_shadow": "Literal Shadow Endpoint", "warmup": "Warm-up Frames",
                "nd": "Desired Max Sampling", "sigma_color": "Filter Color EuD", "sigma_pos": "Filter Pos EuD",
                "sigma_norm": "Filter Norm EuD", "sigma_diff": "Filter Diff EuD", "radius": "Filter Radius",
-               "filter_diff": "Filter Diffuse Flag"}.get(name, name)
+               "filter_diff": "Filter Diffuse Flag", "in_flight": "Frames In Flight",
+               "pass_frames": "Adaptive Pass Frames"}.get(name, name)
         idx = self.attribs()[key][0]
         _check(lib().cpt_renderer_set_attrib(self.h, idx, float(value)), "cpt_renderer_set_attrib")
 
@@ -312,11 +313,13 @@ class Renderer:
             raise CptError("feature buffer smaller than W*H x 9 floats")
         _check(lib().cpt_renderer_mlp_features(self.h, ctypes.c_void_p(d_x.ptr)), "cpt_renderer_mlp_features")
 
-    def mlp_target(self, d_ref: "DeviceBuffer", d_t: "DeviceBuffer"):
-        """Writes the W*H training targets against the converged image in d_ref into d_t."""
+    def mlp_target(self, d_ref: "DeviceBuffer", d_t: "DeviceBuffer", clamp: float = 0.0):
+        """Writes the W*H training targets against the converged image in d_ref into d_t
+        (radiance clamped to `clamp` on both sides when clamp > 0)."""
         if d_ref.nbytes < 12 * self.W * self.H or d_t.nbytes < 4 * self.W * self.H:
             raise CptError("reference or target buffer smaller than the image")
-        _check(lib().cpt_renderer_mlp_target(self.h, ctypes.c_void_p(d_ref.ptr), ctypes.c_void_p(d_t.ptr)),
+        _check(lib().cpt_renderer_mlp_target(self.h, ctypes.c_void_p(d_ref.ptr), float(clamp),
+                                             ctypes.c_void_p(d_t.ptr)),
                "cpt_renderer_mlp_target")
 
     def set_mlp(self, model: "Mlp | None"):

@@ -435,6 +435,15 @@ const char* const kSeed = "Seed";
 const char* const kLiteralShadow = "Literal Shadow Endpoint";
 const char* const kWarmup = "Warm-up Frames";
 const char* const kInFlight = "Frames In Flight";
+// adaptive mode: frames' worth of samples (k * W * H) one adaptive pass
+// allocates from one error map and traces as one wavefront (1 = the
+// reference's one frame per allocation)
+const char* const kPassFrames = "Adaptive Pass Frames";
+// allocation signal from the learned model's output y (log per-sample
+// variance): 0 = nd / (nd + N) * exp(y) / N, the filter-residual estimate's
+// shape; 1 = exp(y) / N, the expected squared error (samples settle where N
+// grows with the per-sample standard deviation)
+const char* const kLearnedSignal = "Learned Signal";
 const char* const kLdsStack = "Traversal LDS Stack";
 const char* const kLdsStackShadow = "Shadow LDS Stack";
 constexpr u32 kMaxInFlight = 90;   // 30 three-frame light-cache groups + partial ones at either end
@@ -506,7 +515,7 @@ struct cpt_renderer {
   }
   // adaptive mode
   DevArray<float> gpos, gnrm, gdif, filtered, err, gauss, err_sum;
-  DevArray<u32> counts, offsets, block_tot, total;
+  DevArray<u32> counts, offsets, block_tot, total, big_list, big_n;
   DevArray<double> block_sums;
   DevArray<float4> rec_acc, rec_pos, rec_nrm, rec_dif;
   float gauss_sigma = -1.f;
@@ -530,7 +539,8 @@ int renderer_alloc(cpt_renderer* r) {
   // adaptive frames draw floor(W*H*e_i/sum e) samples: at most W*H plus float
   // rounding of the ratios, bounded by the margin (and clamped on the device);
   // plain frames trace "Frames In Flight" frames as one wavefront
-  size_t cap = r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL ? n + 1024 : n * (size_t)r->attr(kInFlight);
+  size_t cap = r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL ? n * (size_t)r->attr(kPassFrames) + 1024
+                                                     : n * (size_t)r->attr(kInFlight);
   if (cap > 0xFFFFFFFFull / 2) return fail(CPT_E_LIMIT, "render: %zu path slots exceed 2^31", cap);
   r->cap = (u32)cap;
   r->forget_light_caches();
@@ -556,6 +566,8 @@ int renderer_alloc(cpt_renderer* r) {
     HIP_TRY(r->rec_pos.alloc(cap));
     HIP_TRY(r->rec_nrm.alloc(cap));
     HIP_TRY(r->rec_dif.alloc(cap));
+    HIP_TRY(r->big_list.alloc(n));
+    HIP_TRY(r->big_n.alloc(1));
     for (auto* a : {&r->gpos, &r->gnrm, &r->gdif, &r->filtered})
       HIP_TRY(hipMemsetAsync(a->p, 0, sizeof(float) * n * 3, r->stream));
     HIP_TRY(hipMemsetAsync(r->err.p, 0, sizeof(float) * n, r->stream));
@@ -724,7 +736,7 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
       HIP_TRY(launch_mlp_features(r->result.p, r->filtered.p, r->err.p, r->gpos.p, r->gnrm.p, r->gdif.p,
                                   r->sample_n.p, r->cam, n, r->mlp_x.p, r->stream));
       HIP_TRY(launch_mlp_infer(r->mlp->state(), r->mlp_x.p, n, r->mlp_y.p, r->stream));
-      HIP_TRY(launch_mlp_err(r->mlp_y.p, r->sample_n.p, nd, n, r->mlp_e.p, r->stream));
+      HIP_TRY(launch_mlp_err(r->mlp_y.p, r->sample_n.p, nd, (u32)r->attr(kLearnedSignal), n, r->mlp_e.p, r->stream));
       HIP_TRY(timed_end(r, 9, ev));
       alloc_err = r->mlp_e.p;
     }
@@ -732,8 +744,8 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
     HIP_TRY(launch_error_sum(alloc_err, n, r->block_sums.p, r->err_sum.p, r->stream));
     HIP_TRY(timed_end(r, 6, ev));
     HIP_TRY(timed_begin(r, 7, &ev));
-    HIP_TRY(launch_alloc_scan(alloc_err, r->err_sum.p, n, n, r->counts.p, r->offsets.p, r->block_tot.p, r->total.p,
-                              r->cap, r->stream));
+    HIP_TRY(launch_alloc_scan(alloc_err, r->err_sum.p, n, n * fa.nframes, r->counts.p, r->offsets.p, r->block_tot.p,
+                              r->total.p, r->cap, r->stream));
     HIP_TRY(launch_frame_begin_dev(r->fc.p, r->total.p, r->cap, r->stream));
     HIP_TRY(launch_gen_adaptive(fa, r->offsets.p, r->counts.p, r->total.p, r->cap, r->A.view(), ac.rec, r->stream));
     HIP_TRY(timed_end(r, 7, ev));
@@ -742,10 +754,10 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
     if (rc) return rc;
     HIP_TRY(timed_begin(r, 8, &ev));
     HIP_TRY(launch_accum_adaptive(r->offsets.p, r->counts.p, n, r->cap, ac.rec, r->result.p, r->sample_n.p, r->gpos.p,
-                                  r->gnrm.p, r->gdif.p, r->stream));
+                                  r->gnrm.p, r->gdif.p, r->big_list.p, r->big_n.p, r->stream));
     HIP_TRY(timed_end(r, 8, ev));
   }
-  HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, 1, fa.eye_depth, r->stream));
+  HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, fa.nframes, fa.eye_depth, r->stream));
   return CPT_OK;
 }
 
@@ -799,10 +811,12 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   if (eye_depth < 1 || eye_depth > (u32)kMaxDepth) return fail(CPT_E_LIMIT, "render: eye depth %u not in 1..15", eye_depth);
   if (light_depth < 1 || light_depth > 64) return fail(CPT_E_LIMIT, "render: light depth %u not in 1..64", light_depth);
   if (light_paths < 1 || light_paths > (u32)kBlock) return fail(CPT_E_LIMIT, "render: light paths %u not in 1..256", light_paths);
-  u32 in_flight = r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL ? 1u : (u32)r->attr(kInFlight);
+  const bool adaptive = r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL;
+  u32 in_flight = (u32)r->attr(adaptive ? kPassFrames : kInFlight);
   if (in_flight < 1 || in_flight > kMaxInFlight)
-    return fail(CPT_E_LIMIT, "render: frames in flight %u not in 1..%u", in_flight, kMaxInFlight);
-  if (!r->allocated || (size_t)r->cap < (size_t)r->W * r->H * in_flight) {
+    return fail(CPT_E_LIMIT, "render: %s %u not in 1..%u", adaptive ? kPassFrames : kInFlight, in_flight,
+                kMaxInFlight);
+  if (!r->allocated || (size_t)r->cap < (size_t)r->W * r->H * in_flight + (adaptive ? 1024 : 0)) {
     int rc = renderer_alloc(r);
     if (rc) return rc;
   }
@@ -836,6 +850,7 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   u32 next = (reset || !r->started) ? 0u : r->frame + 1;   // frame counter (:826-828)
   for (u32 done = 0; done < nframes;) {
     u32 k = std::min(in_flight, nframes - done);
+    if (adaptive && next < (u32)r->attr(kWarmup)) k = 1;   // warm-up frames one at a time
     fa.frame0 = next;
     fa.nframes = k;
     rc = render_batch(r, s, fa);
@@ -882,8 +897,13 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   r->attribs.push_back({kLightPaths, 0, 128});
   r->attribs.push_back({kSeed, 0, (double)0x5EED0001u});
   r->attribs.push_back({kLiteralShadow, 2, 0});
-  if (mode == CPT_MODE_BDPT_ADAPTIVE_PLL) r->attribs.push_back({kWarmup, 0, 5});
-  else r->attribs.push_back({kInFlight, 0, 24});
+  if (mode == CPT_MODE_BDPT_ADAPTIVE_PLL) {
+    r->attribs.push_back({kWarmup, 0, 5});
+    r->attribs.push_back({kPassFrames, 0, 1});
+    r->attribs.push_back({kLearnedSignal, 0, 0});
+  } else {
+    r->attribs.push_back({kInFlight, 0, 24});
+  }
   // traversal kernel variant: 8 LDS stack entries, >= 6 waves per SIMD
   // (swept on the sphere grid: 1.58 vs 1.36 Grays/s for 16 entries)
   r->attribs.push_back({kLdsStack, 0, 806});
@@ -1173,10 +1193,12 @@ int cpt_renderer_mlp_features(cpt_renderer* r, float* d_x) {
   return CPT_OK;
 }
 
-int cpt_renderer_mlp_target(cpt_renderer* r, const float* d_ref, float* d_t) {
+int cpt_renderer_mlp_target(cpt_renderer* r, const float* d_ref, float clamp, float* d_t) {
   if (!r || !d_ref || !d_t) return fail(CPT_E_ARG, "cpt_renderer_mlp_target: null argument");
   if (!r->allocated) return fail(CPT_E_STATE, "cpt_renderer_mlp_target: renderer has not rendered");
-  HIP_TRY(launch_mlp_target(r->result.p, d_ref, r->sample_n.p, r->W * r->H, d_t, r->stream));
+  if (!(clamp >= 0.f)) return fail(CPT_E_ARG, "cpt_renderer_mlp_target: clamp must be >= 0");
+  HIP_TRY(launch_mlp_target(r->result.p, d_ref, r->sample_n.p, r->W * r->H, clamp > 0.f ? clamp : INFINITY, d_t,
+                            r->stream));
   HIP_TRY(hipStreamSynchronize(r->stream));
   return CPT_OK;
 }

@@ -157,7 +157,7 @@ hipError_t launch_gen_adaptive(const FrameArgs& fa, const uint32_t* offsets, con
                                const uint32_t* total, uint32_t cap, PathBuf pb, SampleRec rec, hipStream_t st);
 hipError_t launch_accum_adaptive(const uint32_t* offsets, const uint32_t* counts, uint32_t n, uint32_t cap,
                                  SampleRec rec, float* result, uint32_t* sample_n, float* gpos, float* gnrm,
-                                 float* gdif, hipStream_t st);
+                                 float* gdif, uint32_t* big_list, uint32_t* big_n, hipStream_t st);
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
                          uint32_t max_n, int lds_stack, hipStream_t st);
 hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes, uint32_t depth, hipStream_t st);

@@ -1171,32 +1171,11 @@ hipError_t launch_gen_adaptive(const FrameArgs& fa, const uint32_t* offsets, con
   return hipGetLastError();
 }
 
-// pt_atomicApplyPathQueueResult_kernel + pt_accumTempResultToResult_kernel
-// (AdapBilatAtomic.cu:1041-1141) with the per-pixel sums taken over the
-// pixel's samples in queue order instead of float atomics.
-__global__ __launch_bounds__(kBlock) void k_accum_adaptive(const uint32_t* __restrict__ offsets,
-                                                           const uint32_t* __restrict__ counts, uint32_t n,
-                                                           uint32_t cap, SampleRec rec, float* __restrict__ result,
-                                                           uint32_t* __restrict__ sample_n, float* __restrict__ gpos,
-                                                           float* __restrict__ gnrm, float* __restrict__ gdif) {
-  const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
-  if (pix >= n) return;
-  uint32_t q0 = offsets[pix], q1 = q0 + counts[pix];
-  if (q1 > cap) q1 = cap;
-  f3 ts = mk3(0.f, 0.f, 0.f), tp = ts, tn = ts, td = ts;
-  uint32_t tN = 0;
-  for (uint32_t q = q0; q < q1; q++) {
-    float4 a = rec.acc[q];
-    uint32_t N = __float_as_uint(a.w);
-    f3 acc = ld3(a);
-    if (N == 0 || isinf3(acc)) continue;
-    float nf = (float)N;
-    ts = ts + acc;
-    tN += N;
-    tp = tp + ld3(rec.gpos[q]) * nf;
-    tn = tn + ld3(rec.gnrm[q]) * nf;
-    td = td + ld3(rec.gdif[q]) * nf;
-  }
+// Running-mean update of one pixel from the sums of its new records.
+__device__ __forceinline__ void accum_apply(uint32_t pix, uint32_t tN, f3 ts, f3 tp, f3 tn, f3 td,
+                                            float* __restrict__ result, uint32_t* __restrict__ sample_n,
+                                            float* __restrict__ gpos, float* __restrict__ gnrm,
+                                            float* __restrict__ gdif) {
   uint32_t nold = sample_n[pix], nn = tN + nold;
   if (nn > nold) {
     float ri = 1.f / (float)nn, oi = (float)nold * ri;
@@ -1215,13 +1194,110 @@ __global__ __launch_bounds__(kBlock) void k_accum_adaptive(const uint32_t* __res
     }
   }
 }
+
+// Adds record q to the running sums (records with N = 0 or an infinite
+// component are skipped, AdapBilatAtomic.cu:1056-1060).
+__device__ __forceinline__ void accum_record(const SampleRec& rec, uint32_t q, f3& ts, f3& tp, f3& tn, f3& td,
+                                             uint32_t& tN) {
+  float4 a = rec.acc[q];
+  uint32_t N = __float_as_uint(a.w);
+  f3 acc = ld3(a);
+  if (N == 0 || isinf3(acc)) return;
+  float nf = (float)N;
+  ts = ts + acc;
+  tN += N;
+  tp = tp + ld3(rec.gpos[q]) * nf;
+  tn = tn + ld3(rec.gnrm[q]) * nf;
+  td = td + ld3(rec.gdif[q]) * nf;
+}
+
+// Pixels with more records than this are summed by a whole workgroup.
+constexpr uint32_t kAccumSerialMax = 32;
+
+__global__ void k_fill_u32(uint32_t* p, uint32_t v) { *p = v; }
+
+// pt_atomicApplyPathQueueResult_kernel + pt_accumTempResultToResult_kernel
+// (AdapBilatAtomic.cu:1041-1141) with the per-pixel sums taken over the
+// pixel's samples in queue order instead of float atomics.  Pixels holding
+// more than kAccumSerialMax records are listed for k_accum_big.
+__global__ __launch_bounds__(kBlock) void k_accum_adaptive(const uint32_t* __restrict__ offsets,
+                                                           const uint32_t* __restrict__ counts, uint32_t n,
+                                                           uint32_t cap, SampleRec rec, float* __restrict__ result,
+                                                           uint32_t* __restrict__ sample_n, float* __restrict__ gpos,
+                                                           float* __restrict__ gnrm, float* __restrict__ gdif,
+                                                           uint32_t* __restrict__ big_list,
+                                                           uint32_t* __restrict__ big_n) {
+  const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
+  if (pix >= n) return;
+  uint32_t q0 = offsets[pix], q1 = q0 + counts[pix];
+  if (q1 > cap) q1 = cap;
+  if (q1 > q0 + kAccumSerialMax) {
+    big_list[atomicAdd(big_n, 1u)] = pix;
+    return;
+  }
+  f3 ts = mk3(0.f, 0.f, 0.f), tp = ts, tn = ts, td = ts;
+  uint32_t tN = 0;
+  for (uint32_t q = q0; q < q1; q++) accum_record(rec, q, ts, tp, tn, td, tN);
+  accum_apply(pix, tN, ts, tp, tn, td, result, sample_n, gpos, gnrm, gdif);
+}
+
+// Heavily sampled pixels: one workgroup per pixel, thread t sums records
+// q0 + t, q0 + t + 256, ..., then a fixed butterfly within each wave and the
+// four wave sums in order (deterministic for a given record count).
+__global__ __launch_bounds__(kBlock) void k_accum_big(const uint32_t* __restrict__ offsets,
+                                                      const uint32_t* __restrict__ counts, uint32_t cap, SampleRec rec,
+                                                      float* __restrict__ result, uint32_t* __restrict__ sample_n,
+                                                      float* __restrict__ gpos, float* __restrict__ gnrm,
+                                                      float* __restrict__ gdif, const uint32_t* __restrict__ big_list,
+                                                      const uint32_t* __restrict__ big_n) {
+  __shared__ float red[kBlock / 64][13];
+  __shared__ uint32_t redn[kBlock / 64];
+  const uint32_t nb = *big_n;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (uint32_t i = blockIdx.x; i < nb; i += gridDim.x) {
+    const uint32_t pix = big_list[i];
+    uint32_t q0 = offsets[pix], q1 = q0 + counts[pix];
+    if (q1 > cap) q1 = cap;
+    f3 ts = mk3(0.f, 0.f, 0.f), tp = ts, tn = ts, td = ts;
+    uint32_t tN = 0;
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBlock) accum_record(rec, q, ts, tp, tn, td, tN);
+    float v[12] = {ts.x, ts.y, ts.z, tp.x, tp.y, tp.z, tn.x, tn.y, tn.z, td.x, td.y, td.z};
+#pragma unroll
+    for (int k = 0; k < 12; k++)
+      for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
+    for (int o = 32; o >= 1; o >>= 1) tN += __shfl_xor(tN, o, 64);
+    if (lane == 0) {
+#pragma unroll
+      for (int k = 0; k < 12; k++) red[wave][k] = v[k];
+      redn[wave] = tN;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      float a[12];
+      uint32_t sN = 0;
+#pragma unroll
+      for (int k = 0; k < 12; k++) a[k] = 0.f;
+      for (int w = 0; w < kBlock / 64; w++) {
+#pragma unroll
+        for (int k = 0; k < 12; k++) a[k] += red[w][k];
+        sN += redn[w];
+      }
+      accum_apply(pix, sN, mk3(a[0], a[1], a[2]), mk3(a[3], a[4], a[5]), mk3(a[6], a[7], a[8]),
+                  mk3(a[9], a[10], a[11]), result, sample_n, gpos, gnrm, gdif);
+    }
+    __syncthreads();
+  }
+}
+
 hipError_t launch_accum_adaptive(const uint32_t* offsets, const uint32_t* counts, uint32_t n, uint32_t cap,
                                  SampleRec rec, float* result, uint32_t* sample_n, float* gpos, float* gnrm,
-                                 float* gdif, hipStream_t st) {
-  if (!rec_ok(rec) || !offsets || !counts || !result || !sample_n || !gpos || !gnrm || !gdif)
+                                 float* gdif, uint32_t* big_list, uint32_t* big_n, hipStream_t st) {
+  if (!rec_ok(rec) || !offsets || !counts || !result || !sample_n || !gpos || !gnrm || !gdif || !big_list || !big_n)
     return hipErrorInvalidValue;
+  k_fill_u32<<<1, 1, 0, st>>>(big_n, 0u);
   k_accum_adaptive<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(offsets, counts, n, cap, rec, result, sample_n, gpos,
-                                                                  gnrm, gdif);
+                                                                  gnrm, gdif, big_list, big_n);
+  k_accum_big<<<1024, kBlock, 0, st>>>(offsets, counts, cap, rec, result, sample_n, gpos, gnrm, gdif, big_list, big_n);
   return hipGetLastError();
 }
 

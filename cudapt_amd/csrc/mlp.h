@@ -59,8 +59,11 @@ size_t mlp_lds_bytes();
 hipError_t launch_mlp_features(const float* result, const float* filtered, const float* err, const float* gpos,
                                const float* gnrm, const float* gdif, const uint32_t* sample_n, const float* cam_pos,
                                uint32_t n, float* x, hipStream_t s);
-hipError_t launch_mlp_target(const float* result, const float* ref, const uint32_t* sample_n, uint32_t n, float* t,
-                             hipStream_t s);
-hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd, uint32_t n, float* e, hipStream_t s);
+hipError_t launch_mlp_target(const float* result, const float* ref, const uint32_t* sample_n, uint32_t n, float clamp,
+                             float* t, hipStream_t s);
+// signal 0: nd / (nd + N) * exp(y) / N (the filter-residual estimate's shape);
+// signal 1: exp(y) / N (the pixel's expected squared error).
+hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd, uint32_t signal, uint32_t n, float* e,
+                          hipStream_t s);
 
 }  // namespace cpt

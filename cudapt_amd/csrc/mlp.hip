@@ -358,24 +358,25 @@ __global__ __launch_bounds__(256) void k_mlp_features(const float* __restrict__ 
 }
 
 __global__ __launch_bounds__(256) void k_mlp_target(const float* __restrict__ result, const float* __restrict__ ref,
-                                                    const uint32_t* __restrict__ sample_n, uint32_t n,
+                                                    const uint32_t* __restrict__ sample_n, uint32_t n, float clamp,
                                                     float* __restrict__ t) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const float* r = result + 3 * (size_t)i;
   const float* q = ref + 3 * (size_t)i;
-  float d0 = r[0] - q[0], d1 = r[1] - q[1], d2 = r[2] - q[2];
+  float d0 = fminf(r[0], clamp) - fminf(q[0], clamp), d1 = fminf(r[1], clamp) - fminf(q[1], clamp);
+  float d2 = fminf(r[2], clamp) - fminf(q[2], clamp);
   float se = (d0 * d0 + d1 * d1 + d2 * d2) / 3.f;
   t[i] = logf((float)sample_n[i] * se + 1e-12f);
 }
 
 __global__ __launch_bounds__(256) void k_mlp_err(const float* __restrict__ y, const uint32_t* __restrict__ sample_n,
-                                                 uint32_t nd, uint32_t n, float* __restrict__ e) {
+                                                 uint32_t nd, uint32_t signal, uint32_t n, float* __restrict__ e) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const uint32_t N = max(sample_n[i], 1u);
   const float v = expf(fminf(fmaxf(y[i], -60.f), 60.f));
-  e[i] = (float)nd / (float)(nd + N) * v / (float)N;
+  e[i] = signal ? v / (float)N : (float)nd / (float)(nd + N) * v / (float)N;
 }
 
 hipError_t launch_mlp_features(const float* result, const float* filtered, const float* err, const float* gpos,
@@ -388,17 +389,18 @@ hipError_t launch_mlp_features(const float* result, const float* filtered, const
                                                  cam_pos[1], cam_pos[2], n, x);
   return hipGetLastError();
 }
-hipError_t launch_mlp_target(const float* result, const float* ref, const uint32_t* sample_n, uint32_t n, float* t,
-                             hipStream_t s) {
+hipError_t launch_mlp_target(const float* result, const float* ref, const uint32_t* sample_n, uint32_t n, float clamp,
+                             float* t, hipStream_t s) {
   if (!result || !ref || !sample_n || !t) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
-  k_mlp_target<<<(n + 255) / 256, 256, 0, s>>>(result, ref, sample_n, n, t);
+  k_mlp_target<<<(n + 255) / 256, 256, 0, s>>>(result, ref, sample_n, n, clamp, t);
   return hipGetLastError();
 }
-hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd, uint32_t n, float* e, hipStream_t s) {
+hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd, uint32_t signal, uint32_t n, float* e,
+                          hipStream_t s) {
   if (!y || !sample_n || !e) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
-  k_mlp_err<<<(n + 255) / 256, 256, 0, s>>>(y, sample_n, nd, n, e);
+  k_mlp_err<<<(n + 255) / 256, 256, 0, s>>>(y, sample_n, nd, signal, n, e);
   return hipGetLastError();
 }
 

@@ -253,12 +253,14 @@ int cpt_mlp_infer(cpt_mlp* m, const float* d_x, uint32_t n, float* d_y, void* st
 int cpt_renderer_mlp_features(cpt_renderer* r, float* d_x);
 /* Training target of each pixel against a converged image ref (device,
  * W*H x 3 FP32): log(N * |r - ref|^2 / 3 + 1e-12), an estimate of the log
- * per-sample variance. */
-int cpt_renderer_mlp_target(cpt_renderer* r, const float* d_ref, float* d_t);
+ * per-sample variance; with clamp > 0 both r and ref are clamped to clamp
+ * per channel first (the error of the displayed image). */
+int cpt_renderer_mlp_target(cpt_renderer* r, const float* d_ref, float clamp, float* d_t);
 /* Attaches a model (NULL detaches): each adaptive frame then allocates its
- * samples proportional to e' = nd / (nd + N) * exp(y) / N, y the model's output
- * on the features above, in place of the filter-residual estimate.  The model
- * must outlive the attachment. */
+ * samples proportional to e' = nd / (nd + N) * exp(y) / N (attribute "Learned
+ * Signal" 0) or e' = exp(y) / N (1), y the model's output on the features
+ * above, in place of the filter-residual estimate.  The model must outlive the
+ * attachment. */
 int cpt_renderer_set_mlp(cpt_renderer* r, cpt_mlp* m);
 
 #ifdef __cplusplus

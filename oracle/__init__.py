@@ -78,7 +78,7 @@ RENDER_PARAMS_DTYPE = np.dtype([
     ("fov", "<f4"), ("eye_depth", "<u4"), ("light_depth", "<u4"), ("light_paths", "<u4"), ("flags", "<u4"),
     ("seed", "<u8"), ("row0", "<u4"), ("rows", "<u4"),
     ("mode", "<u4"), ("nd", "<u4"), ("sc", "<f4"), ("sp", "<f4"), ("sn", "<f4"), ("sd", "<f4"),
-    ("radius", "<u4"), ("filter_diff", "<u4"), ("warmup", "<u4"), ("pad2", "<u4")])
+    ("radius", "<u4"), ("filter_diff", "<u4"), ("warmup", "<u4"), ("pass_frames", "<u4")])
 
 
 def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:
@@ -139,7 +139,7 @@ def max_threads() -> int:
 
 def render_params(width, height, camera, eye_depth=5, light_depth=5, light_paths=128, seed=0x5EED0001, flags=0,
                   row0=0, rows=0, mode=0, nd=5, sc=50.0, sp=1.0, sn=0.25, sd=0.1, radius=5, filter_diff=0,
-                  warmup=5):
+                  warmup=5, pass_frames=1):
     """Reference-estimator parameters; mode 1 = adaptive proportional allocation with the
     reference's defaults (AdapBilatAtomic.cu:28-38)."""
     p = np.zeros((), RENDER_PARAMS_DTYPE)
@@ -152,6 +152,7 @@ def render_params(width, height, camera, eye_depth=5, light_depth=5, light_paths
     p["row0"], p["rows"] = row0, rows
     p["mode"], p["nd"], p["sc"], p["sp"], p["sn"], p["sd"] = mode, nd, sc, sp, sn, sd
     p["radius"], p["filter_diff"], p["warmup"] = radius, filter_diff, warmup
+    p["pass_frames"] = pass_frames
     return p
 
 

@@ -164,13 +164,17 @@ def features(result, filtered, err, gpos, gnrm, gdif, sample_n, cam_pos) -> np.n
     return x
 
 
-def target(result, ref, sample_n) -> np.ndarray:
-    """log(N * |r - ref|^2 / 3 + 1e-12): log per-sample variance estimate."""
-    d = np.asarray(result, np.float64).reshape(-1, 3) - np.asarray(ref, np.float64).reshape(-1, 3)
+def target(result, ref, sample_n, clamp: float = 0.0) -> np.ndarray:
+    """log(N * |r - ref|^2 / 3 + 1e-12): log per-sample variance estimate
+    (r and ref clamped to `clamp` per channel when clamp > 0)."""
+    c = clamp if clamp > 0 else np.inf
+    d = (np.minimum(np.asarray(result, np.float64).reshape(-1, 3), c) -
+         np.minimum(np.asarray(ref, np.float64).reshape(-1, 3), c))
     return np.log(np.asarray(sample_n, np.float64).ravel() * (d * d).sum(1) / 3.0 + 1e-12)
 
 
-def learned_error(y, sample_n, nd: int) -> np.ndarray:
-    """Allocation signal nd / (nd + N) * exp(y) / N."""
+def learned_error(y, sample_n, nd: int, signal: int = 0) -> np.ndarray:
+    """Allocation signal nd / (nd + N) * exp(y) / N (signal 0) or exp(y) / N (1)."""
     N = np.maximum(np.asarray(sample_n, np.float64).ravel(), 1.0)
-    return nd / (nd + N) * np.exp(np.clip(np.asarray(y, np.float64), -60, 60)) / N
+    v = np.exp(np.clip(np.asarray(y, np.float64), -60, 60)) / N
+    return v if signal else nd / (nd + N) * v

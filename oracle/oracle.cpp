@@ -494,7 +494,8 @@ struct RenderParams {
   u32 radius;      // filter radius
   u32 filter_diff; // Filter Diff Flag
   u32 warmup;      // plain frames before adaptive sampling starts (5)
-  u32 pad2;
+  u32 pass_frames; // adaptive frames allocated together: pass_frames * W * H samples from one
+                   // error map, traced under the pass's first frame (0 or 1: one frame a pass)
 };
 
 // Shadow-segment end for a connection.  The reference passes dist - eps
@@ -623,6 +624,7 @@ template <class R> struct FrameState {
   u64 ext_rays = 0, shadow_rays = 0;
   std::vector<LightVertex<R>> lv;
   u32 lvN = 0;
+  u32 light_group = 0xFFFFFFFFu;       // three-frame group whose cache lv holds
 };
 
 // Light-vertex cache pass (cudaRTBDPTStream.cu:180-363,545-617,875-906).
@@ -935,7 +937,8 @@ static inline void apply_sample_gbuf(FrameState<R>& fs, u32 pix, const PathResul
 
 // One adaptive frame after warm-up (AdapBilatAtomic.cu:1407-1538).
 template <class R>
-static void adaptive_frame(const Scene& s, const RenderParams& p, const Camera<R>& cam, u32 f, FrameState<R>& fs) {
+static void adaptive_frame(const Scene& s, const RenderParams& p, const Camera<R>& cam, u32 f, u32 gen,
+                           FrameState<R>& fs) {
   const u32 W = p.width, H = p.height, n = W * H;
   std::vector<float> col(n * 3), pos(n * 3), nrm(n * 3), dif(n * 3);
   for (size_t i = 0; i < (size_t)n * 3; i++) {
@@ -966,7 +969,7 @@ static void adaptive_frame(const Scene& s, const RenderParams& p, const Camera<R
   }
   float sum = (float)tree_sum(fs.err.data(), n);
   fs.err_sum = sum;
-  for (u32 x = 0; x < n; x++) fs.counts[x] = prop_count(fs.err[x], sum, n);
+  for (u32 x = 0; x < n; x++) fs.counts[x] = prop_count(fs.err[x], sum, gen);
   u64 er = 0, sr = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : er, sr)
   for (i32 ii = 0; ii < (i32)n; ii++) {
@@ -1007,8 +1010,11 @@ static void render_adaptive(const Scene& s, const RenderParams& p, u32 frame0, u
   if (fs.gpos.size() != (size_t)n * 3) {
     fs.gpos.assign(n * 3, (R)0); fs.gnorm.assign(n * 3, (R)0); fs.gdiff.assign(n * 3, (R)0);
   }
-  for (u32 f = frame0; f < frame0 + nframes; f++) {
-    if (f % 3 == 0) light_pass<R>(s, p, f, fs);
+  for (u32 f = frame0; f < frame0 + nframes;) {
+    // the cache of the frame's three-frame group (built at its first frame)
+    if (f % 3 == 0 || fs.light_group != f / 3) light_pass<R>(s, p, f - f % 3, fs);
+    fs.light_group = f / 3;
+    u32 k = 1;
     if (f < p.warmup) {
       if (f == 0) std::fill(fs.N.begin(), fs.N.end(), 0u);
       u64 er = 0, sr = 0;
@@ -1021,8 +1027,10 @@ static void render_adaptive(const Scene& s, const RenderParams& p, u32 frame0, u
       }
       fs.ext_rays += er; fs.shadow_rays += sr;
     } else {
-      adaptive_frame<R>(s, p, cam, f, fs);
+      k = std::min(std::max(p.pass_frames, 1u), frame0 + nframes - f);
+      adaptive_frame<R>(s, p, cam, f, k * n, fs);
     }
+    f += k;
   }
 }
 

@@ -101,3 +101,45 @@ def test_adaptive_spends_samples_where_error_is(adaptive_runs):
     hi = err > np.quantile(err, 0.9)
     lo = err < np.quantile(err, 0.5)
     assert counts[hi].mean() > 3 * max(counts[lo].mean(), 1e-3)
+
+
+@pytest.fixture(scope="module")
+def pass_runs(cpt):
+    """Adaptive passes of 4 frames' worth of samples (Adaptive Pass Frames = 4)."""
+    sc = scenes.cornell()
+    W, H, F = 96, 80, 17
+    S = cpt.Scene(sc)
+    R = cpt.Renderer(W, H, mode=cpt.CPT_MODE_BDPT_ADAPTIVE_PLL)
+    R.set_attrib("Adaptive Pass Frames", 4)
+    gpu = R.render(S, sc["camera"], frames=F)
+    out = dict(gpu=gpu, N=R.sample_counts(), counts=R.read(cpt.CPT_BUF_SAMPLE_COUNTS), stats=R.stats())
+    O = oracle.Scene(sc)
+    p = oracle.render_params(W, H, sc["camera"], mode=1, pass_frames=4)
+    o32 = oracle.Renderer(p, 32)
+    out["r32"], out["n32"] = o32.render(O, 0, F)
+    out["st32"] = o32.adaptive_state()
+    o64 = oracle.Renderer(p, 64)
+    o64.render(O, 0, F)
+    out["r64"] = o64.result64()
+    out["W"], out["H"], out["F"] = W, H, F
+    return out
+
+
+def test_pass_frames_match_oracle(pass_runs):
+    r = pass_runs
+    n = r["W"] * r["H"]
+    # last pass: frames 13..16, 4 * W * H samples from one error map
+    assert r["counts"].sum() <= 4 * n and r["counts"].sum() > 2 * n
+    assert (r["counts"] == r["st32"]["counts"]).mean() > 0.999
+    assert rel_mse(r["gpu"], r["r32"]) <= 1e-6
+    assert rel_mse(r["gpu"], r["r64"]) <= 1e-4
+    assert (r["N"] == r["n32"]).mean() > 0.99
+    assert r["stats"]["frames"] == r["F"]
+
+
+def test_heavily_sampled_pixels_take_the_workgroup_path(pass_runs):
+    """Pixels with more than 32 records in a pass are summed by one workgroup each."""
+    r = pass_runs
+    big = r["counts"] > 32
+    assert big.any()
+    np.testing.assert_allclose(r["gpu"][big], r["r32"][big], rtol=1e-4, atol=1e-6)

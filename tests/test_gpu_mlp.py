@@ -138,15 +138,19 @@ def test_features_match_formulas(adaptive_state):
                                                            0.25, 0.1, 5).view(np.uint32))
 
 
-def test_target_matches_formula(cpt, adaptive_state):
+@pytest.mark.parametrize("clamp", [0.0, 0.5])
+def test_target_matches_formula(cpt, adaptive_state, clamp):
     a = adaptive_state
     rng = np.random.default_rng(4)
     ref = (a["st"]["result"] + rng.normal(0, 0.05, a["st"]["result"].shape)).astype(np.float32)
     d_ref = cpt.DeviceBuffer.from_array(ref)
     d_t = cpt.DeviceBuffer(a["W"] * a["H"] * 4)
-    a["R"].mlp_target(d_ref, d_t)
+    a["R"].mlp_target(d_ref, d_t, clamp)
     t = d_t.download(np.float32, a["W"] * a["H"])
-    np.testing.assert_allclose(t, M.target(a["st"]["result"], ref, a["st"]["N"]), rtol=1e-5, atol=1e-5)
+    exp = M.target(a["st"]["result"], ref, a["st"]["N"], clamp)
+    np.testing.assert_allclose(t, exp, rtol=1e-5, atol=1e-5)
+    if clamp:
+        assert (np.abs(t - M.target(a["st"]["result"], ref, a["st"]["N"])) > 1e-3).any()
 
 
 def test_learned_allocation_follows_model(cpt, adaptive_state):
@@ -156,6 +160,7 @@ def test_learned_allocation_follows_model(cpt, adaptive_state):
     y = m.infer(a["x"])
     expect = M.learned_error(y, a["st"]["N"], 5).astype(np.float32)
     R = a["R"]
+    assert R.attribs()["Learned Signal"][2] == 0
     R.set_mlp(m)
     R.render(a["S"], a["sc"]["camera"], frames=1)
     sig = R.read(cpt.CPT_BUF_ALLOC_ERROR).ravel()
@@ -164,3 +169,20 @@ def test_learned_allocation_follows_model(cpt, adaptive_state):
     ref_counts = oracle.alloc_proportional(sig, oracle.tree_sum(sig), a["W"] * a["H"]).ravel()
     assert np.array_equal(counts, ref_counts)
     R.set_mlp(None)
+
+
+def test_expected_error_signal(cpt):
+    sc = scenes.cornell()
+    S = cpt.Scene(sc)
+    R = cpt.Renderer(40, 30, mode=cpt.CPT_MODE_BDPT_ADAPTIVE_PLL)
+    R.set_attrib("Learned Signal", 1)
+    R.render(S, sc["camera"], frames=6)
+    dx = cpt.DeviceBuffer(40 * 30 * 36)
+    R.mlp_features(dx)
+    m = cpt.Mlp(6)
+    y = m.infer(dx.download(np.float32, 40 * 30 * 9).reshape(-1, 9))
+    N = R.sample_counts()
+    R.set_mlp(m)
+    R.render(S, sc["camera"], frames=1)
+    sig = R.read(cpt.CPT_BUF_ALLOC_ERROR).ravel()
+    np.testing.assert_allclose(sig, M.learned_error(y, N, 5, signal=1), rtol=1e-5, atol=0)

@@ -119,5 +119,7 @@ def test_feature_target_and_signal_formulas():
     assert x[1, 3] == 0.2126 and x[1, 8] == 0.0          # N = 0 counts as 1
     t = M.target(np.array([[1.0, 2.0, 3.0]]), np.array([[0.0, 2.0, 0.0]]), np.array([3]))
     assert abs(t[0] - np.log(10.0 + 1e-12)) < 1e-12
+    t = M.target(np.array([[1.0, 2.0, 3.0]]), np.array([[0.0, 2.0, 0.0]]), np.array([3]), clamp=1.5)
+    assert abs(t[0] - np.log(1.0 + 1.5 ** 2 + 1e-12)) < 1e-12   # r -> (1, 1.5, 1.5), ref -> (0, 1.5, 0)
     s = M.learned_error(np.array([0.0, np.log(8.0)]), np.array([5, 2]), 5)
     np.testing.assert_allclose(s, [0.5 / 5, 5 / 7 * 8 / 2])
