@@ -12,6 +12,14 @@ roofline: the closest-hit extension kernel, algorithmic bytes per launch =
   48 B per triangle tested, over its HIP-event duration on the renderer stream.
 cpu_baseline: the oracle's FP32 reference estimator on the host cores (OpenMP),
   a bounded sample of the same frame (a band of rows at 1 spp).
+time_to_equal_mse: the learned-vs-plain adaptive protocol of cudapt_amd/learned.py
+  on the same scene and resolution: converged images by plain BDPT (4096 spp for
+  the held-out poses, 1024 for the training poses), per-pixel error model trained
+  on plain adaptive runs of the training poses (2000 Adam steps, batch 65,536),
+  then on each held-out pose the time the learned renderer needs to reach the
+  MSE (radiance clamped to [0, 1]) plain adaptive BDPT reaches after 64 frames.
+mlp: MFMA throughput of the model's training step (forward + backward + Adam)
+  at batch 65,536: 28,416 FLOP per sample (3x the 9,472 FLOP forward).
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -60,6 +68,61 @@ def cpu_baseline(sc, seconds_target=10.0, max_frames=8):
 
 
 EXT_KERNEL = "k_trace<0, false, 1"
+MLP_FLOP_PER_SAMPLE = 3 * 2 * (9 * 64 + 64 * 64 + 64)
+BF16_PEAK_TFLOPS = 2500.0
+
+
+def learned_leg(ca, S, args):
+    """Time-to-equal-MSE of learned vs plain adaptive BDPT (cudapt_amd/learned.py)."""
+    from cudapt_amd import learned
+    n_tr, n_te = args.learn_train, args.learn_test
+    cams = learned.sphere_grid_poses(n_tr + n_te)
+    t0 = time.perf_counter()
+    refs_tr = [learned.converged(S, c, WIDTH, HEIGHT, args.learn_train_spp, depth=EYE_DEPTH) for c in cams[:n_tr]]
+    refs_te = [learned.converged(S, c, WIDTH, HEIGHT, args.learn_ref_spp, depth=EYE_DEPTH) for c in cams[n_tr:]]
+    log(f"learned leg: converged images in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    X, T = learned.collect(S, cams[:n_tr], refs_tr, WIDTH, HEIGHT, keep=args.learn_keep, depth=EYE_DEPTH,
+                           pass_frames=args.pass_frames)
+    log(f"learned leg: {len(T)} training samples in {time.perf_counter() - t0:.1f} s")
+    m = ca.Mlp(0xA22)
+    losses, train_s = learned.train(m, X, T, steps=args.learn_steps, batch=65536, lr=1e-3)
+    log(f"learned leg: trained in {train_s:.2f} s, loss {losses[0][1]:.3f} -> {losses[-1][1]:.3f}")
+    rows = learned.time_to_equal_mse(S, m, cams[n_tr:], refs_te, WIDTH, HEIGHT, frames=64, depth=EYE_DEPTH,
+                                     pass_frames=args.pass_frames, signal=1)
+    ratios = [r["learned_s"] / r["plain_adaptive_s"] for r in rows if r["learned_s"]]
+    bdpt = [r["plain_bdpt_s"] / r["plain_adaptive_s"] for r in rows if r["plain_bdpt_s"]]
+    gm = float(np.exp(np.mean(np.log(ratios)))) if len(ratios) == len(rows) else None
+    # MFMA throughput of one training step at batch 65,536
+    batch = 65536
+    dx = ca.DeviceBuffer.from_array(np.ascontiguousarray(X[:batch], np.float32))
+    dt = ca.DeviceBuffer.from_array(np.ascontiguousarray(T[:batch], np.float32))
+    for _ in range(20):
+        m.train_step_device(dx, dt, batch, lr=0.0, want_loss=False)
+    ca.synchronize()
+    reps = 200
+    ts = time.perf_counter()
+    for _ in range(reps):
+        m.train_step_device(dx, dt, batch, lr=0.0, want_loss=False)
+    ca.synchronize()
+    step_s = (time.perf_counter() - ts) / reps
+    tflops = MLP_FLOP_PER_SAMPLE * batch / step_s / 1e12
+    return {
+        "learned_over_plain_adaptive": round(gm, 4) if gm is not None else None,
+        "plain_bdpt_over_plain_adaptive": round(float(np.exp(np.mean(np.log(bdpt)))), 4) if bdpt else None,
+        "poses": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
+        "protocol": {"scene": "sphere grid", "width": WIDTH, "height": HEIGHT, "depth": EYE_DEPTH,
+                     "train_poses": n_tr, "heldout_poses": n_te, "train_ref_spp": args.learn_train_spp,
+                     "heldout_ref_spp": args.learn_ref_spp, "plain_frames": 64, "pass_frames": args.pass_frames,
+                     "train_steps": args.learn_steps, "batch": 65536, "lr": 1e-3, "train_samples": int(len(T)),
+                     "mse": "radiance clamped to [0, 1]", "signal": "exp(y) / N",
+                     "timing": "synchronised wall time per render call, MSE readback excluded"},
+        "train_s": round(train_s, 3),
+        "final_loss": round(float(losses[-1][1]), 4),
+    }, {"train_step_us": round(step_s * 1e6, 2), "flop_per_step": MLP_FLOP_PER_SAMPLE * batch,
+        "achieved": round(tflops, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": round(tflops / BF16_PEAK_TFLOPS, 5), "bound": "mfma",
+        "note": "wall time of train kernel + gradient-sum/Adam kernel per step"}
 
 
 def hbm_traffic(args):
@@ -79,7 +142,7 @@ def hbm_traffic(args):
         d = tempfile.mkdtemp(prefix="cpt_pmc_")
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t",
                "--", sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
-               "--no-traffic", "--frames", str(args.frames), "--in-flight", str(args.in_flight),
+               "--no-traffic", "--no-learned", "--frames", str(args.frames), "--in-flight", str(args.in_flight),
                "--lds-stack", str(args.lds_stack), "--copy-calib", str(calib >> 20)]
         log(f"traffic pass: {counter}")
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -118,6 +181,14 @@ def main():
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the time-to-equal-MSE leg")
+    ap.add_argument("--learn-train", type=int, default=8)
+    ap.add_argument("--learn-test", type=int, default=2)
+    ap.add_argument("--learn-train-spp", type=int, default=1024)
+    ap.add_argument("--learn-ref-spp", type=int, default=4096)
+    ap.add_argument("--learn-keep", type=float, default=0.0625)
+    ap.add_argument("--learn-steps", type=int, default=2000)
+    ap.add_argument("--pass-frames", type=int, default=8, help="adaptive pass size in frames (learned leg)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (one-GPU build; "
@@ -192,7 +263,7 @@ def main():
                       "shadow": round(st["shadow_ms"], 3), "light_pass": round(st["light_ms"], 3),
                       "all": round(st["total_ms"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "traffic_detail": None,
                      "kernel": "k_trace<0,false,1> (closest-hit extension)",
                      "launches": launches, "avg_launch_us": round(st["extend_ms"] * 1e3 / launches, 2),
                      "bytes_per_launch": int(bytes_alg / launches),
@@ -200,9 +271,15 @@ def main():
                      "tris_per_ray": round(st["extend_tris"] / max(st["extension_rays"], 1), 2)},
         "time_to_equal_mse": None,
     }
-    del R, S
+    del R
+    res["mlp"] = None
+    if not args.no_learned:
+        res["time_to_equal_mse"], res["mlp"] = learned_leg(ca, S, args)
+    del S
     if not args.no_traffic:
-        res["roofline"]["traffic"] = hbm_traffic(args)
+        tr = hbm_traffic(args)
+        res["roofline"]["traffic_detail"] = tr
+        res["roofline"]["traffic"] = tr["bytes_per_launch"] if tr else None
     if not args.no_cpu:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(sc)

@@ -399,6 +399,9 @@ int cpt_trace_counters(cpt_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* 
 
 namespace {
 
+bool is_adaptive(int mode) { return mode != CPT_MODE_BDPT_STREAM; }
+bool uses_filter(int mode) { return mode == CPT_MODE_BDPT_ADAPTIVE_PLL || mode == CPT_MODE_BDPT_ADAPTIVE_BINARY; }
+
 struct Attrib {
   const char* name;
   int type;  // 0 uint, 1 int, 2 bool, 3 float, 4 enum
@@ -516,6 +519,10 @@ struct cpt_renderer {
   // adaptive mode
   DevArray<float> gpos, gnrm, gdif, filtered, err, gauss, err_sum;
   DevArray<u32> counts, offsets, block_tot, total, big_list, big_n;
+  DevArray<float> sq, maxe;                  // second moment (W*H*3), max of the error map
+  DevArray<u32> flags, rank, kept;           // round-based allocation
+  DevArray<RoundState> rounds;
+  u32 last_rounds = 0;
   DevArray<double> block_sums;
   DevArray<float4> rec_acc, rec_pos, rec_nrm, rec_dif;
   float gauss_sigma = -1.f;
@@ -539,8 +546,7 @@ int renderer_alloc(cpt_renderer* r) {
   // adaptive frames draw floor(W*H*e_i/sum e) samples: at most W*H plus float
   // rounding of the ratios, bounded by the margin (and clamped on the device);
   // plain frames trace "Frames In Flight" frames as one wavefront
-  size_t cap = r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL ? n * (size_t)r->attr(kPassFrames) + 1024
-                                                     : n * (size_t)r->attr(kInFlight);
+  size_t cap = is_adaptive(r->mode) ? n * (size_t)r->attr(kPassFrames) + 1024 : n * (size_t)r->attr(kInFlight);
   if (cap > 0xFFFFFFFFull / 2) return fail(CPT_E_LIMIT, "render: %zu path slots exceed 2^31", cap);
   r->cap = (u32)cap;
   r->forget_light_caches();
@@ -550,8 +556,14 @@ int renderer_alloc(cpt_renderer* r) {
   HIP_TRY(r->sh_o.alloc(cap));
   HIP_TRY(r->sh_d.alloc(cap));
   HIP_TRY(r->sh_c.alloc(cap));
-  if (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL) {
+  if (is_adaptive(r->mode)) {
     HIP_TRY(r->gpos.alloc(n * 3));
+    HIP_TRY(r->sq.alloc(n * 3));
+    HIP_TRY(r->flags.alloc(n));
+    HIP_TRY(r->rank.alloc(n));
+    HIP_TRY(r->rounds.alloc(1));
+    HIP_TRY(r->maxe.alloc(1));
+    HIP_TRY(r->kept.alloc(1));
     HIP_TRY(r->gnrm.alloc(n * 3));
     HIP_TRY(r->gdif.alloc(n * 3));
     HIP_TRY(r->filtered.alloc(n * 3));
@@ -568,7 +580,7 @@ int renderer_alloc(cpt_renderer* r) {
     HIP_TRY(r->rec_dif.alloc(cap));
     HIP_TRY(r->big_list.alloc(n));
     HIP_TRY(r->big_n.alloc(1));
-    for (auto* a : {&r->gpos, &r->gnrm, &r->gdif, &r->filtered})
+    for (auto* a : {&r->gpos, &r->gnrm, &r->gdif, &r->filtered, &r->sq})
       HIP_TRY(hipMemsetAsync(a->p, 0, sizeof(float) * n * 3, r->stream));
     HIP_TRY(hipMemsetAsync(r->err.p, 0, sizeof(float) * n, r->stream));
     HIP_TRY(hipMemsetAsync(r->counts.p, 0, sizeof(u32) * n, r->stream));
@@ -700,6 +712,36 @@ int run_filter(cpt_renderer* r) {
   return CPT_OK;
 }
 
+// Round-based sample allocation into r->counts / r->offsets / r->total:
+// window bisection (AdapBilatBinary) or window = max error (AdapEmp).  Rounds
+// run on the device in batches; the host reads the done flag between batches.
+// After kMaxRounds rounds the remaining slots are filled as the reference's
+// fill kernel does.
+constexpr u32 kMaxRounds = 1024;
+constexpr u32 kRoundBatch = 8;
+int allocate_rounds(cpt_renderer* r, const FrameArgs& fa, const float* err, u32 gen, u32 nd) {
+  const u32 n = r->W * r->H;
+  const bool bisect = r->mode == CPT_MODE_BDPT_ADAPTIVE_BINARY;
+  const float desired = (float)n / (float)(nd ? nd : 1u);
+  const float min_prob = (float)r->attr("Min Trace Probability");
+  HIP_TRY(launch_error_max(err, n, r->maxe.p, r->stream));
+  HIP_TRY(launch_round_init(r->rounds.p, r->maxe.p, gen, bisect ? 1u : 0u, desired, min_prob, r->counts.p, n,
+                            r->stream));
+  for (u32 done_rounds = 0; done_rounds < kMaxRounds; done_rounds += kRoundBatch) {
+    for (u32 j = 0; j < kRoundBatch; j++)
+      HIP_TRY(launch_round(fa, err, n, r->rounds.p, r->flags.p, r->rank.p, r->block_tot.p, r->kept.p, r->counts.p,
+                           r->stream));
+    RoundState h;
+    HIP_TRY(hipMemcpyAsync(&h, r->rounds.p, sizeof h, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    r->last_rounds = h.rounds;
+    if (h.done) break;
+  }
+  HIP_TRY(launch_round_fill(r->rounds.p, n, r->counts.p, r->stream));
+  HIP_TRY(launch_scan_counts(r->counts.p, n, r->offsets.p, r->block_tot.p, r->total.p, r->stream));
+  return CPT_OK;
+}
+
 int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   DevScene sc = s->dev();
   const u32 n = r->W * r->H;
@@ -713,8 +755,11 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   Accum ac;
   ac.result = r->result.p; ac.sample_n = r->sample_n.p;
   ac.gpos = r->gpos.p; ac.gnrm = r->gnrm.p; ac.gdif = r->gdif.p;
+  ac.sq = r->sq.p;
   ac.rec = r->rec();
-  ac.variant = 1;
+  // AdapBilatAtomic's eye bounce adds the reflection offset; the bisection and
+  // empirical files use the plain stream's bounce
+  ac.variant = kVariantGbuf | (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL ? kVariantReflOffset : 0);
   if (fa.frame0 < (u32)r->attr(kWarmup)) {         // plain 1 spp frames with G-buffer (:1372-1406)
     ac.apply = 1;
     HIP_TRY(launch_frame_begin(r->fc.p, n, r->stream));
@@ -722,12 +767,20 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
     int rc = bounce_loop(r, sc, fa, n, ac);
     if (rc) return rc;
   } else {
-    HIP_TRY(timed_begin(r, 5, &ev));
-    int rc = run_filter(r);
-    if (rc) return rc;
-    HIP_TRY(timed_end(r, 5, ev));
+    int rc = CPT_OK;
     const u32 nd = (u32)r->attr("Desired Max Sampling");
+    const u32 gen = n * fa.nframes;                 // queue slots of this pass
     const float* alloc_err = r->err.p;
+    if (r->mode == CPT_MODE_BDPT_ADAPTIVE_EMP) {    // AdapEmp.cu:1209
+      HIP_TRY(timed_begin(r, 6, &ev));
+      HIP_TRY(launch_emp_error(r->result.p, r->sq.p, r->sample_n.p, n, r->err.p, r->stream));
+      HIP_TRY(timed_end(r, 6, ev));
+    } else {
+      HIP_TRY(timed_begin(r, 5, &ev));
+      rc = run_filter(r);
+      if (rc) return rc;
+      HIP_TRY(timed_end(r, 5, ev));
+    }
     if (r->mlp) {                                  // learned estimate replaces the filter residual
       HIP_TRY(r->mlp_x.alloc((size_t)n * kMlpIn));
       HIP_TRY(r->mlp_y.alloc(n));
@@ -740,12 +793,18 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
       HIP_TRY(timed_end(r, 9, ev));
       alloc_err = r->mlp_e.p;
     }
-    HIP_TRY(timed_begin(r, 6, &ev));
-    HIP_TRY(launch_error_sum(alloc_err, n, r->block_sums.p, r->err_sum.p, r->stream));
-    HIP_TRY(timed_end(r, 6, ev));
-    HIP_TRY(timed_begin(r, 7, &ev));
-    HIP_TRY(launch_alloc_scan(alloc_err, r->err_sum.p, n, n * fa.nframes, r->counts.p, r->offsets.p, r->block_tot.p,
-                              r->total.p, r->cap, r->stream));
+    if (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL) {     // proportional counts (AdapBilatAtomic.cu:817-842)
+      HIP_TRY(timed_begin(r, 6, &ev));
+      HIP_TRY(launch_error_sum(alloc_err, n, r->block_sums.p, r->err_sum.p, r->stream));
+      HIP_TRY(timed_end(r, 6, ev));
+      HIP_TRY(timed_begin(r, 7, &ev));
+      HIP_TRY(launch_alloc_scan(alloc_err, r->err_sum.p, n, gen, r->counts.p, r->offsets.p, r->block_tot.p,
+                                r->total.p, r->cap, r->stream));
+    } else {                                         // rounds (AdapBilatBinary.cu:1348-1388, AdapEmp.cu:1232-1260)
+      HIP_TRY(timed_begin(r, 7, &ev));
+      rc = allocate_rounds(r, fa, alloc_err, gen, nd);
+      if (rc) return rc;
+    }
     HIP_TRY(launch_frame_begin_dev(r->fc.p, r->total.p, r->cap, r->stream));
     HIP_TRY(launch_gen_adaptive(fa, r->offsets.p, r->counts.p, r->total.p, r->cap, r->A.view(), ac.rec, r->stream));
     HIP_TRY(timed_end(r, 7, ev));
@@ -754,7 +813,7 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
     if (rc) return rc;
     HIP_TRY(timed_begin(r, 8, &ev));
     HIP_TRY(launch_accum_adaptive(r->offsets.p, r->counts.p, n, r->cap, ac.rec, r->result.p, r->sample_n.p, r->gpos.p,
-                                  r->gnrm.p, r->gdif.p, r->big_list.p, r->big_n.p, r->stream));
+                                  r->gnrm.p, r->gdif.p, r->sq.p, r->big_list.p, r->big_n.p, r->stream));
     HIP_TRY(timed_end(r, 8, ev));
   }
   HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, fa.nframes, fa.eye_depth, r->stream));
@@ -765,7 +824,7 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
 // (src/cudaRTBDPTStream.cu:874-946), then applied to the running mean in
 // frame order: identical to fa.nframes sequential calls.
 int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
-  if (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL) return render_adaptive_frame(r, s, fa);
+  if (is_adaptive(r->mode)) return render_adaptive_frame(r, s, fa);
   DevScene sc = s->dev();
   const u32 n = r->W * r->H * fa.nframes;
   hipEvent_t ev = nullptr;
@@ -803,7 +862,7 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   }
   u32 eye_depth = (u32)r->attr(kEyeDepth), light_depth = (u32)r->attr(kLightDepth);
   u32 light_paths = (u32)r->attr(kLightPaths);
-  if (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL) {
+  if (is_adaptive(r->mode)) {
     u32 radius = (u32)r->attr("Filter Radius");
     if (radius > (u32)kMaxFilterRadius) return fail(CPT_E_LIMIT, "render: filter radius %u exceeds 31", radius);
     if (r->attr(kWarmup) < 1) return fail(CPT_E_LIMIT, "render: warm-up frames must be >= 1");
@@ -811,7 +870,7 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   if (eye_depth < 1 || eye_depth > (u32)kMaxDepth) return fail(CPT_E_LIMIT, "render: eye depth %u not in 1..15", eye_depth);
   if (light_depth < 1 || light_depth > 64) return fail(CPT_E_LIMIT, "render: light depth %u not in 1..64", light_depth);
   if (light_paths < 1 || light_paths > (u32)kBlock) return fail(CPT_E_LIMIT, "render: light paths %u not in 1..256", light_paths);
-  const bool adaptive = r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL;
+  const bool adaptive = is_adaptive(r->mode);
   u32 in_flight = (u32)r->attr(adaptive ? kPassFrames : kInFlight);
   if (in_flight < 1 || in_flight > kMaxInFlight)
     return fail(CPT_E_LIMIT, "render: %s %u not in 1..%u", adaptive ? kPassFrames : kInFlight, in_flight,
@@ -873,7 +932,7 @@ extern "C" {
 
 int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t height) {
   if (!out) return fail(CPT_E_ARG, "cpt_renderer_create: null output");
-  if (mode != CPT_MODE_BDPT_STREAM && mode != CPT_MODE_BDPT_ADAPTIVE_PLL)
+  if (mode < CPT_MODE_BDPT_STREAM || mode > CPT_MODE_BDPT_ADAPTIVE_EMP)
     return fail(CPT_E_ARG, "cpt_renderer_create: unknown mode %d", mode);
   if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 26))
     return fail(CPT_E_LIMIT, "cpt_renderer_create: %ux%u outside 1..2^26 pixels", width, height);
@@ -881,8 +940,13 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   r->mode = mode;
   r->W = width;
   r->H = height;
-  if (mode == CPT_MODE_BDPT_ADAPTIVE_PLL) {
-    // src/cudaRTBDPTStreamAdapBilatAtomic.cu:26-71, indices 0-14 as the reference numbers them
+  if (mode == CPT_MODE_BDPT_ADAPTIVE_EMP) {
+    // src/cudaRTBDPTStreamAdapEmp.cu:24-35, indices 0-3
+    r->attribs = {{"Adaptive Mode", 4, 0}, {"Desired Max Sampling", 0, 5},
+                  {"Min Trace Probability", 3, 0}, {"Debug Mode", 4, 0}};
+  } else if (is_adaptive(mode)) {
+    // src/cudaRTBDPTStreamAdapBilatAtomic.cu:26-71 and src/cudaRTBDPTStreamAdapBilatBinary.cu:26-68,
+    // indices 0-14 as the reference numbers them
     r->attribs = {{"Adaptive Mode", 4, 0},          {"Desired Max Sampling", 0, 5},
                   {"Min Trace Probability", 3, 0},  {"Debug Mode", 4, 0},
                   {"Filter Color EuD", 3, 50},      {"Filter Pos EuD", 3, 1},
@@ -897,7 +961,7 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   r->attribs.push_back({kLightPaths, 0, 128});
   r->attribs.push_back({kSeed, 0, (double)0x5EED0001u});
   r->attribs.push_back({kLiteralShadow, 2, 0});
-  if (mode == CPT_MODE_BDPT_ADAPTIVE_PLL) {
+  if (is_adaptive(mode)) {
     r->attribs.push_back({kWarmup, 0, 5});
     r->attribs.push_back({kPassFrames, 0, 1});
     r->attribs.push_back({kLearnedSignal, 0, 0});
@@ -973,8 +1037,10 @@ int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out) {
     case CPT_BUF_FILTERED:
     case CPT_BUF_ERROR:
     case CPT_BUF_SAMPLE_COUNTS: {
-      if (r->mode != CPT_MODE_BDPT_ADAPTIVE_PLL)
-        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive mode", which);
+      if (!is_adaptive(r->mode))
+        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive modes", which);
+      if (which == CPT_BUF_FILTERED && !uses_filter(r->mode))
+        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: the empirical-variance mode does not filter");
       const void* src = which == CPT_BUF_GBUF_POS ? (const void*)r->gpos.p
                       : which == CPT_BUF_GBUF_NORM ? (const void*)r->gnrm.p
                       : which == CPT_BUF_GBUF_DIFF ? (const void*)r->gdif.p
@@ -984,9 +1050,14 @@ int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out) {
       HIP_TRY(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, r->stream));
       break;
     }
+    case CPT_BUF_SECOND_MOMENT:
+      if (!is_adaptive(r->mode))
+        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive modes", which);
+      HIP_TRY(hipMemcpyAsync(out, r->sq.p, n * 12, hipMemcpyDeviceToHost, r->stream));
+      break;
     case CPT_BUF_ALLOC_ERROR: {
-      if (r->mode != CPT_MODE_BDPT_ADAPTIVE_PLL)
-        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive mode", which);
+      if (!is_adaptive(r->mode))
+        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive modes", which);
       const float* src = r->mlp && r->mlp_e.p ? r->mlp_e.p : r->err.p;
       HIP_TRY(hipMemcpyAsync(out, src, n * 4, hipMemcpyDeviceToHost, r->stream));
       break;
@@ -1182,7 +1253,7 @@ int cpt_mlp_infer(cpt_mlp* m, const float* d_x, uint32_t n, float* d_y, void* st
 
 int cpt_renderer_mlp_features(cpt_renderer* r, float* d_x) {
   if (!r || !d_x) return fail(CPT_E_ARG, "cpt_renderer_mlp_features: null argument");
-  if (r->mode != CPT_MODE_BDPT_ADAPTIVE_PLL || !r->allocated || !r->have_cam)
+  if (!uses_filter(r->mode) || !r->allocated || !r->have_cam)
     return fail(CPT_E_STATE, "cpt_renderer_mlp_features: needs an adaptive renderer that has rendered");
   // the filter state the next adaptive frame allocates from
   int rc = run_filter(r);
@@ -1205,8 +1276,8 @@ int cpt_renderer_mlp_target(cpt_renderer* r, const float* d_ref, float clamp, fl
 
 int cpt_renderer_set_mlp(cpt_renderer* r, cpt_mlp* m) {
   if (!r) return fail(CPT_E_ARG, "cpt_renderer_set_mlp: null renderer");
-  if (m && r->mode != CPT_MODE_BDPT_ADAPTIVE_PLL)
-    return fail(CPT_E_STATE, "cpt_renderer_set_mlp: the model drives the adaptive renderer only");
+  if (m && !uses_filter(r->mode))
+    return fail(CPT_E_STATE, "cpt_renderer_set_mlp: the model drives the filter-based adaptive renderers only");
   r->mlp = m;
   return CPT_OK;
 }

@@ -131,10 +131,13 @@ struct Accum {
   float* gpos;         // W*H*3 running means (adaptive)
   float* gnrm;
   float* gdif;
+  float* sq;           // W*H*3 running mean of squared samples (adaptive), or null
   SampleRec rec;
   int apply;
-  int variant;         // 1: adaptive-file eye bounce (reflection offset, G-buffer)
+  int variant;         // bit 0: reflection offset of AdapBilatAtomic's eye bounce; bit 1: first-hit G-buffer
 };
+constexpr int kVariantReflOffset = 1;
+constexpr int kVariantGbuf = 2;
 hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, PathBuf nxt, const float4* hits,
                         FrameCounters* fc, int bounce, uint32_t max_n, ShadowBuf sb, const LightVert* lv,
                         const Accum& ac, hipStream_t st);
@@ -157,7 +160,40 @@ hipError_t launch_gen_adaptive(const FrameArgs& fa, const uint32_t* offsets, con
                                const uint32_t* total, uint32_t cap, PathBuf pb, SampleRec rec, hipStream_t st);
 hipError_t launch_accum_adaptive(const uint32_t* offsets, const uint32_t* counts, uint32_t n, uint32_t cap,
                                  SampleRec rec, float* result, uint32_t* sample_n, float* gpos, float* gnrm,
-                                 float* gdif, uint32_t* big_list, uint32_t* big_n, hipStream_t st);
+                                 float* gdif, float* sq, uint32_t* big_list, uint32_t* big_n, hipStream_t st);
+
+// Empirical squared error (pt_calculateEmpiricalSquareError_kernel, AdapEmp.cu:949-960)
+// and the maximum of an error map (thrust::maximum reduce from 0, AdapEmp.cu:1211).
+hipError_t launch_emp_error(const float* result, const float* sq, const uint32_t* sample_n, uint32_t n, float* err,
+                            hipStream_t st);
+hipError_t launch_error_max(const float* err, uint32_t n, float* max_out, hipStream_t st);
+
+// Round-based allocation (pt_genTempAdapPathQueue_kernel rounds + remove_if +
+// pt_fillTempAdapPathQueue_kernel; AdapBilatBinary.cu:1348-1388, AdapEmp.cu:1232-1260).
+// Round r keeps pixel p unless u * window > max(e_p, min_prob); the first
+// kept pixels in pixel order take the remaining queue slots.  Binary: window
+// bisected towards gen / nd kept pixels a round; empirical: window = max e.
+struct RoundState {
+  float left, right, window, maxe;
+  uint32_t accum;      // queue slots filled so far
+  uint32_t gen;        // slots to fill
+  uint32_t done;
+  uint32_t rounds;
+  uint32_t bisect;     // 1: AdapBilatBinary window bisection, 0: fixed window = max e
+  float desired;       // kept pixels per round the bisection aims at
+  float min_prob;
+  uint32_t pad;
+};
+hipError_t launch_round_init(RoundState* rs, const float* maxe, uint32_t gen, uint32_t bisect, float desired,
+                             float min_prob, uint32_t* counts, uint32_t n, hipStream_t st);
+// One round: keep flags, their exclusive scan, count update, state update.
+hipError_t launch_round(const FrameArgs& fa, const float* err, uint32_t n, RoundState* rs, uint32_t* flags,
+                        uint32_t* rank, uint32_t* block_tot, uint32_t* kept, uint32_t* counts, hipStream_t st);
+// Unfilled slots go to pixels 0, 1, ... in order (wrapping past W*H).
+hipError_t launch_round_fill(const RoundState* rs, uint32_t n, uint32_t* counts, hipStream_t st);
+// Exclusive scan of given per-pixel counts into offsets and their total.
+hipError_t launch_scan_counts(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint32_t* block_tot,
+                              uint32_t* total, hipStream_t st);
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
                          uint32_t max_n, int lds_stack, hipStream_t st);
 hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes, uint32_t depth, hipStream_t st);

@@ -703,6 +703,12 @@ __device__ __forceinline__ void apply_pixel_gbuf(const Accum& ac, uint32_t pix, 
     dst[k][1] = ri * w.y + oi * dst[k][1];
     dst[k][2] = ri * w.z + oi * dst[k][2];
   }
+  if (ac.sq) {                                     // squareResult (AdapEmp.cu:927-944)
+    float* q = ac.sq + (size_t)pix * 3;
+    q[0] = fmaxf(ri * (acc.x * acc.x) + oi * q[0], 0.f);
+    q[1] = fmaxf(ri * (acc.y * acc.y) + oi * q[1], 0.f);
+    q[2] = fmaxf(ri * (acc.z * acc.z) + oi * q[2], 0.f);
+  }
 }
 
 // Append with one atomic per workgroup: returns this lane's slot when pred.
@@ -762,7 +768,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
       acc = acc + psample; N++; term = true;
     } else {
       float w = (1.f - h.z) - h.w;
-      const bool gbuf = ac.variant == 1 && depth == 0;
+      const bool gbuf = (ac.variant & kVariantGbuf) && depth == 0;
       Surf s = fetch_surf(sc, leaf, w, h.z, h.w, gbuf);
       f3 nl = dot(s.norm, dir) < 0.f ? s.norm : s.norm * -1.f;
       if (gbuf) {                                          // AdapBilatAtomic.cu:650-655
@@ -770,7 +776,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
         ac.rec.gnrm[sid] = make_float4(nl.x, nl.y, nl.z, 0.f);
         ac.rec.gdif[sid] = make_float4(s.diff.x, s.diff.y, s.diff.z, 0.f);
       }
-      Bounce b = sample_bounce(st, inc, orig, dir, h.x, s, nl, RT_EYE, ac.variant == 1);
+      Bounce b = sample_bounce(st, inc, orig, dir, h.x, s, nl, RT_EYE, (ac.variant & kVariantReflOffset) != 0);
       psample = psample + vmul(s.emissive, mul);
       f3 inMul = mul, inDir = dir * -1.f;
       float contrib = length(mul) * length(b.mul);
@@ -857,7 +863,7 @@ hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, Pa
     return hipErrorInvalidValue;
   if (ac.apply == 1 && (!rec_ok(ac.rec) || !ac.gpos || !ac.gnrm || !ac.gdif)) return hipErrorInvalidValue;
   if (ac.apply == 2 && !ac.rec.acc) return hipErrorInvalidValue;
-  if (ac.variant == 1 && !rec_ok(ac.rec)) return hipErrorInvalidValue;
+  if ((ac.variant & kVariantGbuf) && !rec_ok(ac.rec)) return hipErrorInvalidValue;
   k_shade<<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, fa, cur, nxt, hits, fc, bounce, sb, lv, ac);
   return hipGetLastError();
 }
@@ -1172,10 +1178,10 @@ hipError_t launch_gen_adaptive(const FrameArgs& fa, const uint32_t* offsets, con
 }
 
 // Running-mean update of one pixel from the sums of its new records.
-__device__ __forceinline__ void accum_apply(uint32_t pix, uint32_t tN, f3 ts, f3 tp, f3 tn, f3 td,
+__device__ __forceinline__ void accum_apply(uint32_t pix, uint32_t tN, f3 ts, f3 tp, f3 tn, f3 td, f3 tq,
                                             float* __restrict__ result, uint32_t* __restrict__ sample_n,
                                             float* __restrict__ gpos, float* __restrict__ gnrm,
-                                            float* __restrict__ gdif) {
+                                            float* __restrict__ gdif, float* __restrict__ sq) {
   uint32_t nold = sample_n[pix], nn = tN + nold;
   if (nn > nold) {
     float ri = 1.f / (float)nn, oi = (float)nold * ri;
@@ -1192,19 +1198,24 @@ __device__ __forceinline__ void accum_apply(uint32_t pix, uint32_t tN, f3 ts, f3
       dst[k][1] = ri * vals[k].y + oi * dst[k][1];
       dst[k][2] = ri * vals[k].z + oi * dst[k][2];
     }
+    float* q = sq + (size_t)pix * 3;                // squareResult (AdapEmp.cu:927-944)
+    q[0] = fmaxf(ri * tq.x + oi * q[0], 0.f);
+    q[1] = fmaxf(ri * tq.y + oi * q[1], 0.f);
+    q[2] = fmaxf(ri * tq.z + oi * q[2], 0.f);
   }
 }
 
 // Adds record q to the running sums (records with N = 0 or an infinite
 // component are skipped, AdapBilatAtomic.cu:1056-1060).
 __device__ __forceinline__ void accum_record(const SampleRec& rec, uint32_t q, f3& ts, f3& tp, f3& tn, f3& td,
-                                             uint32_t& tN) {
+                                             f3& tq, uint32_t& tN) {
   float4 a = rec.acc[q];
   uint32_t N = __float_as_uint(a.w);
   f3 acc = ld3(a);
   if (N == 0 || isinf3(acc)) return;
   float nf = (float)N;
   ts = ts + acc;
+  tq = tq + vmul(acc, acc);
   tN += N;
   tp = tp + ld3(rec.gpos[q]) * nf;
   tn = tn + ld3(rec.gnrm[q]) * nf;
@@ -1225,20 +1236,21 @@ __global__ __launch_bounds__(kBlock) void k_accum_adaptive(const uint32_t* __res
                                                            uint32_t cap, SampleRec rec, float* __restrict__ result,
                                                            uint32_t* __restrict__ sample_n, float* __restrict__ gpos,
                                                            float* __restrict__ gnrm, float* __restrict__ gdif,
-                                                           uint32_t* __restrict__ big_list,
+                                                           float* __restrict__ sq, uint32_t* __restrict__ big_list,
                                                            uint32_t* __restrict__ big_n) {
   const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
   if (pix >= n) return;
   uint32_t q0 = offsets[pix], q1 = q0 + counts[pix];
   if (q1 > cap) q1 = cap;
+  if (q0 > q1) q0 = q1;
   if (q1 > q0 + kAccumSerialMax) {
     big_list[atomicAdd(big_n, 1u)] = pix;
     return;
   }
-  f3 ts = mk3(0.f, 0.f, 0.f), tp = ts, tn = ts, td = ts;
+  f3 ts = mk3(0.f, 0.f, 0.f), tp = ts, tn = ts, td = ts, tq = ts;
   uint32_t tN = 0;
-  for (uint32_t q = q0; q < q1; q++) accum_record(rec, q, ts, tp, tn, td, tN);
-  accum_apply(pix, tN, ts, tp, tn, td, result, sample_n, gpos, gnrm, gdif);
+  for (uint32_t q = q0; q < q1; q++) accum_record(rec, q, ts, tp, tn, td, tq, tN);
+  accum_apply(pix, tN, ts, tp, tn, td, tq, result, sample_n, gpos, gnrm, gdif, sq);
 }
 
 // Heavily sampled pixels: one workgroup per pixel, thread t sums records
@@ -1248,9 +1260,11 @@ __global__ __launch_bounds__(kBlock) void k_accum_big(const uint32_t* __restrict
                                                       const uint32_t* __restrict__ counts, uint32_t cap, SampleRec rec,
                                                       float* __restrict__ result, uint32_t* __restrict__ sample_n,
                                                       float* __restrict__ gpos, float* __restrict__ gnrm,
-                                                      float* __restrict__ gdif, const uint32_t* __restrict__ big_list,
+                                                      float* __restrict__ gdif, float* __restrict__ sq,
+                                                      const uint32_t* __restrict__ big_list,
                                                       const uint32_t* __restrict__ big_n) {
-  __shared__ float red[kBlock / 64][13];
+  constexpr int kV = 15;
+  __shared__ float red[kBlock / 64][kV];
   __shared__ uint32_t redn[kBlock / 64];
   const uint32_t nb = *big_n;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -1258,32 +1272,32 @@ __global__ __launch_bounds__(kBlock) void k_accum_big(const uint32_t* __restrict
     const uint32_t pix = big_list[i];
     uint32_t q0 = offsets[pix], q1 = q0 + counts[pix];
     if (q1 > cap) q1 = cap;
-    f3 ts = mk3(0.f, 0.f, 0.f), tp = ts, tn = ts, td = ts;
+    f3 ts = mk3(0.f, 0.f, 0.f), tp = ts, tn = ts, td = ts, tq = ts;
     uint32_t tN = 0;
-    for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBlock) accum_record(rec, q, ts, tp, tn, td, tN);
-    float v[12] = {ts.x, ts.y, ts.z, tp.x, tp.y, tp.z, tn.x, tn.y, tn.z, td.x, td.y, td.z};
+    for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBlock) accum_record(rec, q, ts, tp, tn, td, tq, tN);
+    float v[kV] = {ts.x, ts.y, ts.z, tp.x, tp.y, tp.z, tn.x, tn.y, tn.z, td.x, td.y, td.z, tq.x, tq.y, tq.z};
 #pragma unroll
-    for (int k = 0; k < 12; k++)
+    for (int k = 0; k < kV; k++)
       for (int o = 32; o >= 1; o >>= 1) v[k] += __shfl_xor(v[k], o, 64);
     for (int o = 32; o >= 1; o >>= 1) tN += __shfl_xor(tN, o, 64);
     if (lane == 0) {
 #pragma unroll
-      for (int k = 0; k < 12; k++) red[wave][k] = v[k];
+      for (int k = 0; k < kV; k++) red[wave][k] = v[k];
       redn[wave] = tN;
     }
     __syncthreads();
     if (threadIdx.x == 0) {
-      float a[12];
+      float a[kV];
       uint32_t sN = 0;
 #pragma unroll
-      for (int k = 0; k < 12; k++) a[k] = 0.f;
+      for (int k = 0; k < kV; k++) a[k] = 0.f;
       for (int w = 0; w < kBlock / 64; w++) {
 #pragma unroll
-        for (int k = 0; k < 12; k++) a[k] += red[w][k];
+        for (int k = 0; k < kV; k++) a[k] += red[w][k];
         sN += redn[w];
       }
       accum_apply(pix, sN, mk3(a[0], a[1], a[2]), mk3(a[3], a[4], a[5]), mk3(a[6], a[7], a[8]),
-                  mk3(a[9], a[10], a[11]), result, sample_n, gpos, gnrm, gdif);
+                  mk3(a[9], a[10], a[11]), mk3(a[12], a[13], a[14]), result, sample_n, gpos, gnrm, gdif, sq);
     }
     __syncthreads();
   }
@@ -1291,13 +1305,188 @@ __global__ __launch_bounds__(kBlock) void k_accum_big(const uint32_t* __restrict
 
 hipError_t launch_accum_adaptive(const uint32_t* offsets, const uint32_t* counts, uint32_t n, uint32_t cap,
                                  SampleRec rec, float* result, uint32_t* sample_n, float* gpos, float* gnrm,
-                                 float* gdif, uint32_t* big_list, uint32_t* big_n, hipStream_t st) {
-  if (!rec_ok(rec) || !offsets || !counts || !result || !sample_n || !gpos || !gnrm || !gdif || !big_list || !big_n)
+                                 float* gdif, float* sq, uint32_t* big_list, uint32_t* big_n, hipStream_t st) {
+  if (!rec_ok(rec) || !offsets || !counts || !result || !sample_n || !gpos || !gnrm || !gdif || !sq || !big_list ||
+      !big_n)
     return hipErrorInvalidValue;
   k_fill_u32<<<1, 1, 0, st>>>(big_n, 0u);
   k_accum_adaptive<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(offsets, counts, n, cap, rec, result, sample_n, gpos,
-                                                                  gnrm, gdif, big_list, big_n);
-  k_accum_big<<<1024, kBlock, 0, st>>>(offsets, counts, cap, rec, result, sample_n, gpos, gnrm, gdif, big_list, big_n);
+                                                                  gnrm, gdif, sq, big_list, big_n);
+  k_accum_big<<<1024, kBlock, 0, st>>>(offsets, counts, cap, rec, result, sample_n, gpos, gnrm, gdif, sq, big_list,
+                                       big_n);
+  return hipGetLastError();
+}
+
+// ---- empirical error and round-based allocation -----------------------------
+
+__global__ __launch_bounds__(kBlock) void k_emp_error(const float* __restrict__ r, const float* __restrict__ q,
+                                                      const uint32_t* __restrict__ sample_n, uint32_t n,
+                                                      float* __restrict__ err) {
+  const uint32_t x = blockIdx.x * kBlock + threadIdx.x;
+  if (x >= n) return;
+  const uint32_t N = sample_n[x];
+  const float* a = r + (size_t)x * 3;
+  const float* b = q + (size_t)x * 3;
+  float e = ((b[0] - a[0] * a[0]) + (b[1] - a[1] * a[1]) + (b[2] - a[2] * a[2])) * (float)N / ((float)(N - 1u) * 3.f);
+  err[x] = fminf(e, 1.0f);
+}
+hipError_t launch_emp_error(const float* result, const float* sq, const uint32_t* sample_n, uint32_t n, float* err,
+                            hipStream_t st) {
+  if (!result || !sq || !sample_n || !err) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  k_emp_error<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(result, sq, sample_n, n, err);
+  return hipGetLastError();
+}
+
+// Maximum with 0 as the initial value: non-negative floats order as their bits.
+__global__ __launch_bounds__(kBlock) void k_error_max(const float* __restrict__ e, uint32_t n,
+                                                      uint32_t* __restrict__ out) {
+  uint32_t m = 0;
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock)
+    m = max(m, __float_as_uint(fmaxf(e[i], 0.f)));
+  for (int o = 32; o >= 1; o >>= 1) m = max(m, (uint32_t)__shfl_xor((int)m, o, 64));
+  if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+hipError_t launch_error_max(const float* err, uint32_t n, float* max_out, hipStream_t st) {
+  if (!err || !max_out) return hipErrorInvalidValue;
+  k_fill_u32<<<1, 1, 0, st>>>((uint32_t*)max_out, 0u);
+  uint32_t nb = (n + kBlock - 1) / kBlock;
+  if (nb > 1024) nb = 1024;
+  if (nb) k_error_max<<<nb, kBlock, 0, st>>>(err, n, (uint32_t*)max_out);
+  return hipGetLastError();
+}
+
+__global__ void k_round_init(RoundState* rs, const float* maxe, uint32_t gen, uint32_t bisect, float desired,
+                             float min_prob) {
+  RoundState s;
+  s.maxe = *maxe;
+  s.left = 0.f;
+  s.right = s.maxe * 2.f;
+  s.window = bisect ? (s.left + s.right) * 0.5f : s.maxe;
+  s.accum = 0;
+  s.gen = gen;
+  s.done = gen == 0 ? 1u : 0u;
+  s.rounds = 0;
+  s.bisect = bisect;
+  s.desired = desired;
+  s.min_prob = min_prob;
+  s.pad = 0;
+  *rs = s;
+}
+__global__ __launch_bounds__(kBlock) void k_zero_u32(uint32_t* __restrict__ p, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+hipError_t launch_round_init(RoundState* rs, const float* maxe, uint32_t gen, uint32_t bisect, float desired,
+                             float min_prob, uint32_t* counts, uint32_t n, hipStream_t st) {
+  if (!rs || !maxe || !counts) return hipErrorInvalidValue;
+  k_round_init<<<1, 1, 0, st>>>(rs, maxe, gen, bisect, desired, min_prob);
+  if (n) k_zero_u32<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(counts, n);
+  return hipGetLastError();
+}
+
+// pt_genTempAdapPathQueue_kernel: keep unless u * window > max(e, min_prob),
+// u from the stream of (frame, slots filled so far) at the pixel.
+__global__ __launch_bounds__(kBlock) void k_round_keep(uint64_t seed, uint32_t frame, const float* __restrict__ err,
+                                                       uint32_t n, const RoundState* __restrict__ rs,
+                                                       uint32_t* __restrict__ flags) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || rs->done) return;
+  uint64_t s = 0;
+  const uint64_t inc = ((uint64_t)i << 1) | 1u;
+  pcg32_seed(s, inc, stream_key(seed, frame, 3u, rs->accum));
+  const float u = uniform01(s, inc);
+  flags[i] = (u * rs->window > fmaxf(err[i], rs->min_prob)) ? 0u : 1u;
+}
+// Kept pixels of rank < remaining slots take one sample each.
+__global__ __launch_bounds__(kBlock) void k_round_apply(const uint32_t* __restrict__ flags,
+                                                        const uint32_t* __restrict__ rank, uint32_t n,
+                                                        const RoundState* __restrict__ rs,
+                                                        const uint32_t* __restrict__ kept,
+                                                        uint32_t* __restrict__ counts) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n || rs->done) return;
+  const uint32_t take = min(*kept, rs->gen - rs->accum);
+  if (flags[i] && rank[i] < take) counts[i] += 1u;
+}
+// remove_if + size bookkeeping + the bisection of AdapBilatBinary.cu:1363-1371.
+__global__ void k_round_update(RoundState* rs, const uint32_t* kept) {
+  RoundState s = *rs;
+  if (s.done) return;
+  const uint32_t take = min(*kept, s.gen - s.accum);
+  s.accum += take;
+  s.rounds++;
+  if (take == 0u) {
+    s.done = 1u;
+  } else if (s.bisect) {
+    if ((float)take < s.desired) s.right = s.window;
+    else if ((float)take > s.desired) s.left = s.window;
+    s.window = (s.left + s.right) * 0.5f;
+  }
+  if (s.accum >= s.gen) s.done = 1u;
+  *rs = s;
+}
+__global__ __launch_bounds__(kBlock) void k_round_fill(const RoundState* __restrict__ rs, uint32_t n,
+                                                       uint32_t* __restrict__ counts) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t unfilled = rs->gen > rs->accum ? rs->gen - rs->accum : 0u;
+  counts[i] += unfilled / n + (i < unfilled % n ? 1u : 0u);
+}
+
+// Exclusive scan of given per-element values (SRC = 1) into offsets.
+__global__ __launch_bounds__(kBlock) void k_scan_local_u32(const uint32_t* __restrict__ in, uint32_t n,
+                                                           uint32_t* __restrict__ offsets,
+                                                           uint32_t* __restrict__ btot) {
+  __shared__ uint32_t s[kBlock];
+  const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
+  uint32_t c[kScanPer], tsum = 0;
+#pragma unroll
+  for (int k = 0; k < kScanPer; k++) {
+    uint32_t i = base + k;
+    c[k] = i < n ? in[i] : 0u;
+    tsum += c[k];
+  }
+  s[threadIdx.x] = tsum;
+  __syncthreads();
+  for (int o = 1; o < kBlock; o <<= 1) {
+    uint32_t v = (int)threadIdx.x >= o ? s[threadIdx.x - o] : 0u;
+    __syncthreads();
+    s[threadIdx.x] += v;
+    __syncthreads();
+  }
+  uint32_t run = s[threadIdx.x] - tsum;
+#pragma unroll
+  for (int k = 0; k < kScanPer; k++) {
+    uint32_t i = base + k;
+    if (i < n) offsets[i] = run;
+    run += c[k];
+  }
+  if (threadIdx.x == kBlock - 1) btot[blockIdx.x] = s[kBlock - 1];
+}
+hipError_t launch_scan_counts(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint32_t* block_tot,
+                              uint32_t* total, hipStream_t st) {
+  if (!counts || !offsets || !block_tot || !total) return hipErrorInvalidValue;
+  uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
+  if (nb) k_scan_local_u32<<<nb, kBlock, 0, st>>>(counts, n, offsets, block_tot);
+  k_scan_totals<<<1, kBlock, 0, st>>>(block_tot, nb, total);
+  if (n) k_scan_add<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(offsets, block_tot, n);
+  return hipGetLastError();
+}
+hipError_t launch_round(const FrameArgs& fa, const float* err, uint32_t n, RoundState* rs, uint32_t* flags,
+                        uint32_t* rank, uint32_t* block_tot, uint32_t* kept, uint32_t* counts, hipStream_t st) {
+  if (!err || !rs || !flags || !rank || !block_tot || !kept || !counts || n == 0) return hipErrorInvalidValue;
+  const uint32_t g = (n + kBlock - 1) / kBlock;
+  k_round_keep<<<g, kBlock, 0, st>>>(fa.seed, fa.frame0, err, n, rs, flags);
+  hipError_t e = launch_scan_counts(flags, n, rank, block_tot, kept, st);
+  if (e != hipSuccess) return e;
+  k_round_apply<<<g, kBlock, 0, st>>>(flags, rank, n, rs, kept, counts);
+  k_round_update<<<1, 1, 0, st>>>(rs, kept);
+  return hipGetLastError();
+}
+hipError_t launch_round_fill(const RoundState* rs, uint32_t n, uint32_t* counts, hipStream_t st) {
+  if (!rs || !counts) return hipErrorInvalidValue;
+  if (n) k_round_fill<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(rs, n, counts);
   return hipGetLastError();
 }
 

@@ -149,8 +149,13 @@ int cpt_trace_counters(cpt_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* 
  *      src/raytracer.cpp:19-31,574-625) ---------------------------------- */
 
 enum {
-  CPT_MODE_BDPT_STREAM = 0,       /* cudaRTBDPTStream (src/cudaRTBDPTStream.cu) */
-  CPT_MODE_BDPT_ADAPTIVE_PLL = 1  /* cudaRTBDPTStreamAdapBilatAtomic (src/cudaRTBDPTStreamAdapBilatAtomic.cu) */
+  CPT_MODE_BDPT_STREAM = 0,          /* cudaRTBDPTStream (src/cudaRTBDPTStream.cu) */
+  CPT_MODE_BDPT_ADAPTIVE_PLL = 1,    /* cudaRTBDPTStreamAdapBilatAtomic (src/cudaRTBDPTStreamAdapBilatAtomic.cu):
+                                        filter residual, counts proportional to the error */
+  CPT_MODE_BDPT_ADAPTIVE_BINARY = 2, /* cudaRTBDPTStreamAdapBilatBinary (src/cudaRTBDPTStreamAdapBilatBinary.cu):
+                                        filter residual, rounds with a bisected acceptance window */
+  CPT_MODE_BDPT_ADAPTIVE_EMP = 3     /* cudaRTBDPTStreamAdapEmp (src/cudaRTBDPTStreamAdapEmp.cu): empirical
+                                        variance, rounds with the window at the maximum error */
 };
 
 /* RTRenderer::Init(w, h) + SetRendererMode(mode) (src/raytracer.cpp:574-608). */
@@ -187,8 +192,9 @@ enum {
   CPT_BUF_FILTERED = 5,   /* float x3: cross-bilateral output */
   CPT_BUF_ERROR = 6,      /* float x1: per-pixel error estimate */
   CPT_BUF_SAMPLE_COUNTS = 7, /* uint32: samples drawn per pixel in the last adaptive frame */
-  CPT_BUF_ALLOC_ERROR = 8    /* float x1: the error signal the last allocation used (the learned
+  CPT_BUF_ALLOC_ERROR = 8,   /* float x1: the error signal the last allocation used (the learned
                                 estimate when a model is attached, else CPT_BUF_ERROR) */
+  CPT_BUF_SECOND_MOMENT = 9  /* float x3: running mean of squared samples (squareResult, AdapEmp.cu:927-944) */
 };
 int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out);
 

@@ -78,7 +78,8 @@ RENDER_PARAMS_DTYPE = np.dtype([
     ("fov", "<f4"), ("eye_depth", "<u4"), ("light_depth", "<u4"), ("light_paths", "<u4"), ("flags", "<u4"),
     ("seed", "<u8"), ("row0", "<u4"), ("rows", "<u4"),
     ("mode", "<u4"), ("nd", "<u4"), ("sc", "<f4"), ("sp", "<f4"), ("sn", "<f4"), ("sd", "<f4"),
-    ("radius", "<u4"), ("filter_diff", "<u4"), ("warmup", "<u4"), ("pass_frames", "<u4")])
+    ("radius", "<u4"), ("filter_diff", "<u4"), ("warmup", "<u4"), ("pass_frames", "<u4"),
+    ("min_prob", "<f4"), ("pad3", "<u4")])
 
 
 def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:
@@ -139,7 +140,7 @@ def max_threads() -> int:
 
 def render_params(width, height, camera, eye_depth=5, light_depth=5, light_paths=128, seed=0x5EED0001, flags=0,
                   row0=0, rows=0, mode=0, nd=5, sc=50.0, sp=1.0, sn=0.25, sd=0.1, radius=5, filter_diff=0,
-                  warmup=5, pass_frames=1):
+                  warmup=5, pass_frames=1, min_prob=0.0):
     """Reference-estimator parameters; mode 1 = adaptive proportional allocation with the
     reference's defaults (AdapBilatAtomic.cu:28-38)."""
     p = np.zeros((), RENDER_PARAMS_DTYPE)
@@ -153,6 +154,7 @@ def render_params(width, height, camera, eye_depth=5, light_depth=5, light_paths
     p["mode"], p["nd"], p["sc"], p["sp"], p["sn"], p["sd"] = mode, nd, sc, sp, sn, sd
     p["radius"], p["filter_diff"], p["warmup"] = radius, filter_diff, warmup
     p["pass_frames"] = pass_frames
+    p["min_prob"] = min_prob
     return p
 
 
@@ -197,7 +199,8 @@ class Renderer:
         out = {}
         for k, name, shape, dt in ((0, "gpos", (self.H, self.W, 3), np.float32), (1, "gnorm", (self.H, self.W, 3), np.float32),
                                    (2, "gdiff", (self.H, self.W, 3), np.float32), (3, "filtered", (self.H, self.W, 3), np.float32),
-                                   (4, "err", (self.H, self.W), np.float32), (5, "counts", (self.H, self.W), np.uint32)):
+                                   (4, "err", (self.H, self.W), np.float32), (5, "counts", (self.H, self.W), np.uint32),
+                                   (6, "sq", (self.H, self.W, 3), np.float32), (7, "rounds", (1,), np.uint32)):
             a = np.zeros(shape, dt)
             s = np.zeros(1, np.float32)
             lib().orc_adaptive_state(self.h, k, a.ctypes.data, s.ctypes.data)
@@ -242,6 +245,18 @@ def empirical_error(mean, sq, N):
     lib().orc_empirical_error(np.ascontiguousarray(mean, np.float32).ravel(), np.ascontiguousarray(sq, np.float32).ravel(),
                               np.ascontiguousarray(N, np.uint32).ravel(), n, out)
     return out.reshape(N.shape)
+
+
+def alloc_rounds(err, gen, bisect, desired, min_prob=0.0, seed=0x5EED0001, frame=0):
+    """Round-based allocation (AdapBilatBinary / AdapEmp) of an error map: (counts, rounds)."""
+    e = np.ascontiguousarray(err, np.float32).ravel()
+    out = np.zeros(e.size, np.uint32)
+    L = lib()
+    L.orc_alloc_rounds.restype = ctypes.c_uint32
+    L.orc_alloc_rounds.argtypes = [f32p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_float,
+                                   ctypes.c_float, ctypes.c_uint64, ctypes.c_uint32, u32p]
+    r = L.orc_alloc_rounds(e, e.size, gen, int(bisect), desired, min_prob, seed, frame, out)
+    return out.reshape(np.shape(err)), int(r)
 
 
 def alloc_proportional(err, total, gen):

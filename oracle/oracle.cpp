@@ -488,7 +488,8 @@ struct RenderParams {
   u64 seed;
   u32 row0, rows; // rows [row0, row0 + rows) of the frame are traced (rows = 0: all)
   // adaptive mode (cudaRTBDPTStreamAdapBilatAtomic.cu:26-38)
-  u32 mode;        // 0 plain (cudaRTBDPTStream), 1 adaptive proportional (AdapBilatAtomic)
+  u32 mode;        // 0 plain (cudaRTBDPTStream), 1 adaptive proportional (AdapBilatAtomic),
+                   // 2 rounds with bisection (AdapBilatBinary), 3 empirical variance rounds (AdapEmp)
   u32 nd;          // DesiredSamplingN
   float sc, sp, sn, sd;   // filter colour / position / normal / diffuse sigmas
   u32 radius;      // filter radius
@@ -496,6 +497,8 @@ struct RenderParams {
   u32 warmup;      // plain frames before adaptive sampling starts (5)
   u32 pass_frames; // adaptive frames allocated together: pass_frames * W * H samples from one
                    // error map, traced under the pass's first frame (0 or 1: one frame a pass)
+  float min_prob;  // MinTraceProb (rounds)
+  u32 pad3;
 };
 
 // Shadow-segment end for a connection.  The reference passes dist - eps
@@ -618,6 +621,8 @@ template <class R> struct FrameState {
   std::vector<R> result;      // W*H*3 running mean
   std::vector<u32> N;         // W*H sample counts
   std::vector<R> gpos, gnorm, gdiff;   // W*H*3 first-hit running means (adaptive mode)
+  std::vector<R> sq;                   // W*H*3 running mean of squared samples (squareResult)
+  u32 rounds = 0;                      // last round-based allocation: rounds run
   std::vector<float> filtered, err;    // last adaptive frame: filter output, error map
   std::vector<u32> counts;             // last adaptive frame: samples per pixel
   float err_sum = 0.f;
@@ -933,6 +938,52 @@ static inline void apply_sample_gbuf(FrameState<R>& fs, u32 pix, const PathResul
     q[0] = ri * w.x + oi * q[0]; q[1] = ri * w.y + oi * q[1]; q[2] = ri * w.z + oi * q[2];
   };
   upd(fs.gpos, pr.gpos); upd(fs.gnorm, pr.gnorm); upd(fs.gdiff, pr.gdiff);
+  // squareResult of pt_applyPathQueueResult_kernel (AdapEmp.cu:927-944)
+  R* q = &fs.sq[pix * 3];
+  q[0] = std::fmax(ri * (pr.accum.x * pr.accum.x) + oi * q[0], (R)0);
+  q[1] = std::fmax(ri * (pr.accum.y * pr.accum.y) + oi * q[1], (R)0);
+  q[2] = std::fmax(ri * (pr.accum.z * pr.accum.z) + oi * q[2], (R)0);
+}
+
+// Round-based allocation: pt_genTempAdapPathQueue_kernel rounds, remove_if,
+// pt_fillTempAdapPathQueue_kernel (AdapBilatBinary.cu:1348-1388 with the window
+// bisection; AdapEmp.cu:1232-1260 with the window at max e).  Round r keeps
+// pixel x unless u * window > max(e_x, minProb), u from the PCG32 stream of
+// (frame, slots filled so far) at pixel x; the first kept pixels in pixel order
+// take the remaining slots; unfilled slots go to pixels 0, 1, ... (wrapping).
+static const u32 kMaxRounds = 1024;
+static void alloc_rounds(const RenderParams& p, u32 frame, const std::vector<float>& err, u32 n, u32 gen,
+                         bool bisect, float desired, float min_prob, std::vector<u32>& counts, u32& rounds) {
+  float maxe = 0.f;                                   // thrust::maximum reduce from 0
+  for (u32 x = 0; x < n; x++) maxe = std::max(maxe, std::fmax(err[x], 0.f));
+  float left = 0.f, right = maxe * 2.f;
+  float window = bisect ? (left + right) * 0.5f : maxe;
+  counts.assign(n, 0u);
+  std::vector<unsigned char> keep(n);
+  u32 accum = 0;
+  rounds = 0;
+  while (accum < gen && rounds < kMaxRounds) {
+    u32 kept = 0;
+    for (u32 x = 0; x < n; x++) {
+      Pcg32 rng; pcg32_seed(rng, stream_key(p.seed, frame, 3, accum), x);
+      float u = uniform<float>(rng);
+      keep[x] = (u * window > std::fmax(err[x], min_prob)) ? 0 : 1;
+      kept += keep[x];
+    }
+    u32 take = std::min(kept, gen - accum), rank = 0;
+    for (u32 x = 0; x < n; x++)
+      if (keep[x]) { if (rank < take) counts[x]++; rank++; }
+    accum += take;
+    rounds++;
+    if (take == 0) break;
+    if (bisect) {
+      if ((float)take < desired) right = window;
+      else if ((float)take > desired) left = window;
+      window = (left + right) * 0.5f;
+    }
+  }
+  u32 unfilled = gen > accum ? gen - accum : 0u;
+  for (u32 x = 0; x < n; x++) counts[x] += unfilled / n + (x < unfilled % n ? 1u : 0u);
 }
 
 // One adaptive frame after warm-up (AdapBilatAtomic.cu:1407-1538).
@@ -944,44 +995,63 @@ static void adaptive_frame(const Scene& s, const RenderParams& p, const Camera<R
   for (size_t i = 0; i < (size_t)n * 3; i++) {
     col[i] = (float)fs.result[i]; pos[i] = (float)fs.gpos[i]; nrm[i] = (float)fs.gnorm[i]; dif[i] = (float)fs.gdiff[i];
   }
-  std::vector<float> g;
-  gaussian_table(p.sc, p.radius, g);                       // spatial sigma = colour sigma (:1327)
-  fs.filtered.assign(n * 3, 0.f);
   fs.err.assign(n, 0.f);
   fs.counts.assign(n, 0u);
-  const float* emit = p.filter_diff ? dif.data() : nullptr;
-  if (sizeof(R) == 4) {
-    bilateral_defined(col.data(), pos.data(), nrm.data(), emit, W, H, p.sc, p.sp, p.sn, p.sd, p.radius, g,
-                      fs.filtered.data());
+  if (p.mode == 3) {
+    // pt_calculateEmpiricalSquareError_kernel (AdapEmp.cu:949-960)
+    fs.filtered.clear();
+    for (u32 x = 0; x < n; x++) {
+      u32 sn = fs.N[x];
+      const R* a = &fs.result[x * 3];
+      const R* b = &fs.sq[x * 3];
+      R e = ((b[0] - a[0] * a[0]) + (b[1] - a[1] * a[1]) + (b[2] - a[2] * a[2])) * (R)sn / ((R)(u32)(sn - 1u) * (R)3);
+      fs.err[x] = std::fmin((float)e, 1.0f);
+    }
   } else {
-    bilateral<double>(col.data(), pos.data(), nrm.data(), emit, W, H, p.sc, p.sp, p.sn, p.sd, p.radius, g,
-                      fs.filtered.data());
+    std::vector<float> g;
+    gaussian_table(p.sc, p.radius, g);                     // spatial sigma = colour sigma (:1327)
+    fs.filtered.assign(n * 3, 0.f);
+    const float* emit = p.filter_diff ? dif.data() : nullptr;
+    if (sizeof(R) == 4) {
+      bilateral_defined(col.data(), pos.data(), nrm.data(), emit, W, H, p.sc, p.sp, p.sn, p.sd, p.radius, g,
+                        fs.filtered.data());
+    } else {
+      bilateral<double>(col.data(), pos.data(), nrm.data(), emit, W, H, p.sc, p.sp, p.sn, p.sd, p.radius, g,
+                        fs.filtered.data());
+    }
+    // pt_calculateSquareError_kernel (:1143-1156; AdapBilatBinary.cu:1031-1044)
+    for (u32 x = 0; x < n; x++) {
+      u32 sn = fs.N[x];
+      float red = (float)p.nd / (float)(p.nd + sn);
+      if (p.nd + sn < sn) red = 0;
+      const float* a = &fs.filtered[x * 3];
+      const float* b = &col[x * 3];
+      float d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
+      fs.err[x] = red * ((d0 * d0 + d1 * d1) + d2 * d2) / 3.f;
+    }
   }
-  // pt_calculateSquareError_kernel (:1143-1156)
-  for (u32 x = 0; x < n; x++) {
-    u32 sn = fs.N[x];
-    float red = (float)p.nd / (float)(p.nd + sn);
-    if (p.nd + sn < sn) red = 0;
-    const float* a = &fs.filtered[x * 3];
-    const float* b = &col[x * 3];
-    float d0 = a[0] - b[0], d1 = a[1] - b[1], d2 = a[2] - b[2];
-    fs.err[x] = red * ((d0 * d0 + d1 * d1) + d2 * d2) / 3.f;
+  if (p.mode == 1) {
+    float sum = (float)tree_sum(fs.err.data(), n);
+    fs.err_sum = sum;
+    for (u32 x = 0; x < n; x++) fs.counts[x] = prop_count(fs.err[x], sum, gen);
+  } else {
+    alloc_rounds(p, f, fs.err, n, gen, p.mode == 2, (float)n / (float)(p.nd ? p.nd : 1u), p.min_prob, fs.counts,
+                 fs.rounds);
   }
-  float sum = (float)tree_sum(fs.err.data(), n);
-  fs.err_sum = sum;
-  for (u32 x = 0; x < n; x++) fs.counts[x] = prop_count(fs.err[x], sum, gen);
+  const int variant = p.mode == 1 ? 1 : 0;      // reflection offset of AdapBilatAtomic's eye bounce
   u64 er = 0, sr = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : er, sr)
   for (i32 ii = 0; ii < (i32)n; ii++) {
     u32 pix = (u32)ii, y = pix / W, x = pix - y * W;
-    V3<R> ts = mk((R)0, (R)0, (R)0), tp = ts, tn = ts, td = ts;
+    V3<R> ts = mk((R)0, (R)0, (R)0), tp = ts, tn = ts, td = ts, tq = ts;
     u32 tN = 0;
     for (u32 k = 0; k < fs.counts[pix]; k++) {
-      PathResult<R> pr = eye_path<R>(s, p, cam, fs, f, x, y, k, 1);
+      PathResult<R> pr = eye_path<R>(s, p, cam, fs, f, x, y, k, variant);
       er += pr.ext_rays; sr += pr.shadow_rays;
       if (pr.N == 0 || isinf3(pr.accum)) continue;        // pt_atomicApplyPathQueueResult_kernel (:1056-1061)
       R nf = (R)pr.N;
       ts = ts + pr.accum; tN += pr.N;
+      tq = tq + vmul(pr.accum, pr.accum);
       tp = tp + pr.gpos * nf; tn = tn + pr.gnorm * nf; td = td + pr.gdiff * nf;
     }
     // pt_accumTempResultToResult_kernel (:1117-1140)
@@ -998,6 +1068,10 @@ static void adaptive_frame(const Scene& s, const RenderParams& p, const Camera<R
         q[0] = ri * v.x + oi * q[0]; q[1] = ri * v.y + oi * q[1]; q[2] = ri * v.z + oi * q[2];
       };
       upd(fs.gpos, tp); upd(fs.gnorm, tn); upd(fs.gdiff, td);
+      R* q = &fs.sq[pix * 3];
+      q[0] = std::fmax(ri * tq.x + oi * q[0], (R)0);
+      q[1] = std::fmax(ri * tq.y + oi * q[1], (R)0);
+      q[2] = std::fmax(ri * tq.z + oi * q[2], (R)0);
     }
   }
   fs.ext_rays += er; fs.shadow_rays += sr;
@@ -1009,7 +1083,9 @@ static void render_adaptive(const Scene& s, const RenderParams& p, u32 frame0, u
   const u32 W = p.width, H = p.height, n = W * H;
   if (fs.gpos.size() != (size_t)n * 3) {
     fs.gpos.assign(n * 3, (R)0); fs.gnorm.assign(n * 3, (R)0); fs.gdiff.assign(n * 3, (R)0);
+    fs.sq.assign(n * 3, (R)0);
   }
+  const int variant = p.mode == 1 ? 1 : 0;
   for (u32 f = frame0; f < frame0 + nframes;) {
     // the cache of the frame's three-frame group (built at its first frame)
     if (f % 3 == 0 || fs.light_group != f / 3) light_pass<R>(s, p, f - f % 3, fs);
@@ -1021,7 +1097,7 @@ static void render_adaptive(const Scene& s, const RenderParams& p, u32 frame0, u
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : er, sr)
       for (i32 ii = 0; ii < (i32)n; ii++) {
         u32 pix = (u32)ii, y = pix / W, x = pix - y * W;
-        PathResult<R> pr = eye_path<R>(s, p, cam, fs, f, x, y, 0, 1);
+        PathResult<R> pr = eye_path<R>(s, p, cam, fs, f, x, y, 0, variant);
         er += pr.ext_rays; sr += pr.shadow_rays;
         apply_sample_gbuf(fs, pix, pr);
       }
@@ -1157,7 +1233,7 @@ void orc_render(void* scene, void* rh, u32 frame0, u32 nframes, float* result, u
   size_t n = (size_t)r->p.width * r->p.height;
   if (r->prec == 64) {
     std::vector<double> g; if (gbuf) g.assign(n * 9, 0.0);
-    if (r->p.mode == 1) render_adaptive<double>(*s, r->p, frame0, nframes, r->f64);
+    if (r->p.mode != 0) render_adaptive<double>(*s, r->p, frame0, nframes, r->f64);
     else render_frames<double>(*s, r->p, frame0, nframes, r->f64, gbuf ? g.data() : nullptr);
     if (result) for (size_t i = 0; i < n * 3; i++) result[i] = (float)r->f64.result[i];
     if (N) std::memcpy(N, r->f64.N.data(), n * 4);
@@ -1165,7 +1241,7 @@ void orc_render(void* scene, void* rh, u32 frame0, u32 nframes, float* result, u
     if (lvN) *lvN = r->f64.lvN;
   } else {
     std::vector<float> g; if (gbuf) g.assign(n * 9, 0.f);
-    if (r->p.mode == 1) render_adaptive<float>(*s, r->p, frame0, nframes, r->f32);
+    if (r->p.mode != 0) render_adaptive<float>(*s, r->p, frame0, nframes, r->f32);
     else render_frames<float>(*s, r->p, frame0, nframes, r->f32, gbuf ? g.data() : nullptr);
     if (result) std::memcpy(result, r->f32.result.data(), n * 12);
     if (N) std::memcpy(N, r->f32.N.data(), n * 4);
@@ -1189,6 +1265,8 @@ void orc_adaptive_state(void* rh, u32 which, void* out, float* sum) {
     else if (which == 3) { if (fs.filtered.size()) std::memcpy(out, fs.filtered.data(), n * 12); }
     else if (which == 4) { if (fs.err.size()) std::memcpy(out, fs.err.data(), n * 4); }
     else if (which == 5) { if (fs.counts.size()) std::memcpy(out, fs.counts.data(), n * 4); }
+    else if (which == 6) cp(fs.sq);
+    else if (which == 7) *(u32*)out = fs.rounds;
     if (sum) *sum = fs.err_sum;
   };
   if (r->prec == 64) dump(r->f64); else dump(r->f32);
@@ -1270,6 +1348,20 @@ void orc_empirical_error(const float* mean, const float* sq, const u32* N, u32 n
     out[x] = std::fmin(e, 1.0f);
   }
 }
+// Round-based allocation of a given error map (alloc_rounds); returns the rounds run.
+u32 orc_alloc_rounds(const float* err, u32 n, u32 gen, u32 bisect, float desired, float min_prob, u64 seed,
+                     u32 frame, u32* counts) {
+  RenderParams p;
+  std::memset(&p, 0, sizeof p);
+  p.seed = seed;
+  std::vector<float> e(err, err + n);
+  std::vector<u32> c;
+  u32 rounds = 0;
+  alloc_rounds(p, frame, e, n, gen, bisect != 0, desired, min_prob, c, rounds);
+  std::memcpy(counts, c.data(), sizeof(u32) * n);
+  return rounds;
+}
+
 // pt_genAdapPathQueue_kernel counts (AdapBilatAtomic.cu:830-831): floor(gen * e/sum)
 void orc_alloc_proportional(const float* err, u32 n, float sum, u32 gen, u32* counts) {
   for (u32 x = 0; x < n; x++) { float r = err[x] / sum; counts[x] = (u32)((float)gen * r); }

@@ -333,3 +333,76 @@ def test_error_estimators():
     err = np.random.default_rng(1).uniform(size=1000).astype(np.float32)
     c = oracle.alloc_proportional(err, float(err.sum()), 1000)
     assert c.sum() <= 1000 and c.sum() >= 1000 - 1000
+
+
+# ---- round-based allocation (AdapBilatBinary / AdapEmp) ----------------------
+
+def test_rounds_constant_error_is_uniform():
+    """Every pixel at the maximum error is kept every round: gen / n rounds of one sample each."""
+    e = np.full(64, 0.25, np.float32)
+    counts, rounds = oracle.alloc_rounds(e, 64 * 3, bisect=False, desired=64 / 5)
+    assert rounds == 3 and (counts == 3).all()
+
+
+def test_rounds_single_hot_pixel_hits_the_round_cap_then_fills():
+    """Only the hot pixel survives u * max > e: one slot a round, 1024 rounds, then
+    the fill gives the remaining slots to pixels 0, 1, ... in order."""
+    n = 2048
+    e = np.zeros(n, np.float32)
+    e[700] = 1.0
+    counts, rounds = oracle.alloc_rounds(e, n, bisect=False, desired=n / 5)
+    assert rounds == 1024
+    assert counts[700] == 1024 + (1 if 700 < n - 1024 else 0)
+    expect = np.zeros(n, np.uint32)
+    expect[:n - 1024] = 1
+    expect[700] += 1024
+    assert np.array_equal(counts, expect)
+
+
+def test_rounds_min_probability_keeps_cold_pixels():
+    n = 4096
+    e = np.zeros(n, np.float32)
+    e[5] = 1.0
+    counts, rounds = oracle.alloc_rounds(e, n, bisect=False, desired=n / 5, min_prob=0.5)
+    assert counts.sum() == n and rounds < 10
+    frac = (counts[np.arange(n) != 5] > 0).mean()
+    assert 0.3 < frac < 0.9            # each round keeps a cold pixel with probability 1/2
+
+
+def test_rounds_bisection_aims_at_desired_round_size():
+    rng = np.random.default_rng(3)
+    e = rng.exponential(1.0, 8192).astype(np.float32)
+    counts, rounds = oracle.alloc_rounds(e, 8192, bisect=True, desired=8192 / 5)
+    assert counts.sum() == 8192
+    assert 3 <= rounds <= 12
+    hi, lo = e > np.quantile(e, 0.9), e < np.quantile(e, 0.1)
+    assert counts[hi].mean() > 2 * counts[lo].mean()
+
+
+def test_rounds_sum_to_gen_and_are_deterministic():
+    rng = np.random.default_rng(8)
+    e = rng.uniform(0, 1, 3000).astype(np.float32)
+    a, ra = oracle.alloc_rounds(e, 9000, bisect=True, desired=600)
+    b, rb = oracle.alloc_rounds(e, 9000, bisect=True, desired=600)
+    c, rc = oracle.alloc_rounds(e, 9000, bisect=True, desired=600, frame=1)
+    assert a.sum() == 9000 and np.array_equal(a, b) and ra == rb
+    assert not np.array_equal(a, c)
+
+
+def test_empirical_error_modes_render():
+    """Binary and empirical modes: FP32 and FP64 restatements agree, every pass
+    draws exactly its W*H slots."""
+    sc = scenes.cornell()
+    O = oracle.Scene(sc)
+    for mode in (2, 3):
+        p = oracle.render_params(24, 20, sc["camera"], mode=mode)
+        r32 = oracle.Renderer(p, 32)
+        a, _ = r32.render(O, 0, 8)
+        st = r32.adaptive_state()
+        assert st["counts"].sum() == 24 * 20 and st["rounds"][0] >= 1
+        r64 = oracle.Renderer(p, 64)
+        r64.render(O, 0, 8)
+        b = r64.result64()
+        assert ((a - b) ** 2).mean() <= 1e-6 * (b ** 2).mean()
+        if mode == 3:
+            assert st["err"].max() <= 1.0
