@@ -528,7 +528,7 @@ struct cpt_renderer {
   float gauss_sigma = -1.f;
   u32 gauss_radius = 0;
   cpt_mlp* mlp = nullptr;              // learned error model (adaptive mode)
-  DevArray<float> mlp_x, mlp_y, mlp_e;
+  DevArray<float> mlp_x, mlp_y, mlp_e, sq_f, sq_e;
   u32 cap = 0;
   long long slot_group[kLightSlots];   // three-frame group whose light cache each slot holds (-1: none)
   void forget_light_caches() { for (auto& g : slot_group) g = -1; }
@@ -694,9 +694,18 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
 }
 
 // One frame of cudaRTBDPTStreamAdapBilatAtomic::Render (AdapBilatAtomic.cu:1338-1538).
+MlpFeatureArgs feature_args(const cpt_renderer* r) {
+  MlpFeatureArgs a;
+  a.result = r->result.p; a.filtered = r->filtered.p; a.err = r->err.p; a.sq = r->sq.p; a.sq_f = r->sq_f.p;
+  a.gpos = r->gpos.p; a.gnrm = r->gnrm.p; a.gdif = r->gdif.p;
+  a.sample_n = r->sample_n.p;
+  a.cam[0] = r->cam[0]; a.cam[1] = r->cam[1]; a.cam[2] = r->cam[2];
+  return a;
+}
+
 // Cross-bilateral filter of the running mean and the weighted squared residual
 // (src/cudaRTBDPTStreamAdapBilatAtomic.cu:1409-1430) into r->filtered / r->err.
-int run_filter(cpt_renderer* r) {
+int run_filter(cpt_renderer* r, const float* src, float* dst, float* err) {
   float sc_ = (float)r->attr("Filter Color EuD"), sp_ = (float)r->attr("Filter Pos EuD");
   float sn_ = (float)r->attr("Filter Norm EuD"), sd_ = (float)r->attr("Filter Diff EuD");
   u32 radius = (u32)r->attr("Filter Radius");
@@ -707,8 +716,32 @@ int run_filter(cpt_renderer* r) {
   f.width = r->W; f.height = r->H; f.radius = radius; f.nd = (u32)r->attr("Desired Max Sampling");
   f.kc = 2.f * sc_ * sc_; f.kp = 2.f * sp_ * sp_; f.kn = 2.f * sn_ * sn_; f.kd = 2.f * sd_ * sd_;
   f.use_diff = r->attr("Filter Diffuse Flag") != 0;
-  HIP_TRY(launch_filter_error(f, r->result.p, r->gpos.p, r->gnrm.p, r->gdif.p, r->gauss.p, r->sample_n.p,
-                              r->filtered.p, r->err.p, r->stream));
+  HIP_TRY(launch_filter_error(f, src, r->gpos.p, r->gnrm.p, r->gdif.p, r->gauss.p, r->sample_n.p, dst, err,
+                              r->stream));
+  return CPT_OK;
+}
+int run_filter(cpt_renderer* r) { return run_filter(r, r->result.p, r->filtered.p, r->err.p); }
+
+// Inputs of the learned model: the second moment filtered like the image (its
+// residual is discarded), then the per-pixel features.
+int model_buffers(cpt_renderer* r) {
+  const size_t n = (size_t)r->W * r->H;
+  HIP_TRY(r->sq_f.alloc(n * 3));
+  HIP_TRY(r->sq_e.alloc(n));
+  if (r->mlp) {
+    HIP_TRY(r->mlp_x.alloc(n * kMlpIn));
+    HIP_TRY(r->mlp_y.alloc(n));
+    HIP_TRY(r->mlp_e.alloc(n));
+  }
+  return CPT_OK;
+}
+int model_features(cpt_renderer* r, float* d_x) {
+  const size_t n = (size_t)r->W * r->H;
+  int rc = model_buffers(r);
+  if (rc) return rc;
+  rc = run_filter(r, r->sq.p, r->sq_f.p, r->sq_e.p);
+  if (rc) return rc;
+  HIP_TRY(launch_mlp_features(feature_args(r), (u32)n, d_x, r->stream));
   return CPT_OK;
 }
 
@@ -782,12 +815,9 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
       HIP_TRY(timed_end(r, 5, ev));
     }
     if (r->mlp) {                                  // learned estimate replaces the filter residual
-      HIP_TRY(r->mlp_x.alloc((size_t)n * kMlpIn));
-      HIP_TRY(r->mlp_y.alloc(n));
-      HIP_TRY(r->mlp_e.alloc(n));
       HIP_TRY(timed_begin(r, 9, &ev));
-      HIP_TRY(launch_mlp_features(r->result.p, r->filtered.p, r->err.p, r->gpos.p, r->gnrm.p, r->gdif.p,
-                                  r->sample_n.p, r->cam, n, r->mlp_x.p, r->stream));
+      rc = model_features(r, r->mlp_x.p);
+      if (rc) return rc;
       HIP_TRY(launch_mlp_infer(r->mlp->state(), r->mlp_x.p, n, r->mlp_y.p, r->stream));
       HIP_TRY(launch_mlp_err(r->mlp_y.p, r->sample_n.p, nd, (u32)r->attr(kLearnedSignal), n, r->mlp_e.p, r->stream));
       HIP_TRY(timed_end(r, 9, ev));
@@ -881,6 +911,10 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   }
   int rc = lv_alloc(r);
   if (rc) return rc;
+  if (r->mlp) {
+    rc = model_buffers(r);
+    if (rc) return rc;
+  }
   float cam[9] = {pos[0], pos[1], pos[2], dir[0], dir[1], dir[2], up[0], up[1], up[2]};
   bool reset = !r->have_cam || std::memcmp(cam, r->cam, sizeof cam) != 0 || r->scene != s ||
                r->scene_mesh_version != s->mesh_version || r->scene_mat_version != s->mat_version;
@@ -1258,8 +1292,8 @@ int cpt_renderer_mlp_features(cpt_renderer* r, float* d_x) {
   // the filter state the next adaptive frame allocates from
   int rc = run_filter(r);
   if (rc) return rc;
-  HIP_TRY(launch_mlp_features(r->result.p, r->filtered.p, r->err.p, r->gpos.p, r->gnrm.p, r->gdif.p, r->sample_n.p,
-                              r->cam, r->W * r->H, d_x, r->stream));
+  rc = model_features(r, d_x);
+  if (rc) return rc;
   HIP_TRY(hipStreamSynchronize(r->stream));
   return CPT_OK;
 }

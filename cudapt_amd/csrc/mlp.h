@@ -56,9 +56,19 @@ size_t mlp_lds_bytes();
 // Renderer-side kernels (images W*H x 3 FP32, n = W*H pixels):
 // the 9 features of each pixel, the log per-sample-variance training target
 // against a converged image, and the allocation signal from model outputs y.
-hipError_t launch_mlp_features(const float* result, const float* filtered, const float* err, const float* gpos,
-                               const float* gnrm, const float* gdif, const uint32_t* sample_n, const float* cam_pos,
-                               uint32_t n, float* x, hipStream_t s);
+struct MlpFeatureArgs {
+  const float* result;      // running mean
+  const float* filtered;    // cross-bilateral output
+  const float* err;         // filter residual
+  const float* sq;          // running mean of squared samples
+  const float* sq_f;        // the same, cross-bilateral filtered
+  const float* gpos;
+  const float* gnrm;
+  const float* gdif;
+  const uint32_t* sample_n;
+  float cam[3];
+};
+hipError_t launch_mlp_features(const MlpFeatureArgs& a, uint32_t n, float* x, hipStream_t s);
 hipError_t launch_mlp_target(const float* result, const float* ref, const uint32_t* sample_n, uint32_t n, float clamp,
                              float* t, hipStream_t s);
 // signal 0: nd / (nd + N) * exp(y) / N (the filter-residual estimate's shape);

@@ -337,24 +337,27 @@ hipError_t launch_mlp_pack(const MlpState& st, hipStream_t s) {
 
 __device__ __forceinline__ float luminance(const float* c) { return 0.2126f * c[0] + 0.7152f * c[1] + 0.0722f * c[2]; }
 
-__global__ __launch_bounds__(256) void k_mlp_features(const float* __restrict__ result, const float* __restrict__ filtered,
-                                                      const float* __restrict__ err, const float* __restrict__ gpos,
-                                                      const float* __restrict__ gnrm, const float* __restrict__ gdif,
-                                                      const uint32_t* __restrict__ sample_n, float cx, float cy,
-                                                      float cz, uint32_t n, float* __restrict__ x) {
+__global__ __launch_bounds__(256) void k_mlp_features(MlpFeatureArgs a, uint32_t n, float* __restrict__ x) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
+  const size_t i3 = 3 * (size_t)i;
+  const float* r = a.result + i3;
+  const float* q = a.sq + i3;
   float* o = x + (size_t)i * kMlpIn;
-  o[0] = log1pf(luminance(result + 3 * (size_t)i));
-  o[1] = log1pf(luminance(filtered + 3 * (size_t)i));
-  o[2] = logf(err[i] + 1e-12f) * 0.1f;
-  o[3] = luminance(gdif + 3 * (size_t)i);
-  o[4] = gnrm[3 * (size_t)i];
-  o[5] = gnrm[3 * (size_t)i + 1];
-  o[6] = gnrm[3 * (size_t)i + 2];
-  float dx = gpos[3 * (size_t)i] - cx, dy = gpos[3 * (size_t)i + 1] - cy, dz = gpos[3 * (size_t)i + 2] - cz;
-  o[7] = log1pf(sqrtf(dx * dx + dy * dy + dz * dz)) * 0.2f;
-  o[8] = logf((float)max(sample_n[i], 1u)) * 0.2f;
+  o[0] = log1pf(luminance(r));
+  o[1] = log1pf(luminance(a.filtered + i3));
+  o[2] = logf(a.err[i] + 1e-12f) * 0.1f;
+  float v = ((q[0] - r[0] * r[0]) + (q[1] - r[1] * r[1]) + (q[2] - r[2] * r[2])) / 3.f;
+  o[3] = logf(fmaxf(v, 0.f) + 1e-12f) * 0.1f;
+  o[4] = luminance(a.gdif + i3);
+  const float* qf = a.sq_f + i3;
+  const float* f = a.filtered + i3;
+  float vf = ((qf[0] - f[0] * f[0]) + (qf[1] - f[1] * f[1]) + (qf[2] - f[2] * f[2])) / 3.f;
+  o[5] = logf(fmaxf(vf, 0.f) + 1e-12f) * 0.1f;
+  float dx = a.gpos[i3] - a.cam[0], dy = a.gpos[i3 + 1] - a.cam[1], dz = a.gpos[i3 + 2] - a.cam[2];
+  o[6] = log1pf(sqrtf(dx * dx + dy * dy + dz * dz)) * 0.2f;
+  o[7] = logf((float)max(a.sample_n[i], 1u)) * 0.2f;
+  o[8] = a.gnrm[i3 + 1];
 }
 
 __global__ __launch_bounds__(256) void k_mlp_target(const float* __restrict__ result, const float* __restrict__ ref,
@@ -379,14 +382,12 @@ __global__ __launch_bounds__(256) void k_mlp_err(const float* __restrict__ y, co
   e[i] = signal ? v / (float)N : (float)nd / (float)(nd + N) * v / (float)N;
 }
 
-hipError_t launch_mlp_features(const float* result, const float* filtered, const float* err, const float* gpos,
-                               const float* gnrm, const float* gdif, const uint32_t* sample_n, const float* cam_pos,
-                               uint32_t n, float* x, hipStream_t s) {
-  if (!result || !filtered || !err || !gpos || !gnrm || !gdif || !sample_n || !cam_pos || !x)
+hipError_t launch_mlp_features(const MlpFeatureArgs& a, uint32_t n, float* x, hipStream_t s) {
+  if (!a.result || !a.filtered || !a.err || !a.sq || !a.sq_f || !a.gpos || !a.gnrm || !a.gdif || !a.sample_n ||
+      !x)
     return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
-  k_mlp_features<<<(n + 255) / 256, 256, 0, s>>>(result, filtered, err, gpos, gnrm, gdif, sample_n, cam_pos[0],
-                                                 cam_pos[1], cam_pos[2], n, x);
+  k_mlp_features<<<(n + 255) / 256, 256, 0, s>>>(a, n, x);
   return hipGetLastError();
 }
 hipError_t launch_mlp_target(const float* result, const float* ref, const uint32_t* sample_n, uint32_t n, float clamp,

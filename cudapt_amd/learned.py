@@ -62,8 +62,8 @@ def converged(scene: Scene, cam, W, H, spp, depth=8, in_flight=64, seed=REF_SEED
     return R.render(scene, cam, frames=spp)
 
 
-def _adaptive(W, H, depth, pass_frames=1, signal=0, **attribs):
-    R = Renderer(W, H, mode=CPT_MODE_BDPT_ADAPTIVE_PLL, eye_depth=depth, light_depth=depth, **attribs)
+def _adaptive(W, H, depth, pass_frames=1, signal=0, mode=CPT_MODE_BDPT_ADAPTIVE_PLL, **attribs):
+    R = Renderer(W, H, mode=mode, eye_depth=depth, light_depth=depth, **attribs)
     R.set_attrib("Adaptive Pass Frames", pass_frames)
     R.set_attrib("Learned Signal", signal)
     return R
@@ -124,13 +124,18 @@ def mse_curve(scene: Scene, cam, ref, W, H, frames, model: Mlp | None = None, mo
     """(frames, cumulative render seconds, MSE) after each render call of
     pass_frames frames.  With stop_below the run ends at the first call whose
     MSE is at or below it."""
-    if mode == CPT_MODE_BDPT_ADAPTIVE_PLL:
-        R = _adaptive(W, H, depth, pass_frames, signal)
+    if mode != CPT_MODE_BDPT_STREAM:
+        R = _adaptive(W, H, depth, pass_frames, signal, mode=mode)
         if model is not None:
             R.set_mlp(model)
     else:
         R = Renderer(W, H, mode=mode, eye_depth=depth, light_depth=depth)
         R.set_attrib("Frames In Flight", pass_frames)
+    # allocations happen on the first render: done on a nearby camera (whose
+    # accumulation the timed run's camera change discards), outside the timing
+    warm = dict(cam, pos=tuple(np.asarray(cam["pos"]) + 1e-3))
+    R.render(scene, warm, frames=min(frames, 6 + 2 * pass_frames), result=False)
+    synchronize()
     fs, ts, ms = [], [], []
     total = 0.0
     done = 0

@@ -250,12 +250,15 @@ int cpt_mlp_train_step(cpt_mlp* m, const float* d_x, const float* d_target, uint
 /* Inference of n samples into d_y (device FP32). */
 int cpt_mlp_infer(cpt_mlp* m, const float* d_x, uint32_t n, float* d_y, void* stream);
 
-/* Per-pixel features of an adaptive renderer's current state (after at least
- * one adaptive frame), W*H x 9 FP32 into device memory, with L = Rec.709
- * luminance, r = running mean, f = filtered image, e = filter-residual error,
- * a = first-hit albedo, n = first-hit normal, p = first-hit position, c = camera:
- *   log1p(L(r)), log1p(L(f)), log(e + 1e-12) / 10, L(a), n.x, n.y, n.z,
- *   log1p(|p - c|) / 5, log(N) / 5. */
+/* Per-pixel features of a filter-based adaptive renderer's current state, W*H
+ * x 9 FP32 into device memory, with L = Rec.709 luminance, r = running mean,
+ * s = running mean of squared samples, f = filtered image and e = filter
+ * residual (both recomputed from the current state, as the next pass will),
+ * s' = s cross-bilateral filtered with the same guides, v = max(0, mean over
+ * channels of s - r^2), v' = max(0, mean of s' - f^2), a = first-hit albedo,
+ * n = first-hit normal, p = first-hit position, c = camera:
+ *   log1p(L(r)), log1p(L(f)), log(e + 1e-12) / 10, log(v + 1e-12) / 10, L(a),
+ *   log(v' + 1e-12) / 10, log1p(|p - c|) / 5, log(max(N, 1)) / 5, n.y. */
 int cpt_renderer_mlp_features(cpt_renderer* r, float* d_x);
 /* Training target of each pixel against a converged image ref (device,
  * W*H x 3 FP32): log(N * |r - ref|^2 / 3 + 1e-12), an estimate of the log

@@ -143,24 +143,29 @@ def luminance(c):
     return (np.float32(0.2126) * c[..., 0] + np.float32(0.7152) * c[..., 1]) + np.float32(0.0722) * c[..., 2]
 
 
-def features(result, filtered, err, gpos, gnrm, gdif, sample_n, cam_pos) -> np.ndarray:
+def features(result, filtered, err, sq, sq_f, gpos, gnrm, gdif, sample_n, cam_pos) -> np.ndarray:
     """The 9 per-pixel inputs of the model (FP64 evaluation of the defined formulas)."""
     r = np.asarray(result, np.float64).reshape(-1, 3)
     f = np.asarray(filtered, np.float64).reshape(-1, 3)
     e = np.asarray(err, np.float64).ravel()
+    s = np.asarray(sq, np.float64).reshape(-1, 3)
     a = np.asarray(gdif, np.float64).reshape(-1, 3)
+    sf = np.asarray(sq_f, np.float64).reshape(-1, 3)
     nr = np.asarray(gnrm, np.float64).reshape(-1, 3)
     ps = np.asarray(gpos, np.float64).reshape(-1, 3)
-    N = np.maximum(np.asarray(sample_n, np.float64).ravel(), 1.0)
+    N = np.asarray(sample_n, np.float64).ravel()
     w = np.array([0.2126, 0.7152, 0.0722])
+    v = np.maximum((s - r * r).mean(1), 0.0)
     x = np.empty((len(e), N_IN))
     x[:, 0] = np.log1p(r @ w)
     x[:, 1] = np.log1p(f @ w)
     x[:, 2] = np.log(e + 1e-12) * 0.1
-    x[:, 3] = a @ w
-    x[:, 4:7] = nr
-    x[:, 7] = np.log1p(np.linalg.norm(ps - np.asarray(cam_pos, np.float64), axis=1)) * 0.2
-    x[:, 8] = np.log(N) * 0.2
+    x[:, 3] = np.log(v + 1e-12) * 0.1
+    x[:, 4] = a @ w
+    x[:, 5] = np.log(np.maximum((sf - f * f).mean(1), 0.0) + 1e-12) * 0.1
+    x[:, 6] = np.log1p(np.linalg.norm(ps - np.asarray(cam_pos, np.float64), axis=1)) * 0.2
+    x[:, 7] = np.log(np.maximum(N, 1.0)) * 0.2
+    x[:, 8] = nr[:, 1]
     return x
 
 

@@ -121,7 +121,8 @@ def adaptive_state(cpt):
     x = dx.download(np.float32, W * H * 9).reshape(-1, 9)
     st = {k: R.read(b) for k, b in (("result", cpt.CPT_BUF_RESULT), ("filtered", cpt.CPT_BUF_FILTERED),
                                     ("err", cpt.CPT_BUF_ERROR), ("gpos", cpt.CPT_BUF_GBUF_POS),
-                                    ("gnrm", cpt.CPT_BUF_GBUF_NORM), ("gdif", cpt.CPT_BUF_GBUF_DIFF))}
+                                    ("gnrm", cpt.CPT_BUF_GBUF_NORM), ("gdif", cpt.CPT_BUF_GBUF_DIFF),
+                                    ("sq", cpt.CPT_BUF_SECOND_MOMENT), ("counts", cpt.CPT_BUF_SAMPLE_COUNTS))}
     st["N"] = R.sample_counts()
     return dict(sc=sc, S=S, R=R, x=x, st=st, W=W, H=H)
 
@@ -129,9 +130,16 @@ def adaptive_state(cpt):
 def test_features_match_formulas(adaptive_state):
     a = adaptive_state
     st = a["st"]
-    ref = M.features(st["result"], st["filtered"], st["err"], st["gpos"], st["gnrm"], st["gdif"], st["N"],
-                     a["sc"]["camera"]["pos"])
-    np.testing.assert_allclose(a["x"], ref, rtol=1e-5, atol=2e-6)
+    sqf = oracle.bilateral_defined(st["sq"], st["gpos"], st["gnrm"], None, 50.0, 1.0, 0.25, 0.1, 5)
+    ref = M.features(st["result"], st["filtered"], st["err"], st["sq"], sqf, st["gpos"], st["gnrm"], st["gdif"],
+                     st["N"], a["sc"]["camera"]["pos"])
+    # the variance proxies cancel (s - r^2): compared where they are well above rounding
+    ok = np.ones(ref.shape, bool)
+    for col, (s, r) in ((3, (st["sq"], st["result"])), (5, (sqf, st["filtered"]))):
+        s, r = s.reshape(-1, 3).astype(np.float64), r.reshape(-1, 3).astype(np.float64)
+        ok[:, col] = (s - r * r).mean(1) > 1e-4 * np.maximum(s.mean(1), 1e-30)
+        assert ok[:, col].mean() > 0.5
+    np.testing.assert_allclose(a["x"][ok], ref[ok], rtol=1e-5, atol=2e-6)
     # the filter state is the one the next frame allocates from
     np.testing.assert_array_equal(st["filtered"].view(np.uint32),
                                   oracle.bilateral_defined(st["result"], st["gpos"], st["gnrm"], None, 50.0, 1.0,

@@ -112,11 +112,13 @@ def test_feature_target_and_signal_formulas():
     n = np.array([[0.0, 1.0, 0.0], [0.0, 0.0, -1.0]])
     pos = np.array([[3.0, 4.0, 0.0], [0.0, 0.0, 0.0]])
     N = np.array([np.e ** 5, 0.0])
-    x = M.features(r, f, e, pos, n, a, N, (0.0, 0.0, 0.0))
-    np.testing.assert_allclose(x[0], [np.log(2), 1.0, 0.0, 0.5, 0, 1, 0, np.log(6) * 0.2, 1.0], atol=1e-12)
+    sq = np.array([[1.0 + np.e ** 10, 1.0 + np.e ** 10, 1.0 + np.e ** 10], [0.0, 0.0, 0.0]])
+    sqf = np.array([[(np.e - 1) ** 2 + np.e ** 20] * 3, [0.0, 0.0, 0.0]])
+    x = M.features(r, f, e, sq, sqf, pos, n, a, N, (0.0, 0.0, 0.0))
+    np.testing.assert_allclose(x[0], [np.log(2), 1.0, 0.0, 1.0, 0.5, 2.0, np.log(6) * 0.2, 1.0, 1.0], atol=1e-12)
     np.testing.assert_allclose(x[1, :2], [0, 0])
-    assert abs(x[1, 2] - np.log(1e-12) * 0.1) < 1e-12
-    assert x[1, 3] == 0.2126 and x[1, 8] == 0.0          # N = 0 counts as 1
+    assert abs(x[1, 2] - np.log(1e-12) * 0.1) < 1e-12 and abs(x[1, 3] - np.log(1e-12) * 0.1) < 1e-12
+    assert x[1, 4] == 0.2126 and x[1, 7] == 0.0 and x[1, 8] == 0.0   # N = 0 counts as 1
     t = M.target(np.array([[1.0, 2.0, 3.0]]), np.array([[0.0, 2.0, 0.0]]), np.array([3]))
     assert abs(t[0] - np.log(10.0 + 1e-12)) < 1e-12
     t = M.target(np.array([[1.0, 2.0, 3.0]]), np.array([[0.0, 2.0, 0.0]]), np.array([3]), clamp=1.5)
