@@ -112,3 +112,16 @@ def test_black_scene(cpt):
     R = cpt.Renderer(32, 32)
     img = R.render(S, sc["camera"], frames=3)
     assert (img == 0).all()
+
+
+def test_rmse_harness_on_device(cpt, cornell_runs):
+    """RMSE against the 16-spp FP64 image falls as frames accumulate."""
+    from cudapt_amd import rmse
+    sc, S = cornell_runs["sc"], cornell_runs["S"]
+    R = cpt.Renderer(W, H)
+    h = rmse.RmseHarness(R, S, sc["camera"], cornell_runs["r64"].astype(np.float32), frames_per_step=2)
+    rows = h.collect(interval=1e-9, count=8)
+    assert [r.frames for r in rows] == [2, 4, 6, 8, 10, 12, 14, 16]
+    # the same streams as the 16-frame FP64 run: the last row differs by precision only
+    assert rows[-1].rmse < 1e-2 * float(np.sqrt((cornell_runs["r64"] ** 2).mean()))
+    assert rows[0].rmse > 10 * rows[-1].rmse
