@@ -84,6 +84,18 @@ def lib():
             "cpt_renderer_stats": [vp, vp, i32],
             "cpt_renderer_set_profiling": [vp, i32],
             "cpt_renderer_destroy": [vp],
+            "cpt_batch_create": [vp, vp, c32],
+            "cpt_batch_get_config": [vp, vp],
+            "cpt_batch_set_config": [vp, vp],
+            "cpt_batch_render": [vp, vp, c32],
+            "cpt_batch_read_result": [vp, c32, vp],
+            "cpt_batch_sample_counts": [vp, c32, vp],
+            "cpt_batch_stats": [vp, vp, i32],
+            "cpt_batch_reset": [vp],
+            "cpt_batch_autotune": [vp, vp, c32, vp, vp, c32],
+            "cpt_batch_candidates": [vp, vp],
+            "cpt_batch_candidate": [vp, c32, vp],
+            "cpt_batch_destroy": [vp],
             "cpt_mlp_create": [vp, ctypes.c_uint64],
             "cpt_mlp_destroy": [vp],
             "cpt_mlp_set_params": [vp, vp, c32],
@@ -336,6 +348,97 @@ class Renderer:
         s = _Stats()
         _check(lib().cpt_renderer_stats(self.h, ctypes.byref(s), int(reset)), "cpt_renderer_stats")
         return {n: getattr(s, n) for n, _ in s._fields_}
+
+
+class _View(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32), ("cam_pos", ctypes.c_float * 3),
+                ("cam_dir", ctypes.c_float * 3), ("cam_up", ctypes.c_float * 3), ("fov", ctypes.c_float)]
+
+
+class BatchConfig(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in ("frames_in_flight", "extend_chunk", "extend_workgroups",
+                                               "traversal_variant", "use_graph", "eye_depth", "light_depth",
+                                               "light_paths")] + [("seed", ctypes.c_uint64)]
+
+    def as_dict(self):
+        return {n: getattr(self, n) for n, _ in self._fields_}
+
+
+class Batch:
+    """Many camera views of uneven resolution rendered as one wavefront (plain streaming BDPT)."""
+
+    def __init__(self, views, **config):
+        arr = (_View * len(views))()
+        for a, v in zip(arr, views):
+            a.width, a.height = int(v["width"]), int(v["height"])
+            a.cam_pos[:] = [float(x) for x in v["pos"]]
+            a.cam_dir[:] = [float(x) for x in v["dir"]]
+            a.cam_up[:] = [float(x) for x in v["up"]]
+            a.fov = float(v["fov"])
+        h = ctypes.c_void_p()
+        _check(lib().cpt_batch_create(ctypes.byref(h), arr, len(views)), "cpt_batch_create")
+        self.h = h
+        self.views = [(int(v["width"]), int(v["height"])) for v in views]
+        if config:
+            self.set_config(**config)
+
+    def __del__(self):
+        if getattr(self, "h", None) and _lib is not None:
+            _lib.cpt_batch_destroy(self.h)
+            self.h = None
+
+    def config(self) -> dict:
+        c = BatchConfig()
+        _check(lib().cpt_batch_get_config(self.h, ctypes.byref(c)), "cpt_batch_get_config")
+        return c.as_dict()
+
+    def set_config(self, **kw):
+        c = BatchConfig()
+        _check(lib().cpt_batch_get_config(self.h, ctypes.byref(c)), "cpt_batch_get_config")
+        for k, v in kw.items():
+            setattr(c, k, int(v))
+        _check(lib().cpt_batch_set_config(self.h, ctypes.byref(c)), "cpt_batch_set_config")
+
+    def render(self, scene: Scene, frames: int):
+        _check(lib().cpt_batch_render(self.h, scene.h, frames), "cpt_batch_render")
+
+    def result(self, view: int) -> np.ndarray:
+        w, h = self.views[view]
+        out = np.zeros((h, w, 3), np.float32)
+        _check(lib().cpt_batch_read_result(self.h, view, _ptr(out)), "cpt_batch_read_result")
+        return out
+
+    def sample_counts(self, view: int) -> np.ndarray:
+        w, h = self.views[view]
+        out = np.zeros((h, w), np.uint32)
+        _check(lib().cpt_batch_sample_counts(self.h, view, _ptr(out)), "cpt_batch_sample_counts")
+        return out
+
+    def stats(self, reset: bool = False) -> dict:
+        s = _Stats()
+        _check(lib().cpt_batch_stats(self.h, ctypes.byref(s), int(reset)), "cpt_batch_stats")
+        return {n: getattr(s, n) for n, _ in s._fields_}
+
+    def reset(self):
+        _check(lib().cpt_batch_reset(self.h), "cpt_batch_reset")
+
+    def candidates(self) -> list:
+        n = ctypes.c_uint32()
+        _check(lib().cpt_batch_candidates(self.h, ctypes.byref(n)), "cpt_batch_candidates")
+        out = []
+        for i in range(n.value):
+            c = BatchConfig()
+            _check(lib().cpt_batch_candidate(self.h, i, ctypes.byref(c)), "cpt_batch_candidate")
+            out.append(c.as_dict())
+        return out
+
+    def autotune(self, scene: Scene, iterations: int = 2):
+        """Times every candidate; returns (chosen config, per-candidate ms per iteration)."""
+        n = len(self.candidates())
+        ms = (ctypes.c_double * n)()
+        best = BatchConfig()
+        _check(lib().cpt_batch_autotune(self.h, scene.h, iterations, ctypes.byref(best), ms, n), "cpt_batch_autotune")
+        return best.as_dict(), list(ms)
 
 
 class Mlp:

@@ -527,6 +527,8 @@ struct cpt_renderer {
   DevArray<float4> rec_acc, rec_pos, rec_nrm, rec_dif;
   float gauss_sigma = -1.f;
   u32 gauss_radius = 0;
+  u32 persist_chunk = 0, persist_grid = 0;   // persistent extend grid (0: one thread per ray)
+  DevArray<u32> queue;                       // its global work-queue counter
   cpt_mlp* mlp = nullptr;              // learned error model (adaptive mode)
   DevArray<float> mlp_x, mlp_y, mlp_e, sq_f, sq_e;
   u32 cap = 0;
@@ -676,8 +678,12 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
   for (u32 b = 0; b < fa.eye_depth; b++) {
     r->extend_launches++;
     HIP_TRY(timed_begin(r, 0, &ev));
-    HIP_TRY(launch_extend(sc, cur->view(), r->fc.p, (int)b, max_n, r->hits.p, r->stats.p, (int)r->attr(kLdsStack),
-                          r->stream));
+    if (r->persist_chunk)
+      HIP_TRY(launch_extend_persist(sc, cur->view(), r->fc.p, (int)b, r->hits.p, r->stats.p, (int)r->attr(kLdsStack),
+                                    r->persist_chunk, r->persist_grid, r->queue.p, r->stream));
+    else
+      HIP_TRY(launch_extend(sc, cur->view(), r->fc.p, (int)b, max_n, r->hits.p, r->stats.p, (int)r->attr(kLdsStack),
+                            r->stream));
     HIP_TRY(timed_end(r, 0, ev));
     HIP_TRY(timed_begin(r, 2, &ev));
     HIP_TRY(launch_shade(sc, fa, cur->view(), nxt->view(), r->hits.p, r->fc.p, (int)b, max_n, sb, r->lv.p, ac,
@@ -1313,6 +1319,381 @@ int cpt_renderer_set_mlp(cpt_renderer* r, cpt_mlp* m) {
   if (m && !uses_filter(r->mode))
     return fail(CPT_E_STATE, "cpt_renderer_set_mlp: the model drives the filter-based adaptive renderers only");
   r->mlp = m;
+  return CPT_OK;
+}
+
+}  // extern "C"
+
+// ---- multi-view batch ---------------------------------------------------------
+
+struct cpt_batch {
+  cpt_renderer* r = nullptr;           // the renderer over the concatenated pixels (W = total, H = 1)
+  std::vector<cpt_view> views;
+  std::vector<ViewDesc> vd;
+  DevArray<ViewDesc> d_views;
+  DevArray<u32> frame_dev;
+  u32 total = 0;
+  cpt_batch_config cfg;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  u32 graph_k = 0;
+  bool started = false;
+  const cpt_scene* scene = nullptr;
+  u32 mesh_v = 0, mat_v = 0;
+  const void* buf_key = nullptr;       // path buffer the graph was captured on
+  int cus = 0;
+  double ms = 0.0;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graph_k = 0;
+  }
+  ~cpt_batch() {
+    drop_graph();
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (r) cpt_renderer_destroy(r);
+  }
+};
+
+namespace {
+
+constexpr u32 kMaxViews = 4096;
+const u32 kChunks[] = {0u, 256u, 1024u, 4096u};
+const u32 kWgsPerCu[] = {4u, 8u, 16u};
+const u32 kVariants[] = {806u, 1606u};
+
+cpt_batch_config default_batch_config() {
+  cpt_batch_config c;
+  std::memset(&c, 0, sizeof c);
+  c.frames_in_flight = 24;
+  c.extend_chunk = 0;
+  c.extend_workgroups = 8;
+  c.traversal_variant = 806;
+  c.use_graph = 1;
+  c.eye_depth = 5;
+  c.light_depth = 5;
+  c.light_paths = 128;
+  c.seed = 0x5EED0001u;
+  return c;
+}
+
+int batch_check_config(const cpt_batch_config& c) {
+  if (c.frames_in_flight < 1 || c.frames_in_flight > kMaxInFlight)
+    return fail(CPT_E_LIMIT, "batch: frames in flight %u not in 1..%u", c.frames_in_flight, kMaxInFlight);
+  if (c.extend_chunk && (c.extend_workgroups < 1 || c.extend_workgroups > 64))
+    return fail(CPT_E_LIMIT, "batch: extend workgroups per CU %u not in 1..64", c.extend_workgroups);
+  if (c.eye_depth < 1 || c.eye_depth > (u32)kMaxDepth) return fail(CPT_E_LIMIT, "batch: eye depth not in 1..15");
+  if (c.light_depth < 1 || c.light_depth > 64) return fail(CPT_E_LIMIT, "batch: light depth not in 1..64");
+  if (c.light_paths < 1 || c.light_paths > (u32)kBlock) return fail(CPT_E_LIMIT, "batch: light paths not in 1..256");
+  return CPT_OK;
+}
+
+void batch_apply_config(cpt_batch* b) {
+  cpt_renderer* r = b->r;
+  r->set_attr(kEyeDepth, b->cfg.eye_depth);
+  r->set_attr(kLightDepth, b->cfg.light_depth);
+  r->set_attr(kLightPaths, b->cfg.light_paths);
+  r->set_attr(kSeed, (double)b->cfg.seed);
+  r->set_attr(kInFlight, b->cfg.frames_in_flight);
+  r->set_attr(kLdsStack, b->cfg.traversal_variant);
+  r->persist_chunk = b->cfg.extend_chunk;
+  r->persist_grid = b->cfg.extend_chunk ? b->cfg.extend_workgroups * (u32)(b->cus > 0 ? b->cus : 256) : 0u;
+  b->drop_graph();
+  b->started = false;
+}
+
+// One iteration of k frames of every view: light caches of the frames'
+// groups, eye paths, the bounce loop, the per-frame running-mean updates, and
+// the device frame counter advance.  No host synchronisation: capturable.
+int batch_enqueue(cpt_batch* b, cpt_scene* s, u32 k) {
+  cpt_renderer* r = b->r;
+  DevScene sc = s->dev();
+  FrameArgs fa;
+  std::memset(&fa, 0, sizeof fa);
+  fa.width = b->total;
+  fa.height = 1;
+  fa.eye_depth = b->cfg.eye_depth;
+  fa.light_depth = b->cfg.light_depth;
+  fa.light_paths = b->cfg.light_paths;
+  fa.seed = b->cfg.seed;
+  fa.nframes = k;
+  fa.lv_stride = b->cfg.light_paths * (b->cfg.light_depth + 1);
+  fa.n_views = (u32)b->vd.size();
+  fa.views = b->d_views.p;
+  fa.frame_dev = b->frame_dev.p;
+  LightJobs jobs;
+  std::memset(&jobs, 0, sizeof jobs);
+  jobs.dev_groups = (k + 2) / 3 + 1;                 // groups k frames can span
+  const u32 n = b->total * k;
+  HIP_TRY(launch_light_pass(sc, fa, jobs, r->lv.p, r->fc.p, r->stream));
+  HIP_TRY(launch_frame_begin(r->fc.p, n, r->stream));
+  HIP_TRY(launch_gen_eye(fa, r->A.view(), r->stream));
+  Accum ac;
+  std::memset(&ac, 0, sizeof ac);
+  ac.result = r->result.p;
+  ac.sample_n = r->sample_n.p;
+  ac.rec.acc = r->rec_acc.p;
+  ac.apply = 2;
+  int rc = bounce_loop(r, sc, fa, n, ac);
+  if (rc) return rc;
+  HIP_TRY(launch_apply_frames(fa, r->rec_acc.p, r->result.p, r->sample_n.p, r->stream));
+  HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, k, fa.eye_depth, r->stream));
+  HIP_TRY(launch_frame_advance(b->frame_dev.p, k, r->stream));
+  return CPT_OK;
+}
+
+int batch_prepare(cpt_batch* b, cpt_scene* s) {
+  if (!s->has_bvh) return fail(CPT_E_STATE, "cpt_batch_render: scene has no BVH");
+  if (!s->uploaded) {
+    int rc = cpt_scene_upload(s);
+    if (rc) return rc;
+  }
+  if (b->scene != s || b->mesh_v != s->mesh_version || b->mat_v != s->mat_version) {
+    b->drop_graph();
+    b->started = false;
+    b->scene = s;
+    b->mesh_v = s->mesh_version;
+    b->mat_v = s->mat_version;
+  }
+  cpt_renderer* r = b->r;
+  if (!r->allocated || (size_t)r->cap < (size_t)b->total * b->cfg.frames_in_flight) {
+    int rc = renderer_alloc(r);
+    if (rc) return rc;
+  }
+  int rc = lv_alloc(r);
+  if (rc) return rc;
+  HIP_TRY(r->queue.alloc(1));
+  if (b->buf_key != (const void*)r->A.ray_o.p) {   // buffers moved: a captured graph points at the old ones
+    b->drop_graph();
+    b->buf_key = (const void*)r->A.ray_o.p;
+  }
+  if (!b->started) {
+    u32 zero = 0;
+    HIP_TRY(hipMemcpyAsync(b->frame_dev.p, &zero, sizeof zero, hipMemcpyHostToDevice, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    b->started = true;
+  }
+  return CPT_OK;
+}
+
+int batch_capture(cpt_batch* b, cpt_scene* s, u32 k) {
+  b->drop_graph();
+  cpt_renderer* r = b->r;
+  HIP_TRY(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+  const uint64_t launches = r->extend_launches;       // counted per replay instead
+  int rc = batch_enqueue(b, s, k);
+  r->extend_launches = launches;
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(r->stream, &g);
+  if (rc) {
+    if (g) (void)hipGraphDestroy(g);
+    return rc;
+  }
+  HIP_TRY(e);
+  b->graph = g;
+  HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph_k = k;
+  return CPT_OK;
+}
+
+int batch_render(cpt_batch* b, cpt_scene* s, u32 frames) {
+  int rc = batch_prepare(b, s);
+  if (rc) return rc;
+  cpt_renderer* r = b->r;
+  const u32 K = b->cfg.frames_in_flight;
+  HIP_TRY(hipEventRecord(b->e0, r->stream));
+  for (u32 done = 0; done < frames;) {
+    u32 k = std::min(K, frames - done);
+    if (b->cfg.use_graph && k == K) {
+      if (!b->exec || b->graph_k != K) {
+        rc = batch_capture(b, s, K);
+        if (rc) return rc;
+      }
+      HIP_TRY(hipGraphLaunch(b->exec, r->stream));
+      r->extend_launches += b->cfg.eye_depth;
+    } else {
+      rc = batch_enqueue(b, s, k);
+      if (rc) return rc;
+    }
+    done += k;
+  }
+  HIP_TRY(hipEventRecord(b->e1, r->stream));
+  HIP_TRY(hipEventSynchronize(b->e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
+  b->ms += ms;
+  return CPT_OK;
+}
+
+int batch_candidate(const cpt_batch* b, u32 i, cpt_batch_config* out) {
+  u32 idx = 0;
+  for (u32 ch : kChunks)
+    for (u32 w = 0; w < (ch ? 3u : 1u); w++)
+      for (u32 v : kVariants) {
+        if (idx++ == i) {
+          *out = b->cfg;
+          out->extend_chunk = ch;
+          out->extend_workgroups = ch ? kWgsPerCu[w] : b->cfg.extend_workgroups;
+          out->traversal_variant = v;
+          return CPT_OK;
+        }
+      }
+  return fail(CPT_E_ARG, "cpt_batch_candidate: index %u out of range", i);
+}
+u32 batch_candidate_count() { return (u32)(2 * (1 + 3 * 3)); }
+
+}  // namespace
+
+extern "C" {
+
+int cpt_batch_create(cpt_batch** out, const cpt_view* views, uint32_t n_views) {
+  if (!out || !views) return fail(CPT_E_ARG, "cpt_batch_create: null argument");
+  *out = nullptr;
+  if (n_views == 0 || n_views > kMaxViews) return fail(CPT_E_LIMIT, "cpt_batch_create: %u views not in 1..4096", n_views);
+  auto* b = new (std::nothrow) cpt_batch();
+  if (!b) return fail(CPT_E_NOMEM, "cpt_batch_create: out of host memory");
+  uint64_t total = 0;
+  for (u32 v = 0; v < n_views; v++) {
+    const cpt_view& V = views[v];
+    if (V.width == 0 || V.height == 0) { delete b; return fail(CPT_E_ARG, "cpt_batch_create: view %u is empty", v); }
+    ViewDesc d;
+    std::memset(&d, 0, sizeof d);
+    d.offset = (u32)total;
+    d.width = V.width;
+    d.height = V.height;
+    for (int k = 0; k < 3; k++) { d.cam_pos[k] = V.cam_pos[k]; d.cam_dir[k] = V.cam_dir[k]; }
+    camera_basis(V.cam_dir, V.cam_up, d.cam_right, d.cam_up);
+    d.fov = V.fov;
+    b->vd.push_back(d);
+    total += (uint64_t)V.width * V.height;
+  }
+  if (total > (1ull << 26)) { delete b; return fail(CPT_E_LIMIT, "cpt_batch_create: %llu pixels exceed 2^26", (unsigned long long)total); }
+  b->views.assign(views, views + n_views);
+  b->total = (u32)total;
+  b->cfg = default_batch_config();
+  int rc = cpt_renderer_create(&b->r, CPT_MODE_BDPT_STREAM, b->total, 1);
+  if (rc) { delete b; return rc; }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    b->cus = 256;
+  if (b->d_views.upload(b->vd.data(), b->vd.size(), b->r->stream) != hipSuccess || b->frame_dev.alloc(1) != hipSuccess ||
+      hipEventCreate(&b->e0) != hipSuccess || hipEventCreate(&b->e1) != hipSuccess) {
+    delete b;
+    return fail(CPT_E_NOMEM, "cpt_batch_create: device allocation");
+  }
+  batch_apply_config(b);
+  *out = b;
+  return CPT_OK;
+}
+
+int cpt_batch_get_config(cpt_batch* b, cpt_batch_config* cfg) {
+  if (!b || !cfg) return fail(CPT_E_ARG, "cpt_batch_get_config: null argument");
+  *cfg = b->cfg;
+  return CPT_OK;
+}
+
+int cpt_batch_set_config(cpt_batch* b, const cpt_batch_config* cfg) {
+  if (!b || !cfg) return fail(CPT_E_ARG, "cpt_batch_set_config: null argument");
+  int rc = batch_check_config(*cfg);
+  if (rc) return rc;
+  b->cfg = *cfg;
+  batch_apply_config(b);
+  return CPT_OK;
+}
+
+int cpt_batch_render(cpt_batch* b, cpt_scene* s, uint32_t frames) {
+  if (!b || !s) return fail(CPT_E_ARG, "cpt_batch_render: null argument");
+  return batch_render(b, s, frames);
+}
+
+int cpt_batch_read_result(cpt_batch* b, uint32_t view, float* out) {
+  if (!b || !out) return fail(CPT_E_ARG, "cpt_batch_read_result: null argument");
+  if (view >= b->vd.size()) return fail(CPT_E_ARG, "cpt_batch_read_result: view %u out of range", view);
+  if (!b->r->allocated) return fail(CPT_E_STATE, "cpt_batch_read_result: nothing rendered yet");
+  const ViewDesc& d = b->vd[view];
+  HIP_TRY(hipMemcpyAsync(out, b->r->result.p + (size_t)d.offset * 3, (size_t)d.width * d.height * 12,
+                         hipMemcpyDeviceToHost, b->r->stream));
+  HIP_TRY(hipStreamSynchronize(b->r->stream));
+  return CPT_OK;
+}
+
+int cpt_batch_sample_counts(cpt_batch* b, uint32_t view, uint32_t* out) {
+  if (!b || !out) return fail(CPT_E_ARG, "cpt_batch_sample_counts: null argument");
+  if (view >= b->vd.size()) return fail(CPT_E_ARG, "cpt_batch_sample_counts: view %u out of range", view);
+  if (!b->r->allocated) return fail(CPT_E_STATE, "cpt_batch_sample_counts: nothing rendered yet");
+  const ViewDesc& d = b->vd[view];
+  HIP_TRY(hipMemcpyAsync(out, b->r->sample_n.p + d.offset, (size_t)d.width * d.height * 4, hipMemcpyDeviceToHost,
+                         b->r->stream));
+  HIP_TRY(hipStreamSynchronize(b->r->stream));
+  return CPT_OK;
+}
+
+int cpt_batch_stats(cpt_batch* b, cpt_stats* out, int reset) {
+  if (!b || !out) return fail(CPT_E_ARG, "cpt_batch_stats: null argument");
+  int rc = cpt_renderer_stats(b->r, out, reset);
+  if (rc) return rc;
+  out->total_ms = b->ms;
+  if (reset) b->ms = 0.0;
+  return CPT_OK;
+}
+
+int cpt_batch_reset(cpt_batch* b) {
+  if (!b) return fail(CPT_E_ARG, "cpt_batch_reset: null batch");
+  b->started = false;
+  return CPT_OK;
+}
+
+int cpt_batch_candidates(cpt_batch* b, uint32_t* n) {
+  if (!b || !n) return fail(CPT_E_ARG, "cpt_batch_candidates: null argument");
+  *n = batch_candidate_count();
+  return CPT_OK;
+}
+
+int cpt_batch_candidate(cpt_batch* b, uint32_t i, cpt_batch_config* cfg) {
+  if (!b || !cfg) return fail(CPT_E_ARG, "cpt_batch_candidate: null argument");
+  return batch_candidate(b, i, cfg);
+}
+
+int cpt_batch_autotune(cpt_batch* b, cpt_scene* s, uint32_t iterations, cpt_batch_config* best, double* ms,
+                       uint32_t n_ms) {
+  if (!b || !s) return fail(CPT_E_ARG, "cpt_batch_autotune: null argument");
+  if (iterations < 1) return fail(CPT_E_ARG, "cpt_batch_autotune: iterations must be >= 1");
+  const cpt_batch_config base = b->cfg;
+  const u32 n = batch_candidate_count();
+  double best_ms = 0.0;
+  cpt_batch_config best_cfg = base;
+  for (u32 i = 0; i < n; i++) {
+    cpt_batch_config c;
+    int rc = batch_candidate(b, i, &c);
+    if (rc) return rc;
+    b->cfg = c;
+    batch_apply_config(b);
+    const u32 K = c.frames_in_flight;
+    rc = batch_render(b, s, K);                      // capture + warm
+    if (rc) return rc;
+    b->ms = 0.0;
+    rc = batch_render(b, s, K * iterations);
+    if (rc) return rc;
+    const double per = b->ms / iterations;
+    if (ms && i < n_ms) ms[i] = per;
+    if (i == 0 || per < best_ms) { best_ms = per; best_cfg = c; }
+  }
+  b->ms = 0.0;
+  b->cfg = best_cfg;
+  batch_apply_config(b);
+  HIP_TRY(hipMemset(b->r->stats.p, 0, sizeof(DevStats)));
+  if (best) *best = best_cfg;
+  return CPT_OK;
+}
+
+int cpt_batch_destroy(cpt_batch* b) {
+  if (!b) return fail(CPT_E_ARG, "cpt_batch_destroy: null batch");
+  if (b->r && b->r->stream) (void)hipStreamSynchronize(b->r->stream);
+  delete b;
   return CPT_OK;
 }
 

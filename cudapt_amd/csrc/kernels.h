@@ -68,6 +68,14 @@ struct LightVert {
 };
 static_assert(sizeof(LightVert) == 96, "LightVert is 96 bytes");
 
+// One camera view of a multi-view batch: its pixels are [offset, offset + W*H)
+// of the batch's concatenated pixel space.
+struct ViewDesc {
+  uint32_t offset, width, height, pad;
+  float cam_pos[3], cam_dir[3], cam_up[3], cam_right[3];
+  float fov, pad2[3];
+};
+
 struct FrameArgs {
   uint32_t width, height;
   float cam_pos[3], cam_dir[3], cam_up[3], cam_right[3];
@@ -78,7 +86,9 @@ struct FrameArgs {
   uint32_t frame0;       // first frame of the batch (frame == frame0 for single-frame work)
   uint32_t nframes;      // frames traced together as one wavefront
   uint32_t lv_stride;    // light vertices per cache slot
-  uint32_t pad0;
+  uint32_t n_views;      // multi-view batch: views (0: one view, width x height)
+  const ViewDesc* views; // multi-view batch: pixel p belongs to the last view whose offset <= p
+  const uint32_t* frame_dev;  // when set, the batch's first frame is read here (replayed graphs)
 };
 
 // Light-vertex cache slots to (re)build: slot s holds the cache of the
@@ -88,6 +98,9 @@ struct LightJobs {
   uint32_t n;
   uint32_t slot[kLightSlots];
   uint32_t frame[kLightSlots];
+  // > 0: ignore the lists and (re)build the caches of every group of frames
+  // [*fa.frame_dev, + fa.nframes) with up to dev_groups workgroups
+  uint32_t dev_groups;
 };
 
 // Device counters of one frame (zeroed by frame_begin).
@@ -197,5 +210,13 @@ hipError_t launch_scan_counts(const uint32_t* counts, uint32_t n, uint32_t* offs
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
                          uint32_t max_n, int lds_stack, hipStream_t st);
 hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes, uint32_t depth, hipStream_t st);
+// *frame_dev += k (the device frame counter of replayed graphs).
+hipError_t launch_frame_advance(uint32_t* frame_dev, uint32_t k, hipStream_t st);
+// Closest hit of the bounce's live paths by a persistent grid: each workgroup
+// takes `chunk` rays at a time from the global counter `queue` until the
+// queue is drained.
+hipError_t launch_extend_persist(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, float4* hits,
+                                 DevStats* stats, int lds_stack, uint32_t chunk, uint32_t grid, uint32_t* queue,
+                                 hipStream_t st);
 
 }  // namespace cpt

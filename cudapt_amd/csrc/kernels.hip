@@ -422,7 +422,28 @@ __device__ __forceinline__ double shadow_max(double dist, uint32_t flags) {
   return (flags & 1u) ? dist - kBiasEps : dist - 2.0 * kBiasEps;
 }
 
+// First frame of the batch: the argument, or the device counter of a replayed graph.
+__device__ __forceinline__ uint32_t frame_base(const FrameArgs& fa) {
+  return fa.frame_dev ? *fa.frame_dev : fa.frame0;
+}
+
+// View of a batch-global pixel (0 without a view table).
+__device__ __forceinline__ uint32_t view_of(const FrameArgs& fa, uint32_t pix) {
+  uint32_t lo = 0, hi = fa.n_views;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (fa.views[mid].offset <= pix) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+// The pixel's index within its view: the PCG32 stream of an eye path is keyed
+// by it, so every view of a batch traces the samples a one-view renderer would.
+__device__ __forceinline__ uint32_t local_pixel(const FrameArgs& fa, uint32_t pix) {
+  return fa.views ? pix - fa.views[view_of(fa, pix)].offset : pix;
+}
+
 // ---------------------------------------------------------------- frame
+__global__ void k_fill_u32(uint32_t* p, uint32_t v) { *p = v; }
 __global__ void k_frame_begin(FrameCounters* fc, uint32_t n_paths) {
   uint32_t i = threadIdx.x;
   if (i < 16) { fc->active[i] = i == 0 ? n_paths : 0u; fc->shadow[i] = 0u; }
@@ -455,6 +476,14 @@ hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes
   k_frame_end<<<1, 64, 0, st>>>(fc, stats, nframes, depth);
   return hipGetLastError();
 }
+__global__ void k_frame_advance(uint32_t* frame_dev, uint32_t k) {
+  if (threadIdx.x == 0) *frame_dev += k;
+}
+hipError_t launch_frame_advance(uint32_t* frame_dev, uint32_t k, hipStream_t st) {
+  if (!frame_dev) return hipErrorInvalidValue;
+  k_frame_advance<<<1, 64, 0, st>>>(frame_dev, k);
+  return hipGetLastError();
+}
 
 __device__ __forceinline__ void store_lv(LightVert& V, f3 pos, f3 norm, f3 irrad, f3 irradDir, const Surf& s) {
   V.pos[0] = pos.x; V.pos[1] = pos.y; V.pos[2] = pos.z;
@@ -474,9 +503,18 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(DevScene sc, FrameArgs fa
   __shared__ uint64_t lstk[CPT_LDS_STACK * kBlock];
   __shared__ uint32_t wcnt[kBlock / 64];
   const uint32_t x = threadIdx.x, L = fa.light_paths;
-  const uint32_t slot = jobs.slot[blockIdx.x];
+  uint32_t slot, job_frame;
+  if (jobs.dev_groups) {                       // groups of the device-counted batch
+    const uint32_t f0 = frame_base(fa), g = f0 / 3u + blockIdx.x;
+    if (g > (f0 + fa.nframes - 1u) / 3u) return;
+    slot = g % (uint32_t)kLightSlots;
+    job_frame = 3u * g;
+  } else {
+    slot = jobs.slot[blockIdx.x];
+    job_frame = jobs.frame[blockIdx.x];
+  }
   LightVert* __restrict__ lv = lv_all + (size_t)slot * fa.lv_stride;
-  const uint64_t light_key = stream_key(fa.seed, jobs.frame[blockIdx.x], 2u, 0u);
+  const uint64_t light_key = stream_key(fa.seed, job_frame, 2u, 0u);
   const uint64_t inc = ((uint64_t)x << 1) | 1u;
   uint64_t st = 0;
   bool live = false;
@@ -555,13 +593,14 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(DevScene sc, FrameArgs fa
 }
 hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, const LightJobs& jobs, LightVert* lv,
                              FrameCounters* fc, hipStream_t st) {
-  if (jobs.n == 0) return hipSuccess;
+  if (jobs.n == 0 && jobs.dev_groups == 0) return hipSuccess;
   if (!lv || !fc || fa.light_paths > (uint32_t)kBlock || (sc.n_light_tris && !sc.light_tris) ||
-      jobs.n > (uint32_t)kLightSlots)
+      jobs.n > (uint32_t)kLightSlots || jobs.dev_groups > (uint32_t)kLightSlots ||
+      (jobs.dev_groups && !fa.frame_dev))
     return hipErrorInvalidValue;
   for (uint32_t j = 0; j < jobs.n; j++)
     if (jobs.slot[j] >= (uint32_t)kLightSlots) return hipErrorInvalidValue;
-  k_light_pass<<<jobs.n, kBlock, 0, st>>>(sc, fa, jobs, lv, fc);
+  k_light_pass<<<jobs.dev_groups ? jobs.dev_groups : jobs.n, kBlock, 0, st>>>(sc, fa, jobs, lv, fc);
   return hipGetLastError();
 }
 
@@ -583,23 +622,36 @@ __device__ __forceinline__ void apply_pixel(float* result, uint32_t* sample_n, u
 
 // Eye ray of pixel (x, y) drawn from PCG32 stream `key`, pixel as sequence
 // (pt_genPathQueue_kernel, src/cudaRTBDPTStream.cu:619-640; the adaptive queue
-// of AdapBilatAtomic.cu:826-840 uses the same pinhole).
+// of AdapBilatAtomic.cu:826-840 uses the same pinhole).  In a multi-view batch
+// `pix` is the batch-global pixel: the camera and the pixel's position come
+// from its view, the RNG stream stays keyed by the global pixel.
 __device__ __forceinline__ void write_eye_path(const FrameArgs& fa, PathBuf& pb, uint32_t slot, uint32_t pix,
                                                uint32_t sid, uint64_t key) {
-  uint32_t y = pix / fa.width, x = pix - y * fa.width;
-  float W = (float)fa.width, H = (float)fa.height;
-  float tf = tanf(fa.fov * 0.5f);
+  uint32_t lp = pix, vw = fa.width, vh = fa.height;
+  const float *cpos = fa.cam_pos, *cdir = fa.cam_dir, *cup = fa.cam_up, *cright = fa.cam_right;
+  float fov = fa.fov;
+  if (fa.views) {
+    const ViewDesc& V = fa.views[view_of(fa, pix)];
+    lp = pix - V.offset;
+    vw = V.width;
+    vh = V.height;
+    cpos = V.cam_pos; cdir = V.cam_dir; cup = V.cam_up; cright = V.cam_right;
+    fov = V.fov;
+  }
+  uint32_t y = lp / vw, x = lp - y * vw;
+  float W = (float)vw, H = (float)vh;
+  float tf = tanf(fov * 0.5f);
   float u = ((((2.f * ((float)x + 0.5f)) / W - 1.f) * tf) * W) / H;
   float v = ((2.f * ((float)y + 0.5f)) / H - 1.f) * tf;
-  uint64_t inc = ((uint64_t)pix << 1) | 1u, st;
+  uint64_t inc = ((uint64_t)lp << 1) | 1u, st;
   pcg32_seed(st, inc, key);
   float au = u + ((uniform01(st, inc) - 0.5f) / H) * tf;
   float av = v + ((uniform01(st, inc) - 0.5f) / H) * tf;
-  f3 right = mk3(fa.cam_right[0], fa.cam_right[1], fa.cam_right[2]);
-  f3 up = mk3(fa.cam_up[0], fa.cam_up[1], fa.cam_up[2]);
-  f3 cd = mk3(fa.cam_dir[0], fa.cam_dir[1], fa.cam_dir[2]);
+  f3 right = mk3(cright[0], cright[1], cright[2]);
+  f3 up = mk3(cup[0], cup[1], cup[2]);
+  f3 cd = mk3(cdir[0], cdir[1], cdir[2]);
   f3 d = normalize((right * au + up * av) + cd);
-  pb.ray_o[slot] = make_float4(fa.cam_pos[0], fa.cam_pos[1], fa.cam_pos[2], kInfF);
+  pb.ray_o[slot] = make_float4(cpos[0], cpos[1], cpos[2], kInfF);
   pb.ray_d[slot] = make_float4(d.x, d.y, d.z, (float)kBiasEps);
   pb.pix[slot] = pix;
   pb.sid[slot] = sid;
@@ -621,7 +673,7 @@ __global__ __launch_bounds__(kBlock) void k_gen_eye(FrameArgs fa, PathBuf pb, Sa
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= wh * fa.nframes) return;
   uint32_t k = i / wh, pix = i - k * wh;
-  write_eye_path(fa, pb, i, pix, i, stream_key(fa.seed, fa.frame0 + k, 1u, 0u));
+  write_eye_path(fa, pb, i, pix, i, stream_key(fa.seed, frame_base(fa) + k, 1u, 0u));
   pb.depth[i] = k << 8;
   if (rec.acc) zero_rec(rec, i);
 }
@@ -646,9 +698,10 @@ __global__ __launch_bounds__(kBlock) void k_apply_frames(FrameArgs fa, const flo
   const uint32_t wh = fa.width * fa.height;
   const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
   if (pix >= wh) return;
+  const uint32_t f0 = frame_base(fa);
   for (uint32_t k = 0; k < fa.nframes; k++) {
     float4 a = rec_acc[(size_t)k * wh + pix];
-    apply_pixel(result, sample_n, pix, fa.frame0 + k, ld3(a), __float_as_uint(a.w));
+    apply_pixel(result, sample_n, pix, f0 + k, ld3(a), __float_as_uint(a.w));
   }
 }
 hipError_t launch_apply_frames(const FrameArgs& fa, const float4* rec_acc, float* result, uint32_t* sample_n,
@@ -676,6 +729,63 @@ hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc
     default: CPT_EXT(16, 0); break;
   }
 #undef CPT_EXT
+  return hipGetLastError();
+}
+
+// Persistent closest-hit grid over a global queue of the bounce's live rays.
+template <int STK, int MINW>
+__global__ __launch_bounds__(kBlock, MINW) void k_extend_persist(DevScene sc, const float4* __restrict__ ro,
+                                                                 const float4* __restrict__ rd,
+                                                                 const uint32_t* __restrict__ countp, uint32_t chunk,
+                                                                 uint32_t* __restrict__ queue,
+                                                                 float4* __restrict__ out,
+                                                                 DevStats* __restrict__ stats) {
+  __shared__ uint64_t lstk[STK * kBlock];
+  __shared__ uint32_t base_s;
+  const uint32_t count = *countp;
+  uint32_t nn = 0, nt = 0;
+  for (;;) {
+    if (threadIdx.x == 0) base_s = atomicAdd(queue, chunk);
+    __syncthreads();
+    const uint32_t base = base_s;
+    __syncthreads();
+    if (base >= count) break;
+    const uint32_t end = min(base + chunk, count);
+    for (uint32_t i = base + threadIdx.x; i < end; i += kBlock) {
+      float4 a = ro[i], b = rd[i];
+      TraceHit h;
+      bool hit = traverse4<false, STK>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt);
+      float4 r;
+      r.x = hit ? h.t : kInfF;
+      r.y = __uint_as_float(hit ? h.leaf : kInvalid);
+      r.z = h.u;
+      r.w = h.v;
+      out[i] = r;
+    }
+  }
+  if (stats) {
+    uint32_t sn = wave_sum(nn), st = wave_sum(nt);
+    if ((threadIdx.x & 63) == 0 && (sn | st)) {
+      atomicAdd(&stats->ext_nodes, (unsigned long long)sn);
+      atomicAdd(&stats->ext_tris, (unsigned long long)st);
+    }
+  }
+}
+hipError_t launch_extend_persist(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, float4* hits,
+                                 DevStats* stats, int lds_stack, uint32_t chunk, uint32_t grid, uint32_t* queue,
+                                 hipStream_t st) {
+  if (!pathbuf_ok(pb) || !fc || !hits || !sc.nodes4 || !sc.tris || !queue || chunk == 0 || grid == 0)
+    return hipErrorInvalidValue;
+  k_fill_u32<<<1, 1, 0, st>>>(queue, 0u);
+  const uint32_t* cnt = &fc->active[bounce];
+#define CPT_EXTP(S, W) k_extend_persist<S, W><<<grid, kBlock, 0, st>>>(sc, pb.ray_o, pb.ray_d, cnt, chunk, queue, hits, stats)
+  switch (lds_stack) {
+    case 8: CPT_EXTP(8, 0); break;
+    case 806: CPT_EXTP(8, 6); break;
+    case 1606: CPT_EXTP(16, 6); break;
+    default: CPT_EXTP(16, 0); break;
+  }
+#undef CPT_EXTP
   return hipGetLastError();
 }
 
@@ -752,13 +862,13 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
     const uint32_t dk = cur.depth[i];
     depth = dk & 0xFFu;
     fk = dk >> 8;
-    const uint32_t frame = fa.frame0 + fk;
+    const uint32_t frame = frame_base(fa) + fk;
     const uint32_t slot = (frame / 3u) % (uint32_t)kLightSlots;
     const uint32_t lvN = fc->light_n[slot];
     const LightVert* __restrict__ lvb = lv + (size_t)slot * fa.lv_stride;
     float4 m4 = cur.mul[i], s4 = cur.smp[i], a4 = cur.acc[i];
     st = cur.rng[i];
-    const uint64_t inc = ((uint64_t)pix << 1) | 1u;
+    const uint64_t inc = ((uint64_t)local_pixel(fa, pix) << 1) | 1u;
     float4 h = hits[i];
     f3 orig = ld3(ro), dir = ld3(rd), mul = ld3(m4), psample = ld3(s4), acc = ld3(a4);
     uint32_t N = __float_as_uint(a4.w);
@@ -822,8 +932,8 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
       }
     }
     if (term) {
-      if (ac.apply == 0) apply_pixel(ac.result, ac.sample_n, pix, fa.frame0 + fk, acc, N);
-      else if (ac.apply == 1) apply_pixel_gbuf(ac, pix, fa.frame0 + fk, acc, N, sid);
+      if (ac.apply == 0) apply_pixel(ac.result, ac.sample_n, pix, frame, acc, N);
+      else if (ac.apply == 1) apply_pixel_gbuf(ac, pix, frame, acc, N, sid);
       else ac.rec.acc[sid] = make_float4(acc.x, acc.y, acc.z, __uint_as_float(N));
     } else {
       survive = true;
@@ -1225,7 +1335,6 @@ __device__ __forceinline__ void accum_record(const SampleRec& rec, uint32_t q, f
 // Pixels with more records than this are summed by a whole workgroup.
 constexpr uint32_t kAccumSerialMax = 32;
 
-__global__ void k_fill_u32(uint32_t* p, uint32_t v) { *p = v; }
 
 // pt_atomicApplyPathQueueResult_kernel + pt_accumTempResultToResult_kernel
 // (AdapBilatAtomic.cu:1041-1141) with the per-pixel sums taken over the

@@ -216,6 +216,57 @@ int cpt_renderer_set_profiling(cpt_renderer* r, int on);
 /* <ns>::CleanMem (src/raytracer.cpp:25) + destruction. */
 int cpt_renderer_destroy(cpt_renderer* r);
 
+/* ---- Multi-view batch -------------------------------------------------------
+ * No counterpart in the reference (one RTRenderer renders one view).  Many
+ * camera views of uneven resolution are traced as one wavefront: their pixels
+ * are concatenated, every iteration traces frames_in_flight frames of every
+ * view, and each view keeps its own running mean (plain streaming BDPT,
+ * cudaRTBDPTStream::Render per view).  A view's samples are the ones a
+ * one-view renderer of that camera draws (eye streams are keyed by the
+ * view-local pixel, the light-vertex caches by frame), so its image equals
+ * that renderer's bit for bit. */
+typedef struct cpt_batch cpt_batch;
+typedef struct cpt_view {
+  uint32_t width, height;
+  float cam_pos[3], cam_dir[3], cam_up[3];
+  float fov;
+} cpt_view;
+typedef struct cpt_batch_config {
+  uint32_t frames_in_flight;  /* frames of every view per iteration, 1..90 */
+  uint32_t extend_chunk;      /* 0: one thread per ray; else rays a workgroup takes from the global queue */
+  uint32_t extend_workgroups; /* persistent grid: workgroups per compute unit (extend_chunk > 0) */
+  uint32_t traversal_variant; /* LDS stack variant: 806 (8 entries, >= 6 waves/SIMD), 1606, 8, 16 */
+  uint32_t use_graph;         /* 1: capture one iteration as a hipGraph and replay it */
+  uint32_t eye_depth, light_depth, light_paths;
+  uint64_t seed;
+} cpt_batch_config;
+
+/* Defaults: 24 frames in flight, chunk 0, variant 806, graph on, depths 5/5,
+ * 128 light paths, seed 0x5EED0001 (the plain renderer's). */
+int cpt_batch_create(cpt_batch** out, const cpt_view* views, uint32_t n_views);
+int cpt_batch_get_config(cpt_batch* b, cpt_batch_config* cfg);
+/* Restarts accumulation. */
+int cpt_batch_set_config(cpt_batch* b, const cpt_batch_config* cfg);
+/* Renders `frames` frames of every view (full iterations of frames_in_flight,
+ * then one shorter iteration for the rest). */
+int cpt_batch_render(cpt_batch* b, cpt_scene* s, uint32_t frames);
+int cpt_batch_read_result(cpt_batch* b, uint32_t view, float* out);      /* W*H*3 running mean */
+int cpt_batch_sample_counts(cpt_batch* b, uint32_t view, uint32_t* out); /* W*H */
+int cpt_batch_stats(cpt_batch* b, cpt_stats* out, int reset);
+int cpt_batch_reset(cpt_batch* b);
+/* Autotuner: times `iterations` iterations of each candidate configuration
+ * (extend chunk x workgroups per CU x traversal variant, with the current
+ * frames in flight) and keeps the fastest.  *best receives it; ms (may be
+ * NULL, n_ms entries) the per-iteration milliseconds of candidates in the
+ * order cpt_batch_candidate enumerates them.  Accumulation restarts. */
+int cpt_batch_autotune(cpt_batch* b, cpt_scene* s, uint32_t iterations, cpt_batch_config* best, double* ms,
+                       uint32_t n_ms);
+/* Number of autotuner candidates, and candidate i (the current config with
+ * its extend_chunk, extend_workgroups and traversal_variant replaced). */
+int cpt_batch_candidates(cpt_batch* b, uint32_t* n);
+int cpt_batch_candidate(cpt_batch* b, uint32_t i, cpt_batch_config* cfg);
+int cpt_batch_destroy(cpt_batch* b);
+
 /* ---- Learned per-pixel error model ----------------------------------------
  * No counterpart in the reference: CudaPT includes and links FANN
  * (src/cudaRTBDPTStreamBilat.cu:11-12, CudaPT.vcxproj:129) but calls no fann_*
