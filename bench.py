@@ -20,6 +20,10 @@ time_to_equal_mse: the learned-vs-plain adaptive protocol of cudapt_amd/learned.
   MSE (radiance clamped to [0, 1]) plain adaptive BDPT reaches after 64 frames.
 mlp: MFMA throughput of the model's training step (forward + backward + Adam)
   at batch 65,536: 28,416 FLOP per sample (3x the 9,472 FLOP forward).
+multiview: 32 seeded views of uneven resolution (256x256 .. 1920x1080) over the
+  1,051,296-triangle fBm heightfield, one wavefront per iteration, autotuned
+  extend queue chunk / workgroups / traversal variant, hipGraph-replayed
+  iterations, 32 spp per view.
 
 python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -142,7 +146,7 @@ def hbm_traffic(args):
         d = tempfile.mkdtemp(prefix="cpt_pmc_")
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t",
                "--", sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
-               "--no-traffic", "--no-learned", "--frames", str(args.frames), "--in-flight", str(args.in_flight),
+               "--no-traffic", "--no-learned", "--no-multiview", "--frames", str(args.frames), "--in-flight", str(args.in_flight),
                "--lds-stack", str(args.lds_stack), "--copy-calib", str(calib >> 20)]
         log(f"traffic pass: {counter}")
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -168,6 +172,36 @@ def hbm_traffic(args):
             "write_bytes": int(write * wfac), "fetch_scale": round(ffac, 3), "write_scale": round(wfac, 3)}
 
 
+def multiview_leg(ca, args):
+    """Batched multi-view rendering: autotune, then 32 spp of every view."""
+    from cudapt_amd import scenes
+    t0 = time.perf_counter()
+    sc = scenes.heightfield()
+    S = ca.Scene(sc)
+    views = scenes.heightfield_poses(args.views)
+    log(f"multiview: scene {len(sc['idx'])} triangles, {len(views)} views built in {time.perf_counter() - t0:.1f} s")
+    B = ca.Batch(views, frames_in_flight=args.mv_in_flight, eye_depth=EYE_DEPTH, light_depth=LIGHT_DEPTH)
+    t0 = time.perf_counter()
+    best, cand_ms = B.autotune(S, iterations=1)
+    log(f"multiview: autotuned in {time.perf_counter() - t0:.1f} s: {best}")
+    B.reset()
+    B.stats(reset=True)
+    ca.synchronize()
+    t0 = time.perf_counter()
+    B.render(S, args.mv_spp)
+    wall = time.perf_counter() - t0
+    st = B.stats()
+    px = sum(v["width"] * v["height"] for v in views)
+    res = {"views": len(views), "pixels": px, "spp": args.mv_spp, "triangles": int(len(sc["idx"])),
+           "seconds": round(wall, 4), "gpu_ms": round(st["total_ms"], 3),
+           "mrays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / wall / 1e6, 2),
+           "samples_per_s": round(st["eye_paths"] / wall, 1),
+           "resolutions": sorted({(v["width"], v["height"]) for v in views}),
+           "autotuned": best, "candidates_ms_per_iteration": [round(m, 3) for m in cand_ms]}
+    del B, S
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -182,6 +216,10 @@ def main():
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
     ap.add_argument("--no-learned", action="store_true", help="skip the time-to-equal-MSE leg")
+    ap.add_argument("--no-multiview", action="store_true", help="skip the batched multi-view leg")
+    ap.add_argument("--views", type=int, default=32)
+    ap.add_argument("--mv-spp", type=int, default=32)
+    ap.add_argument("--mv-in-flight", type=int, default=3)
     ap.add_argument("--learn-train", type=int, default=8)
     ap.add_argument("--learn-test", type=int, default=2)
     ap.add_argument("--learn-train-spp", type=int, default=1024)
@@ -276,6 +314,9 @@ def main():
     if not args.no_learned:
         res["time_to_equal_mse"], res["mlp"] = learned_leg(ca, S, args)
     del S
+    res["multiview"] = None
+    if not args.no_multiview:
+        res["multiview"] = multiview_leg(ca, args)
     if not args.no_traffic:
         tr = hbm_traffic(args)
         res["roofline"]["traffic_detail"] = tr

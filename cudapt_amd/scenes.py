@@ -229,6 +229,104 @@ def sphere_grid(grid=64, seed=0x5EED0002, spacing=2.0, light_radiance=10.0):
         mats=mats, camera=cam)
 
 
+def _value_noise(rng, shape, cells):
+    """Bilinearly interpolated random lattice of `cells` cells over `shape`."""
+    cy, cx = cells
+    lat = rng.uniform(-1.0, 1.0, (cy + 2, cx + 2))
+    h, w = shape
+    y = np.linspace(0, cy, h, dtype=np.float64)
+    x = np.linspace(0, cx, w, dtype=np.float64)
+    y0, x0 = np.floor(y).astype(int), np.floor(x).astype(int)
+    fy, fx = (y - y0)[:, None], (x - x0)[None, :]
+    a = lat[y0][:, x0]
+    b = lat[y0][:, x0 + 1]
+    c = lat[y0 + 1][:, x0]
+    d = lat[y0 + 1][:, x0 + 1]
+    return (a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy
+
+
+def heightfield(nx=1024, nz=512, boxes=200, seed=0xC3, extent=(256.0, 128.0), amplitude=18.0, octaves=6,
+                light_radiance=12.0):
+    """Adaptive-sampling workload: an nx x nz fBm heightfield (octaves of value
+    noise, 2 triangles per cell: 1,048,576 triangles at 1024 x 512) with
+    `boxes` random boxes (12 triangles each) standing on it, lit by four
+    emissive icospheres above it.
+
+    The stated workload lights the scene with an HDR environment map, which the
+    reference estimator cannot evaluate (it knows emissive triangles only,
+    cudaRTBDPTStream.cu:57-77); the emitters here take its place.
+    """
+    rng = np.random.default_rng(seed)
+    ex, ez = extent
+    hgt = np.zeros((nz + 1, nx + 1))
+    amp, cells = 1.0, (2, 4)
+    for _ in range(octaves):
+        hgt += amp * _value_noise(rng, hgt.shape, cells)
+        amp *= 0.5
+        cells = (cells[0] * 2, cells[1] * 2)
+    hgt *= amplitude
+    xs = np.linspace(-ex / 2, ex / 2, nx + 1)
+    zs = np.linspace(-ez / 2, ez / 2, nz + 1)
+    X, Z = np.meshgrid(xs, zs)
+    pos = np.stack([X, hgt, Z], -1).reshape(-1, 3)
+    gz, gx = np.gradient(hgt, zs, xs)
+    nrm = np.stack([-gx, np.ones_like(gx), -gz], -1)
+    nrm /= np.linalg.norm(nrm, axis=-1, keepdims=True)
+    nrm = nrm.reshape(-1, 3)
+    i = np.arange(nz)[:, None] * (nx + 1) + np.arange(nx)[None, :]
+    a, b, c, d = i, i + 1, i + nx + 2, i + nx + 1
+    idx = np.concatenate([np.stack([a, d, c], -1).reshape(-1, 3), np.stack([a, c, b], -1).reshape(-1, 3)])
+    ground = material(diffuse=(0.45, 0.42, 0.38), specular=0.2, roughness=0.9)
+    mats = [ground]
+    mat = [np.zeros(len(idx), np.uint32)]
+    bld = _Builder()
+    bld.nv = len(pos)
+    for k in range(boxes):
+        bx, bz = rng.uniform(-ex / 2 * 0.9, ex / 2 * 0.9), rng.uniform(-ez / 2 * 0.9, ez / 2 * 0.9)
+        ix = int(round((bx + ex / 2) / ex * nx))
+        iz = int(round((bz + ez / 2) / ez * nz))
+        half = rng.uniform(0.5, 3.0, 3)
+        par = rng.uniform(0.0, 1.0, 6)
+        m = len(mats)
+        mats.append(material(diffuse=tuple(0.2 + 0.6 * par[:3]), metallic=par[3], roughness=0.05 + 0.95 * par[4],
+                             specular=par[5], sheen=rng.uniform(), clearcoat=rng.uniform()))
+        bld.add_box((bx, hgt[iz, ix] + half[1] * 0.8, bz), tuple(half), rng.uniform(0, 2 * math.pi), m)
+    lm = len(mats)
+    mats.append(material(diffuse=(0.0, 0.0, 0.0), emissive=(light_radiance,) * 3))
+    for lx, lz in ((-ex / 4, -ez / 4), (ex / 4, -ez / 4), (-ex / 4, ez / 4), (ex / 4, ez / 4)):
+        p, n, f = icosphere((lx, amplitude * 2.5 + 30.0, lz), 6.0, 1)
+        bld.add_mesh(p, n, f, lm)
+    mats_arr = np.zeros(len(mats), MATERIAL_DTYPE)
+    for j, m in enumerate(mats):
+        mats_arr[j] = m
+    cam = dict(pos=(0.0, amplitude * 2.0 + 25.0, ez / 2 + 40.0), fov=math.radians(45.0), up=(0.0, 1.0, 0.0))
+    d = np.array([0.0, -(amplitude + 25.0), -(ez / 2 + 40.0)])
+    cam["dir"] = tuple(d / np.linalg.norm(d))
+    extra_mat = np.asarray(bld.mat, np.uint32)
+    return dict(pos=np.concatenate([pos.astype(np.float32), np.asarray(bld.pos, np.float32).reshape(-1, 3)]),
+                nrm=np.concatenate([nrm.astype(np.float32), np.asarray(bld.nrm, np.float32).reshape(-1, 3)]),
+                idx=np.concatenate([idx.astype(np.uint32), np.asarray(bld.idx, np.uint32).reshape(-1, 3)]),
+                mat=np.concatenate(mat + [extra_mat]), mats=mats_arr, camera=cam)
+
+
+def heightfield_poses(n, seed0=0xCA3E0000, resolutions=((256, 256), (640, 480), (1280, 720), (1920, 1080)),
+                      extent=(256.0, 128.0), amplitude=18.0):
+    """n seeded camera views over the heightfield, resolutions drawn from `resolutions`."""
+    ex, ez = extent
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(seed0 + i)
+        w, h = resolutions[int(rng.integers(len(resolutions)))]
+        pos = np.array([rng.uniform(-ex / 2, ex / 2), amplitude * rng.uniform(1.5, 3.0) + 10.0,
+                        rng.uniform(-ez / 2, ez / 2)])
+        tgt = np.array([rng.uniform(-ex / 3, ex / 3), 0.0, rng.uniform(-ez / 3, ez / 3)])
+        d = tgt - pos
+        d /= np.linalg.norm(d)
+        out.append(dict(width=w, height=h, pos=tuple(pos), dir=tuple(d), up=(0.0, 1.0, 0.0),
+                        fov=math.radians(50.0)))
+    return out
+
+
 def read_obj(text: str):
     """Positions and faces of an OBJ file; polygons are fan-triangulated
     (assimp's aiProcess_Triangulate order for convex faces)."""
