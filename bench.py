@@ -8,8 +8,9 @@ before timing; the timed region is K renders of 64 frames each.
 
 metric/value: Mrays/s = (extension + shadow rays traced) / wall time.
 roofline: the closest-hit extension kernel, algorithmic bytes per launch =
-  48 B per ray (32 B ray in + 16 B hit out) + 128 B per 4-wide node visited +
-  48 B per triangle tested, over its HIP-event duration on the renderer stream.
+  48 B per ray (32 B ray in + 16 B hit out) + 128 B (FP32 layout) or 64 B
+  (quantized layout) per 4-wide node visited + 48 B per triangle tested, over
+  its HIP-event duration on the renderer stream.
 cpu_baseline: the oracle's FP32 reference estimator on the host cores (OpenMP),
   a bounded sample of the same frame (a band of rows at 1 spp).
 time_to_equal_mse: the learned-vs-plain adaptive protocol of cudapt_amd/learned.py
@@ -147,7 +148,8 @@ def hbm_traffic(args):
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t",
                "--", sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
                "--no-traffic", "--no-learned", "--no-multiview", "--frames", str(args.frames), "--in-flight", str(args.in_flight),
-               "--lds-stack", str(args.lds_stack), "--copy-calib", str(calib >> 20)]
+               "--lds-stack", str(args.lds_stack), "--node-format", str(args.node_format),
+               "--copy-calib", str(calib >> 20)]
         log(f"traffic pass: {counter}")
         r = subprocess.run(cmd, capture_output=True, text=True)
         files = [os.path.join(root, f) for root, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
@@ -212,6 +214,7 @@ def main():
     ap.add_argument("--in-flight", type=int, default=64, help="frames traced as one wavefront")
     ap.add_argument("--lds-stack", type=int, default=806, help="traversal variant: LDS stack entries (8/16/24; "
                     "806/807/808/1606 = 8 or 16 entries with >= 6, 7 or 8 waves per SIMD)")
+    ap.add_argument("--node-format", type=int, default=0, help="BVH nodes: 0 = 128-byte FP32, 1 = 64-byte quantized")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
@@ -253,6 +256,7 @@ def main():
     R.set_attrib("Frames In Flight", args.in_flight)
     R.set_attrib("Traversal LDS Stack", args.lds_stack)
     R.set_attrib("Shadow LDS Stack", args.lds_stack)
+    R.set_attrib("Node Format", args.node_format)
     cam = sc["camera"]
     for w in range(args.warmup):
         R.render(S, cam, frames=args.frames, result=False)
@@ -274,7 +278,8 @@ def main():
     rays = st["extension_rays"] + st["shadow_rays"]
     mrays = rays / wall / 1e6
     launches = max(st["extend_launches"], 1)
-    bytes_alg = 48 * st["extension_rays"] + 128 * st["extend_nodes"] + 48 * st["extend_tris"]
+    node_bytes = 64 if args.node_format else 128
+    bytes_alg = 48 * st["extension_rays"] + node_bytes * st["extend_nodes"] + 48 * st["extend_tris"]
     ext_s = st["extend_ms"] / 1e3
     achieved = bytes_alg / ext_s / 1e9 if ext_s > 0 else 0.0
     res = {
@@ -294,7 +299,7 @@ def main():
                                "depth 8, 64 spp per step, light-vertex cache every 3rd frame",
                    "triangles": int(len(sc["idx"])), "width": WIDTH, "height": HEIGHT, "spp": args.frames,
                    "eye_depth": EYE_DEPTH, "light_depth": LIGHT_DEPTH, "frames_in_flight": args.in_flight,
-                   "traversal_variant": args.lds_stack},
+                   "traversal_variant": args.lds_stack, "node_format": args.node_format},
         "samples_per_s": round(st["eye_paths"] / wall, 1),
         "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},
         "kernel_ms": {"extend": round(st["extend_ms"], 3), "shade": round(st["shade_ms"], 3),

@@ -358,10 +358,12 @@ class _View(ctypes.Structure):
 class BatchConfig(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint32) for n in ("frames_in_flight", "extend_chunk", "extend_workgroups",
                                                "traversal_variant", "use_graph", "eye_depth", "light_depth",
-                                               "light_paths")] + [("seed", ctypes.c_uint64)]
+                                               "light_paths")] + [("seed", ctypes.c_uint64),
+                                                                  ("node_format", ctypes.c_uint32),
+                                                                  ("pad", ctypes.c_uint32)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_}
+        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
 
 
 class Batch:

@@ -254,4 +254,71 @@ void collapse_bvh4(const float* /*pos3*/, const u32* /*idx3*/, BvhHost& b, u32 l
   }
 }
 
+
+// ---- 64-byte quantized nodes -------------------------------------------------
+
+namespace {
+
+inline float q_decode(float o, u32 q, float step) { return o + (float)q * step; }
+
+// Smallest exponent e (step 2^(e-127)) with o + 255 * step >= hi.
+u32 q_exponent(float o, float hi) {
+  for (u32 e = 1; e < 255; e++) {
+    float step = ldexpf(1.f, (int)e - 127);
+    if (q_decode(o, 255u, step) >= hi) return e;
+  }
+  return 0;
+}
+
+}  // namespace
+
+bool quantize_bvh4(BvhHost& b) {
+  const std::vector<Node4>& in = b.nodes4;
+  std::vector<Node4Q>& out = b.nodes4q;
+  out.assign(in.size(), Node4Q());
+  for (size_t i = 0; i < in.size(); i++) {
+    const Node4& n = in[i];
+    Node4Q& q = out[i];
+    std::memset(&q, 0, sizeof q);
+    const float* lo[3] = {n.lox, n.loy, n.loz};
+    const float* hi[3] = {n.hix, n.hiy, n.hiz};
+    uint8_t* qlo[3] = {q.lox, q.loy, q.loz};
+    uint8_t* qhi[3] = {q.hix, q.hiy, q.hiz};
+    u32 exps = 0;
+    for (int a = 0; a < 3; a++) {
+      float o = INFINITY, top = -INFINITY;
+      for (int c = 0; c < 4; c++) {
+        if (n.child[c] == kInvalid) continue;
+        o = std::min(o, lo[a][c]);
+        top = std::max(top, hi[a][c]);
+      }
+      if (!(o <= top)) { o = 0.f; top = 0.f; }          // no valid child
+      u32 e = q_exponent(o, top);
+      if (e == 0) return false;
+      const float step = ldexpf(1.f, (int)e - 127);
+      q.o[a] = o;
+      exps |= e << (8 * a);
+      for (int c = 0; c < 4; c++) {
+        if (n.child[c] == kInvalid) { qlo[a][c] = 255; qhi[a][c] = 0; continue; }
+        double fl = std::floor(((double)lo[a][c] - (double)o) / (double)step);
+        u32 ql = (u32)std::max(0.0, std::min(255.0, fl));
+        while (ql > 0 && q_decode(o, ql, step) > lo[a][c]) ql--;
+        double fh = std::ceil(((double)hi[a][c] - (double)o) / (double)step);
+        u32 qh = (u32)std::max(0.0, std::min(255.0, fh));
+        while (qh < 255 && q_decode(o, qh, step) < hi[a][c]) qh++;
+        if (q_decode(o, ql, step) > lo[a][c] || q_decode(o, qh, step) < hi[a][c]) return false;
+        qlo[a][c] = (uint8_t)ql;
+        qhi[a][c] = (uint8_t)qh;
+      }
+    }
+    q.exps = exps;
+    for (int c = 0; c < 4; c++) {
+      if (n.count[c] > 0xFFFFu) return false;
+      q.count[c] = (uint16_t)n.count[c];
+      q.child[c] = n.child[c];
+    }
+  }
+  return true;
+}
+
 }  // namespace cpt

@@ -77,6 +77,7 @@ struct cpt_scene {
   u32 mesh_version = 0, mat_version = 0, leaf_max = 4;
   hipStream_t stream = nullptr;
   DevArray<Node4> d_nodes4;
+  DevArray<Node4Q> d_nodes4q;
   DevArray<float4> d_tris, d_vpos, d_vnrm, d_bnodes;
   DevArray<uint4> d_tri_verts;
   DevArray<uint2> d_boff;
@@ -88,6 +89,7 @@ struct cpt_scene {
   DevScene dev() const {
     DevScene s;
     s.nodes4 = d_nodes4.p;
+    s.nodes4q = d_nodes4q.p;
     s.tris = d_tris.p;
     s.tri_verts = d_tri_verts.p;
     s.vpos = d_vpos.p;
@@ -223,6 +225,7 @@ int cpt_bvh_build_sah(cpt_scene* s, uint32_t max_depth) {
   if (!build_bvh_sah(s->pos.data(), s->idx.data(), s->nt, max_depth, s->bvh))
     return fail(CPT_E_STATE, "cpt_bvh_build_sah: build failed");
   collapse_bvh4(s->pos.data(), s->idx.data(), s->bvh, s->leaf_max);
+  if (!quantize_bvh4(s->bvh)) return fail(CPT_E_STATE, "cpt_bvh_build_sah: a node cannot be quantized to 64 bytes");
   if (s->bvh.depth4 > 31) return fail(CPT_E_LIMIT, "cpt_bvh_build_sah: 4-wide depth %u exceeds 31", s->bvh.depth4);
   s->has_bvh = true;
   s->uploaded = false;
@@ -288,6 +291,7 @@ int cpt_scene_upload(cpt_scene* s) {
     boff[i] = make_uint2(bn[i].a, bn[i].n);
   }
   HIP_TRY(s->d_nodes4.upload(s->bvh.nodes4.data(), s->bvh.nodes4.size(), s->stream));
+  HIP_TRY(s->d_nodes4q.upload(s->bvh.nodes4q.data(), s->bvh.nodes4q.size(), s->stream));
   HIP_TRY(s->d_tris.upload(tris.data(), tris.size(), s->stream));
   HIP_TRY(s->d_tri_verts.upload(tv.data(), tv.size(), s->stream));
   HIP_TRY(s->d_vpos.upload(vp.data(), vp.size(), s->stream));
@@ -317,7 +321,7 @@ int cpt_trace_closest_device(cpt_scene* s, const cpt_ray* d_rays, uint32_t n, ui
                              void* stream, uint32_t layout) {
   if (!s || (n && (!d_rays || !d_hits))) return fail(CPT_E_ARG, "cpt_trace_closest_device: null argument");
   if (!s->uploaded) return fail(CPT_E_STATE, "cpt_trace_closest_device: scene not uploaded");
-  if (layout > 1) return fail(CPT_E_ARG, "cpt_trace_closest_device: layout %u not 0 or 1", layout);
+  if (layout > 2) return fail(CPT_E_ARG, "cpt_trace_closest_device: layout %u not 0, 1 or 2", layout);
   hipStream_t st = stream ? (hipStream_t)stream : s->stream;
   HIP_TRY(launch_trace(s->dev(), (const float4*)d_rays, n, nullptr, cull, (float4*)d_hits, 1, 0, layout,
                        s->d_stats.p, st));
@@ -442,6 +446,9 @@ const char* const kInFlight = "Frames In Flight";
 // allocates from one error map and traces as one wavefront (1 = the
 // reference's one frame per allocation)
 const char* const kPassFrames = "Adaptive Pass Frames";
+// BVH node format of the traversal kernels: 0 = 128-byte FP32 4-wide nodes,
+// 1 = the same nodes quantized to 64 bytes
+const char* const kNodeFormat = "Node Format";
 // allocation signal from the learned model's output y (log per-sample
 // variance): 0 = nd / (nd + N) * exp(y) / N, the filter-residual estimate's
 // shape; 1 = exp(y) / N, the expected squared error (samples settle where N
@@ -678,12 +685,13 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
   for (u32 b = 0; b < fa.eye_depth; b++) {
     r->extend_launches++;
     HIP_TRY(timed_begin(r, 0, &ev));
+    const int quant = r->attr(kNodeFormat) != 0;
     if (r->persist_chunk)
       HIP_TRY(launch_extend_persist(sc, cur->view(), r->fc.p, (int)b, r->hits.p, r->stats.p, (int)r->attr(kLdsStack),
-                                    r->persist_chunk, r->persist_grid, r->queue.p, r->stream));
+                                    quant, r->persist_chunk, r->persist_grid, r->queue.p, r->stream));
     else
       HIP_TRY(launch_extend(sc, cur->view(), r->fc.p, (int)b, max_n, r->hits.p, r->stats.p, (int)r->attr(kLdsStack),
-                            r->stream));
+                            quant, r->stream));
     HIP_TRY(timed_end(r, 0, ev));
     HIP_TRY(timed_begin(r, 2, &ev));
     HIP_TRY(launch_shade(sc, fa, cur->view(), nxt->view(), r->hits.p, r->fc.p, (int)b, max_n, sb, r->lv.p, ac,
@@ -691,7 +699,8 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
     HIP_TRY(timed_end(r, 2, ev));
     if (b + 1 < fa.eye_depth) {
       HIP_TRY(timed_begin(r, 1, &ev));
-      HIP_TRY(launch_shadow(sc, sb, nxt->view(), r->fc.p, (int)b, max_n, (int)r->attr(kLdsStackShadow), r->stream));
+      HIP_TRY(launch_shadow(sc, sb, nxt->view(), r->fc.p, (int)b, max_n, (int)r->attr(kLdsStackShadow), quant,
+                            r->stream));
       HIP_TRY(timed_end(r, 1, ev));
     }
     std::swap(cur, nxt);
@@ -1012,6 +1021,7 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   // (swept on the sphere grid: 1.58 vs 1.36 Grays/s for 16 entries)
   r->attribs.push_back({kLdsStack, 0, 806});
   r->attribs.push_back({kLdsStackShadow, 0, 806});
+  r->attribs.push_back({kNodeFormat, 0, 0});
   int rc = ensure_stream(r->stream);
   if (rc) { delete r; return rc; }
   *out = r;
@@ -1389,6 +1399,7 @@ int batch_check_config(const cpt_batch_config& c) {
   if (c.eye_depth < 1 || c.eye_depth > (u32)kMaxDepth) return fail(CPT_E_LIMIT, "batch: eye depth not in 1..15");
   if (c.light_depth < 1 || c.light_depth > 64) return fail(CPT_E_LIMIT, "batch: light depth not in 1..64");
   if (c.light_paths < 1 || c.light_paths > (u32)kBlock) return fail(CPT_E_LIMIT, "batch: light paths not in 1..256");
+  if (c.node_format > 1) return fail(CPT_E_ARG, "batch: node format %u not 0 or 1", c.node_format);
   return CPT_OK;
 }
 
@@ -1400,6 +1411,7 @@ void batch_apply_config(cpt_batch* b) {
   r->set_attr(kSeed, (double)b->cfg.seed);
   r->set_attr(kInFlight, b->cfg.frames_in_flight);
   r->set_attr(kLdsStack, b->cfg.traversal_variant);
+  r->set_attr(kNodeFormat, b->cfg.node_format ? 1 : 0);
   r->persist_chunk = b->cfg.extend_chunk;
   r->persist_grid = b->cfg.extend_chunk ? b->cfg.extend_workgroups * (u32)(b->cus > 0 ? b->cus : 256) : 0u;
   b->drop_graph();

@@ -35,11 +35,29 @@ struct alignas(128) Node4 {
 };
 static_assert(sizeof(Node4) == 128, "Node4 must be one 128-byte line");
 
+// The same node in 64 bytes: child boxes quantized to 8 bits on a per-node
+// grid (origin o, power-of-two step 2^(e-127) per axis), decoded as
+// o + q * step in FP32.  The host picks every q so that the decoded box
+// contains the child's FP32 box (lo rounded down, hi up, checked with the
+// decode arithmetic), so culling stays conservative and hits unchanged.
+// Loaded as four 16-byte vectors: {o, exponents}, {lo/hi x, lo/hi y},
+// {lo/hi z, counts}, {children}.  Counts above 65535 cannot be stored (the
+// build reports it).
+struct alignas(64) Node4Q {
+  float o[3];
+  u32 exps;            // ex | ey << 8 | ez << 16
+  uint8_t lox[4], hix[4], loy[4], hiy[4], loz[4], hiz[4];
+  uint16_t count[4];
+  u32 child[4];
+};
+static_assert(sizeof(Node4Q) == 64, "Node4Q must be 64 bytes");
+
 // Host result of the SAH build.
 struct BvhHost {
   std::vector<BinNode> nodes;      // DFS preorder, reference CompactBVH order
   std::vector<u32> reorder;        // leaf order -> original triangle id
   std::vector<Node4> nodes4;       // 4-wide re-layout (BFS order, root 0)
+  std::vector<Node4Q> nodes4q;     // the same nodes quantized to 64 bytes
   u32 depth4 = 0, leaves4 = 0;
 };
 
@@ -48,5 +66,7 @@ bool build_bvh_sah(const float* pos3, const u32* idx3, u32 nt, u32 max_depth, Bv
 // Collapse the binary tree into 4-wide nodes; subtrees of <= leaf_max
 // triangles become one leaf range.
 void collapse_bvh4(const float* pos3, const u32* idx3, BvhHost& b, u32 leaf_max);
+// Quantize nodes4 into nodes4q; false when a node cannot be encoded.
+bool quantize_bvh4(BvhHost& b);
 
 }  // namespace cpt

@@ -19,6 +19,7 @@ struct DevMaterial {
 // Device-resident scene.  Triangles are in BVH leaf order.
 struct DevScene {
   const Node4* nodes4;         // 128-byte 4-wide nodes
+  const Node4Q* nodes4q;       // the same nodes, 64-byte quantized
   const float4* tris;          // 3 per triangle: p0 (w = original id bits), e0 = p1-p0, e1 = p2-p0
   const uint4* tri_verts;      // v0, v1, v2, material
   const float4* vpos;          // vertex positions
@@ -135,7 +136,7 @@ hipError_t launch_apply_frames(const FrameArgs& fa, const float4* rec_acc, float
                                hipStream_t st);
 // lds_stack: traversal stack entries kept in LDS (8, 16 or 24; the rest in private memory)
 hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, uint32_t max_n,
-                         float4* hits, DevStats* stats, int lds_stack, hipStream_t st);
+                         float4* hits, DevStats* stats, int lds_stack, int quant, hipStream_t st);
 // apply: 0 = plain running mean at termination, 1 = running mean + G-buffer
 // (adaptive warm-up), 2 = write the sample record only (adaptive frame).
 struct Accum {
@@ -208,7 +209,7 @@ hipError_t launch_round_fill(const RoundState* rs, uint32_t n, uint32_t* counts,
 hipError_t launch_scan_counts(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint32_t* block_tot,
                               uint32_t* total, hipStream_t st);
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
-                         uint32_t max_n, int lds_stack, hipStream_t st);
+                         uint32_t max_n, int lds_stack, int quant, hipStream_t st);
 hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes, uint32_t depth, hipStream_t st);
 // *frame_dev += k (the device frame counter of replayed graphs).
 hipError_t launch_frame_advance(uint32_t* frame_dev, uint32_t k, hipStream_t st);
@@ -216,7 +217,7 @@ hipError_t launch_frame_advance(uint32_t* frame_dev, uint32_t k, hipStream_t st)
 // takes `chunk` rays at a time from the global counter `queue` until the
 // queue is drained.
 hipError_t launch_extend_persist(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, float4* hits,
-                                 DevStats* stats, int lds_stack, uint32_t chunk, uint32_t grid, uint32_t* queue,
-                                 hipStream_t st);
+                                 DevStats* stats, int lds_stack, int quant, uint32_t chunk, uint32_t grid,
+                                 uint32_t* queue, hipStream_t st);
 
 }  // namespace cpt

@@ -105,8 +105,53 @@ __device__ __forceinline__ void cswap(uint64_t& a, uint64_t& b) {
   a = lo; b = hi;
 }
 
+// Child boxes and links of a 4-wide node, from the 128-byte FP32 layout or
+// decoded from the 64-byte quantized one (lo = o + q * step, the arithmetic
+// the host checked containment with).
+struct NodeView {
+  float lx[4], hx[4], ly[4], hy[4], lz[4], hz[4];
+  uint32_t ch[4], cn[4];
+};
+template <bool Q>
+__device__ __forceinline__ void load_node(const DevScene& sc, uint32_t node, NodeView& v) {
+  if (!Q) {
+    const Node4* N = sc.nodes4 + node;
+    const float4 lx = *(const float4*)N->lox, hx = *(const float4*)N->hix;
+    const float4 ly = *(const float4*)N->loy, hy = *(const float4*)N->hiy;
+    const float4 lz = *(const float4*)N->loz, hz = *(const float4*)N->hiz;
+    const uint4 ch = *(const uint4*)N->child, cn = *(const uint4*)N->count;
+    v.lx[0] = lx.x; v.lx[1] = lx.y; v.lx[2] = lx.z; v.lx[3] = lx.w;
+    v.hx[0] = hx.x; v.hx[1] = hx.y; v.hx[2] = hx.z; v.hx[3] = hx.w;
+    v.ly[0] = ly.x; v.ly[1] = ly.y; v.ly[2] = ly.z; v.ly[3] = ly.w;
+    v.hy[0] = hy.x; v.hy[1] = hy.y; v.hy[2] = hy.z; v.hy[3] = hy.w;
+    v.lz[0] = lz.x; v.lz[1] = lz.y; v.lz[2] = lz.z; v.lz[3] = lz.w;
+    v.hz[0] = hz.x; v.hz[1] = hz.y; v.hz[2] = hz.z; v.hz[3] = hz.w;
+    v.ch[0] = ch.x; v.ch[1] = ch.y; v.ch[2] = ch.z; v.ch[3] = ch.w;
+    v.cn[0] = cn.x; v.cn[1] = cn.y; v.cn[2] = cn.z; v.cn[3] = cn.w;
+  } else {
+    const uint4* N = (const uint4*)(sc.nodes4q + node);
+    const uint4 A = N[0], B = N[1], C = N[2], D = N[3];
+    const float ox = __uint_as_float(A.x), oy = __uint_as_float(A.y), oz = __uint_as_float(A.z);
+    const float sx = __uint_as_float((A.w & 0xFFu) << 23), sy = __uint_as_float(((A.w >> 8) & 0xFFu) << 23);
+    const float sz = __uint_as_float(((A.w >> 16) & 0xFFu) << 23);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      const int sh = 8 * c;
+      v.lx[c] = ox + (float)((B.x >> sh) & 0xFFu) * sx;
+      v.hx[c] = ox + (float)((B.y >> sh) & 0xFFu) * sx;
+      v.ly[c] = oy + (float)((B.z >> sh) & 0xFFu) * sy;
+      v.hy[c] = oy + (float)((B.w >> sh) & 0xFFu) * sy;
+      v.lz[c] = oz + (float)((C.x >> sh) & 0xFFu) * sz;
+      v.hz[c] = oz + (float)((C.y >> sh) & 0xFFu) * sz;
+    }
+    v.cn[0] = C.z & 0xFFFFu; v.cn[1] = C.z >> 16; v.cn[2] = C.w & 0xFFFFu; v.cn[3] = C.w >> 16;
+    v.ch[0] = D.x; v.ch[1] = D.y; v.ch[2] = D.z; v.ch[3] = D.w;
+  }
+}
+
 // lstk points at this thread's column of a [STK][kBlock] u64 array in LDS.
-template <bool ANY, int STK = CPT_LDS_STACK>
+// Q: nodes in the 64-byte quantized layout.
+template <bool ANY, int STK = CPT_LDS_STACK, bool Q = false>
 __device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps, bool cull, uint64_t* lstk,
                           TraceHit& h, uint32_t& nn, uint32_t& nt) {
   const f3 idir = mk3(1.f / escape_zero(d.x), 1.f / escape_zero(d.y), 1.f / escape_zero(d.z));
@@ -116,22 +161,17 @@ __device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps,
   int sp = 0;
   uint32_t node = 0;
   for (;;) {
-    const Node4* N = sc.nodes4 + node;
-    const float4 lx = *(const float4*)N->lox, hx = *(const float4*)N->hix;
-    const float4 ly = *(const float4*)N->loy, hy = *(const float4*)N->hiy;
-    const float4 lz = *(const float4*)N->loz, hz = *(const float4*)N->hiz;
-    const uint4 ch = *(const uint4*)N->child, cn = *(const uint4*)N->count;
+    NodeView nv;
+    load_node<Q>(sc, node, nv);
     nn++;
-    const float lx_[4] = {lx.x, lx.y, lx.z, lx.w}, hx_[4] = {hx.x, hx.y, hx.z, hx.w};
-    const float ly_[4] = {ly.x, ly.y, ly.z, ly.w}, hy_[4] = {hy.x, hy.y, hy.z, hy.w};
-    const float lz_[4] = {lz.x, lz.y, lz.z, lz.w}, hz_[4] = {hz.x, hz.y, hz.z, hz.w};
-    const uint32_t ch_[4] = {ch.x, ch.y, ch.z, ch.w}, cn_[4] = {cn.x, cn.y, cn.z, cn.w};
+    const uint32_t* ch_ = nv.ch;
+    const uint32_t* cn_ = nv.cn;
     uint64_t key[4];
 #pragma unroll
     for (int c = 0; c < 4; c++) {
-      float t0x = (lx_[c] - o.x) * idir.x, t1x = (hx_[c] - o.x) * idir.x;
-      float t0y = (ly_[c] - o.y) * idir.y, t1y = (hy_[c] - o.y) * idir.y;
-      float t0z = (lz_[c] - o.z) * idir.z, t1z = (hz_[c] - o.z) * idir.z;
+      float t0x = (nv.lx[c] - o.x) * idir.x, t1x = (nv.hx[c] - o.x) * idir.x;
+      float t0y = (nv.ly[c] - o.y) * idir.y, t1y = (nv.hy[c] - o.y) * idir.y;
+      float t0z = (nv.lz[c] - o.z) * idir.z, t1z = (nv.hz[c] - o.z) * idir.z;
       float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
       float tfar = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * 1.0000004f;
       bool hit = ch_[c] != kInvalid && tnear <= tfar && tnear <= h.t + cull_slack(h.t, eps);
@@ -232,8 +272,10 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace(DevScene sc, const float
   if (i < count) {
     float4 a = ro[(size_t)i * stride], b = rd[(size_t)i * stride];
     TraceHit h;
-    bool hit = LAYOUT == 0 ? traverse4<ANY, STK>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, lstk + threadIdx.x, h, nn, nt)
-                           : traverse_bin<ANY>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, h, nn, nt);
+    bool hit = LAYOUT == 0   ? traverse4<ANY, STK>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, lstk + threadIdx.x, h, nn, nt)
+               : LAYOUT == 2 ? traverse4<ANY, STK, true>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, lstk + threadIdx.x, h,
+                                                         nn, nt)
+                             : traverse_bin<ANY>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, h, nn, nt);
     if (OUT == 0) {
       float4 r;
       r.x = __uint_as_float(hit ? h.orig : kInvalid);
@@ -268,16 +310,16 @@ hipError_t launch_trace(const DevScene& sc, const float4* rays, uint32_t n, cons
   dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
   const float4* ro = rays;
   const float4* rd = rays + 1;
+  if (layout > 2 || (layout == 2 && !sc.nodes4q) || (layout == 0 && !sc.nodes4)) return hipErrorInvalidValue;
+#define CPT_TR(L, A, O) k_trace<L, A, O><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats)
   if (any_hit) {
-    if (layout == 0) k_trace<0, true, 2><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
-    else k_trace<1, true, 2><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
+    if (layout == 0) CPT_TR(0, true, 2); else if (layout == 2) CPT_TR(2, true, 2); else CPT_TR(1, true, 2);
   } else if (out_orig) {
-    if (layout == 0) k_trace<0, false, 0><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
-    else k_trace<1, false, 0><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
+    if (layout == 0) CPT_TR(0, false, 0); else if (layout == 2) CPT_TR(2, false, 0); else CPT_TR(1, false, 0);
   } else {
-    if (layout == 0) k_trace<0, false, 1><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
-    else k_trace<1, false, 1><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats);
+    if (layout == 0) CPT_TR(0, false, 1); else if (layout == 2) CPT_TR(2, false, 1); else CPT_TR(1, false, 1);
   }
+#undef CPT_TR
   return hipGetLastError();
 }
 
@@ -713,27 +755,38 @@ hipError_t launch_apply_frames(const FrameArgs& fa, const float4* rec_acc, float
 }
 
 hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, uint32_t max_n,
-                         float4* hits, DevStats* stats, int lds_stack, hipStream_t st) {
+                         float4* hits, DevStats* stats, int lds_stack, int quant, hipStream_t st) {
   if (max_n == 0) return hipSuccess;
-  if (!pathbuf_ok(pb) || !fc || !hits || !sc.nodes4 || !sc.tris) return hipErrorInvalidValue;
+  if (!pathbuf_ok(pb) || !fc || !hits || !sc.tris || !(quant ? (const void*)sc.nodes4q : (const void*)sc.nodes4))
+    return hipErrorInvalidValue;
   dim3 grid((max_n + kBlock - 1) / kBlock), block(kBlock);
   const uint32_t* cnt = &fc->active[bounce];
-#define CPT_EXT(S, W) k_trace<0, false, 1, S, W><<<grid, block, 0, st>>>(sc, pb.ray_o, pb.ray_d, 1, cnt, max_n, 0, hits, stats)
-  switch (lds_stack) {
-    case 8: CPT_EXT(8, 0); break;
-    case 24: CPT_EXT(24, 0); break;
-    case 806: CPT_EXT(8, 6); break;      // 8 LDS entries, >= 6 waves per SIMD
-    case 807: CPT_EXT(8, 7); break;      // 8 LDS entries, >= 7 waves per SIMD
-    case 808: CPT_EXT(8, 8); break;      // 8 LDS entries, >= 8 waves per SIMD
-    case 1606: CPT_EXT(16, 6); break;
-    default: CPT_EXT(16, 0); break;
+#define CPT_EXT(L, S, W) k_trace<L, false, 1, S, W><<<grid, block, 0, st>>>(sc, pb.ray_o, pb.ray_d, 1, cnt, max_n, 0, hits, stats)
+  if (quant) {
+    switch (lds_stack) {
+      case 8: CPT_EXT(2, 8, 0); break;
+      case 806: CPT_EXT(2, 8, 6); break;
+      case 808: CPT_EXT(2, 8, 8); break;
+      case 1606: CPT_EXT(2, 16, 6); break;
+      default: CPT_EXT(2, 16, 0); break;
+    }
+  } else {
+    switch (lds_stack) {
+      case 8: CPT_EXT(0, 8, 0); break;
+      case 24: CPT_EXT(0, 24, 0); break;
+      case 806: CPT_EXT(0, 8, 6); break;      // 8 LDS entries, >= 6 waves per SIMD
+      case 807: CPT_EXT(0, 8, 7); break;      // 8 LDS entries, >= 7 waves per SIMD
+      case 808: CPT_EXT(0, 8, 8); break;      // 8 LDS entries, >= 8 waves per SIMD
+      case 1606: CPT_EXT(0, 16, 6); break;
+      default: CPT_EXT(0, 16, 0); break;
+    }
   }
 #undef CPT_EXT
   return hipGetLastError();
 }
 
 // Persistent closest-hit grid over a global queue of the bounce's live rays.
-template <int STK, int MINW>
+template <int STK, int MINW, bool Q>
 __global__ __launch_bounds__(kBlock, MINW) void k_extend_persist(DevScene sc, const float4* __restrict__ ro,
                                                                  const float4* __restrict__ rd,
                                                                  const uint32_t* __restrict__ countp, uint32_t chunk,
@@ -754,7 +807,7 @@ __global__ __launch_bounds__(kBlock, MINW) void k_extend_persist(DevScene sc, co
     for (uint32_t i = base + threadIdx.x; i < end; i += kBlock) {
       float4 a = ro[i], b = rd[i];
       TraceHit h;
-      bool hit = traverse4<false, STK>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt);
+      bool hit = traverse4<false, STK, Q>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt);
       float4 r;
       r.x = hit ? h.t : kInfF;
       r.y = __uint_as_float(hit ? h.leaf : kInvalid);
@@ -772,18 +825,19 @@ __global__ __launch_bounds__(kBlock, MINW) void k_extend_persist(DevScene sc, co
   }
 }
 hipError_t launch_extend_persist(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, float4* hits,
-                                 DevStats* stats, int lds_stack, uint32_t chunk, uint32_t grid, uint32_t* queue,
-                                 hipStream_t st) {
-  if (!pathbuf_ok(pb) || !fc || !hits || !sc.nodes4 || !sc.tris || !queue || chunk == 0 || grid == 0)
+                                 DevStats* stats, int lds_stack, int quant, uint32_t chunk, uint32_t grid,
+                                 uint32_t* queue, hipStream_t st) {
+  if (!pathbuf_ok(pb) || !fc || !hits || !sc.tris || !queue || chunk == 0 || grid == 0 ||
+      !(quant ? (const void*)sc.nodes4q : (const void*)sc.nodes4))
     return hipErrorInvalidValue;
   k_fill_u32<<<1, 1, 0, st>>>(queue, 0u);
   const uint32_t* cnt = &fc->active[bounce];
-#define CPT_EXTP(S, W) k_extend_persist<S, W><<<grid, kBlock, 0, st>>>(sc, pb.ray_o, pb.ray_d, cnt, chunk, queue, hits, stats)
+#define CPT_EXTP(S, W, Q) k_extend_persist<S, W, Q><<<grid, kBlock, 0, st>>>(sc, pb.ray_o, pb.ray_d, cnt, chunk, queue, hits, stats)
   switch (lds_stack) {
-    case 8: CPT_EXTP(8, 0); break;
-    case 806: CPT_EXTP(8, 6); break;
-    case 1606: CPT_EXTP(16, 6); break;
-    default: CPT_EXTP(16, 0); break;
+    case 8: if (quant) CPT_EXTP(8, 0, true); else CPT_EXTP(8, 0, false); break;
+    case 806: if (quant) CPT_EXTP(8, 6, true); else CPT_EXTP(8, 6, false); break;
+    case 1606: if (quant) CPT_EXTP(16, 6, true); else CPT_EXTP(16, 6, false); break;
+    default: if (quant) CPT_EXTP(16, 0, true); else CPT_EXTP(16, 0, false); break;
   }
 #undef CPT_EXTP
   return hipGetLastError();
@@ -978,7 +1032,7 @@ hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, Pa
   return hipGetLastError();
 }
 
-template <int STK, int MINW = 0>
+template <int STK, int MINW = 0, bool Q = false>
 __global__ __launch_bounds__(kBlock, MINW) void k_shadow(DevScene sc, ShadowBuf sb, PathBuf nxt,
                                                    const FrameCounters* __restrict__ fc, int bounce) {
   __shared__ uint64_t lstk[STK * kBlock];
@@ -988,7 +1042,7 @@ __global__ __launch_bounds__(kBlock, MINW) void k_shadow(DevScene sc, ShadowBuf 
   float4 a = sb.ray_o[i], b = sb.ray_d[i], c = sb.contrib[i];
   TraceHit h;
   uint32_t nn = 0, nt = 0;
-  if (!traverse4<true, STK>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt)) {
+  if (!traverse4<true, STK, Q>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt)) {
     uint32_t slot = __float_as_uint(c.w);
     float4 acc = nxt.acc[slot];
     acc.x = acc.x + c.x;
@@ -999,19 +1053,33 @@ __global__ __launch_bounds__(kBlock, MINW) void k_shadow(DevScene sc, ShadowBuf 
   }
 }
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
-                         uint32_t max_n, int lds_stack, hipStream_t st) {
+                         uint32_t max_n, int lds_stack, int quant, hipStream_t st) {
   if (max_n == 0) return hipSuccess;
-  if (!pathbuf_ok(nxt) || !sb.ray_o || !sb.ray_d || !sb.contrib || !fc) return hipErrorInvalidValue;
+  if (!pathbuf_ok(nxt) || !sb.ray_o || !sb.ray_d || !sb.contrib || !fc || !sc.tris ||
+      !(quant ? (const void*)sc.nodes4q : (const void*)sc.nodes4))
+    return hipErrorInvalidValue;
   dim3 grid((max_n + kBlock - 1) / kBlock), block(kBlock);
-  switch (lds_stack) {
-    case 8: k_shadow<8><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
-    case 24: k_shadow<24><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
-    case 806: k_shadow<8, 6><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
-    case 807: k_shadow<8, 7><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
-    case 808: k_shadow<8, 8><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
-    case 1606: k_shadow<16, 6><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
-    default: k_shadow<16><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce); break;
+#define CPT_SH(S, W, Q) k_shadow<S, W, Q><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce)
+  if (quant) {
+    switch (lds_stack) {
+      case 8: CPT_SH(8, 0, true); break;
+      case 806: CPT_SH(8, 6, true); break;
+      case 808: CPT_SH(8, 8, true); break;
+      case 1606: CPT_SH(16, 6, true); break;
+      default: CPT_SH(16, 0, true); break;
+    }
+  } else {
+    switch (lds_stack) {
+      case 8: CPT_SH(8, 0, false); break;
+      case 24: CPT_SH(24, 0, false); break;
+      case 806: CPT_SH(8, 6, false); break;
+      case 807: CPT_SH(8, 7, false); break;
+      case 808: CPT_SH(8, 8, false); break;
+      case 1606: CPT_SH(16, 6, false); break;
+      default: CPT_SH(16, 0, false); break;
+    }
   }
+#undef CPT_SH
   return hipGetLastError();
 }
 

@@ -120,7 +120,8 @@ int cpt_trace_closest(cpt_scene* s, const cpt_ray* rays, uint32_t n, uint32_t cu
 
 /* Same over device arrays already resident in HBM, launched on `stream`
  * (a hipStream_t, NULL = the scene's stream).  layout 0 = 4-wide nodes,
- * 1 = the reference's binary CompactBVH layout. */
+ * 1 = the reference's binary CompactBVH layout, 2 = 4-wide nodes quantized to
+ * 64 bytes (the same hits: culling stays conservative). */
 int cpt_trace_closest_device(cpt_scene* s, const cpt_ray* d_rays, uint32_t n, uint32_t cull, cpt_hit* d_hits,
                              void* stream, uint32_t layout);
 
@@ -239,6 +240,8 @@ typedef struct cpt_batch_config {
   uint32_t use_graph;         /* 1: capture one iteration as a hipGraph and replay it */
   uint32_t eye_depth, light_depth, light_paths;
   uint64_t seed;
+  uint32_t node_format;       /* 0: 128-byte FP32 4-wide nodes, 1: 64-byte quantized nodes */
+  uint32_t pad;
 } cpt_batch_config;
 
 /* Defaults: 24 frames in flight, chunk 0, variant 806, graph on, depths 5/5,

@@ -125,3 +125,18 @@ def test_rmse_harness_on_device(cpt, cornell_runs):
     # the same streams as the 16-frame FP64 run: the last row differs by precision only
     assert rows[-1].rmse < 1e-2 * float(np.sqrt((cornell_runs["r64"] ** 2).mean()))
     assert rows[0].rmse > 10 * rows[-1].rmse
+
+
+@pytest.mark.parametrize("variant", [806, 1606, 16])
+def test_quantized_node_format_same_image(cpt, cornell_runs, variant):
+    """Traversal over 64-byte quantized nodes returns the same hits, so the
+    rendered frames are bit-identical to the 128-byte layout's."""
+    sc, S = cornell_runs["sc"], cornell_runs["S"]
+    a = cpt.Renderer(64, 48)
+    ra = a.render(S, sc["camera"], frames=6)
+    b = cpt.Renderer(64, 48)
+    b.set_attrib("Node Format", 1)
+    b.set_attrib("Traversal LDS Stack", variant)
+    b.set_attrib("Shadow LDS Stack", variant)
+    rb = b.render(S, sc["camera"], frames=6)
+    assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))

@@ -115,3 +115,24 @@ def test_empty_batch(cornell):
     sc, S, O = cornell
     out = S.trace(np.zeros((0, 8), np.float32))
     assert len(out) == 0
+
+
+def test_quantized_nodes_cornell_bit_exact(cpt, cornell):
+    """64-byte quantized nodes: conservative boxes, the same records."""
+    sc, S, O = cornell
+    rays = np.concatenate([camera_rays(sc["camera"], 128, 128), vertex_rays(sc, 20000, seed=12),
+                           random_rays(30000, [-0.99, 0.01, -0.99], [0.99, 1.99, 0.99], seed=13)])
+    ref, _ = O.trace(rays, "brute")
+    q = _gpu_hits(cpt, S, rays, layout=2)
+    assert hits_equal(q, ref).all()
+
+
+def test_quantized_nodes_sphere_grid_bit_exact(cpt, sphere_grid):
+    sc, S, O = sphere_grid
+    rays = np.concatenate([camera_rays(sc["camera"], 320, 180, jitter=True, seed=14),
+                           random_rays(60000, [-64, 0.0, -64], [64, 6.0, 64], seed=15),
+                           vertex_rays(sc, 3000, seed=16)])
+    ref, _ = O.trace(rays, "defined")
+    q = _gpu_hits(cpt, S, rays, layout=2)
+    eq = hits_equal(q, ref)
+    assert eq.all(), f"{(~eq).sum()} hits differ"
