@@ -464,7 +464,7 @@ constexpr u32 kMaxInFlight = 90;   // 30 three-frame light-cache groups + partia
 
 struct cpt_mlp {
   hipStream_t stream = nullptr;
-  DevArray<float> w, m, v, grad, loss, b1, b2, w3, b3, part, loss_part;
+  DevArray<float> w, m, v, grad, loss, b1, b2, w3, b3, part, loss_part, part2;
   DevArray<uint16_t> w1t, w2, w2t;
   int step = 0;
   ~cpt_mlp() {
@@ -1279,10 +1279,11 @@ int cpt_mlp_train_step(cpt_mlp* m, const float* d_x, const float* d_target, uint
   u32 nwg = (n + 63) / 64;
   HIP_TRY(m->part.alloc((size_t)nwg * kMlpParams));
   HIP_TRY(m->loss_part.alloc(nwg));
+  HIP_TRY(m->part2.alloc((size_t)mlp_reduce_chunks(nwg) * kMlpParams));
   MlpState s = m->state();
   HIP_TRY(launch_mlp_train(s, d_x, d_target, n, m->part.p, m->loss_part.p, st));
   int step = apply ? m->step + 1 : (m->step > 0 ? m->step : 1);
-  HIP_TRY(launch_mlp_adam(s, m->part.p, nwg, m->loss_part.p, lr, step, apply != 0, st));
+  HIP_TRY(launch_mlp_adam(s, m->part.p, nwg, m->loss_part.p, lr, step, apply != 0, m->part2.p, st));
   if (apply) m->step = step;
   if (loss) {
     HIP_TRY(hipStreamSynchronize(st));

@@ -44,10 +44,13 @@ struct MlpState {
 // and one loss partial per workgroup.  The loss is mean squared error.
 hipError_t launch_mlp_train(const MlpState& st, const float* x, const float* t, uint32_t n, float* part,
                             float* loss_part, hipStream_t s);
-// Sums the partials in workgroup order into st.grad / st.loss, and when
-// `apply` is set performs Adam step `step` (1-based) and refreshes the operands.
+// Sums the partials into st.grad / st.loss in a fixed order (chunks of
+// workgroups, then the chunks), and when `apply` is set performs Adam step
+// `step` (1-based) and refreshes the operands.  part2 holds
+// mlp_reduce_chunks(nwg) x 4865 floats.
 hipError_t launch_mlp_adam(const MlpState& st, const float* part, uint32_t nwg, const float* loss_part, float lr,
-                           int step, int apply, hipStream_t s);
+                           int step, int apply, float* part2, hipStream_t s);
+uint32_t mlp_reduce_chunks(uint32_t nwg);
 hipError_t launch_mlp_infer(const MlpState& st, const float* x, uint32_t n, float* y, hipStream_t s);
 // Rebuilds the operand pack from st.w.
 hipError_t launch_mlp_pack(const MlpState& st, hipStream_t s);

@@ -11,8 +11,9 @@
 //             dZ1 = dH1 * gelu'(Z1), dW2 = H1^T dZ2, dW1 = X^T dZ1 (all MFMA)
 // Operands sit in LDS in the orientation each MFMA reads contiguously (an
 // activation is stored once per orientation it is read in).  Per-wave weight
-// gradients go to a partials array; k_mlp_adam sums them in workgroup order
-// (deterministic) and applies Adam (Kingma & Ba 2015).
+// gradients go to a partials array; k_mlp_reduce sums them in chunks of
+// kReduceChunk workgroups and k_mlp_adam sums the chunk totals, both in a fixed
+// order (deterministic), and applies Adam (Kingma & Ba 2015).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -32,10 +33,15 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // GELU, tanh form (Hendrycks & Gimpel 2016), and its derivative.
+// tanh(u) = 1 - 2 / (1 + e^(2u)) with the hardware exponential and reciprocal
+// (absolute error ~1e-7, far below the BF16 rounding of the activations).
+__device__ __forceinline__ float fast_tanh(float u) {
+  return 1.f - __fdividef(2.f, 1.f + __expf(2.f * u));
+}
 __device__ __forceinline__ void gelu_fwd(float x, float& g, float& dg) {
   const float c = 0.7978845608028654f, a = 0.044715f;
   float x2 = x * x;
-  float th = tanhf(c * (x + a * x2 * x));
+  float th = fast_tanh(c * (x + a * x2 * x));
   g = 0.5f * x * (1.f + th);
   dg = 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * c * (1.f + 3.f * a * x2);
 }
@@ -66,19 +72,25 @@ struct MlpFwdLds {
   float ys[64];                        // y per pixel
 };
 // Training adds the other operand orientations and the backward buffers.
+// Training adds the other operand orientations and the backward buffers,
+// within 80 KB so that two workgroups share a CU: dZ1^T reuses H1's space (H1
+// is dead after the forward pass), dZ2 and dZ2^T reuse H2's (dead after dW3),
+// and the GELU derivative at Z1 is recomputed in the backward pass from X.
 struct MlpLds : MlpFwdLds {
   alignas(16) uint16_t w2[64 * 64];    // W2    [in][out]
   alignas(16) uint16_t xt[32 * 64];    // X^T   [feature][pixel]
   alignas(16) uint16_t h1t[64 * 64];   // H1^T  [hidden][pixel]
-  alignas(16) uint16_t dz2[64 * 64];   // dZ2   [pixel][out]
-  alignas(16) uint16_t dz2t[64 * 64];  // dZ2^T [out][pixel]
-  alignas(16) uint16_t dz1t[64 * 64];  // dZ1^T [hidden][pixel]
-  float h2[64 * 65];                   // H2    [pixel][hidden], FP32, padded rows
+  union alignas(16) {
+    float h2[64 * 65];                 // H2    [pixel][hidden], FP32, padded rows (forward)
+    struct {
+      uint16_t dz2[64 * 64];           // dZ2   [pixel][out]   (backward)
+      uint16_t dz2t[64 * 64];          // dZ2^T [out][pixel]
+    } bw;
+  } u;
   float dy[64];
-  // GELU derivatives at Z1, Z2 in the MFMA output layout: [tile * 4 + element][lane]
-  float dg1[64 * 64];
-  float dg2[64 * 64];
+  float dg2[64 * 64];                  // GELU derivative at Z2, MFMA output layout [tile * 4 + element][lane]
 };
+static_assert(sizeof(MlpLds) <= 80 * 1024, "two training workgroups per CU");
 
 template <bool TRAIN, class L>
 __device__ __forceinline__ void load_weights(L& s, const MlpDevWeights& w) {
@@ -118,10 +130,7 @@ __device__ __forceinline__ float forward(L& s, const float* __restrict__ x, uint
       gelu_fwd(z[e] + s.b1[hd], g, dg);
       uint16_t hb = f2bf(g);
       s.h1[px * 64 + hd] = hb;
-      if constexpr (TRAIN) {
-        s.dg1[(t * 4 + e) * 64 + l] = dg;
-        s.h1t[hd * 64 + px] = hb;
-      }
+      if constexpr (TRAIN) s.h1t[hd * 64 + px] = hb;
     }
   }
   __syncthreads();
@@ -142,7 +151,7 @@ __device__ __forceinline__ float forward(L& s, const float* __restrict__ x, uint
         part[e] = fmaf(g, s.w3[hd], part[e]);
         if constexpr (TRAIN) {
           s.dg2[(t * 4 + e) * 64 + l] = dg;
-          s.h2[px * 65 + hd] = g;
+          s.u.h2[px * 65 + hd] = g;
         }
       }
     }
@@ -178,12 +187,13 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDevWeights w, const float* 
   // dW3 (lane = hidden unit), db3
   {
     float g3 = 0.f;
-    for (int px = 0; px < 64; px++) g3 = fmaf(s.h2[px * 65 + l], s.dy[px], g3);
+    for (int px = 0; px < 64; px++) g3 = fmaf(s.u.h2[px * 65 + l], s.dy[px], g3);
     P[kMlpOffW3 + l] = g3;
     float db3 = s.dy[l];
     for (int o = 32; o >= 1; o >>= 1) db3 += __shfl_xor(db3, o, 64);
     if (l == 0) P[kMlpOffB3] = db3;
   }
+  __syncthreads();                                 // H2 is dead: its space takes dZ2
   // dZ2 = dy w3^T * gelu'(Z2), both orientations; db2 = column sums
   float db2[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 4
@@ -195,8 +205,8 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDevWeights w, const float* 
       float g = s.dy[px] * s.w3[hd] * s.dg2[(t * 4 + e) * 64 + l];
       db2[t & 3] += g;
       uint16_t hb = f2bf(g);
-      s.dz2[px * 64 + hd] = hb;
-      s.dz2t[hd * 64 + px] = hb;
+      s.u.bw.dz2[px * 64 + hd] = hb;
+      s.u.bw.dz2t[hd * 64 + px] = hb;
     }
   }
 #pragma unroll
@@ -207,18 +217,23 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDevWeights w, const float* 
     if (l < 16) P[kMlpOffB2 + c * 16 + l] = v;
   }
   __syncthreads();
-  // dH1 = dZ2 W2^T -> dZ1 = dH1 * gelu'(Z1) (transposed), db1
+  // dH1 = dZ2 W2^T -> dZ1 = dH1 * gelu'(Z1) (transposed, into H1's space), db1;
+  // Z1 is recomputed by the same MFMA as in the forward pass
+  uint16_t* dz1t = s.h1;
   float db1[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll 4
   for (int t = 0; t < 16; t++) {
     const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
-    f32x4 dh = mfma_tile(s.dz2, 64, m0, s.w2, 64, n0, 64);
+    f32x4 dh = mfma_tile(s.u.bw.dz2, 64, m0, s.w2, 64, n0, 64);
+    f32x4 z = mfma_tile(s.x, 32, m0, s.w1t, 32, n0, 32);
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       const int px = m0 + rq + e, hd = n0 + col;
-      float g = dh[e] * s.dg1[(t * 4 + e) * 64 + l];
+      float g1, dg1;
+      gelu_fwd(z[e] + s.b1[hd], g1, dg1);
+      float g = dh[e] * dg1;
       db1[t & 3] += g;
-      s.dz1t[hd * 64 + px] = f2bf(g);
+      dz1t[hd * 64 + px] = f2bf(g);
     }
   }
 #pragma unroll
@@ -232,13 +247,13 @@ __global__ __launch_bounds__(64) void k_mlp_train(MlpDevWeights w, const float* 
   // dW2[in][out] = H1^T dZ2 : A = H1^T [in][px], Bt = dZ2^T [out][px]
   for (int t = 0; t < 16; t++) {
     const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
-    f32x4 g = mfma_tile(s.h1t, 64, m0, s.dz2t, 64, n0, 64);
+    f32x4 g = mfma_tile(s.h1t, 64, m0, s.u.bw.dz2t, 64, n0, 64);
 #pragma unroll
     for (int e = 0; e < 4; e++) P[kMlpOffW2 + (m0 + rq + e) * 64 + n0 + col] = g[e];
   }
   // dW1[feature][hidden] = X^T dZ1 : A = X^T [feature][px] (rows 0..15), Bt = dZ1^T [hidden][px]
   for (int c = 0; c < 4; c++) {
-    f32x4 g = mfma_tile(s.xt, 64, 0, s.dz1t, 64, c * 16, 64);
+    f32x4 g = mfma_tile(s.xt, 64, 0, dz1t, 64, c * 16, 64);
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       const int f = rq + e;
@@ -279,18 +294,42 @@ __device__ __forceinline__ void mlp_pack_one(MlpDevWeights d, uint32_t p, float 
 
 // Sum of the per-workgroup gradients (workgroup order), Adam, and the BF16
 // operand copies in the layouts the MFMA tiles read.
-__global__ __launch_bounds__(256) void k_mlp_adam(MlpState st, const float* __restrict__ part, uint32_t nwg,
-                                                  const float* __restrict__ loss_part, float lr, float beta1,
-                                                  float beta2, float eps, float bc1, float bc2, int apply) {
+// Chunk c of the per-workgroup gradients: part2[c][p] = sum over workgroups
+// c*kReduceChunk .. of part[wg][p], in workgroup order.
+constexpr uint32_t kReduceChunk = 16;
+__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ part, uint32_t nwg,
+                                                    float* __restrict__ part2) {
   const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-  if (p == 0) {
+  if (p >= (uint32_t)kMlpParams) return;
+  const uint32_t i0 = blockIdx.y * kReduceChunk, i1 = min(i0 + kReduceChunk, nwg);
+  float g = 0.f;
+  for (uint32_t i = i0; i < i1; i++) g += part[(size_t)i * kMlpParams + p];
+  part2[(size_t)blockIdx.y * kMlpParams + p] = g;
+}
+
+// The last workgroup sums the loss partials (strided per thread, then a fixed
+// tree); the others sum the chunk totals of one parameter each and step Adam.
+__global__ __launch_bounds__(256) void k_mlp_adam(MlpState st, const float* __restrict__ part2, uint32_t nch,
+                                                  const float* __restrict__ loss_part, uint32_t nwg, float lr,
+                                                  float beta1, float beta2, float eps, float bc1, float bc2,
+                                                  int apply) {
+  if (blockIdx.x == gridDim.x - 1) {
+    __shared__ float red[256];
     float L = 0.f;
-    for (uint32_t i = 0; i < nwg; i++) L += loss_part[i];
-    st.loss[0] = L;
+    for (uint32_t i = threadIdx.x; i < nwg; i += 256) L += loss_part[i];
+    red[threadIdx.x] = L;
+    __syncthreads();
+    for (int o = 128; o >= 1; o >>= 1) {
+      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) st.loss[0] = red[0];
+    return;
   }
+  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
   if (p >= (uint32_t)kMlpParams) return;
   float g = 0.f;
-  for (uint32_t i = 0; i < nwg; i++) g += part[(size_t)i * kMlpParams + p];
+  for (uint32_t c = 0; c < nch; c++) g += part2[(size_t)c * kMlpParams + p];
   st.grad[p] = g;
   float w = st.w[p];
   if (apply) {
@@ -312,10 +351,14 @@ hipError_t launch_mlp_train(const MlpState& st, const float* x, const float* t, 
   return hipGetLastError();
 }
 hipError_t launch_mlp_adam(const MlpState& st, const float* part, uint32_t nwg, const float* loss_part, float lr,
-                           int step, int apply, hipStream_t s) {
+                           int step, int apply, float* part2, hipStream_t s) {
+  if (!part || !loss_part || !part2 || nwg == 0) return hipErrorInvalidValue;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
-  k_mlp_adam<<<(kMlpParams + 255) / 256, 256, 0, s>>>(st, part, nwg, loss_part, lr, b1, b2, eps, bc1, bc2, apply);
+  const uint32_t nch = mlp_reduce_chunks(nwg);
+  const uint32_t pb = (kMlpParams + 255) / 256;
+  k_mlp_reduce<<<dim3(pb, nch), 256, 0, s>>>(part, nwg, part2);
+  k_mlp_adam<<<pb + 1, 256, 0, s>>>(st, part2, nch, loss_part, nwg, lr, b1, b2, eps, bc1, bc2, apply);
   return hipGetLastError();
 }
 hipError_t launch_mlp_infer(const MlpState& st, const float* x, uint32_t n, float* y, hipStream_t s) {
@@ -406,5 +449,6 @@ hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd,
 }
 
 size_t mlp_lds_bytes() { return sizeof(MlpLds); }
+uint32_t mlp_reduce_chunks(uint32_t nwg) { return (nwg + kReduceChunk - 1) / kReduceChunk; }
 
 }  // namespace cpt
