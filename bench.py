@@ -51,7 +51,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(sc, seconds_target=10.0, max_frames=8):
+def cpu_baseline(sc, seconds_target=12.0, max_frames=64):
     """Oracle FP32 reference estimator over whole 1280x720 frames at 1 spp each
     (frames 0, 1, ... of the same stream) until ~seconds_target of wall time."""
     import oracle
