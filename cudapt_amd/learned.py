@@ -31,7 +31,9 @@ from . import (CPT_MODE_BDPT_ADAPTIVE_PLL, CPT_MODE_BDPT_STREAM, DeviceBuffer, M
 
 POSE_SEED = 0xCA3E0000
 REF_SEED = 0x0EF5EED5
-CLAMP = 1.0          # the error is measured on radiance clamped to [0, 1] (the displayed image)
+# the error is measured on radiance clamped to [0, 1] (the displayed image), the
+# form left commented out in CUDAPTWindow::CalculateRMSE (src/cudapt.cpp:960)
+CLAMP = 1.0
 
 
 def orbit_poses(center, radius_range, height_range, target_extent, n, seed0=POSE_SEED, fov=math.radians(45.0)):
