@@ -7,7 +7,9 @@ eye depth 8, 64 frames = 64 samples per pixel through the wavefront BDPT
 before timing; the timed region is K renders of 64 frames each.
 
 metric/value: Mrays/s = (extension + shadow rays traced) / wall time.
-roofline: the closest-hit extension kernel, algorithmic bytes per launch =
+roofline: the closest-hit extension kernel (by default k_trace_dyn: persistent
+  waves that take the next 64 rays from a global queue once all their lanes are
+  done), algorithmic bytes per launch =
   48 B per ray (32 B ray in + 16 B hit out) + 128 B (FP32 layout) or 64 B
   (quantized layout) per 4-wide node visited + 48 B per triangle tested, over
   its HIP-event duration on the renderer stream.
@@ -46,6 +48,16 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 WIDTH, HEIGHT, SPP, EYE_DEPTH, LIGHT_DEPTH = 1280, 720, 64, 8, 8
 
+
+
+def extend_kernel_name(args) -> str:
+    """Template instance the renderer launches for extension rays."""
+    stk, minw = {806: (8, 6), 1606: (16, 6), 8: (8, 0)}.get(args.lds_stack, (16, 0))
+    q = "true" if args.node_format else "false"
+    if args.refill:
+        return f"k_trace_dyn<1,{stk},{minw},{q}> (closest-hit extension, refill at {args.refill} idle lanes)"
+    layout = 2 if args.node_format else 0
+    return f"k_trace<{layout},false,1,{stk},{minw}> (closest-hit extension, one thread per ray)"
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
@@ -149,6 +161,8 @@ def hbm_traffic(args):
                "--", sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
                "--no-traffic", "--no-learned", "--no-multiview", "--frames", str(args.frames), "--in-flight", str(args.in_flight),
                "--lds-stack", str(args.lds_stack), "--node-format", str(args.node_format),
+               "--refill", str(args.refill), "--refill-groups", str(args.refill_groups),
+               "--refill-shadow", str(args.refill_shadow),
                "--copy-calib", str(calib >> 20)]
         log(f"traffic pass: {counter}")
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -214,7 +228,11 @@ def main():
     ap.add_argument("--in-flight", type=int, default=64, help="frames traced as one wavefront")
     ap.add_argument("--lds-stack", type=int, default=806, help="traversal variant: LDS stack entries (8/16/24; "
                     "806/807/808/1606 = 8 or 16 entries with >= 6, 7 or 8 waves per SIMD)")
-    ap.add_argument("--node-format", type=int, default=0, help="BVH nodes: 0 = 128-byte FP32, 1 = 64-byte quantized")
+    ap.add_argument("--node-format", type=int, default=1, help="BVH nodes: 0 = 128-byte FP32, 1 = 64-byte quantized")
+    ap.add_argument("--refill", type=int, default=64,
+                    help="traversal lanes refill from a ray queue once this many of a wave's are idle (0: off)")
+    ap.add_argument("--refill-shadow", type=int, default=0, help="the same for shadow rays")
+    ap.add_argument("--refill-groups", type=int, default=8, help="workgroups per CU of the refilling grid")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
@@ -257,6 +275,9 @@ def main():
     R.set_attrib("Traversal LDS Stack", args.lds_stack)
     R.set_attrib("Shadow LDS Stack", args.lds_stack)
     R.set_attrib("Node Format", args.node_format)
+    R.set_attrib("Traversal Refill", args.refill)
+    R.set_attrib("Refill Workgroups", args.refill_groups)
+    R.set_attrib("Shadow Refill", args.refill_shadow)
     cam = sc["camera"]
     for w in range(args.warmup):
         R.render(S, cam, frames=args.frames, result=False)
@@ -299,7 +320,9 @@ def main():
                                "depth 8, 64 spp per step, light-vertex cache every 3rd frame",
                    "triangles": int(len(sc["idx"])), "width": WIDTH, "height": HEIGHT, "spp": args.frames,
                    "eye_depth": EYE_DEPTH, "light_depth": LIGHT_DEPTH, "frames_in_flight": args.in_flight,
-                   "traversal_variant": args.lds_stack, "node_format": args.node_format},
+                   "traversal_variant": args.lds_stack, "node_format": args.node_format,
+                   "traversal_refill": args.refill, "refill_groups": args.refill_groups,
+                   "shadow_refill": args.refill_shadow},
         "samples_per_s": round(st["eye_paths"] / wall, 1),
         "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},
         "kernel_ms": {"extend": round(st["extend_ms"], 3), "shade": round(st["shade_ms"], 3),
@@ -307,7 +330,7 @@ def main():
                       "all": round(st["total_ms"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "traffic_detail": None,
-                     "kernel": "k_trace<0,false,1> (closest-hit extension)",
+                     "kernel": extend_kernel_name(args),
                      "launches": launches, "avg_launch_us": round(st["extend_ms"] * 1e3 / launches, 2),
                      "bytes_per_launch": int(bytes_alg / launches),
                      "nodes_per_ray": round(st["extend_nodes"] / max(st["extension_rays"], 1), 2),

@@ -360,10 +360,10 @@ class BatchConfig(ctypes.Structure):
                                                "traversal_variant", "use_graph", "eye_depth", "light_depth",
                                                "light_paths")] + [("seed", ctypes.c_uint64),
                                                                   ("node_format", ctypes.c_uint32),
-                                                                  ("pad", ctypes.c_uint32)]
+                                                                  ("refill", ctypes.c_uint32)]
 
     def as_dict(self):
-        return {n: getattr(self, n) for n, _ in self._fields_ if n != "pad"}
+        return {n: getattr(self, n) for n, _ in self._fields_}
 
 
 class Batch:

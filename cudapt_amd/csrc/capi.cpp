@@ -449,6 +449,14 @@ const char* const kPassFrames = "Adaptive Pass Frames";
 // BVH node format of the traversal kernels: 0 = 128-byte FP32 4-wide nodes,
 // 1 = the same nodes quantized to 64 bytes
 const char* const kNodeFormat = "Node Format";
+// traversal scheduling: 0 = one ray per thread; 1..64 = a persistent grid
+// whose lanes take the next ray from a global queue when theirs is finished,
+// a wave refilling once that many of its lanes are idle
+const char* const kRefill = "Traversal Refill";
+// the same for shadow rays
+const char* const kRefillShadow = "Shadow Refill";
+// workgroups per compute unit of the refilling grid
+const char* const kRefillGroups = "Refill Workgroups";
 // allocation signal from the learned model's output y (log per-sample
 // variance): 0 = nd / (nd + N) * exp(y) / N, the filter-residual estimate's
 // shape; 1 = exp(y) / N, the expected squared error (samples settle where N
@@ -535,6 +543,7 @@ struct cpt_renderer {
   float gauss_sigma = -1.f;
   u32 gauss_radius = 0;
   u32 persist_chunk = 0, persist_grid = 0;   // persistent extend grid (0: one thread per ray)
+  int cus = 0;                               // compute units of the device
   DevArray<u32> queue;                       // its global work-queue counter
   cpt_mlp* mlp = nullptr;              // learned error model (adaptive mode)
   DevArray<float> mlp_x, mlp_y, mlp_e, sq_f, sq_e;
@@ -595,6 +604,7 @@ int renderer_alloc(cpt_renderer* r) {
     HIP_TRY(hipMemsetAsync(r->counts.p, 0, sizeof(u32) * n, r->stream));
   }
   HIP_TRY(r->rec_acc.alloc(cap));
+  HIP_TRY(r->queue.alloc(1));
   HIP_TRY(r->fc.alloc(1));
   HIP_TRY(r->stats.alloc(1));
   HIP_TRY(r->result.alloc(n * 3));
@@ -686,7 +696,15 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
     r->extend_launches++;
     HIP_TRY(timed_begin(r, 0, &ev));
     const int quant = r->attr(kNodeFormat) != 0;
-    if (r->persist_chunk)
+    const uint32_t refill = (uint32_t)std::min(64.0, std::max(0.0, r->attr(kRefill)));
+    const uint32_t refill_sh = (uint32_t)std::min(64.0, std::max(0.0, r->attr(kRefillShadow)));
+    const uint32_t dyn_grid =
+        (uint32_t)(r->cus > 0 ? r->cus : 256) * (uint32_t)std::min(32.0, std::max(1.0, r->attr(kRefillGroups)));
+    if (refill)
+      HIP_TRY(launch_trace_dyn(sc, cur->view().ray_o, cur->view().ray_d, r->fc.p->active + b, r->hits.p, nullptr,
+                               PathBuf{}, r->stats.p, (int)r->attr(kLdsStack), quant, 0, dyn_grid, refill, r->queue.p,
+                               r->stream));
+    else if (r->persist_chunk)
       HIP_TRY(launch_extend_persist(sc, cur->view(), r->fc.p, (int)b, r->hits.p, r->stats.p, (int)r->attr(kLdsStack),
                                     quant, r->persist_chunk, r->persist_grid, r->queue.p, r->stream));
     else
@@ -699,8 +717,13 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
     HIP_TRY(timed_end(r, 2, ev));
     if (b + 1 < fa.eye_depth) {
       HIP_TRY(timed_begin(r, 1, &ev));
-      HIP_TRY(launch_shadow(sc, sb, nxt->view(), r->fc.p, (int)b, max_n, (int)r->attr(kLdsStackShadow), quant,
-                            r->stream));
+      if (refill_sh)
+        HIP_TRY(launch_trace_dyn(sc, sb.ray_o, sb.ray_d, r->fc.p->shadow + b, nullptr, sb.contrib, nxt->view(),
+                                 nullptr, (int)r->attr(kLdsStackShadow), quant, 1, dyn_grid, refill_sh, r->queue.p,
+                                 r->stream));
+      else
+        HIP_TRY(launch_shadow(sc, sb, nxt->view(), r->fc.p, (int)b, max_n, (int)r->attr(kLdsStackShadow), quant,
+                              r->stream));
       HIP_TRY(timed_end(r, 1, ev));
     }
     std::swap(cur, nxt);
@@ -1021,9 +1044,15 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   // (swept on the sphere grid: 1.58 vs 1.36 Grays/s for 16 entries)
   r->attribs.push_back({kLdsStack, 0, 806});
   r->attribs.push_back({kLdsStackShadow, 0, 806});
-  r->attribs.push_back({kNodeFormat, 0, 0});
+  r->attribs.push_back({kNodeFormat, 0, 1});
+  r->attribs.push_back({kRefill, 0, 64});
+  r->attribs.push_back({kRefillShadow, 0, 0});
+  r->attribs.push_back({kRefillGroups, 0, 8});
   int rc = ensure_stream(r->stream);
   if (rc) { delete r; return rc; }
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&r->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+    r->cus = 256;
   *out = r;
   return CPT_OK;
 }
@@ -1373,8 +1402,13 @@ struct cpt_batch {
 namespace {
 
 constexpr u32 kMaxViews = 4096;
-const u32 kChunks[] = {0u, 256u, 1024u, 4096u};
-const u32 kWgsPerCu[] = {4u, 8u, 16u};
+// extension-ray schedules the autotuner tries, each with every traversal
+// variant: {persistent chunk, refill threshold, workgroups per CU}
+struct Schedule {
+  u32 chunk, refill, wgs;
+};
+const Schedule kSchedules[] = {{0, 0, 8},     {256, 0, 8},  {1024, 0, 8}, {4096, 0, 8}, {1024, 0, 4},
+                               {1024, 0, 16}, {0, 64, 6},   {0, 64, 8},   {0, 48, 8},   {0, 32, 8}};
 const u32 kVariants[] = {806u, 1606u};
 
 cpt_batch_config default_batch_config() {
@@ -1383,6 +1417,8 @@ cpt_batch_config default_batch_config() {
   c.frames_in_flight = 24;
   c.extend_chunk = 0;
   c.extend_workgroups = 8;
+  c.refill = 64;
+  c.node_format = 1;
   c.traversal_variant = 806;
   c.use_graph = 1;
   c.eye_depth = 5;
@@ -1395,8 +1431,9 @@ cpt_batch_config default_batch_config() {
 int batch_check_config(const cpt_batch_config& c) {
   if (c.frames_in_flight < 1 || c.frames_in_flight > kMaxInFlight)
     return fail(CPT_E_LIMIT, "batch: frames in flight %u not in 1..%u", c.frames_in_flight, kMaxInFlight);
-  if (c.extend_chunk && (c.extend_workgroups < 1 || c.extend_workgroups > 64))
-    return fail(CPT_E_LIMIT, "batch: extend workgroups per CU %u not in 1..64", c.extend_workgroups);
+  if ((c.extend_chunk || c.refill) && (c.extend_workgroups < 1 || c.extend_workgroups > 32))
+    return fail(CPT_E_LIMIT, "batch: extend workgroups per CU %u not in 1..32", c.extend_workgroups);
+  if (c.refill > 64) return fail(CPT_E_LIMIT, "batch: refill threshold %u not in 0..64", c.refill);
   if (c.eye_depth < 1 || c.eye_depth > (u32)kMaxDepth) return fail(CPT_E_LIMIT, "batch: eye depth not in 1..15");
   if (c.light_depth < 1 || c.light_depth > 64) return fail(CPT_E_LIMIT, "batch: light depth not in 1..64");
   if (c.light_paths < 1 || c.light_paths > (u32)kBlock) return fail(CPT_E_LIMIT, "batch: light paths not in 1..256");
@@ -1413,8 +1450,10 @@ void batch_apply_config(cpt_batch* b) {
   r->set_attr(kInFlight, b->cfg.frames_in_flight);
   r->set_attr(kLdsStack, b->cfg.traversal_variant);
   r->set_attr(kNodeFormat, b->cfg.node_format ? 1 : 0);
-  r->persist_chunk = b->cfg.extend_chunk;
-  r->persist_grid = b->cfg.extend_chunk ? b->cfg.extend_workgroups * (u32)(b->cus > 0 ? b->cus : 256) : 0u;
+  r->set_attr(kRefill, b->cfg.refill);
+  r->set_attr(kRefillGroups, b->cfg.extend_workgroups);
+  r->persist_chunk = b->cfg.refill ? 0u : b->cfg.extend_chunk;
+  r->persist_grid = r->persist_chunk ? b->cfg.extend_workgroups * (u32)(b->cus > 0 ? b->cus : 256) : 0u;
   b->drop_graph();
   b->started = false;
 }
@@ -1544,20 +1583,20 @@ int batch_render(cpt_batch* b, cpt_scene* s, u32 frames) {
 
 int batch_candidate(const cpt_batch* b, u32 i, cpt_batch_config* out) {
   u32 idx = 0;
-  for (u32 ch : kChunks)
-    for (u32 w = 0; w < (ch ? 3u : 1u); w++)
-      for (u32 v : kVariants) {
-        if (idx++ == i) {
-          *out = b->cfg;
-          out->extend_chunk = ch;
-          out->extend_workgroups = ch ? kWgsPerCu[w] : b->cfg.extend_workgroups;
-          out->traversal_variant = v;
-          return CPT_OK;
-        }
+  for (const Schedule& sc : kSchedules)
+    for (u32 v : kVariants) {
+      if (idx++ == i) {
+        *out = b->cfg;
+        out->extend_chunk = sc.chunk;
+        out->refill = sc.refill;
+        out->extend_workgroups = sc.wgs;
+        out->traversal_variant = v;
+        return CPT_OK;
       }
+    }
   return fail(CPT_E_ARG, "cpt_batch_candidate: index %u out of range", i);
 }
-u32 batch_candidate_count() { return (u32)(2 * (1 + 3 * 3)); }
+u32 batch_candidate_count() { return (u32)(sizeof(kSchedules) / sizeof(kSchedules[0]) * 2); }
 
 }  // namespace
 

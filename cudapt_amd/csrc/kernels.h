@@ -211,6 +211,13 @@ hipError_t launch_scan_counts(const uint32_t* counts, uint32_t n, uint32_t* offs
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
                          uint32_t max_n, int lds_stack, int quant, hipStream_t st);
 hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes, uint32_t depth, hipStream_t st);
+// Closest hit (shadow = 0: hits into out) or connection visibility (shadow = 1:
+// contributions into nxt) of the `*count` rays by a persistent grid whose
+// lanes refill themselves with the next ray from the global counter `queue`.
+hipError_t launch_trace_dyn(const DevScene& sc, const float4* ro, const float4* rd, const uint32_t* count,
+                            float4* out, const float4* contrib, PathBuf nxt, DevStats* stats, int lds_stack,
+                            int quant, int shadow, uint32_t grid, uint32_t refill, uint32_t* queue,
+                            hipStream_t st);
 // *frame_dev += k (the device frame counter of replayed graphs).
 hipError_t launch_frame_advance(uint32_t* frame_dev, uint32_t k, hipStream_t st);
 // Closest hit of the bounce's live paths by a persistent grid: each workgroup

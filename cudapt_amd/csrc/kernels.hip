@@ -212,6 +212,75 @@ __device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps,
   return h.leaf != kInvalid;
 }
 
+// ---- traversal as a resumable step, for lanes that refill themselves -------
+// The same per-node operations as traverse4 in the same order (identical hits),
+// split so that a lane whose ray has finished can start another while the rest
+// of its wave is still traversing.
+struct RayState {
+  f3 o, d, idir, o2;
+  float eps;
+  TraceHit h;
+  uint32_t node;
+  int sp;
+};
+__device__ __forceinline__ void ray_begin(RayState& R, f3 o, f3 d, float maxd, float eps) {
+  R.o = o;
+  R.d = d;
+  R.idir = mk3(1.f / escape_zero(d.x), 1.f / escape_zero(d.y), 1.f / escape_zero(d.z));
+  R.o2 = o + d * eps;
+  R.eps = eps;
+  R.h.t = maxd; R.h.u = R.h.v = 0.f; R.h.leaf = kInvalid; R.h.orig = kInvalid;
+  R.node = 0;
+  R.sp = 0;
+}
+// One node of the traversal; true when the ray is finished.
+template <bool ANY, int STK, bool Q>
+__device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, uint64_t* lstk, uint64_t* pstk,
+                                         uint32_t& nn, uint32_t& nt) {
+  NodeView nv;
+  load_node<Q>(sc, R.node, nv);
+  nn++;
+  uint64_t key[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    float t0x = (nv.lx[c] - R.o.x) * R.idir.x, t1x = (nv.hx[c] - R.o.x) * R.idir.x;
+    float t0y = (nv.ly[c] - R.o.y) * R.idir.y, t1y = (nv.hy[c] - R.o.y) * R.idir.y;
+    float t0z = (nv.lz[c] - R.o.z) * R.idir.z, t1z = (nv.hz[c] - R.o.z) * R.idir.z;
+    float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
+    float tfar = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * 1.0000004f;
+    bool hit = nv.ch[c] != kInvalid && tnear <= tfar && tnear <= R.h.t + cull_slack(R.h.t, R.eps);
+    key[c] = ~0ull;
+    if (hit) {
+      if (nv.cn[c] > 0) {
+        test_leaf(sc.tris, nv.ch[c], nv.cn[c], R.o2, R.d, false, R.h, nt);
+        if (ANY && R.h.leaf != kInvalid) return true;
+      } else {
+        key[c] = ((uint64_t)__float_as_uint(tnear) << 32) | nv.ch[c];
+      }
+    }
+  }
+  cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[1], key[2]);
+  if (key[0] != ~0ull) {
+    R.node = (uint32_t)key[0];
+#pragma unroll
+    for (int c = 3; c >= 1; c--) {
+      if (key[c] != ~0ull && R.sp < CPT_STACK_TOTAL) {
+        if (R.sp < STK) lstk[R.sp * kBlock] = key[c];
+        else pstk[R.sp - STK] = key[c];
+        R.sp++;
+      }
+    }
+    return false;
+  }
+  while (R.sp > 0) {
+    R.sp--;
+    uint64_t e = R.sp < STK ? lstk[R.sp * kBlock] : pstk[R.sp - STK];
+    float tn = __uint_as_float((uint32_t)(e >> 32));
+    if (tn <= R.h.t + cull_slack(R.h.t, R.eps)) { R.node = (uint32_t)e; return false; }
+  }
+  return true;
+}
+
 // Same query over the reference's binary layout (right child popped first).
 template <bool ANY>
 __device__ bool traverse_bin(const DevScene& sc, f3 o, f3 d, float maxd, float eps, bool cull, TraceHit& h,
@@ -840,6 +909,117 @@ hipError_t launch_extend_persist(const DevScene& sc, PathBuf pb, const FrameCoun
     default: if (quant) CPT_EXTP(16, 0, true); else CPT_EXTP(16, 0, false); break;
   }
 #undef CPT_EXTP
+  return hipGetLastError();
+}
+
+// Dynamic refill: a persistent grid whose lanes each trace one ray at a time
+// and take the next as soon as theirs is finished.  A wave hands out ray
+// indices from a chunk of kDynChunk it took from the global counter `queue`
+// (one atomic per chunk); lanes refill once at least `refill` (1..64) are idle.
+// OUT 1: closest hit (t, leaf, u, v) into out[ray]; OUT 3: the connection's
+// visibility (k_shadow's epilogue).
+constexpr uint32_t kDynChunk = 256;
+template <int OUT, int STK, int MINW, bool Q>
+__global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const float4* __restrict__ ro,
+                                                            const float4* __restrict__ rd,
+                                                            const uint32_t* __restrict__ countp,
+                                                            uint32_t* __restrict__ queue, float4* __restrict__ out,
+                                                            const float4* __restrict__ contrib, PathBuf nxt,
+                                                            DevStats* __restrict__ stats, uint32_t refill) {
+  constexpr bool ANY = OUT == 3;
+  __shared__ uint64_t lstk_all[STK * kBlock];
+  uint64_t* lstk = lstk_all + threadIdx.x;
+  uint64_t pstk[CPT_STACK_TOTAL - STK];
+  const uint32_t count = *countp;
+  const int lane = threadIdx.x & 63;
+  const uint64_t below = (1ull << lane) - 1ull;
+  uint32_t cnext = 0, cend = 0;            // the wave's chunk (uniform across the wave)
+  bool exhausted = false;                  // uniform
+  bool active = false;
+  uint32_t ray = 0;
+  RayState R;
+  uint32_t nn = 0, nt = 0;
+  for (;;) {
+    const uint64_t idle = __ballot(!active);
+    const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (!exhausted && n_idle >= refill) {
+      const uint32_t rank = (uint32_t)__popcll(idle & below);
+      uint32_t given = 0;
+      while (given < n_idle) {
+        if (cnext >= cend) {
+          uint32_t b = 0;
+          if (lane == 0) b = atomicAdd(queue, kDynChunk);
+          b = __shfl(b, 0, 64);
+          if (b >= count) { exhausted = true; break; }
+          cnext = b;
+          cend = min(b + kDynChunk, count);
+        }
+        const uint32_t take = min(n_idle - given, cend - cnext);
+        if (!active && rank >= given && rank < given + take) {
+          ray = cnext + (rank - given);
+          const float4 a = ro[ray], b4 = rd[ray];
+          ray_begin(R, ld3(a), ld3(b4), a.w, b4.w);
+          active = true;
+        }
+        cnext += take;
+        given += take;
+      }
+    }
+    if (__ballot(active) == 0ull) break;   // the queue is drained and every lane idle
+    if (active && ray_step<ANY, STK, Q>(sc, R, lstk, pstk, nn, nt)) {
+      active = false;
+      const bool hit = R.h.leaf != kInvalid;
+      if (OUT == 1) {
+        float4 r;
+        r.x = hit ? R.h.t : kInfF;
+        r.y = __uint_as_float(hit ? R.h.leaf : kInvalid);
+        r.z = R.h.u;
+        r.w = R.h.v;
+        out[ray] = r;
+      } else if (!hit) {
+        const float4 c = contrib[ray];
+        const uint32_t slot = __float_as_uint(c.w);
+        float4 acc = nxt.acc[slot];
+        acc.x = acc.x + c.x;
+        acc.y = acc.y + c.y;
+        acc.z = acc.z + c.z;
+        acc.w = __uint_as_float(__float_as_uint(acc.w) + 4u);
+        nxt.acc[slot] = acc;
+      }
+    }
+  }
+  if (stats && OUT == 1) {
+    uint32_t sn = wave_sum(nn), st = wave_sum(nt);
+    if (lane == 0 && (sn | st)) {
+      atomicAdd(&stats->ext_nodes, (unsigned long long)sn);
+      atomicAdd(&stats->ext_tris, (unsigned long long)st);
+    }
+  }
+}
+hipError_t launch_trace_dyn(const DevScene& sc, const float4* ro, const float4* rd, const uint32_t* count,
+                            float4* out, const float4* contrib, PathBuf nxt, DevStats* stats, int lds_stack,
+                            int quant, int shadow, uint32_t grid, uint32_t refill, uint32_t* queue,
+                            hipStream_t st) {
+  if (!ro || !rd || !count || !queue || grid == 0 || refill < 1 || refill > 64 || !sc.tris ||
+      !(quant ? (const void*)sc.nodes4q : (const void*)sc.nodes4))
+    return hipErrorInvalidValue;
+  if (shadow ? (!contrib || !pathbuf_ok(nxt)) : !out) return hipErrorInvalidValue;
+  k_fill_u32<<<1, 1, 0, st>>>(queue, 0u);
+#define CPT_DYN(O, S, W, Qb) k_trace_dyn<O, S, W, Qb><<<grid, kBlock, 0, st>>>(sc, ro, rd, count, queue, out, contrib, nxt, stats, refill)
+#define CPT_DYN_V(O, Qb)                            \
+  switch (lds_stack) {                              \
+    case 8: CPT_DYN(O, 8, 0, Qb); break;            \
+    case 806: CPT_DYN(O, 8, 6, Qb); break;          \
+    case 1606: CPT_DYN(O, 16, 6, Qb); break;        \
+    default: CPT_DYN(O, 16, 0, Qb); break;          \
+  }
+  if (shadow) {
+    if (quant) { CPT_DYN_V(3, true) } else { CPT_DYN_V(3, false) }
+  } else {
+    if (quant) { CPT_DYN_V(1, true) } else { CPT_DYN_V(1, false) }
+  }
+#undef CPT_DYN_V
+#undef CPT_DYN
   return hipGetLastError();
 }
 

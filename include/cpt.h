@@ -235,17 +235,19 @@ typedef struct cpt_view {
 typedef struct cpt_batch_config {
   uint32_t frames_in_flight;  /* frames of every view per iteration, 1..90 */
   uint32_t extend_chunk;      /* 0: one thread per ray; else rays a workgroup takes from the global queue */
-  uint32_t extend_workgroups; /* persistent grid: workgroups per compute unit (extend_chunk > 0) */
+  uint32_t extend_workgroups; /* persistent grids: workgroups per compute unit (extend_chunk or refill > 0) */
   uint32_t traversal_variant; /* LDS stack variant: 806 (8 entries, >= 6 waves/SIMD), 1606, 8, 16 */
   uint32_t use_graph;         /* 1: capture one iteration as a hipGraph and replay it */
   uint32_t eye_depth, light_depth, light_paths;
   uint64_t seed;
   uint32_t node_format;       /* 0: 128-byte FP32 4-wide nodes, 1: 64-byte quantized nodes */
-  uint32_t pad;
+  uint32_t refill;            /* extension rays: 0 = as extend_chunk says; 1..64 = persistent waves whose
+                                 lanes take new rays once this many are idle (overrides extend_chunk) */
 } cpt_batch_config;
 
-/* Defaults: 24 frames in flight, chunk 0, variant 806, graph on, depths 5/5,
- * 128 light paths, seed 0x5EED0001 (the plain renderer's). */
+/* Defaults: 24 frames in flight, refill 64 with 8 workgroups per CU, variant
+ * 806, quantized nodes, graph on, depths 5/5, 128 light paths, seed 0x5EED0001
+ * (the plain renderer's). */
 int cpt_batch_create(cpt_batch** out, const cpt_view* views, uint32_t n_views);
 int cpt_batch_get_config(cpt_batch* b, cpt_batch_config* cfg);
 /* Restarts accumulation. */

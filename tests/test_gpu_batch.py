@@ -37,10 +37,13 @@ def setup(cpt):
 
 
 @pytest.mark.parametrize("cfg", [dict(frames_in_flight=4), dict(frames_in_flight=6, use_graph=0),
-                                 dict(frames_in_flight=3, extend_chunk=256, extend_workgroups=4),
+                                 dict(frames_in_flight=3, extend_chunk=256, extend_workgroups=4, refill=0),
                                  dict(frames_in_flight=11, extend_chunk=1024, extend_workgroups=8,
-                                      traversal_variant=1606)],
-                         ids=["graph-k4", "eager-k6", "persistent-k3", "persistent-k11"])
+                                      traversal_variant=1606, refill=0),
+                                 dict(frames_in_flight=5, refill=16, extend_workgroups=2),
+                                 dict(frames_in_flight=4, refill=0, node_format=0, use_graph=0)],
+                         ids=["graph-k4", "eager-k6", "persistent-k3", "persistent-k11", "refill16-k5",
+                              "per-thread-fp32-k4"])
 def test_views_equal_one_view_renderers(cpt, setup, cfg):
     B = cpt.Batch(setup["views"], **cfg)
     B.render(setup["S"], 7)
@@ -82,5 +85,7 @@ def test_batch_argument_errors(cpt, setup):
     B = cpt.Batch(setup["views"])
     with pytest.raises(cpt.CptError):
         B.set_config(frames_in_flight=0)
+    with pytest.raises(cpt.CptError):
+        B.set_config(refill=65)
     with pytest.raises(cpt.CptError):
         B.result(0)                              # nothing rendered yet

@@ -140,3 +140,23 @@ def test_quantized_node_format_same_image(cpt, cornell_runs, variant):
     b.set_attrib("Shadow LDS Stack", variant)
     rb = b.render(S, sc["camera"], frames=6)
     assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
+
+
+@pytest.mark.parametrize("variant,node_format,refill", [(806, 0, 8), (806, 1, 1), (1606, 0, 64), (16, 1, 16)])
+def test_refill_traversal_same_image(cpt, cornell_runs, variant, node_format, refill):
+    """Lanes that refill themselves from the ray queue trace the same rays with
+    the same per-node operations: frames bit-identical to one ray per thread."""
+    sc, S = cornell_runs["sc"], cornell_runs["S"]
+    a, b = cpt.Renderer(64, 48), cpt.Renderer(64, 48)
+    for R in (a, b):
+        R.set_attrib("Node Format", node_format)
+        R.set_attrib("Traversal LDS Stack", variant)
+        R.set_attrib("Shadow LDS Stack", variant)
+    b.set_attrib("Traversal Refill", refill)
+    b.set_attrib("Shadow Refill", 65 - refill)
+    ra = a.render(S, sc["camera"], frames=6)
+    rb = b.render(S, sc["camera"], frames=6)
+    assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
+    assert np.array_equal(a.sample_counts(), b.sample_counts())
+    sa, sb = a.stats(), b.stats()
+    assert sa["extension_rays"] == sb["extension_rays"] and sa["extend_nodes"] == sb["extend_nodes"]
