@@ -50,13 +50,17 @@ WIDTH, HEIGHT, SPP, EYE_DEPTH, LIGHT_DEPTH = 1280, 720, 64, 8, 8
 
 
 
-def extend_kernel_name(args) -> str:
-    """Template instance the renderer launches for extension rays."""
+def extend_kernel_name(args, demangled=False) -> str:
+    """Template instance the renderer launches for extension rays (demangled:
+    the prefix of the name the profiler reports)."""
     stk, minw = {806: (8, 6), 1606: (16, 6), 8: (8, 0)}.get(args.lds_stack, (16, 0))
     q = "true" if args.node_format else "false"
+    layout = 2 if args.node_format else 0
+    if demangled:
+        return (f"k_trace_dyn<1, {stk}, {minw}, {q}>" if args.refill
+                else f"k_trace<{layout}, false, 1, {stk}, {minw}>")
     if args.refill:
         return f"k_trace_dyn<1,{stk},{minw},{q}> (closest-hit extension, refill at {args.refill} idle lanes)"
-    layout = 2 if args.node_format else 0
     return f"k_trace<{layout},false,1,{stk},{minw}> (closest-hit extension, one thread per ray)"
 
 def log(*a):
@@ -84,7 +88,6 @@ def cpu_baseline(sc, seconds_target=12.0, max_frames=64):
                       f"{platform.processor() or platform.machine()})"}
 
 
-EXT_KERNEL = "k_trace<0, false, 1"
 MLP_FLOP_PER_SAMPLE = 3 * 2 * (9 * 64 + 64 * 64 + 64)
 BF16_PEAK_TFLOPS = 2500.0
 
@@ -154,6 +157,7 @@ def hbm_traffic(args):
     if not os.path.exists(prof):
         return None
     calib = 256 << 20
+    ext_name = extend_kernel_name(args, demangled=True)
     out = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
         d = tempfile.mkdtemp(prefix="cpt_pmc_")
@@ -173,7 +177,7 @@ def hbm_traffic(args):
         ext, copy = [], []
         for row in csv.DictReader(open(files[0])):
             v = float(row["Counter_Value"]) * 1024.0
-            if EXT_KERNEL in row["Kernel_Name"]:
+            if ext_name in row["Kernel_Name"]:
                 ext.append(v)
             elif "copyBuffer" in row["Kernel_Name"]:
                 copy.append(v)
@@ -334,7 +338,9 @@ def main():
                      "launches": launches, "avg_launch_us": round(st["extend_ms"] * 1e3 / launches, 2),
                      "bytes_per_launch": int(bytes_alg / launches),
                      "nodes_per_ray": round(st["extend_nodes"] / max(st["extension_rays"], 1), 2),
-                     "tris_per_ray": round(st["extend_tris"] / max(st["extension_rays"], 1), 2)},
+                     "tris_per_ray": round(st["extend_tris"] / max(st["extension_rays"], 1), 2),
+                     "shadow_nodes_per_ray": round(st["shadow_nodes"] / max(st["shadow_rays"], 1), 2),
+                     "shadow_tris_per_ray": round(st["shadow_tris"] / max(st["shadow_rays"], 1), 2)},
         "time_to_equal_mse": None,
     }
     del R

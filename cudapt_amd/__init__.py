@@ -38,7 +38,8 @@ class _Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("extension_rays", "shadow_rays", "eye_paths", "frames",
                                               "extend_launches", "extend_nodes", "extend_tris")] + \
                [(n, ctypes.c_double) for n in ("extend_ms", "shadow_ms", "shade_ms", "light_ms", "total_ms",
-                                               "filter_ms", "error_ms", "gen_ms", "accum_ms", "mlp_ms")]
+                                               "filter_ms", "error_ms", "gen_ms", "accum_ms", "mlp_ms")] + \
+               [(n, ctypes.c_uint64) for n in ("shadow_nodes", "shadow_tris")]
 
 
 def lib():

@@ -692,12 +692,12 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
   PathSet* nxt = &r->B;
   ShadowBuf sb;
   sb.ray_o = r->sh_o.p; sb.ray_d = r->sh_d.p; sb.contrib = r->sh_c.p;
+  const uint32_t refill_sh = (uint32_t)std::min(64.0, std::max(0.0, r->attr(kRefillShadow)));
   for (u32 b = 0; b < fa.eye_depth; b++) {
     r->extend_launches++;
     HIP_TRY(timed_begin(r, 0, &ev));
     const int quant = r->attr(kNodeFormat) != 0;
     const uint32_t refill = (uint32_t)std::min(64.0, std::max(0.0, r->attr(kRefill)));
-    const uint32_t refill_sh = (uint32_t)std::min(64.0, std::max(0.0, r->attr(kRefillShadow)));
     const uint32_t dyn_grid =
         (uint32_t)(r->cus > 0 ? r->cus : 256) * (uint32_t)std::min(32.0, std::max(1.0, r->attr(kRefillGroups)));
     if (refill)
@@ -719,11 +719,11 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
       HIP_TRY(timed_begin(r, 1, &ev));
       if (refill_sh)
         HIP_TRY(launch_trace_dyn(sc, sb.ray_o, sb.ray_d, r->fc.p->shadow + b, nullptr, sb.contrib, nxt->view(),
-                                 nullptr, (int)r->attr(kLdsStackShadow), quant, 1, dyn_grid, refill_sh, r->queue.p,
+                                 r->stats.p, (int)r->attr(kLdsStackShadow), quant, 1, dyn_grid, refill_sh, r->queue.p,
                                  r->stream));
       else
         HIP_TRY(launch_shadow(sc, sb, nxt->view(), r->fc.p, (int)b, max_n, (int)r->attr(kLdsStackShadow), quant,
-                              r->stream));
+                              r->stats.p, r->stream));
       HIP_TRY(timed_end(r, 1, ev));
     }
     std::swap(cur, nxt);
@@ -1169,6 +1169,8 @@ int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset) {
   out->frames = h.frames;
   out->extend_nodes = h.ext_nodes;
   out->extend_tris = h.ext_tris;
+  out->shadow_nodes = h.sh_nodes;
+  out->shadow_tris = h.sh_tris;
   out->extend_launches = r->extend_launches;
   out->extend_ms = r->ms[0];
   out->shadow_ms = r->ms[1];

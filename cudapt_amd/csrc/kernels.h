@@ -116,7 +116,7 @@ struct FrameCounters {
 struct DevStats {
   unsigned long long ext_rays, shadow_rays, eye_paths, frames;
   unsigned long long ext_nodes, ext_tris;
-  unsigned long long pad[2];
+  unsigned long long sh_nodes, sh_tris;
 };
 
 constexpr int kMaxDepth = 15;
@@ -209,7 +209,7 @@ hipError_t launch_round_fill(const RoundState* rs, uint32_t n, uint32_t* counts,
 hipError_t launch_scan_counts(const uint32_t* counts, uint32_t n, uint32_t* offsets, uint32_t* block_tot,
                               uint32_t* total, hipStream_t st);
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
-                         uint32_t max_n, int lds_stack, int quant, hipStream_t st);
+                         uint32_t max_n, int lds_stack, int quant, DevStats* stats, hipStream_t st);
 hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes, uint32_t depth, hipStream_t st);
 // Closest hit (shadow = 0: hits into out) or connection visibility (shadow = 1:
 // contributions into nxt) of the `*count` rays by a persistent grid whose

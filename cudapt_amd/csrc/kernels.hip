@@ -916,6 +916,10 @@ hipError_t launch_extend_persist(const DevScene& sc, PathBuf pb, const FrameCoun
 // and take the next as soon as theirs is finished.  A wave hands out ray
 // indices from a chunk of kDynChunk it took from the global counter `queue`
 // (one atomic per chunk); lanes refill once at least `refill` (1..64) are idle.
+// A finished lane writes its result at the wave's next refill point, together
+// with the other finished lanes, not in the traversal loop: the shadow
+// epilogue's read-modify-write would otherwise stall the whole wave on its load
+// every time one lane finishes.
 // OUT 1: closest hit (t, leaf, u, v) into out[ray]; OUT 3: the connection's
 // visibility (k_shadow's epilogue).
 constexpr uint32_t kDynChunk = 256;
@@ -936,12 +940,34 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
   uint32_t cnext = 0, cend = 0;            // the wave's chunk (uniform across the wave)
   bool exhausted = false;                  // uniform
   bool active = false;
+  bool pending = false;                    // finished, result not yet written
   uint32_t ray = 0;
   RayState R;
   uint32_t nn = 0, nt = 0;
   for (;;) {
     const uint64_t idle = __ballot(!active);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
+    if (n_idle >= refill && pending) {
+      pending = false;
+      const bool hit = R.h.leaf != kInvalid;
+      if (OUT == 1) {
+        float4 r;
+        r.x = hit ? R.h.t : kInfF;
+        r.y = __uint_as_float(hit ? R.h.leaf : kInvalid);
+        r.z = R.h.u;
+        r.w = R.h.v;
+        out[ray] = r;
+      } else if (!hit) {
+        const float4 c = contrib[ray];
+        const uint32_t slot = __float_as_uint(c.w);
+        float4 acc = nxt.acc[slot];
+        acc.x = acc.x + c.x;
+        acc.y = acc.y + c.y;
+        acc.z = acc.z + c.z;
+        acc.w = __uint_as_float(__float_as_uint(acc.w) + 4u);
+        nxt.acc[slot] = acc;
+      }
+    }
     if (!exhausted && n_idle >= refill) {
       const uint32_t rank = (uint32_t)__popcll(idle & below);
       uint32_t given = 0;
@@ -965,34 +991,18 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
         given += take;
       }
     }
-    if (__ballot(active) == 0ull) break;   // the queue is drained and every lane idle
+    // every lane idle with nothing left to take: the results were written above
+    if (__ballot(active) == 0ull) break;
     if (active && ray_step<ANY, STK, Q>(sc, R, lstk, pstk, nn, nt)) {
       active = false;
-      const bool hit = R.h.leaf != kInvalid;
-      if (OUT == 1) {
-        float4 r;
-        r.x = hit ? R.h.t : kInfF;
-        r.y = __uint_as_float(hit ? R.h.leaf : kInvalid);
-        r.z = R.h.u;
-        r.w = R.h.v;
-        out[ray] = r;
-      } else if (!hit) {
-        const float4 c = contrib[ray];
-        const uint32_t slot = __float_as_uint(c.w);
-        float4 acc = nxt.acc[slot];
-        acc.x = acc.x + c.x;
-        acc.y = acc.y + c.y;
-        acc.z = acc.z + c.z;
-        acc.w = __uint_as_float(__float_as_uint(acc.w) + 4u);
-        nxt.acc[slot] = acc;
-      }
+      pending = true;
     }
   }
-  if (stats && OUT == 1) {
+  if (stats) {
     uint32_t sn = wave_sum(nn), st = wave_sum(nt);
     if (lane == 0 && (sn | st)) {
-      atomicAdd(&stats->ext_nodes, (unsigned long long)sn);
-      atomicAdd(&stats->ext_tris, (unsigned long long)st);
+      atomicAdd(OUT == 1 ? &stats->ext_nodes : &stats->sh_nodes, (unsigned long long)sn);
+      atomicAdd(OUT == 1 ? &stats->ext_tris : &stats->sh_tris, (unsigned long long)st);
     }
   }
 }
@@ -1214,32 +1224,41 @@ hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, Pa
 
 template <int STK, int MINW = 0, bool Q = false>
 __global__ __launch_bounds__(kBlock, MINW) void k_shadow(DevScene sc, ShadowBuf sb, PathBuf nxt,
-                                                   const FrameCounters* __restrict__ fc, int bounce) {
+                                                   const FrameCounters* __restrict__ fc, int bounce,
+                                                   DevStats* __restrict__ stats) {
   __shared__ uint64_t lstk[STK * kBlock];
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const uint32_t count = fc->shadow[bounce];
-  if (i >= count) return;
-  float4 a = sb.ray_o[i], b = sb.ray_d[i], c = sb.contrib[i];
-  TraceHit h;
   uint32_t nn = 0, nt = 0;
-  if (!traverse4<true, STK, Q>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt)) {
-    uint32_t slot = __float_as_uint(c.w);
-    float4 acc = nxt.acc[slot];
-    acc.x = acc.x + c.x;
-    acc.y = acc.y + c.y;
-    acc.z = acc.z + c.z;
-    acc.w = __uint_as_float(__float_as_uint(acc.w) + 4u);
-    nxt.acc[slot] = acc;
+  if (i < count) {
+    float4 a = sb.ray_o[i], b = sb.ray_d[i], c = sb.contrib[i];
+    TraceHit h;
+    if (!traverse4<true, STK, Q>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt)) {
+      uint32_t slot = __float_as_uint(c.w);
+      float4 acc = nxt.acc[slot];
+      acc.x = acc.x + c.x;
+      acc.y = acc.y + c.y;
+      acc.z = acc.z + c.z;
+      acc.w = __uint_as_float(__float_as_uint(acc.w) + 4u);
+      nxt.acc[slot] = acc;
+    }
+  }
+  if (stats) {
+    uint32_t sn = wave_sum(nn), st = wave_sum(nt);
+    if ((threadIdx.x & 63) == 0 && (sn | st)) {
+      atomicAdd(&stats->sh_nodes, (unsigned long long)sn);
+      atomicAdd(&stats->sh_tris, (unsigned long long)st);
+    }
   }
 }
 hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const FrameCounters* fc, int bounce,
-                         uint32_t max_n, int lds_stack, int quant, hipStream_t st) {
+                         uint32_t max_n, int lds_stack, int quant, DevStats* stats, hipStream_t st) {
   if (max_n == 0) return hipSuccess;
   if (!pathbuf_ok(nxt) || !sb.ray_o || !sb.ray_d || !sb.contrib || !fc || !sc.tris ||
       !(quant ? (const void*)sc.nodes4q : (const void*)sc.nodes4))
     return hipErrorInvalidValue;
   dim3 grid((max_n + kBlock - 1) / kBlock), block(kBlock);
-#define CPT_SH(S, W, Q) k_shadow<S, W, Q><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce)
+#define CPT_SH(S, W, Q) k_shadow<S, W, Q><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce, stats)
   if (quant) {
     switch (lds_stack) {
       case 8: CPT_SH(8, 0, true); break;

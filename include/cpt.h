@@ -208,6 +208,7 @@ typedef struct cpt_stats {
   double extend_ms, shadow_ms, shade_ms, light_ms, total_ms;
   double filter_ms, error_ms, gen_ms, accum_ms;   /* adaptive phases (PERFBREAKDOWN, AdapBilatAtomic.cu:1409-1536) */
   double mlp_ms;                                  /* learned error model: features + inference */
+  uint64_t shadow_nodes, shadow_tris;             /* 4-wide nodes visited / triangles tested by shadow rays */
 } cpt_stats;
 int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset);
 

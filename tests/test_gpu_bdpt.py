@@ -160,3 +160,5 @@ def test_refill_traversal_same_image(cpt, cornell_runs, variant, node_format, re
     assert np.array_equal(a.sample_counts(), b.sample_counts())
     sa, sb = a.stats(), b.stats()
     assert sa["extension_rays"] == sb["extension_rays"] and sa["extend_nodes"] == sb["extend_nodes"]
+    assert sa["shadow_nodes"] == sb["shadow_nodes"] > 0 and sa["shadow_tris"] == sb["shadow_tris"]
+
