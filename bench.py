@@ -53,7 +53,8 @@ WIDTH, HEIGHT, SPP, EYE_DEPTH, LIGHT_DEPTH = 1280, 720, 64, 8, 8
 def extend_kernel_name(args, demangled=False) -> str:
     """Template instance the renderer launches for extension rays (demangled:
     the prefix of the name the profiler reports)."""
-    stk, minw = {806: (8, 6), 1606: (16, 6), 8: (8, 0)}.get(args.lds_stack, (16, 0))
+    stk, minw = {806: (8, 6), 807: (8, 7), 808: (8, 8), 1606: (16, 6), 8: (8, 0), 24: (24, 0)}.get(args.lds_stack,
+                                                                                                 (16, 0))
     q = "true" if args.node_format else "false"
     layout = 2 if args.node_format else 0
     if demangled:
@@ -164,7 +165,8 @@ def hbm_traffic(args):
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t",
                "--", sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
                "--no-traffic", "--no-learned", "--no-multiview", "--frames", str(args.frames), "--in-flight", str(args.in_flight),
-               "--lds-stack", str(args.lds_stack), "--node-format", str(args.node_format),
+               "--lds-stack", str(args.lds_stack), "--lds-stack-shadow", str(args.lds_stack_shadow),
+               "--node-format", str(args.node_format),
                "--refill", str(args.refill), "--refill-groups", str(args.refill_groups),
                "--refill-shadow", str(args.refill_shadow),
                "--copy-calib", str(calib >> 20)]
@@ -230,8 +232,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--frames", type=int, default=SPP)
     ap.add_argument("--in-flight", type=int, default=64, help="frames traced as one wavefront")
-    ap.add_argument("--lds-stack", type=int, default=806, help="traversal variant: LDS stack entries (8/16/24; "
+    ap.add_argument("--lds-stack", type=int, default=807, help="traversal variant: LDS stack entries (8/16/24; "
                     "806/807/808/1606 = 8 or 16 entries with >= 6, 7 or 8 waves per SIMD)")
+    ap.add_argument("--lds-stack-shadow", type=int, default=808, help="the same for shadow rays (0: as --lds-stack)")
     ap.add_argument("--node-format", type=int, default=1, help="BVH nodes: 0 = 128-byte FP32, 1 = 64-byte quantized")
     ap.add_argument("--refill", type=int, default=64,
                     help="traversal lanes refill from a ray queue once this many of a wave's are idle (0: off)")
@@ -277,7 +280,7 @@ def main():
     R = ca.Renderer(WIDTH, HEIGHT, eye_depth=EYE_DEPTH, light_depth=LIGHT_DEPTH)
     R.set_attrib("Frames In Flight", args.in_flight)
     R.set_attrib("Traversal LDS Stack", args.lds_stack)
-    R.set_attrib("Shadow LDS Stack", args.lds_stack)
+    R.set_attrib("Shadow LDS Stack", args.lds_stack_shadow or args.lds_stack)
     R.set_attrib("Node Format", args.node_format)
     R.set_attrib("Traversal Refill", args.refill)
     R.set_attrib("Refill Workgroups", args.refill_groups)
@@ -324,7 +327,8 @@ def main():
                                "depth 8, 64 spp per step, light-vertex cache every 3rd frame",
                    "triangles": int(len(sc["idx"])), "width": WIDTH, "height": HEIGHT, "spp": args.frames,
                    "eye_depth": EYE_DEPTH, "light_depth": LIGHT_DEPTH, "frames_in_flight": args.in_flight,
-                   "traversal_variant": args.lds_stack, "node_format": args.node_format,
+                   "traversal_variant": args.lds_stack,
+                   "shadow_variant": args.lds_stack_shadow or args.lds_stack, "node_format": args.node_format,
                    "traversal_refill": args.refill, "refill_groups": args.refill_groups,
                    "shadow_refill": args.refill_shadow},
         "samples_per_s": round(st["eye_paths"] / wall, 1),

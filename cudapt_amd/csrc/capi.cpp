@@ -938,6 +938,9 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   if (eye_depth < 1 || eye_depth > (u32)kMaxDepth) return fail(CPT_E_LIMIT, "render: eye depth %u not in 1..15", eye_depth);
   if (light_depth < 1 || light_depth > 64) return fail(CPT_E_LIMIT, "render: light depth %u not in 1..64", light_depth);
   if (light_paths < 1 || light_paths > (u32)kBlock) return fail(CPT_E_LIMIT, "render: light paths %u not in 1..256", light_paths);
+  for (const char* a : {kLdsStack, kLdsStackShadow})
+    if (!traversal_variant_ok((int)r->attr(a)))
+      return fail(CPT_E_ARG, "render: %s %g is not a traversal variant (8, 16, 24, 806, 807, 808, 1606)", a, r->attr(a));
   const bool adaptive = is_adaptive(r->mode);
   u32 in_flight = (u32)r->attr(adaptive ? kPassFrames : kInFlight);
   if (in_flight < 1 || in_flight > kMaxInFlight)
@@ -1042,8 +1045,8 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   }
   // traversal kernel variant: 8 LDS stack entries, >= 6 waves per SIMD
   // (swept on the sphere grid: 1.58 vs 1.36 Grays/s for 16 entries)
-  r->attribs.push_back({kLdsStack, 0, 806});
-  r->attribs.push_back({kLdsStackShadow, 0, 806});
+  r->attribs.push_back({kLdsStack, 0, 807});
+  r->attribs.push_back({kLdsStackShadow, 0, 808});
   r->attribs.push_back({kNodeFormat, 0, 1});
   r->attribs.push_back({kRefill, 0, 64});
   r->attribs.push_back({kRefillShadow, 0, 0});
@@ -1411,7 +1414,7 @@ struct Schedule {
 };
 const Schedule kSchedules[] = {{0, 0, 8},     {256, 0, 8},  {1024, 0, 8}, {4096, 0, 8}, {1024, 0, 4},
                                {1024, 0, 16}, {0, 64, 6},   {0, 64, 8},   {0, 48, 8},   {0, 32, 8}};
-const u32 kVariants[] = {806u, 1606u};
+const u32 kVariants[] = {807u, 806u};
 
 cpt_batch_config default_batch_config() {
   cpt_batch_config c;
@@ -1421,7 +1424,7 @@ cpt_batch_config default_batch_config() {
   c.extend_workgroups = 8;
   c.refill = 64;
   c.node_format = 1;
-  c.traversal_variant = 806;
+  c.traversal_variant = 807;
   c.use_graph = 1;
   c.eye_depth = 5;
   c.light_depth = 5;
@@ -1440,6 +1443,8 @@ int batch_check_config(const cpt_batch_config& c) {
   if (c.light_depth < 1 || c.light_depth > 64) return fail(CPT_E_LIMIT, "batch: light depth not in 1..64");
   if (c.light_paths < 1 || c.light_paths > (u32)kBlock) return fail(CPT_E_LIMIT, "batch: light paths not in 1..256");
   if (c.node_format > 1) return fail(CPT_E_ARG, "batch: node format %u not 0 or 1", c.node_format);
+  if (!traversal_variant_ok((int)c.traversal_variant))
+    return fail(CPT_E_ARG, "batch: traversal variant %u not 8, 16, 24, 806, 807, 808 or 1606", c.traversal_variant);
   return CPT_OK;
 }
 

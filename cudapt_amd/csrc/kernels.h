@@ -134,7 +134,13 @@ hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st);
 // Running-mean update of every pixel from the batch's per-frame records, frames in order.
 hipError_t launch_apply_frames(const FrameArgs& fa, const float4* rec_acc, float* result, uint32_t* sample_n,
                                hipStream_t st);
-// lds_stack: traversal stack entries kept in LDS (8, 16 or 24; the rest in private memory)
+// lds_stack: traversal variant, the stack entries kept in LDS (the rest in
+// private memory) and the minimum resident waves per SIMD the register
+// allocator is held to: 8, 16, 24 (no minimum), 806, 807, 808 (8 entries, >= 6,
+// 7 or 8 waves), 1606 (16 entries, >= 6 waves).  Other values: invalid argument.
+inline bool traversal_variant_ok(int v) {
+  return v == 8 || v == 16 || v == 24 || v == 806 || v == 807 || v == 808 || v == 1606;
+}
 hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc, int bounce, uint32_t max_n,
                          float4* hits, DevStats* stats, int lds_stack, int quant, hipStream_t st);
 // apply: 0 = plain running mean at termination, 1 = running mean + G-buffer

@@ -149,73 +149,24 @@ __device__ __forceinline__ void load_node(const DevScene& sc, uint32_t node, Nod
   }
 }
 
-// lstk points at this thread's column of a [STK][kBlock] u64 array in LDS.
-// Q: nodes in the 64-byte quantized layout.
-template <bool ANY, int STK = CPT_LDS_STACK, bool Q = false>
-__device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps, bool cull, uint64_t* lstk,
-                          TraceHit& h, uint32_t& nn, uint32_t& nt) {
-  const f3 idir = mk3(1.f / escape_zero(d.x), 1.f / escape_zero(d.y), 1.f / escape_zero(d.z));
-  const f3 o2 = o + d * eps;
-  h.t = maxd; h.u = h.v = 0.f; h.leaf = kInvalid; h.orig = kInvalid;
-  uint64_t pstk[CPT_STACK_TOTAL - STK];
-  int sp = 0;
-  uint32_t node = 0;
-  for (;;) {
-    NodeView nv;
-    load_node<Q>(sc, node, nv);
-    nn++;
-    const uint32_t* ch_ = nv.ch;
-    const uint32_t* cn_ = nv.cn;
-    uint64_t key[4];
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      float t0x = (nv.lx[c] - o.x) * idir.x, t1x = (nv.hx[c] - o.x) * idir.x;
-      float t0y = (nv.ly[c] - o.y) * idir.y, t1y = (nv.hy[c] - o.y) * idir.y;
-      float t0z = (nv.lz[c] - o.z) * idir.z, t1z = (nv.hz[c] - o.z) * idir.z;
-      float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
-      float tfar = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * 1.0000004f;
-      bool hit = ch_[c] != kInvalid && tnear <= tfar && tnear <= h.t + cull_slack(h.t, eps);
-      key[c] = ~0ull;
-      if (hit) {
-        if (cn_[c] > 0) {
-          test_leaf(sc.tris, ch_[c], cn_[c], o2, d, cull, h, nt);
-          if (ANY && h.leaf != kInvalid) return true;
-        } else {
-          key[c] = ((uint64_t)__float_as_uint(tnear) << 32) | ch_[c];
-        }
-      }
-    }
-    // sort interior children near-first (tnear >= 0, so float bits order = value order)
-    cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[1], key[2]);
-    if (key[0] != ~0ull) {
-      node = (uint32_t)key[0];
-#pragma unroll
-      for (int c = 3; c >= 1; c--) {
-        if (key[c] != ~0ull && sp < CPT_STACK_TOTAL) {
-          if (sp < STK) lstk[sp * kBlock] = key[c];
-          else pstk[sp - STK] = key[c];
-          sp++;
-        }
-      }
-      continue;
-    }
-    // pop the nearest pending node still in front of the best hit
-    bool found = false;
-    while (sp > 0) {
-      sp--;
-      uint64_t e = sp < STK ? lstk[sp * kBlock] : pstk[sp - STK];
-      float tn = __uint_as_float((uint32_t)(e >> 32));
-      if (tn <= h.t + cull_slack(h.t, eps)) { node = (uint32_t)e; found = true; break; }
-    }
-    if (!found) break;
+// Instantiates launch macro M(stack entries, min waves per SIMD) for the
+// traversal variant v (traversal_variant_ok); other values return an error.
+#define CPT_VARIANT_SWITCH(v, M)            \
+  switch (v) {                              \
+    case 8: M(8, 0); break;                 \
+    case 16: M(16, 0); break;               \
+    case 24: M(24, 0); break;               \
+    case 806: M(8, 6); break;               \
+    case 807: M(8, 7); break;               \
+    case 808: M(8, 8); break;               \
+    case 1606: M(16, 6); break;             \
+    default: return hipErrorInvalidValue;   \
   }
-  return h.leaf != kInvalid;
-}
 
-// ---- traversal as a resumable step, for lanes that refill themselves -------
-// The same per-node operations as traverse4 in the same order (identical hits),
-// split so that a lane whose ray has finished can start another while the rest
-// of its wave is still traversing.
+// ---- 4-wide traversal ------------------------------------------------------
+// The state of one ray's traversal, advanced one node at a time by ray_step, so
+// that a lane whose ray has finished can start another while the rest of its
+// wave is still traversing (k_trace_dyn) or simply loop (traverse4).
 struct RayState {
   f3 o, d, idir, o2;
   float eps;
@@ -233,14 +184,27 @@ __device__ __forceinline__ void ray_begin(RayState& R, f3 o, f3 d, float maxd, f
   R.node = 0;
   R.sp = 0;
 }
-// One node of the traversal; true when the ray is finished.
+__device__ __forceinline__ uint32_t pick4(const uint32_t (&a)[4], uint32_t c) {
+  return c == 0 ? a[0] : c == 1 ? a[1] : c == 2 ? a[2] : a[3];
+}
+// One node: the four child boxes against the ray (culled beyond the best hit
+// plus slack); the leaf children hit are then tested as one run of triangles
+// (child order, triangle order), so that lanes whose leaves sit under different
+// children test them in the same loop; interior children are culled again with
+// the updated best hit, sorted near-first, the nearest taken next and the rest
+// pushed (LDS stack top, private stack below it).  The closest hit is the
+// (t, original id) minimum, which neither the test order nor the conservative
+// culling changes.  Returns true when the ray is finished.
+// lstk points at this thread's column of a [STK][kBlock] u64 array in LDS.
 template <bool ANY, int STK, bool Q>
-__device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, uint64_t* lstk, uint64_t* pstk,
+__device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool cull, uint64_t* lstk, uint64_t* pstk,
                                          uint32_t& nn, uint32_t& nt) {
   NodeView nv;
   load_node<Q>(sc, R.node, nv);
   nn++;
+  const float lim = R.h.t + cull_slack(R.h.t, R.eps);
   uint64_t key[4];
+  uint32_t lf[4], ln[4];
 #pragma unroll
   for (int c = 0; c < 4; c++) {
     float t0x = (nv.lx[c] - R.o.x) * R.idir.x, t1x = (nv.hx[c] - R.o.x) * R.idir.x;
@@ -248,16 +212,34 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, uint64
     float t0z = (nv.lz[c] - R.o.z) * R.idir.z, t1z = (nv.hz[c] - R.o.z) * R.idir.z;
     float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
     float tfar = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * 1.0000004f;
-    bool hit = nv.ch[c] != kInvalid && tnear <= tfar && tnear <= R.h.t + cull_slack(R.h.t, R.eps);
-    key[c] = ~0ull;
-    if (hit) {
-      if (nv.cn[c] > 0) {
-        test_leaf(sc.tris, nv.ch[c], nv.cn[c], R.o2, R.d, false, R.h, nt);
-        if (ANY && R.h.leaf != kInvalid) return true;
-      } else {
-        key[c] = ((uint64_t)__float_as_uint(tnear) << 32) | nv.ch[c];
+    bool hit = nv.ch[c] != kInvalid && tnear <= tfar && tnear <= lim;
+    const bool leaf = nv.cn[c] > 0;
+    lf[c] = nv.ch[c];
+    ln[c] = hit && leaf ? nv.cn[c] : 0u;
+    key[c] = hit && !leaf ? ((uint64_t)__float_as_uint(tnear) << 32) | nv.ch[c] : ~0ull;
+  }
+  const uint32_t ntri = ln[0] + ln[1] + ln[2] + ln[3];
+  if (ntri) {
+    nt += ntri;
+    uint32_t c = 0, i = lf[0], e = lf[0] + ln[0];
+    for (uint32_t k = 0; k < ntri; k++) {
+      while (i == e) {
+        c++;
+        i = pick4(lf, c);
+        e = i + pick4(ln, c);
       }
+      const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
+      float u = 0.f, v = 0.f;
+      const float t = isect_tri(R.o2, R.d, ld3(a), ld3(b), ld3(cc), cull, u, v);
+      const uint32_t oid = __float_as_uint(a.w);
+      if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i; R.h.orig = oid; }
+      if (ANY && R.h.leaf != kInvalid) return true;
+      i++;
     }
+    const float lim2 = R.h.t + cull_slack(R.h.t, R.eps);
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+      if (key[q] != ~0ull && __uint_as_float((uint32_t)(key[q] >> 32)) > lim2) key[q] = ~0ull;
   }
   cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[1], key[2]);
   if (key[0] != ~0ull) {
@@ -272,6 +254,7 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, uint64
     }
     return false;
   }
+  // pop the nearest pending node still in front of the best hit
   while (R.sp > 0) {
     R.sp--;
     uint64_t e = R.sp < STK ? lstk[R.sp * kBlock] : pstk[R.sp - STK];
@@ -279,6 +262,20 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, uint64
     if (tn <= R.h.t + cull_slack(R.h.t, R.eps)) { R.node = (uint32_t)e; return false; }
   }
   return true;
+}
+
+// Closest (ANY: any) hit of one ray, the loop of ray_step.
+// Q: nodes in the 64-byte quantized layout.
+template <bool ANY, int STK = CPT_LDS_STACK, bool Q = false>
+__device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps, bool cull, uint64_t* lstk,
+                          TraceHit& h, uint32_t& nn, uint32_t& nt) {
+  RayState R;
+  ray_begin(R, o, d, maxd, eps);
+  uint64_t pstk[CPT_STACK_TOTAL - STK];
+  while (!ray_step<ANY, STK, Q>(sc, R, cull, lstk, pstk, nn, nt)) {
+  }
+  h = R.h;
+  return h.leaf != kInvalid;
 }
 
 // Same query over the reference's binary layout (right child popped first).
@@ -832,24 +829,14 @@ hipError_t launch_extend(const DevScene& sc, PathBuf pb, const FrameCounters* fc
   const uint32_t* cnt = &fc->active[bounce];
 #define CPT_EXT(L, S, W) k_trace<L, false, 1, S, W><<<grid, block, 0, st>>>(sc, pb.ray_o, pb.ray_d, 1, cnt, max_n, 0, hits, stats)
   if (quant) {
-    switch (lds_stack) {
-      case 8: CPT_EXT(2, 8, 0); break;
-      case 806: CPT_EXT(2, 8, 6); break;
-      case 808: CPT_EXT(2, 8, 8); break;
-      case 1606: CPT_EXT(2, 16, 6); break;
-      default: CPT_EXT(2, 16, 0); break;
-    }
+#define CPT_EXT_Q(S, W) CPT_EXT(2, S, W)
+#define CPT_EXT_F(S, W) CPT_EXT(0, S, W)
+    CPT_VARIANT_SWITCH(lds_stack, CPT_EXT_Q)
   } else {
-    switch (lds_stack) {
-      case 8: CPT_EXT(0, 8, 0); break;
-      case 24: CPT_EXT(0, 24, 0); break;
-      case 806: CPT_EXT(0, 8, 6); break;      // 8 LDS entries, >= 6 waves per SIMD
-      case 807: CPT_EXT(0, 8, 7); break;      // 8 LDS entries, >= 7 waves per SIMD
-      case 808: CPT_EXT(0, 8, 8); break;      // 8 LDS entries, >= 8 waves per SIMD
-      case 1606: CPT_EXT(0, 16, 6); break;
-      default: CPT_EXT(0, 16, 0); break;
-    }
+    CPT_VARIANT_SWITCH(lds_stack, CPT_EXT_F)
   }
+#undef CPT_EXT_Q
+#undef CPT_EXT_F
 #undef CPT_EXT
   return hipGetLastError();
 }
@@ -902,12 +889,15 @@ hipError_t launch_extend_persist(const DevScene& sc, PathBuf pb, const FrameCoun
   k_fill_u32<<<1, 1, 0, st>>>(queue, 0u);
   const uint32_t* cnt = &fc->active[bounce];
 #define CPT_EXTP(S, W, Q) k_extend_persist<S, W, Q><<<grid, kBlock, 0, st>>>(sc, pb.ray_o, pb.ray_d, cnt, chunk, queue, hits, stats)
-  switch (lds_stack) {
-    case 8: if (quant) CPT_EXTP(8, 0, true); else CPT_EXTP(8, 0, false); break;
-    case 806: if (quant) CPT_EXTP(8, 6, true); else CPT_EXTP(8, 6, false); break;
-    case 1606: if (quant) CPT_EXTP(16, 6, true); else CPT_EXTP(16, 6, false); break;
-    default: if (quant) CPT_EXTP(16, 0, true); else CPT_EXTP(16, 0, false); break;
+#define CPT_EXTP_Q(S, W) CPT_EXTP(S, W, true)
+#define CPT_EXTP_F(S, W) CPT_EXTP(S, W, false)
+  if (quant) {
+    CPT_VARIANT_SWITCH(lds_stack, CPT_EXTP_Q)
+  } else {
+    CPT_VARIANT_SWITCH(lds_stack, CPT_EXTP_F)
   }
+#undef CPT_EXTP_Q
+#undef CPT_EXTP_F
 #undef CPT_EXTP
   return hipGetLastError();
 }
@@ -993,7 +983,7 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
     }
     // every lane idle with nothing left to take: the results were written above
     if (__ballot(active) == 0ull) break;
-    if (active && ray_step<ANY, STK, Q>(sc, R, lstk, pstk, nn, nt)) {
+    if (active && ray_step<ANY, STK, Q>(sc, R, false, lstk, pstk, nn, nt)) {
       active = false;
       pending = true;
     }
@@ -1016,19 +1006,19 @@ hipError_t launch_trace_dyn(const DevScene& sc, const float4* ro, const float4* 
   if (shadow ? (!contrib || !pathbuf_ok(nxt)) : !out) return hipErrorInvalidValue;
   k_fill_u32<<<1, 1, 0, st>>>(queue, 0u);
 #define CPT_DYN(O, S, W, Qb) k_trace_dyn<O, S, W, Qb><<<grid, kBlock, 0, st>>>(sc, ro, rd, count, queue, out, contrib, nxt, stats, refill)
-#define CPT_DYN_V(O, Qb)                            \
-  switch (lds_stack) {                              \
-    case 8: CPT_DYN(O, 8, 0, Qb); break;            \
-    case 806: CPT_DYN(O, 8, 6, Qb); break;          \
-    case 1606: CPT_DYN(O, 16, 6, Qb); break;        \
-    default: CPT_DYN(O, 16, 0, Qb); break;          \
-  }
+#define CPT_DYN_3Q(S, W) CPT_DYN(3, S, W, true)
+#define CPT_DYN_3F(S, W) CPT_DYN(3, S, W, false)
+#define CPT_DYN_1Q(S, W) CPT_DYN(1, S, W, true)
+#define CPT_DYN_1F(S, W) CPT_DYN(1, S, W, false)
   if (shadow) {
-    if (quant) { CPT_DYN_V(3, true) } else { CPT_DYN_V(3, false) }
+    if (quant) { CPT_VARIANT_SWITCH(lds_stack, CPT_DYN_3Q) } else { CPT_VARIANT_SWITCH(lds_stack, CPT_DYN_3F) }
   } else {
-    if (quant) { CPT_DYN_V(1, true) } else { CPT_DYN_V(1, false) }
+    if (quant) { CPT_VARIANT_SWITCH(lds_stack, CPT_DYN_1Q) } else { CPT_VARIANT_SWITCH(lds_stack, CPT_DYN_1F) }
   }
-#undef CPT_DYN_V
+#undef CPT_DYN_3Q
+#undef CPT_DYN_3F
+#undef CPT_DYN_1Q
+#undef CPT_DYN_1F
 #undef CPT_DYN
   return hipGetLastError();
 }
@@ -1260,24 +1250,14 @@ hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const Fr
   dim3 grid((max_n + kBlock - 1) / kBlock), block(kBlock);
 #define CPT_SH(S, W, Q) k_shadow<S, W, Q><<<grid, block, 0, st>>>(sc, sb, nxt, fc, bounce, stats)
   if (quant) {
-    switch (lds_stack) {
-      case 8: CPT_SH(8, 0, true); break;
-      case 806: CPT_SH(8, 6, true); break;
-      case 808: CPT_SH(8, 8, true); break;
-      case 1606: CPT_SH(16, 6, true); break;
-      default: CPT_SH(16, 0, true); break;
-    }
+#define CPT_SH_Q(S, W) CPT_SH(S, W, true)
+#define CPT_SH_F(S, W) CPT_SH(S, W, false)
+    CPT_VARIANT_SWITCH(lds_stack, CPT_SH_Q)
   } else {
-    switch (lds_stack) {
-      case 8: CPT_SH(8, 0, false); break;
-      case 24: CPT_SH(24, 0, false); break;
-      case 806: CPT_SH(8, 6, false); break;
-      case 807: CPT_SH(8, 7, false); break;
-      case 808: CPT_SH(8, 8, false); break;
-      case 1606: CPT_SH(16, 6, false); break;
-      default: CPT_SH(16, 0, false); break;
-    }
+    CPT_VARIANT_SWITCH(lds_stack, CPT_SH_F)
   }
+#undef CPT_SH_Q
+#undef CPT_SH_F
 #undef CPT_SH
   return hipGetLastError();
 }

@@ -88,4 +88,6 @@ def test_batch_argument_errors(cpt, setup):
     with pytest.raises(cpt.CptError):
         B.set_config(refill=65)
     with pytest.raises(cpt.CptError):
+        B.set_config(traversal_variant=805)
+    with pytest.raises(cpt.CptError):
         B.result(0)                              # nothing rendered yet

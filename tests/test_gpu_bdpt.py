@@ -127,12 +127,13 @@ def test_rmse_harness_on_device(cpt, cornell_runs):
     assert rows[0].rmse > 10 * rows[-1].rmse
 
 
-@pytest.mark.parametrize("variant", [806, 1606, 16])
+@pytest.mark.parametrize("variant", [806, 807, 1606, 16])
 def test_quantized_node_format_same_image(cpt, cornell_runs, variant):
     """Traversal over 64-byte quantized nodes returns the same hits, so the
     rendered frames are bit-identical to the 128-byte layout's."""
     sc, S = cornell_runs["sc"], cornell_runs["S"]
     a = cpt.Renderer(64, 48)
+    a.set_attrib("Node Format", 0)
     ra = a.render(S, sc["camera"], frames=6)
     b = cpt.Renderer(64, 48)
     b.set_attrib("Node Format", 1)
@@ -142,7 +143,8 @@ def test_quantized_node_format_same_image(cpt, cornell_runs, variant):
     assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
 
 
-@pytest.mark.parametrize("variant,node_format,refill", [(806, 0, 8), (806, 1, 1), (1606, 0, 64), (16, 1, 16)])
+@pytest.mark.parametrize("variant,node_format,refill", [(806, 0, 8), (807, 1, 1), (1606, 0, 64), (16, 1, 16),
+                                                         (808, 1, 64), (24, 0, 32)])
 def test_refill_traversal_same_image(cpt, cornell_runs, variant, node_format, refill):
     """Lanes that refill themselves from the ray queue trace the same rays with
     the same per-node operations: frames bit-identical to one ray per thread."""
@@ -162,3 +164,12 @@ def test_refill_traversal_same_image(cpt, cornell_runs, variant, node_format, re
     assert sa["extension_rays"] == sb["extension_rays"] and sa["extend_nodes"] == sb["extend_nodes"]
     assert sa["shadow_nodes"] == sb["shadow_nodes"] > 0 and sa["shadow_tris"] == sb["shadow_tris"]
 
+
+
+@pytest.mark.parametrize("attr", ["Traversal LDS Stack", "Shadow LDS Stack"])
+def test_unknown_traversal_variant_is_an_error(cpt, cornell_runs, attr):
+    sc, S = cornell_runs["sc"], cornell_runs["S"]
+    R = cpt.Renderer(16, 12)
+    R.set_attrib(attr, 805)
+    with pytest.raises(cpt.CptError, match="traversal variant"):
+        R.render(S, sc["camera"], frames=1)
