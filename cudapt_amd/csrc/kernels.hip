@@ -904,8 +904,10 @@ hipError_t launch_extend_persist(const DevScene& sc, PathBuf pb, const FrameCoun
 
 // Dynamic refill: a persistent grid whose lanes each trace one ray at a time
 // and take the next as soon as theirs is finished.  A wave hands out ray
-// indices from a chunk of kDynChunk it took from the global counter `queue`
-// (one atomic per chunk); lanes refill once at least `refill` (1..64) are idle.
+// indices from a chunk it took from the global counter `queue` (one atomic per
+// chunk): kDynChunk rays when the launch has enough for every wave to take
+// several, else 64, so that a small batch still spreads over every wave; lanes
+// refill once at least `refill` (1..64) are idle.
 // A finished lane writes its result at the wave's next refill point, together
 // with the other finished lanes, not in the traversal loop: the shadow
 // epilogue's read-modify-write would otherwise stall the whole wave on its load
@@ -927,7 +929,17 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
   const uint32_t count = *countp;
   const int lane = threadIdx.x & 63;
   const uint64_t below = (1ull << lane) - 1ull;
-  uint32_t cnext = 0, cend = 0;            // the wave's chunk (uniform across the wave)
+  const uint32_t waves = gridDim.x * (kBlock / 64);
+  const uint32_t chunk = (uint64_t)count >= (uint64_t)kDynChunk * waves ? kDynChunk : 64u;
+  // When one chunk per wave covers the launch, wave w takes chunk w and no
+  // atomic is issued (every wave's atomic on the one counter would cost more
+  // than a small launch's traversal); otherwise every chunk comes from the
+  // counter, in the order waves ask.
+  const bool fixed = (uint64_t)count <= (uint64_t)waves * chunk;
+  const uint32_t wave = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint64_t first = fixed ? (uint64_t)wave * chunk : (uint64_t)count;
+  uint32_t cnext = (uint32_t)min<uint64_t>(first, count);   // the wave's chunk (uniform across the wave)
+  uint32_t cend = (uint32_t)min<uint64_t>(first + chunk, count);
   bool exhausted = false;                  // uniform
   bool active = false;
   bool pending = false;                    // finished, result not yet written
@@ -963,12 +975,13 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
       uint32_t given = 0;
       while (given < n_idle) {
         if (cnext >= cend) {
-          uint32_t b = 0;
-          if (lane == 0) b = atomicAdd(queue, kDynChunk);
-          b = __shfl(b, 0, 64);
+          if (fixed) { exhausted = true; break; }
+          uint32_t q = 0;
+          if (lane == 0) q = atomicAdd(queue, chunk);
+          const uint64_t b = __shfl(q, 0, 64);
           if (b >= count) { exhausted = true; break; }
-          cnext = b;
-          cend = min(b + kDynChunk, count);
+          cnext = (uint32_t)b;
+          cend = (uint32_t)min<uint64_t>(b + chunk, count);
         }
         const uint32_t take = min(n_idle - given, cend - cnext);
         if (!active && rank >= given && rank < given + take) {
