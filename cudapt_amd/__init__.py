@@ -66,6 +66,7 @@ def lib():
             "cpt_bvh_build_sah": [vp, c32],
             "cpt_bvh_export": [vp, vp, vp, vp, vp],
             "cpt_bvh4_info": [vp, vp, vp, vp],
+            "cpt_bvh4_set_leaf_size": [vp, c32],
             "cpt_scene_upload": [vp],
             "cpt_scene_destroy": [vp],
             "cpt_trace_closest": [vp, vp, c32, c32, vp],
@@ -223,6 +224,10 @@ class Scene:
         _check(lib().cpt_bvh_export(self.h, _ptr(bounds), _ptr(off), _ptr(reorder), ctypes.byref(n)),
                "cpt_bvh_export")
         return bounds, off, reorder
+
+    def set_leaf_size(self, max_tris: int):
+        """Most triangles per leaf of the 4-wide device BVH (1..64)."""
+        _check(lib().cpt_bvh4_set_leaf_size(self.h, int(max_tris)), "cpt_bvh4_set_leaf_size")
 
     def bvh4_info(self):
         a, b, c = ctypes.c_uint32(), ctypes.c_uint32(), ctypes.c_uint32()

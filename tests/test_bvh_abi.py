@@ -71,3 +71,21 @@ def test_errors_are_reported_not_fatal():
         S.set_mesh(np.zeros((3, 3), np.float32), None, np.array([[0, 1, 5]], np.uint32), np.zeros(1, np.uint32))
     with pytest.raises(ca.CptError):
         ca.Renderer(0, 10)
+
+
+def test_leaf_size_changes_the_4wide_tree_only():
+    """Smaller leaves: more 4-wide nodes and leaves, the binary SAH tree unchanged."""
+    S = ca.Scene(scenes.sphere_grid(grid=8), upload=False)
+    b0 = S.bvh()
+    info = {}
+    for L in (1, 2, 4, 8, 64):
+        S.set_leaf_size(L)
+        info[L] = S.bvh4_info()
+        assert all(np.array_equal(x, y) for x, y in zip(S.bvh(), b0))
+    assert info[1]["nodes"] > info[2]["nodes"] > info[4]["nodes"] > info[8]["nodes"] >= info[64]["nodes"]
+    assert info[1]["leaves"] >= S.nt                       # one triangle per leaf
+    S.set_leaf_size(2)
+    assert S.bvh4_info() == info[2]
+    for bad in (0, 65):
+        with pytest.raises(ca.CptError, match="not in 1..64"):
+            S.set_leaf_size(bad)

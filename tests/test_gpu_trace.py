@@ -136,3 +136,18 @@ def test_quantized_nodes_sphere_grid_bit_exact(cpt, sphere_grid):
     q = _gpu_hits(cpt, S, rays, layout=2)
     eq = hits_equal(q, ref)
     assert eq.all(), f"{(~eq).sum()} hits differ"
+
+
+@pytest.mark.parametrize("leaf", [1, 4, 16])
+def test_leaf_sizes_bit_exact(cpt, sphere_grid, leaf):
+    """Leaves of any size hold the same triangles: identical hits in both node layouts."""
+    sc, _, O = sphere_grid
+    S = cpt.Scene(sc, upload=False)
+    S.set_leaf_size(leaf)
+    S.upload()
+    rays = np.concatenate([camera_rays(sc["camera"], 160, 90, jitter=True, seed=17),
+                           random_rays(30000, [-64, 0.0, -64], [64, 6.0, 64], seed=18)])
+    ref, _ = O.trace(rays, "defined")
+    for layout in (0, 2):
+        eq = hits_equal(_gpu_hits(cpt, S, rays, layout=layout), ref)
+        assert eq.all(), f"leaf {leaf} layout {layout}: {(~eq).sum()} hits differ"
