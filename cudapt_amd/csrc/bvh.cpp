@@ -158,8 +158,160 @@ bool build_bvh_sah(const float* pos3, const u32* idx3, u32 nt, u32 max_depth, Bv
   return !out.nodes.empty();
 }
 
+// ---- the device tree ----------------------------------------------------------
+// The reference's SAH costs a split with the children's *centroid* bounds
+// (bvhhelper.cpp:119-164), so a triangle much larger than its neighbours is
+// placed by its centroid alone and every ancestor's box grows to its size: a
+// ground quad spanning a scene ends up under a chain of scene-sized boxes that
+// most rays must enter.  The device tree is built for traversal instead:
+// 32 centroid bins on each axis costed with the triangles' real bounds, plus
+// a candidate that splits off the node's oversized triangles (box surface area
+// at least a quarter of the node's), down to single triangles (the 4-wide
+// collapse then forms leaves); below depth 32, median splits bound the depth.
+// Hits do not depend on the tree (closest = (t, original id) minimum).
+namespace {
+
+constexpr int kDevBins = 32;
+constexpr u32 kDevMedianDepth = 32;
+
+inline double half_area(const Box& b) {
+  double dx = (double)b.mx[0] - b.mn[0], dy = (double)b.mx[1] - b.mn[1], dz = (double)b.mx[2] - b.mn[2];
+  if (dx < 0 || dy < 0 || dz < 0) return 0.0;
+  return dx * dy + dx * dz + dy * dz;
+}
+
+struct DevBuilder {
+  const float* cen;
+  const Box* tbox;
+  u32* order;
+  std::vector<BinNode>* nodes;
+
+  void build(u32 start, u32 n, u32 depth) {
+    const u32 me = (u32)nodes->size();
+    nodes->push_back(BinNode());
+    Box nb, cb;
+    box_empty(nb);
+    box_empty(cb);
+    for (u32 i = 0; i < n; i++) {
+      const u32 t = order[start + i];
+      box_merge(nb, tbox[t]);
+      box_grow(cb, cen + (size_t)t * 3);
+    }
+    for (int k = 0; k < 3; k++) { (*nodes)[me].mn[k] = nb.mn[k]; (*nodes)[me].mx[k] = nb.mx[k]; }
+    if (n <= 1) { (*nodes)[me].a = start; (*nodes)[me].n = n; return; }
+    u32* first = order + start;
+    u32* last = first + n;
+    u32* mid = nullptr;
+    int axis = 0;
+    for (int k = 1; k < 3; k++)
+      if (cb.mx[k] - cb.mn[k] > cb.mx[axis] - cb.mn[axis]) axis = k;
+    if (depth < kDevMedianDepth) {
+      double best = INFINITY;
+      int best_axis = -1, best_bin = -1;
+      bool best_big = false;
+      // oversized triangles against the rest
+      const double big_area = 0.25 * half_area(nb);
+      Box bb, rb;
+      box_empty(bb);
+      box_empty(rb);
+      u32 nbig = 0;
+      for (u32 i = 0; i < n; i++) {
+        const u32 t = first[i];
+        if (half_area(tbox[t]) >= big_area) { box_merge(bb, tbox[t]); nbig++; }
+        else box_merge(rb, tbox[t]);
+      }
+      if (nbig > 0 && nbig < n) {
+        best = half_area(bb) * nbig + half_area(rb) * (n - nbig);
+        best_big = true;
+      }
+      // binned centroid splits, real bounds
+      for (int ax = 0; ax < 3; ax++) {
+        const float lo = cb.mn[ax], ext = cb.mx[ax] - cb.mn[ax];
+        if (!(ext > 0.f)) continue;
+        u32 cnt[kDevBins] = {};
+        Box bx[kDevBins];
+        for (int k = 0; k < kDevBins; k++) box_empty(bx[k]);
+        const float scale = (float)kDevBins / ext;
+        for (u32 i = 0; i < n; i++) {
+          const u32 t = first[i];
+          int k = (int)((cen[(size_t)t * 3 + ax] - lo) * scale);
+          k = k < 0 ? 0 : (k >= kDevBins ? kDevBins - 1 : k);
+          cnt[k]++;
+          box_merge(bx[k], tbox[t]);
+        }
+        double right_area[kDevBins];
+        u32 right_n[kDevBins];
+        Box acc;
+        box_empty(acc);
+        u32 an = 0;
+        for (int k = kDevBins - 1; k > 0; k--) {
+          box_merge(acc, bx[k]);
+          an += cnt[k];
+          right_area[k] = half_area(acc);
+          right_n[k] = an;
+        }
+        box_empty(acc);
+        an = 0;
+        for (int k = 0; k < kDevBins - 1; k++) {          // split after bin k
+          box_merge(acc, bx[k]);
+          an += cnt[k];
+          if (an == 0 || right_n[k + 1] == 0) continue;
+          const double c = half_area(acc) * an + right_area[k + 1] * right_n[k + 1];
+          if (c < best) { best = c; best_axis = ax; best_bin = k; best_big = false; }
+        }
+      }
+      if (best_big) {
+        mid = std::partition(first, last, [&](u32 t) { return half_area(tbox[t]) >= big_area; });
+      } else if (best_axis >= 0) {
+        const float lo = cb.mn[best_axis], scale = (float)kDevBins / (cb.mx[best_axis] - cb.mn[best_axis]);
+        mid = std::partition(first, last, [&](u32 t) {
+          int k = (int)((cen[(size_t)t * 3 + best_axis] - lo) * scale);
+          k = k < 0 ? 0 : (k >= kDevBins ? kDevBins - 1 : k);
+          return k <= best_bin;
+        });
+      }
+    }
+    if (mid == nullptr || mid == first || mid == last) {   // median split (deep or degenerate)
+      mid = first + n / 2;
+      std::nth_element(first, mid, last,
+                       [&](u32 x, u32 y) { return cen[(size_t)x * 3 + axis] < cen[(size_t)y * 3 + axis]; });
+    }
+    const u32 nl = (u32)(mid - first);
+    build(start, nl, depth + 1);
+    const u32 right = (u32)nodes->size();
+    build(start + nl, n - nl, depth + 1);
+    (*nodes)[me].a = right - me;
+    (*nodes)[me].n = 0;
+  }
+};
+
+}  // namespace
+
+bool build_bvh_device(const float* pos3, const u32* idx3, u32 nt, BvhHost& out) {
+  out.dnodes.clear();
+  out.dreorder.resize(nt);
+  if (nt == 0) return false;
+  std::vector<float> cen((size_t)nt * 3);
+  std::vector<Box> tb(nt);
+  for (u32 i = 0; i < nt; i++) {
+    const float* a = pos3 + (size_t)idx3[i * 3] * 3;
+    const float* b = pos3 + (size_t)idx3[i * 3 + 1] * 3;
+    const float* c = pos3 + (size_t)idx3[i * 3 + 2] * 3;
+    for (int k = 0; k < 3; k++) {
+      cen[(size_t)i * 3 + k] = ((a[k] + b[k]) + c[k]) / 3.f;
+      tb[i].mn[k] = std::fmin(c[k], std::fmin(b[k], a[k]));
+      tb[i].mx[k] = std::fmax(c[k], std::fmax(b[k], a[k]));
+    }
+    out.dreorder[i] = i;
+  }
+  out.dnodes.reserve((size_t)nt * 2);
+  DevBuilder bld{cen.data(), tb.data(), out.dreorder.data(), &out.dnodes};
+  bld.build(0, nt, 0);
+  return true;
+}
+
 void collapse_bvh4(const float* /*pos3*/, const u32* /*idx3*/, BvhHost& b, u32 leaf_max) {
-  const std::vector<BinNode>& bn = b.nodes;
+  const std::vector<BinNode>& bn = b.dnodes;
   u32 N = (u32)bn.size();
   std::vector<u32> cnt(N, 0), first(N, 0);
   auto is_empty = [&](u32 i) { return bn[i].n == 0 && bn[i].mn[0] > bn[i].mx[0]; };

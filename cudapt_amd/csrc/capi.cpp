@@ -81,7 +81,7 @@ struct cpt_scene {
   hipStream_t stream = nullptr;
   DevArray<Node4> d_nodes4;
   DevArray<Node4Q> d_nodes4q;
-  DevArray<float4> d_tris, d_vpos, d_vnrm, d_bnodes;
+  DevArray<float4> d_tris, d_tris_bin, d_vpos, d_vnrm, d_bnodes;
   DevArray<uint4> d_tri_verts;
   DevArray<uint2> d_boff;
   DevArray<DevMaterial> d_mats;
@@ -103,6 +103,7 @@ struct cpt_scene {
     s.n_tris = nt;
     s.bnodes = d_bnodes.p;
     s.boff = d_boff.p;
+    s.tris_bin = d_tris_bin.p;
     return s;
   }
   ~cpt_scene() {
@@ -123,12 +124,16 @@ int upload_materials(cpt_scene* s) {
   static_assert(sizeof(DevMaterial) == sizeof(cpt_material), "material layouts agree");
   if (!s->mats.empty()) std::memcpy(dm.data(), s->mats.data(), sizeof(cpt_material) * s->mats.size());
   HIP_TRY(s->d_mats.upload(dm.data(), dm.size(), s->stream));
+  // emissive triangles listed in the reference's leaf order (the order light
+  // subpaths pick them in), each as its index in the device tree's order
+  std::vector<u32> dev_index(s->nt);
+  for (u32 i = 0; i < s->nt; i++) dev_index[s->bvh.dreorder[i]] = i;
   std::vector<u32> light;
   for (u32 i = 0; i < s->nt; i++) {
     u32 o = s->bvh.reorder[i];
     u32 m = s->mat[o];
     const float* e = s->mats[m].emissive;
-    if (std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) > 0.f) light.push_back(i);
+    if (std::sqrt(e[0] * e[0] + e[1] * e[1] + e[2] * e[2]) > 0.f) light.push_back(dev_index[o]);
   }
   s->n_light = (u32)light.size();
   HIP_TRY(s->d_light.upload(light.data(), light.size(), s->stream));
@@ -225,7 +230,8 @@ int cpt_bvh_build_sah(cpt_scene* s, uint32_t max_depth) {
   if (!s) return fail(CPT_E_ARG, "cpt_bvh_build_sah: null scene");
   if (s->nt == 0) return fail(CPT_E_STATE, "cpt_bvh_build_sah: empty mesh");
   if (max_depth == 0 || max_depth > 64) return fail(CPT_E_ARG, "cpt_bvh_build_sah: max_depth %u not in 1..64", max_depth);
-  if (!build_bvh_sah(s->pos.data(), s->idx.data(), s->nt, max_depth, s->bvh))
+  if (!build_bvh_sah(s->pos.data(), s->idx.data(), s->nt, max_depth, s->bvh) ||
+      !build_bvh_device(s->pos.data(), s->idx.data(), s->nt, s->bvh))
     return fail(CPT_E_STATE, "cpt_bvh_build_sah: build failed");
   collapse_bvh4(s->pos.data(), s->idx.data(), s->bvh, s->leaf_max);
   if (!quantize_bvh4(s->bvh)) return fail(CPT_E_STATE, "cpt_bvh_build_sah: a node cannot be quantized to 64 bytes");
@@ -280,18 +286,28 @@ int cpt_scene_upload(cpt_scene* s) {
   int rc = ensure_stream(s->stream);
   if (rc) return rc;
   const u32 nt = s->nt;
-  std::vector<float4> tris((size_t)nt * 3);
+  // triangle intersection data in a tree's leaf order (src/cudaRTCommon.cu:503-514,
+  // host FP32 edges): the device tree's for the 4-wide layouts, the reference
+  // tree's for its binary layout
+  auto tri_data = [&](const std::vector<u32>& order) {
+    std::vector<float4> out((size_t)nt * 3);
+    for (u32 i = 0; i < nt; i++) {
+      u32 o = order[i];
+      const float* a = &s->pos[(size_t)s->idx[o * 3] * 3];
+      const float* b = &s->pos[(size_t)s->idx[o * 3 + 1] * 3];
+      const float* c = &s->pos[(size_t)s->idx[o * 3 + 2] * 3];
+      float ow;
+      std::memcpy(&ow, &o, 4);
+      out[i * 3] = make_float4(a[0], a[1], a[2], ow);
+      out[i * 3 + 1] = make_float4(b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.f);
+      out[i * 3 + 2] = make_float4(c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.f);
+    }
+    return out;
+  };
+  std::vector<float4> tris = tri_data(s->bvh.dreorder), tris_bin = tri_data(s->bvh.reorder);
   std::vector<uint4> tv(nt);
-  for (u32 i = 0; i < nt; i++) {                // src/cudaRTCommon.cu:503-514, host FP32 edges
-    u32 o = s->bvh.reorder[i];
-    const float* a = &s->pos[(size_t)s->idx[o * 3] * 3];
-    const float* b = &s->pos[(size_t)s->idx[o * 3 + 1] * 3];
-    const float* c = &s->pos[(size_t)s->idx[o * 3 + 2] * 3];
-    float ow;
-    std::memcpy(&ow, &o, 4);
-    tris[i * 3] = make_float4(a[0], a[1], a[2], ow);
-    tris[i * 3 + 1] = make_float4(b[0] - a[0], b[1] - a[1], b[2] - a[2], 0.f);
-    tris[i * 3 + 2] = make_float4(c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.f);
+  for (u32 i = 0; i < nt; i++) {
+    u32 o = s->bvh.dreorder[i];
     tv[i] = make_uint4(s->idx[o * 3], s->idx[o * 3 + 1], s->idx[o * 3 + 2], s->mat[o]);
   }
   std::vector<float4> vp(s->nv), vn(s->nv);
@@ -310,6 +326,7 @@ int cpt_scene_upload(cpt_scene* s) {
   HIP_TRY(s->d_nodes4.upload(s->bvh.nodes4.data(), s->bvh.nodes4.size(), s->stream));
   HIP_TRY(s->d_nodes4q.upload(s->bvh.nodes4q.data(), s->bvh.nodes4q.size(), s->stream));
   HIP_TRY(s->d_tris.upload(tris.data(), tris.size(), s->stream));
+  HIP_TRY(s->d_tris_bin.upload(tris_bin.data(), tris_bin.size(), s->stream));
   HIP_TRY(s->d_tri_verts.upload(tv.data(), tv.size(), s->stream));
   HIP_TRY(s->d_vpos.upload(vp.data(), vp.size(), s->stream));
   HIP_TRY(s->d_vnrm.upload(vn.data(), vn.size(), s->stream));

@@ -56,6 +56,8 @@ static_assert(sizeof(Node4Q) == 64, "Node4Q must be 64 bytes");
 struct BvhHost {
   std::vector<BinNode> nodes;      // DFS preorder, reference CompactBVH order
   std::vector<u32> reorder;        // leaf order -> original triangle id
+  std::vector<BinNode> dnodes;     // the device tree's binary SAH build, same format
+  std::vector<u32> dreorder;       // device leaf order -> original triangle id
   std::vector<Node4> nodes4;       // 4-wide re-layout (BFS order, root 0)
   std::vector<Node4Q> nodes4q;     // the same nodes quantized to 64 bytes
   u32 depth4 = 0, leaves4 = 0;
@@ -63,7 +65,9 @@ struct BvhHost {
 
 // SAH build with NPBVHHelper semantics (src/bvhhelper.cpp:55-311).
 bool build_bvh_sah(const float* pos3, const u32* idx3, u32 nt, u32 max_depth, BvhHost& out);
-// Collapse the binary tree into 4-wide nodes; subtrees of <= leaf_max
+// The device tree: binned SAH over triangle bounds (dnodes / dreorder).
+bool build_bvh_device(const float* pos3, const u32* idx3, u32 nt, BvhHost& out);
+// Collapse the device binary tree into 4-wide nodes; subtrees of <= leaf_max
 // triangles become one leaf range.
 void collapse_bvh4(const float* pos3, const u32* idx3, BvhHost& b, u32 leaf_max);
 // Quantize nodes4 into nodes4q; false when a node cannot be encoded.

@@ -30,6 +30,7 @@ struct DevScene {
   uint32_t n_tris;
   const float4* bnodes;        // reference binary layout: 2 per node (min, max)
   const uint2* boff;           //   (offset-or-triStart, triN)
+  const float4* tris_bin;      //   its triangles, in the reference tree's leaf order
 };
 
 // Per-path SoA state of the wavefront (one slot per live eye path).

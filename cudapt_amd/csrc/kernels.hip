@@ -303,7 +303,7 @@ __device__ bool traverse_bin(const DevScene& sc, f3 o, f3 d, float maxd, float e
       if (mn.x > mx.x) continue;                     // empty leaf of a degenerate split
       if (sp < 62) { stk[++sp] = n + 1; stk[++sp] = n + off.x; }
     } else {
-      test_leaf(sc.tris, off.x, off.y, o2, d, cull, h, nt);
+      test_leaf(sc.tris_bin, off.x, off.y, o2, d, cull, h, nt);
       if (ANY && h.leaf != kInvalid) return true;
     }
   }
