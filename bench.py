@@ -232,13 +232,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--frames", type=int, default=SPP)
     ap.add_argument("--in-flight", type=int, default=64, help="frames traced as one wavefront")
-    ap.add_argument("--lds-stack", type=int, default=807, help="traversal variant: LDS stack entries (8/16/24; "
+    ap.add_argument("--lds-stack", type=int, default=806, help="traversal variant: LDS stack entries (8/16/24; "
                     "806/807/808/1606 = 8 or 16 entries with >= 6, 7 or 8 waves per SIMD)")
-    ap.add_argument("--lds-stack-shadow", type=int, default=808, help="the same for shadow rays (0: as --lds-stack)")
+    ap.add_argument("--lds-stack-shadow", type=int, default=806, help="the same for shadow rays (0: as --lds-stack)")
     ap.add_argument("--node-format", type=int, default=1, help="BVH nodes: 0 = 128-byte FP32, 1 = 64-byte quantized")
     ap.add_argument("--refill", type=int, default=64,
                     help="traversal lanes refill from a ray queue once this many of a wave's are idle (0: off)")
-    ap.add_argument("--refill-shadow", type=int, default=0, help="the same for shadow rays")
+    ap.add_argument("--refill-shadow", type=int, default=16, help="the same for shadow rays")
     ap.add_argument("--refill-groups", type=int, default=8, help="workgroups per CU of the refilling grid")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",

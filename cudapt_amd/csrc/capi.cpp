@@ -1079,11 +1079,11 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   }
   // traversal kernel variant: 8 LDS stack entries, >= 6 waves per SIMD
   // (swept on the sphere grid: 1.58 vs 1.36 Grays/s for 16 entries)
-  r->attribs.push_back({kLdsStack, 0, 807});
-  r->attribs.push_back({kLdsStackShadow, 0, 808});
+  r->attribs.push_back({kLdsStack, 0, 806});
+  r->attribs.push_back({kLdsStackShadow, 0, 806});
   r->attribs.push_back({kNodeFormat, 0, 1});
   r->attribs.push_back({kRefill, 0, 64});
-  r->attribs.push_back({kRefillShadow, 0, 0});
+  r->attribs.push_back({kRefillShadow, 0, 16});
   r->attribs.push_back({kRefillGroups, 0, 8});
   int rc = ensure_stream(r->stream);
   if (rc) { delete r; return rc; }
@@ -1448,7 +1448,7 @@ struct Schedule {
 };
 const Schedule kSchedules[] = {{0, 0, 8},     {256, 0, 8},  {1024, 0, 8}, {4096, 0, 8}, {1024, 0, 4},
                                {1024, 0, 16}, {0, 64, 6},   {0, 64, 8},   {0, 48, 8},   {0, 32, 8}};
-const u32 kVariants[] = {807u, 806u};
+const u32 kVariants[] = {806u, 807u};
 
 cpt_batch_config default_batch_config() {
   cpt_batch_config c;
@@ -1458,7 +1458,7 @@ cpt_batch_config default_batch_config() {
   c.extend_workgroups = 8;
   c.refill = 64;
   c.node_format = 1;
-  c.traversal_variant = 807;
+  c.traversal_variant = 806;
   c.use_graph = 1;
   c.eye_depth = 5;
   c.light_depth = 5;

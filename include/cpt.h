@@ -244,7 +244,7 @@ typedef struct cpt_batch_config {
   uint32_t extend_chunk;      /* 0: one thread per ray; else rays a workgroup takes from the global queue */
   uint32_t extend_workgroups; /* persistent grids: workgroups per compute unit (extend_chunk or refill > 0) */
   uint32_t traversal_variant; /* extension-ray traversal variant: LDS stack entries and minimum waves per
-                                 SIMD: 807 (8 entries, >= 7 waves), 806, 808, 1606, 8, 16, 24 */
+                                 SIMD: 806 (8 entries, >= 6 waves), 807, 808, 1606, 8, 16, 24 */
   uint32_t use_graph;         /* 1: capture one iteration as a hipGraph and replay it */
   uint32_t eye_depth, light_depth, light_paths;
   uint64_t seed;
@@ -254,9 +254,9 @@ typedef struct cpt_batch_config {
 } cpt_batch_config;
 
 /* Defaults: 24 frames in flight, refill 64 with 8 workgroups per CU, variant
- * 807 (shadow rays: the renderer's "Shadow LDS Stack", 808), quantized nodes,
- * graph on, depths 5/5, 128 light paths, seed 0x5EED0001 (the plain
- * renderer's). */
+ * 806 (shadow rays: the renderer's "Shadow LDS Stack" 806 and "Shadow Refill"
+ * 16), quantized nodes, graph on, depths 5/5, 128 light paths, seed
+ * 0x5EED0001 (the plain renderer's). */
 int cpt_batch_create(cpt_batch** out, const cpt_view* views, uint32_t n_views);
 int cpt_batch_get_config(cpt_batch* b, cpt_batch_config* cfg);
 /* Restarts accumulation. */

@@ -154,6 +154,8 @@ def test_refill_traversal_same_image(cpt, cornell_runs, variant, node_format, re
         R.set_attrib("Node Format", node_format)
         R.set_attrib("Traversal LDS Stack", variant)
         R.set_attrib("Shadow LDS Stack", variant)
+    a.set_attrib("Traversal Refill", 0)
+    a.set_attrib("Shadow Refill", 0)
     b.set_attrib("Traversal Refill", refill)
     b.set_attrib("Shadow Refill", 65 - refill)
     ra = a.render(S, sc["camera"], frames=6)
