@@ -491,6 +491,10 @@ const char* const kRefill = "Traversal Refill";
 const char* const kRefillShadow = "Shadow Refill";
 // workgroups per compute unit of the refilling grid
 const char* const kRefillGroups = "Refill Workgroups";
+// path capacity below which rays go one thread per ray whatever the refill
+// settings: a grid that can hold the whole launch gains nothing from refilling
+// (one frame of 800 x 600 per call: shadow 10 % faster one thread per ray)
+const char* const kRefillMinPaths = "Refill Min Paths";
 // allocation signal from the learned model's output y (log per-sample
 // variance): 0 = nd / (nd + N) * exp(y) / N, the filter-residual estimate's
 // shape; 1 = exp(y) / N, the expected squared error (samples settle where N
@@ -726,12 +730,13 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
   PathSet* nxt = &r->B;
   ShadowBuf sb;
   sb.ray_o = r->sh_o.p; sb.ray_d = r->sh_d.p; sb.contrib = r->sh_c.p;
-  const uint32_t refill_sh = (uint32_t)std::min(64.0, std::max(0.0, r->attr(kRefillShadow)));
+  const bool big = (double)max_n >= r->attr(kRefillMinPaths);
+  const uint32_t refill_sh = big ? (uint32_t)std::min(64.0, std::max(0.0, r->attr(kRefillShadow))) : 0u;
   for (u32 b = 0; b < fa.eye_depth; b++) {
     r->extend_launches++;
     HIP_TRY(timed_begin(r, 0, &ev));
     const int quant = r->attr(kNodeFormat) != 0;
-    const uint32_t refill = (uint32_t)std::min(64.0, std::max(0.0, r->attr(kRefill)));
+    const uint32_t refill = big ? (uint32_t)std::min(64.0, std::max(0.0, r->attr(kRefill))) : 0u;
     const uint32_t dyn_grid =
         (uint32_t)(r->cus > 0 ? r->cus : 256) * (uint32_t)std::min(32.0, std::max(1.0, r->attr(kRefillGroups)));
     if (refill)
@@ -1084,6 +1089,7 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   r->attribs.push_back({kNodeFormat, 0, 1});
   r->attribs.push_back({kRefill, 0, 64});
   r->attribs.push_back({kRefillShadow, 0, 16});
+  r->attribs.push_back({kRefillMinPaths, 0, (double)(1u << 20)});
   r->attribs.push_back({kRefillGroups, 0, 8});
   int rc = ensure_stream(r->stream);
   if (rc) { delete r; return rc; }
@@ -1493,6 +1499,7 @@ void batch_apply_config(cpt_batch* b) {
   r->set_attr(kNodeFormat, b->cfg.node_format ? 1 : 0);
   r->set_attr(kRefill, b->cfg.refill);
   r->set_attr(kRefillGroups, b->cfg.extend_workgroups);
+  r->set_attr(kRefillMinPaths, 0);               // the batch's schedule applies at every size
   r->persist_chunk = b->cfg.refill ? 0u : b->cfg.extend_chunk;
   r->persist_grid = r->persist_chunk ? b->cfg.extend_workgroups * (u32)(b->cus > 0 ? b->cus : 256) : 0u;
   b->drop_graph();

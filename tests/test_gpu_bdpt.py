@@ -158,6 +158,7 @@ def test_refill_traversal_same_image(cpt, cornell_runs, variant, node_format, re
     a.set_attrib("Shadow Refill", 0)
     b.set_attrib("Traversal Refill", refill)
     b.set_attrib("Shadow Refill", 65 - refill)
+    b.set_attrib("Refill Min Paths", 0)
     ra = a.render(S, sc["camera"], frames=6)
     rb = b.render(S, sc["camera"], frames=6)
     assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
