@@ -498,7 +498,8 @@ const char* const kRefillMinPaths = "Refill Min Paths";
 // allocation signal from the learned model's output y (log per-sample
 // variance): 0 = nd / (nd + N) * exp(y) / N, the filter-residual estimate's
 // shape; 1 = exp(y) / N, the expected squared error (samples settle where N
-// grows with the per-sample standard deviation)
+// grows with the per-sample standard deviation); 2 = exp(y) / N^2, the error
+// one more sample removes (sigma^2 / N - sigma^2 / (N + 1) ~ sigma^2 / N^2)
 const char* const kLearnedSignal = "Learned Signal";
 const char* const kLdsStack = "Traversal LDS Stack";
 const char* const kLdsStackShadow = "Shadow LDS Stack";

@@ -422,7 +422,8 @@ __global__ __launch_bounds__(256) void k_mlp_err(const float* __restrict__ y, co
   if (i >= n) return;
   const uint32_t N = max(sample_n[i], 1u);
   const float v = expf(fminf(fmaxf(y[i], -60.f), 60.f));
-  e[i] = signal ? v / (float)N : (float)nd / (float)(nd + N) * v / (float)N;
+  const float fn = (float)N;
+  e[i] = signal == 2 ? v / fn / fn : signal ? v / fn : (float)nd / (float)(nd + N) * v / fn;
 }
 
 hipError_t launch_mlp_features(const MlpFeatureArgs& a, uint32_t n, float* x, hipStream_t s) {

@@ -179,7 +179,8 @@ def target(result, ref, sample_n, clamp: float = 0.0) -> np.ndarray:
 
 
 def learned_error(y, sample_n, nd: int, signal: int = 0) -> np.ndarray:
-    """Allocation signal nd / (nd + N) * exp(y) / N (signal 0) or exp(y) / N (1)."""
+    """Allocation signal nd / (nd + N) * exp(y) / N (signal 0), exp(y) / N (1)
+    or exp(y) / N^2 (2)."""
     N = np.maximum(np.asarray(sample_n, np.float64).ravel(), 1.0)
     v = np.exp(np.clip(np.asarray(y, np.float64), -60, 60)) / N
-    return v if signal else nd / (nd + N) * v
+    return v / N if signal == 2 else v if signal else nd / (nd + N) * v

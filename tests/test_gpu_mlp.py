@@ -179,11 +179,12 @@ def test_learned_allocation_follows_model(cpt, adaptive_state):
     R.set_mlp(None)
 
 
-def test_expected_error_signal(cpt):
+@pytest.mark.parametrize("signal", [1, 2])
+def test_expected_error_signal(cpt, signal):
     sc = scenes.cornell()
     S = cpt.Scene(sc)
     R = cpt.Renderer(40, 30, mode=cpt.CPT_MODE_BDPT_ADAPTIVE_PLL)
-    R.set_attrib("Learned Signal", 1)
+    R.set_attrib("Learned Signal", signal)
     R.render(S, sc["camera"], frames=6)
     dx = cpt.DeviceBuffer(40 * 30 * 36)
     R.mlp_features(dx)
@@ -193,4 +194,4 @@ def test_expected_error_signal(cpt):
     R.set_mlp(m)
     R.render(S, sc["camera"], frames=1)
     sig = R.read(cpt.CPT_BUF_ALLOC_ERROR).ravel()
-    np.testing.assert_allclose(sig, M.learned_error(y, N, 5, signal=1), rtol=1e-5, atol=0)
+    np.testing.assert_allclose(sig, M.learned_error(y, N, 5, signal=signal), rtol=1e-5, atol=0)

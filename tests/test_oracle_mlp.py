@@ -125,3 +125,7 @@ def test_feature_target_and_signal_formulas():
     assert abs(t[0] - np.log(1.0 + 1.5 ** 2 + 1e-12)) < 1e-12   # r -> (1, 1.5, 1.5), ref -> (0, 1.5, 0)
     s = M.learned_error(np.array([0.0, np.log(8.0)]), np.array([5, 2]), 5)
     np.testing.assert_allclose(s, [0.5 / 5, 5 / 7 * 8 / 2])
+    np.testing.assert_allclose(M.learned_error(np.array([0.0, np.log(8.0)]), np.array([5, 2]), 5, signal=1),
+                               [1 / 5, 8 / 2])
+    np.testing.assert_allclose(M.learned_error(np.array([0.0, np.log(8.0)]), np.array([5, 2]), 5, signal=2),
+                               [1 / 25, 8 / 4])
