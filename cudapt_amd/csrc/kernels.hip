@@ -134,15 +134,17 @@ __device__ __forceinline__ void load_node(const DevScene& sc, uint32_t node, Nod
     const float ox = __uint_as_float(A.x), oy = __uint_as_float(A.y), oz = __uint_as_float(A.z);
     const float sx = __uint_as_float((A.w & 0xFFu) << 23), sy = __uint_as_float(((A.w >> 8) & 0xFFu) << 23);
     const float sz = __uint_as_float(((A.w >> 16) & 0xFFu) << 23);
+    // q * 2^k is exact for 8-bit q, so fma(q, step, o) rounds once, exactly
+    // as o + q * step does: the same bits as the host's containment check
 #pragma unroll
     for (int c = 0; c < 4; c++) {
       const int sh = 8 * c;
-      v.lx[c] = ox + (float)((B.x >> sh) & 0xFFu) * sx;
-      v.hx[c] = ox + (float)((B.y >> sh) & 0xFFu) * sx;
-      v.ly[c] = oy + (float)((B.z >> sh) & 0xFFu) * sy;
-      v.hy[c] = oy + (float)((B.w >> sh) & 0xFFu) * sy;
-      v.lz[c] = oz + (float)((C.x >> sh) & 0xFFu) * sz;
-      v.hz[c] = oz + (float)((C.y >> sh) & 0xFFu) * sz;
+      v.lx[c] = __builtin_fmaf((float)((B.x >> sh) & 0xFFu), sx, ox);
+      v.hx[c] = __builtin_fmaf((float)((B.y >> sh) & 0xFFu), sx, ox);
+      v.ly[c] = __builtin_fmaf((float)((B.z >> sh) & 0xFFu), sy, oy);
+      v.hy[c] = __builtin_fmaf((float)((B.w >> sh) & 0xFFu), sy, oy);
+      v.lz[c] = __builtin_fmaf((float)((C.x >> sh) & 0xFFu), sz, oz);
+      v.hz[c] = __builtin_fmaf((float)((C.y >> sh) & 0xFFu), sz, oz);
     }
     v.cn[0] = C.z & 0xFFFFu; v.cn[1] = C.z >> 16; v.cn[2] = C.w & 0xFFFFu; v.cn[3] = C.w >> 16;
     v.ch[0] = D.x; v.ch[1] = D.y; v.ch[2] = D.z; v.ch[3] = D.w;
