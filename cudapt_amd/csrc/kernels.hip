@@ -186,9 +186,6 @@ __device__ __forceinline__ void ray_begin(RayState& R, f3 o, f3 d, float maxd, f
   R.node = 0;
   R.sp = 0;
 }
-__device__ __forceinline__ uint32_t pick4(const uint32_t (&a)[4], uint32_t c) {
-  return c == 0 ? a[0] : c == 1 ? a[1] : c == 2 ? a[2] : a[3];
-}
 // One node: the four child boxes against the ray (culled beyond the best hit
 // plus slack); the leaf children hit are then tested as one run of triangles
 // (child order, triangle order), so that lanes whose leaves sit under different
@@ -220,23 +217,21 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool c
     ln[c] = hit && leaf ? nv.cn[c] : 0u;
     key[c] = hit && !leaf ? ((uint64_t)__float_as_uint(tnear) << 32) | nv.ch[c] : ~0ull;
   }
-  const uint32_t ntri = ln[0] + ln[1] + ln[2] + ln[3];
+  // run position k lies in child c when p_c <= k < p_(c+1) (prefix sums of the
+  // counts); its triangle is k + (first_c - p_c): selects, no branches
+  const uint32_t p1 = ln[0], p2 = p1 + ln[1], p3 = p2 + ln[2];
+  const uint32_t ntri = p3 + ln[3];
   if (ntri) {
     nt += ntri;
-    uint32_t c = 0, i = lf[0], e = lf[0] + ln[0];
+    const uint32_t d0 = lf[0], d1 = lf[1] - p1, d2 = lf[2] - p2, d3 = lf[3] - p3;
     for (uint32_t k = 0; k < ntri; k++) {
-      while (i == e) {
-        c++;
-        i = pick4(lf, c);
-        e = i + pick4(ln, c);
-      }
+      const uint32_t i = k + (k < p1 ? d0 : k < p2 ? d1 : k < p3 ? d2 : d3);
       const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
       float u = 0.f, v = 0.f;
       const float t = isect_tri(R.o2, R.d, ld3(a), ld3(b), ld3(cc), cull, u, v);
       const uint32_t oid = __float_as_uint(a.w);
       if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i; R.h.orig = oid; }
       if (ANY && R.h.leaf != kInvalid) return true;
-      i++;
     }
     const float lim2 = R.h.t + cull_slack(R.h.t, R.eps);
 #pragma unroll
