@@ -165,6 +165,14 @@ __device__ __forceinline__ void load_node(const DevScene& sc, uint32_t node, Nod
     default: return hipErrorInvalidValue;   \
   }
 
+// The top of a thread's traversal stack: its column of a [entries][kBlock] u64
+// array in LDS, typed as LDS so that the pop (LDS or private, by depth) stays
+// two plain accesses instead of one generic one.
+typedef __attribute__((address_space(3))) uint64_t LdsU64;
+__device__ __forceinline__ LdsU64* lds_column(uint64_t* shared_base) {
+  return (LdsU64*)(shared_base + threadIdx.x);
+}
+
 // ---- 4-wide traversal ------------------------------------------------------
 // The state of one ray's traversal, advanced one node at a time by ray_step, so
 // that a lane whose ray has finished can start another while the rest of its
@@ -196,7 +204,7 @@ __device__ __forceinline__ void ray_begin(RayState& R, f3 o, f3 d, float maxd, f
 // culling changes.  Returns true when the ray is finished.
 // lstk points at this thread's column of a [STK][kBlock] u64 array in LDS.
 template <bool ANY, int STK, bool Q>
-__device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool cull, uint64_t* lstk, uint64_t* pstk,
+__device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool cull, LdsU64* lstk, uint64_t* pstk,
                                          uint32_t& nn, uint32_t& nt) {
   NodeView nv;
   load_node<Q>(sc, R.node, nv);
@@ -294,7 +302,7 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool c
 // Closest (ANY: any) hit of one ray, the loop of ray_step.
 // Q: nodes in the 64-byte quantized layout.
 template <bool ANY, int STK = CPT_LDS_STACK, bool Q = false>
-__device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps, bool cull, uint64_t* lstk,
+__device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps, bool cull, LdsU64* lstk,
                           TraceHit& h, uint32_t& nn, uint32_t& nt) {
   RayState R;
   ray_begin(R, o, d, maxd, eps);
@@ -365,8 +373,8 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace(DevScene sc, const float
   if (i < count) {
     float4 a = ro[(size_t)i * stride], b = rd[(size_t)i * stride];
     TraceHit h;
-    bool hit = LAYOUT == 0   ? traverse4<ANY, STK>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, lstk + threadIdx.x, h, nn, nt)
-               : LAYOUT == 2 ? traverse4<ANY, STK, true>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, lstk + threadIdx.x, h,
+    bool hit = LAYOUT == 0   ? traverse4<ANY, STK>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, lds_column(lstk), h, nn, nt)
+               : LAYOUT == 2 ? traverse4<ANY, STK, true>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, lds_column(lstk), h,
                                                          nn, nt)
                              : traverse_bin<ANY>(sc, ld3(a), ld3(b), a.w, b.w, cull != 0, h, nn, nt);
     if (OUT == 0) {
@@ -696,7 +704,7 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(DevScene sc, FrameArgs fa
       uint32_t slot = cur + pre + below;
       TraceHit h;
       uint32_t nn = 0, nt = 0;
-      if (!traverse4<false>(sc, orig, dir, kInfF, (float)kBiasEps, false, lstk + x, h, nn, nt)) {
+      if (!traverse4<false>(sc, orig, dir, kInfF, (float)kBiasEps, false, (LdsU64*)(lstk + x), h, nn, nt)) {
         lv[slot] = lv[x];
         live = false;
       } else {
@@ -890,7 +898,7 @@ __global__ __launch_bounds__(kBlock, MINW) void k_extend_persist(DevScene sc, co
     for (uint32_t i = base + threadIdx.x; i < end; i += kBlock) {
       float4 a = ro[i], b = rd[i];
       TraceHit h;
-      bool hit = traverse4<false, STK, Q>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt);
+      bool hit = traverse4<false, STK, Q>(sc, ld3(a), ld3(b), a.w, b.w, false, lds_column(lstk), h, nn, nt);
       float4 r;
       r.x = hit ? h.t : kInfF;
       r.y = __uint_as_float(hit ? h.leaf : kInvalid);
@@ -951,7 +959,7 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
                                                             DevStats* __restrict__ stats, uint32_t refill) {
   constexpr bool ANY = OUT == 3;
   __shared__ uint64_t lstk_all[STK * kBlock];
-  uint64_t* lstk = lstk_all + threadIdx.x;
+  LdsU64* lstk = lds_column(lstk_all);
   uint64_t pstk[CPT_STACK_TOTAL - STK];
   const uint32_t count = *countp;
   const int lane = threadIdx.x & 63;
@@ -1263,7 +1271,7 @@ __global__ __launch_bounds__(kBlock, MINW) void k_shadow(DevScene sc, ShadowBuf 
   if (i < count) {
     float4 a = sb.ray_o[i], b = sb.ray_d[i], c = sb.contrib[i];
     TraceHit h;
-    if (!traverse4<true, STK, Q>(sc, ld3(a), ld3(b), a.w, b.w, false, lstk + threadIdx.x, h, nn, nt)) {
+    if (!traverse4<true, STK, Q>(sc, ld3(a), ld3(b), a.w, b.w, false, lds_column(lstk), h, nn, nt)) {
       uint32_t slot = __float_as_uint(c.w);
       float4 acc = nxt.acc[slot];
       acc.x = acc.x + c.x;
