@@ -67,6 +67,8 @@ def lib():
             "cpt_bvh_export": [vp, vp, vp, vp, vp],
             "cpt_bvh4_info": [vp, vp, vp, vp],
             "cpt_bvh4_set_leaf_size": [vp, c32],
+            "cpt_scene_add_texture": [vp, c32, c32, vp, vp],
+            "cpt_scene_clear_textures": [vp],
             "cpt_scene_upload": [vp],
             "cpt_scene_destroy": [vp],
             "cpt_trace_closest": [vp, vp, c32, c32, vp],
@@ -181,8 +183,10 @@ class Scene:
         _check(L.cpt_scene_create(ctypes.byref(h)), "cpt_scene_create")
         self.h = h
         if sc is not None:
-            self.set_mesh(sc["pos"], sc.get("nrm"), sc["idx"], sc["mat"])
+            self.set_mesh(sc["pos"], sc.get("nrm"), sc["idx"], sc["mat"], sc.get("uv"))
             self.set_materials(sc["mats"])
+            for t in sc.get("textures", ()):
+                self.add_texture(t)
             self.build_bvh(max_depth)
             if upload:
                 self.upload()
@@ -192,14 +196,27 @@ class Scene:
             _lib.cpt_scene_destroy(self.h)
             self.h = None
 
-    def set_mesh(self, pos, nrm, idx, mat):
+    def set_mesh(self, pos, nrm, idx, mat, uv=None):
         self._pos = np.ascontiguousarray(pos, np.float32)
         self._nrm = None if nrm is None else np.ascontiguousarray(nrm, np.float32)
+        self._uv = None if uv is None else np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        if self._uv is not None and len(self._uv) != len(self._pos):
+            raise CptError("uv: one pair per vertex")
         self._idx = np.ascontiguousarray(idx, np.uint32)
         self._mat = np.ascontiguousarray(mat, np.uint32)
         self.nt = len(self._idx)
-        _check(lib().cpt_scene_set_mesh(self.h, _ptr(self._pos), _ptr(self._nrm), None, len(self._pos),
+        _check(lib().cpt_scene_set_mesh(self.h, _ptr(self._pos), _ptr(self._nrm), _ptr(self._uv), len(self._pos),
                                         _ptr(self._idx), _ptr(self._mat), self.nt), "cpt_scene_set_mesh")
+
+    def add_texture(self, rgba) -> int:
+        """Appends an H x W x 4 float texture (RGBA); returns its id."""
+        t = np.ascontiguousarray(rgba, np.float32)
+        if t.ndim != 3 or t.shape[2] != 4:
+            raise CptError("texture: H x W x 4 floats")
+        i = ctypes.c_uint32()
+        _check(lib().cpt_scene_add_texture(self.h, t.shape[1], t.shape[0], _ptr(t), ctypes.byref(i)),
+               "cpt_scene_add_texture")
+        return i.value
 
     def set_materials(self, mats):
         self._mats = np.ascontiguousarray(mats)

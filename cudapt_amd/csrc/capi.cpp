@@ -68,6 +68,8 @@ struct DevArray {
 
 struct cpt_scene {
   std::vector<float> pos, nrm, uv;
+  std::vector<float4> tex;              // RTScene::m_pTextures' texels (RGBA), concatenated
+  std::vector<uint4> tex_desc;          // first texel, width, height
   std::vector<u32> idx, mat;
   std::vector<cpt_material> mats;
   u32 nv = 0, nt = 0;
@@ -81,7 +83,8 @@ struct cpt_scene {
   hipStream_t stream = nullptr;
   DevArray<Node4> d_nodes4;
   DevArray<Node4Q> d_nodes4q;
-  DevArray<float4> d_tris, d_tris_bin, d_vpos, d_vnrm, d_bnodes;
+  DevArray<float4> d_tris, d_tris_bin, d_vpos, d_vnrm, d_bnodes, d_tex;
+  DevArray<uint4> d_tex_desc;
   DevArray<uint4> d_tri_verts;
   DevArray<uint2> d_boff;
   DevArray<DevMaterial> d_mats;
@@ -104,6 +107,9 @@ struct cpt_scene {
     s.bnodes = d_bnodes.p;
     s.boff = d_boff.p;
     s.tris_bin = d_tris_bin.p;
+    s.tex = d_tex.p;
+    s.tex_desc = d_tex_desc.p;
+    s.n_tex = (u32)tex_desc.size();
     return s;
   }
   ~cpt_scene() {
@@ -206,6 +212,31 @@ int cpt_scene_set_mesh(cpt_scene* s, const float* pos3, const float* nrm3, const
   s->has_bvh = false;
   s->uploaded = false;
   s->mesh_version++;
+  return CPT_OK;
+}
+
+int cpt_scene_add_texture(cpt_scene* s, uint32_t width, uint32_t height, const float* rgba, uint32_t* id) {
+  if (!s || !rgba) return fail(CPT_E_ARG, "cpt_scene_add_texture: null argument");
+  if (width == 0 || height == 0 || width > 65536 || height > 65536)
+    return fail(CPT_E_ARG, "cpt_scene_add_texture: size %ux%u not in 1..65536", width, height);
+  const size_t n = (size_t)width * height;
+  if (s->tex.size() + n > 0xFFFFFFFFull) return fail(CPT_E_LIMIT, "cpt_scene_add_texture: more than 2^32 texels");
+  const u32 first = (u32)s->tex.size();
+  s->tex.resize(s->tex.size() + n);
+  std::memcpy(&s->tex[first], rgba, n * sizeof(float4));
+  s->tex_desc.push_back(make_uint4(first, width, height, 0u));
+  if (id) *id = (u32)s->tex_desc.size() - 1;
+  s->uploaded = false;
+  s->mat_version++;                             // shading changes: renderers start over
+  return CPT_OK;
+}
+
+int cpt_scene_clear_textures(cpt_scene* s) {
+  if (!s) return fail(CPT_E_ARG, "cpt_scene_clear_textures: null scene");
+  s->tex.clear();
+  s->tex_desc.clear();
+  s->uploaded = false;
+  s->mat_version++;
   return CPT_OK;
 }
 
@@ -312,8 +343,8 @@ int cpt_scene_upload(cpt_scene* s) {
   }
   std::vector<float4> vp(s->nv), vn(s->nv);
   for (u32 i = 0; i < s->nv; i++) {
-    vp[i] = make_float4(s->pos[i * 3], s->pos[i * 3 + 1], s->pos[i * 3 + 2], 0.f);
-    vn[i] = make_float4(s->nrm[i * 3], s->nrm[i * 3 + 1], s->nrm[i * 3 + 2], 0.f);
+    vp[i] = make_float4(s->pos[i * 3], s->pos[i * 3 + 1], s->pos[i * 3 + 2], s->uv[i * 2]);
+    vn[i] = make_float4(s->nrm[i * 3], s->nrm[i * 3 + 1], s->nrm[i * 3 + 2], s->uv[i * 2 + 1]);
   }
   const auto& bn = s->bvh.nodes;
   std::vector<float4> bnodes(bn.size() * 2);
@@ -331,6 +362,8 @@ int cpt_scene_upload(cpt_scene* s) {
   HIP_TRY(s->d_vpos.upload(vp.data(), vp.size(), s->stream));
   HIP_TRY(s->d_vnrm.upload(vn.data(), vn.size(), s->stream));
   HIP_TRY(s->d_bnodes.upload(bnodes.data(), bnodes.size(), s->stream));
+  HIP_TRY(s->d_tex.upload(s->tex.data(), s->tex.size(), s->stream));
+  HIP_TRY(s->d_tex_desc.upload(s->tex_desc.data(), s->tex_desc.size(), s->stream));
   HIP_TRY(s->d_boff.upload(boff.data(), boff.size(), s->stream));
   rc = upload_materials(s);
   if (rc) return rc;

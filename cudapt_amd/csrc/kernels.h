@@ -22,8 +22,8 @@ struct DevScene {
   const Node4Q* nodes4q;       // the same nodes, 64-byte quantized
   const float4* tris;          // 3 per triangle: p0 (w = original id bits), e0 = p1-p0, e1 = p2-p0
   const uint4* tri_verts;      // v0, v1, v2, material
-  const float4* vpos;          // vertex positions
-  const float4* vnrm;          // vertex normals
+  const float4* vpos;          // vertex positions, w = texture u
+  const float4* vnrm;          // vertex normals, w = texture v
   const DevMaterial* mats;
   const uint32_t* light_tris;  // leaf-order emissive triangles
   uint32_t n_light_tris;
@@ -31,6 +31,9 @@ struct DevScene {
   const float4* bnodes;        // reference binary layout: 2 per node (min, max)
   const uint2* boff;           //   (offset-or-triStart, triN)
   const float4* tris_bin;      //   its triangles, in the reference tree's leaf order
+  const float4* tex;           // texels (RGBA) of every texture, concatenated
+  const uint4* tex_desc;       // per texture: first texel, width, height
+  uint32_t n_tex;
 };
 
 // Per-path SoA state of the wavefront (one slot per live eye path).

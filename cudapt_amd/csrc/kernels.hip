@@ -482,21 +482,71 @@ struct Surf {
   f3 norm, pos, diff, emissive;
   float trans, spec, metal, rough;
 };
+// tex2D of a float4 texture with normalized coordinates, wrap addressing and
+// linear filtering (the reference's texture objects, src/cudaRTCommon.cu:416-441):
+// the coordinate wraps to [0, 1), scales to texels, and the four texels around
+// x - 0.5, y - 0.5 (indices wrapped) are blended in x, then in y.
+__device__ __forceinline__ float4 tex_sample(const DevScene& sc, uint32_t id, float u, float v) {
+  const uint4 d = sc.tex_desc[id];
+  const int W = (int)d.y, H = (int)d.z;
+  if (!(fabsf(u) < 1e30f)) u = 0.f;                 // NaN and infinities sample texel (0, 0)
+  if (!(fabsf(v) < 1e30f)) v = 0.f;
+  const float fu = u - floorf(u), fv = v - floorf(v);
+  const float x = fu * (float)W - 0.5f, y = fv * (float)H - 0.5f;
+  const float fx = floorf(x), fy = floorf(y);
+  const float a = x - fx, b = y - fy;
+  int i0 = min(max((int)fx, -1), W - 1), j0 = min(max((int)fy, -1), H - 1);
+  if (i0 < 0) i0 += W;
+  if (j0 < 0) j0 += H;
+  const int i1 = i0 + 1 == W ? 0 : i0 + 1, j1 = j0 + 1 == H ? 0 : j0 + 1;
+  const float4 t00 = sc.tex[d.x + (uint32_t)(j0 * W + i0)], t10 = sc.tex[d.x + (uint32_t)(j0 * W + i1)];
+  const float4 t01 = sc.tex[d.x + (uint32_t)(j1 * W + i0)], t11 = sc.tex[d.x + (uint32_t)(j1 * W + i1)];
+  const float ia = 1.f - a, ib = 1.f - b;
+  float4 r;
+  r.x = (t00.x * ia + t10.x * a) * ib + (t01.x * ia + t11.x * a) * b;
+  r.y = (t00.y * ia + t10.y * a) * ib + (t01.y * ia + t11.y * a) * b;
+  r.z = (t00.z * ia + t10.z * a) * ib + (t01.z * ia + t11.z * a) * b;
+  r.w = (t00.w * ia + t10.w * a) * ib + (t01.w * ia + t11.w * a) * b;
+  return r;
+}
+
+// Shading inputs at a hit: interpolated normal (and position), material, and
+// GetMaterialColors' textures (src/cudaRTCommon.cu:369-406): a diffuse texture
+// gives the colour and scales transparency by its alpha, an emissive texture
+// gives the emission; texture ids outside the scene's textures mean none.
 __device__ __forceinline__ Surf fetch_surf(const DevScene& sc, uint32_t leaf, float w, float u, float v, bool want_pos) {
   Surf s;
   uint4 tv = sc.tri_verts[leaf];
   const DevMaterial& m = sc.mats[tv.w];
-  f3 n0 = ld3(sc.vnrm[tv.x]), n1 = ld3(sc.vnrm[tv.y]), n2 = ld3(sc.vnrm[tv.z]);
+  const float4 r0 = sc.vnrm[tv.x], r1 = sc.vnrm[tv.y], r2 = sc.vnrm[tv.z];
+  f3 n0 = ld3(r0), n1 = ld3(r1), n2 = ld3(r2);
   s.norm = (n0 * w + n1 * u) + n2 * v;
-  if (want_pos) {
-    f3 q0 = ld3(sc.vpos[tv.x]), q1 = ld3(sc.vpos[tv.y]), q2 = ld3(sc.vpos[tv.z]);
-    s.pos = (q0 * w + q1 * u) + q2 * v;
+  const bool dtex = (uint32_t)m.diffuseTexId < sc.n_tex, etex = (uint32_t)m.emissiveTexId < sc.n_tex;
+  if (want_pos || dtex || etex) {
+    const float4 p0 = sc.vpos[tv.x], p1 = sc.vpos[tv.y], p2 = sc.vpos[tv.z];
+    f3 q0 = ld3(p0), q1 = ld3(p1), q2 = ld3(p2);
+    s.pos = want_pos ? (q0 * w + q1 * u) + q2 * v : mk3(0.f, 0.f, 0.f);
+    // uv = uv0 * w + uv1 * u + uv2 * v (cudaRTBDPTStream.cu:379-382)
+    const float tu = (p0.w * w + p1.w * u) + p2.w * v, tvv = (r0.w * w + r1.w * u) + r2.w * v;
+    s.diff = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+    s.trans = m.transparency;
+    if (dtex) {
+      const float4 t = tex_sample(sc, (uint32_t)m.diffuseTexId, tu, tvv);
+      s.diff = mk3(t.x, t.y, t.z);
+      s.trans = t.w * m.transparency;
+    }
+    s.emissive = mk3(m.emissive[0], m.emissive[1], m.emissive[2]);
+    if (etex) {
+      const float4 t = tex_sample(sc, (uint32_t)m.emissiveTexId, tu, tvv);
+      s.emissive = mk3(t.x, t.y, t.z);
+    }
   } else {
     s.pos = mk3(0.f, 0.f, 0.f);
+    s.diff = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
+    s.emissive = mk3(m.emissive[0], m.emissive[1], m.emissive[2]);
+    s.trans = m.transparency;
   }
-  s.diff = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
-  s.emissive = mk3(m.emissive[0], m.emissive[1], m.emissive[2]);
-  s.trans = m.transparency; s.spec = m.specular; s.metal = m.metallic; s.rough = m.roughness;
+  s.spec = m.specular; s.metal = m.metallic; s.rough = m.roughness;
   return s;
 }
 

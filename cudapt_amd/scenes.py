@@ -122,6 +122,45 @@ def cornell(seed: int = 0xC1, light_radiance: float = 15.0):
     return b.build(cam)
 
 
+def textured_cornell(seed: int = 0xC1, glass: bool = False):
+    """The Cornell box with textures on four materials (texture path test scene).
+
+    Every quad's corners get uv (0,0), (s,0), (s,s), (0,s), s per material:
+    - floor: 8 x 8 two-colour checker, s = 2.5 (wrap addressing);
+    - back wall: 16 x 4 red-to-blue ramp (bilinear blending);
+    - short box: 4 x 4 checker whose alpha alternates 1 and 0.3; with `glass`
+      the material's transparency is 0.9 (transparency = alpha x transparency);
+    - ceiling light: 2 x 2 emissive texture (emission from the texture).
+    """
+    sc = cornell(seed)
+    nq = len(sc["pos"]) // 4
+    quad_uv = np.asarray([(0, 0), (1, 0), (1, 1), (0, 1)], np.float32)
+    mat_of_quad = sc["mat"][::2]                    # two triangles per quad, in order
+    scale = np.ones(len(sc["mats"]), np.float32)
+    scale[0] = 2.5
+    uv = (quad_uv[None] * scale[mat_of_quad][:, None, None]).reshape(nq * 4, 2)
+    c0, c1 = np.array([0.8, 0.8, 0.75, 1.0]), np.array([0.2, 0.3, 0.6, 1.0])
+    ii, jj = np.mgrid[0:8, 0:8]
+    checker = np.where(((ii + jj) % 2 == 0)[..., None], c0, c1).astype(np.float32)
+    ramp = np.zeros((4, 16, 4), np.float32)
+    t = np.linspace(0.0, 1.0, 16)
+    ramp[..., 0] = 0.8 * (1 - t)
+    ramp[..., 1] = 0.3
+    ramp[..., 2] = 0.8 * t
+    ramp[..., 3] = 1.0
+    ii, jj = np.mgrid[0:4, 0:4]
+    boxtex = np.where(((ii + jj) % 2 == 0)[..., None], np.array([0.9, 0.6, 0.3, 1.0]),
+                     np.array([0.3, 0.9, 0.6, 0.3])).astype(np.float32)
+    light = np.array([[[20, 20, 20, 1], [10, 14, 18, 1]], [[18, 14, 10, 1], [16, 16, 16, 1]]], np.float32)
+    mats = sc["mats"].copy()
+    mats[0]["diffuseTexId"] = 0
+    mats[2]["diffuseTexId"] = 1
+    mats[6]["diffuseTexId"] = 2
+    mats[6]["transparency"] = 0.9 if glass else 0.0
+    mats[7]["emissiveTexId"] = 3
+    return dict(sc, uv=uv, mats=mats, textures=[checker, ramp, boxtex, light])
+
+
 def uv_sphere(center, radius, n_lon=8, n_lat=5):
     """UV sphere with n_lat bands: 2*n_lon cap triangles + 2*n_lon*(n_lat-2) quads' triangles."""
     pos, nrm, idx = [], [], []

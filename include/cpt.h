@@ -91,6 +91,16 @@ int cpt_scene_set_materials(cpt_scene* s, const cpt_material* m, uint32_t n);
  * reference's material dirty flag does (src/cudaRTBDPTStream.cu:845-850). */
 int cpt_scene_update_materials(cpt_scene* s);
 
+/* RTScene::m_pTextures (include/raytracer.h:276; loaded by LoadTexture,
+ * src/raytracer.cpp:161-179, as float RGBA): appends a width x height texture
+ * of 4 floats per texel, row-major, and returns its id, the index materials'
+ * diffuseTexId / emissiveTexId refer to.  Sampled as the reference's texture
+ * objects are (src/cudaRTCommon.cu:416-441): normalized coordinates, wrap
+ * addressing, linear filtering.  Ids outside the added textures mean none.
+ * The next render uploads the textures and restarts accumulation. */
+int cpt_scene_add_texture(cpt_scene* s, uint32_t width, uint32_t height, const float* rgba, uint32_t* id);
+int cpt_scene_clear_textures(cpt_scene* s);
+
 /* NPBVHHelper::CreateBVH + CompactBVH::InitialCompactBVH
  * (include/bvhhelper.h:43-54, src/bvhhelper.cpp:55-311): binned SAH, 16 bins,
  * depth <= max_depth (reference default 32), then the triangle reorder of

@@ -37,12 +37,17 @@ class RTScene {
   // one material index per triangle; then the SAH BVH of
   // src/raytracer.cpp:267-279 (max depth 32).
   bool SetMesh(const float* pos3, const float* nrm3, uint32_t nv, const uint32_t* idx3, const uint32_t* mat,
-               uint32_t nt, uint32_t max_depth = 32) {
-    return h_ && cpt_scene_set_mesh(h_, pos3, nrm3, nullptr, nv, idx3, mat, nt) == CPT_OK &&
+               uint32_t nt, uint32_t max_depth = 32, const float* uv2 = nullptr) {
+    return h_ && cpt_scene_set_mesh(h_, pos3, nrm3, uv2, nv, idx3, mat, nt) == CPT_OK &&
            cpt_bvh_build_sah(h_, max_depth) == CPT_OK;
   }
   // RTScene::m_pMaterials.
   bool SetMaterials(const cpt_material* m, uint32_t n) { return h_ && cpt_scene_set_materials(h_, m, n) == CPT_OK; }
+  // RTScene::m_pTextures after LoadTexture (src/raytracer.cpp:161-179): a
+  // width x height RGBA float texture; id is what materials' texture ids name.
+  bool AddTexture(uint32_t width, uint32_t height, const float* rgba, uint32_t& id) {
+    return h_ && cpt_scene_add_texture(h_, width, height, rgba, &id) == CPT_OK;
+  }
   // RTScene::UpdateMaterialsDirtyFlag (src/raytracer.cpp:284-300): pushes edited
   // materials; renderers drawing the scene restart accumulation.
   bool UpdateMaterialsDirtyFlag() { return h_ && cpt_scene_update_materials(h_) == CPT_OK; }

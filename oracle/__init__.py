@@ -34,6 +34,11 @@ def lib():
         L.orc_scene_create.restype = vp
         L.orc_scene_create.argtypes = [f32p, f32p, c32, u32p, u32p, c32, vp, c32, c32]
         L.orc_scene_destroy.argtypes = [vp]
+        L.orc_scene_set_uv.argtypes = [vp, f32p]
+        L.orc_scene_add_texture.restype = c32
+        L.orc_scene_add_texture.argtypes = [vp, c32, c32, f32p]
+        L.orc_tex_sample.argtypes = [vp, c32, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                     np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
         L.orc_scene_bvh.restype = c32
         L.orc_scene_bvh.argtypes = [vp, vp, vp, vp, c32]
         L.orc_scene_light_tris.restype = c32
@@ -103,6 +108,23 @@ class Scene:
         if not self.h:
             raise RuntimeError("oracle: BVH build failed")
         self.nt = len(self.idx)
+        if sc.get("uv") is not None:
+            self.uv = np.ascontiguousarray(sc["uv"], np.float32).reshape(-1)
+            L.orc_scene_set_uv(self.h, self.uv)
+        self.textures = []
+        for t in sc.get("textures", ()):
+            self.add_texture(t)
+
+    def add_texture(self, rgba) -> int:
+        t = np.ascontiguousarray(rgba, np.float32)
+        self.textures.append(t)
+        return lib().orc_scene_add_texture(self.h, t.shape[1], t.shape[0], t.ravel())
+
+    def tex_sample(self, tex_id: int, u: float, v: float, bits: int = 32) -> np.ndarray:
+        """One texture lookup (wrap, bilinear) in FP32 or FP64."""
+        out = np.zeros(4, np.float64)
+        lib().orc_tex_sample(self.h, tex_id, float(u), float(v), bits, out)
+        return out
 
     def __del__(self):
         if getattr(self, "h", None):

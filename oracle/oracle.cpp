@@ -175,6 +175,9 @@ struct Scene {
   std::vector<float> bmin, bmax;     // 3 per node
   std::vector<u32> offOrTSTN;        // 2 per node
   std::vector<u32> lightTri;         // leaf-order emissive triangles
+  std::vector<float> uv;             // 2 per vertex (zeros unless set)
+  std::vector<float> tex;            // RTScene::m_pTextures: RGBA texels, concatenated
+  std::vector<u32> texDesc;          // 3 per texture: first texel, width, height
 };
 
 // ---------------------------------------------------------------- BVH build
@@ -519,6 +522,34 @@ template <class R> struct HitShade {
   R trans, specular, metallic, roughness;
 };
 
+// tex2D<float4> of the reference's texture objects (src/cudaRTCommon.cu:416-441):
+// normalized coordinates wrapped to [0, 1), linear filtering between the four
+// texels around (x - 1/2, y - 1/2) with wrapped indices, blended in x then y.
+// NaN and infinite coordinates sample as 0.  CUDA's texture unit computes the
+// blend weights in 9-bit fixed point; this is the exact-weight form.
+template <class R>
+static void tex_sample(const Scene& s, u32 id, R u, R v, R out[4]) {
+  const u32 first = s.texDesc[id * 3], Wu = s.texDesc[id * 3 + 1], Hu = s.texDesc[id * 3 + 2];
+  const int W = (int)Wu, H = (int)Hu;
+  if (!(std::fabs(u) < (R)1e30)) u = 0;
+  if (!(std::fabs(v) < (R)1e30)) v = 0;
+  const R fu = u - std::floor(u), fv = v - std::floor(v);
+  const R x = fu * (R)W - (R)0.5, y = fv * (R)H - (R)0.5;
+  const R fx = std::floor(x), fy = std::floor(y);
+  const R a = x - fx, b = y - fy;
+  int i0 = std::min(std::max((int)fx, -1), W - 1), j0 = std::min(std::max((int)fy, -1), H - 1);
+  if (i0 < 0) i0 += W;
+  if (j0 < 0) j0 += H;
+  const int i1 = i0 + 1 == W ? 0 : i0 + 1, j1 = j0 + 1 == H ? 0 : j0 + 1;
+  const float* t00 = &s.tex[(first + (size_t)j0 * W + i0) * 4];
+  const float* t10 = &s.tex[(first + (size_t)j0 * W + i1) * 4];
+  const float* t01 = &s.tex[(first + (size_t)j1 * W + i0) * 4];
+  const float* t11 = &s.tex[(first + (size_t)j1 * W + i1) * 4];
+  const R ia = (R)1 - a, ib = (R)1 - b;
+  for (int c = 0; c < 4; c++)
+    out[c] = ((R)t00[c] * ia + (R)t10[c] * a) * ib + ((R)t01[c] * ia + (R)t11[c] * a) * b;
+}
+
 template <class R>
 static HitShade<R> fetch_hit(const Scene& s, u32 leaf, R w, R u, R v) {
   HitShade<R> hs;
@@ -530,6 +561,24 @@ static HitShade<R> fetch_hit(const Scene& s, u32 leaf, R w, R u, R v) {
   hs.pos = (q0 * w + q1 * u) + q2 * v;
   hs.diff = cvt<R>(m.diffuse); hs.emissive = cvt<R>(m.emissive);
   hs.trans = m.transparency; hs.specular = m.specular; hs.metallic = m.metallic; hs.roughness = m.roughness;
+  // GetMaterialColors (src/cudaRTCommon.cu:369-406); uv = uv0 w + uv1 u + uv2 v
+  // (cudaRTBDPTStream.cu:379-382); ids outside the textures mean none
+  const u32 ntex = (u32)(s.texDesc.size() / 3);
+  const bool dtex = (u32)m.diffuseTexId < ntex, etex = (u32)m.emissiveTexId < ntex;
+  if (dtex || etex) {
+    const R tu = ((R)s.uv[t[0] * 2] * w + (R)s.uv[t[1] * 2] * u) + (R)s.uv[t[2] * 2] * v;
+    const R tv = ((R)s.uv[t[0] * 2 + 1] * w + (R)s.uv[t[1] * 2 + 1] * u) + (R)s.uv[t[2] * 2 + 1] * v;
+    R c[4];
+    if (dtex) {
+      tex_sample<R>(s, (u32)m.diffuseTexId, tu, tv, c);
+      hs.diff = mk(c[0], c[1], c[2]);
+      hs.trans = c[3] * (R)m.transparency;
+    }
+    if (etex) {
+      tex_sample<R>(s, (u32)m.emissiveTexId, tu, tv, c);
+      hs.emissive = mk(c[0], c[1], c[2]);
+    }
+  }
   return hs;
 }
 
@@ -1133,6 +1182,7 @@ void* orc_scene_create(const float* pos3, const float* nrm3, u32 nv, const u32* 
                        const void* materials, u32 nmat, u32 max_depth) {
   Scene* s = new Scene();
   s->nv = nv; s->nt = nt;
+  s->uv.assign((size_t)nv * 2, 0.f);
   s->pos.assign(pos3, pos3 + (size_t)nv * 3);
   if (nrm3) s->nrm.assign(nrm3, nrm3 + (size_t)nv * 3); else s->nrm.assign((size_t)nv * 3, 0.f);
   s->mats.resize(nmat);
@@ -1159,6 +1209,34 @@ void* orc_scene_create(const float* pos3, const float* nrm3, u32 nv, const u32* 
   return s;
 }
 void orc_scene_destroy(void* h) { delete (Scene*)h; }
+// Per-vertex texture coordinates (2 floats per vertex).
+void orc_scene_set_uv(void* h, const float* uv2) {
+  Scene* s = (Scene*)h;
+  s->uv.assign(uv2, uv2 + (size_t)s->nv * 2);
+}
+// Appends a w x h RGBA texture; returns its id.
+u32 orc_scene_add_texture(void* h, u32 w, u32 hgt, const float* rgba) {
+  Scene* s = (Scene*)h;
+  const u32 first = (u32)(s->tex.size() / 4);
+  s->tex.insert(s->tex.end(), rgba, rgba + (size_t)w * hgt * 4);
+  s->texDesc.push_back(first);
+  s->texDesc.push_back(w);
+  s->texDesc.push_back(hgt);
+  return (u32)(s->texDesc.size() / 3 - 1);
+}
+// One texture lookup (the tex_sample above), FP32 or FP64.
+void orc_tex_sample(void* h, u32 id, double u, double v, int bits, double* out) {
+  const Scene* s = (const Scene*)h;
+  if (bits == 64) {
+    double c[4];
+    tex_sample<double>(*s, id, u, v, c);
+    for (int k = 0; k < 4; k++) out[k] = c[k];
+  } else {
+    float c[4];
+    tex_sample<float>(*s, id, (float)u, (float)v, c);
+    for (int k = 0; k < 4; k++) out[k] = c[k];
+  }
+}
 
 // Exports the compact BVH: bounds6 (min xyz, max xyz), offOrTSTN2, reorder, nodeN.
 u32 orc_scene_bvh(void* h, float* bounds6, u32* off2, u32* reorder, u32 cap) {
