@@ -69,6 +69,7 @@ def lib():
             "cpt_bvh4_set_leaf_size": [vp, c32],
             "cpt_scene_add_texture": [vp, c32, c32, vp, vp],
             "cpt_scene_clear_textures": [vp],
+            "cpt_trace_cost": [vp, vp, c32, c32, c32, vp],
             "cpt_scene_upload": [vp],
             "cpt_scene_destroy": [vp],
             "cpt_trace_closest": [vp, vp, c32, c32, vp],
@@ -257,6 +258,13 @@ class Scene:
         hits = np.zeros(len(r), HIT_DTYPE)
         _check(lib().cpt_trace_closest(self.h, _ptr(r), len(r), int(cull), _ptr(hits)), "cpt_trace_closest")
         return hits
+
+    def trace_cost(self, rays: np.ndarray, layout: int = 2, cull: bool = False) -> np.ndarray:
+        """(n, 2) uint32: node visits and triangle tests of each ray's closest-hit traversal."""
+        r = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+        out = np.zeros((len(r), 2), np.uint32)
+        _check(lib().cpt_trace_cost(self.h, _ptr(r), len(r), int(cull), layout, _ptr(out)), "cpt_trace_cost")
+        return out
 
     def trace_device(self, d_rays_ptr: int, n: int, d_hits_ptr: int, stream_ptr: int = 0, layout: int = 0,
                      cull: bool = False):

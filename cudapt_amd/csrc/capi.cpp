@@ -420,6 +420,21 @@ int cpt_trace_closest(cpt_scene* s, const cpt_ray* rays, uint32_t n, uint32_t cu
   return CPT_OK;
 }
 
+int cpt_trace_cost(cpt_scene* s, const cpt_ray* rays, uint32_t n, uint32_t cull, uint32_t layout, uint32_t* cost2) {
+  if (!s || (n && (!rays || !cost2))) return fail(CPT_E_ARG, "cpt_trace_cost: null argument");
+  if (!s->uploaded) return fail(CPT_E_STATE, "cpt_trace_cost: scene not uploaded");
+  if (layout > 2) return fail(CPT_E_ARG, "cpt_trace_cost: layout %u not 0, 1 or 2", layout);
+  if (n == 0) return CPT_OK;
+  HIP_TRY(s->d_rays.alloc((size_t)n * 2));
+  HIP_TRY(s->d_hits.alloc(n));                  // 16 bytes per ray; the costs take 8
+  HIP_TRY(hipMemcpyAsync(s->d_rays.p, rays, sizeof(cpt_ray) * n, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(launch_trace(s->dev(), s->d_rays.p, n, nullptr, cull, s->d_hits.p, 0, 0, layout, s->d_stats.p, s->stream,
+                       1));
+  HIP_TRY(hipMemcpyAsync(cost2, s->d_hits.p, sizeof(uint32_t) * 2 * n, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return CPT_OK;
+}
+
 int cpt_filter_cross_bilateral_device(const float* d_color, const float* d_pos, const float* d_norm,
                                       const float* d_diff, uint32_t width, uint32_t height, float sigma_color,
                                       float sigma_pos, float sigma_norm, float sigma_diff, uint32_t radius,

@@ -129,7 +129,7 @@ constexpr int kBlock = 256;
 
 // ---- launches -------------------------------------------------------------
 hipError_t launch_trace(const DevScene& sc, const float4* rays, uint32_t n, const uint32_t* count, uint32_t cull,
-                        float4* hits_out, int out_orig, int any_hit, uint32_t layout, DevStats* stats, hipStream_t st);
+                        float4* hits_out, int out_orig, int any_hit, uint32_t layout, DevStats* stats, hipStream_t st, int cost = 0);
 hipError_t launch_frame_begin(FrameCounters* fc, uint32_t n_paths, hipStream_t st);
 hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, const LightJobs& jobs, LightVert* lv,
                              FrameCounters* fc, hipStream_t st);

@@ -391,8 +391,10 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace(DevScene sc, const float
       r.z = h.u;
       r.w = h.v;
       out[i] = r;
-    } else {
+    } else if (OUT == 2) {
       ((uint32_t*)out)[i] = hit ? 1u : 0u;
+    } else {
+      ((uint2*)out)[i] = make_uint2(nn, nt);      // this ray's node visits and triangle tests
     }
   }
   if (stats) {
@@ -406,14 +408,16 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace(DevScene sc, const float
 
 hipError_t launch_trace(const DevScene& sc, const float4* rays, uint32_t n, const uint32_t* count, uint32_t cull,
                         float4* hits_out, int out_orig, int any_hit, uint32_t layout, DevStats* stats,
-                        hipStream_t st) {
+                        hipStream_t st, int cost) {
   if (n == 0) return hipSuccess;
   dim3 grid((n + kBlock - 1) / kBlock), block(kBlock);
   const float4* ro = rays;
   const float4* rd = rays + 1;
   if (layout > 2 || (layout == 2 && !sc.nodes4q) || (layout == 0 && !sc.nodes4)) return hipErrorInvalidValue;
 #define CPT_TR(L, A, O) k_trace<L, A, O><<<grid, block, 0, st>>>(sc, ro, rd, 2, count, n, cull, hits_out, stats)
-  if (any_hit) {
+  if (cost) {
+    if (layout == 0) CPT_TR(0, false, 3); else if (layout == 2) CPT_TR(2, false, 3); else CPT_TR(1, false, 3);
+  } else if (any_hit) {
     if (layout == 0) CPT_TR(0, true, 2); else if (layout == 2) CPT_TR(2, true, 2); else CPT_TR(1, true, 2);
   } else if (out_orig) {
     if (layout == 0) CPT_TR(0, false, 0); else if (layout == 2) CPT_TR(2, false, 0); else CPT_TR(1, false, 0);

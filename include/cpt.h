@@ -159,6 +159,13 @@ int cpt_filter_cross_bilateral_device(const float* d_color, const float* d_pos, 
                                       const uint32_t* d_sample_n, uint32_t nd, float* d_filtered, float* d_err,
                                       void* stream);
 
+/* Traversal cost per ray, the counterpart of the reference's BVH debug
+ * renderer (TraceCost / TraceDepth, src/cudaRTCommon.cu:129-273;
+ * src/cudaRTBVHDebug.cu): cost2[2i] = nodes visited, cost2[2i+1] = triangles
+ * tested by ray i's closest-hit traversal in `layout` (as for
+ * cpt_trace_closest_device).  Host arrays. */
+int cpt_trace_cost(cpt_scene* s, const cpt_ray* rays, uint32_t n, uint32_t cull, uint32_t layout, uint32_t* cost2);
+
 /* Node visits and triangle tests of every traversal since the last reset. */
 int cpt_trace_counters(cpt_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* rays, int reset);
 
