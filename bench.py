@@ -164,7 +164,8 @@ def hbm_traffic(args):
         d = tempfile.mkdtemp(prefix="cpt_pmc_")
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t",
                "--", sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
-               "--no-traffic", "--no-learned", "--no-multiview", "--frames", str(args.frames), "--in-flight", str(args.in_flight),
+               "--no-traffic", "--no-learned", "--no-multiview", "--mega-steps", "0", "--frames", str(args.frames),
+               "--in-flight", str(args.in_flight),
                "--lds-stack", str(args.lds_stack), "--lds-stack-shadow", str(args.lds_stack_shadow),
                "--node-format", str(args.node_format),
                "--refill", str(args.refill), "--refill-groups", str(args.refill_groups),
@@ -192,6 +193,44 @@ def hbm_traffic(args):
     write, wfac = out["WRITE_SIZE"]
     return {"bytes_per_launch": int(fetch * ffac + write * wfac), "fetch_bytes": int(fetch * ffac),
             "write_bytes": int(write * wfac), "fetch_scale": round(ffac, 3), "write_scale": round(wfac, 3)}
+
+
+def megakernel_leg(ca, S, sc, args):
+    """The same workload with the megakernel schedule (one thread per sample
+    through every bounce, the reference's regeneration baseline): one step of
+    each schedule compared bit for bit, then `--mega-steps` timed steps."""
+    def renderer(mega):
+        R = ca.Renderer(WIDTH, HEIGHT, eye_depth=EYE_DEPTH, light_depth=LIGHT_DEPTH)
+        R.set_attrib("Frames In Flight", args.in_flight)
+        R.set_attrib("Traversal LDS Stack", args.lds_stack)
+        R.set_attrib("Shadow LDS Stack", args.lds_stack_shadow or args.lds_stack)
+        R.set_attrib("Node Format", args.node_format)
+        R.set_attrib("Traversal Refill", args.refill)
+        R.set_attrib("Refill Workgroups", args.refill_groups)
+        R.set_attrib("Shadow Refill", args.refill_shadow)
+        R.set_attrib("Megakernel", 1 if mega else 0)
+        return R
+    cam = sc["camera"]
+    Rw, Rm = renderer(False), renderer(True)
+    img_w = Rw.render(S, cam, frames=args.frames)
+    img_m = Rm.render(S, cam, frames=args.frames)
+    same = bool(np.array_equal(img_w.view(np.uint32), img_m.view(np.uint32)))
+    del Rw, img_w, img_m
+    log(f"megakernel: image identical to the wavefront's: {same}")
+    Rm.stats(reset=True)
+    Rm.set_profiling(True)
+    ca.synchronize()
+    t0 = time.perf_counter()
+    for k in range(args.mega_steps):
+        Rm.render(S, cam, frames=args.frames, result=False)
+        ca.synchronize()
+        log(f"megakernel step {k + 1}/{args.mega_steps}")
+    wall = time.perf_counter() - t0
+    st = Rm.stats(reset=True)
+    rays = st["extension_rays"] + st["shadow_rays"]
+    return {"mrays_per_s": round(rays / wall / 1e6, 2), "ms_per_step": round(wall / args.mega_steps * 1e3, 3),
+            "kernel_ms_per_step": round(st["extend_ms"] / args.mega_steps, 3), "same_image": same,
+            "steps": args.mega_steps}
 
 
 def multiview_leg(ca, args):
@@ -245,6 +284,7 @@ def main():
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
     ap.add_argument("--no-learned", action="store_true", help="skip the time-to-equal-MSE leg")
     ap.add_argument("--no-multiview", action="store_true", help="skip the batched multi-view leg")
+    ap.add_argument("--mega-steps", type=int, default=2, help="timed steps of the megakernel schedule (0: skip)")
     ap.add_argument("--views", type=int, default=32)
     ap.add_argument("--mv-spp", type=int, default=32)
     ap.add_argument("--mv-in-flight", type=int, default=3)
@@ -348,6 +388,10 @@ def main():
         "time_to_equal_mse": None,
     }
     del R
+    res["megakernel"] = None
+    if args.mega_steps > 0:
+        res["megakernel"] = megakernel_leg(ca, S, sc, args)
+        res["megakernel"]["wavefront_speedup"] = round(mrays / max(res["megakernel"]["mrays_per_s"], 1e-9), 3)
     res["mlp"] = None
     if not args.no_learned:
         res["time_to_equal_mse"], res["mlp"] = learned_leg(ca, S, args)

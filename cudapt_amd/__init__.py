@@ -319,7 +319,7 @@ class Renderer:
                "nd": "Desired Max Sampling", "sigma_color": "Filter Color EuD", "sigma_pos": "Filter Pos EuD",
                "sigma_norm": "Filter Norm EuD", "sigma_diff": "Filter Diff EuD", "radius": "Filter Radius",
                "filter_diff": "Filter Diffuse Flag", "in_flight": "Frames In Flight",
-               "pass_frames": "Adaptive Pass Frames"}.get(name, name)
+               "pass_frames": "Adaptive Pass Frames", "megakernel": "Megakernel"}.get(name, name)
         idx = self.attribs()[key][0]
         _check(lib().cpt_renderer_set_attrib(self.h, idx, float(value)), "cpt_renderer_set_attrib")
 

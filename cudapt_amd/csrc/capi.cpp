@@ -549,6 +549,11 @@ const char* const kRefillMinPaths = "Refill Min Paths";
 // grows with the per-sample standard deviation); 2 = exp(y) / N^2, the error
 // one more sample removes (sigma^2 / N - sigma^2 / (N + 1) ~ sigma^2 / N^2)
 const char* const kLearnedSignal = "Learned Signal";
+// 1: the plain stream renderer traces each batch with one megakernel (one thread
+// carries its sample through every bounce, the reference's regeneration
+// baseline) instead of the wavefront passes; same image, bit for bit.  Its
+// time is reported as extension time.
+const char* const kMegakernel = "Megakernel";
 const char* const kLdsStack = "Traversal LDS Stack";
 const char* const kLdsStackShadow = "Shadow LDS Stack";
 constexpr u32 kMaxInFlight = 90;   // 30 three-frame light-cache groups + partial ones at either end
@@ -991,7 +996,6 @@ int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
     HIP_TRY(timed_end(r, 3, ev));
   }
   HIP_TRY(launch_frame_begin(r->fc.p, n, r->stream));
-  HIP_TRY(launch_gen_eye(fa, r->A.view(), r->stream));
   Accum ac;
   std::memset(&ac, 0, sizeof ac);
   ac.result = r->result.p;
@@ -999,8 +1003,17 @@ int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   ac.rec.acc = r->rec_acc.p;
   ac.apply = 2;
   ac.variant = 0;
-  int rc = bounce_loop(r, sc, fa, n, ac);
-  if (rc) return rc;
+  if (r->attr(kMegakernel) != 0) {
+    r->extend_launches++;
+    HIP_TRY(timed_begin(r, 0, &ev));
+    HIP_TRY(launch_mega(sc, fa, r->fc.p, r->lv.p, ac, (int)r->attr(kLdsStack), r->attr(kNodeFormat) != 0,
+                        r->stats.p, r->stream));
+    HIP_TRY(timed_end(r, 0, ev));
+  } else {
+    HIP_TRY(launch_gen_eye(fa, r->A.view(), r->stream));
+    int rc = bounce_loop(r, sc, fa, n, ac);
+    if (rc) return rc;
+  }
   HIP_TRY(timed_begin(r, 8, &ev));
   HIP_TRY(launch_apply_frames(fa, r->rec_acc.p, r->result.p, r->sample_n.p, r->stream));
   HIP_TRY(timed_end(r, 8, ev));
@@ -1130,6 +1143,7 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
     r->attribs.push_back({kLearnedSignal, 0, 0});
   } else {
     r->attribs.push_back({kInFlight, 0, 24});
+    r->attribs.push_back({kMegakernel, 0, 0});
   }
   // traversal kernel variant: 8 LDS stack entries, >= 6 waves per SIMD
   // (swept on the sphere grid: 1.58 vs 1.36 Grays/s for 16 entries)

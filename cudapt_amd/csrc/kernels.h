@@ -165,6 +165,11 @@ constexpr int kVariantGbuf = 2;
 hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, PathBuf nxt, const float4* hits,
                         FrameCounters* fc, int bounce, uint32_t max_n, ShadowBuf sb, const LightVert* lv,
                         const Accum& ac, hipStream_t st);
+// Whole batch of the plain stream renderer in one kernel: eye paths, every
+// bounce's closest hit, shading and connection occlusion per thread; results
+// into ac.rec.acc (ac.apply must be 2), counters into fc and stats.
+hipError_t launch_mega(const DevScene& sc, const FrameArgs& fa, FrameCounters* fc, const LightVert* lv,
+                       const Accum& ac, int lds_stack, int quant, DevStats* stats, hipStream_t st);
 hipError_t launch_frame_begin_dev(FrameCounters* fc, const uint32_t* total, uint32_t cap, hipStream_t st);
 hipError_t launch_gen_eye_rec(const FrameArgs& fa, PathBuf pb, SampleRec rec, hipStream_t st);
 

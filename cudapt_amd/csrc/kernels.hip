@@ -817,13 +817,27 @@ __device__ __forceinline__ void apply_pixel(float* result, uint32_t* sample_n, u
   sample_n[pix] = nn;
 }
 
+// One eye path in registers: the ray (o, max distance; d, epsilon), pixel,
+// sample slot, depth, frame offset in the batch, throughput, emission gathered
+// so far, the accumulated radiance with its sample count (acc.w, as bits) and
+// the RNG state.  PathBuf holds the same fields as columns.
+struct PathState {
+  float4 ro, rd, mul, smp, acc;
+  uint32_t pix, sid, depth, fk;
+  uint64_t rng;
+};
+// The connection's shadow ray: origin and max distance, direction and epsilon,
+// contribution added to the path's accumulator when nothing occludes it.
+struct ShadowRay {
+  float4 o, d, c;
+};
+
 // Eye ray of pixel (x, y) drawn from PCG32 stream `key`, pixel as sequence
 // (pt_genPathQueue_kernel, src/cudaRTBDPTStream.cu:619-640; the adaptive queue
 // of AdapBilatAtomic.cu:826-840 uses the same pinhole).  In a multi-view batch
 // `pix` is the batch-global pixel: the camera and the pixel's position come
 // from its view, the RNG stream stays keyed by the global pixel.
-__device__ __forceinline__ void write_eye_path(const FrameArgs& fa, PathBuf& pb, uint32_t slot, uint32_t pix,
-                                               uint32_t sid, uint64_t key) {
+__device__ __forceinline__ PathState eye_path(const FrameArgs& fa, uint32_t pix, uint32_t sid, uint64_t key) {
   uint32_t lp = pix, vw = fa.width, vh = fa.height;
   const float *cpos = fa.cam_pos, *cdir = fa.cam_dir, *cup = fa.cam_up, *cright = fa.cam_right;
   float fov = fa.fov;
@@ -848,15 +862,31 @@ __device__ __forceinline__ void write_eye_path(const FrameArgs& fa, PathBuf& pb,
   f3 up = mk3(cup[0], cup[1], cup[2]);
   f3 cd = mk3(cdir[0], cdir[1], cdir[2]);
   f3 d = normalize((right * au + up * av) + cd);
-  pb.ray_o[slot] = make_float4(cpos[0], cpos[1], cpos[2], kInfF);
-  pb.ray_d[slot] = make_float4(d.x, d.y, d.z, (float)kBiasEps);
-  pb.pix[slot] = pix;
-  pb.sid[slot] = sid;
+  PathState p;
+  p.ro = make_float4(cpos[0], cpos[1], cpos[2], kInfF);
+  p.rd = make_float4(d.x, d.y, d.z, (float)kBiasEps);
+  p.pix = pix;
+  p.sid = sid;
+  p.depth = 0;
+  p.fk = 0;
+  p.mul = make_float4(1.f, 1.f, 1.f, 0.f);
+  p.smp = make_float4(0.f, 0.f, 0.f, 0.f);
+  p.acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  p.rng = st;
+  return p;
+}
+__device__ __forceinline__ void write_eye_path(const FrameArgs& fa, PathBuf& pb, uint32_t slot, uint32_t pix,
+                                               uint32_t sid, uint64_t key) {
+  const PathState p = eye_path(fa, pix, sid, key);
+  pb.ray_o[slot] = p.ro;
+  pb.ray_d[slot] = p.rd;
+  pb.pix[slot] = p.pix;
+  pb.sid[slot] = p.sid;
   pb.depth[slot] = 0;
-  pb.mul[slot] = make_float4(1.f, 1.f, 1.f, 0.f);
-  pb.smp[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-  pb.acc[slot] = make_float4(0.f, 0.f, 0.f, 0.f);
-  pb.rng[slot] = st;
+  pb.mul[slot] = p.mul;
+  pb.smp[slot] = p.smp;
+  pb.acc[slot] = p.acc;
+  pb.rng[slot] = p.rng;
 }
 
 __device__ __forceinline__ void zero_rec(SampleRec& rec, uint32_t q) {
@@ -1179,6 +1209,113 @@ __device__ __forceinline__ void block_append2(bool p0, bool p1, uint32_t* ctr0, 
   s1 = base1 + pre1 + __popcll(m1 & below);
 }
 
+// One eye-path vertex (pt_traceSample_kernel, src/cudaRTBDPTStream.cu:365-543,
+// and the connection set-up of pt_connectEyeLightPath_kernel, :665-723):
+// emission, BRDF sample, Russian roulette, termination into the accumulators,
+// and for a surviving diffuse bounce the light-vertex connection's shadow ray.
+// p holds the path before the bounce and, when `survive`, after it.
+__device__ __forceinline__ void shade_path(const DevScene& sc, const FrameArgs& fa, const FrameCounters* fc,
+                                           const LightVert* __restrict__ lv, const Accum& ac, PathState& p, float4 h,
+                                           bool& survive, bool& emit, ShadowRay& sr) {
+  const uint32_t pix = p.pix, sid = p.sid;
+  uint32_t depth = p.depth;
+  const uint32_t frame = frame_base(fa) + p.fk;
+  const uint32_t slot = (frame / 3u) % (uint32_t)kLightSlots;
+  const uint32_t lvN = fc->light_n[slot];
+  const LightVert* __restrict__ lvb = lv + (size_t)slot * fa.lv_stride;
+  uint64_t st = p.rng;
+  const float4 ro = p.ro, rd = p.rd, m4 = p.mul, s4 = p.smp, a4 = p.acc;
+  float4 o_out, d_out, mul_out, smp_out, acc_out;
+  float4& so = sr.o;
+  float4& sd = sr.d;
+  float4& sc4 = sr.c;
+  const uint64_t inc = ((uint64_t)local_pixel(fa, pix) << 1) | 1u;
+  f3 orig = ld3(ro), dir = ld3(rd), mul = ld3(m4), psample = ld3(s4), acc = ld3(a4);
+  uint32_t N = __float_as_uint(a4.w);
+  uint32_t leaf = __float_as_uint(h.y);
+  bool term = false;
+  if (leaf == kInvalid) {
+    acc = acc + psample; N++; term = true;
+  } else {
+    float w = (1.f - h.z) - h.w;
+    const bool gbuf = (ac.variant & kVariantGbuf) && depth == 0;
+    Surf s = fetch_surf(sc, leaf, w, h.z, h.w, gbuf);
+    f3 nl = dot(s.norm, dir) < 0.f ? s.norm : s.norm * -1.f;
+    if (gbuf) {                                          // AdapBilatAtomic.cu:650-655
+      ac.rec.gpos[sid] = make_float4(s.pos.x, s.pos.y, s.pos.z, 0.f);
+      ac.rec.gnrm[sid] = make_float4(nl.x, nl.y, nl.z, 0.f);
+      ac.rec.gdif[sid] = make_float4(s.diff.x, s.diff.y, s.diff.z, 0.f);
+    }
+    Bounce b = sample_bounce(st, inc, orig, dir, h.x, s, nl, RT_EYE, (ac.variant & kVariantReflOffset) != 0);
+    psample = psample + vmul(s.emissive, mul);
+    f3 inMul = mul, inDir = dir * -1.f;
+    float contrib = length(mul) * length(b.mul);
+    if (length(s.emissive) > 0.f) contrib = 0.f;
+    if (uniform01(st, inc) > contrib || depth + 1 >= fa.eye_depth) {
+      acc = acc + psample; N++; term = true;
+    } else {
+      mul = vmul(mul, b.mul);
+      dir = b.dir;
+      depth++;
+      orig = b.orig;
+      if (b.type == RT_DIFF && lvN > 0) {
+        f3 pos = orig + nl * (float)kBiasEps;
+        uint32_t li = (uint32_t)(uniform01(st, inc) * (float)lvN);
+        if (li >= lvN) li = lvN - 1;
+        const LightVert& L = lvb[li];
+        f3 lpos = mk3(L.pos[0], L.pos[1], L.pos[2]);
+        f3 lirr = mk3(L.irrad[0], L.irrad[1], L.irrad[2]);
+        f3 toL = normalize(lpos - pos);
+        float dist = length(lpos - pos);
+        if (length(lirr) > 0.f && dot(nl, toL) > 0.f) {
+          f3 irrad, irradDir;
+          if ((double)dist > kFltEps) {
+            irrad = shading(toL * -1.f, mk3(L.irradDir[0], L.irradDir[1], L.irradDir[2]), lirr,
+                            mk3(L.norm[0], L.norm[1], L.norm[2]), mk3(L.diff[0], L.diff[1], L.diff[2]), L.metallic,
+                            L.roughness, L.specular, 1.f, 1.f) +
+                    mk3(L.emissive[0], L.emissive[1], L.emissive[2]);
+            irradDir = toL;
+          } else {
+            irrad = lirr;
+            irradDir = mk3(L.irradDir[0], L.irradDir[1], L.irradDir[2]) * -1.f;
+          }
+          f3 c = vmax0(shading(inDir, irradDir, irrad, nl, s.diff, s.metal, s.rough, s.spec, 1.f - s.trans, 1.f));
+          if (length(c) > 0.f) {
+            f3 pay = vmul(c, inMul);
+            emit = true;
+            so = make_float4(pos.x, pos.y, pos.z, (float)shadow_max((double)dist, fa.flags));
+            sd = make_float4(toL.x, toL.y, toL.z, (float)kBiasEps);
+            sc4 = make_float4(pay.x, pay.y, pay.z, 0.f);
+          }
+        }
+      }
+    }
+  }
+  if (term) {
+    if (ac.apply == 0) apply_pixel(ac.result, ac.sample_n, pix, frame, acc, N);
+    else if (ac.apply == 1) apply_pixel_gbuf(ac, pix, frame, acc, N, sid);
+    else ac.rec.acc[sid] = make_float4(acc.x, acc.y, acc.z, __uint_as_float(N));
+  } else {
+    survive = true;
+    o_out = make_float4(orig.x, orig.y, orig.z, kInfF);
+    d_out = make_float4(dir.x, dir.y, dir.z, (float)kBiasEps);
+    mul_out = make_float4(mul.x, mul.y, mul.z, 0.f);
+    smp_out = make_float4(psample.x, psample.y, psample.z, 0.f);
+    acc_out = make_float4(acc.x, acc.y, acc.z, __uint_as_float(N));
+  }
+  if (survive) {
+    p.ro = o_out;
+    p.rd = d_out;
+    p.mul = mul_out;
+    p.smp = smp_out;
+    p.acc = acc_out;
+    p.depth = depth;
+  }
+  p.rng = st;
+}
+
+// The shading pass of one bounce over the live paths: survivors compacted into
+// nxt, connection shadow rays into sb (contribution .w = the survivor's slot).
 __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, PathBuf cur, PathBuf nxt,
                                                   const float4* __restrict__ hits, FrameCounters* __restrict__ fc,
                                                   int bounce, ShadowBuf sb, const LightVert* __restrict__ lv,
@@ -1187,117 +1324,41 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
   const uint32_t count = fc->active[bounce];
   if (blockIdx.x * kBlock >= count) return;               // whole workgroup idle
   bool survive = false, emit = false;
-  float4 o_out = make_float4(0.f, 0.f, 0.f, 0.f), d_out = o_out, mul_out = o_out, smp_out = o_out, acc_out = o_out;
-  float4 so = o_out, sd = o_out, sc4 = o_out;
-  uint32_t pix = 0, sid = 0, depth = 0, fk = 0;
-  uint64_t st = 0;
+  PathState p;
+  ShadowRay sr;
+  sr.o = sr.d = sr.c = make_float4(0.f, 0.f, 0.f, 0.f);
   if (i < count) {
-    float4 ro = cur.ray_o[i], rd = cur.ray_d[i];
-    pix = cur.pix[i];
-    sid = cur.sid[i];
+    p.ro = cur.ray_o[i];
+    p.rd = cur.ray_d[i];
+    p.pix = cur.pix[i];
+    p.sid = cur.sid[i];
     const uint32_t dk = cur.depth[i];
-    depth = dk & 0xFFu;
-    fk = dk >> 8;
-    const uint32_t frame = frame_base(fa) + fk;
-    const uint32_t slot = (frame / 3u) % (uint32_t)kLightSlots;
-    const uint32_t lvN = fc->light_n[slot];
-    const LightVert* __restrict__ lvb = lv + (size_t)slot * fa.lv_stride;
-    float4 m4 = cur.mul[i], s4 = cur.smp[i], a4 = cur.acc[i];
-    st = cur.rng[i];
-    const uint64_t inc = ((uint64_t)local_pixel(fa, pix) << 1) | 1u;
-    float4 h = hits[i];
-    f3 orig = ld3(ro), dir = ld3(rd), mul = ld3(m4), psample = ld3(s4), acc = ld3(a4);
-    uint32_t N = __float_as_uint(a4.w);
-    uint32_t leaf = __float_as_uint(h.y);
-    bool term = false;
-    if (leaf == kInvalid) {
-      acc = acc + psample; N++; term = true;
-    } else {
-      float w = (1.f - h.z) - h.w;
-      const bool gbuf = (ac.variant & kVariantGbuf) && depth == 0;
-      Surf s = fetch_surf(sc, leaf, w, h.z, h.w, gbuf);
-      f3 nl = dot(s.norm, dir) < 0.f ? s.norm : s.norm * -1.f;
-      if (gbuf) {                                          // AdapBilatAtomic.cu:650-655
-        ac.rec.gpos[sid] = make_float4(s.pos.x, s.pos.y, s.pos.z, 0.f);
-        ac.rec.gnrm[sid] = make_float4(nl.x, nl.y, nl.z, 0.f);
-        ac.rec.gdif[sid] = make_float4(s.diff.x, s.diff.y, s.diff.z, 0.f);
-      }
-      Bounce b = sample_bounce(st, inc, orig, dir, h.x, s, nl, RT_EYE, (ac.variant & kVariantReflOffset) != 0);
-      psample = psample + vmul(s.emissive, mul);
-      f3 inMul = mul, inDir = dir * -1.f;
-      float contrib = length(mul) * length(b.mul);
-      if (length(s.emissive) > 0.f) contrib = 0.f;
-      if (uniform01(st, inc) > contrib || depth + 1 >= fa.eye_depth) {
-        acc = acc + psample; N++; term = true;
-      } else {
-        mul = vmul(mul, b.mul);
-        dir = b.dir;
-        depth++;
-        orig = b.orig;
-        if (b.type == RT_DIFF && lvN > 0) {
-          f3 pos = orig + nl * (float)kBiasEps;
-          uint32_t li = (uint32_t)(uniform01(st, inc) * (float)lvN);
-          if (li >= lvN) li = lvN - 1;
-          const LightVert& L = lvb[li];
-          f3 lpos = mk3(L.pos[0], L.pos[1], L.pos[2]);
-          f3 lirr = mk3(L.irrad[0], L.irrad[1], L.irrad[2]);
-          f3 toL = normalize(lpos - pos);
-          float dist = length(lpos - pos);
-          if (length(lirr) > 0.f && dot(nl, toL) > 0.f) {
-            f3 irrad, irradDir;
-            if ((double)dist > kFltEps) {
-              irrad = shading(toL * -1.f, mk3(L.irradDir[0], L.irradDir[1], L.irradDir[2]), lirr,
-                              mk3(L.norm[0], L.norm[1], L.norm[2]), mk3(L.diff[0], L.diff[1], L.diff[2]), L.metallic,
-                              L.roughness, L.specular, 1.f, 1.f) +
-                      mk3(L.emissive[0], L.emissive[1], L.emissive[2]);
-              irradDir = toL;
-            } else {
-              irrad = lirr;
-              irradDir = mk3(L.irradDir[0], L.irradDir[1], L.irradDir[2]) * -1.f;
-            }
-            f3 c = vmax0(shading(inDir, irradDir, irrad, nl, s.diff, s.metal, s.rough, s.spec, 1.f - s.trans, 1.f));
-            if (length(c) > 0.f) {
-              f3 pay = vmul(c, inMul);
-              emit = true;
-              so = make_float4(pos.x, pos.y, pos.z, (float)shadow_max((double)dist, fa.flags));
-              sd = make_float4(toL.x, toL.y, toL.z, (float)kBiasEps);
-              sc4 = make_float4(pay.x, pay.y, pay.z, 0.f);
-            }
-          }
-        }
-      }
-    }
-    if (term) {
-      if (ac.apply == 0) apply_pixel(ac.result, ac.sample_n, pix, frame, acc, N);
-      else if (ac.apply == 1) apply_pixel_gbuf(ac, pix, frame, acc, N, sid);
-      else ac.rec.acc[sid] = make_float4(acc.x, acc.y, acc.z, __uint_as_float(N));
-    } else {
-      survive = true;
-      o_out = make_float4(orig.x, orig.y, orig.z, kInfF);
-      d_out = make_float4(dir.x, dir.y, dir.z, (float)kBiasEps);
-      mul_out = make_float4(mul.x, mul.y, mul.z, 0.f);
-      smp_out = make_float4(psample.x, psample.y, psample.z, 0.f);
-      acc_out = make_float4(acc.x, acc.y, acc.z, __uint_as_float(N));
-    }
+    p.depth = dk & 0xFFu;
+    p.fk = dk >> 8;
+    p.mul = cur.mul[i];
+    p.smp = cur.smp[i];
+    p.acc = cur.acc[i];
+    p.rng = cur.rng[i];
+    shade_path(sc, fa, fc, lv, ac, p, hits[i], survive, emit, sr);
   }
   uint32_t slot = 0, sslot = 0;
   block_append2(survive, emit, &fc->active[bounce + 1], &fc->shadow[bounce], slot, sslot);
   if (survive) {
-    nxt.ray_o[slot] = o_out;
-    nxt.ray_d[slot] = d_out;
-    nxt.pix[slot] = pix;
-    nxt.sid[slot] = sid;
-    nxt.depth[slot] = depth | (fk << 8);
-    nxt.mul[slot] = mul_out;
-    nxt.smp[slot] = smp_out;
-    nxt.acc[slot] = acc_out;
-    nxt.rng[slot] = st;
+    nxt.ray_o[slot] = p.ro;
+    nxt.ray_d[slot] = p.rd;
+    nxt.pix[slot] = p.pix;
+    nxt.sid[slot] = p.sid;
+    nxt.depth[slot] = p.depth | (p.fk << 8);
+    nxt.mul[slot] = p.mul;
+    nxt.smp[slot] = p.smp;
+    nxt.acc[slot] = p.acc;
+    nxt.rng[slot] = p.rng;
   }
   if (emit) {
-    sc4.w = __uint_as_float(slot);
-    sb.ray_o[sslot] = so;
-    sb.ray_d[sslot] = sd;
-    sb.contrib[sslot] = sc4;
+    sr.c.w = __uint_as_float(slot);
+    sb.ray_o[sslot] = sr.o;
+    sb.ray_d[sslot] = sr.d;
+    sb.contrib[sslot] = sr.c;
   }
 }
 hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, PathBuf nxt, const float4* hits,
@@ -1361,6 +1422,108 @@ hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const Fr
 #undef CPT_SH_Q
 #undef CPT_SH_F
 #undef CPT_SH
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- megakernel
+// The reference's regeneration baseline (cudaRTBDPTRegen: one thread carries
+// its sample through every bounce, src/cudaRTBDPTRegen.cu) over this library's
+// functions: one thread per sample slot of the batch generates the eye path
+// (eye_path), and per bounce traces the closest hit (traverse4), shades it
+// (shade_path) and traces the connection's occlusion ray, adding the
+// contribution when nothing occludes it.  The per-path arithmetic and its order
+// are those of the wavefront passes (k_gen_eye, k_trace_dyn, k_shade,
+// k_shadow), so the image is the same bit for bit; only the schedule differs:
+// no path buffers, no compaction, and lanes of a wave idle once their paths
+// end.  The frame counters receive the same per-bounce counts.
+template <int STK, bool Q>
+__global__ __launch_bounds__(kBlock) void k_mega(DevScene sc, FrameArgs fa, FrameCounters* __restrict__ fc,
+                                                 const LightVert* __restrict__ lv, Accum ac,
+                                                 DevStats* __restrict__ stats) {
+  __shared__ uint64_t lstk_all[STK * kBlock];
+  __shared__ uint32_t n_live[16], n_shadow[16];
+  LdsU64* lstk = lds_column(lstk_all);
+  if (threadIdx.x < 16) { n_live[threadIdx.x] = 0u; n_shadow[threadIdx.x] = 0u; }
+  __syncthreads();
+  const uint32_t wh = fa.width * fa.height;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint32_t en = 0, et = 0, sn = 0, st = 0;
+  bool live = i < wh * fa.nframes;
+  PathState p;
+  if (live) {
+    const uint32_t k = i / wh, pix = i - k * wh;
+    p = eye_path(fa, pix, i, stream_key(fa.seed, frame_base(fa) + k, 1u, 0u));
+    p.fk = k;
+  }
+  const uint32_t depth = fa.eye_depth < 16u ? fa.eye_depth : 16u;
+  for (uint32_t b = 0; b < depth; b++) {
+    if (__ballot(live) == 0ull) break;                    // wave-uniform
+    bool survive = false, emit = false;
+    if (live) {
+      TraceHit h;
+      const bool hit = traverse4<false, STK, Q>(sc, ld3(p.ro), ld3(p.rd), p.ro.w, p.rd.w, false, lstk, h, en, et);
+      const float4 hv = make_float4(hit ? h.t : kInfF, __uint_as_float(hit ? h.leaf : kInvalid), h.u, h.v);
+      ShadowRay sr;
+      sr.o = sr.d = sr.c = make_float4(0.f, 0.f, 0.f, 0.f);
+      shade_path(sc, fa, fc, lv, ac, p, hv, survive, emit, sr);
+      if (emit) {
+        TraceHit hs;
+        if (!traverse4<true, STK, Q>(sc, ld3(sr.o), ld3(sr.d), sr.o.w, sr.d.w, false, lstk, hs, sn, st)) {
+          p.acc.x = p.acc.x + sr.c.x;
+          p.acc.y = p.acc.y + sr.c.y;
+          p.acc.z = p.acc.z + sr.c.z;
+          p.acc.w = __uint_as_float(__float_as_uint(p.acc.w) + 4u);
+        }
+      }
+      live = survive;
+    }
+    const uint32_t cs = (uint32_t)__popcll(__ballot(survive)), ce = (uint32_t)__popcll(__ballot(emit));
+    if (lane == 0) {
+      if (cs) atomicAdd(&n_live[b], cs);
+      if (ce) atomicAdd(&n_shadow[b], ce);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < depth) {
+    const uint32_t b = threadIdx.x;
+    if (b + 1 < 16 && n_live[b]) atomicAdd(&fc->active[b + 1], n_live[b]);
+    if (n_shadow[b]) atomicAdd(&fc->shadow[b], n_shadow[b]);
+  }
+  if (stats) {
+    const uint32_t a0 = wave_sum(en), a1 = wave_sum(et), a2 = wave_sum(sn), a3 = wave_sum(st);
+    if (lane == 0) {
+      if (a0 | a1) {
+        atomicAdd(&stats->ext_nodes, (unsigned long long)a0);
+        atomicAdd(&stats->ext_tris, (unsigned long long)a1);
+      }
+      if (a2 | a3) {
+        atomicAdd(&stats->sh_nodes, (unsigned long long)a2);
+        atomicAdd(&stats->sh_tris, (unsigned long long)a3);
+      }
+    }
+  }
+}
+hipError_t launch_mega(const DevScene& sc, const FrameArgs& fa, FrameCounters* fc, const LightVert* lv,
+                       const Accum& ac, int lds_stack, int quant, DevStats* stats, hipStream_t st) {
+  const uint64_t n = (uint64_t)fa.width * fa.height * fa.nframes;
+  if (n == 0) return hipSuccess;
+  if (n > 0xFFFFFFFFull - kBlock || !fc || !lv || !ac.rec.acc || ac.apply != 2 || (ac.variant & kVariantGbuf) ||
+      fa.eye_depth > 16 || !sc.tris ||
+      !(quant ? (const void*)sc.nodes4q : (const void*)sc.nodes4))
+    return hipErrorInvalidValue;
+  dim3 grid((uint32_t)((n + kBlock - 1) / kBlock)), block(kBlock);
+#define CPT_MG(S, W, Q) k_mega<S, Q><<<grid, block, 0, st>>>(sc, fa, fc, lv, ac, stats)
+  if (quant) {
+#define CPT_MG_Q(S, W) CPT_MG(S, W, true)
+#define CPT_MG_F(S, W) CPT_MG(S, W, false)
+    CPT_VARIANT_SWITCH(lds_stack, CPT_MG_Q)
+  } else {
+    CPT_VARIANT_SWITCH(lds_stack, CPT_MG_F)
+  }
+#undef CPT_MG_Q
+#undef CPT_MG_F
+#undef CPT_MG
   return hipGetLastError();
 }
 

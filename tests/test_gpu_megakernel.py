@@ -1,0 +1,81 @@
+"""The megakernel schedule (one thread carries its sample through every bounce,
+the reference's regeneration baseline, src/cudaRTBDPTRegen.cu) against the
+wavefront passes.
+
+Both run the same per-path functions in the same order, so the gate is
+equality: the same frames bit for bit, the same sample counts, and the same
+ray and traversal counters."""
+import numpy as np
+import pytest
+
+from cudapt_amd import scenes
+
+pytestmark = pytest.mark.gpu
+
+SCENES = {
+    "cornell": lambda: scenes.cornell(),
+    "textured_glass": lambda: scenes.textured_cornell(glass=True),
+    "sphere_grid": lambda: scenes.sphere_grid(grid=12),
+}
+
+
+@pytest.fixture(scope="module")
+def scene_objs(cpt):
+    out = {}
+    for k, f in SCENES.items():
+        sc = f()
+        out[k] = (sc, cpt.Scene(sc))
+    return out
+
+
+def _pair(cpt, W, H, **attribs):
+    a, b = cpt.Renderer(W, H), cpt.Renderer(W, H)
+    for R in (a, b):
+        for k, v in attribs.items():
+            R.set_attrib(k, v)
+    b.set_attrib("megakernel", 1)
+    return a, b
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("variant,node_format", [(806, 1), (16, 0)])
+def test_megakernel_same_image(cpt, scene_objs, name, variant, node_format):
+    sc, S = scene_objs[name]
+    a, b = _pair(cpt, 96, 64, **{"Node Format": node_format, "Traversal LDS Stack": variant,
+                                 "Shadow LDS Stack": variant, "in_flight": 3, "eye_depth": 8})
+    ra = a.render(S, sc["camera"], frames=7)
+    rb = b.render(S, sc["camera"], frames=7)
+    assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
+    assert np.array_equal(a.sample_counts(), b.sample_counts())
+    sa, sb = a.stats(), b.stats()
+    for k in ("extension_rays", "shadow_rays", "eye_paths", "frames", "extend_nodes", "extend_tris",
+              "shadow_nodes", "shadow_tris"):
+        assert sa[k] == sb[k], k
+    assert sb["shadow_rays"] > 0 and sb["extension_rays"] > sb["eye_paths"]
+
+
+def test_megakernel_large_batch_same_image(cpt, scene_objs):
+    """A batch above the refill threshold: the wavefront side runs the
+    refilling persistent traversal, the megakernel one thread per sample."""
+    sc, S = scene_objs["sphere_grid"]
+    a, b = _pair(cpt, 320, 240, **{"in_flight": 16})
+    ra = a.render(S, sc["camera"], frames=16)
+    rb = b.render(S, sc["camera"], frames=16)
+    assert 320 * 240 * 16 >= a.attribs()["Refill Min Paths"][2]
+    assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
+    assert a.stats()["extension_rays"] == b.stats()["extension_rays"]
+
+
+def test_megakernel_not_an_adaptive_attribute(cpt):
+    R = cpt.Renderer(16, 12, mode=cpt.CPT_MODE_BDPT_ADAPTIVE_PLL)
+    with pytest.raises(KeyError):
+        R.set_attrib("megakernel", 1)
+
+
+def test_megakernel_unknown_variant_is_an_error(cpt, scene_objs):
+    sc, S = scene_objs["cornell"]
+    R = cpt.Renderer(16, 12)
+    R.set_attrib("megakernel", 1)
+    R.set_attrib("Traversal LDS Stack", 805)
+    with pytest.raises(cpt.CptError):
+        R.render(S, sc["camera"], frames=1)
