@@ -21,6 +21,8 @@ CPT_MODE_BDPT_STREAM = 0
 CPT_MODE_BDPT_ADAPTIVE_PLL = 1
 CPT_MODE_BDPT_ADAPTIVE_BINARY = 2
 CPT_MODE_BDPT_ADAPTIVE_EMP = 3
+CPT_MODE_PT_STREAM = 4
+CPT_MODE_PT_STREAM_REGEN = 5
 (CPT_BUF_RESULT, CPT_BUF_SAMPLE_N, CPT_BUF_GBUF_POS, CPT_BUF_GBUF_NORM, CPT_BUF_GBUF_DIFF, CPT_BUF_FILTERED,
  CPT_BUF_ERROR, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_ALLOC_ERROR, CPT_BUF_SECOND_MOMENT) = range(10)
 MLP_INPUTS, MLP_HIDDEN, MLP_PARAMS = 9, 64, 4865

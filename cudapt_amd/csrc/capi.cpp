@@ -485,7 +485,8 @@ int cpt_trace_counters(cpt_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* 
 
 namespace {
 
-bool is_adaptive(int mode) { return mode != CPT_MODE_BDPT_STREAM; }
+bool is_adaptive(int mode) { return mode >= CPT_MODE_BDPT_ADAPTIVE_PLL && mode <= CPT_MODE_BDPT_ADAPTIVE_EMP; }
+bool is_pt(int mode) { return mode == CPT_MODE_PT_STREAM || mode == CPT_MODE_PT_STREAM_REGEN; }
 bool uses_filter(int mode) { return mode == CPT_MODE_BDPT_ADAPTIVE_PLL || mode == CPT_MODE_BDPT_ADAPTIVE_BINARY; }
 
 struct Attrib {
@@ -632,6 +633,8 @@ struct cpt_renderer {
   u32 last_rounds = 0;
   DevArray<double> block_sums;
   DevArray<float4> rec_acc, rec_pos, rec_nrm, rec_dif;
+  DevArray<u32> term_bounce, tend;           // path regeneration: where each path ended, per-frame last bounce
+  DevArray<uint64_t> term_rng;               // and the RNG state it ended in
   float gauss_sigma = -1.f;
   u32 gauss_radius = 0;
   u32 persist_chunk = 0, persist_grid = 0;   // persistent extend grid (0: one thread per ray)
@@ -696,6 +699,11 @@ int renderer_alloc(cpt_renderer* r) {
     HIP_TRY(hipMemsetAsync(r->counts.p, 0, sizeof(u32) * n, r->stream));
   }
   HIP_TRY(r->rec_acc.alloc(cap));
+  if (r->mode == CPT_MODE_PT_STREAM_REGEN) {
+    HIP_TRY(r->term_bounce.alloc(cap));
+    HIP_TRY(r->term_rng.alloc(cap));
+    HIP_TRY(r->tend.alloc(kMaxInFlight));
+  }
   HIP_TRY(r->queue.alloc(1));
   HIP_TRY(r->fc.alloc(1));
   HIP_TRY(r->stats.alloc(1));
@@ -917,6 +925,7 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
     HIP_TRY(timed_end(r, 3, ev));
   }
   Accum ac;
+  std::memset(&ac, 0, sizeof ac);
   ac.result = r->result.p; ac.sample_n = r->sample_n.p;
   ac.gpos = r->gpos.p; ac.gnrm = r->gnrm.p; ac.gdif = r->gdif.p;
   ac.sq = r->sq.p;
@@ -989,7 +998,10 @@ int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   DevScene sc = s->dev();
   const u32 n = r->W * r->H * fa.nframes;
   hipEvent_t ev = nullptr;
-  LightJobs jobs = light_jobs(r, fa.frame0, fa.nframes);
+  const bool pt = is_pt(r->mode), regen = r->mode == CPT_MODE_PT_STREAM_REGEN;
+  LightJobs jobs;
+  std::memset(&jobs, 0, sizeof jobs);
+  if (!pt) jobs = light_jobs(r, fa.frame0, fa.nframes);   // the path tracers have no light pass
   if (jobs.n) {
     HIP_TRY(timed_begin(r, 3, &ev));
     HIP_TRY(launch_light_pass(sc, fa, jobs, r->lv.p, r->fc.p, r->stream));
@@ -1002,8 +1014,13 @@ int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   ac.sample_n = r->sample_n.p;
   ac.rec.acc = r->rec_acc.p;
   ac.apply = 2;
-  ac.variant = 0;
-  if (r->attr(kMegakernel) != 0) {
+  ac.variant = pt ? kVariantPT | (regen ? kVariantPTRegen : 0) : 0;
+  if (regen) {
+    ac.term_bounce = r->term_bounce.p;
+    ac.term_rng = r->term_rng.p;
+    ac.tend = r->tend.p;
+  }
+  if (!pt && r->attr(kMegakernel) != 0) {
     r->extend_launches++;
     HIP_TRY(timed_begin(r, 0, &ev));
     HIP_TRY(launch_mega(sc, fa, r->fc.p, r->lv.p, ac, (int)r->attr(kLdsStack), r->attr(kNodeFormat) != 0,
@@ -1014,10 +1031,20 @@ int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
     int rc = bounce_loop(r, sc, fa, n, ac);
     if (rc) return rc;
   }
+  if (regen) {                                     // the restarts, as a second wavefront
+    HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, fa.nframes, fa.eye_depth, r->stream));
+    HIP_TRY(timed_begin(r, 7, &ev));
+    HIP_TRY(launch_regen(fa, r->A.view(), ac, r->fc.p, r->stream));
+    HIP_TRY(timed_end(r, 7, ev));
+    ac.term_bounce = nullptr;
+    ac.term_rng = nullptr;
+    int rc = bounce_loop(r, sc, fa, n, ac);
+    if (rc) return rc;
+  }
   HIP_TRY(timed_begin(r, 8, &ev));
-  HIP_TRY(launch_apply_frames(fa, r->rec_acc.p, r->result.p, r->sample_n.p, r->stream));
+  HIP_TRY(launch_apply_frames(fa, r->rec_acc.p, r->result.p, r->sample_n.p, pt ? 1 : 0, r->stream));
   HIP_TRY(timed_end(r, 8, ev));
-  HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, fa.nframes, fa.eye_depth, r->stream));
+  HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, regen ? 0u : fa.nframes, fa.eye_depth, r->stream));
   return CPT_OK;
 }
 
@@ -1108,7 +1135,7 @@ extern "C" {
 
 int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t height) {
   if (!out) return fail(CPT_E_ARG, "cpt_renderer_create: null output");
-  if (mode < CPT_MODE_BDPT_STREAM || mode > CPT_MODE_BDPT_ADAPTIVE_EMP)
+  if (mode < CPT_MODE_BDPT_STREAM || mode > CPT_MODE_PT_STREAM_REGEN)
     return fail(CPT_E_ARG, "cpt_renderer_create: unknown mode %d", mode);
   if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 26))
     return fail(CPT_E_LIMIT, "cpt_renderer_create: %ux%u outside 1..2^26 pixels", width, height);
@@ -1132,7 +1159,8 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
                   {"Avg Gen Time", 3, 0},           {"Avg Trace Time", 3, 0},
                   {"Avg Accum Time", 3, 0}};
   }
-  r->attribs.push_back({kEyeDepth, 0, 5});
+  // NORMALRAY_BOUND_MAX: 5 in the BDPT files, 10 in the path tracers (cudaRTPTStream.cu:7)
+  r->attribs.push_back({kEyeDepth, 0, is_pt(mode) ? 10.0 : 5.0});
   r->attribs.push_back({kLightDepth, 0, 5});
   r->attribs.push_back({kLightPaths, 0, 128});
   r->attribs.push_back({kSeed, 0, (double)0x5EED0001u});
@@ -1143,7 +1171,7 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
     r->attribs.push_back({kLearnedSignal, 0, 0});
   } else {
     r->attribs.push_back({kInFlight, 0, 24});
-    r->attribs.push_back({kMegakernel, 0, 0});
+    if (mode == CPT_MODE_BDPT_STREAM) r->attribs.push_back({kMegakernel, 0, 0});
   }
   // traversal kernel variant: 8 LDS stack entries, >= 6 waves per SIMD
   // (swept on the sphere grid: 1.58 vs 1.36 Grays/s for 16 entries)
@@ -1603,7 +1631,7 @@ int batch_enqueue(cpt_batch* b, cpt_scene* s, u32 k) {
   ac.apply = 2;
   int rc = bounce_loop(r, sc, fa, n, ac);
   if (rc) return rc;
-  HIP_TRY(launch_apply_frames(fa, r->rec_acc.p, r->result.p, r->sample_n.p, r->stream));
+  HIP_TRY(launch_apply_frames(fa, r->rec_acc.p, r->result.p, r->sample_n.p, 0, r->stream));
   HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, k, fa.eye_depth, r->stream));
   HIP_TRY(launch_frame_advance(b->frame_dev.p, k, r->stream));
   return CPT_OK;

@@ -137,7 +137,7 @@ hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, const Ligh
 hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st);
 // Running-mean update of every pixel from the batch's per-frame records, frames in order.
 hipError_t launch_apply_frames(const FrameArgs& fa, const float4* rec_acc, float* result, uint32_t* sample_n,
-                               hipStream_t st);
+                               int pt, hipStream_t st);
 // lds_stack: traversal variant, the stack entries kept in LDS (the rest in
 // private memory) and the minimum resident waves per SIMD the register
 // allocator is held to: 8, 16, 24 (no minimum), 806, 807, 808 (8 entries, >= 6,
@@ -158,10 +158,18 @@ struct Accum {
   float* sq;           // W*H*3 running mean of squared samples (adaptive), or null
   SampleRec rec;
   int apply;
-  int variant;         // bit 0: reflection offset of AdapBilatAtomic's eye bounce; bit 1: first-hit G-buffer
+  int variant;         // kVariant* bits
+  // path regeneration, first wavefront: per sample slot, the bounce its path
+  // ended at and its RNG state then (null: not recorded); per frame of the
+  // batch, the last such bounce
+  uint32_t* term_bounce;
+  uint64_t* term_rng;
+  uint32_t* tend;
 };
-constexpr int kVariantReflOffset = 1;
-constexpr int kVariantGbuf = 2;
+constexpr int kVariantReflOffset = 1;  // reflection offset of AdapBilatAtomic's eye bounce
+constexpr int kVariantGbuf = 2;        // first-hit G-buffer
+constexpr int kVariantPT = 4;          // path tracing: no connection, total internal reflection reflects
+constexpr int kVariantPTRegen = 8;     // the regenerating path tracer's diffuse-to-specular cut
 hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, PathBuf nxt, const float4* hits,
                         FrameCounters* fc, int bounce, uint32_t max_n, ShadowBuf sb, const LightVert* lv,
                         const Accum& ac, hipStream_t st);
@@ -170,6 +178,9 @@ hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, Pa
 // into ac.rec.acc (ac.apply must be 2), counters into fc and stats.
 hipError_t launch_mega(const DevScene& sc, const FrameArgs& fa, FrameCounters* fc, const LightVert* lv,
                        const Accum& ac, int lds_stack, int quant, DevStats* stats, hipStream_t st);
+// Path regeneration's second wavefront: T per frame from ac.term_bounce, the
+// counters of fc reset, and the restarted paths appended to pb.
+hipError_t launch_regen(const FrameArgs& fa, PathBuf pb, const Accum& ac, FrameCounters* fc, hipStream_t st);
 hipError_t launch_frame_begin_dev(FrameCounters* fc, const uint32_t* total, uint32_t cap, hipStream_t st);
 hipError_t launch_gen_eye_rec(const FrameArgs& fa, PathBuf pb, SampleRec rec, hipStream_t st);
 

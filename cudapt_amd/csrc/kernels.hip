@@ -560,8 +560,11 @@ struct Bounce {
   int type;
 };
 // BRDF sample of one bounce (src/cudaRTBDPTStream.cu:228-329 and :402-502).
+// tir_reflect: a refraction past the critical angle turns into a reflection
+// (the path tracers, cudaRTPTStream.cu:152-160); the BDPT files take the
+// refracted direction as computed.
 __device__ Bounce sample_bounce(uint64_t& st, uint64_t inc, f3 ro, f3 rd, float t, const Surf& s, f3 nl, int cur,
-                                bool refl_offset = false) {
+                                bool refl_offset = false, bool tir_reflect = false) {
   float x0 = uniform01(st, inc), x1 = uniform01(st, inc);
   f3 hDir = sample_ggx(x0, x1, s.rough, nl);
   f3 ndir = rd * -1.f;
@@ -579,7 +582,8 @@ __device__ Bounce sample_bounce(uint64_t& st, uint64_t inc, f3 ro, f3 rd, float 
     float nnt = into ? nc / nt : nt / nc;
     float ddn = dot(hDir, rd);
     float cos2t = 1.f - nnt * nnt * (1.f - ddn * ddn);
-    refrDir = normalize(rd * nnt - hDir * (ddn * nnt + sqrtf(cos2t)));
+    if (tir_reflect && cos2t < 0.f) reflProb = 1.f;
+    else refrDir = normalize(rd * nnt - hDir * (ddn * nnt + sqrtf(cos2t)));
   }
   if (reflProb > 0.f) {
     reflDir = normalize(rd - (hDir * 2.f) * dot(hDir, rd));
@@ -641,6 +645,9 @@ __device__ __forceinline__ uint32_t local_pixel(const FrameArgs& fa, uint32_t pi
 
 // ---------------------------------------------------------------- frame
 __global__ void k_fill_u32(uint32_t* p, uint32_t v) { *p = v; }
+__global__ void k_fill_n_u32(uint32_t* p, uint32_t n, uint32_t v) {
+  for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
 __global__ void k_frame_begin(FrameCounters* fc, uint32_t n_paths) {
   uint32_t i = threadIdx.x;
   if (i < 16) { fc->active[i] = i == 0 ? n_paths : 0u; fc->shadow[i] = 0u; }
@@ -824,8 +831,21 @@ __device__ __forceinline__ void apply_pixel(float* result, uint32_t* sample_n, u
 struct PathState {
   float4 ro, rd, mul, smp, acc;
   uint32_t pix, sid, depth, fk;
+  uint32_t lim;    // depth bound of this path (0: the frame's eye depth)
+  uint32_t type;   // type of the ray being traced (RT_EYE, RT_DIFF, RT_SPEC)
   uint64_t rng;
 };
+// PathBuf::depth packs depth (bits 0-7), frame offset (8-15), depth bound
+// (16-23) and ray type (24-25).
+__device__ __forceinline__ uint32_t pack_dk(const PathState& p) {
+  return p.depth | (p.fk << 8) | (p.lim << 16) | (p.type << 24);
+}
+__device__ __forceinline__ void unpack_dk(PathState& p, uint32_t dk) {
+  p.depth = dk & 0xFFu;
+  p.fk = (dk >> 8) & 0xFFu;
+  p.lim = (dk >> 16) & 0xFFu;
+  p.type = dk >> 24;
+}
 // The connection's shadow ray: origin and max distance, direction and epsilon,
 // contribution added to the path's accumulator when nothing occludes it.
 struct ShadowRay {
@@ -837,7 +857,8 @@ struct ShadowRay {
 // of AdapBilatAtomic.cu:826-840 uses the same pinhole).  In a multi-view batch
 // `pix` is the batch-global pixel: the camera and the pixel's position come
 // from its view, the RNG stream stays keyed by the global pixel.
-__device__ __forceinline__ PathState eye_path(const FrameArgs& fa, uint32_t pix, uint32_t sid, uint64_t key) {
+// st: the path's PCG32 state, positioned where the jitter draws start.
+__device__ __forceinline__ PathState eye_path_at(const FrameArgs& fa, uint32_t pix, uint32_t sid, uint64_t st) {
   uint32_t lp = pix, vw = fa.width, vh = fa.height;
   const float *cpos = fa.cam_pos, *cdir = fa.cam_dir, *cup = fa.cam_up, *cright = fa.cam_right;
   float fov = fa.fov;
@@ -854,8 +875,7 @@ __device__ __forceinline__ PathState eye_path(const FrameArgs& fa, uint32_t pix,
   float tf = tanf(fov * 0.5f);
   float u = ((((2.f * ((float)x + 0.5f)) / W - 1.f) * tf) * W) / H;
   float v = ((2.f * ((float)y + 0.5f)) / H - 1.f) * tf;
-  uint64_t inc = ((uint64_t)lp << 1) | 1u, st;
-  pcg32_seed(st, inc, key);
+  const uint64_t inc = ((uint64_t)lp << 1) | 1u;
   float au = u + ((uniform01(st, inc) - 0.5f) / H) * tf;
   float av = v + ((uniform01(st, inc) - 0.5f) / H) * tf;
   f3 right = mk3(cright[0], cright[1], cright[2]);
@@ -869,11 +889,19 @@ __device__ __forceinline__ PathState eye_path(const FrameArgs& fa, uint32_t pix,
   p.sid = sid;
   p.depth = 0;
   p.fk = 0;
+  p.lim = 0;
+  p.type = RT_EYE;
   p.mul = make_float4(1.f, 1.f, 1.f, 0.f);
   p.smp = make_float4(0.f, 0.f, 0.f, 0.f);
   p.acc = make_float4(0.f, 0.f, 0.f, 0.f);
   p.rng = st;
   return p;
+}
+__device__ __forceinline__ PathState eye_path(const FrameArgs& fa, uint32_t pix, uint32_t sid, uint64_t key) {
+  const uint64_t inc = ((uint64_t)local_pixel(fa, pix) << 1) | 1u;
+  uint64_t st;
+  pcg32_seed(st, inc, key);
+  return eye_path_at(fa, pix, sid, st);
 }
 __device__ __forceinline__ void write_eye_path(const FrameArgs& fa, PathBuf& pb, uint32_t slot, uint32_t pix,
                                                uint32_t sid, uint64_t key) {
@@ -918,24 +946,45 @@ hipError_t launch_gen_eye_rec(const FrameArgs& fa, PathBuf pb, SampleRec rec, hi
   return hipGetLastError();
 }
 
+// The path tracers' running mean (pt_applyPathQueueResult_kernel,
+// src/cudaRTPTStream.cu:303-325): the frame's per-pixel mean acc / n weighted
+// 1 / (frame + 1), whatever the number of samples the pixel took.
+__device__ __forceinline__ void apply_pixel_pt(float* result, uint32_t* sample_n, uint32_t pix, uint32_t frame, f3 acc,
+                                               uint32_t n) {
+  const uint32_t nold = frame == 0 ? 0u : sample_n[pix];
+  if (n > 0) {
+    const float fn = (float)n;
+    const f3 sr = mk3(acc.x / fn, acc.y / fn, acc.z / fn);
+    const float ri = 1.f / (float)(frame + 1), oi = 1.f - ri;
+    float* r = result + (size_t)pix * 3;
+    r[0] = fmaxf(ri * sr.x + oi * r[0], 0.f);
+    r[1] = fmaxf(ri * sr.y + oi * r[1], 0.f);
+    r[2] = fmaxf(ri * sr.z + oi * r[2], 0.f);
+  }
+  sample_n[pix] = nold + n;
+}
+
 // Frames of the batch applied in order: pt_applyPathQueueResult_kernel's
-// running mean (src/cudaRTBDPTStream.cu:741-778) once per frame.
+// running mean (src/cudaRTBDPTStream.cu:741-778) once per frame (pt: the path
+// tracers' mean).
 __global__ __launch_bounds__(kBlock) void k_apply_frames(FrameArgs fa, const float4* __restrict__ rec_acc,
-                                                         float* __restrict__ result, uint32_t* __restrict__ sample_n) {
+                                                         float* __restrict__ result, uint32_t* __restrict__ sample_n,
+                                                         int pt) {
   const uint32_t wh = fa.width * fa.height;
   const uint32_t pix = blockIdx.x * kBlock + threadIdx.x;
   if (pix >= wh) return;
   const uint32_t f0 = frame_base(fa);
   for (uint32_t k = 0; k < fa.nframes; k++) {
     float4 a = rec_acc[(size_t)k * wh + pix];
-    apply_pixel(result, sample_n, pix, f0 + k, ld3(a), __float_as_uint(a.w));
+    if (pt) apply_pixel_pt(result, sample_n, pix, f0 + k, ld3(a), __float_as_uint(a.w));
+    else apply_pixel(result, sample_n, pix, f0 + k, ld3(a), __float_as_uint(a.w));
   }
 }
 hipError_t launch_apply_frames(const FrameArgs& fa, const float4* rec_acc, float* result, uint32_t* sample_n,
-                               hipStream_t st) {
+                               int pt, hipStream_t st) {
   if (!rec_acc || !result || !sample_n) return hipErrorInvalidValue;
   uint32_t wh = fa.width * fa.height;
-  k_apply_frames<<<(wh + kBlock - 1) / kBlock, kBlock, 0, st>>>(fa, rec_acc, result, sample_n);
+  k_apply_frames<<<(wh + kBlock - 1) / kBlock, kBlock, 0, st>>>(fa, rec_acc, result, sample_n, pt);
   return hipGetLastError();
 }
 
@@ -1246,19 +1295,25 @@ __device__ __forceinline__ void shade_path(const DevScene& sc, const FrameArgs& 
       ac.rec.gnrm[sid] = make_float4(nl.x, nl.y, nl.z, 0.f);
       ac.rec.gdif[sid] = make_float4(s.diff.x, s.diff.y, s.diff.z, 0.f);
     }
-    Bounce b = sample_bounce(st, inc, orig, dir, h.x, s, nl, RT_EYE, (ac.variant & kVariantReflOffset) != 0);
+    const bool pt = (ac.variant & kVariantPT) != 0;
+    Bounce b = sample_bounce(st, inc, orig, dir, h.x, s, nl, RT_EYE, (ac.variant & kVariantReflOffset) != 0, pt);
     psample = psample + vmul(s.emissive, mul);
     f3 inMul = mul, inDir = dir * -1.f;
     float contrib = length(mul) * length(b.mul);
     if (length(s.emissive) > 0.f) contrib = 0.f;
-    if (uniform01(st, inc) > contrib || depth + 1 >= fa.eye_depth) {
+    // the regenerating path tracer also ends a diffuse-to-specular step
+    // (cudaRTPTStreamRegen.cu:241)
+    if ((ac.variant & kVariantPTRegen) && p.type == RT_DIFF && b.type == RT_SPEC) contrib = 0.f;
+    const uint32_t lim = p.lim ? p.lim : fa.eye_depth;
+    if (uniform01(st, inc) > contrib || depth + 1 >= lim) {
       acc = acc + psample; N++; term = true;
     } else {
       mul = vmul(mul, b.mul);
       dir = b.dir;
       depth++;
       orig = b.orig;
-      if (b.type == RT_DIFF && lvN > 0) {
+      p.type = b.type;
+      if (!pt && b.type == RT_DIFF && lvN > 0) {
         f3 pos = orig + nl * (float)kBiasEps;
         uint32_t li = (uint32_t)(uniform01(st, inc) * (float)lvN);
         if (li >= lvN) li = lvN - 1;
@@ -1292,6 +1347,10 @@ __device__ __forceinline__ void shade_path(const DevScene& sc, const FrameArgs& 
     }
   }
   if (term) {
+    if (ac.term_bounce) {                                 // regeneration: where and in which state it ended
+      ac.term_bounce[sid] = depth;
+      ac.term_rng[sid] = st;
+    }
     if (ac.apply == 0) apply_pixel(ac.result, ac.sample_n, pix, frame, acc, N);
     else if (ac.apply == 1) apply_pixel_gbuf(ac, pix, frame, acc, N, sid);
     else ac.rec.acc[sid] = make_float4(acc.x, acc.y, acc.z, __uint_as_float(N));
@@ -1332,9 +1391,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
     p.rd = cur.ray_d[i];
     p.pix = cur.pix[i];
     p.sid = cur.sid[i];
-    const uint32_t dk = cur.depth[i];
-    p.depth = dk & 0xFFu;
-    p.fk = dk >> 8;
+    unpack_dk(p, cur.depth[i]);
     p.mul = cur.mul[i];
     p.smp = cur.smp[i];
     p.acc = cur.acc[i];
@@ -1348,7 +1405,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
     nxt.ray_d[slot] = p.rd;
     nxt.pix[slot] = p.pix;
     nxt.sid[slot] = p.sid;
-    nxt.depth[slot] = p.depth | (p.fk << 8);
+    nxt.depth[slot] = pack_dk(p);
     nxt.mul[slot] = p.mul;
     nxt.smp[slot] = p.smp;
     nxt.acc[slot] = p.acc;
@@ -1422,6 +1479,97 @@ hipError_t launch_shadow(const DevScene& sc, ShadowBuf sb, PathBuf nxt, const Fr
 #undef CPT_SH_Q
 #undef CPT_SH_F
 #undef CPT_SH
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- path regeneration
+// cudaRTPTStreamRegen (src/cudaRTPTStreamRegen.cu:300-478): a slot whose path
+// ends while other paths of the frame are still being traced restarts once,
+// for the same pixel with its RNG state continuing (pt_reinitRegenPathStream_kernel),
+// and the frame stops when its last first-generation path ends; a restarted
+// path still running then adds what it gathered (pt_applyPathQueueResult_kernel).
+// While the stream holds the whole frame (W*H <= 640,000 there), a path that
+// ended at bounce b runs bounces b+1 .. T of the restart, T the frame's last
+// first-generation bounce, which no other path influences.  So the restarts
+// are traced as a second wavefront after the first: tend[k] = T of frame k,
+// and the restart's depth bound is min(eye depth, T - b).
+
+// tend[k] = max over the frame's slots of the bounce its path ended at.
+__global__ __launch_bounds__(kBlock) void k_term_max(const uint32_t* __restrict__ tb, uint32_t wh,
+                                                     uint32_t* __restrict__ tend) {
+  __shared__ uint32_t wmax[kBlock / 64];
+  const uint32_t k = blockIdx.y;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  uint32_t v = i < wh ? tb[(size_t)k * wh + i] : 0u;
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v = max(v, (uint32_t)__shfl_xor(v, o, 64));
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = 0;
+    for (int w = 0; w < kBlock / 64; w++) m = max(m, wmax[w]);
+    if (m) atomicMax(&tend[k], m);
+  }
+}
+
+// The restarts: slot i of frame k when its path ended before tend[k].
+__global__ __launch_bounds__(kBlock) void k_gen_regen(FrameArgs fa, PathBuf pb, const uint32_t* __restrict__ tb,
+                                                      const uint64_t* __restrict__ trng,
+                                                      const uint32_t* __restrict__ tend,
+                                                      const float4* __restrict__ rec_acc,
+                                                      FrameCounters* __restrict__ fc) {
+  __shared__ uint32_t wc[kBlock / 64], base;
+  const uint32_t wh = fa.width * fa.height;
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  bool take = false;
+  PathState p;
+  if (i < wh * fa.nframes) {
+    const uint32_t k = i / wh, pix = i - k * wh;
+    const uint32_t b = tb[i], T = tend[k];
+    if (b < T) {
+      take = true;
+      p = eye_path_at(fa, pix, i, trng[i]);
+      p.fk = k;
+      p.lim = min(T - b, fa.eye_depth);
+      p.acc = rec_acc[i];
+    }
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const unsigned long long m = __ballot(take);
+  if (lane == 0) wc[wid] = (uint32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kBlock / 64; w++) t += wc[w];
+    base = t ? atomicAdd(&fc->active[0], t) : 0u;
+  }
+  __syncthreads();
+  if (take) {
+    uint32_t slot = base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    for (int w = 0; w < wid; w++) slot += wc[w];
+    pb.ray_o[slot] = p.ro;
+    pb.ray_d[slot] = p.rd;
+    pb.pix[slot] = p.pix;
+    pb.sid[slot] = p.sid;
+    pb.depth[slot] = pack_dk(p);
+    pb.mul[slot] = p.mul;
+    pb.smp[slot] = p.smp;
+    pb.acc[slot] = p.acc;
+    pb.rng[slot] = p.rng;
+  }
+}
+hipError_t launch_regen(const FrameArgs& fa, PathBuf pb, const Accum& ac, FrameCounters* fc, hipStream_t st) {
+  const uint32_t wh = fa.width * fa.height;
+  const uint64_t n = (uint64_t)wh * fa.nframes;
+  if (n == 0) return hipSuccess;
+  if (!pathbuf_ok(pb) || !fc || !ac.term_bounce || !ac.term_rng || !ac.tend || !ac.rec.acc || fa.nframes > 255 ||
+      fa.views)
+    return hipErrorInvalidValue;
+  k_fill_n_u32<<<1, kBlock, 0, st>>>(ac.tend, fa.nframes, 0u);
+  k_term_max<<<dim3((wh + kBlock - 1) / kBlock, fa.nframes), kBlock, 0, st>>>(ac.term_bounce, wh, ac.tend);
+  k_frame_begin<<<1, 64, 0, st>>>(fc, 0u);
+  k_gen_regen<<<(uint32_t)((n + kBlock - 1) / kBlock), kBlock, 0, st>>>(fa, pb, ac.term_bounce, ac.term_rng, ac.tend,
+                                                                         ac.rec.acc, fc);
   return hipGetLastError();
 }
 

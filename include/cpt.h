@@ -178,8 +178,12 @@ enum {
                                         filter residual, counts proportional to the error */
   CPT_MODE_BDPT_ADAPTIVE_BINARY = 2, /* cudaRTBDPTStreamAdapBilatBinary (src/cudaRTBDPTStreamAdapBilatBinary.cu):
                                         filter residual, rounds with a bisected acceptance window */
-  CPT_MODE_BDPT_ADAPTIVE_EMP = 3     /* cudaRTBDPTStreamAdapEmp (src/cudaRTBDPTStreamAdapEmp.cu): empirical
+  CPT_MODE_BDPT_ADAPTIVE_EMP = 3,    /* cudaRTBDPTStreamAdapEmp (src/cudaRTBDPTStreamAdapEmp.cu): empirical
                                         variance, rounds with the window at the maximum error */
+  CPT_MODE_PT_STREAM = 4,            /* cudaRTPTStream (src/cudaRTPTStream.cu): unidirectional path tracing
+                                        over the same stream, eye depth 10, a per-frame running mean */
+  CPT_MODE_PT_STREAM_REGEN = 5       /* cudaRTPTStreamRegen (src/cudaRTPTStreamRegen.cu): as 4, and a pixel
+                                        whose path ends while the frame is still tracing restarts once */
 };
 
 /* RTRenderer::Init(w, h) + SetRendererMode(mode) (src/raytracer.cpp:574-608). */

@@ -40,6 +40,8 @@ def lib():
         L.orc_tex_sample.argtypes = [vp, c32, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                      np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
         L.orc_scene_bvh.restype = c32
+        L.orc_sample_bounce.argtypes = [f32p, f32p, ctypes.c_float, f32p, f32p, c64, c64, ctypes.c_int, f32p]
+        L.orc_sample_bounce.restype = ctypes.c_int
         L.orc_scene_bvh.argtypes = [vp, vp, vp, vp, c32]
         L.orc_scene_light_tris.restype = c32
         L.orc_scene_light_tris.argtypes = [vp, vp, c32]
@@ -234,6 +236,15 @@ class Renderer:
         out = np.zeros((2048, 21), np.float32)
         n = lib().orc_light_vertices(self.h, out.ravel(), 2048)
         return out[:n]
+
+
+def sample_bounce(ro, rd, t, norm, diff, trans, specular, metallic, roughness, nl, state, seq, tir=False):
+    """One FP32 BRDF sample at a hit: (origin, direction, throughput, ray type 1 diffuse / 2 specular)."""
+    f = lambda v: np.ascontiguousarray(v, np.float32)
+    hs = f([*norm, *diff, trans, specular, metallic, roughness])
+    out = np.zeros(9, np.float32)
+    ty = lib().orc_sample_bounce(f(ro), f(rd), float(t), hs, f(nl), int(state), int(seq), int(bool(tir)), out)
+    return out[0:3], out[3:6], out[6:9], int(ty)
 
 
 def bilateral(color, pos, norm, emit, sc, sp, sn, se, radius, prec=32):

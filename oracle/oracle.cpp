@@ -492,7 +492,9 @@ struct RenderParams {
   u32 row0, rows; // rows [row0, row0 + rows) of the frame are traced (rows = 0: all)
   // adaptive mode (cudaRTBDPTStreamAdapBilatAtomic.cu:26-38)
   u32 mode;        // 0 plain (cudaRTBDPTStream), 1 adaptive proportional (AdapBilatAtomic),
-                   // 2 rounds with bisection (AdapBilatBinary), 3 empirical variance rounds (AdapEmp)
+                   // 2 rounds with bisection (AdapBilatBinary), 3 empirical variance rounds (AdapEmp),
+                   // 4 path tracing (cudaRTPTStream), 5 with regeneration (cudaRTPTStreamRegen),
+                   // 6 as 5 through the reference's own loop (render_pt_literal)
   u32 nd;          // DesiredSamplingN
   float sc, sp, sn, sd;   // filter colour / position / normal / diffuse sigmas
   u32 radius;      // filter radius
@@ -588,8 +590,10 @@ enum { RT_EYE = 0, RT_DIFF = 1, RT_SPEC = 2, RT_LIGHT = 3 };
 // (cudaRTBDPTStream.cu:228-329 and :402-502).
 template <class R> struct Bounce { V3<R> orig, dir, mul; int type; };
 template <class R>
+// tir_reflect: the path tracers turn a refraction past the critical angle into
+// a reflection (cudaRTPTStream.cu:152-160); the BDPT files do not.
 static Bounce<R> sample_bounce(Pcg32& rng, V3<R> ro, V3<R> rd, R t, const HitShade<R>& hs, V3<R> nl, int curType,
-                               bool refl_offset = false) {
+                               bool refl_offset = false, bool tir_reflect = false) {
   R x0 = uniform<R>(rng), x1 = uniform<R>(rng);
   V3<R> hDir = sample_ggx(x0, x1, hs.roughness, nl);
   V3<R> ndir = rd * (R)-1;
@@ -606,7 +610,8 @@ static Bounce<R> sample_bounce(Pcg32& rng, V3<R> ro, V3<R> rd, R t, const HitSha
     R nnt = into ? nc / nt : nt / nc;
     R ddn = dot(hDir, rd);
     R cos2t = (R)1 - nnt * nnt * ((R)1 - ddn * ddn);
-    refrDir = normalize(rd * nnt - hDir * (ddn * nnt + std::sqrt(cos2t)));
+    if (tir_reflect && cos2t < (R)0) reflProb = (R)1;
+    else refrDir = normalize(rd * nnt - hDir * (ddn * nnt + std::sqrt(cos2t)));
   }
   if (reflProb > (R)0) {
     reflDir = normalize(rd - (hDir * (R)2) * dot(hDir, rd));
@@ -840,6 +845,168 @@ static void render_frames(const Scene& s, const RenderParams& p, u32 frame0, u32
       }
     }
     fs.ext_rays += er; fs.shadow_rays += sr;
+  }
+}
+
+// ---------------------------------------------------------------- path tracers
+// One path of cudaRTPTStream / cudaRTPTStreamRegen (PTPathVertex,
+// cudaRTPTStream.cu:22-48): pt_begin draws the eye ray from the path's RNG
+// (pt_genPathQueue_kernel :266-284, pt_reinitRegenPathStream_kernel
+// cudaRTPTStreamRegen.cu:306-326, which keep accum and N), pt_step is one
+// bounce of pt_traceSample_kernel (cudaRTPTStream.cu:91-264): emission
+// gathered, BRDF sample with total internal reflection turned into a
+// reflection, Russian roulette, no connection.  regen: the diffuse-to-specular
+// cut of cudaRTPTStreamRegen.cu:241.
+template <class R> struct PtPath {
+  Pcg32 rng;
+  V3<R> orig, dir, mul, psample, accum;
+  u32 N = 0, depth = 0, x = 0, y = 0;
+  int type = RT_EYE;
+  bool ended = false;
+};
+template <class R>
+static void pt_begin(const RenderParams& p, const Camera<R>& cam, PtPath<R>& P) {
+  P.dir = eye_dir(cam, P.x, P.y, (R)p.width, (R)p.height, P.rng);
+  P.orig = cam.pos;
+  P.mul = mk((R)1, (R)1, (R)1);
+  P.psample = mk((R)0, (R)0, (R)0);
+  P.depth = 0;
+  P.type = RT_EYE;
+  P.ended = false;
+}
+// Returns true when the path ends (its sample added to accum, N).
+template <class R>
+static bool pt_step(const Scene& s, PtPath<R>& P, u32 limit, bool regen, u64& rays) {
+  HitR<R> h;
+  rays++;
+  if (!trace_defined<R>(s, P.orig, P.dir, (R)kInf, (R)kBiasEps, false, h)) {
+    P.accum = P.accum + P.psample; P.N++; P.ended = true;
+    return true;
+  }
+  R w = ((R)1 - h.u) - h.v;
+  HitShade<R> hs = fetch_hit<R>(s, h.leaf, w, h.u, h.v);
+  V3<R> nl = dot(hs.norm, P.dir) < (R)0 ? hs.norm : hs.norm * (R)-1;
+  Bounce<R> b = sample_bounce<R>(P.rng, P.orig, P.dir, h.t, hs, nl, RT_EYE, false, true);
+  P.psample = P.psample + vmul(hs.emissive, P.mul);
+  R contrib = length(P.mul) * length(b.mul);
+  if (length(hs.emissive) > (R)0) contrib = (R)0;
+  if (regen && P.type == RT_DIFF && b.type == RT_SPEC) contrib = (R)0;
+  if (uniform<R>(P.rng) > contrib || P.depth + 1 >= limit) {
+    P.accum = P.accum + P.psample; P.N++; P.ended = true;
+    return true;
+  }
+  P.mul = vmul(P.mul, b.mul); P.dir = b.dir; P.depth++;
+  P.orig = b.orig; P.type = b.type;
+  return false;
+}
+
+// pt_applyPathQueueResult_kernel (cudaRTPTStream.cu:303-325): the frame's
+// per-pixel mean weighted 1 / (frame + 1).
+template <class R>
+static inline void apply_pt(R* res, u32& Nold, u32 frame, V3<R> accum, u32 n) {
+  if (n > 0) {
+    R fn = (R)n;
+    R sx = accum.x / fn, sy = accum.y / fn, sz = accum.z / fn;
+    R ri = (R)1 / (R)(frame + 1), oi = (R)1 - ri;
+    res[0] = std::fmax(ri * sx + oi * res[0], (R)0);
+    res[1] = std::fmax(ri * sy + oi * res[1], (R)0);
+    res[2] = std::fmax(ri * sz + oi * res[2], (R)0);
+  }
+  Nold += n;
+}
+
+// Frames of the path tracers (mode 4: cudaRTPTStream::Render, :327-432; mode 5:
+// cudaRTPTStreamRegen::Render, :374-478), all rows.  Regeneration: a pixel whose
+// path ended at bounce b < T, T the frame's last bounce of a first path,
+// restarts once with its RNG continuing and traces bounces b+1 .. T (depth
+// bound min(eye depth, T - b)); a restart still running at T adds what it
+// gathered, as an ended one does.  That is the reference's schedule while its
+// stream holds the whole frame (W*H <= 640,000); render_pt_literal runs that
+// schedule itself.
+template <class R>
+static void render_pt(const Scene& s, const RenderParams& p, u32 frame0, u32 nframes, FrameState<R>& fs) {
+  Camera<R> cam = make_camera<R>(p);
+  const u32 W = p.width, H = p.height, n = W * H;
+  const bool regen = p.mode == 5;
+  std::vector<PtPath<R>> P(n);
+  std::vector<u32> term(n);
+  for (u32 f = frame0; f < frame0 + nframes; f++) {
+    if (f == 0) std::fill(fs.N.begin(), fs.N.end(), 0u);
+    u64 er = 0;
+    u32 T = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : er) reduction(max : T)
+    for (i32 y = 0; y < (i32)H; y++) {
+      for (u32 x = 0; x < W; x++) {
+        const u32 pix = (u32)y * W + x;
+        PtPath<R>& A = P[pix];
+        A = PtPath<R>();
+        A.x = x; A.y = (u32)y;
+        pcg32_seed(A.rng, stream_key(p.seed, f, DOMAIN_EYE, 0), pix);
+        A.accum = mk((R)0, (R)0, (R)0);
+        pt_begin(p, cam, A);
+        while (!pt_step(s, A, p.eye_depth, regen, er)) {
+        }
+        term[pix] = A.depth;
+        T = std::max(T, A.depth);
+      }
+    }
+    if (regen) {
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : er)
+      for (i32 y = 0; y < (i32)H; y++) {
+        for (u32 x = 0; x < W; x++) {
+          const u32 pix = (u32)y * W + x;
+          if (term[pix] >= T) continue;
+          PtPath<R>& A = P[pix];
+          pt_begin(p, cam, A);
+          while (!pt_step(s, A, std::min(T - term[pix], p.eye_depth), true, er)) {
+          }
+        }
+      }
+    }
+    for (u32 pix = 0; pix < n; pix++) apply_pt(&fs.result[pix * 3], fs.N[pix], f, P[pix].accum, P[pix].N);
+    fs.ext_rays += er;
+  }
+}
+
+// cudaRTPTStreamRegen::Render's loop as written (:424-471), for a frame the
+// stream holds whole: per iteration the restarts queued by the previous one
+// begin, every live first path and restart takes one bounce, ended restarts
+// leave the list for good, ended first paths queue their restart; the loop
+// stops when no first path is live, and a restart still live adds its partial
+// sample in pt_applyPathQueueResult_kernel.  Single-threaded.
+template <class R>
+static void render_pt_literal(const Scene& s, const RenderParams& p, u32 frame0, u32 nframes, FrameState<R>& fs) {
+  Camera<R> cam = make_camera<R>(p);
+  const u32 W = p.width, H = p.height, n = W * H;
+  std::vector<PtPath<R>> P(n);
+  for (u32 f = frame0; f < frame0 + nframes; f++) {
+    if (f == 0) std::fill(fs.N.begin(), fs.N.end(), 0u);
+    u64 er = 0;
+    std::vector<u32> stream, regen_live, regen_queued;
+    for (u32 pix = 0; pix < n; pix++) {
+      PtPath<R>& A = P[pix];
+      A = PtPath<R>();
+      A.x = pix % W; A.y = pix / W;
+      pcg32_seed(A.rng, stream_key(p.seed, f, DOMAIN_EYE, 0), pix);
+      A.accum = mk((R)0, (R)0, (R)0);
+      pt_begin(p, cam, A);
+      stream.push_back(pix);
+    }
+    while (!stream.empty()) {
+      for (u32 pix : regen_queued) { pt_begin(p, cam, P[pix]); regen_live.push_back(pix); }
+      regen_queued.clear();
+      for (u32 pix : stream) pt_step(s, P[pix], p.eye_depth, true, er);
+      for (u32 pix : regen_live) pt_step(s, P[pix], p.eye_depth, true, er);
+      std::vector<u32> keep;
+      for (u32 pix : regen_live) if (!P[pix].ended) keep.push_back(pix);
+      regen_live.swap(keep);
+      keep.clear();
+      for (u32 pix : stream) (P[pix].ended ? regen_queued : keep).push_back(pix);
+      stream.swap(keep);
+    }
+    for (u32 pix : regen_live) { P[pix].accum = P[pix].accum + P[pix].psample; P[pix].N++; }
+    for (u32 pix = 0; pix < n; pix++) apply_pt(&fs.result[pix * 3], fs.N[pix], f, P[pix].accum, P[pix].N);
+    fs.ext_rays += er;
   }
 }
 
@@ -1225,6 +1392,28 @@ u32 orc_scene_add_texture(void* h, u32 w, u32 hgt, const float* rgba) {
   return (u32)(s->texDesc.size() / 3 - 1);
 }
 // One texture lookup (the tex_sample above), FP32 or FP64.
+// One BRDF sample (FP32) at a hit: hs = norm xyz, diff xyz, trans, specular,
+// metallic, roughness; the RNG is PCG32 (state, sequence).  out = new origin,
+// direction, throughput (9 floats); returns the new ray type.
+int orc_sample_bounce(const float* ro, const float* rd, float t, const float* hs, const float* nl, u64 state, u64 seq,
+                      int tir, float* out) {
+  HitShade<float> h;
+  h.norm = mk(hs[0], hs[1], hs[2]);
+  h.pos = mk(0.f, 0.f, 0.f);
+  h.diff = mk(hs[3], hs[4], hs[5]);
+  h.emissive = mk(0.f, 0.f, 0.f);
+  h.trans = hs[6]; h.specular = hs[7]; h.metallic = hs[8]; h.roughness = hs[9];
+  Pcg32 rng;
+  rng.state = state;
+  rng.inc = (seq << 1u) | 1u;
+  Bounce<float> b = sample_bounce<float>(rng, mk(ro[0], ro[1], ro[2]), mk(rd[0], rd[1], rd[2]), t, h,
+                                         mk(nl[0], nl[1], nl[2]), RT_EYE, false, tir != 0);
+  out[0] = b.orig.x; out[1] = b.orig.y; out[2] = b.orig.z;
+  out[3] = b.dir.x; out[4] = b.dir.y; out[5] = b.dir.z;
+  out[6] = b.mul.x; out[7] = b.mul.y; out[8] = b.mul.z;
+  return b.type;
+}
+
 void orc_tex_sample(void* h, u32 id, double u, double v, int bits, double* out) {
   const Scene* s = (const Scene*)h;
   if (bits == 64) {
@@ -1311,7 +1500,9 @@ void orc_render(void* scene, void* rh, u32 frame0, u32 nframes, float* result, u
   size_t n = (size_t)r->p.width * r->p.height;
   if (r->prec == 64) {
     std::vector<double> g; if (gbuf) g.assign(n * 9, 0.0);
-    if (r->p.mode != 0) render_adaptive<double>(*s, r->p, frame0, nframes, r->f64);
+    if (r->p.mode == 6) render_pt_literal<double>(*s, r->p, frame0, nframes, r->f64);
+    else if (r->p.mode >= 4) render_pt<double>(*s, r->p, frame0, nframes, r->f64);
+    else if (r->p.mode != 0) render_adaptive<double>(*s, r->p, frame0, nframes, r->f64);
     else render_frames<double>(*s, r->p, frame0, nframes, r->f64, gbuf ? g.data() : nullptr);
     if (result) for (size_t i = 0; i < n * 3; i++) result[i] = (float)r->f64.result[i];
     if (N) std::memcpy(N, r->f64.N.data(), n * 4);
@@ -1319,7 +1510,9 @@ void orc_render(void* scene, void* rh, u32 frame0, u32 nframes, float* result, u
     if (lvN) *lvN = r->f64.lvN;
   } else {
     std::vector<float> g; if (gbuf) g.assign(n * 9, 0.f);
-    if (r->p.mode != 0) render_adaptive<float>(*s, r->p, frame0, nframes, r->f32);
+    if (r->p.mode == 6) render_pt_literal<float>(*s, r->p, frame0, nframes, r->f32);
+    else if (r->p.mode >= 4) render_pt<float>(*s, r->p, frame0, nframes, r->f32);
+    else if (r->p.mode != 0) render_adaptive<float>(*s, r->p, frame0, nframes, r->f32);
     else render_frames<float>(*s, r->p, frame0, nframes, r->f32, gbuf ? g.data() : nullptr);
     if (result) std::memcpy(result, r->f32.result.data(), n * 12);
     if (N) std::memcpy(N, r->f32.N.data(), n * 4);
