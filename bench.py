@@ -164,7 +164,7 @@ def hbm_traffic(args):
         d = tempfile.mkdtemp(prefix="cpt_pmc_")
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t",
                "--", sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
-               "--no-traffic", "--no-learned", "--no-multiview", "--mega-steps", "0", "--frames", str(args.frames),
+               "--no-traffic", "--no-learned", "--no-multiview", "--baseline-steps", "0", "--frames", str(args.frames),
                "--in-flight", str(args.in_flight),
                "--lds-stack", str(args.lds_stack), "--lds-stack-shadow", str(args.lds_stack_shadow),
                "--node-format", str(args.node_format),
@@ -195,12 +195,18 @@ def hbm_traffic(args):
             "write_bytes": int(write * wfac), "fetch_scale": round(ffac, 3), "write_scale": round(wfac, 3)}
 
 
-def megakernel_leg(ca, S, sc, args):
-    """The same workload with the megakernel schedule (one thread per sample
-    through every bounce, the reference's regeneration baseline): one step of
-    each schedule compared bit for bit, then `--mega-steps` timed steps."""
-    def renderer(mega):
-        R = ca.Renderer(WIDTH, HEIGHT, eye_depth=EYE_DEPTH, light_depth=LIGHT_DEPTH)
+def baselines_leg(ca, S, sc, args):
+    """The reference's comparison set on the same scene, resolution and spp:
+    the megakernel schedule of the same estimator (one thread per sample through
+    every bounce; one step compared bit for bit with the wavefront's), and the
+    unidirectional path tracers (cudaRTPTStream, cudaRTPTStreamRegen, eye depth
+    10), each timed over `--baseline-steps` steps."""
+    def renderer(mode=0, mega=False):
+        R = ca.Renderer(WIDTH, HEIGHT, mode=mode)
+        if mode == 0:
+            R.set_attrib("Eye Depth", EYE_DEPTH)
+            R.set_attrib("Light Depth", LIGHT_DEPTH)
+            R.set_attrib("Megakernel", 1 if mega else 0)
         R.set_attrib("Frames In Flight", args.in_flight)
         R.set_attrib("Traversal LDS Stack", args.lds_stack)
         R.set_attrib("Shadow LDS Stack", args.lds_stack_shadow or args.lds_stack)
@@ -208,29 +214,42 @@ def megakernel_leg(ca, S, sc, args):
         R.set_attrib("Traversal Refill", args.refill)
         R.set_attrib("Refill Workgroups", args.refill_groups)
         R.set_attrib("Shadow Refill", args.refill_shadow)
-        R.set_attrib("Megakernel", 1 if mega else 0)
         return R
+
+    def timed(R, label):
+        R.render(S, cam, frames=args.frames, result=False)          # first-call allocations
+        R.stats(reset=True)
+        R.set_profiling(True)
+        ca.synchronize()
+        t0 = time.perf_counter()
+        for k in range(args.baseline_steps):
+            R.render(S, cam, frames=args.frames, result=False)
+            ca.synchronize()
+            log(f"{label} step {k + 1}/{args.baseline_steps}")
+        wall = time.perf_counter() - t0
+        st = R.stats(reset=True)
+        rays = st["extension_rays"] + st["shadow_rays"]
+        return {"mrays_per_s": round(rays / wall / 1e6, 2), "samples_per_s": round(st["eye_paths"] / wall, 1),
+                "ms_per_step": round(wall / args.baseline_steps * 1e3, 3), "steps": args.baseline_steps}
+
     cam = sc["camera"]
-    Rw, Rm = renderer(False), renderer(True)
+    out = {}
+    Rw, Rm = renderer(), renderer(mega=True)
     img_w = Rw.render(S, cam, frames=args.frames)
     img_m = Rm.render(S, cam, frames=args.frames)
     same = bool(np.array_equal(img_w.view(np.uint32), img_m.view(np.uint32)))
     del Rw, img_w, img_m
     log(f"megakernel: image identical to the wavefront's: {same}")
-    Rm.stats(reset=True)
-    Rm.set_profiling(True)
-    ca.synchronize()
-    t0 = time.perf_counter()
-    for k in range(args.mega_steps):
-        Rm.render(S, cam, frames=args.frames, result=False)
-        ca.synchronize()
-        log(f"megakernel step {k + 1}/{args.mega_steps}")
-    wall = time.perf_counter() - t0
-    st = Rm.stats(reset=True)
-    rays = st["extension_rays"] + st["shadow_rays"]
-    return {"mrays_per_s": round(rays / wall / 1e6, 2), "ms_per_step": round(wall / args.mega_steps * 1e3, 3),
-            "kernel_ms_per_step": round(st["extend_ms"] / args.mega_steps, 3), "same_image": same,
-            "steps": args.mega_steps}
+    Rm.reset()
+    out["megakernel"] = timed(Rm, "megakernel")
+    out["megakernel"]["same_image"] = same
+    del Rm
+    for mode, name in ((ca.CPT_MODE_PT_STREAM, "pt_stream"), (ca.CPT_MODE_PT_STREAM_REGEN, "pt_stream_regen")):
+        R = renderer(mode)
+        out[name] = timed(R, name)
+        out[name]["eye_depth"] = 10
+        del R
+    return out
 
 
 def multiview_leg(ca, args):
@@ -284,7 +303,8 @@ def main():
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
     ap.add_argument("--no-learned", action="store_true", help="skip the time-to-equal-MSE leg")
     ap.add_argument("--no-multiview", action="store_true", help="skip the batched multi-view leg")
-    ap.add_argument("--mega-steps", type=int, default=2, help="timed steps of the megakernel schedule (0: skip)")
+    ap.add_argument("--baseline-steps", type=int, default=2,
+                    help="timed steps of each comparison baseline: megakernel, path tracers (0: skip)")
     ap.add_argument("--views", type=int, default=32)
     ap.add_argument("--mv-spp", type=int, default=32)
     ap.add_argument("--mv-in-flight", type=int, default=3)
@@ -388,10 +408,11 @@ def main():
         "time_to_equal_mse": None,
     }
     del R
-    res["megakernel"] = None
-    if args.mega_steps > 0:
-        res["megakernel"] = megakernel_leg(ca, S, sc, args)
-        res["megakernel"]["wavefront_speedup"] = round(mrays / max(res["megakernel"]["mrays_per_s"], 1e-9), 3)
+    res["baselines"] = None
+    if args.baseline_steps > 0:
+        res["baselines"] = baselines_leg(ca, S, sc, args)
+        res["baselines"]["wavefront_over_megakernel"] = round(
+            mrays / max(res["baselines"]["megakernel"]["mrays_per_s"], 1e-9), 3)
     res["mlp"] = None
     if not args.no_learned:
         res["time_to_equal_mse"], res["mlp"] = learned_leg(ca, S, args)

@@ -72,6 +72,7 @@ def lib():
             "cpt_scene_add_texture": [vp, c32, c32, vp, vp],
             "cpt_scene_clear_textures": [vp],
             "cpt_trace_cost": [vp, vp, c32, c32, c32, vp],
+            "cpt_trace_depth": [vp, vp, c32, c32, vp],
             "cpt_scene_upload": [vp],
             "cpt_scene_destroy": [vp],
             "cpt_trace_closest": [vp, vp, c32, c32, vp],
@@ -266,6 +267,16 @@ class Scene:
         r = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
         out = np.zeros((len(r), 2), np.uint32)
         _check(lib().cpt_trace_cost(self.h, _ptr(r), len(r), int(cull), layout, _ptr(out)), "cpt_trace_cost")
+        return out
+
+    def trace_depth(self, rays: np.ndarray, spec_depth=None) -> np.ndarray:
+        """(n, 3) uint32 of the BVH debug renderer's depth queries over the reference tree:
+        TraceDepth {hit, depth, leaf} (spec_depth None) or TraceDepthParent(spec_depth)
+        {hit, depth or 0xFFFFFFFF, parent node}."""
+        r = np.ascontiguousarray(rays).view(np.float32).reshape(-1, 8)
+        out = np.zeros((len(r), 3), np.uint32)
+        spec = 0xFFFFFFFF if spec_depth is None else int(spec_depth)
+        _check(lib().cpt_trace_depth(self.h, _ptr(r), len(r), spec, _ptr(out)), "cpt_trace_depth")
         return out
 
     def trace_device(self, d_rays_ptr: int, n: int, d_hits_ptr: int, stream_ptr: int = 0, layout: int = 0,

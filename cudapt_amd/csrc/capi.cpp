@@ -435,6 +435,19 @@ int cpt_trace_cost(cpt_scene* s, const cpt_ray* rays, uint32_t n, uint32_t cull,
   return CPT_OK;
 }
 
+int cpt_trace_depth(cpt_scene* s, const cpt_ray* rays, uint32_t n, uint32_t spec_depth, uint32_t* out3) {
+  if (!s || (n && (!rays || !out3))) return fail(CPT_E_ARG, "cpt_trace_depth: null argument");
+  if (!s->uploaded) return fail(CPT_E_STATE, "cpt_trace_depth: scene not uploaded");
+  if (n == 0) return CPT_OK;
+  HIP_TRY(s->d_rays.alloc((size_t)n * 2));
+  HIP_TRY(s->d_hits.alloc(n));                  // 16 bytes per ray; the results take 12
+  HIP_TRY(hipMemcpyAsync(s->d_rays.p, rays, sizeof(cpt_ray) * n, hipMemcpyHostToDevice, s->stream));
+  HIP_TRY(launch_bvh_depth(s->dev(), s->d_rays.p, n, spec_depth, (uint32_t*)s->d_hits.p, s->stream));
+  HIP_TRY(hipMemcpyAsync(out3, s->d_hits.p, sizeof(uint32_t) * 3 * n, hipMemcpyDeviceToHost, s->stream));
+  HIP_TRY(hipStreamSynchronize(s->stream));
+  return CPT_OK;
+}
+
 int cpt_filter_cross_bilateral_device(const float* d_color, const float* d_pos, const float* d_norm,
                                       const float* d_diff, uint32_t width, uint32_t height, float sigma_color,
                                       float sigma_pos, float sigma_norm, float sigma_diff, uint32_t radius,

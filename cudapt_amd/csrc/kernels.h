@@ -134,6 +134,10 @@ hipError_t launch_frame_begin(FrameCounters* fc, uint32_t n_paths, hipStream_t s
 hipError_t launch_light_pass(const DevScene& sc, const FrameArgs& fa, const LightJobs& jobs, LightVert* lv,
                              FrameCounters* fc, hipStream_t st);
 // nframes x W*H eye paths, slot = frame offset * W*H + pixel.
+// The BVH debug renderer's TraceDepth (spec = 0xFFFFFFFF) or
+// TraceDepthParent(spec) per ray over the reference tree: out = 3 u32 per ray.
+hipError_t launch_bvh_depth(const DevScene& sc, const float4* rays, uint32_t n, uint32_t spec, uint32_t* out,
+                            hipStream_t st);
 hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st);
 // Running-mean update of every pixel from the batch's per-frame records, frames in order.
 hipError_t launch_apply_frames(const FrameArgs& fa, const float4* rec_acc, float* result, uint32_t* sample_n,

@@ -345,6 +345,87 @@ __device__ bool traverse_bin(const DevScene& sc, f3 o, f3 d, float maxd, float e
   return h.leaf != kInvalid;
 }
 
+// ---- the BVH debug renderer's depth queries ---------------------------------
+// CURay::IntersectAABB (src/cudaRTCommon.cu:24-40) as written: the entry
+// distance (0 within M_EPSILON of the origin), kInfF on a miss.
+__device__ __forceinline__ float ref_aabb(f3 o, f3 rcp, float4 mn, float4 mx) {
+  const float t0x = (mn.x - o.x) * rcp.x, t1x = (mx.x - o.x) * rcp.x;
+  const float t0y = (mn.y - o.y) * rcp.y, t1y = (mx.y - o.y) * rcp.y;
+  const float t0z = (mn.z - o.z) * rcp.z, t1z = (mx.z - o.z) * rcp.z;
+  const float minmax = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+  const float maxmin = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fminf(t0z, t1z));
+  if (minmax >= maxmin && (minmax > 0.f || maxmin > 0.f)) return ((double)maxmin > 1e-9) ? maxmin : 0.f;
+  return kInfF;
+}
+// TraceDepth / TraceDepthParent (src/cudaRTCommon.cu:129-231) as written, over
+// the reference's binary tree: right child popped first, a 32-entry command
+// stack, at most 2048 pops.  spec == 0xFFFFFFFF: TraceDepth, out = {hit,
+// depth, leaf}; otherwise TraceDepthParent(spec), out = {hit, depth
+// (0xFFFFFFFF for -1), parent node}.
+__global__ __launch_bounds__(kBlock) void k_bvh_depth(DevScene sc, const float4* __restrict__ rays, uint32_t n,
+                                                      uint32_t spec, uint32_t* __restrict__ out) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const float4 a = rays[2 * i], b = rays[2 * i + 1];
+  const f3 o = ld3(a), d = ld3(b);
+  const f3 rcp = mk3(1.f / escape_zero(d.x), 1.f / escape_zero(d.y), 1.f / escape_zero(d.z));
+  float minI = a.w;
+  uint32_t cmd[32], par[32], dep[32];
+  cmd[0] = 0; par[0] = 0; dep[0] = 0;
+  int sp = 0;
+  uint32_t traced = 0;
+  uint32_t r0 = 0, r1 = 0, r2 = 0;
+  if (spec == 0xFFFFFFFFu) {
+    uint32_t depth = 0;
+    bool leaf = false;
+    while (sp >= 0 && traced++ < 2048u) {
+      const uint32_t nd = cmd[sp--];
+      const float tb = ref_aabb(o, rcp, sc.bnodes[2 * nd], sc.bnodes[2 * nd + 1]);
+      if (tb >= 0.f && tb < minI) {
+        const uint2 off = sc.boff[nd];
+        if (off.y == 0) {
+          if (sp < 30) { cmd[++sp] = nd + 1; cmd[++sp] = nd + off.x; }
+        } else {
+          leaf = true; minI = tb; depth = (uint32_t)(sp + 2);
+        }
+      } else {
+        depth = max(depth, (uint32_t)(sp + 1));
+      }
+    }
+    r0 = depth != 0; r1 = depth; r2 = leaf;
+  } else {
+    int depth = -1;
+    uint32_t parent = 0;
+    bool any = false;
+    while (sp >= 0 && traced++ < 2048u) {
+      const int cp = sp--;
+      const uint32_t nd = cmd[cp];
+      const float tb = ref_aabb(o, rcp, sc.bnodes[2 * nd], sc.bnodes[2 * nd + 1]);
+      if (tb >= 0.f && tb < minI) {
+        any = true;
+        const uint32_t cd = dep[cp];
+        if (spec == cd) { minI = tb; depth = (int)cd; parent = par[cp]; break; }
+        const uint2 off = sc.boff[nd];
+        if (off.y == 0 && sp < 30) {
+          ++sp; cmd[sp] = nd + 1; par[sp] = nd; dep[sp] = cd + 1;
+          ++sp; cmd[sp] = nd + off.x; par[sp] = nd; dep[sp] = cd + 1;
+        }
+      }
+    }
+    r0 = any; r1 = any ? (uint32_t)depth : 0xFFFFFFFFu; r2 = any ? parent : 0u;
+  }
+  out[3 * i] = r0;
+  out[3 * i + 1] = r1;
+  out[3 * i + 2] = r2;
+}
+hipError_t launch_bvh_depth(const DevScene& sc, const float4* rays, uint32_t n, uint32_t spec, uint32_t* out,
+                            hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  if (!rays || !out || !sc.bnodes || !sc.boff) return hipErrorInvalidValue;
+  k_bvh_depth<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, rays, n, spec, out);
+  return hipGetLastError();
+}
+
 __device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
 #pragma unroll
   for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -1,6 +1,8 @@
-"""Traversal-cost heat maps: the counterpart of the reference's BVH debug
-renderer (cudaRTBVHDebug, "TraceCost" mode; TraceCost in
-src/cudaRTCommon.cu:232-273), over this library's traversal.
+"""Traversal-cost and BVH-depth heat maps: the counterpart of the reference's
+BVH debug renderer (cudaRTBVHDebug: "TraceCost" mode, TraceCost in
+src/cudaRTCommon.cu:232-273, over this library's traversal; "Normal" and
+"BVHDepth" modes, TraceDepth / TraceDepthParent in src/cudaRTCommon.cu:129-231,
+as written over the reference's binary tree).
 
 One primary ray per pixel through the pixel centre (the debug renderer's
 pinhole, src/cudaRTBVHDebug.cu:113-117: no jitter), traced closest-hit; the
@@ -46,6 +48,42 @@ def cost_map(scene: Scene, camera, width: int, height: int, layout: int = 2):
     cost = scene.trace_cost(rays, layout=layout).reshape(height, width, 2)
     hit = (scene.trace(rays)["tri"] != MISS).reshape(height, width)
     return cost, hit
+
+
+def depth_map(scene: Scene, camera, width: int, height: int, spec_depth=None):
+    """The debug renderer's "Normal" mode (TraceDepth, spec_depth None): (H, W)
+    depth, leaf and hit; its "BVHDepth" mode (TraceDepthParent): (H, W) depth
+    (0xFFFFFFFF: no box at spec_depth), parent node and hit.  Over the reference
+    tree, with the max distance M_INF as the debug renderer passes it."""
+    rays = primary_rays(camera, width, height)
+    out = scene.trace_depth(rays, spec_depth).reshape(height, width, 3)
+    return out[..., 1], out[..., 2], out[..., 0] != 0
+
+
+def depth_rgb(depth: np.ndarray, leaf: np.ndarray, scale: float = 0.1) -> np.ndarray:
+    """"Normal" mode colouring: depth * scale, green where a leaf box was entered, red elsewhere."""
+    v = depth.astype(np.float32) * np.float32(scale)
+    img = np.zeros(depth.shape + (3,), np.float32)
+    img[..., 1] = np.where(leaf, v, 0.0)
+    img[..., 0] = np.where(leaf, 0.0, v)
+    return img
+
+
+def parent_rgb(depth: np.ndarray, parent: np.ndarray, hit: np.ndarray, spec_depth: int) -> np.ndarray:
+    """"BVHDepth" mode colouring: green by a hash of the parent node where a box at
+    spec_depth was entered, red 0.3 where boxes were entered but none at that
+    depth.  The reference seeds cuRAND with the parent id for the green value;
+    a splitmix64 hash stands in for it (the value only tells boxes apart)."""
+    z = parent.astype(np.uint64) + np.uint64(0x9E3779B97F4A7C15)
+    z = (z ^ (z >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    z = (z ^ (z >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    z = z ^ (z >> np.uint64(31))
+    col = (z >> np.uint64(40)).astype(np.float32) / np.float32(1 << 24)
+    img = np.zeros(depth.shape + (3,), np.float32)
+    at = hit & (depth == np.uint32(spec_depth))
+    img[..., 1] = np.where(at, col, 0.0)
+    img[..., 0] = np.where(hit & ~at, 0.3, 0.0)
+    return img
 
 
 def to_rgb(cost: np.ndarray, scale: float = 0.1) -> np.ndarray:

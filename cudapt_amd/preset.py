@@ -34,7 +34,7 @@ import re
 
 import numpy as np
 
-from . import (CPT_MODE_BDPT_ADAPTIVE_BINARY, CPT_MODE_BDPT_ADAPTIVE_PLL, CPT_MODE_BDPT_STREAM)
+from . import (CPT_MODE_BDPT_ADAPTIVE_BINARY, CPT_MODE_BDPT_ADAPTIVE_PLL, CPT_MODE_BDPT_STREAM, CPT_MODE_PT_STREAM)
 from .scenes import MATERIAL_DTYPE
 
 # RTRenderer::RENDERER_MODE with FULLSPECTRAL off (include/raytracer.h:1-25,295-304)
@@ -42,7 +42,7 @@ REFERENCE_MODES = ["CPU_DEBUG", "rastBasic", "cudaRTPT", "cudaRTPTStream", "cuda
                    "cudaRTBDPTStreamBilat", "cudaRTBDPTStreamAdapBilatBinary", "cudaRTBDPTStreamAdapBilatAtomic",
                    "cudaRTBDPTAtomicAdapBilatAtomic", "cudaRTDebug", "cudaRTBVHDebug"]
 # the ones this library renders
-MODE_OF_REFERENCE = {"cudaRTBDPTStream": CPT_MODE_BDPT_STREAM,
+MODE_OF_REFERENCE = {"cudaRTBDPTStream": CPT_MODE_BDPT_STREAM, "cudaRTPTStream": CPT_MODE_PT_STREAM,
                      "cudaRTBDPTStreamAdapBilatBinary": CPT_MODE_BDPT_ADAPTIVE_BINARY,
                      "cudaRTBDPTStreamAdapBilatAtomic": CPT_MODE_BDPT_ADAPTIVE_PLL}
 

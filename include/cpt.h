@@ -166,6 +166,17 @@ int cpt_filter_cross_bilateral_device(const float* d_color, const float* d_pos, 
  * cpt_trace_closest_device).  Host arrays. */
 int cpt_trace_cost(cpt_scene* s, const cpt_ray* rays, uint32_t n, uint32_t cull, uint32_t layout, uint32_t* cost2);
 
+/* The BVH debug renderer's depth queries (src/cudaRTBVHDebug.cu modes
+ * "Normal" and "BVHDepth"), as written over the reference's binary tree (the
+ * layout cpt_bvh_export returns), one per ray; each ray's max distance bounds
+ * the walk, epsilon is unused.  spec_depth = CPT_DEPTH_NONE: TraceDepth
+ * (src/cudaRTCommon.cu:184-231), out3[3i..] = {entered a box, depth, reached
+ * a leaf}; otherwise TraceDepthParent(spec_depth) (src/cudaRTCommon.cu:129-182),
+ * out3[3i..] = {entered a box, depth or 0xFFFFFFFF when no box at spec_depth was
+ * entered, parent node of that box}.  Host arrays. */
+#define CPT_DEPTH_NONE 0xFFFFFFFFu
+int cpt_trace_depth(cpt_scene* s, const cpt_ray* rays, uint32_t n, uint32_t spec_depth, uint32_t* out3);
+
 /* Node visits and triangle tests of every traversal since the last reset. */
 int cpt_trace_counters(cpt_scene* s, uint64_t* nodes, uint64_t* tris, uint64_t* rays, int reset);
 

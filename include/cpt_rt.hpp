@@ -75,7 +75,9 @@ class RTRenderer {
     BDPT_STREAM = CPT_MODE_BDPT_STREAM,
     BDPT_STREAM_ADAP_BILAT_ATOMIC = CPT_MODE_BDPT_ADAPTIVE_PLL,
     BDPT_STREAM_ADAP_BILAT_BINARY = CPT_MODE_BDPT_ADAPTIVE_BINARY,
-    BDPT_STREAM_ADAP_EMP = CPT_MODE_BDPT_ADAPTIVE_EMP
+    BDPT_STREAM_ADAP_EMP = CPT_MODE_BDPT_ADAPTIVE_EMP,
+    PT_STREAM = CPT_MODE_PT_STREAM,               // DEFINE_RT(PTSC, cudaRTPTStream)
+    PT_STREAM_REGEN = CPT_MODE_PT_STREAM_REGEN    // DEFINE_RT(PTSCRegen, cudaRTPTStreamRegen), commented out there
   };
   RTRenderer() = default;
   ~RTRenderer() { release(); }

@@ -40,6 +40,7 @@ def lib():
         L.orc_tex_sample.argtypes = [vp, c32, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                      np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")]
         L.orc_scene_bvh.restype = c32
+        L.orc_trace_depth.argtypes = [vp, f32p, c32, c32, u32p]
         L.orc_sample_bounce.argtypes = [f32p, f32p, ctypes.c_float, f32p, f32p, c64, c64, ctypes.c_int, f32p]
         L.orc_sample_bounce.restype = ctypes.c_int
         L.orc_scene_bvh.argtypes = [vp, vp, vp, vp, c32]
@@ -156,6 +157,16 @@ class Scene:
         m = {"brute": 0, "defined": 1, "reference": 2}[mode]
         lib().orc_trace(self.h, rays.ravel(), len(rays), int(cull), m, prec, hits.ctypes.data, stats.ctypes.data)
         return hits, stats
+
+    def trace_depth(self, rays: np.ndarray, spec_depth=None):
+        """(n, 3) uint32 of the BVH debug renderer's depth queries over the reference tree:
+        TraceDepth {hit, depth, leaf} (spec_depth None) or TraceDepthParent(spec_depth)
+        {hit, depth or 0xFFFFFFFF, parent}."""
+        rays = np.ascontiguousarray(rays, np.float32)
+        out = np.zeros((len(rays), 3), np.uint32)
+        spec = 0xFFFFFFFF if spec_depth is None else int(spec_depth)
+        lib().orc_trace_depth(self.h, rays.ravel(), len(rays), spec, out.ravel())
+        return out
 
 
 def max_threads() -> int:

@@ -431,6 +431,68 @@ static bool trace_refbin(const Scene& s, V3<R> o, V3<R> d, R maxDist, R eps, boo
   return false;
 }
 
+// TraceDepth (cudaRTCommon.cu:184-231) as written, over the reference tree:
+// right child popped first, a 32-entry command stack, at most 2048 pops.
+// depth = stack pointer + 2 at the last leaf box entered, and at least stack
+// pointer + 1 at every box missed; returns depth != 0.
+template <class R>
+static bool trace_depth_literal(const Scene& s, V3<R> o, V3<R> d, R maxDist, u32& depth, bool& leaf) {
+  R minI = maxDist;
+  u32 cmd[32]; cmd[0] = 0;
+  i32 sp = 0;
+  depth = 0; leaf = false;
+  u32 traced = 0;
+  if (s.nodeN == 0) return false;
+  while (sp >= 0 && traced++ < 2048u) {
+    u32 n = cmd[sp--];
+    R tb = intersect_aabb(o, d, cvt<R>(&s.bmin[n * 3]), cvt<R>(&s.bmax[n * 3]));
+    if (tb >= (R)0 && tb < minI) {
+      u32 a = s.offOrTSTN[n * 2], cnt = s.offOrTSTN[n * 2 + 1];
+      if (cnt == 0) {
+        if (sp < 32 - 2) { cmd[++sp] = n + 1; cmd[++sp] = n + a; }
+      } else {
+        leaf = true; minI = tb; depth = (u32)(sp + 2);
+      }
+    } else {
+      depth = std::max(depth, (u32)(sp + 1));
+    }
+  }
+  return depth != 0;
+}
+
+// TraceDepthParent (cudaRTCommon.cu:129-182) as written: the first box at tree
+// depth `spec` the right-first walk enters, and its parent; depth = -1 when
+// the walk enters boxes but none at that depth; returns whether it entered any.
+template <class R>
+static bool trace_depth_parent_literal(const Scene& s, V3<R> o, V3<R> d, R maxDist, u32 spec, i32& depth,
+                                       u32& parent) {
+  R minI = maxDist;
+  u32 cmd[32], par[32], dep[32];
+  cmd[0] = 0; par[0] = 0; dep[0] = 0;
+  i32 sp = 0;
+  depth = -1; parent = 0;
+  bool any = false;
+  u32 traced = 0;
+  if (s.nodeN == 0) return false;
+  while (sp >= 0 && traced++ < 2048u) {
+    i32 cp = sp--;
+    u32 n = cmd[cp];
+    R tb = intersect_aabb(o, d, cvt<R>(&s.bmin[n * 3]), cvt<R>(&s.bmax[n * 3]));
+    if (tb >= (R)0 && tb < minI) {
+      any = true;
+      u32 cd = dep[cp];
+      if (spec == cd) { minI = tb; depth = (i32)cd; parent = par[cp]; break; }
+      u32 a = s.offOrTSTN[n * 2], cnt = s.offOrTSTN[n * 2 + 1];
+      if (cnt == 0 && sp < 32 - 2) {
+        ++sp; cmd[sp] = n + 1; par[sp] = n; dep[sp] = cd + 1;
+        ++sp; cmd[sp] = n + a; par[sp] = n; dep[sp] = cd + 1;
+      }
+    }
+  }
+  if (!any) { depth = -1; parent = 0; }
+  return any;
+}
+
 // ---------------------------------------------------------------- shading
 // cudaRTCommon.cu:284-365
 template <class R> static inline bool prob_rand(Pcg32& rng, R prob) {
@@ -1448,6 +1510,28 @@ u32 orc_scene_light_tris(void* h, u32* out, u32 cap) {
 // rays: n x 8 floats {ox,oy,oz,maxDist, dx,dy,dz,eps}; hits: n x {u32 tri, f32 t,u,v}.
 // mode 0 = brute, 1 = defined (conservative BVH), 2 = reference TracePrimitive.
 // prec 32 or 64. stats[2] = stack drops, cap hits (mode 2).
+// The BVH debug renderer's depth queries (FP32), rays as for orc_trace (max
+// distance r[3]): spec = 0xFFFFFFFF: TraceDepth, out[3i..] = {hit, depth, leaf};
+// otherwise TraceDepthParent(spec): {hit, depth (0xFFFFFFFF for -1), parent}.
+void orc_trace_depth(void* h, const float* rays, u32 n, u32 spec, u32* out) {
+  Scene* s = (Scene*)h;
+#pragma omp parallel for schedule(dynamic, 256)
+  for (i32 i = 0; i < (i32)n; i++) {
+    const float* r = rays + (size_t)i * 8;
+    V3<float> org = cvt<float>(r), dir = cvt<float>(r + 4);
+    u32* q = out + (size_t)i * 3;
+    if (spec == 0xFFFFFFFFu) {
+      u32 depth = 0; bool leaf = false;
+      bool hit = trace_depth_literal<float>(*s, org, dir, r[3], depth, leaf);
+      q[0] = hit; q[1] = depth; q[2] = leaf;
+    } else {
+      i32 depth = -1; u32 parent = 0;
+      bool hit = trace_depth_parent_literal<float>(*s, org, dir, r[3], spec, depth, parent);
+      q[0] = hit; q[1] = (u32)depth; q[2] = parent;
+    }
+  }
+}
+
 void orc_trace(void* h, const float* rays, u32 n, u32 cull, u32 mode, u32 prec, void* hits, u32* stats) {
   Scene* s = (Scene*)h;
   Hit* out = (Hit*)hits;

@@ -1,10 +1,12 @@
 """Traversal-cost heat maps (the BVH debug renderer's TraceCost mode) over the
 device traversal: per-ray counts agree with the aggregate counters, rays that
 miss the scene's box visit only the root, and the device tree's 4-wide nodes
-take fewer visits than the reference's binary tree."""
+take fewer visits than the reference's binary tree.  Depth maps (its "Normal"
+and "BVHDepth" modes) equal the oracle's restatement for every ray."""
 import numpy as np
 import pytest
 
+import oracle
 from cudapt_amd import heatmap, scenes
 
 pytestmark = pytest.mark.gpu
@@ -46,3 +48,32 @@ def test_device_tree_is_cheaper_than_reference_tree(grid):
     assert dev[..., 0].sum() < ref[..., 0].sum()
     rgb = heatmap.to_rgb(dev)
     assert rgb.shape == (64, 96, 3) and np.isfinite(rgb).all() and rgb.max() > 0
+
+
+@pytest.mark.parametrize("name", ["cornell", "grid"])
+@pytest.mark.parametrize("spec", [None, 0, 1, 3, 7, 12, 40])
+def test_depth_maps_equal_the_oracle(grid, name, spec):
+    sc, S = grid if name == "grid" else (scenes.cornell(), None)
+    if S is None:
+        import cudapt_amd
+        S = cudapt_amd.Scene(sc)
+    O = oracle.Scene(sc)
+    rays = heatmap.primary_rays(sc["camera"], 96, 64)
+    g = S.trace_depth(rays, spec)
+    o = O.trace_depth(rays, spec)
+    assert np.array_equal(g, o)
+    assert g[:, 0].any()
+    if spec is None:
+        assert (g[:, 1] > 0).any() and g[:, 2].any()
+    elif spec <= 7:
+        assert (g[:, 1] == spec).any()
+
+
+def test_depth_map_images(grid):
+    sc, S = grid
+    depth, leaf, hit = heatmap.depth_map(S, sc["camera"], 48, 32)
+    img = heatmap.depth_rgb(depth, leaf)
+    assert img.shape == (32, 48, 3) and np.isfinite(img).all() and img[..., 1].max() > 0
+    d, parent, hit = heatmap.depth_map(S, sc["camera"], 48, 32, spec_depth=4)
+    img2 = heatmap.parent_rgb(d, parent, hit, 4)
+    assert img2[..., 1].max() > 0 and len(np.unique(parent[d == 4])) > 1

@@ -43,7 +43,7 @@ def test_fly_camera_closed_forms():
 
 def test_cornell_preset_header():
     p = _load("cornellwtranball.NPRTENV")
-    assert p["rttech"] == 3 and p["reference_mode"] == "cudaRTPTStream" and p["mode"] is None
+    assert p["rttech"] == 3 and p["reference_mode"] == "cudaRTPTStream" and p["mode"] == 4
     assert p["camera"]["pos"] == pytest.approx((-1.82495, -2.40754, -2.74389))
     pitch, yaw = -0.13, -0.215
     want = np.array([math.cos(yaw) * math.sin(pitch), math.sin(yaw), math.cos(yaw) * math.cos(pitch)])
