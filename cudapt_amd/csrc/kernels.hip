@@ -2058,19 +2058,45 @@ __device__ __forceinline__ void accum_apply(uint32_t pix, uint32_t tN, f3 ts, f3
 
 // Adds record q to the running sums (records with N = 0 or an infinite
 // component are skipped, AdapBilatAtomic.cu:1056-1060).
-__device__ __forceinline__ void accum_record(const SampleRec& rec, uint32_t q, f3& ts, f3& tp, f3& tn, f3& td,
-                                             f3& tq, uint32_t& tN) {
-  float4 a = rec.acc[q];
-  uint32_t N = __float_as_uint(a.w);
-  f3 acc = ld3(a);
+// One sample record: its four columns are loaded together (one round trip),
+// then added in record order, so that loads of several records can be in
+// flight while the sums keep their order.
+struct RecVals {
+  float4 a, p, n, d;
+};
+__device__ __forceinline__ RecVals load_record(const SampleRec& rec, uint32_t q) {
+  RecVals r;
+  r.a = rec.acc[q];
+  r.p = rec.gpos[q];
+  r.n = rec.gnrm[q];
+  r.d = rec.gdif[q];
+  return r;
+}
+__device__ __forceinline__ void add_record(const RecVals& r, f3& ts, f3& tp, f3& tn, f3& td, f3& tq, uint32_t& tN) {
+  uint32_t N = __float_as_uint(r.a.w);
+  f3 acc = ld3(r.a);
   if (N == 0 || isinf3(acc)) return;
   float nf = (float)N;
   ts = ts + acc;
   tq = tq + vmul(acc, acc);
   tN += N;
-  tp = tp + ld3(rec.gpos[q]) * nf;
-  tn = tn + ld3(rec.gnrm[q]) * nf;
-  td = td + ld3(rec.gdif[q]) * nf;
+  tp = tp + ld3(r.p) * nf;
+  tn = tn + ld3(r.n) * nf;
+  td = td + ld3(r.d) * nf;
+}
+// Records q0, q0 + step, ... below q1 in order, four loads in flight.
+__device__ __forceinline__ void accum_records(const SampleRec& rec, uint32_t q0, uint32_t q1, uint32_t step, f3& ts,
+                                              f3& tp, f3& tn, f3& td, f3& tq, uint32_t& tN) {
+  uint32_t q = q0;
+  for (; q1 > 3 * step && q < q1 - 3 * step; q += 4 * step) {
+    const RecVals r0 = load_record(rec, q), r1 = load_record(rec, q + step);
+    const RecVals r2 = load_record(rec, q + 2 * step), r3 = load_record(rec, q + 3 * step);
+    add_record(r0, ts, tp, tn, td, tq, tN);
+    add_record(r1, ts, tp, tn, td, tq, tN);
+    add_record(r2, ts, tp, tn, td, tq, tN);
+    add_record(r3, ts, tp, tn, td, tq, tN);
+  }
+  for (; q < q1; q += step) add_record(load_record(rec, q), ts, tp, tn, td, tq, tN);
 }
 
 // Pixels with more records than this are summed by a whole workgroup.
@@ -2099,7 +2125,7 @@ __global__ __launch_bounds__(kBlock) void k_accum_adaptive(const uint32_t* __res
   }
   f3 ts = mk3(0.f, 0.f, 0.f), tp = ts, tn = ts, td = ts, tq = ts;
   uint32_t tN = 0;
-  for (uint32_t q = q0; q < q1; q++) accum_record(rec, q, ts, tp, tn, td, tq, tN);
+  accum_records(rec, q0, q1, 1u, ts, tp, tn, td, tq, tN);
   accum_apply(pix, tN, ts, tp, tn, td, tq, result, sample_n, gpos, gnrm, gdif, sq);
 }
 
@@ -2124,7 +2150,7 @@ __global__ __launch_bounds__(kBlock) void k_accum_big(const uint32_t* __restrict
     if (q1 > cap) q1 = cap;
     f3 ts = mk3(0.f, 0.f, 0.f), tp = ts, tn = ts, td = ts, tq = ts;
     uint32_t tN = 0;
-    for (uint32_t q = q0 + threadIdx.x; q < q1; q += kBlock) accum_record(rec, q, ts, tp, tn, td, tq, tN);
+    accum_records(rec, q0 + threadIdx.x, q1, kBlock, ts, tp, tn, td, tq, tN);
     float v[kV] = {ts.x, ts.y, ts.z, tp.x, tp.y, tp.z, tn.x, tn.y, tn.z, td.x, td.y, td.z, tq.x, tq.y, tq.z};
 #pragma unroll
     for (int k = 0; k < kV; k++)

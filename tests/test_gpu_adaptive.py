@@ -138,7 +138,7 @@ def test_pass_frames_match_oracle(pass_runs):
 
 
 def test_heavily_sampled_pixels_take_the_workgroup_path(pass_runs):
-    """Pixels with more than 32 records in a pass are summed by one workgroup each."""
+    """Pixels with more than 32 records in a pass are summed by one wave each."""
     r = pass_runs
     big = r["counts"] > 32
     assert big.any()
