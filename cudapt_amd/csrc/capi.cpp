@@ -83,7 +83,7 @@ struct cpt_scene {
   hipStream_t stream = nullptr;
   DevArray<Node4> d_nodes4;
   DevArray<Node4Q> d_nodes4q;
-  DevArray<float4> d_tris, d_tris_bin, d_vpos, d_vnrm, d_bnodes, d_tex;
+  DevArray<float4> d_tris, d_tris_bin, d_vpos, d_vnrm, d_bnodes, d_tex, d_surf;
   DevArray<uint4> d_tex_desc;
   DevArray<uint4> d_tri_verts;
   DevArray<uint2> d_boff;
@@ -101,6 +101,7 @@ struct cpt_scene {
     s.vpos = d_vpos.p;
     s.vnrm = d_vnrm.p;
     s.mats = d_mats.p;
+    s.surf = d_surf.p;
     s.light_tris = d_light.p;
     s.n_light_tris = n_light;
     s.n_tris = nt;
@@ -130,6 +131,23 @@ int upload_materials(cpt_scene* s) {
   static_assert(sizeof(DevMaterial) == sizeof(cpt_material), "material layouts agree");
   if (!s->mats.empty()) std::memcpy(dm.data(), s->mats.data(), sizeof(cpt_material) * s->mats.size());
   HIP_TRY(s->d_mats.upload(dm.data(), dm.size(), s->stream));
+  // per-triangle shading records in the device tree's leaf order
+  std::vector<float4> surf((size_t)s->nt * 8);
+  auto bits = [](u32 x) { float f; std::memcpy(&f, &x, 4); return f; };
+  for (u32 i = 0; i < s->nt; i++) {
+    const u32 o = s->bvh.dreorder[i];
+    const u32 v[3] = {s->idx[o * 3], s->idx[o * 3 + 1], s->idx[o * 3 + 2]};
+    const cpt_material& m = s->mats[s->mat[o]];
+    float4* r = &surf[(size_t)i * 8];
+    for (int k = 0; k < 3; k++)
+      r[k] = make_float4(s->nrm[v[k] * 3], s->nrm[v[k] * 3 + 1], s->nrm[v[k] * 3 + 2], s->uv[v[k] * 2 + 1]);
+    r[3] = make_float4(m.diffuse[0], m.diffuse[1], m.diffuse[2], m.transparency);
+    r[4] = make_float4(m.emissive[0], m.emissive[1], m.emissive[2], m.specular);
+    r[5] = make_float4(m.metallic, m.roughness, bits((u32)m.diffuse_tex_id), bits((u32)m.emissive_tex_id));
+    r[6] = make_float4(s->uv[v[0] * 2], s->uv[v[1] * 2], s->uv[v[2] * 2], 0.f);
+    r[7] = make_float4(bits(v[0]), bits(v[1]), bits(v[2]), bits(s->mat[o]));
+  }
+  HIP_TRY(s->d_surf.upload(surf.data(), surf.size(), s->stream));
   // emissive triangles listed in the reference's leaf order (the order light
   // subpaths pick them in), each as its index in the device tree's order
   std::vector<u32> dev_index(s->nt);

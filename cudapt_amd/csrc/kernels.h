@@ -25,6 +25,11 @@ struct DevScene {
   const float4* vpos;          // vertex positions, w = texture u
   const float4* vnrm;          // vertex normals, w = texture v
   const DevMaterial* mats;
+  // per triangle in the device tree's leaf order, 8 float4: vertex normals with
+  // texture v (3), diffuse + transparency, emissive + specular, (metallic,
+  // roughness, diffuse / emissive texture id bits), vertex texture u (3),
+  // vertex indices + material (bits)
+  const float4* surf;
   const uint32_t* light_tris;  // leaf-order emissive triangles
   uint32_t n_light_tris;
   uint32_t n_tris;

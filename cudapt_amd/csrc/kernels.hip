@@ -599,39 +599,43 @@ __device__ __forceinline__ float4 tex_sample(const DevScene& sc, uint32_t id, fl
 // GetMaterialColors' textures (src/cudaRTCommon.cu:369-406): a diffuse texture
 // gives the colour and scales transparency by its alpha, an emissive texture
 // gives the emission; texture ids outside the scene's textures mean none.
+// The hit's surface from its triangle's 128-byte shading record: normals and
+// material in one round trip after the hit; texture u and the vertex positions
+// only for textured materials and first-hit G-buffers.
 __device__ __forceinline__ Surf fetch_surf(const DevScene& sc, uint32_t leaf, float w, float u, float v, bool want_pos) {
   Surf s;
-  uint4 tv = sc.tri_verts[leaf];
-  const DevMaterial& m = sc.mats[tv.w];
-  const float4 r0 = sc.vnrm[tv.x], r1 = sc.vnrm[tv.y], r2 = sc.vnrm[tv.z];
+  const float4* R = sc.surf + 8 * (size_t)leaf;
+  const float4 r0 = R[0], r1 = R[1], r2 = R[2], md = R[3], me = R[4], mx = R[5];
   f3 n0 = ld3(r0), n1 = ld3(r1), n2 = ld3(r2);
   s.norm = (n0 * w + n1 * u) + n2 * v;
-  const bool dtex = (uint32_t)m.diffuseTexId < sc.n_tex, etex = (uint32_t)m.emissiveTexId < sc.n_tex;
-  if (want_pos || dtex || etex) {
-    const float4 p0 = sc.vpos[tv.x], p1 = sc.vpos[tv.y], p2 = sc.vpos[tv.z];
-    f3 q0 = ld3(p0), q1 = ld3(p1), q2 = ld3(p2);
-    s.pos = want_pos ? (q0 * w + q1 * u) + q2 * v : mk3(0.f, 0.f, 0.f);
-    // uv = uv0 * w + uv1 * u + uv2 * v (cudaRTBDPTStream.cu:379-382)
-    const float tu = (p0.w * w + p1.w * u) + p2.w * v, tvv = (r0.w * w + r1.w * u) + r2.w * v;
-    s.diff = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
-    s.trans = m.transparency;
-    if (dtex) {
-      const float4 t = tex_sample(sc, (uint32_t)m.diffuseTexId, tu, tvv);
-      s.diff = mk3(t.x, t.y, t.z);
-      s.trans = t.w * m.transparency;
-    }
-    s.emissive = mk3(m.emissive[0], m.emissive[1], m.emissive[2]);
-    if (etex) {
-      const float4 t = tex_sample(sc, (uint32_t)m.emissiveTexId, tu, tvv);
-      s.emissive = mk3(t.x, t.y, t.z);
-    }
+  const uint32_t dtid = __float_as_uint(mx.z), etid = __float_as_uint(mx.w);
+  const bool dtex = dtid < sc.n_tex, etex = etid < sc.n_tex;
+  s.diff = ld3(md);
+  s.trans = md.w;
+  s.emissive = ld3(me);
+  if (want_pos) {
+    const float4 ix = R[7];
+    const float4 p0 = sc.vpos[__float_as_uint(ix.x)], p1 = sc.vpos[__float_as_uint(ix.y)],
+                 p2 = sc.vpos[__float_as_uint(ix.z)];
+    s.pos = (ld3(p0) * w + ld3(p1) * u) + ld3(p2) * v;
   } else {
     s.pos = mk3(0.f, 0.f, 0.f);
-    s.diff = mk3(m.diffuse[0], m.diffuse[1], m.diffuse[2]);
-    s.emissive = mk3(m.emissive[0], m.emissive[1], m.emissive[2]);
-    s.trans = m.transparency;
   }
-  s.spec = m.specular; s.metal = m.metallic; s.rough = m.roughness;
+  if (dtex || etex) {
+    // uv = uv0 * w + uv1 * u + uv2 * v (cudaRTBDPTStream.cu:379-382)
+    const float4 tu4 = R[6];
+    const float tu = (tu4.x * w + tu4.y * u) + tu4.z * v, tvv = (r0.w * w + r1.w * u) + r2.w * v;
+    if (dtex) {
+      const float4 t = tex_sample(sc, dtid, tu, tvv);
+      s.diff = mk3(t.x, t.y, t.z);
+      s.trans = t.w * md.w;
+    }
+    if (etex) {
+      const float4 t = tex_sample(sc, etid, tu, tvv);
+      s.emissive = mk3(t.x, t.y, t.z);
+    }
+  }
+  s.spec = me.w; s.metal = mx.x; s.rough = mx.y;
   return s;
 }
 
@@ -1317,10 +1321,11 @@ __device__ __forceinline__ void apply_pixel_gbuf(const Accum& ac, uint32_t pix, 
   }
 }
 
-// Append with one atomic per workgroup: returns this lane's slot when pred.
+// Append with one atomic per workgroup of B threads: returns this lane's slot when pred.
+template <int B>
 __device__ __forceinline__ void block_append2(bool p0, bool p1, uint32_t* ctr0, uint32_t* ctr1, uint32_t& s0,
                                               uint32_t& s1) {
-  __shared__ uint32_t wc0[kBlock / 64], wc1[kBlock / 64], base0, base1;
+  __shared__ uint32_t wc0[B / 64], wc1[B / 64], base0, base1;
   const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   unsigned long long m0 = __ballot(p0), m1 = __ballot(p1);
   unsigned long long below = (1ull << lane) - 1ull;
@@ -1328,7 +1333,7 @@ __device__ __forceinline__ void block_append2(bool p0, bool p1, uint32_t* ctr0, 
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t t0 = 0, t1 = 0;
-    for (int w = 0; w < kBlock / 64; w++) { t0 += wc0[w]; t1 += wc1[w]; }
+    for (int w = 0; w < B / 64; w++) { t0 += wc0[w]; t1 += wc1[w]; }
     base0 = t0 ? atomicAdd(ctr0, t0) : 0u;
     base1 = t1 ? atomicAdd(ctr1, t1) : 0u;
   }
@@ -1456,13 +1461,16 @@ __device__ __forceinline__ void shade_path(const DevScene& sc, const FrameArgs& 
 
 // The shading pass of one bounce over the live paths: survivors compacted into
 // nxt, connection shadow rays into sb (contribution .w = the survivor's slot).
-__global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, PathBuf cur, PathBuf nxt,
+// Workgroup of the shading pass: its compaction takes one atomic per counter
+// per workgroup.
+constexpr int kShadeBlock = 256;
+__global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene sc, FrameArgs fa, PathBuf cur, PathBuf nxt,
                                                   const float4* __restrict__ hits, FrameCounters* __restrict__ fc,
                                                   int bounce, ShadowBuf sb, const LightVert* __restrict__ lv,
                                                   Accum ac) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
   const uint32_t count = fc->active[bounce];
-  if (blockIdx.x * kBlock >= count) return;               // whole workgroup idle
+  if (blockIdx.x * kShadeBlock >= count) return;               // whole workgroup idle
   bool survive = false, emit = false;
   PathState p;
   ShadowRay sr;
@@ -1480,7 +1488,7 @@ __global__ __launch_bounds__(kBlock) void k_shade(DevScene sc, FrameArgs fa, Pat
     shade_path(sc, fa, fc, lv, ac, p, hits[i], survive, emit, sr);
   }
   uint32_t slot = 0, sslot = 0;
-  block_append2(survive, emit, &fc->active[bounce + 1], &fc->shadow[bounce], slot, sslot);
+  block_append2<kShadeBlock>(survive, emit, &fc->active[bounce + 1], &fc->shadow[bounce], slot, sslot);
   if (survive) {
     nxt.ray_o[slot] = p.ro;
     nxt.ray_d[slot] = p.rd;
@@ -1509,7 +1517,8 @@ hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, Pa
   if (ac.apply == 1 && (!rec_ok(ac.rec) || !ac.gpos || !ac.gnrm || !ac.gdif)) return hipErrorInvalidValue;
   if (ac.apply == 2 && !ac.rec.acc) return hipErrorInvalidValue;
   if ((ac.variant & kVariantGbuf) && !rec_ok(ac.rec)) return hipErrorInvalidValue;
-  k_shade<<<(max_n + kBlock - 1) / kBlock, kBlock, 0, st>>>(sc, fa, cur, nxt, hits, fc, bounce, sb, lv, ac);
+  k_shade<<<(max_n + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, st>>>(sc, fa, cur, nxt, hits, fc, bounce, sb, lv,
+                                                                         ac);
   return hipGetLastError();
 }
 
