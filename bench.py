@@ -317,8 +317,8 @@ def main():
     ap.add_argument("--pass-frames", type=int, default=8, help="adaptive pass size in frames (learned leg)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (one-GPU build; "
-              "see the sharding note in DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the library renders on one "
+              "device; several GPUs would each take their own views or frames)", file=sys.stderr)
         sys.exit(2)
 
     import cudapt_amd as ca
