@@ -1349,6 +1349,45 @@ __device__ __forceinline__ void block_append2(bool p0, bool p1, uint32_t* ctr0, 
   s1 = base1 + pre1 + __popcll(m1 & below);
 }
 
+// Append with one atomic per workgroup of B threads, grouped by a 3-bit key
+// (the ray direction's octant): within the workgroup's share of each queue the
+// lanes come key by key, so that a wave of the next traversal holds rays of few
+// octants.  k0 / k1: the key of a lane appending to queue 0 / 1.
+template <int B>
+__device__ __forceinline__ void block_append2_oct(bool p0, uint32_t k0, bool p1, uint32_t k1, uint32_t* ctr0,
+                                                  uint32_t* ctr1, uint32_t& s0, uint32_t& s1) {
+  constexpr int W = B / 64;
+  __shared__ uint32_t c0[8][W], c1[8][W], base0, base1;
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  const unsigned long long below = (1ull << lane) - 1ull;
+  uint32_t r0 = 0, r1 = 0;
+#pragma unroll
+  for (uint32_t b = 0; b < 8; b++) {
+    const unsigned long long m0 = __ballot(p0 && k0 == b), m1 = __ballot(p1 && k1 == b);
+    if (lane == 0) { c0[b][wid] = (uint32_t)__popcll(m0); c1[b][wid] = (uint32_t)__popcll(m1); }
+    if (k0 == b) r0 = (uint32_t)__popcll(m0 & below);
+    if (k1 == b) r1 = (uint32_t)__popcll(m1 & below);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {                  // exclusive prefix over (key, wave), then one atomic per queue
+    uint32_t t0 = 0, t1 = 0;
+    for (int b = 0; b < 8; b++)
+      for (int w = 0; w < W; w++) {
+        const uint32_t a = c0[b][w], c = c1[b][w];
+        c0[b][w] = t0; c1[b][w] = t1;
+        t0 += a; t1 += c;
+      }
+    base0 = t0 ? atomicAdd(ctr0, t0) : 0u;
+    base1 = t1 ? atomicAdd(ctr1, t1) : 0u;
+  }
+  __syncthreads();
+  s0 = p0 ? base0 + c0[k0 & 7u][wid] + r0 : 0u;
+  s1 = p1 ? base1 + c1[k1 & 7u][wid] + r1 : 0u;
+}
+__device__ __forceinline__ uint32_t octant(float4 d) {
+  return (d.x < 0.f ? 1u : 0u) | (d.y < 0.f ? 2u : 0u) | (d.z < 0.f ? 4u : 0u);
+}
+
 // One eye-path vertex (pt_traceSample_kernel, src/cudaRTBDPTStream.cu:365-543,
 // and the connection set-up of pt_connectEyeLightPath_kernel, :665-723):
 // emission, BRDF sample, Russian roulette, termination into the accumulators,
@@ -1493,7 +1532,8 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene sc, FrameArgs fa
     shade_path(sc, fa, fc, lv, ac, p, hits[i], survive, emit, sr);
   }
   uint32_t slot = 0, sslot = 0;
-  block_append2<kShadeBlock>(survive, emit, &fc->active[bounce + 1], &fc->shadow[bounce], slot, sslot);
+  block_append2_oct<kShadeBlock>(survive, survive ? octant(p.rd) : 8u, emit, emit ? octant(sr.d) : 8u,
+                                 &fc->active[bounce + 1], &fc->shadow[bounce], slot, sslot);
   if (survive) {
     nxt.ray_o[slot] = p.ro;
     nxt.ray_d[slot] = p.rd;
