@@ -168,7 +168,8 @@ def hbm_traffic(args):
                "--in-flight", str(args.in_flight),
                "--lds-stack", str(args.lds_stack), "--lds-stack-shadow", str(args.lds_stack_shadow),
                "--node-format", str(args.node_format),
-               "--refill", str(args.refill), "--refill-groups", str(args.refill_groups),
+               "--refill", str(args.refill), "--refill-secondary", str(args.refill_secondary),
+               "--refill-groups", str(args.refill_groups),
                "--refill-shadow", str(args.refill_shadow),
                "--copy-calib", str(calib >> 20)]
         log(f"traffic pass: {counter}")
@@ -212,6 +213,7 @@ def baselines_leg(ca, S, sc, args):
         R.set_attrib("Shadow LDS Stack", args.lds_stack_shadow or args.lds_stack)
         R.set_attrib("Node Format", args.node_format)
         R.set_attrib("Traversal Refill", args.refill)
+        R.set_attrib("Secondary Refill", args.refill_secondary)
         R.set_attrib("Refill Workgroups", args.refill_groups)
         R.set_attrib("Shadow Refill", args.refill_shadow)
         return R
@@ -296,6 +298,8 @@ def main():
     ap.add_argument("--node-format", type=int, default=1, help="BVH nodes: 0 = 128-byte FP32, 1 = 64-byte quantized")
     ap.add_argument("--refill", type=int, default=64,
                     help="traversal lanes refill from a ray queue once this many of a wave's are idle (0: off)")
+    ap.add_argument("--refill-secondary", type=int, default=48,
+                    help="the same for extension rays after the first bounce")
     ap.add_argument("--refill-shadow", type=int, default=16, help="the same for shadow rays")
     ap.add_argument("--refill-groups", type=int, default=8, help="workgroups per CU of the refilling grid")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
@@ -343,6 +347,7 @@ def main():
     R.set_attrib("Shadow LDS Stack", args.lds_stack_shadow or args.lds_stack)
     R.set_attrib("Node Format", args.node_format)
     R.set_attrib("Traversal Refill", args.refill)
+    R.set_attrib("Secondary Refill", args.refill_secondary)
     R.set_attrib("Refill Workgroups", args.refill_groups)
     R.set_attrib("Shadow Refill", args.refill_shadow)
     cam = sc["camera"]
@@ -389,7 +394,8 @@ def main():
                    "eye_depth": EYE_DEPTH, "light_depth": LIGHT_DEPTH, "frames_in_flight": args.in_flight,
                    "traversal_variant": args.lds_stack,
                    "shadow_variant": args.lds_stack_shadow or args.lds_stack, "node_format": args.node_format,
-                   "traversal_refill": args.refill, "refill_groups": args.refill_groups,
+                   "traversal_refill": args.refill, "secondary_refill": args.refill_secondary,
+                   "refill_groups": args.refill_groups,
                    "shadow_refill": args.refill_shadow},
         "samples_per_s": round(st["eye_paths"] / wall, 1),
         "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},

@@ -155,8 +155,10 @@ def test_refill_traversal_same_image(cpt, cornell_runs, variant, node_format, re
         R.set_attrib("Traversal LDS Stack", variant)
         R.set_attrib("Shadow LDS Stack", variant)
     a.set_attrib("Traversal Refill", 0)
+    a.set_attrib("Secondary Refill", 0)
     a.set_attrib("Shadow Refill", 0)
     b.set_attrib("Traversal Refill", refill)
+    b.set_attrib("Secondary Refill", max(1, refill // 2))
     b.set_attrib("Shadow Refill", 65 - refill)
     b.set_attrib("Refill Min Paths", 0)
     ra = a.render(S, sc["camera"], frames=6)
