@@ -921,7 +921,9 @@ struct PathState {
   uint64_t rng;
 };
 // PathBuf::depth packs depth (bits 0-7), frame offset (8-15), depth bound
-// (16-23) and ray type (24-25).
+// (16-23), ray type (24-25) and kFreshPath: a path k_gen_eye started, whose
+// mul / smp / acc columns were not written (throughput 1, nothing gathered).
+constexpr uint32_t kFreshPath = 1u << 26;
 __device__ __forceinline__ uint32_t pack_dk(const PathState& p) {
   return p.depth | (p.fk << 8) | (p.lim << 16) | (p.type << 24);
 }
@@ -929,7 +931,7 @@ __device__ __forceinline__ void unpack_dk(PathState& p, uint32_t dk) {
   p.depth = dk & 0xFFu;
   p.fk = (dk >> 8) & 0xFFu;
   p.lim = (dk >> 16) & 0xFFu;
-  p.type = dk >> 24;
+  p.type = (dk >> 24) & 3u;
 }
 // The connection's shadow ray: origin and max distance, direction and epsilon,
 // contribution added to the path's accumulator when nothing occludes it.
@@ -1018,8 +1020,13 @@ __global__ __launch_bounds__(kBlock) void k_gen_eye(FrameArgs fa, PathBuf pb, Sa
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= wh * fa.nframes) return;
   const uint32_t K = fa.nframes, pix = i / K, k = i - pix * K, sid = k * wh + pix;
-  write_eye_path(fa, pb, i, pix, sid, stream_key(fa.seed, frame_base(fa) + k, 1u, 0u));
-  pb.depth[i] = k << 8;
+  const PathState p = eye_path(fa, pix, sid, stream_key(fa.seed, frame_base(fa) + k, 1u, 0u));
+  pb.ray_o[i] = p.ro;
+  pb.ray_d[i] = p.rd;
+  pb.pix[i] = p.pix;
+  pb.sid[i] = p.sid;
+  pb.depth[i] = (k << 8) | kFreshPath;
+  pb.rng[i] = p.rng;
   if (rec.acc) zero_rec(rec, sid);
 }
 hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st) {
@@ -1524,10 +1531,17 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene sc, FrameArgs fa
     p.rd = cur.ray_d[i];
     p.pix = cur.pix[i];
     p.sid = cur.sid[i];
-    unpack_dk(p, cur.depth[i]);
-    p.mul = cur.mul[i];
-    p.smp = cur.smp[i];
-    p.acc = cur.acc[i];
+    const uint32_t dk = cur.depth[i];
+    unpack_dk(p, dk);
+    if (dk & kFreshPath) {
+      p.mul = make_float4(1.f, 1.f, 1.f, 0.f);
+      p.smp = make_float4(0.f, 0.f, 0.f, 0.f);
+      p.acc = p.smp;
+    } else {
+      p.mul = cur.mul[i];
+      p.smp = cur.smp[i];
+      p.acc = cur.acc[i];
+    }
     p.rng = cur.rng[i];
     shade_path(sc, fa, fc, lv, ac, p, hits[i], survive, emit, sr);
   }
