@@ -641,9 +641,17 @@ struct cpt_renderer {
   std::vector<Ev> events;          // recorded pairs awaiting readback
   std::vector<hipEvent_t> pool;    // reusable events
   double ms[10] = {0};  // extend, shadow, shade, light, total, filter, error sum, generation, accumulation, model
+  // shadow rays of bounce b traced on `side` while bounce b + 1's extension
+  // rays run on `stream` (off for graph-captured batches)
+  bool overlap_shadow = true;
+  hipStream_t side = nullptr;
+  hipEvent_t ev_shaded = nullptr, ev_shadow = nullptr;
   ~cpt_renderer() {
     for (auto& e : events) { pool.push_back(e.a); pool.push_back(e.b); }
     for (auto e : pool) (void)hipEventDestroy(e);
+    if (ev_shaded) (void)hipEventDestroy(ev_shaded);
+    if (ev_shadow) (void)hipEventDestroy(ev_shadow);
+    if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
   hipError_t take(hipEvent_t* e) {
@@ -739,7 +747,7 @@ int renderer_alloc(cpt_renderer* r) {
     HIP_TRY(r->term_rng.alloc(cap));
     HIP_TRY(r->tend.alloc(kMaxInFlight));
   }
-  HIP_TRY(r->queue.alloc(1));
+  HIP_TRY(r->queue.alloc(2));
   HIP_TRY(r->fc.alloc(1));
   HIP_TRY(r->stats.alloc(1));
   HIP_TRY(r->result.alloc(n * 3));
@@ -778,19 +786,19 @@ LightJobs light_jobs(cpt_renderer* r, u32 f0, u32 nf) {
   return j;
 }
 
-hipError_t timed_begin(cpt_renderer* r, int kind, hipEvent_t* a) {
+hipError_t timed_begin(cpt_renderer* r, int kind, hipEvent_t* a, hipStream_t st = nullptr) {
   if (!r->profiling) return hipSuccess;
   hipError_t e = r->take(a);
   if (e) return e;
   (void)kind;
-  return hipEventRecord(*a, r->stream);
+  return hipEventRecord(*a, st ? st : r->stream);
 }
-hipError_t timed_end(cpt_renderer* r, int kind, hipEvent_t a) {
+hipError_t timed_end(cpt_renderer* r, int kind, hipEvent_t a, hipStream_t st = nullptr) {
   if (!r->profiling) return hipSuccess;
   hipEvent_t b;
   hipError_t e = r->take(&b);
   if (e) return e;
-  e = hipEventRecord(b, r->stream);
+  e = hipEventRecord(b, st ? st : r->stream);
   r->events.push_back({a, b, kind});
   return e;
 }
@@ -829,6 +837,19 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
   sb.ray_o = r->sh_o.p; sb.ray_d = r->sh_d.p; sb.contrib = r->sh_c.p;
   const bool big = (double)max_n >= r->attr(kRefillMinPaths);
   const uint32_t refill_sh = big ? (uint32_t)std::min(64.0, std::max(0.0, r->attr(kRefillShadow))) : 0u;
+  // Shadow rays of bounce b go to the side stream once bounce b is shaded and
+  // overlap bounce b + 1's extension rays (which read only ray origins and
+  // directions; the shadow kernel writes only accumulators); shading of bounce
+  // b + 1, which reads those accumulators and rewrites the shadow queue, waits
+  // for it.  The two refilling grids take their rays from separate counters.
+  const bool ov = r->overlap_shadow;
+  if (ov) {
+    if (!r->side) HIP_TRY(hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking));
+    if (!r->ev_shaded) HIP_TRY(hipEventCreateWithFlags(&r->ev_shaded, hipEventDisableTiming));
+    if (!r->ev_shadow) HIP_TRY(hipEventCreateWithFlags(&r->ev_shadow, hipEventDisableTiming));
+  }
+  hipStream_t sst = ov ? r->side : r->stream;
+  bool shadow_pending = false;
   for (u32 b = 0; b < fa.eye_depth; b++) {
     r->extend_launches++;
     HIP_TRY(timed_begin(r, 0, &ev));
@@ -848,23 +869,36 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
       HIP_TRY(launch_extend(sc, cur->view(), r->fc.p, (int)b, max_n, r->hits.p, r->stats.p, (int)r->attr(kLdsStack),
                             quant, r->stream));
     HIP_TRY(timed_end(r, 0, ev));
+    if (shadow_pending) {
+      HIP_TRY(hipStreamWaitEvent(r->stream, r->ev_shadow, 0));
+      shadow_pending = false;
+    }
     HIP_TRY(timed_begin(r, 2, &ev));
     HIP_TRY(launch_shade(sc, fa, cur->view(), nxt->view(), r->hits.p, r->fc.p, (int)b, max_n, sb, r->lv.p, ac,
                          r->stream));
     HIP_TRY(timed_end(r, 2, ev));
     if (b + 1 < fa.eye_depth) {
-      HIP_TRY(timed_begin(r, 1, &ev));
+      if (ov) {
+        HIP_TRY(hipEventRecord(r->ev_shaded, r->stream));
+        HIP_TRY(hipStreamWaitEvent(sst, r->ev_shaded, 0));
+      }
+      HIP_TRY(timed_begin(r, 1, &ev, sst));
       if (refill_sh)
         HIP_TRY(launch_trace_dyn(sc, sb.ray_o, sb.ray_d, r->fc.p->shadow + b, nullptr, sb.contrib, nxt->view(),
-                                 r->stats.p, (int)r->attr(kLdsStackShadow), quant, 1, dyn_grid, refill_sh, r->queue.p,
-                                 r->stream));
+                                 r->stats.p, (int)r->attr(kLdsStackShadow), quant, 1, dyn_grid, refill_sh,
+                                 r->queue.p + (ov ? 1 : 0), sst));
       else
         HIP_TRY(launch_shadow(sc, sb, nxt->view(), r->fc.p, (int)b, max_n, (int)r->attr(kLdsStackShadow), quant,
-                              r->stats.p, r->stream));
-      HIP_TRY(timed_end(r, 1, ev));
+                              r->stats.p, sst));
+      HIP_TRY(timed_end(r, 1, ev, sst));
+      if (ov) {
+        HIP_TRY(hipEventRecord(r->ev_shadow, sst));
+        shadow_pending = true;
+      }
     }
     std::swap(cur, nxt);
   }
+  if (shadow_pending) HIP_TRY(hipStreamWaitEvent(r->stream, r->ev_shadow, 0));
   return CPT_OK;
 }
 
@@ -1695,7 +1729,7 @@ int batch_prepare(cpt_batch* b, cpt_scene* s) {
   }
   int rc = lv_alloc(r);
   if (rc) return rc;
-  HIP_TRY(r->queue.alloc(1));
+  HIP_TRY(r->queue.alloc(2));
   if (b->buf_key != (const void*)r->A.ray_o.p) {   // buffers moved: a captured graph points at the old ones
     b->drop_graph();
     b->buf_key = (const void*)r->A.ray_o.p;
@@ -1806,6 +1840,7 @@ int cpt_batch_create(cpt_batch** out, const cpt_view* views, uint32_t n_views) {
   b->cfg = default_batch_config();
   int rc = cpt_renderer_create(&b->r, CPT_MODE_BDPT_STREAM, b->total, 1);
   if (rc) { delete b; return rc; }
+  b->r->overlap_shadow = false;                  // iterations are captured from one stream
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     b->cus = 256;
