@@ -8,8 +8,8 @@ before timing; the timed region is K renders of 64 frames each.
 
 metric/value: Mrays/s = (extension + shadow rays traced) / wall time.
 roofline: the closest-hit extension kernel (by default k_trace_dyn: persistent
-  waves that take the next 64 rays from a global queue once all their lanes are
-  done), algorithmic bytes per launch =
+  waves that take new rays from a global queue once 64 (first bounce) or 48
+  (later bounces) of their lanes are idle), algorithmic bytes per launch =
   48 B per ray (32 B ray in + 16 B hit out) + 128 B (FP32 layout) or 64 B
   (quantized layout) per 4-wide node visited + 48 B per triangle tested, over
   its HIP-event duration on the renderer stream.
@@ -110,7 +110,7 @@ def learned_leg(ca, S, args):
     losses, train_s = learned.train(m, X, T, steps=args.learn_steps, batch=65536, lr=1e-3)
     log(f"learned leg: trained in {train_s:.2f} s, loss {losses[0][1]:.3f} -> {losses[-1][1]:.3f}")
     rows = learned.time_to_equal_mse(S, m, cams[n_tr:], refs_te, WIDTH, HEIGHT, frames=64, depth=EYE_DEPTH,
-                                     pass_frames=args.pass_frames, signal=1)
+                                     pass_frames=args.pass_frames, signal=3)
     ratios = [r["learned_s"] / r["plain_adaptive_s"] for r in rows if r["learned_s"]]
     bdpt = [r["plain_bdpt_s"] / r["plain_adaptive_s"] for r in rows if r["plain_bdpt_s"]]
     gm = float(np.exp(np.mean(np.log(ratios)))) if len(ratios) == len(rows) else None
@@ -136,7 +136,7 @@ def learned_leg(ca, S, args):
                      "train_poses": n_tr, "heldout_poses": n_te, "train_ref_spp": args.learn_train_spp,
                      "heldout_ref_spp": args.learn_ref_spp, "plain_frames": 64, "pass_frames": args.pass_frames,
                      "train_steps": args.learn_steps, "batch": 65536, "lr": 1e-3, "train_samples": int(len(T)),
-                     "mse": "radiance clamped to [0, 1]", "signal": "exp(y) / N",
+                     "mse": "radiance clamped to [0, 1]", "signal": "sqrt(exp(y) / N * filter residual)",
                      "timing": "synchronised wall time per render call, MSE readback excluded"},
         "train_s": round(train_s, 3),
         "final_loss": round(float(losses[-1][1]), 4),

@@ -583,7 +583,9 @@ const char* const kRefillMinPaths = "Refill Min Paths";
 // variance): 0 = nd / (nd + N) * exp(y) / N, the filter-residual estimate's
 // shape; 1 = exp(y) / N, the expected squared error (samples settle where N
 // grows with the per-sample standard deviation); 2 = exp(y) / N^2, the error
-// one more sample removes (sigma^2 / N - sigma^2 / (N + 1) ~ sigma^2 / N^2)
+// one more sample removes (sigma^2 / N - sigma^2 / (N + 1) ~ sigma^2 / N^2);
+// 3 = sqrt(exp(y) / N * e_plain), the geometric mean with the mode's own error
+// map (filter residual or empirical variance), a hedge where either misjudges
 const char* const kLearnedSignal = "Learned Signal";
 // 1: the plain stream renderer traces each batch with one megakernel (one thread
 // carries its sample through every bounce, the reference's regeneration
@@ -1029,7 +1031,8 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
       rc = model_features(r, r->mlp_x.p);
       if (rc) return rc;
       HIP_TRY(launch_mlp_infer(r->mlp->state(), r->mlp_x.p, n, r->mlp_y.p, r->stream));
-      HIP_TRY(launch_mlp_err(r->mlp_y.p, r->sample_n.p, nd, (u32)r->attr(kLearnedSignal), n, r->mlp_e.p, r->stream));
+      HIP_TRY(launch_mlp_err(r->mlp_y.p, r->sample_n.p, nd, (u32)r->attr(kLearnedSignal), n, r->err.p, r->mlp_e.p,
+                             r->stream));
       HIP_TRY(timed_end(r, 9, ev));
       alloc_err = r->mlp_e.p;
     }

@@ -75,8 +75,11 @@ hipError_t launch_mlp_features(const MlpFeatureArgs& a, uint32_t n, float* x, hi
 hipError_t launch_mlp_target(const float* result, const float* ref, const uint32_t* sample_n, uint32_t n, float clamp,
                              float* t, hipStream_t s);
 // signal 0: nd / (nd + N) * exp(y) / N (the filter-residual estimate's shape);
-// signal 1: exp(y) / N (the pixel's expected squared error).
-hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd, uint32_t signal, uint32_t n, float* e,
+// signal 1: exp(y) / N (the pixel's expected squared error); signal 2:
+// exp(y) / N^2; signal 3: sqrt(exp(y) / N * plain), the geometric mean of the
+// learned estimate and the plain error map (filter residual or empirical).
+hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd, uint32_t signal, uint32_t n,
+                          const float* plain, float* e,
                           hipStream_t s);
 
 }  // namespace cpt

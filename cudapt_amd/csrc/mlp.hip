@@ -417,13 +417,17 @@ __global__ __launch_bounds__(256) void k_mlp_target(const float* __restrict__ re
 }
 
 __global__ __launch_bounds__(256) void k_mlp_err(const float* __restrict__ y, const uint32_t* __restrict__ sample_n,
-                                                 uint32_t nd, uint32_t signal, uint32_t n, float* __restrict__ e) {
+                                                 uint32_t nd, uint32_t signal, uint32_t n,
+                                                 const float* __restrict__ plain, float* __restrict__ e) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
   if (i >= n) return;
   const uint32_t N = max(sample_n[i], 1u);
   const float v = expf(fminf(fmaxf(y[i], -60.f), 60.f));
   const float fn = (float)N;
-  e[i] = signal == 2 ? v / fn / fn : signal ? v / fn : (float)nd / (float)(nd + N) * v / fn;
+  e[i] = signal == 3   ? sqrtf(v / fn * fmaxf(plain[i], 0.f))
+         : signal == 2 ? v / fn / fn
+         : signal      ? v / fn
+                       : (float)nd / (float)(nd + N) * v / fn;
 }
 
 hipError_t launch_mlp_features(const MlpFeatureArgs& a, uint32_t n, float* x, hipStream_t s) {
@@ -441,11 +445,11 @@ hipError_t launch_mlp_target(const float* result, const float* ref, const uint32
   k_mlp_target<<<(n + 255) / 256, 256, 0, s>>>(result, ref, sample_n, n, clamp, t);
   return hipGetLastError();
 }
-hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd, uint32_t signal, uint32_t n, float* e,
-                          hipStream_t s) {
-  if (!y || !sample_n || !e) return hipErrorInvalidValue;
+hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd, uint32_t signal, uint32_t n,
+                          const float* plain, float* e, hipStream_t s) {
+  if (!y || !sample_n || !e || signal > 3 || (signal == 3 && !plain)) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
-  k_mlp_err<<<(n + 255) / 256, 256, 0, s>>>(y, sample_n, nd, signal, n, e);
+  k_mlp_err<<<(n + 255) / 256, 256, 0, s>>>(y, sample_n, nd, signal, n, plain, e);
   return hipGetLastError();
 }
 

@@ -178,9 +178,12 @@ def target(result, ref, sample_n, clamp: float = 0.0) -> np.ndarray:
     return np.log(np.asarray(sample_n, np.float64).ravel() * (d * d).sum(1) / 3.0 + 1e-12)
 
 
-def learned_error(y, sample_n, nd: int, signal: int = 0) -> np.ndarray:
-    """Allocation signal nd / (nd + N) * exp(y) / N (signal 0), exp(y) / N (1)
-    or exp(y) / N^2 (2)."""
+def learned_error(y, sample_n, nd: int, signal: int = 0, plain=None) -> np.ndarray:
+    """Allocation signal nd / (nd + N) * exp(y) / N (signal 0), exp(y) / N (1),
+    exp(y) / N^2 (2) or sqrt(exp(y) / N * plain) (3, plain = the mode's own
+    error map)."""
     N = np.maximum(np.asarray(sample_n, np.float64).ravel(), 1.0)
     v = np.exp(np.clip(np.asarray(y, np.float64), -60, 60)) / N
+    if signal == 3:
+        return np.sqrt(v * np.maximum(np.asarray(plain, np.float64).ravel(), 0.0))
     return v / N if signal == 2 else v if signal else nd / (nd + N) * v

@@ -179,7 +179,7 @@ def test_learned_allocation_follows_model(cpt, adaptive_state):
     R.set_mlp(None)
 
 
-@pytest.mark.parametrize("signal", [1, 2])
+@pytest.mark.parametrize("signal", [1, 2, 3])
 def test_expected_error_signal(cpt, signal):
     sc = scenes.cornell()
     S = cpt.Scene(sc)
@@ -194,4 +194,5 @@ def test_expected_error_signal(cpt, signal):
     R.set_mlp(m)
     R.render(S, sc["camera"], frames=1)
     sig = R.read(cpt.CPT_BUF_ALLOC_ERROR).ravel()
-    np.testing.assert_allclose(sig, M.learned_error(y, N, 5, signal=signal), rtol=1e-5, atol=0)
+    plain = R.read(cpt.CPT_BUF_ERROR).ravel()          # the pass's filter residual
+    np.testing.assert_allclose(sig, M.learned_error(y, N, 5, signal=signal, plain=plain), rtol=1e-5, atol=0)

@@ -129,3 +129,17 @@ def test_feature_target_and_signal_formulas():
                                [1 / 5, 8 / 2])
     np.testing.assert_allclose(M.learned_error(np.array([0.0, np.log(8.0)]), np.array([5, 2]), 5, signal=2),
                                [1 / 25, 8 / 4])
+
+
+def test_learned_error_geometric_mean_signal():
+    """Signal 3 is the geometric mean of exp(y) / N and the mode's own error map:
+    equal inputs give that value back, and it scales as the square root of each."""
+    rng = np.random.default_rng(3)
+    y = rng.normal(-3.0, 1.0, 64)
+    N = rng.integers(1, 50, 64)
+    v = M.learned_error(y, N, 5, signal=1)
+    assert np.allclose(M.learned_error(y, N, 5, signal=3, plain=v), v, rtol=1e-12)
+    plain = rng.uniform(0.0, 0.1, 64)
+    assert np.allclose(M.learned_error(y, N, 5, signal=3, plain=4 * plain),
+                       2 * M.learned_error(y, N, 5, signal=3, plain=plain), rtol=1e-12)
+    assert (M.learned_error(y, N, 5, signal=3, plain=-plain) == 0).all()
