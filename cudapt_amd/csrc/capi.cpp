@@ -644,7 +644,8 @@ struct cpt_renderer {
   std::vector<hipEvent_t> pool;    // reusable events
   double ms[10] = {0};  // extend, shadow, shade, light, total, filter, error sum, generation, accumulation, model
   // shadow rays of bounce b traced on `side` while bounce b + 1's extension
-  // rays run on `stream` (off for graph-captured batches)
+  // rays run on `stream` (a captured batch iteration takes the side stream
+  // into its graph through the same events)
   bool overlap_shadow = true;
   hipStream_t side = nullptr;
   hipEvent_t ev_shaded = nullptr, ev_shadow = nullptr;
@@ -1843,7 +1844,6 @@ int cpt_batch_create(cpt_batch** out, const cpt_view* views, uint32_t n_views) {
   b->cfg = default_batch_config();
   int rc = cpt_renderer_create(&b->r, CPT_MODE_BDPT_STREAM, b->total, 1);
   if (rc) { delete b; return rc; }
-  b->r->overlap_shadow = false;                  // iterations are captured from one stream
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&b->cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
     b->cus = 256;
