@@ -1333,29 +1333,6 @@ __device__ __forceinline__ void apply_pixel_gbuf(const Accum& ac, uint32_t pix, 
   }
 }
 
-// Append with one atomic per workgroup of B threads: returns this lane's slot when pred.
-template <int B>
-__device__ __forceinline__ void block_append2(bool p0, bool p1, uint32_t* ctr0, uint32_t* ctr1, uint32_t& s0,
-                                              uint32_t& s1) {
-  __shared__ uint32_t wc0[B / 64], wc1[B / 64], base0, base1;
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  unsigned long long m0 = __ballot(p0), m1 = __ballot(p1);
-  unsigned long long below = (1ull << lane) - 1ull;
-  if (lane == 0) { wc0[wid] = __popcll(m0); wc1[wid] = __popcll(m1); }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t t0 = 0, t1 = 0;
-    for (int w = 0; w < B / 64; w++) { t0 += wc0[w]; t1 += wc1[w]; }
-    base0 = t0 ? atomicAdd(ctr0, t0) : 0u;
-    base1 = t1 ? atomicAdd(ctr1, t1) : 0u;
-  }
-  __syncthreads();
-  uint32_t pre0 = 0, pre1 = 0;
-  for (int w = 0; w < wid; w++) { pre0 += wc0[w]; pre1 += wc1[w]; }
-  s0 = base0 + pre0 + __popcll(m0 & below);
-  s1 = base1 + pre1 + __popcll(m1 & below);
-}
-
 // Append with one atomic per workgroup of B threads, grouped by a 3-bit key
 // (the ray direction's octant): within the workgroup's share of each queue the
 // lanes come key by key, so that a wave of the next traversal holds rays of few
