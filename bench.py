@@ -68,6 +68,24 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def copy_peak_gbs(ca, mib=1024, reps=8):
+    """Measured device-to-device copy rate (bytes read + written per second) of
+    a `mib` MiB buffer: the practical HBM ceiling beside the nominal 8 TB/s."""
+    nbytes = mib << 20
+    src, dst = ca.DeviceBuffer(nbytes), ca.DeviceBuffer(nbytes)
+    copy = lambda: ca._check(ca.lib().cpt_memcpy_dtod(ctypes.c_void_p(dst.ptr), ctypes.c_void_p(src.ptr), nbytes),
+                             "cpt_memcpy_dtod")
+    copy()
+    ca.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        copy()
+    ca.synchronize()
+    dt = time.perf_counter() - t0
+    del src, dst
+    return 2.0 * nbytes * reps / dt / 1e9
+
+
 def cpu_baseline(sc, seconds_target=12.0, max_frames=64):
     """Oracle FP32 reference estimator over whole 1280x720 frames at 1 spp each
     (frames 0, 1, ... of the same stream) until ~seconds_target of wall time."""
@@ -414,6 +432,10 @@ def main():
         "time_to_equal_mse": None,
     }
     del R
+    peak = copy_peak_gbs(ca)
+    log(f"device-to-device copy: {peak:.0f} GB/s")
+    res["roofline"]["copy_peak_gbs"] = round(peak, 1)
+    res["roofline"]["frac_of_copy_peak"] = round(achieved / peak, 4)
     res["baselines"] = None
     if args.baseline_steps > 0:
         res["baselines"] = baselines_leg(ca, S, sc, args)
