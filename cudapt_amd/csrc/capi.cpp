@@ -832,7 +832,15 @@ int update_gauss(cpt_renderer* r, float sigma, u32 radius) {
   return CPT_OK;
 }
 
-int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 max_n, const Accum& ac) {
+// gen_eye: the batch starts from fresh eye paths in r->A.  With refilling
+// waves at bounce 0, quantized nodes and one view nothing writes them: the first extension and
+// shading compute each slot's eye path where k_gen_eye would have stored it
+// (3 GB of path state at 1280x720 and 64 frames).  Otherwise k_gen_eye writes
+// them here: a multi-view batch finds each pixel's view by a search of
+// dependent loads, which a refilling wave would wait for (4% slower over 32
+// views).
+int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 max_n, const Accum& ac,
+                bool gen_eye = false) {
   hipEvent_t ev = nullptr;
   PathSet* cur = &r->A;
   PathSet* nxt = &r->B;
@@ -852,6 +860,9 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
     if (!r->ev_shadow) HIP_TRY(hipEventCreateWithFlags(&r->ev_shadow, hipEventDisableTiming));
   }
   hipStream_t sst = ov ? r->side : r->stream;
+  const bool eye0 =
+      gen_eye && big && r->attr(kRefill) >= 1.0 && r->attr(kNodeFormat) != 0 && fa.n_views == 0;
+  if (gen_eye && !eye0) HIP_TRY(launch_gen_eye(fa, r->A.view(), r->stream));
   bool shadow_pending = false;
   for (u32 b = 0; b < fa.eye_depth; b++) {
     r->extend_launches++;
@@ -864,7 +875,7 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
     if (refill)
       HIP_TRY(launch_trace_dyn(sc, cur->view().ray_o, cur->view().ray_d, r->fc.p->active + b, r->hits.p, nullptr,
                                PathBuf{}, r->stats.p, (int)r->attr(kLdsStack), quant, 0, dyn_grid, refill, r->queue.p,
-                               r->stream));
+                               r->stream, eye0 && b == 0 ? &fa : nullptr));
     else if (r->persist_chunk)
       HIP_TRY(launch_extend_persist(sc, cur->view(), r->fc.p, (int)b, r->hits.p, r->stats.p, (int)r->attr(kLdsStack),
                                     quant, r->persist_chunk, r->persist_grid, r->queue.p, r->stream));
@@ -878,7 +889,7 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
     }
     HIP_TRY(timed_begin(r, 2, &ev));
     HIP_TRY(launch_shade(sc, fa, cur->view(), nxt->view(), r->hits.p, r->fc.p, (int)b, max_n, sb, r->lv.p, ac,
-                         r->stream));
+                         r->stream, eye0 && b == 0));
     HIP_TRY(timed_end(r, 2, ev));
     if (b + 1 < fa.eye_depth) {
       if (ov) {
@@ -1101,8 +1112,7 @@ int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
                         r->stats.p, r->stream));
     HIP_TRY(timed_end(r, 0, ev));
   } else {
-    HIP_TRY(launch_gen_eye(fa, r->A.view(), r->stream));
-    int rc = bounce_loop(r, sc, fa, n, ac);
+    int rc = bounce_loop(r, sc, fa, n, ac, true);
     if (rc) return rc;
   }
   if (regen) {                                     // the restarts, as a second wavefront
@@ -1698,14 +1708,13 @@ int batch_enqueue(cpt_batch* b, cpt_scene* s, u32 k) {
   const u32 n = b->total * k;
   HIP_TRY(launch_light_pass(sc, fa, jobs, r->lv.p, r->fc.p, r->stream));
   HIP_TRY(launch_frame_begin(r->fc.p, n, r->stream));
-  HIP_TRY(launch_gen_eye(fa, r->A.view(), r->stream));
   Accum ac;
   std::memset(&ac, 0, sizeof ac);
   ac.result = r->result.p;
   ac.sample_n = r->sample_n.p;
   ac.rec.acc = r->rec_acc.p;
   ac.apply = 2;
-  int rc = bounce_loop(r, sc, fa, n, ac);
+  int rc = bounce_loop(r, sc, fa, n, ac, true);
   if (rc) return rc;
   HIP_TRY(launch_apply_frames(fa, r->rec_acc.p, r->result.p, r->sample_n.p, 0, r->stream));
   HIP_TRY(launch_frame_end(r->fc.p, r->stats.p, k, fa.eye_depth, r->stream));

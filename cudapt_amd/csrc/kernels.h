@@ -179,9 +179,12 @@ constexpr int kVariantReflOffset = 1;  // reflection offset of AdapBilatAtomic's
 constexpr int kVariantGbuf = 2;        // first-hit G-buffer
 constexpr int kVariantPT = 4;          // path tracing: no connection, total internal reflection reflects
 constexpr int kVariantPTRegen = 8;     // the regenerating path tracer's diffuse-to-specular cut
+// eye != 0: the bounce's paths are the eye paths k_gen_eye would have written
+// to cur (bounce 0 of a fresh batch); they are recomputed from their slot and
+// cur is not read.
 hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, PathBuf nxt, const float4* hits,
                         FrameCounters* fc, int bounce, uint32_t max_n, ShadowBuf sb, const LightVert* lv,
-                        const Accum& ac, hipStream_t st);
+                        const Accum& ac, hipStream_t st, int eye = 0);
 // Whole batch of the plain stream renderer in one kernel: eye paths, every
 // bounce's closest hit, shading and connection occlusion per thread; results
 // into ac.rec.acc (ac.apply must be 2), counters into fc and stats.
@@ -249,10 +252,12 @@ hipError_t launch_frame_end(FrameCounters* fc, DevStats* stats, uint32_t nframes
 // Closest hit (shadow = 0: hits into out) or connection visibility (shadow = 1:
 // contributions into nxt) of the `*count` rays by a persistent grid whose
 // lanes refill themselves with the next ray from the global counter `queue`.
+// eye (closest hit over quantized nodes only): ray i is the eye ray of path
+// slot i of the batch `*eye` describes, computed in place of reading ro / rd.
 hipError_t launch_trace_dyn(const DevScene& sc, const float4* ro, const float4* rd, const uint32_t* count,
                             float4* out, const float4* contrib, PathBuf nxt, DevStats* stats, int lds_stack,
                             int quant, int shadow, uint32_t grid, uint32_t refill, uint32_t* queue,
-                            hipStream_t st);
+                            hipStream_t st, const FrameArgs* eye = nullptr);
 // *frame_dev += k (the device frame counter of replayed graphs).
 hipError_t launch_frame_advance(uint32_t* frame_dev, uint32_t k, hipStream_t st);
 // Closest hit of the bounce's live paths by a persistent grid: each workgroup

@@ -1015,19 +1015,25 @@ __device__ __forceinline__ void zero_rec(SampleRec& rec, uint32_t q) {
 // frame-major slots give it 64 neighbouring pixels (extension 21 % faster on
 // the 262k-triangle grid at 64 frames).  The sample record stays
 // frame-major (sid = frame offset * W*H + pixel) for k_apply_frames.
+// The eye path of slot i and its packed depth word.  The first extension and
+// shading of a batch can call this in place of reading what k_gen_eye wrote.
+__device__ __forceinline__ PathState eye_slot(const FrameArgs& fa, uint32_t i, uint32_t& dk) {
+  const uint32_t wh = fa.width * fa.height, K = fa.nframes, pix = i / K, k = i - pix * K;
+  dk = (k << 8) | kFreshPath;
+  return eye_path(fa, pix, k * wh + pix, stream_key(fa.seed, frame_base(fa) + k, 1u, 0u));
+}
 __global__ __launch_bounds__(kBlock) void k_gen_eye(FrameArgs fa, PathBuf pb, SampleRec rec) {
-  const uint32_t wh = fa.width * fa.height;
   uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= wh * fa.nframes) return;
-  const uint32_t K = fa.nframes, pix = i / K, k = i - pix * K, sid = k * wh + pix;
-  const PathState p = eye_path(fa, pix, sid, stream_key(fa.seed, frame_base(fa) + k, 1u, 0u));
+  if (i >= fa.width * fa.height * fa.nframes) return;
+  uint32_t dk;
+  const PathState p = eye_slot(fa, i, dk);
   pb.ray_o[i] = p.ro;
   pb.ray_d[i] = p.rd;
   pb.pix[i] = p.pix;
   pb.sid[i] = p.sid;
-  pb.depth[i] = (k << 8) | kFreshPath;
+  pb.depth[i] = dk;
   pb.rng[i] = p.rng;
-  if (rec.acc) zero_rec(rec, sid);
+  if (rec.acc) zero_rec(rec, p.sid);
 }
 hipError_t launch_gen_eye(const FrameArgs& fa, PathBuf pb, hipStream_t st) {
   if (!pathbuf_ok(pb)) return hipErrorInvalidValue;
@@ -1180,13 +1186,17 @@ hipError_t launch_extend_persist(const DevScene& sc, PathBuf pb, const FrameCoun
 // OUT 1: closest hit (t, leaf, u, v) into out[ray]; OUT 3: the connection's
 // visibility (k_shadow's epilogue).
 constexpr uint32_t kDynChunk = 256;
-template <int OUT, int STK, int MINW, bool Q>
+// EYE (closest hit, quantized nodes): ray i is the eye ray of path slot i of
+// the batch fa describes, computed in place of reading ro / rd.  A separate
+// instantiation, so that the plain kernels' code is not touched by it.
+template <int OUT, int STK, int MINW, bool Q, bool EYE = false>
 __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const float4* __restrict__ ro,
                                                             const float4* __restrict__ rd,
                                                             const uint32_t* __restrict__ countp,
                                                             uint32_t* __restrict__ queue, float4* __restrict__ out,
                                                             const float4* __restrict__ contrib, PathBuf nxt,
-                                                            DevStats* __restrict__ stats, uint32_t refill) {
+                                                            DevStats* __restrict__ stats, uint32_t refill,
+                                                            FrameArgs fa) {
   constexpr bool ANY = OUT == 3;
   __shared__ uint64_t lstk_all[STK * kBlock];
   LdsU64* lstk = lds_column(lstk_all);
@@ -1251,8 +1261,14 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
         const uint32_t take = min(n_idle - given, cend - cnext);
         if (!active && rank >= given && rank < given + take) {
           ray = cnext + (rank - given);
-          const float4 a = ro[ray], b4 = rd[ray];
-          ray_begin(R, ld3(a), ld3(b4), a.w, b4.w);
+          if (EYE) {
+            uint32_t dk;
+            const PathState e = eye_slot(fa, ray, dk);
+            ray_begin(R, ld3(e.ro), ld3(e.rd), e.ro.w, e.rd.w);
+          } else {
+            const float4 a = ro[ray], b4 = rd[ray];
+            ray_begin(R, ld3(a), ld3(b4), a.w, b4.w);
+          }
           active = true;
         }
         cnext += take;
@@ -1277,19 +1293,24 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
 hipError_t launch_trace_dyn(const DevScene& sc, const float4* ro, const float4* rd, const uint32_t* count,
                             float4* out, const float4* contrib, PathBuf nxt, DevStats* stats, int lds_stack,
                             int quant, int shadow, uint32_t grid, uint32_t refill, uint32_t* queue,
-                            hipStream_t st) {
+                            hipStream_t st, const FrameArgs* eye) {
   if (!ro || !rd || !count || !queue || grid == 0 || refill < 1 || refill > 64 || !sc.tris ||
       !(quant ? (const void*)sc.nodes4q : (const void*)sc.nodes4))
     return hipErrorInvalidValue;
-  if (shadow ? (!contrib || !pathbuf_ok(nxt)) : !out) return hipErrorInvalidValue;
+  if (shadow ? (!contrib || !pathbuf_ok(nxt) || eye) : !out) return hipErrorInvalidValue;
+  const FrameArgs fa = eye ? *eye : FrameArgs{};
   k_fill_u32<<<1, 1, 0, st>>>(queue, 0u);
-#define CPT_DYN(O, S, W, Qb) k_trace_dyn<O, S, W, Qb><<<grid, kBlock, 0, st>>>(sc, ro, rd, count, queue, out, contrib, nxt, stats, refill)
+#define CPT_DYN(O, S, W, Qb) k_trace_dyn<O, S, W, Qb><<<grid, kBlock, 0, st>>>(sc, ro, rd, count, queue, out, contrib, nxt, stats, refill, fa)
+#define CPT_DYN_1E(S, W) k_trace_dyn<1, S, W, true, true><<<grid, kBlock, 0, st>>>(sc, ro, rd, count, queue, out, contrib, nxt, stats, refill, fa)
 #define CPT_DYN_3Q(S, W) CPT_DYN(3, S, W, true)
 #define CPT_DYN_3F(S, W) CPT_DYN(3, S, W, false)
 #define CPT_DYN_1Q(S, W) CPT_DYN(1, S, W, true)
 #define CPT_DYN_1F(S, W) CPT_DYN(1, S, W, false)
   if (shadow) {
     if (quant) { CPT_VARIANT_SWITCH(lds_stack, CPT_DYN_3Q) } else { CPT_VARIANT_SWITCH(lds_stack, CPT_DYN_3F) }
+  } else if (eye) {
+    if (!quant) return hipErrorInvalidValue;
+    CPT_VARIANT_SWITCH(lds_stack, CPT_DYN_1E)
   } else {
     if (quant) { CPT_VARIANT_SWITCH(lds_stack, CPT_DYN_1Q) } else { CPT_VARIANT_SWITCH(lds_stack, CPT_DYN_1F) }
   }
@@ -1297,6 +1318,7 @@ hipError_t launch_trace_dyn(const DevScene& sc, const float4* ro, const float4* 
 #undef CPT_DYN_3F
 #undef CPT_DYN_1Q
 #undef CPT_DYN_1F
+#undef CPT_DYN_1E
 #undef CPT_DYN
   return hipGetLastError();
 }
@@ -1492,10 +1514,12 @@ __device__ __forceinline__ void shade_path(const DevScene& sc, const FrameArgs& 
 // Workgroup of the shading pass: its compaction takes one atomic per counter
 // per workgroup.
 constexpr int kShadeBlock = 256;
-__global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene sc, FrameArgs fa, PathBuf cur, PathBuf nxt,
+// Five waves per SIMD: 96 VGPRs without spills; left to itself the compiler
+// takes 97 and four waves, and the pass is then 5-7 % slower.
+__global__ __launch_bounds__(kShadeBlock, 5) void k_shade(DevScene sc, FrameArgs fa, PathBuf cur, PathBuf nxt,
                                                   const float4* __restrict__ hits, FrameCounters* __restrict__ fc,
                                                   int bounce, ShadowBuf sb, const LightVert* __restrict__ lv,
-                                                  Accum ac) {
+                                                  Accum ac, int eye) {
   const uint32_t i = blockIdx.x * kShadeBlock + threadIdx.x;
   const uint32_t count = fc->active[bounce];
   if (blockIdx.x * kShadeBlock >= count) return;               // whole workgroup idle
@@ -1504,11 +1528,17 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene sc, FrameArgs fa
   ShadowRay sr;
   sr.o = sr.d = sr.c = make_float4(0.f, 0.f, 0.f, 0.f);
   if (i < count) {
-    p.ro = cur.ray_o[i];
-    p.rd = cur.ray_d[i];
-    p.pix = cur.pix[i];
-    p.sid = cur.sid[i];
-    const uint32_t dk = cur.depth[i];
+    uint32_t dk;
+    if (eye) {
+      p = eye_slot(fa, i, dk);
+    } else {
+      p.ro = cur.ray_o[i];
+      p.rd = cur.ray_d[i];
+      p.pix = cur.pix[i];
+      p.sid = cur.sid[i];
+      dk = cur.depth[i];
+      p.rng = cur.rng[i];
+    }
     unpack_dk(p, dk);
     if (dk & kFreshPath) {
       p.mul = make_float4(1.f, 1.f, 1.f, 0.f);
@@ -1519,7 +1549,6 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene sc, FrameArgs fa
       p.smp = cur.smp[i];
       p.acc = cur.acc[i];
     }
-    p.rng = cur.rng[i];
     shade_path(sc, fa, fc, lv, ac, p, hits[i], survive, emit, sr);
   }
   uint32_t slot = 0, sslot = 0;
@@ -1545,7 +1574,7 @@ __global__ __launch_bounds__(kShadeBlock) void k_shade(DevScene sc, FrameArgs fa
 }
 hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, PathBuf nxt, const float4* hits,
                         FrameCounters* fc, int bounce, uint32_t max_n, ShadowBuf sb, const LightVert* lv,
-                        const Accum& ac, hipStream_t st) {
+                        const Accum& ac, hipStream_t st, int eye) {
   if (max_n == 0) return hipSuccess;
   if (!pathbuf_ok(cur) || !pathbuf_ok(nxt) || !hits || !fc || !lv || !sb.ray_o || !sb.ray_d || !sb.contrib ||
       !ac.result || !ac.sample_n)
@@ -1554,7 +1583,7 @@ hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, Pa
   if (ac.apply == 2 && !ac.rec.acc) return hipErrorInvalidValue;
   if ((ac.variant & kVariantGbuf) && !rec_ok(ac.rec)) return hipErrorInvalidValue;
   k_shade<<<(max_n + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, st>>>(sc, fa, cur, nxt, hits, fc, bounce, sb, lv,
-                                                                         ac);
+                                                                         ac, eye);
   return hipGetLastError();
 }
 
