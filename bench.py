@@ -52,16 +52,19 @@ WIDTH, HEIGHT, SPP, EYE_DEPTH, LIGHT_DEPTH = 1280, 720, 64, 8, 8
 
 def extend_kernel_name(args, demangled=False) -> str:
     """Template instance the renderer launches for extension rays (demangled:
-    the prefix of the name the profiler reports)."""
+    the prefix of the name the profiler reports; with refilling waves it covers
+    both instances, the first bounce's, which computes its eye rays, and the
+    later bounces')."""
     stk, minw = {806: (8, 6), 807: (8, 7), 808: (8, 8), 1606: (16, 6), 8: (8, 0), 24: (24, 0)}.get(args.lds_stack,
                                                                                                  (16, 0))
     q = "true" if args.node_format else "false"
     layout = 2 if args.node_format else 0
     if demangled:
-        return (f"k_trace_dyn<1, {stk}, {minw}, {q}>" if args.refill
+        return (f"k_trace_dyn<1, {stk}, {minw}, {q}," if args.refill
                 else f"k_trace<{layout}, false, 1, {stk}, {minw}>")
     if args.refill:
-        return f"k_trace_dyn<1,{stk},{minw},{q}> (closest-hit extension, refill at {args.refill} idle lanes)"
+        return (f"k_trace_dyn<1,{stk},{minw},{q},EYE> (closest-hit extension, refill at {args.refill} idle lanes; "
+                f"EYE at the first bounce: eye rays computed in the kernel)")
     return f"k_trace<{layout},false,1,{stk},{minw}> (closest-hit extension, one thread per ray)"
 
 def log(*a):
@@ -432,10 +435,11 @@ def main():
         "time_to_equal_mse": None,
     }
     del R
-    peak = copy_peak_gbs(ca)
-    log(f"device-to-device copy: {peak:.0f} GB/s")
-    res["roofline"]["copy_peak_gbs"] = round(peak, 1)
-    res["roofline"]["frac_of_copy_peak"] = round(achieved / peak, 4)
+    if not args.copy_calib:      # a counter pass scales by its one calibration copy: no other large copies
+        peak = copy_peak_gbs(ca)
+        log(f"device-to-device copy: {peak:.0f} GB/s")
+        res["roofline"]["copy_peak_gbs"] = round(peak, 1)
+        res["roofline"]["frac_of_copy_peak"] = round(achieved / peak, 4)
     res["baselines"] = None
     if args.baseline_steps > 0:
         res["baselines"] = baselines_leg(ca, S, sc, args)
