@@ -167,11 +167,29 @@ def learned_leg(ca, S, args):
         "note": "wall time of train kernel + gradient-sum/Adam kernel per step"}
 
 
+def counter_per_launch(path, ext_name, calib):
+    """(mean bytes per extension dispatch, scale) from one `rocprofv3 --pmc
+    <counter> --output-format csv` file (values in KiB).  Dispatches whose name
+    contains ext_name are averaged; the scale is calib / what the counter read
+    for the largest copy, which must be the calibration copy of calib bytes
+    (the counter runs make no larger one).  None if either is missing."""
+    import csv
+    ext, copy = [], []
+    for row in csv.DictReader(open(path)):
+        v = float(row["Counter_Value"]) * 1024.0
+        if ext_name in row["Kernel_Name"]:
+            ext.append(v)
+        elif "copyBuffer" in row["Kernel_Name"]:
+            copy.append(v)
+    if not ext or not copy or max(copy) <= 0:
+        return None
+    return sum(ext) / len(ext), calib / max(copy)
+
+
 def hbm_traffic(args):
     """HBM bytes per extension launch from two rocprofv3 --pmc passes (FETCH_SIZE,
     then WRITE_SIZE, KiB per dispatch) over one step of the same workload, each
     scaled by what the counter reads for a 256 MiB device-to-device copy."""
-    import csv
     import shutil
     import subprocess
     import tempfile
@@ -199,18 +217,11 @@ def hbm_traffic(args):
         if r.returncode != 0 or not files:
             log(f"traffic pass {counter} failed (rc {r.returncode}); traffic reported as null")
             return None
-        ext, copy = [], []
-        for row in csv.DictReader(open(files[0])):
-            v = float(row["Counter_Value"]) * 1024.0
-            if ext_name in row["Kernel_Name"]:
-                ext.append(v)
-            elif "copyBuffer" in row["Kernel_Name"]:
-                copy.append(v)
+        got = counter_per_launch(files[0], ext_name, calib)
         shutil.rmtree(d, ignore_errors=True)
-        if not ext or not copy:
+        if got is None:
             return None
-        factor = calib / max(copy)
-        out[counter] = (sum(ext) / len(ext), factor)
+        out[counter] = got
     fetch, ffac = out["FETCH_SIZE"]
     write, wfac = out["WRITE_SIZE"]
     return {"bytes_per_launch": int(fetch * ffac + write * wfac), "fetch_bytes": int(fetch * ffac),

@@ -147,7 +147,10 @@ def test_quantized_node_format_same_image(cpt, cornell_runs, variant):
                                                          (808, 1, 64), (24, 0, 32)])
 def test_refill_traversal_same_image(cpt, cornell_runs, variant, node_format, refill):
     """Lanes that refill themselves from the ray queue trace the same rays with
-    the same per-node operations: frames bit-identical to one ray per thread."""
+    the same per-node operations: frames bit-identical to one ray per thread.
+    With quantized nodes the refilling side's first bounce also computes each
+    slot's eye ray and path state in the extension and shading kernels, where
+    the other side reads what k_gen_eye stored."""
     sc, S = cornell_runs["sc"], cornell_runs["S"]
     a, b = cpt.Renderer(64, 48), cpt.Renderer(64, 48)
     for R in (a, b):
