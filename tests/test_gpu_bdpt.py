@@ -173,6 +173,33 @@ def test_refill_traversal_same_image(cpt, cornell_runs, variant, node_format, re
     assert sa["shadow_nodes"] == sb["shadow_nodes"] > 0 and sa["shadow_tris"] == sb["shadow_tris"]
 
 
+@pytest.mark.parametrize("mode", [0, 4, 5])
+def test_computed_eye_rays_same_image(cpt, cornell_runs, mode):
+    """In every stream mode (BDPT, path tracing, path tracing with restarts),
+    a first bounce on refilling waves, which computes each slot's eye ray and
+    path state from the slot index, gives the frames of a first bounce at one
+    ray per thread, which reads what k_gen_eye stored.  Three frames per
+    wavefront over six frames: the slot's frame offset and the batch's first
+    frame both enter the eye path.  Both sides call the same eye_slot; the
+    oracle comparisons above pin it through k_gen_eye."""
+    sc, S = cornell_runs["sc"], cornell_runs["S"]
+    a, b = cpt.Renderer(64, 48, mode=mode), cpt.Renderer(64, 48, mode=mode)
+    for R in (a, b):
+        R.set_attrib("Node Format", 1)
+        R.set_attrib("Refill Min Paths", 0)
+        R.set_attrib("Frames In Flight", 3)
+    a.set_attrib("Traversal Refill", 0)
+    b.set_attrib("Traversal Refill", 64)
+    ra = a.render(S, sc["camera"], frames=6)
+    rb = b.render(S, sc["camera"], frames=6)
+    assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
+    assert np.array_equal(a.sample_counts(), b.sample_counts())
+    sa, sb = a.stats(), b.stats()
+    assert sa["extension_rays"] == sb["extension_rays"] > 64 * 48 * 6
+    assert sa["extend_nodes"] == sb["extend_nodes"]
+    assert ra.any()
+
+
 
 @pytest.mark.parametrize("attr", ["Traversal LDS Stack", "Shadow LDS Stack"])
 def test_unknown_traversal_variant_is_an_error(cpt, cornell_runs, attr):
