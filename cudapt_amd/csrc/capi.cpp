@@ -649,11 +649,16 @@ struct cpt_renderer {
   bool overlap_shadow = true;
   hipStream_t side = nullptr;
   hipEvent_t ev_shaded = nullptr, ev_shadow = nullptr;
+  // the light pass of a plain batch also runs on `side`, beside the first
+  // extension (which reads no light vertex); the first shading waits for it
+  hipEvent_t ev_light = nullptr;
+  bool light_pending = false;
   ~cpt_renderer() {
     for (auto& e : events) { pool.push_back(e.a); pool.push_back(e.b); }
     for (auto e : pool) (void)hipEventDestroy(e);
     if (ev_shaded) (void)hipEventDestroy(ev_shaded);
     if (ev_shadow) (void)hipEventDestroy(ev_shadow);
+    if (ev_light) (void)hipEventDestroy(ev_light);
     if (side) (void)hipStreamDestroy(side);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -887,6 +892,10 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
       HIP_TRY(hipStreamWaitEvent(r->stream, r->ev_shadow, 0));
       shadow_pending = false;
     }
+    if (r->light_pending) {
+      HIP_TRY(hipStreamWaitEvent(r->stream, r->ev_light, 0));
+      r->light_pending = false;
+    }
     HIP_TRY(timed_begin(r, 2, &ev));
     HIP_TRY(launch_shade(sc, fa, cur->view(), nxt->view(), r->hits.p, r->fc.p, (int)b, max_n, sb, r->lv.p, ac,
                          r->stream, eye0 && b == 0));
@@ -913,6 +922,10 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
     std::swap(cur, nxt);
   }
   if (shadow_pending) HIP_TRY(hipStreamWaitEvent(r->stream, r->ev_shadow, 0));
+  if (r->light_pending) {                        // no bounce shaded: join the light pass here
+    HIP_TRY(hipStreamWaitEvent(r->stream, r->ev_light, 0));
+    r->light_pending = false;
+  }
   return CPT_OK;
 }
 
@@ -1087,10 +1100,27 @@ int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   LightJobs jobs;
   std::memset(&jobs, 0, sizeof jobs);
   if (!pt) jobs = light_jobs(r, fa.frame0, fa.nframes);   // the path tracers have no light pass
+  const bool mega = !pt && r->attr(kMegakernel) != 0;
   if (jobs.n) {
-    HIP_TRY(timed_begin(r, 3, &ev));
-    HIP_TRY(launch_light_pass(sc, fa, jobs, r->lv.p, r->fc.p, r->stream));
-    HIP_TRY(timed_end(r, 3, ev));
+    // The first extension reads no light vertex: with the side stream the
+    // pass runs there, after everything queued so far (earlier shading reads
+    // the cache slots it rebuilds), and the first shading waits for it.
+    // It writes only the cache and FrameCounters::light_n.
+    hipStream_t ls = r->stream;
+    if (r->overlap_shadow && !mega) {
+      if (!r->side) HIP_TRY(hipStreamCreateWithFlags(&r->side, hipStreamNonBlocking));
+      if (!r->ev_light) HIP_TRY(hipEventCreateWithFlags(&r->ev_light, hipEventDisableTiming));
+      HIP_TRY(hipEventRecord(r->ev_light, r->stream));
+      HIP_TRY(hipStreamWaitEvent(r->side, r->ev_light, 0));
+      ls = r->side;
+    }
+    HIP_TRY(timed_begin(r, 3, &ev, ls));
+    HIP_TRY(launch_light_pass(sc, fa, jobs, r->lv.p, r->fc.p, ls));
+    HIP_TRY(timed_end(r, 3, ev, ls));
+    if (ls != r->stream) {
+      HIP_TRY(hipEventRecord(r->ev_light, ls));
+      r->light_pending = true;
+    }
   }
   HIP_TRY(launch_frame_begin(r->fc.p, n, r->stream));
   Accum ac;
@@ -1105,7 +1135,7 @@ int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
     ac.term_rng = r->term_rng.p;
     ac.tend = r->tend.p;
   }
-  if (!pt && r->attr(kMegakernel) != 0) {
+  if (mega) {
     r->extend_launches++;
     HIP_TRY(timed_begin(r, 0, &ev));
     HIP_TRY(launch_mega(sc, fa, r->fc.p, r->lv.p, ac, (int)r->attr(kLdsStack), r->attr(kNodeFormat) != 0,
