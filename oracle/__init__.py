@@ -73,6 +73,17 @@ def lib():
                                             ctypes.c_float, ctypes.c_float, c32, f32p]
         L.orc_expf.restype = ctypes.c_float
         L.orc_expf.argtypes = [ctypes.c_float]
+        d, i32p = ctypes.c_double, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+        ci = ctypes.c_int
+        L.orc_d_ggx.restype = d
+        L.orc_d_ggx.argtypes = [ci, d, d]
+        L.orc_vis_smith.restype = d
+        L.orc_vis_smith.argtypes = [ci, d, d, d]
+        L.orc_f_schlick.argtypes = [ci, f64p, d, f64p]
+        L.orc_sample_ggx.argtypes = [ci, d, d, d, f64p, f64p]
+        L.orc_basis.argtypes = [ci, f64p, f64p, f64p]
+        L.orc_shading.argtypes = [ci, f64p, f64p]
+        L.orc_sample_bounce_many.argtypes = [ci, f64p, f64p, d, f64p, f64p, c64, c32, ci, i32p, f64p, f64p, f64p]
         L.orc_tree_sum.restype = ctypes.c_double
         L.orc_tree_sum.argtypes = [f32p, c32]
         _lib = L
@@ -247,6 +258,64 @@ class Renderer:
         out = np.zeros((2048, 21), np.float32)
         n = lib().orc_light_vertices(self.h, out.ravel(), 2048)
         return out[:n]
+
+
+def _v(x, n=3):
+    a = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1))
+    assert a.size == n
+    return a
+
+
+def d_ggx(rough, noh, prec=64):
+    """D_GGX (cudaRTCommon.cu:309-315)."""
+    return lib().orc_d_ggx(prec, float(rough), float(noh))
+
+
+def vis_smith(rough, nov, nol, prec=64):
+    """Vis_SmithJointApprox (cudaRTCommon.cu:299-307)."""
+    return lib().orc_vis_smith(prec, float(rough), float(nov), float(nol))
+
+
+def f_schlick(f0, voh, prec=64):
+    """F_Schlick (cudaRTCommon.cu:317-322)."""
+    out = np.zeros(3)
+    lib().orc_f_schlick(prec, _v(f0), float(voh), out)
+    return out
+
+
+def sample_ggx(xi0, xi1, rough, n, prec=64):
+    """ImportanceSampleGGX (cudaRTCommon.cu:324-345): the half vector."""
+    out = np.zeros(3)
+    lib().orc_sample_ggx(prec, float(xi0), float(xi1), float(rough), _v(n), out)
+    return out
+
+
+def basis(w, prec=64):
+    """The tangent frame of the samplers ((|w.x| > 0.1 ? y : x) x w)."""
+    u, v = np.zeros(3), np.zeros(3)
+    lib().orc_basis(prec, _v(w), u, v)
+    return u, v
+
+
+def shading(out_dir, in_dir, irrad, n, diff, metallic, rough, spec, dw, sw, prec=64):
+    """GetShadingResult (cudaRTBDPTStream.cu:642-663)."""
+    a = np.concatenate([_v(out_dir), _v(in_dir), _v(irrad), _v(n), _v(diff),
+                        np.array([metallic, rough, spec, dw, sw], np.float64)])
+    out = np.zeros(3)
+    lib().orc_shading(prec, a, out)
+    return out
+
+
+def sample_bounce_many(ro, rd, t, norm, diff, trans, specular, metallic, roughness, nl, seed, n, prec=64,
+                       refl_offset=False):
+    """n eye-bounce BRDF samples at one hit (sample i from its own PCG32 stream):
+    (type, orig, dir, mul) arrays; type 2 = specular branch, 1 = diffuse."""
+    hs = np.concatenate([_v(norm), _v(diff), np.array([trans, specular, metallic, roughness], np.float64)])
+    ty = np.zeros(n, np.int32)
+    o, d, m = (np.zeros((n, 3)) for _ in range(3))
+    lib().orc_sample_bounce_many(prec, _v(ro), _v(rd), float(t), hs, _v(nl), int(seed), int(n), int(refl_offset),
+                                 ty, o.reshape(-1), d.reshape(-1), m.reshape(-1))
+    return ty, o, d, m
 
 
 def sample_bounce(ro, rd, t, norm, diff, trans, specular, metallic, roughness, nl, state, seq, tir=False):

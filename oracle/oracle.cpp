@@ -31,6 +31,25 @@
 //                      Pinned by the emitter-only and all-black scenes and the
 //                      hand-checked single-connection case; the FP32 GPU run is
 //                      held to relative MSE 1e-4 of the FP64 run of this code.
+//   d_ggx, vis_smith,  D_GGX, Vis_SmithJointApprox, F_Schlick, ImportanceSampleGGX
+//   f_schlick,         (src/cudaRTCommon.cu:284-345) and GetShadingResult
+//   sample_ggx,        (src/cudaRTBDPTStream.cu:642-663) -- pinned by GGX
+//   shading            normalisation quadrature, the normal-incidence value
+//                      1/(pi r^4), a KS test of the sampler against the
+//                      quadrature CDF of D (n.h), Schlick and Smith hand values,
+//                      a Lambert white furnace and the specular lobe's missing
+//                      1/(4 NoL NoV) by hand (tests/test_oracle_shading.py).
+//   sample_bounce      the eye/light bounce's BRDF sampling
+//                      (src/cudaRTBDPTStream.cu:395-502) -- pinned by a KS test
+//                      of the r2sin = 1 - r2cos^2 hemisphere against its closed
+//                      CDF, branch frequencies lerp(|F|, 1, metallic) and
+//                      transparency, and each branch's weight in closed form.
+//   eye_path RR        Russian roulette u > |T| |f| without reweighting
+//                      (:515-520) -- pinned by the mean path length in a closed
+//                      grey box.
+//   light_pass         (:180-363,545-617) -- pinned by a hand-worked two-bounce
+//                      subpath: cache slot order, irrad zeroed before a
+//                      non-diffuse bounce (:331-332), the sample update.
 //   bilateral_posnorm  cudaBilateralFilter::updateGaussian and
 //                      bilaterial_posnorm(emit)_kernel
 //                      (src/cudaBilateralFilter.cu:5-142), including the
@@ -1393,6 +1412,24 @@ static void render_adaptive(const Scene& s, const RenderParams& p, u32 frame0, u
 // ====================================================================== C ABI
 using namespace orc;
 
+template <class R> static V3<R> v3d(const double* p) { return mk((R)p[0], (R)p[1], (R)p[2]); }
+template <class R> static void put3(V3<R> a, double* o) { o[0] = (double)a.x; o[1] = (double)a.y; o[2] = (double)a.z; }
+template <class R>
+static void bounce_many(const double* ro, const double* rd, double t, const double* hs, const double* nl, u64 seed,
+                        u32 n, int refl_offset, i32* type, double* orig, double* dir, double* mul) {
+  HitShade<R> h;
+  h.norm = v3d<R>(hs); h.pos = mk((R)0, (R)0, (R)0); h.diff = v3d<R>(hs + 3); h.emissive = mk((R)0, (R)0, (R)0);
+  h.trans = (R)hs[6]; h.specular = (R)hs[7]; h.metallic = (R)hs[8]; h.roughness = (R)hs[9];
+  for (u32 i = 0; i < n; i++) {
+    // independent streams: PCG32 streams that share an initial state and differ only in
+    // the increment give related first draws, so each sample gets its own hashed state
+    Pcg32 rng; pcg32_seed(rng, splitmix64_mix(seed + (u64)i * 0x9E3779B97F4A7C15ull), i);
+    Bounce<R> b = sample_bounce<R>(rng, v3d<R>(ro), v3d<R>(rd), (R)t, h, v3d<R>(nl), RT_EYE, refl_offset != 0);
+    type[i] = b.type;
+    put3(b.orig, orig + (size_t)i * 3); put3(b.dir, dir + (size_t)i * 3); put3(b.mul, mul + (size_t)i * 3);
+  }
+}
+
 extern "C" {
 
 u32 orc_pcg32_stream(u64 initstate, u64 initseq, u32 n, u32* out) {
@@ -1474,6 +1511,48 @@ int orc_sample_bounce(const float* ro, const float* rd, float t, const float* hs
   out[3] = b.dir.x; out[4] = b.dir.y; out[5] = b.dir.z;
   out[6] = b.mul.x; out[7] = b.mul.y; out[8] = b.mul.z;
   return b.type;
+}
+
+// Shading building blocks one at a time (cudaRTCommon.cu:284-345,
+// cudaRTBDPTStream.cu:642-663), in FP32 (prec 32) or FP64, for the CPU tests
+// that pin them by quadrature, closed forms and sampling statistics.
+
+double orc_d_ggx(int prec, double rough, double noh) {
+  return prec == 64 ? d_ggx<double>(rough, noh) : (double)d_ggx<float>((float)rough, (float)noh);
+}
+double orc_vis_smith(int prec, double rough, double nov, double nol) {
+  return prec == 64 ? vis_smith<double>(rough, nov, nol) : (double)vis_smith<float>((float)rough, (float)nov, (float)nol);
+}
+void orc_f_schlick(int prec, const double* f0, double voh, double* out) {
+  if (prec == 64) put3(f_schlick<double>(v3d<double>(f0), voh), out);
+  else put3(f_schlick<float>(v3d<float>(f0), (float)voh), out);
+}
+void orc_sample_ggx(int prec, double xi0, double xi1, double rough, const double* n, double* out) {
+  if (prec == 64) put3(sample_ggx<double>(xi0, xi1, rough, v3d<double>(n)), out);
+  else put3(sample_ggx<float>((float)xi0, (float)xi1, (float)rough, v3d<float>(n)), out);
+}
+void orc_basis(int prec, const double* w, double* u, double* v) {
+  if (prec == 64) { V3<double> a, b; basis(v3d<double>(w), a, b); put3(a, u); put3(b, v); }
+  else { V3<float> a, b; basis(v3d<float>(w), a, b); put3(a, u); put3(b, v); }
+}
+// a = outDir3 inDir3 irrad3 n3 diff3 metallic roughness specular diffW specW
+void orc_shading(int prec, const double* a, double* out) {
+  if (prec == 64)
+    put3(shading<double>(v3d<double>(a), v3d<double>(a + 3), v3d<double>(a + 6), v3d<double>(a + 9),
+                         v3d<double>(a + 12), a[15], a[16], a[17], a[18], a[19]), out);
+  else
+    put3(shading<float>(v3d<float>(a), v3d<float>(a + 3), v3d<float>(a + 6), v3d<float>(a + 9), v3d<float>(a + 12),
+                        (float)a[15], (float)a[16], (float)a[17], (float)a[18], (float)a[19]), out);
+}
+// n independent BRDF samples of the eye bounce at one hit, sample i drawing
+// from PCG32 stream i with the state hashed from (seed, i).  hs = norm3 diff3 trans specular metallic
+// roughness.  Per sample: the branch (RT_SPEC 2 / RT_DIFF 1), the next ray's
+// origin and direction, and its throughput weight.
+void orc_sample_bounce_many(int prec, const double* ro, const double* rd, double t, const double* hs,
+                            const double* nl, u64 seed, u32 n, int refl_offset, i32* type, double* orig,
+                            double* dir, double* mul) {
+  if (prec == 64) bounce_many<double>(ro, rd, t, hs, nl, seed, n, refl_offset, type, orig, dir, mul);
+  else bounce_many<float>(ro, rd, t, hs, nl, seed, n, refl_offset, type, orig, dir, mul);
 }
 
 void orc_tex_sample(void* h, u32 id, double u, double v, int bits, double* out) {
