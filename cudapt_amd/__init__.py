@@ -24,7 +24,7 @@ CPT_MODE_BDPT_ADAPTIVE_EMP = 3
 CPT_MODE_PT_STREAM = 4
 CPT_MODE_PT_STREAM_REGEN = 5
 (CPT_BUF_RESULT, CPT_BUF_SAMPLE_N, CPT_BUF_GBUF_POS, CPT_BUF_GBUF_NORM, CPT_BUF_GBUF_DIFF, CPT_BUF_FILTERED,
- CPT_BUF_ERROR, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_ALLOC_ERROR, CPT_BUF_SECOND_MOMENT) = range(10)
+ CPT_BUF_ERROR, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_ALLOC_ERROR, CPT_BUF_SECOND_MOMENT, CPT_BUF_PIXEL_PATHS) = range(11)
 MLP_INPUTS, MLP_HIDDEN, MLP_PARAMS = 9, 64, 4865
 
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("max_dist", "<f4"), ("d", "<f4", 3), ("eps", "<f4")])
@@ -332,7 +332,9 @@ class Renderer:
                "nd": "Desired Max Sampling", "sigma_color": "Filter Color EuD", "sigma_pos": "Filter Pos EuD",
                "sigma_norm": "Filter Norm EuD", "sigma_diff": "Filter Diff EuD", "radius": "Filter Radius",
                "filter_diff": "Filter Diffuse Flag", "in_flight": "Frames In Flight",
-               "pass_frames": "Adaptive Pass Frames", "megakernel": "Megakernel"}.get(name, name)
+               "pass_frames": "Adaptive Pass Frames", "megakernel": "Megakernel",
+               "sigma_spatial": "Filter Spatial EuD", "sigma_depth": "Filter Depth EuD",
+               "min_paths": "Min Pixel Paths", "max_paths": "Max Pixel Paths"}.get(name, name)
         idx = self.attribs()[key][0]
         _check(lib().cpt_renderer_set_attrib(self.h, idx, float(value)), "cpt_renderer_set_attrib")
 
@@ -353,7 +355,7 @@ class Renderer:
 
     def read(self, which: int):
         """A renderer buffer: CPT_BUF_* (result, counts, G-buffer, filtered, error, per-frame sample counts)."""
-        if which in (CPT_BUF_SAMPLE_N, CPT_BUF_SAMPLE_COUNTS):
+        if which in (CPT_BUF_SAMPLE_N, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_PIXEL_PATHS):
             out = np.zeros((self.H, self.W), np.uint32)
         elif which in (CPT_BUF_ERROR, CPT_BUF_ALLOC_ERROR):
             out = np.zeros((self.H, self.W), np.float32)
@@ -565,8 +567,8 @@ def filter_cross_bilateral(color, pos, norm, diff, sample_n, sigma_color, sigma_
     df = DeviceBuffer(W * H * 12)
     de = DeviceBuffer(W * H * 4)
     vp = ctypes.c_void_p
-    _check(lib().cpt_filter_cross_bilateral_device(vp(bufs[0].ptr), vp(bufs[1].ptr), vp(bufs[2].ptr),
-                                                   vp(db.ptr) if db else None, W, H, sigma_color, sigma_pos,
-                                                   sigma_norm, sigma_diff, radius, vp(dn.ptr), nd, vp(df.ptr),
-                                                   vp(de.ptr), None), "cpt_filter_cross_bilateral_device")
+    name = "cpt_filter_cross_bilateral_device"
+    _check(getattr(lib(), name)(vp(bufs[0].ptr), vp(bufs[1].ptr), vp(bufs[2].ptr), vp(db.ptr) if db else None, W, H,
+                                sigma_color, sigma_pos, sigma_norm, sigma_diff, radius, vp(dn.ptr), nd, vp(df.ptr),
+                                vp(de.ptr), None), name)
     return df.download(np.float32, W * H * 3).reshape(H, W, 3), de.download(np.float32, W * H).reshape(H, W)

@@ -164,6 +164,30 @@ int upload_materials(cpt_scene* s) {
   return CPT_OK;
 }
 
+// cudaBilateralFilter::updateGaussian (src/cudaBilateralFilter.cu:5-24): the
+// spatial table of 2 * radius + 1 taps, sigma = the colour sigma as the
+// reference's callers pass it, zero beyond.
+constexpr int kFilterTable = 2 * kMaxFilterRadius + 2;
+void filter_table(float sigma_color, u32 radius, float g[kFilterTable]) {
+  for (int i = 0; i < kFilterTable; i++) g[i] = 0.f;
+  for (u32 i = 0; i < 2 * radius + 1; i++) {
+    int x = (int)i - (int)radius;
+    g[i] = std::exp(-(float)(x * x) / (2 * sigma_color * sigma_color));
+  }
+}
+
+FilterArgs filter_args(u32 width, u32 height, float sc, float sp, float sn, float sd, u32 radius, u32 nd,
+                       bool use_diff) {
+  FilterArgs f;
+  f.width = width; f.height = height; f.radius = radius; f.nd = nd;
+  f.kc = 2.f * sc * sc; f.kp = 2.f * sp * sp;
+  f.kn = 2.f * sn * sn; f.kd = 2.f * sd * sd;
+  f.use_diff = use_diff;
+  f.kz = 0.f;
+  f.cam[0] = f.cam[1] = f.cam[2] = 0.f;
+  return f;
+}
+
 }  // namespace
 
 extern "C" {
@@ -476,19 +500,13 @@ int cpt_filter_cross_bilateral_device(const float* d_color, const float* d_pos, 
   if (width == 0 || height == 0) return fail(CPT_E_ARG, "cpt_filter_cross_bilateral_device: empty image");
   if (radius > (u32)kMaxFilterRadius) return fail(CPT_E_LIMIT, "cpt_filter_cross_bilateral_device: radius %u > 31", radius);
   hipStream_t st = (hipStream_t)stream;
-  float g[64] = {0};
-  for (u32 i = 0; i < 2 * radius + 1; i++) {
-    int x = (int)i - (int)radius;
-    g[i] = std::exp(-(float)(x * x) / (2 * sigma_color * sigma_color));
-  }
+  float g[kFilterTable];
+  filter_table(sigma_color, radius, g);
   float* d_g = nullptr;
   HIP_TRY(hipMalloc((void**)&d_g, sizeof g));
   HIP_TRY(hipMemcpyAsync(d_g, g, sizeof g, hipMemcpyHostToDevice, st));
-  FilterArgs f;
-  f.width = width; f.height = height; f.radius = radius; f.nd = nd;
-  f.kc = 2.f * sigma_color * sigma_color; f.kp = 2.f * sigma_pos * sigma_pos;
-  f.kn = 2.f * sigma_norm * sigma_norm; f.kd = 2.f * sigma_diff * sigma_diff;
-  f.use_diff = d_diff != nullptr;
+  const FilterArgs f = filter_args(width, height, sigma_color, sigma_pos, sigma_norm, sigma_diff, radius, nd,
+                                   d_diff != nullptr);
   hipError_t e = launch_filter_error(f, d_color, d_pos, d_norm, d_diff, d_g, d_sample_n, d_filtered, d_err, st);
   hipError_t e2 = hipStreamSynchronize(st);
   (void)hipFree(d_g);
@@ -555,6 +573,18 @@ const char* const kLightPaths = "Light Paths";
 const char* const kSeed = "Seed";
 const char* const kLiteralShadow = "Literal Shadow Endpoint";
 const char* const kWarmup = "Warm-up Frames";
+// the adaptive files' output selection (AdapBilatAtomic.cu:30-31): which image a Render call returns
+const char* const kDebugMode = "Debug Mode";
+// Filter and allocation extensions, each off at 0 (the reference's behaviour):
+// the spatial sigma of the cross-bilateral filter (0 = the colour sigma, as the
+// reference passes it); a relative-depth term (z_q - z_p) / z_p with this sigma,
+// z the first-hit distance from the camera; the fewest paths a pixel gets (the
+// warm-up frames are raised to it); the most paths a pixel gets (a pixel that has
+// them is given no more, and a pass's proportional count stops at the cap)
+const char* const kFilterSpatial = "Filter Spatial EuD";
+const char* const kFilterDepth = "Filter Depth EuD";
+const char* const kMinPaths = "Min Pixel Paths";
+const char* const kMaxPaths = "Max Pixel Paths";
 const char* const kInFlight = "Frames In Flight";
 // adaptive mode: frames' worth of samples (k * W * H) one adaptive pass
 // allocates from one error map and traces as one wavefront (1 = the
@@ -678,6 +708,7 @@ struct cpt_renderer {
   // adaptive mode
   DevArray<float> gpos, gnrm, gdif, filtered, err, gauss, err_sum;
   DevArray<u32> counts, offsets, block_tot, total, big_list, big_n;
+  DevArray<u32> paths;                  // eye paths traced per pixel since the last restart
   DevArray<float> sq, maxe;                  // second moment (W*H*3), max of the error map
   DevArray<u32> flags, rank, kept;           // round-based allocation
   DevArray<RoundState> rounds;
@@ -735,6 +766,7 @@ int renderer_alloc(cpt_renderer* r) {
     HIP_TRY(r->gauss.alloc(64));
     HIP_TRY(r->err_sum.alloc(1));
     HIP_TRY(r->counts.alloc(n));
+    HIP_TRY(r->paths.alloc(n));
     HIP_TRY(r->offsets.alloc(n));
     HIP_TRY(r->block_tot.alloc(n / 1024 + 2));
     HIP_TRY(r->total.alloc(1));
@@ -748,6 +780,7 @@ int renderer_alloc(cpt_renderer* r) {
       HIP_TRY(hipMemsetAsync(a->p, 0, sizeof(float) * n * 3, r->stream));
     HIP_TRY(hipMemsetAsync(r->err.p, 0, sizeof(float) * n, r->stream));
     HIP_TRY(hipMemsetAsync(r->counts.p, 0, sizeof(u32) * n, r->stream));
+    HIP_TRY(hipMemsetAsync(r->paths.p, 0, sizeof(u32) * n, r->stream));
   }
   HIP_TRY(r->rec_acc.alloc(cap));
   if (r->mode == CPT_MODE_PT_STREAM_REGEN) {
@@ -929,6 +962,15 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
   return CPT_OK;
 }
 
+// After a failed enqueue: waits for whatever the side stream still runs (the
+// light pass, a bounce's shadow rays), so no kernel writes the light caches or
+// accumulators after the call has returned, and drops the pending light join.
+void join_side(cpt_renderer* r) {
+  if (r->side) (void)hipStreamSynchronize(r->side);
+  (void)hipStreamSynchronize(r->stream);
+  r->light_pending = false;
+}
+
 // One frame of cudaRTBDPTStreamAdapBilatAtomic::Render (AdapBilatAtomic.cu:1338-1538).
 MlpFeatureArgs feature_args(const cpt_renderer* r) {
   MlpFeatureArgs a;
@@ -946,12 +988,18 @@ int run_filter(cpt_renderer* r, const float* src, float* dst, float* err) {
   float sn_ = (float)r->attr("Filter Norm EuD"), sd_ = (float)r->attr("Filter Diff EuD");
   u32 radius = (u32)r->attr("Filter Radius");
   if (radius > (u32)kMaxFilterRadius) return fail(CPT_E_LIMIT, "render: filter radius %u exceeds 31", radius);
-  int rc = update_gauss(r, sc_, radius);
+  // the spatial table takes the colour sigma as the reference's callers pass it
+  // (AdapBilatAtomic.cu:1327) unless a spatial sigma of its own is set
+  const float ss = (float)r->attr(kFilterSpatial);
+  int rc = update_gauss(r, ss > 0.f ? ss : sc_, radius);
   if (rc) return rc;
-  FilterArgs f;
-  f.width = r->W; f.height = r->H; f.radius = radius; f.nd = (u32)r->attr("Desired Max Sampling");
-  f.kc = 2.f * sc_ * sc_; f.kp = 2.f * sp_ * sp_; f.kn = 2.f * sn_ * sn_; f.kd = 2.f * sd_ * sd_;
-  f.use_diff = r->attr("Filter Diffuse Flag") != 0;
+  FilterArgs f = filter_args(r->W, r->H, sc_, sp_, sn_, sd_, radius, (u32)r->attr("Desired Max Sampling"),
+                             r->attr("Filter Diffuse Flag") != 0);
+  const float sz = (float)r->attr(kFilterDepth);
+  if (sz > 0.f) {
+    f.kz = 2.f * sz * sz;
+    f.cam[0] = r->cam[0]; f.cam[1] = r->cam[1]; f.cam[2] = r->cam[2];
+  }
   HIP_TRY(launch_filter_error(f, src, r->gpos.p, r->gnrm.p, r->gdif.p, r->gauss.p, r->sample_n.p, dst, err,
                               r->stream));
   return CPT_OK;
@@ -988,7 +1036,7 @@ int model_features(cpt_renderer* r, float* d_x) {
 // fill kernel does.
 constexpr u32 kMaxRounds = 1024;
 constexpr u32 kRoundBatch = 8;
-int allocate_rounds(cpt_renderer* r, const FrameArgs& fa, const float* err, u32 gen, u32 nd) {
+int allocate_rounds(cpt_renderer* r, const FrameArgs& fa, const float* err, u32 gen, u32 nd, u32 maxp) {
   const u32 n = r->W * r->H;
   const bool bisect = r->mode == CPT_MODE_BDPT_ADAPTIVE_BINARY;
   const float desired = (float)n / (float)(nd ? nd : 1u);
@@ -1007,8 +1055,15 @@ int allocate_rounds(cpt_renderer* r, const FrameArgs& fa, const float* err, u32 
     if (h.done) break;
   }
   HIP_TRY(launch_round_fill(r->rounds.p, n, r->counts.p, r->stream));
+  HIP_TRY(launch_cap_counts(r->counts.p, r->paths.p, maxp, n, r->stream));
   HIP_TRY(launch_scan_counts(r->counts.p, n, r->offsets.p, r->block_tot.p, r->total.p, r->stream));
   return CPT_OK;
+}
+
+// Plain 1-spp warm-up frames before adaptive sampling starts (AdapBilatAtomic.cu:1372): at least
+// "Warm-up Frames", raised to "Min Pixel Paths" so every pixel has that many paths.
+u32 warmup_frames(const cpt_renderer* r) {
+  return (u32)std::max(r->attr(kWarmup), r->attr(kMinPaths));
 }
 
 int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
@@ -1030,12 +1085,14 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   // AdapBilatAtomic's eye bounce adds the reflection offset; the bisection and
   // empirical files use the plain stream's bounce
   ac.variant = kVariantGbuf | (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL ? kVariantReflOffset : 0);
-  if (fa.frame0 < (u32)r->attr(kWarmup)) {         // plain 1 spp frames with G-buffer (:1372-1406)
+  const u32 maxp = (u32)r->attr(kMaxPaths);
+  if (fa.frame0 < warmup_frames(r)) {              // plain 1 spp frames with G-buffer (:1372-1406)
     ac.apply = 1;
     HIP_TRY(launch_frame_begin(r->fc.p, n, r->stream));
     HIP_TRY(launch_gen_eye_rec(fa, r->A.view(), ac.rec, r->stream));
     int rc = bounce_loop(r, sc, fa, n, ac);
     if (rc) return rc;
+    HIP_TRY(launch_add_paths(r->paths.p, nullptr, n, fa.frame0 == 0, r->stream));
   } else {
     int rc = CPT_OK;
     const u32 nd = (u32)r->attr("Desired Max Sampling");
@@ -1061,18 +1118,21 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
       HIP_TRY(timed_end(r, 9, ev));
       alloc_err = r->mlp_e.p;
     }
+    // pixels that have their "Max Pixel Paths" get no weight in the allocation
+    HIP_TRY(launch_cap_error(const_cast<float*>(alloc_err), r->paths.p, maxp, n, r->stream));
     if (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL) {     // proportional counts (AdapBilatAtomic.cu:817-842)
       HIP_TRY(timed_begin(r, 6, &ev));
       HIP_TRY(launch_error_sum(alloc_err, n, r->block_sums.p, r->err_sum.p, r->stream));
       HIP_TRY(timed_end(r, 6, ev));
       HIP_TRY(timed_begin(r, 7, &ev));
       HIP_TRY(launch_alloc_scan(alloc_err, r->err_sum.p, n, gen, r->counts.p, r->offsets.p, r->block_tot.p,
-                                r->total.p, r->cap, r->stream));
+                                r->total.p, r->cap, r->paths.p, maxp, r->stream));
     } else {                                         // rounds (AdapBilatBinary.cu:1348-1388, AdapEmp.cu:1232-1260)
       HIP_TRY(timed_begin(r, 7, &ev));
-      rc = allocate_rounds(r, fa, alloc_err, gen, nd);
+      rc = allocate_rounds(r, fa, alloc_err, gen, nd, maxp);
       if (rc) return rc;
     }
+    HIP_TRY(launch_add_paths(r->paths.p, r->counts.p, n, 0, r->stream));
     HIP_TRY(launch_frame_begin_dev(r->fc.p, r->total.p, r->cap, r->stream));
     HIP_TRY(launch_gen_adaptive(fa, r->offsets.p, r->counts.p, r->total.p, r->cap, r->A.view(), ac.rec, r->stream));
     HIP_TRY(timed_end(r, 7, ev));
@@ -1162,6 +1222,34 @@ int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   return CPT_OK;
 }
 
+// The image a Render call hands back (src/cudaRTBDPTStreamAdapBilatAtomic.cu:1539-1552 and the same lines of the
+// Binary and Emp files), chosen by the adaptive modes' "Debug Mode": 0 "None" the running mean; 1 "Traced" the
+// paths traced per pixel in the last frame as grey (the reference adds 1 per traced path through its running
+// mean, AdapBilatBinary.cu:1417-1421; here the running mean is left intact so the mode can be switched back);
+// 2 "Prob" max(0, e) and 3 "Prob With Limit" max(Min Trace Probability, e) as grey, e the per-pixel error map
+// (pt_applyPixelProbToResult_kernel, AdapBilatAtomic.cu:944-951); 4 "Filtered Result" the cross-bilateral image.
+int copy_result(cpt_renderer* r, float* result) {
+  const size_t n = (size_t)r->W * r->H;
+  const u32 debug = is_adaptive(r->mode) ? (u32)r->attr(kDebugMode) : 0u;
+  if (debug == 0) {
+    HIP_TRY(hipMemcpyAsync(result, r->result.p, n * 12, hipMemcpyDeviceToHost, r->stream));
+  } else if (debug == 4) {
+    HIP_TRY(hipMemcpyAsync(result, r->filtered.p, n * 12, hipMemcpyDeviceToHost, r->stream));
+  } else {
+    std::vector<float> grey(n);
+    HIP_TRY(hipMemcpyAsync(grey.data(), debug == 1 ? (const void*)r->counts.p : (const void*)r->err.p, n * 4,
+                           hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    const float lo = debug == 3 ? (float)r->attr("Min Trace Probability") : 0.f;
+    for (size_t i = 0; i < n; i++) {
+      float v = debug == 1 ? (float)((const u32*)grey.data())[i] : std::fmax(lo, grey[i]);
+      result[i * 3] = result[i * 3 + 1] = result[i * 3 + 2] = v;
+    }
+  }
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return CPT_OK;
+}
+
 int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* dir, const float* up, float fov,
                   uint32_t nframes, float* result) {
   if (!r || !s || !pos || !dir || !up) return fail(CPT_E_ARG, "render: null argument");
@@ -1176,6 +1264,10 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
     u32 radius = (u32)r->attr("Filter Radius");
     if (radius > (u32)kMaxFilterRadius) return fail(CPT_E_LIMIT, "render: filter radius %u exceeds 31", radius);
     if (r->attr(kWarmup) < 1) return fail(CPT_E_LIMIT, "render: warm-up frames must be >= 1");
+    const double dm = r->attr(kDebugMode);
+    if (dm != std::floor(dm) || dm > 4) return fail(CPT_E_ARG, "render: Debug Mode %g not in 0..4", dm);
+    if (dm == 4 && !uses_filter(r->mode))
+      return fail(CPT_E_ARG, "render: Debug Mode 4 (Filtered Result) needs a filtering mode; the empirical mode has none");
   }
   if (eye_depth < 1 || eye_depth > (u32)kMaxDepth) return fail(CPT_E_LIMIT, "render: eye depth %u not in 1..15", eye_depth);
   if (light_depth < 1 || light_depth > 64) return fail(CPT_E_LIMIT, "render: light depth %u not in 1..64", light_depth);
@@ -1226,19 +1318,21 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   u32 next = (reset || !r->started) ? 0u : r->frame + 1;   // frame counter (:826-828)
   for (u32 done = 0; done < nframes;) {
     u32 k = std::min(in_flight, nframes - done);
-    if (adaptive && next < (u32)r->attr(kWarmup)) k = 1;   // warm-up frames one at a time
+    if (adaptive && next < warmup_frames(r)) k = 1;        // warm-up frames one at a time
     fa.frame0 = next;
     fa.nframes = k;
     rc = render_batch(r, s, fa);
-    if (rc) return rc;
+    if (rc) {
+      join_side(r);
+      return rc;
+    }
     r->frame = next + k - 1;
     r->started = true;
     next += k;
     done += k;
   }
   HIP_TRY(timed_end(r, 4, ev_all));
-  if (result)
-    HIP_TRY(hipMemcpyAsync(result, r->result.p, sizeof(float) * 3 * r->W * r->H, hipMemcpyDeviceToHost, r->stream));
+  if (result) return copy_result(r, result);
   HIP_TRY(hipStreamSynchronize(r->stream));
   return CPT_OK;
 }
@@ -1260,12 +1354,12 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   if (mode == CPT_MODE_BDPT_ADAPTIVE_EMP) {
     // src/cudaRTBDPTStreamAdapEmp.cu:24-35, indices 0-3
     r->attribs = {{"Adaptive Mode", 4, 0}, {"Desired Max Sampling", 0, 5},
-                  {"Min Trace Probability", 3, 0}, {"Debug Mode", 4, 0}};
+                  {"Min Trace Probability", 3, 0}, {kDebugMode, 4, 0}};
   } else if (is_adaptive(mode)) {
     // src/cudaRTBDPTStreamAdapBilatAtomic.cu:26-71 and src/cudaRTBDPTStreamAdapBilatBinary.cu:26-68,
     // indices 0-14 as the reference numbers them
     r->attribs = {{"Adaptive Mode", 4, 0},          {"Desired Max Sampling", 0, 5},
-                  {"Min Trace Probability", 3, 0},  {"Debug Mode", 4, 0},
+                  {"Min Trace Probability", 3, 0},  {kDebugMode, 4, 0},
                   {"Filter Color EuD", 3, 50},      {"Filter Pos EuD", 3, 1},
                   {"Filter Norm EuD", 3, 0.25},     {"Filter Diff EuD", 3, 0.1},
                   {"Filter Radius", 0, 5},          {"Filter Diffuse Flag", 2, 0},
@@ -1281,6 +1375,12 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
   r->attribs.push_back({kLiteralShadow, 2, 0});
   if (is_adaptive(mode)) {
     r->attribs.push_back({kWarmup, 0, 5});
+    r->attribs.push_back({kMinPaths, 0, 0});
+    r->attribs.push_back({kMaxPaths, 0, 0});
+    if (mode != CPT_MODE_BDPT_ADAPTIVE_EMP) {
+      r->attribs.push_back({kFilterSpatial, 3, 0});
+      r->attribs.push_back({kFilterDepth, 3, 0});
+    }
     r->attribs.push_back({kPassFrames, 0, 1});
     r->attribs.push_back({kLearnedSignal, 0, 0});
   } else {
@@ -1338,7 +1438,8 @@ int cpt_renderer_set_attrib(cpt_renderer* r, uint32_t idx, double value) {
   if (idx >= r->attribs.size()) return fail(CPT_E_ARG, "cpt_renderer_set_attrib: index %u out of range", idx);
   if (!(value >= 0) || value > 1.8e19) return fail(CPT_E_ARG, "cpt_renderer_set_attrib: value out of range");
   r->attribs[idx].value = value;
-  r->have_cam = false;                             // parameters change the estimator: restart
+  // parameters change the estimator: restart; the output selection only picks the image handed back
+  if (std::strcmp(r->attribs[idx].name, kDebugMode) != 0) r->have_cam = false;
   return CPT_OK;
 }
 
@@ -1378,6 +1479,11 @@ int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out) {
       HIP_TRY(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, r->stream));
       break;
     }
+    case CPT_BUF_PIXEL_PATHS:
+      if (!is_adaptive(r->mode))
+        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive modes", which);
+      HIP_TRY(hipMemcpyAsync(out, r->paths.p, n * 4, hipMemcpyDeviceToHost, r->stream));
+      break;
     case CPT_BUF_SECOND_MOMENT:
       if (!is_adaptive(r->mode))
         return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive modes", which);
@@ -1823,7 +1929,10 @@ int batch_render(cpt_batch* b, cpt_scene* s, u32 frames) {
       r->extend_launches += b->cfg.eye_depth;
     } else {
       rc = batch_enqueue(b, s, k);
-      if (rc) return rc;
+      if (rc) {
+        join_side(r);
+        return rc;
+      }
     }
     done += k;
   }

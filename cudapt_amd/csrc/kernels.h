@@ -201,13 +201,25 @@ struct FilterArgs {
   uint32_t width, height, radius, nd;
   float kc, kp, kn, kd;   // 2 sigma^2 per term
   int use_diff;
+  // depth term (no counterpart in the reference filter): kz = 2 sigma_z^2 of the
+  // relative depth difference (z_q - z_p) / z_p, z = |first-hit position - cam|;
+  // kz = 0 leaves it out
+  float kz;
+  float cam[3];
 };
 hipError_t launch_filter_error(const FilterArgs& fa, const float* color, const float* gpos, const float* gnrm,
                                const float* gdif, const float* gauss, const uint32_t* sample_n, float* filtered,
                                float* err, hipStream_t st);
 hipError_t launch_error_sum(const float* err, uint32_t n, double* block_sums, float* sum_out, hipStream_t st);
+// paths / maxp: per-pixel path counts and the cap a pass's count stops at (maxp 0: no cap)
 hipError_t launch_alloc_scan(const float* err, const float* sum, uint32_t n, uint32_t gen, uint32_t* counts,
-                             uint32_t* offsets, uint32_t* block_tot, uint32_t* total, uint32_t cap, hipStream_t st);
+                             uint32_t* offsets, uint32_t* block_tot, uint32_t* total, uint32_t cap,
+                             const uint32_t* paths, uint32_t maxp, hipStream_t st);
+// Per-pixel path cap (maxp 0: no-ops): error of capped pixels to 0; counts
+// stopped at maxp - paths; paths += counts (or += 1 with counts null).
+hipError_t launch_cap_error(float* err, const uint32_t* paths, uint32_t maxp, uint32_t n, hipStream_t st);
+hipError_t launch_cap_counts(uint32_t* counts, const uint32_t* paths, uint32_t maxp, uint32_t n, hipStream_t st);
+hipError_t launch_add_paths(uint32_t* paths, const uint32_t* counts, uint32_t n, int reset, hipStream_t st);
 hipError_t launch_gen_adaptive(const FrameArgs& fa, const uint32_t* offsets, const uint32_t* counts,
                                const uint32_t* total, uint32_t cap, PathBuf pb, SampleRec rec, hipStream_t st);
 hipError_t launch_accum_adaptive(const uint32_t* offsets, const uint32_t* counts, uint32_t n, uint32_t cap,

@@ -1887,6 +1887,9 @@ __global__ __launch_bounds__(kTile * kTile) void k_filter_error(FilterArgs fa, c
   float* sp = sc + 3 * (LDS ? TA : 0);
   float* sn = sp + 3 * (LDS ? TA : 0);
   float* sd = sn + 3 * (LDS ? TA : 0);
+  float* sz = sd + 3 * (LDS && DIFF ? TA : 0);     // depth per halo pixel (depth term on)
+  const bool depth = fa.kz > 0.f;
+  const float cam[3] = {fa.cam[0], fa.cam[1], fa.cam[2]};
   for (int k = tid; k < 2 * R + 1; k += kTile * kTile) sg[k] = gauss[k];
   if (LDS) {
     for (int k = tid; k < TA; k += kTile * kTile) {
@@ -1899,6 +1902,7 @@ __global__ __launch_bounds__(kTile * kTile) void k_filter_error(FilterArgs fa, c
         sn[k * 3 + c] = gnrm[g + c];
         if (DIFF) sd[k * 3 + c] = gdif[g + c];
       }
+      if (depth) sz[k] = sqrtf(sqdist3(gpos + g, cam));
     }
   }
   __syncthreads();
@@ -1913,22 +1917,30 @@ __global__ __launch_bounds__(kTile * kTile) void k_filter_error(FilterArgs fa, c
     cn[c] = gnrm[ind * 3 + c];
     cd[c] = DIFF ? gdif[ind * 3 + c] : 0.f;
   }
+  const float zc = depth ? sqrtf(sqdist3(cp, cam)) : 0.f;
   float sum = 0.f, t0 = 0.f, t1 = 0.f, t2 = 0.f;
   for (int i = -R; i <= R; i++) {
     for (int j = -R; j <= R; j++) {
       const float *qc, *qp, *qn, *qd = nullptr;
+      float zq = 0.f;
       if (LDS) {
         int k = (ty + R + i) * TW + (tx + R + j);
         qc = sc + k * 3; qp = sp + k * 3; qn = sn + k * 3;
         if (DIFF) qd = sd + k * 3;
+        if (depth) zq = sz[k];
       } else {
         size_t g = ((size_t)ref_clamp_i((int)y + i, H) * W + ref_clamp_i((int)x + j, W)) * 3;
         qc = color + g; qp = gpos + g; qn = gnrm + g;
         if (DIFF) qd = gdif + g;
+        if (depth) zq = sqrtf(sqdist3(qp, cam));
       }
       float a = sqdist3(qc, cc) / fa.kc + sqdist3(qp, cp) / fa.kp;
       a = a + sqdist3(qn, cn) / fa.kn;
       if (DIFF) a = a + sqdist3(qd, cd) / fa.kd;
+      if (depth) {
+        const float dz = zc > 0.f ? (zq - zc) / zc : 0.f;
+        a = a + (dz * dz) / fa.kz;
+      }
       float f = (sg[i + R] * sg[j + R]) * cpt_expf(-a);
       t0 = t0 + f * qc[0];
       t1 = t1 + f * qc[1];
@@ -1958,7 +1970,7 @@ hipError_t launch_filter_error(const FilterArgs& fa, const float* color, const f
   const bool lds = (int)fa.radius <= kLdsRadius;
   const int TW = kTile + 2 * (int)fa.radius;
   size_t bytes = sizeof(float) * (2 * kMaxFilterRadius + 2);
-  if (lds) bytes += sizeof(float) * 3 * TW * TW * (fa.use_diff ? 4 : 3);
+  if (lds) bytes += sizeof(float) * TW * TW * (3 * (fa.use_diff ? 4 : 3) + (fa.kz > 0.f ? 1 : 0));
   if (fa.use_diff) {
     if (lds) k_filter_error<true, true><<<grid, block, bytes, st>>>(fa, color, gpos, gnrm, gdif, gauss, sample_n, filtered, err);
     else k_filter_error<true, false><<<grid, block, bytes, st>>>(fa, color, gpos, gnrm, gdif, gauss, sample_n, filtered, err);
@@ -2008,9 +2020,19 @@ constexpr int kScanPer = 4;                       // elements per thread in the 
 constexpr int kScanBlock = kBlock * kScanPer;     // 1024 elements per workgroup
 
 // Counts and workgroup-local exclusive offsets.
+// With a per-pixel cap (maxp > 0) a count stops at maxp - paths.
+__device__ __forceinline__ uint32_t cap_count(uint32_t c, const uint32_t* __restrict__ paths, uint32_t maxp,
+                                              uint32_t i) {
+  if (maxp == 0u) return c;
+  const uint32_t have = paths[i];
+  const uint32_t room = have < maxp ? maxp - have : 0u;
+  return c < room ? c : room;
+}
+
 __global__ __launch_bounds__(kBlock) void k_scan_local(const float* __restrict__ err, const float* __restrict__ sum,
                                                        uint32_t n, uint32_t gen, uint32_t* __restrict__ counts,
-                                                       uint32_t* __restrict__ offsets, uint32_t* __restrict__ btot) {
+                                                       uint32_t* __restrict__ offsets, uint32_t* __restrict__ btot,
+                                                       const uint32_t* __restrict__ paths, uint32_t maxp) {
   __shared__ uint32_t s[kBlock];
   const float sm = *sum;
   const uint32_t base = blockIdx.x * kScanBlock + threadIdx.x * kScanPer;
@@ -2018,7 +2040,7 @@ __global__ __launch_bounds__(kBlock) void k_scan_local(const float* __restrict__
 #pragma unroll
   for (int k = 0; k < kScanPer; k++) {
     uint32_t i = base + k;
-    c[k] = i < n ? prop_count(err[i], sm, gen) : 0u;
+    c[k] = i < n ? cap_count(prop_count(err[i], sm, gen), paths, maxp, i) : 0u;
     tsum += c[k];
   }
   s[threadIdx.x] = tsum;
@@ -2069,10 +2091,12 @@ __global__ __launch_bounds__(kBlock) void k_scan_add(uint32_t* __restrict__ offs
   if (i < n) offsets[i] += btot[i / kScanBlock];
 }
 hipError_t launch_alloc_scan(const float* err, const float* sum, uint32_t n, uint32_t gen, uint32_t* counts,
-                             uint32_t* offsets, uint32_t* block_tot, uint32_t* total, uint32_t cap, hipStream_t st) {
+                             uint32_t* offsets, uint32_t* block_tot, uint32_t* total, uint32_t cap,
+                             const uint32_t* paths, uint32_t maxp, hipStream_t st) {
   (void)cap;
+  if (maxp && !paths) return hipErrorInvalidValue;
   uint32_t nb = (n + kScanBlock - 1) / kScanBlock;
-  k_scan_local<<<nb, kBlock, 0, st>>>(err, sum, n, gen, counts, offsets, block_tot);
+  k_scan_local<<<nb, kBlock, 0, st>>>(err, sum, n, gen, counts, offsets, block_tot, paths, maxp);
   k_scan_totals<<<1, kBlock, 0, st>>>(block_tot, nb, total);
   k_scan_add<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(offsets, block_tot, n);
   return hipGetLastError();
@@ -2444,4 +2468,41 @@ hipError_t launch_round_fill(const RoundState* rs, uint32_t n, uint32_t* counts,
   return hipGetLastError();
 }
 
+
+// ---------------------------------------------------------------- per-pixel path caps
+// A pixel that already has maxp paths gets no error weight (the allocation
+// gives it nothing); counts of a round-based pass stop at maxp - paths.
+__global__ __launch_bounds__(kBlock) void k_cap_error(float* __restrict__ err, const uint32_t* __restrict__ paths,
+                                                      uint32_t maxp, uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n && paths[i] >= maxp) err[i] = 0.f;
+}
+__global__ __launch_bounds__(kBlock) void k_cap_counts(uint32_t* __restrict__ counts,
+                                                       const uint32_t* __restrict__ paths, uint32_t maxp,
+                                                       uint32_t n) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) counts[i] = cap_count(counts[i], paths, maxp, i);
+}
+// paths += counts (an adaptive pass), or += 1 (a warm-up frame); reset first
+// when `reset`.
+__global__ __launch_bounds__(kBlock) void k_add_paths(uint32_t* __restrict__ paths,
+                                                      const uint32_t* __restrict__ counts, uint32_t n, int reset) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) paths[i] = (reset ? 0u : paths[i]) + (counts ? counts[i] : 1u);
+}
+hipError_t launch_cap_error(float* err, const uint32_t* paths, uint32_t maxp, uint32_t n, hipStream_t st) {
+  if (!maxp || n == 0) return hipSuccess;
+  k_cap_error<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(err, paths, maxp, n);
+  return hipGetLastError();
+}
+hipError_t launch_cap_counts(uint32_t* counts, const uint32_t* paths, uint32_t maxp, uint32_t n, hipStream_t st) {
+  if (!maxp || n == 0) return hipSuccess;
+  k_cap_counts<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(counts, paths, maxp, n);
+  return hipGetLastError();
+}
+hipError_t launch_add_paths(uint32_t* paths, const uint32_t* counts, uint32_t n, int reset, hipStream_t st) {
+  if (n == 0) return hipSuccess;
+  k_add_paths<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(paths, counts, n, reset);
+  return hipGetLastError();
+}
 }  // namespace cpt

@@ -233,7 +233,9 @@ enum {
   CPT_BUF_SAMPLE_COUNTS = 7, /* uint32: samples drawn per pixel in the last adaptive frame */
   CPT_BUF_ALLOC_ERROR = 8,   /* float x1: the error signal the last allocation used (the learned
                                 estimate when a model is attached, else CPT_BUF_ERROR) */
-  CPT_BUF_SECOND_MOMENT = 9  /* float x3: running mean of squared samples (squareResult, AdapEmp.cu:927-944) */
+  CPT_BUF_SECOND_MOMENT = 9, /* float x3: running mean of squared samples (squareResult, AdapEmp.cu:927-944) */
+  CPT_BUF_PIXEL_PATHS = 10   /* uint32: eye paths traced per pixel since accumulation restarted (adaptive
+                                modes; what "Min Pixel Paths" / "Max Pixel Paths" bound) */
 };
 int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out);
 

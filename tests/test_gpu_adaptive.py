@@ -244,3 +244,49 @@ def test_empirical_mode_has_no_filter(cpt):
         R.read(cpt.CPT_BUF_FILTERED)
     with pytest.raises(cpt.CptError):
         R.set_mlp(cpt.Mlp(1))
+
+
+@pytest.mark.parametrize("mode", [1, 2, 3])
+def test_debug_mode_selects_the_returned_image(cpt, mode):
+    """Debug Mode (AdapBilatAtomic.cu:1539-1552) picks the image a Render call
+    returns and leaves the estimator alone: the running mean is bit-identical
+    to a renderer with Debug Mode 0, and switching the mode back restarts
+    nothing."""
+    sc = scenes.cornell()
+    W, H, F = 48, 40, 8
+    S = cpt.Scene(sc)
+    kw = {"Min Trace Probability": 0.05}
+    base = cpt.Renderer(W, H, mode=mode, **kw).render(S, sc["camera"], frames=F)
+    outs = {}
+    for dm in (1, 2, 3, 4):
+        if dm == 4 and mode == cpt.CPT_MODE_BDPT_ADAPTIVE_EMP:
+            continue
+        R = cpt.Renderer(W, H, mode=mode, **kw)
+        R.set_attrib("Debug Mode", dm)
+        out = R.render(S, sc["camera"], frames=F)
+        assert np.array_equal(R.read(cpt.CPT_BUF_RESULT).view(np.uint32), base.view(np.uint32))
+        err, counts = R.read(cpt.CPT_BUF_ERROR), R.read(cpt.CPT_BUF_SAMPLE_COUNTS)
+        want = {1: counts.astype(np.float32), 2: np.maximum(err, 0.0), 3: np.maximum(err, np.float32(0.05))}
+        if dm == 4:
+            assert np.array_equal(out, R.read(cpt.CPT_BUF_FILTERED))
+        else:
+            assert np.array_equal(out, np.repeat(want[dm][..., None], 3, axis=2))
+        outs[dm] = out
+        R.set_attrib("Debug Mode", 0)               # back to the running mean, one more frame, no restart
+        nxt = R.render(S, sc["camera"], frames=1)
+        assert R.sample_counts().sum() > 0 and np.isfinite(nxt).all()
+        ref = cpt.Renderer(W, H, mode=mode, **kw).render(S, sc["camera"], frames=F + 1)
+        assert np.array_equal(nxt.view(np.uint32), ref.view(np.uint32))
+    assert not np.array_equal(outs[2], base)
+
+
+def test_debug_mode_checks(cpt):
+    sc = scenes.cornell()
+    S = cpt.Scene(sc)
+    R = cpt.Renderer(16, 16, mode=cpt.CPT_MODE_BDPT_ADAPTIVE_EMP)
+    R.set_attrib("Debug Mode", 4)                    # the empirical mode has no filtered image
+    with pytest.raises(cpt.CptError, match="Debug Mode 4"):
+        R.render(S, sc["camera"], frames=1)
+    R.set_attrib("Debug Mode", 5)
+    with pytest.raises(cpt.CptError, match="not in 0..4"):
+        R.render(S, sc["camera"], frames=1)
