@@ -81,6 +81,8 @@ def lib():
             "cpt_trace_counters": [vp, vp, vp, vp, i32],
             "cpt_filter_cross_bilateral_device": [vp, vp, vp, vp, c32, c32, f32, f32, f32, f32, c32, vp, c32, vp,
                                                   vp, vp],
+            "cpt_filter_cross_bilateral_device_noalloc": [vp, vp, vp, vp, c32, c32, f32, f32, f32, f32, c32, vp,
+                                                          c32, vp, vp, vp],
             "cpt_renderer_create": [vp, i32, c32, c32],
             "cpt_render_iteration": [vp, vp, vp, vp, vp, f32, vp],
             "cpt_render_frames": [vp, vp, vp, vp, vp, f32, c32, vp],
@@ -558,8 +560,11 @@ def make_rays(o, d, max_dist=1e20, eps=1e-4) -> np.ndarray:
 
 
 def filter_cross_bilateral(color, pos, norm, diff, sample_n, sigma_color, sigma_pos, sigma_norm, sigma_diff, radius,
-                           nd=5):
-    """Device cross-bilateral + weighted squared residual on host images (copies in and out)."""
+                           nd=5, noalloc=False):
+    """Device cross-bilateral + weighted squared residual on host images (copies in and out).
+
+    noalloc: call cpt_filter_cross_bilateral_device_noalloc, which stages its
+    table in the library's static device scratch instead of allocating."""
     H, W = color.shape[:2]
     bufs = [DeviceBuffer.from_array(np.ascontiguousarray(a, np.float32)) for a in (color, pos, norm)]
     db = DeviceBuffer.from_array(np.ascontiguousarray(diff, np.float32)) if diff is not None else None
@@ -567,7 +572,7 @@ def filter_cross_bilateral(color, pos, norm, diff, sample_n, sigma_color, sigma_
     df = DeviceBuffer(W * H * 12)
     de = DeviceBuffer(W * H * 4)
     vp = ctypes.c_void_p
-    name = "cpt_filter_cross_bilateral_device"
+    name = "cpt_filter_cross_bilateral_device_noalloc" if noalloc else "cpt_filter_cross_bilateral_device"
     _check(getattr(lib(), name)(vp(bufs[0].ptr), vp(bufs[1].ptr), vp(bufs[2].ptr), vp(db.ptr) if db else None, W, H,
                                 sigma_color, sigma_pos, sigma_norm, sigma_diff, radius, vp(dn.ptr), nd, vp(df.ptr),
                                 vp(de.ptr), None), name)

@@ -6,6 +6,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -186,6 +187,33 @@ FilterArgs filter_args(u32 width, u32 height, float sc, float sp, float sn, floa
   f.kz = 0.f;
   f.cam[0] = f.cam[1] = f.cam[2] = 0.f;
   return f;
+}
+
+// Staging of cpt_filter_cross_bilateral_device_noalloc: the spatial table the
+// filter kernel reads, kept in a file-scope device array instead of a
+// per-call allocation.  Every device the library runs on holds its own copy
+// (256 bytes) until the process ends.  In host code `scratch` names a host-side
+// stand-in, never the device copy: kernels and copies take the address that
+// hipGetSymbolAddress returns for the current device, cached per device.
+__device__ __align__(256) float scratch[kFilterTable];
+constexpr int kScratchDevices = 64;
+std::mutex g_scratch_mu;
+float* g_scratch_ptr[kScratchDevices] = {};
+
+int filter_scratch(float** out) {
+  int dev = 0;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kScratchDevices)
+    return fail(CPT_E_LIMIT, "filter scratch: device %d beyond the %d the library tracks", dev, kScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(CPT_E_HIP, "filter scratch: hipGetSymbolAddress returned no address on device %d", dev);
+    g_scratch_ptr[dev] = (float*)p;
+  }
+  *out = g_scratch_ptr[dev];
+  return CPT_OK;
 }
 
 }  // namespace
@@ -510,6 +538,34 @@ int cpt_filter_cross_bilateral_device(const float* d_color, const float* d_pos, 
   hipError_t e = launch_filter_error(f, d_color, d_pos, d_norm, d_diff, d_g, d_sample_n, d_filtered, d_err, st);
   hipError_t e2 = hipStreamSynchronize(st);
   (void)hipFree(d_g);
+  HIP_TRY(e);
+  HIP_TRY(e2);
+  return CPT_OK;
+}
+
+int cpt_filter_cross_bilateral_device_noalloc(const float* d_color, const float* d_pos, const float* d_norm,
+                                              const float* d_diff, uint32_t width, uint32_t height,
+                                              float sigma_color, float sigma_pos, float sigma_norm, float sigma_diff,
+                                              uint32_t radius, const uint32_t* d_sample_n, uint32_t nd,
+                                              float* d_filtered, float* d_err, void* stream) {
+  if (!d_color || !d_pos || !d_norm || !d_sample_n || !d_filtered || !d_err)
+    return fail(CPT_E_ARG, "cpt_filter_cross_bilateral_device_noalloc: null argument");
+  if (width == 0 || height == 0) return fail(CPT_E_ARG, "cpt_filter_cross_bilateral_device_noalloc: empty image");
+  if (radius > (u32)kMaxFilterRadius)
+    return fail(CPT_E_LIMIT, "cpt_filter_cross_bilateral_device_noalloc: radius %u > %d: the %d-entry scratch table "
+                "holds 2 * radius + 1 taps", radius, kMaxFilterRadius, kFilterTable);
+  float* d_g = nullptr;
+  int rc = filter_scratch(&d_g);
+  if (rc) return rc;
+  hipStream_t st = (hipStream_t)stream;
+  float g[kFilterTable];
+  filter_table(sigma_color, radius, g);
+  // every entry is rewritten, so nothing of an earlier call's table survives
+  HIP_TRY(hipMemcpyAsync(d_g, g, sizeof g, hipMemcpyHostToDevice, st));
+  const FilterArgs f = filter_args(width, height, sigma_color, sigma_pos, sigma_norm, sigma_diff, radius, nd,
+                                   d_diff != nullptr);
+  hipError_t e = launch_filter_error(f, d_color, d_pos, d_norm, d_diff, d_g, d_sample_n, d_filtered, d_err, st);
+  hipError_t e2 = hipStreamSynchronize(st);       // g is on this stack frame until the copy has run
   HIP_TRY(e);
   HIP_TRY(e2);
   return CPT_OK;

@@ -159,6 +159,23 @@ int cpt_filter_cross_bilateral_device(const float* d_color, const float* d_pos, 
                                       const uint32_t* d_sample_n, uint32_t nd, float* d_filtered, float* d_err,
                                       void* stream);
 
+/* No counterpart in the reference: cpt_filter_cross_bilateral_device for
+ * callers that cannot allocate device memory (tight serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments,
+ * kernels and results; the spatial Gaussian table, which the entry point above
+ * stages in a per-call allocation, lives in a file-scope device array of 64
+ * floats instead (radius <= 31; a larger radius is CPT_E_LIMIT).  It calls no
+ * hipMalloc, hipFree or allocating library routine.  Every call on one GPU
+ * shares that GPU's array and rewrites all of it: calls that overlap on one GPU
+ * (two host threads, or two streams) race on the table.  Returns once the
+ * filter has run on `stream` (NULL = the null stream): it synchronises that
+ * stream, so it is not for use under stream capture. */
+int cpt_filter_cross_bilateral_device_noalloc(const float* d_color, const float* d_pos, const float* d_norm,
+                                              const float* d_diff, uint32_t width, uint32_t height,
+                                              float sigma_color, float sigma_pos, float sigma_norm, float sigma_diff,
+                                              uint32_t radius, const uint32_t* d_sample_n, uint32_t nd,
+                                              float* d_filtered, float* d_err, void* stream);
+
 /* Traversal cost per ray, the counterpart of the reference's BVH debug
  * renderer (TraceCost / TraceDepth, src/cudaRTCommon.cu:129-273;
  * src/cudaRTBVHDebug.cu): cost2[2i] = nodes visited, cost2[2i+1] = triangles
