@@ -10,9 +10,13 @@ metric/value: Mrays/s = (extension + shadow rays traced) / wall time.
 roofline: the closest-hit extension kernel (by default k_trace_dyn: persistent
   waves that take new rays from a global queue once 64 (first bounce) or 48
   (later bounces) of their lanes are idle), algorithmic bytes per launch =
-  48 B per ray (32 B ray in + 16 B hit out) + 128 B (FP32 layout) or 64 B
-  (quantized layout) per 4-wide node visited + 48 B per triangle tested, over
-  its HIP-event duration on the renderer stream.
+  32 B per ray read from memory (first-bounce eye rays are computed in the
+  kernel and read nothing) + 16 B hit out per ray + 128 B (FP32 layout) or
+  64 B (quantized layout) per 4-wide node visited + 48 B per triangle tested,
+  over its HIP-event duration on the renderer stream.  Beside it, from
+  rocprofv3 counter passes: the HBM bytes actually moved (FETCH_SIZE +
+  WRITE_SIZE, calibrated on a copy) and the VALU issue rate (SQ_INSTS_VALU),
+  and the limiter they name.
 cpu_baseline: the oracle's FP32 reference estimator on the host cores (OpenMP),
   a bounded sample of the same frame (a band of rows at 1 spp).
 time_to_equal_mse: the learned-vs-plain adaptive protocol of cudapt_amd/learned.py
@@ -114,82 +118,151 @@ MLP_FLOP_PER_SAMPLE = 3 * 2 * (9 * 64 + 64 * 64 + 64)
 BF16_PEAK_TFLOPS = 2500.0
 
 
-def learned_leg(ca, S, args):
-    """Time-to-equal-MSE of learned vs plain adaptive BDPT (cudapt_amd/learned.py)."""
-    from cudapt_amd import learned
+def mlp_step_throughput(ca, m, X, T, batch=65536, reps=200):
+    """Wall time and MFMA throughput of one training step at `batch` (forward,
+    backward, gradient sum and Adam), steps issued back to back."""
+    dx = ca.DeviceBuffer.from_array(np.ascontiguousarray(X[:batch], np.float32))
+    dt = ca.DeviceBuffer.from_array(np.ascontiguousarray(T[:batch], np.float32))
+    m.train_steps_device(dx, dt, batch, batch, 20, lr=0.0)
+    ca.synchronize()
+    ts = time.perf_counter()
+    m.train_steps_device(dx, dt, batch, batch, reps, lr=0.0)          # one call, steps back to back
+    ca.synchronize()
+    step_s = (time.perf_counter() - ts) / reps
+    tflops = MLP_FLOP_PER_SAMPLE * batch / step_s / 1e12
+    return {"train_step_us": round(step_s * 1e6, 2), "flop_per_step": MLP_FLOP_PER_SAMPLE * batch,
+            "achieved": round(tflops, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": round(tflops / BF16_PEAK_TFLOPS, 5), "bound": "mfma",
+            "note": f"{reps} steps (k_mlp_train + k_mlp_adam) in one cpt_mlp_train_steps call, wall time per step"}
+
+
+def learned_leg(ca, args):
+    """Time-to-equal-MSE of learned vs plain adaptive BDPT on the adaptive
+    heightfield workload (cudapt_amd/learned.py): the 1,050,976-triangle
+    heightfield under its HDR environment, 1920x1080, 16 training poses of
+    low-spp / 4096-spp pairs, 4 held-out poses."""
+    from cudapt_amd import learned, scenes
+    W, H = args.learn_width, args.learn_height
+    t0 = time.perf_counter()
+    sc = scenes.heightfield(light="env")
+    S = ca.Scene(sc)
     n_tr, n_te = args.learn_train, args.learn_test
-    cams = learned.sphere_grid_poses(n_tr + n_te)
+    cams = learned.heightfield_poses(n_tr + n_te, width=W, height=H)
+    attrs = dict(scenes.ADAPTIVE_C3)
+    log(f"learned leg: scene in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
-    refs_tr = [learned.converged(S, c, WIDTH, HEIGHT, args.learn_train_spp, depth=EYE_DEPTH) for c in cams[:n_tr]]
-    refs_te = [learned.converged(S, c, WIDTH, HEIGHT, args.learn_ref_spp, depth=EYE_DEPTH) for c in cams[n_tr:]]
-    log(f"learned leg: converged images in {time.perf_counter() - t0:.1f} s")
+    refs = [learned.converged(S, c, W, H, args.learn_ref_spp, depth=EYE_DEPTH) for c in cams]
+    ref_s = time.perf_counter() - t0
+    log(f"learned leg: {len(refs)} converged {args.learn_ref_spp}-spp images in {ref_s:.1f} s")
     t0 = time.perf_counter()
-    X, T = learned.collect(S, cams[:n_tr], refs_tr, WIDTH, HEIGHT, keep=args.learn_keep, depth=EYE_DEPTH,
-                           pass_frames=args.pass_frames)
+    X, T = learned.collect(S, cams[:n_tr], refs[:n_tr], W, H, checkpoints=(6, 8, 12, 16, 24, 32),
+                           keep=args.learn_keep, depth=EYE_DEPTH, pass_frames=args.pass_frames, attribs=attrs)
     log(f"learned leg: {len(T)} training samples in {time.perf_counter() - t0:.1f} s")
     m = ca.Mlp(0xA22)
     losses, train_s = learned.train(m, X, T, steps=args.learn_steps, batch=65536, lr=1e-3)
     log(f"learned leg: trained in {train_s:.2f} s, loss {losses[0][1]:.3f} -> {losses[-1][1]:.3f}")
-    rows = learned.time_to_equal_mse(S, m, cams[n_tr:], refs_te, WIDTH, HEIGHT, frames=64, depth=EYE_DEPTH,
-                                     pass_frames=args.pass_frames, signal=3)
+    rows = learned.time_to_equal_mse(S, m, cams[n_tr:], refs[n_tr:], W, H, frames=args.learn_frames,
+                                     depth=EYE_DEPTH, pass_frames=args.pass_frames, signal=args.learn_signal,
+                                     attribs=attrs)
     ratios = [r["learned_s"] / r["plain_adaptive_s"] for r in rows if r["learned_s"]]
     bdpt = [r["plain_bdpt_s"] / r["plain_adaptive_s"] for r in rows if r["plain_bdpt_s"]]
     gm = float(np.exp(np.mean(np.log(ratios)))) if len(ratios) == len(rows) else None
-    # MFMA throughput of one training step at batch 65,536
-    batch = 65536
-    dx = ca.DeviceBuffer.from_array(np.ascontiguousarray(X[:batch], np.float32))
-    dt = ca.DeviceBuffer.from_array(np.ascontiguousarray(T[:batch], np.float32))
-    for _ in range(20):
-        m.train_step_device(dx, dt, batch, lr=0.0, want_loss=False)
-    ca.synchronize()
-    reps = 200
-    ts = time.perf_counter()
-    for _ in range(reps):
-        m.train_step_device(dx, dt, batch, lr=0.0, want_loss=False)
-    ca.synchronize()
-    step_s = (time.perf_counter() - ts) / reps
-    tflops = MLP_FLOP_PER_SAMPLE * batch / step_s / 1e12
+    for r in rows:
+        r["learned_over_plain_adaptive"] = r["learned_s"] / r["plain_adaptive_s"] if r["learned_s"] else None
+    mlp = mlp_step_throughput(ca, m, X, T)
+    del S
     return {
         "learned_over_plain_adaptive": round(gm, 4) if gm is not None else None,
         "plain_bdpt_over_plain_adaptive": round(float(np.exp(np.mean(np.log(bdpt)))), 4) if bdpt else None,
+        "target": "<= 0.5",
         "poses": [{k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()} for r in rows],
-        "protocol": {"scene": "sphere grid", "width": WIDTH, "height": HEIGHT, "depth": EYE_DEPTH,
-                     "train_poses": n_tr, "heldout_poses": n_te, "train_ref_spp": args.learn_train_spp,
-                     "heldout_ref_spp": args.learn_ref_spp, "plain_frames": 64, "pass_frames": args.pass_frames,
-                     "train_steps": args.learn_steps, "batch": 65536, "lr": 1e-3, "train_samples": int(len(T)),
-                     "mse": "radiance clamped to [0, 1]", "signal": "sqrt(exp(y) / N * filter residual)",
-                     "timing": "synchronised wall time per render call, MSE readback excluded"},
+        "protocol": {"scene": "heightfield 1,050,976 tris, 512x256 HDR environment of Gaussian lobes",
+                     "width": W, "height": H, "depth": EYE_DEPTH, "train_poses": n_tr, "heldout_poses": n_te,
+                     "ref_spp": args.learn_ref_spp, "training_pairs": "plain adaptive at 6, 8, 12, 16, 24 and "
+                     f"32 spp against the {args.learn_ref_spp}-spp image", "plain_frames": args.learn_frames,
+                     "pass_frames": args.pass_frames, "adaptive": attrs, "train_steps": args.learn_steps,
+                     "batch": 65536, "lr": 1e-3, "train_samples": int(len(T)),
+                     "mse": "radiance clamped to [0, 1]", "signal": args.learn_signal,
+                     "timing": "synchronised wall time per render call, MSE readback excluded",
+                     "reference_images_s": round(ref_s, 2)},
         "train_s": round(train_s, 3),
         "final_loss": round(float(losses[-1][1]), 4),
-    }, {"train_step_us": round(step_s * 1e6, 2), "flop_per_step": MLP_FLOP_PER_SAMPLE * batch,
-        "achieved": round(tflops, 2), "peak": BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
-        "frac": round(tflops / BF16_PEAK_TFLOPS, 5), "bound": "mfma",
-        "note": "wall time of train kernel + gradient-sum/Adam kernel per step"}
+    }, mlp
+
+
+def adaptive_leg(ca, args):
+    """Adaptive sampling + cross-bilateral filter on the heightfield under its
+    HDR environment, 1920x1080, 32 spp on average (5 warm-up frames, then
+    per-frame proportional allocation), filter radius 7, colour / normal /
+    relative-depth sigmas 0.2 / 0.1 / 0.05, 4 to 256 eye paths per pixel."""
+    from cudapt_amd import scenes
+    W, H, F = 1920, 1080, args.adaptive_spp
+    t0 = time.perf_counter()
+    sc = scenes.heightfield(light="env")
+    S = ca.Scene(sc)
+    log(f"adaptive leg: scene in {time.perf_counter() - t0:.1f} s")
+    R = ca.Renderer(W, H, mode=ca.CPT_MODE_BDPT_ADAPTIVE_PLL, eye_depth=EYE_DEPTH, light_depth=LIGHT_DEPTH,
+                    **scenes.ADAPTIVE_C3)
+    cam = sc["camera"]
+    R.render(S, dict(cam, pos=tuple(np.asarray(cam["pos"]) + 1e-3)), frames=8, result=False)   # allocations
+    R.stats(reset=True)
+    R.set_profiling(True)
+    ca.synchronize()
+    t0 = time.perf_counter()
+    R.render(S, cam, frames=F, result=False)
+    ca.synchronize()
+    wall = time.perf_counter() - t0
+    st = R.stats(reset=True)
+    paths = R.read(ca.CPT_BUF_PIXEL_PATHS)
+    filt_frames = F - 5
+    res = {"workload": f"heightfield 1,050,976 tris + HDR environment, {W}x{H}, {F} spp average, proportional "
+                       "allocation per frame, filter radius 7, sigmas colour/normal/depth 0.2/0.1/0.05, "
+                       "4..256 paths per pixel",
+           "seconds": round(wall, 4), "frames": F,
+           "mrays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / wall / 1e6, 2),
+           "host_readback": "none in the timed region: the running mean stays in HBM (the reference copies it "
+                         "to the host after every frame, src/cudaRTBDPTStream.cu:949, 11 MB per 1280x720 image)",
+        "samples_per_s": round(st["eye_paths"] / wall, 1),
+           "paths_per_pixel": {"mean": round(float(paths.mean()), 3), "min": int(paths.min()),
+                               "max": int(paths.max())},
+           "filter_ms_per_frame": round(st["filter_ms"] / max(filt_frames, 1), 4),
+           "filter_gpix_per_s": round(W * H * filt_frames / (st["filter_ms"] / 1e3) / 1e9, 3)
+           if st["filter_ms"] > 0 else None,
+           "phase_ms": {k: round(st[k], 3) for k in ("extend_ms", "shade_ms", "shadow_ms", "filter_ms", "error_ms",
+                                                   "gen_ms", "accum_ms", "total_ms")}}
+    del R, S
+    return res
 
 
 def counter_per_launch(path, ext_name, calib):
-    """(mean bytes per extension dispatch, scale) from one `rocprofv3 --pmc
-    <counter> --output-format csv` file (values in KiB).  Dispatches whose name
-    contains ext_name are averaged; the scale is calib / what the counter read
-    for the largest copy, which must be the calibration copy of calib bytes
-    (the counter runs make no larger one).  None if either is missing."""
+    """(mean counter value per extension dispatch, scale) from one `rocprofv3
+    --pmc <counter> --output-format csv` file.  Dispatches whose name contains
+    ext_name are averaged.  For the byte counters (FETCH_SIZE / WRITE_SIZE,
+    KiB) the scale is calib / what the counter read for the largest copy,
+    which must be the calibration copy of calib bytes (the counter runs make no
+    larger one); calib None: raw values, scale 1.  None if missing."""
     import csv
     ext, copy = [], []
     for row in csv.DictReader(open(path)):
-        v = float(row["Counter_Value"]) * 1024.0
+        v = float(row["Counter_Value"]) * (1024.0 if calib else 1.0)
         if ext_name in row["Kernel_Name"]:
             ext.append(v)
         elif "copyBuffer" in row["Kernel_Name"]:
             copy.append(v)
-    if not ext or not copy or max(copy) <= 0:
+    if not ext:
+        return None
+    if calib is None:
+        return sum(ext) / len(ext), 1.0
+    if not copy or max(copy) <= 0:
         return None
     return sum(ext) / len(ext), calib / max(copy)
 
 
-def hbm_traffic(args):
-    """HBM bytes per extension launch from two rocprofv3 --pmc passes (FETCH_SIZE,
-    then WRITE_SIZE, KiB per dispatch) over one step of the same workload, each
-    scaled by what the counter reads for a 256 MiB device-to-device copy."""
+def counter_passes(args):
+    """Per-extension-launch counters from one rocprofv3 --pmc pass each over one
+    step of the same workload: FETCH_SIZE and WRITE_SIZE (HBM bytes, each
+    scaled by what the counter reads for a 256 MiB device-to-device copy) and
+    SQ_INSTS_VALU (wave64 vector instructions issued)."""
     import shutil
     import subprocess
     import tempfile
@@ -199,25 +272,25 @@ def hbm_traffic(args):
     calib = 256 << 20
     ext_name = extend_kernel_name(args, demangled=True)
     out = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+    for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU"):
         d = tempfile.mkdtemp(prefix="cpt_pmc_")
         cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t",
                "--", sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
-               "--no-traffic", "--no-learned", "--no-multiview", "--baseline-steps", "0", "--frames", str(args.frames),
-               "--in-flight", str(args.in_flight),
+               "--no-traffic", "--no-learned", "--no-multiview", "--baseline-steps", "0", "--adaptive-spp", "0",
+               "--frames", str(args.frames), "--in-flight", str(args.in_flight),
                "--lds-stack", str(args.lds_stack), "--lds-stack-shadow", str(args.lds_stack_shadow),
                "--node-format", str(args.node_format),
                "--refill", str(args.refill), "--refill-secondary", str(args.refill_secondary),
                "--refill-groups", str(args.refill_groups),
                "--refill-shadow", str(args.refill_shadow),
                "--copy-calib", str(calib >> 20)]
-        log(f"traffic pass: {counter}")
+        log(f"counter pass: {counter}")
         r = subprocess.run(cmd, capture_output=True, text=True)
         files = [os.path.join(root, f) for root, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
         if r.returncode != 0 or not files:
-            log(f"traffic pass {counter} failed (rc {r.returncode}); traffic reported as null")
+            log(f"counter pass {counter} failed (rc {r.returncode}); reported as null")
             return None
-        got = counter_per_launch(files[0], ext_name, calib)
+        got = counter_per_launch(files[0], ext_name, calib if counter.endswith("_SIZE") else None)
         shutil.rmtree(d, ignore_errors=True)
         if got is None:
             return None
@@ -225,7 +298,8 @@ def hbm_traffic(args):
     fetch, ffac = out["FETCH_SIZE"]
     write, wfac = out["WRITE_SIZE"]
     return {"bytes_per_launch": int(fetch * ffac + write * wfac), "fetch_bytes": int(fetch * ffac),
-            "write_bytes": int(write * wfac), "fetch_scale": round(ffac, 3), "write_scale": round(wfac, 3)}
+            "write_bytes": int(write * wfac), "fetch_scale": round(ffac, 3), "write_scale": round(wfac, 3),
+            "valu_insts_per_launch": int(out["SQ_INSTS_VALU"][0])}
 
 
 def baselines_leg(ca, S, sc, args):
@@ -290,7 +364,7 @@ def multiview_leg(ca, args):
     """Batched multi-view rendering: autotune, then 32 spp of every view."""
     from cudapt_amd import scenes
     t0 = time.perf_counter()
-    sc = scenes.heightfield()
+    sc = scenes.heightfield(light="env")
     S = ca.Scene(sc)
     views = scenes.heightfield_poses(args.views)
     log(f"multiview: scene {len(sc['idx'])} triangles, {len(views)} views built in {time.perf_counter() - t0:.1f} s")
@@ -309,7 +383,9 @@ def multiview_leg(ca, args):
     res = {"views": len(views), "pixels": px, "spp": args.mv_spp, "triangles": int(len(sc["idx"])),
            "seconds": round(wall, 4), "gpu_ms": round(st["total_ms"], 3),
            "mrays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / wall / 1e6, 2),
-           "samples_per_s": round(st["eye_paths"] / wall, 1),
+           "host_readback": "none in the timed region: the running mean stays in HBM (the reference copies it "
+                         "to the host after every frame, src/cudaRTBDPTStream.cu:949, 11 MB per 1280x720 image)",
+        "samples_per_s": round(st["eye_paths"] / wall, 1),
            "resolutions": sorted({(v["width"], v["height"]) for v in views}),
            "autotuned": best, "candidates_ms_per_iteration": [round(m, 3) for m in cand_ms]}
     del B, S
@@ -344,13 +420,17 @@ def main():
     ap.add_argument("--views", type=int, default=32)
     ap.add_argument("--mv-spp", type=int, default=32)
     ap.add_argument("--mv-in-flight", type=int, default=3)
-    ap.add_argument("--learn-train", type=int, default=8)
-    ap.add_argument("--learn-test", type=int, default=2)
-    ap.add_argument("--learn-train-spp", type=int, default=1024)
+    ap.add_argument("--learn-train", type=int, default=16)
+    ap.add_argument("--learn-test", type=int, default=4)
+    ap.add_argument("--learn-width", type=int, default=1920)
+    ap.add_argument("--learn-height", type=int, default=1080)
     ap.add_argument("--learn-ref-spp", type=int, default=4096)
-    ap.add_argument("--learn-keep", type=float, default=0.0625)
+    ap.add_argument("--learn-keep", type=float, default=1.0 / 32)
     ap.add_argument("--learn-steps", type=int, default=2000)
-    ap.add_argument("--pass-frames", type=int, default=8, help="adaptive pass size in frames (learned leg)")
+    ap.add_argument("--learn-frames", type=int, default=32, help="plain adaptive frames whose MSE is the target")
+    ap.add_argument("--learn-signal", type=int, default=3, help="allocation signal of the learned renderer")
+    ap.add_argument("--pass-frames", type=int, default=1, help="adaptive pass size in frames (learned leg)")
+    ap.add_argument("--adaptive-spp", type=int, default=32, help="average spp of the adaptive leg (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the library renders on one "
@@ -404,9 +484,16 @@ def main():
     mrays = rays / wall / 1e6
     launches = max(st["extend_launches"], 1)
     node_bytes = 64 if args.node_format else 128
-    bytes_alg = 48 * st["extension_rays"] + node_bytes * st["extend_nodes"] + 48 * st["extend_tris"]
+    # first-bounce eye rays are computed inside the extension kernel (no 32 B
+    # ray read) when the refilling grid runs the first bounce over the
+    # quantized nodes, as the renderer's schedule does above Refill Min Paths
+    eye_in_kernel = args.refill >= 1 and args.node_format != 0 and WIDTH * HEIGHT * args.in_flight >= (1 << 20)
+    rays_read = st["extension_rays"] - (st["eye_paths"] if eye_in_kernel else 0)
+    bytes_alg = (32 * rays_read + 16 * st["extension_rays"] + node_bytes * st["extend_nodes"]
+                 + 48 * st["extend_tris"])
     ext_s = st["extend_ms"] / 1e3
     achieved = bytes_alg / ext_s / 1e9 if ext_s > 0 else 0.0
+    launch_s = ext_s / launches
     res = {
         "metric": "Mrays/s + time-to-equal-MSE (learned vs plain adaptive BDPT), 1 MI355X",
         "value": round(mrays, 2),
@@ -429,6 +516,8 @@ def main():
                    "traversal_refill": args.refill, "secondary_refill": args.refill_secondary,
                    "refill_groups": args.refill_groups,
                    "shadow_refill": args.refill_shadow},
+        "host_readback": "none in the timed region: the running mean stays in HBM (the reference copies it "
+                         "to the host after every frame, src/cudaRTBDPTStream.cu:949, 11 MB per 1280x720 image)",
         "samples_per_s": round(st["eye_paths"] / wall, 1),
         "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},
         "kernel_ms": {"extend": round(st["extend_ms"], 3), "shade": round(st["shade_ms"], 3),
@@ -442,7 +531,13 @@ def main():
                      "nodes_per_ray": round(st["extend_nodes"] / max(st["extension_rays"], 1), 2),
                      "tris_per_ray": round(st["extend_tris"] / max(st["extension_rays"], 1), 2),
                      "shadow_nodes_per_ray": round(st["shadow_nodes"] / max(st["shadow_rays"], 1), 2),
-                     "shadow_tris_per_ray": round(st["shadow_tris"] / max(st["shadow_rays"], 1), 2)},
+                     "shadow_tris_per_ray": round(st["shadow_tris"] / max(st["shadow_rays"], 1), 2),
+                     "eye_rays_computed_in_kernel": bool(eye_in_kernel),
+                     "stack_spills_per_ray": round(st["extend_spills"] / max(st["extension_rays"], 1), 5),
+                     "shadow_stack_spills_per_ray": round(st["shadow_spills"] / max(st["shadow_rays"], 1), 5),
+                     "box_tests_per_s": round(4 * st["extend_nodes"] / ext_s, 1) if ext_s > 0 else None,
+                     "tri_tests_per_s": round(st["extend_tris"] / ext_s, 1) if ext_s > 0 else None,
+                     "hbm_measured": None, "valu": None, "limiter": None},
         "time_to_equal_mse": None,
     }
     del R
@@ -456,17 +551,38 @@ def main():
         res["baselines"] = baselines_leg(ca, S, sc, args)
         res["baselines"]["wavefront_over_megakernel"] = round(
             mrays / max(res["baselines"]["megakernel"]["mrays_per_s"], 1e-9), 3)
+    del S
+    res["adaptive"] = adaptive_leg(ca, args) if args.adaptive_spp > 0 else None
     res["mlp"] = None
     if not args.no_learned:
-        res["time_to_equal_mse"], res["mlp"] = learned_leg(ca, S, args)
-    del S
+        res["time_to_equal_mse"], res["mlp"] = learned_leg(ca, args)
     res["multiview"] = None
     if not args.no_multiview:
         res["multiview"] = multiview_leg(ca, args)
     if not args.no_traffic:
-        tr = hbm_traffic(args)
-        res["roofline"]["traffic_detail"] = tr
-        res["roofline"]["traffic"] = tr["bytes_per_launch"] if tr else None
+        tr = counter_passes(args)
+        rl = res["roofline"]
+        rl["traffic_detail"] = tr
+        rl["traffic"] = tr["bytes_per_launch"] if tr else None
+        if tr and launch_s > 0:
+            # what HBM actually delivered (FETCH_SIZE + WRITE_SIZE) against its
+            # peak, beside the algorithmic-byte fraction above
+            gbs = tr["bytes_per_launch"] / launch_s / 1e9
+            rl["hbm_measured"] = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(gbs / HBM_PEAK_GBS, 4)}
+            # VALU issue: a wave64 vector instruction occupies one 16-lane SIMD
+            # for 4 cycles; 256 CUs x 4 SIMDs at 2.4 GHz
+            valu_rate = tr["valu_insts_per_launch"] / launch_s
+            valu_peak = 256 * 4 * 2.4e9 / 4
+            rl["valu"] = {"wave_insts_per_launch": tr["valu_insts_per_launch"],
+                          "achieved": round(valu_rate / 1e9, 1), "peak": round(valu_peak / 1e9, 1),
+                          "unit": "G wave64 instructions/s", "frac": round(valu_rate / valu_peak, 4),
+                          "per_ray": round(tr["valu_insts_per_launch"] * 64 / max(st["extension_rays"] / launches, 1),
+                                           1)}
+            hf, vf = rl["hbm_measured"]["frac"], rl["valu"]["frac"]
+            rl["limiter"] = ("HBM bandwidth" if hf >= 0.6 else "VALU issue" if vf >= 0.6 else
+                             f"latency of dependent node / triangle loads (HBM {hf:.0%} and VALU issue {vf:.0%} of "
+                             "peak; bytes served mostly by L2 / MALL)")
     if not args.no_cpu:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(sc)

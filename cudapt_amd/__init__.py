@@ -23,6 +23,7 @@ CPT_MODE_BDPT_ADAPTIVE_BINARY = 2
 CPT_MODE_BDPT_ADAPTIVE_EMP = 3
 CPT_MODE_PT_STREAM = 4
 CPT_MODE_PT_STREAM_REGEN = 5
+CPT_MODE_PT_DISNEY_MIS = 6
 (CPT_BUF_RESULT, CPT_BUF_SAMPLE_N, CPT_BUF_GBUF_POS, CPT_BUF_GBUF_NORM, CPT_BUF_GBUF_DIFF, CPT_BUF_FILTERED,
  CPT_BUF_ERROR, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_ALLOC_ERROR, CPT_BUF_SECOND_MOMENT, CPT_BUF_PIXEL_PATHS) = range(11)
 MLP_INPUTS, MLP_HIDDEN, MLP_PARAMS = 9, 64, 4865
@@ -36,12 +37,22 @@ class CptError(RuntimeError):
     pass
 
 
+_ATTR_ALIASES = {"eye_depth": "Eye Depth", "light_depth": "Light Depth", "light_paths": "Light Paths",
+                 "seed": "Seed", "literal_shadow": "Literal Shadow Endpoint", "warmup": "Warm-up Frames",
+                 "nd": "Desired Max Sampling", "sigma_color": "Filter Color EuD", "sigma_pos": "Filter Pos EuD",
+                 "sigma_norm": "Filter Norm EuD", "sigma_diff": "Filter Diff EuD", "radius": "Filter Radius",
+                 "filter_diff": "Filter Diffuse Flag", "in_flight": "Frames In Flight",
+                 "pass_frames": "Adaptive Pass Frames", "megakernel": "Megakernel",
+                 "sigma_spatial": "Filter Spatial EuD", "sigma_depth": "Filter Depth EuD",
+                 "min_paths": "Min Pixel Paths", "max_paths": "Max Pixel Paths"}
+
+
 class _Stats(ctypes.Structure):
     _fields_ = [(n, ctypes.c_uint64) for n in ("extension_rays", "shadow_rays", "eye_paths", "frames",
                                               "extend_launches", "extend_nodes", "extend_tris")] + \
                [(n, ctypes.c_double) for n in ("extend_ms", "shadow_ms", "shade_ms", "light_ms", "total_ms",
                                                "filter_ms", "error_ms", "gen_ms", "accum_ms", "mlp_ms")] + \
-               [(n, ctypes.c_uint64) for n in ("shadow_nodes", "shadow_tris")]
+               [(n, ctypes.c_uint64) for n in ("shadow_nodes", "shadow_tris", "extend_spills", "shadow_spills")]
 
 
 def lib():
@@ -71,6 +82,7 @@ def lib():
             "cpt_bvh4_set_leaf_size": [vp, c32],
             "cpt_scene_add_texture": [vp, c32, c32, vp, vp],
             "cpt_scene_clear_textures": [vp],
+            "cpt_scene_set_environment": [vp, c32, c32, vp],
             "cpt_trace_cost": [vp, vp, c32, c32, c32, vp],
             "cpt_trace_depth": [vp, vp, c32, c32, vp],
             "cpt_scene_upload": [vp],
@@ -106,6 +118,9 @@ def lib():
             "cpt_batch_candidates": [vp, vp],
             "cpt_batch_candidate": [vp, c32, vp],
             "cpt_batch_destroy": [vp],
+            "cpt_batch_read_buffer": [vp, c32, i32, vp],
+            "cpt_batch_set_adaptive": [vp, i32, c32, vp],
+            "cpt_batch_set_attrib": [vp, ctypes.c_char_p, ctypes.c_double],
             "cpt_mlp_create": [vp, ctypes.c_uint64],
             "cpt_mlp_destroy": [vp],
             "cpt_mlp_set_params": [vp, vp, c32],
@@ -113,6 +128,7 @@ def lib():
             "cpt_mlp_get_grad": [vp, vp, c32],
             "cpt_mlp_train_step": [vp, vp, vp, c32, f32, i32, vp, vp],
             "cpt_mlp_infer": [vp, vp, c32, vp, vp],
+            "cpt_mlp_train_steps": [vp, vp, vp, c32, c32, c32, c32, f32, vp, vp],
             "cpt_renderer_mlp_features": [vp, vp],
             "cpt_renderer_mlp_target": [vp, vp, f32, vp],
             "cpt_renderer_set_mlp": [vp, vp],
@@ -193,6 +209,8 @@ class Scene:
             self.set_materials(sc["mats"])
             for t in sc.get("textures", ()):
                 self.add_texture(t)
+            if sc.get("env") is not None:
+                self.set_environment(sc["env"])
             self.build_bvh(max_depth)
             if upload:
                 self.upload()
@@ -213,6 +231,17 @@ class Scene:
         self.nt = len(self._idx)
         _check(lib().cpt_scene_set_mesh(self.h, _ptr(self._pos), _ptr(self._nrm), _ptr(self._uv), len(self._pos),
                                         _ptr(self._idx), _ptr(self._mat), self.nt), "cpt_scene_set_mesh")
+
+    def set_environment(self, rgb):
+        """Environment light: an H x W x 3 float latitude-longitude radiance map (None removes it)."""
+        if rgb is None:
+            _check(lib().cpt_scene_set_environment(self.h, 0, 0, None), "cpt_scene_set_environment")
+            return
+        self._env = np.ascontiguousarray(rgb, np.float32)
+        if self._env.ndim != 3 or self._env.shape[2] != 3:
+            raise CptError("environment: H x W x 3 floats")
+        _check(lib().cpt_scene_set_environment(self.h, self._env.shape[1], self._env.shape[0], _ptr(self._env)),
+               "cpt_scene_set_environment")
 
     def add_texture(self, rgba) -> int:
         """Appends an H x W x 4 float texture (RGBA); returns its id."""
@@ -329,14 +358,7 @@ class Renderer:
         return out
 
     def set_attrib(self, name, value):
-        key = {"eye_depth": "Eye Depth", "light_depth": "Light Depth", "light_paths": "Light Paths",
-               "seed": "Seed", "literal_shadow": "Literal Shadow Endpoint", "warmup": "Warm-up Frames",
-               "nd": "Desired Max Sampling", "sigma_color": "Filter Color EuD", "sigma_pos": "Filter Pos EuD",
-               "sigma_norm": "Filter Norm EuD", "sigma_diff": "Filter Diff EuD", "radius": "Filter Radius",
-               "filter_diff": "Filter Diffuse Flag", "in_flight": "Frames In Flight",
-               "pass_frames": "Adaptive Pass Frames", "megakernel": "Megakernel",
-               "sigma_spatial": "Filter Spatial EuD", "sigma_depth": "Filter Depth EuD",
-               "min_paths": "Min Pixel Paths", "max_paths": "Max Pixel Paths"}.get(name, name)
+        key = _ATTR_ALIASES.get(name, name)
         idx = self.attribs()[key][0]
         _check(lib().cpt_renderer_set_attrib(self.h, idx, float(value)), "cpt_renderer_set_attrib")
 
@@ -415,7 +437,8 @@ class BatchConfig(ctypes.Structure):
 
 
 class Batch:
-    """Many camera views of uneven resolution rendered as one wavefront (plain streaming BDPT)."""
+    """Many camera views of uneven resolution rendered as one wavefront (plain
+    streaming BDPT, or per-view adaptive sampling after set_adaptive)."""
 
     def __init__(self, views, **config):
         arr = (_View * len(views))()
@@ -463,6 +486,32 @@ class Batch:
         out = np.zeros((h, w), np.uint32)
         _check(lib().cpt_batch_sample_counts(self.h, view, _ptr(out)), "cpt_batch_sample_counts")
         return out
+
+    def read(self, view: int, which: int) -> np.ndarray:
+        """One view's CPT_BUF_* buffer."""
+        w, h = self.views[view]
+        if which in (CPT_BUF_SAMPLE_N, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_PIXEL_PATHS):
+            out = np.zeros((h, w), np.uint32)
+        elif which in (CPT_BUF_ERROR, CPT_BUF_ALLOC_ERROR):
+            out = np.zeros((h, w), np.float32)
+        else:
+            out = np.zeros((h, w, 3), np.float32)
+        _check(lib().cpt_batch_read_buffer(self.h, view, which, _ptr(out)), "cpt_batch_read_buffer")
+        return out
+
+    def set_adaptive(self, mode: int = 1, pass_frames: int = 1, model: "Mlp | None" = None, **attribs):
+        """Per-view adaptive sampling (mode CPT_MODE_BDPT_ADAPTIVE_PLL, optionally
+        driven by a learned error model), or plain frames (CPT_MODE_BDPT_STREAM);
+        then the given renderer attributes (names or their aliases)."""
+        _check(lib().cpt_batch_set_adaptive(self.h, mode, pass_frames, model.h if model is not None else None),
+               "cpt_batch_set_adaptive")
+        self._model = model                  # the batch holds a pointer to it
+        for k, v in attribs.items():
+            self.set_attrib(k, v)
+
+    def set_attrib(self, name: str, value):
+        key = _ATTR_ALIASES.get(name, name)
+        _check(lib().cpt_batch_set_attrib(self.h, key.encode(), float(value)), "cpt_batch_set_attrib")
 
     def stats(self, reset: bool = False) -> dict:
         s = _Stats()
@@ -528,6 +577,19 @@ class Mlp:
         _check(lib().cpt_mlp_train_step(self.h, vp(d_x.ptr + 36 * x_offset), vp(d_t.ptr + 4 * x_offset), n, lr,
                                         int(apply), ctypes.byref(loss) if want_loss else None, None),
                "cpt_mlp_train_step")
+        return loss.value if want_loss else None
+
+    def train_steps_device(self, d_x: "DeviceBuffer", d_t: "DeviceBuffer", n_samples: int, batch: int, steps: int,
+                           lr: float = 1e-3, first_batch: int = 0, want_loss: bool = False):
+        """`steps` Adam steps over consecutive batches of the device data set in one call
+        (cpt_mlp_train_steps), starting at batch first_batch; returns the last step's loss
+        when want_loss."""
+        if batch < 1 or n_samples < batch or 36 * n_samples > d_x.nbytes or 4 * n_samples > d_t.nbytes:
+            raise CptError(f"{n_samples} samples / batch {batch} do not fit the device arrays")
+        loss = ctypes.c_float()
+        _check(lib().cpt_mlp_train_steps(self.h, ctypes.c_void_p(d_x.ptr), ctypes.c_void_p(d_t.ptr), n_samples,
+                                         batch, first_batch, steps, lr, ctypes.byref(loss) if want_loss else None,
+                                         None), "cpt_mlp_train_steps")
         return loss.value if want_loss else None
 
     def train_step(self, x, t, lr: float = 1e-3, apply: bool = True) -> float:

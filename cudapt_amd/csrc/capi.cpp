@@ -71,6 +71,8 @@ struct cpt_scene {
   std::vector<float> pos, nrm, uv;
   std::vector<float4> tex;              // RTScene::m_pTextures' texels (RGBA), concatenated
   std::vector<uint4> tex_desc;          // first texel, width, height
+  std::vector<float4> env;              // environment radiance, env_w x env_h (empty: none)
+  u32 env_w = 0, env_h = 0;
   std::vector<u32> idx, mat;
   std::vector<cpt_material> mats;
   u32 nv = 0, nt = 0;
@@ -84,7 +86,7 @@ struct cpt_scene {
   hipStream_t stream = nullptr;
   DevArray<Node4> d_nodes4;
   DevArray<Node4Q> d_nodes4q;
-  DevArray<float4> d_tris, d_tris_bin, d_vpos, d_vnrm, d_bnodes, d_tex, d_surf;
+  DevArray<float4> d_tris, d_tris_bin, d_vpos, d_vnrm, d_bnodes, d_tex, d_surf, d_env;
   DevArray<uint4> d_tex_desc;
   DevArray<uint4> d_tri_verts;
   DevArray<uint2> d_boff;
@@ -112,6 +114,9 @@ struct cpt_scene {
     s.tex = d_tex.p;
     s.tex_desc = d_tex_desc.p;
     s.n_tex = (u32)tex_desc.size();
+    s.env = d_env.p;
+    s.env_w = env.empty() ? 0u : env_w;
+    s.env_h = env.empty() ? 0u : env_h;
     return s;
   }
   ~cpt_scene() {
@@ -301,6 +306,25 @@ int cpt_scene_add_texture(cpt_scene* s, uint32_t width, uint32_t height, const f
   return CPT_OK;
 }
 
+int cpt_scene_set_environment(cpt_scene* s, uint32_t width, uint32_t height, const float* rgb) {
+  if (!s) return fail(CPT_E_ARG, "cpt_scene_set_environment: null scene");
+  if (width == 0 || height == 0) {
+    s->env.clear();
+    s->env_w = s->env_h = 0;
+  } else {
+    if (!rgb) return fail(CPT_E_ARG, "cpt_scene_set_environment: null texels");
+    if ((uint64_t)width * height > (1ull << 26))
+      return fail(CPT_E_LIMIT, "cpt_scene_set_environment: %ux%u texels exceed 2^26", width, height);
+    s->env.resize((size_t)width * height);
+    for (size_t i = 0; i < s->env.size(); i++) s->env[i] = make_float4(rgb[i * 3], rgb[i * 3 + 1], rgb[i * 3 + 2], 0.f);
+    s->env_w = width;
+    s->env_h = height;
+  }
+  s->uploaded = false;
+  s->mesh_version++;                    // renderers drawing this scene restart
+  return CPT_OK;
+}
+
 int cpt_scene_clear_textures(cpt_scene* s) {
   if (!s) return fail(CPT_E_ARG, "cpt_scene_clear_textures: null scene");
   s->tex.clear();
@@ -434,6 +458,7 @@ int cpt_scene_upload(cpt_scene* s) {
   HIP_TRY(s->d_bnodes.upload(bnodes.data(), bnodes.size(), s->stream));
   HIP_TRY(s->d_tex.upload(s->tex.data(), s->tex.size(), s->stream));
   HIP_TRY(s->d_tex_desc.upload(s->tex_desc.data(), s->tex_desc.size(), s->stream));
+  HIP_TRY(s->d_env.upload(s->env.data(), s->env.size(), s->stream));
   HIP_TRY(s->d_boff.upload(boff.data(), boff.size(), s->stream));
   rc = upload_materials(s);
   if (rc) return rc;
@@ -592,6 +617,7 @@ namespace {
 
 bool is_adaptive(int mode) { return mode >= CPT_MODE_BDPT_ADAPTIVE_PLL && mode <= CPT_MODE_BDPT_ADAPTIVE_EMP; }
 bool is_pt(int mode) { return mode == CPT_MODE_PT_STREAM || mode == CPT_MODE_PT_STREAM_REGEN; }
+bool is_disney(int mode) { return mode == CPT_MODE_PT_DISNEY_MIS; }
 bool uses_filter(int mode) { return mode == CPT_MODE_BDPT_ADAPTIVE_PLL || mode == CPT_MODE_BDPT_ADAPTIVE_BINARY; }
 
 struct Attrib {
@@ -688,8 +714,8 @@ constexpr u32 kMaxInFlight = 90;   // 30 three-frame light-cache groups + partia
 
 struct cpt_mlp {
   hipStream_t stream = nullptr;
-  DevArray<float> w, m, v, grad, loss, b1, b2, w3, b3, part, loss_part, part2;
-  DevArray<uint16_t> w1t, w2, w2t;
+  DevArray<float> w, m, v, grad, loss, b1, b2, w3, b3, part, loss_part;
+  DevArray<uint16_t> w1f, w2tf, w2f;
   int step = 0;
   ~cpt_mlp() {
     if (stream) (void)hipStreamDestroy(stream);
@@ -697,7 +723,7 @@ struct cpt_mlp {
   MlpState state() {
     MlpState s;
     s.w = w.p; s.m = m.p; s.v = v.p; s.grad = grad.p; s.loss = loss.p;
-    s.dev.w1t = w1t.p; s.dev.w2 = w2.p; s.dev.w2t = w2t.p;
+    s.dev.w1f = w1f.p; s.dev.w2tf = w2tf.p; s.dev.w2f = w2f.p;
     s.dev.b1 = b1.p; s.dev.b2 = b2.p; s.dev.w3 = w3.p; s.dev.b3 = b3.p;
     return s;
   }
@@ -781,6 +807,7 @@ struct cpt_renderer {
   cpt_mlp* mlp = nullptr;              // learned error model (adaptive mode)
   DevArray<float> mlp_x, mlp_y, mlp_e, sq_f, sq_e;
   u32 cap = 0;
+  u32 slot_margin = 1024;              // adaptive path slots beyond W*H*pass frames (rounding of the counts)
   long long slot_group[kLightSlots];   // three-frame group whose light cache each slot holds (-1: none)
   void forget_light_caches() { for (auto& g : slot_group) g = -1; }
   SampleRec rec() {
@@ -797,7 +824,8 @@ int renderer_alloc(cpt_renderer* r) {
   // adaptive frames draw floor(W*H*e_i/sum e) samples: at most W*H plus float
   // rounding of the ratios, bounded by the margin (and clamped on the device);
   // plain frames trace "Frames In Flight" frames as one wavefront
-  size_t cap = is_adaptive(r->mode) ? n * (size_t)r->attr(kPassFrames) + 1024 : n * (size_t)r->attr(kInFlight);
+  size_t cap = is_adaptive(r->mode) ? n * (size_t)r->attr(kPassFrames) + r->slot_margin
+                                    : n * (size_t)r->attr(kInFlight);
   if (cap > 0xFFFFFFFFull / 2) return fail(CPT_E_LIMIT, "render: %zu path slots exceed 2^31", cap);
   r->cap = (u32)cap;
   r->forget_light_caches();
@@ -1028,18 +1056,25 @@ void join_side(cpt_renderer* r) {
 }
 
 // One frame of cudaRTBDPTStreamAdapBilatAtomic::Render (AdapBilatAtomic.cu:1338-1538).
-MlpFeatureArgs feature_args(const cpt_renderer* r) {
+// (off, cam: the first pixel and the camera of one view of a multi-view batch)
+MlpFeatureArgs feature_args(const cpt_renderer* r, size_t off = 0, const float* cam = nullptr) {
   MlpFeatureArgs a;
-  a.result = r->result.p; a.filtered = r->filtered.p; a.err = r->err.p; a.sq = r->sq.p; a.sq_f = r->sq_f.p;
-  a.gpos = r->gpos.p; a.gnrm = r->gnrm.p; a.gdif = r->gdif.p;
-  a.sample_n = r->sample_n.p;
-  a.cam[0] = r->cam[0]; a.cam[1] = r->cam[1]; a.cam[2] = r->cam[2];
+  const size_t o3 = 3 * off;
+  a.result = r->result.p + o3; a.filtered = r->filtered.p + o3; a.err = r->err.p + off; a.sq = r->sq.p + o3;
+  a.sq_f = r->sq_f.p + o3;
+  a.gpos = r->gpos.p + o3; a.gnrm = r->gnrm.p + o3; a.gdif = r->gdif.p + o3;
+  a.sample_n = r->sample_n.p + off;
+  if (!cam) cam = r->cam;
+  a.cam[0] = cam[0]; a.cam[1] = cam[1]; a.cam[2] = cam[2];
   return a;
 }
 
 // Cross-bilateral filter of the running mean and the weighted squared residual
 // (src/cudaRTBDPTStreamAdapBilatAtomic.cu:1409-1430) into r->filtered / r->err.
-int run_filter(cpt_renderer* r, const float* src, float* dst, float* err) {
+// One W x H image starting at pixel `off` of the renderer's buffers, seen
+// from cam (a view of a multi-view batch; the renderer's own image: 0, W, H).
+int run_filter_at(cpt_renderer* r, size_t off, u32 W, u32 H, const float* cam, const float* src, float* dst,
+                  float* err) {
   float sc_ = (float)r->attr("Filter Color EuD"), sp_ = (float)r->attr("Filter Pos EuD");
   float sn_ = (float)r->attr("Filter Norm EuD"), sd_ = (float)r->attr("Filter Diff EuD");
   u32 radius = (u32)r->attr("Filter Radius");
@@ -1049,16 +1084,20 @@ int run_filter(cpt_renderer* r, const float* src, float* dst, float* err) {
   const float ss = (float)r->attr(kFilterSpatial);
   int rc = update_gauss(r, ss > 0.f ? ss : sc_, radius);
   if (rc) return rc;
-  FilterArgs f = filter_args(r->W, r->H, sc_, sp_, sn_, sd_, radius, (u32)r->attr("Desired Max Sampling"),
+  FilterArgs f = filter_args(W, H, sc_, sp_, sn_, sd_, radius, (u32)r->attr("Desired Max Sampling"),
                              r->attr("Filter Diffuse Flag") != 0);
   const float sz = (float)r->attr(kFilterDepth);
   if (sz > 0.f) {
     f.kz = 2.f * sz * sz;
-    f.cam[0] = r->cam[0]; f.cam[1] = r->cam[1]; f.cam[2] = r->cam[2];
+    f.cam[0] = cam[0]; f.cam[1] = cam[1]; f.cam[2] = cam[2];
   }
-  HIP_TRY(launch_filter_error(f, src, r->gpos.p, r->gnrm.p, r->gdif.p, r->gauss.p, r->sample_n.p, dst, err,
-                              r->stream));
+  const size_t o3 = 3 * off;
+  HIP_TRY(launch_filter_error(f, src + o3, r->gpos.p + o3, r->gnrm.p + o3, r->gdif.p + o3, r->gauss.p,
+                              r->sample_n.p + off, dst + o3, err + off, r->stream));
   return CPT_OK;
+}
+int run_filter(cpt_renderer* r, const float* src, float* dst, float* err) {
+  return run_filter_at(r, 0, r->W, r->H, r->cam, src, dst, err);
 }
 int run_filter(cpt_renderer* r) { return run_filter(r, r->result.p, r->filtered.p, r->err.p); }
 
@@ -1122,9 +1161,26 @@ u32 warmup_frames(const cpt_renderer* r) {
   return (u32)std::max(r->attr(kWarmup), r->attr(kMinPaths));
 }
 
-int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
+// One view of a multi-view adaptive batch: its first pixel in the
+// concatenated buffers, its size and its camera position.
+struct ViewSpan {
+  size_t off;
+  u32 W, H;
+  const float* cam;
+};
+// Per-view allocation state of a multi-view adaptive batch: error sums and
+// path totals (one per view), which become the views' queue bases.
+struct ViewAlloc {
+  const std::vector<ViewSpan>* views;
+  float* sums;
+  u32* totals;
+};
+
+int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa, const ViewAlloc* va = nullptr) {
   DevScene sc = s->dev();
   const u32 n = r->W * r->H;
+  const std::vector<ViewSpan> one{{0, r->W, r->H, r->cam}};
+  const std::vector<ViewSpan>& views = va ? *va->views : one;
   hipEvent_t ev = nullptr;
   LightJobs jobs = light_jobs(r, fa.frame0, 1);
   if (jobs.n) {
@@ -1152,7 +1208,6 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   } else {
     int rc = CPT_OK;
     const u32 nd = (u32)r->attr("Desired Max Sampling");
-    const u32 gen = n * fa.nframes;                 // queue slots of this pass
     const float* alloc_err = r->err.p;
     if (r->mode == CPT_MODE_BDPT_ADAPTIVE_EMP) {    // AdapEmp.cu:1209
       HIP_TRY(timed_begin(r, 6, &ev));
@@ -1160,14 +1215,22 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
       HIP_TRY(timed_end(r, 6, ev));
     } else {
       HIP_TRY(timed_begin(r, 5, &ev));
-      rc = run_filter(r);
-      if (rc) return rc;
+      for (const ViewSpan& v : views) {
+        rc = run_filter_at(r, v.off, v.W, v.H, v.cam, r->result.p, r->filtered.p, r->err.p);
+        if (rc) return rc;
+      }
       HIP_TRY(timed_end(r, 5, ev));
     }
     if (r->mlp) {                                  // learned estimate replaces the filter residual
       HIP_TRY(timed_begin(r, 9, &ev));
-      rc = model_features(r, r->mlp_x.p);
+      rc = model_buffers(r);
       if (rc) return rc;
+      for (const ViewSpan& v : views) {            // second moment filtered like the image, then the features
+        rc = run_filter_at(r, v.off, v.W, v.H, v.cam, r->sq.p, r->sq_f.p, r->sq_e.p);
+        if (rc) return rc;
+        HIP_TRY(launch_mlp_features(feature_args(r, v.off, v.cam), v.W * v.H, r->mlp_x.p + v.off * kMlpIn,
+                                    r->stream));
+      }
       HIP_TRY(launch_mlp_infer(r->mlp->state(), r->mlp_x.p, n, r->mlp_y.p, r->stream));
       HIP_TRY(launch_mlp_err(r->mlp_y.p, r->sample_n.p, nd, (u32)r->attr(kLearnedSignal), n, r->err.p, r->mlp_e.p,
                              r->stream));
@@ -1176,16 +1239,32 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
     }
     // pixels that have their "Max Pixel Paths" get no weight in the allocation
     HIP_TRY(launch_cap_error(const_cast<float*>(alloc_err), r->paths.p, maxp, n, r->stream));
-    if (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL) {     // proportional counts (AdapBilatAtomic.cu:817-842)
+    if (r->mode == CPT_MODE_BDPT_ADAPTIVE_PLL && !va) {   // proportional counts (AdapBilatAtomic.cu:817-842)
       HIP_TRY(timed_begin(r, 6, &ev));
       HIP_TRY(launch_error_sum(alloc_err, n, r->block_sums.p, r->err_sum.p, r->stream));
       HIP_TRY(timed_end(r, 6, ev));
       HIP_TRY(timed_begin(r, 7, &ev));
-      HIP_TRY(launch_alloc_scan(alloc_err, r->err_sum.p, n, gen, r->counts.p, r->offsets.p, r->block_tot.p,
-                                r->total.p, r->cap, r->paths.p, maxp, r->stream));
+      HIP_TRY(launch_alloc_scan(alloc_err, r->err_sum.p, n, n * fa.nframes, r->counts.p, r->offsets.p,
+                                r->block_tot.p, r->total.p, r->cap, r->paths.p, maxp, r->stream));
+    } else if (va) {                                 // the same per view, then one queue over all views
+      HIP_TRY(timed_begin(r, 6, &ev));
+      for (size_t k = 0; k < views.size(); k++)
+        HIP_TRY(launch_error_sum(alloc_err + views[k].off, views[k].W * views[k].H, r->block_sums.p, va->sums + k,
+                                 r->stream));
+      HIP_TRY(timed_end(r, 6, ev));
+      HIP_TRY(timed_begin(r, 7, &ev));
+      for (size_t k = 0; k < views.size(); k++) {
+        const ViewSpan& v = views[k];
+        const u32 nv = v.W * v.H;
+        HIP_TRY(launch_alloc_scan(alloc_err + v.off, va->sums + k, nv, nv * fa.nframes, r->counts.p + v.off,
+                                  r->offsets.p + v.off, r->block_tot.p, va->totals + k, r->cap, r->paths.p + v.off,
+                                  maxp, r->stream));
+      }
+      HIP_TRY(launch_view_bases(va->totals, (u32)views.size(), r->total.p, r->stream));
+      HIP_TRY(launch_offset_views(r->offsets.p, n, fa, va->totals, r->stream));
     } else {                                         // rounds (AdapBilatBinary.cu:1348-1388, AdapEmp.cu:1232-1260)
       HIP_TRY(timed_begin(r, 7, &ev));
-      rc = allocate_rounds(r, fa, alloc_err, gen, nd, maxp);
+      rc = allocate_rounds(r, fa, alloc_err, n * fa.nframes, nd, maxp);
       if (rc) return rc;
     }
     HIP_TRY(launch_add_paths(r->paths.p, r->counts.p, n, 0, r->stream));
@@ -1212,10 +1291,11 @@ int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   DevScene sc = s->dev();
   const u32 n = r->W * r->H * fa.nframes;
   hipEvent_t ev = nullptr;
-  const bool pt = is_pt(r->mode), regen = r->mode == CPT_MODE_PT_STREAM_REGEN;
+  const bool pt = is_pt(r->mode), regen = r->mode == CPT_MODE_PT_STREAM_REGEN, disney = is_disney(r->mode);
   LightJobs jobs;
   std::memset(&jobs, 0, sizeof jobs);
-  if (!pt) jobs = light_jobs(r, fa.frame0, fa.nframes);   // the path tracers have no light pass
+  // the path tracers have no light pass (the Disney mode samples the emitters directly)
+  if (!pt && !disney) jobs = light_jobs(r, fa.frame0, fa.nframes);
   const bool mega = !pt && r->attr(kMegakernel) != 0;
   if (jobs.n) {
     // The first extension reads no light vertex: with the side stream the
@@ -1245,7 +1325,7 @@ int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   ac.sample_n = r->sample_n.p;
   ac.rec.acc = r->rec_acc.p;
   ac.apply = 2;
-  ac.variant = pt ? kVariantPT | (regen ? kVariantPTRegen : 0) : 0;
+  ac.variant = pt ? kVariantPT | (regen ? kVariantPTRegen : 0) : disney ? kVariantDisney : 0;
   if (regen) {
     ac.term_bounce = r->term_bounce.p;
     ac.term_rng = r->term_rng.p;
@@ -1336,7 +1416,7 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   if (in_flight < 1 || in_flight > kMaxInFlight)
     return fail(CPT_E_LIMIT, "render: %s %u not in 1..%u", adaptive ? kPassFrames : kInFlight, in_flight,
                 kMaxInFlight);
-  if (!r->allocated || (size_t)r->cap < (size_t)r->W * r->H * in_flight + (adaptive ? 1024 : 0)) {
+  if (!r->allocated || (size_t)r->cap < (size_t)r->W * r->H * in_flight + (adaptive ? r->slot_margin : 0)) {
     int rc = renderer_alloc(r);
     if (rc) return rc;
   }
@@ -1393,13 +1473,53 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   return CPT_OK;
 }
 
+// Pixels off .. off + n of one buffer (CPT_BUF_*) to the host.
+int read_pixels(cpt_renderer* r, int which, size_t off, size_t n, void* out, const char* fn) {
+  if (!r->allocated) return fail(CPT_E_STATE, "%s: nothing rendered yet", fn);
+  const void* src = nullptr;
+  size_t per = 4;                                  // bytes per pixel
+  switch (which) {
+    case CPT_BUF_RESULT: src = r->result.p + 3 * off; per = 12; break;
+    case CPT_BUF_SAMPLE_N: src = r->sample_n.p + off; break;
+    case CPT_BUF_GBUF_POS:
+    case CPT_BUF_GBUF_NORM:
+    case CPT_BUF_GBUF_DIFF:
+    case CPT_BUF_FILTERED:
+    case CPT_BUF_ERROR:
+    case CPT_BUF_SAMPLE_COUNTS:
+    case CPT_BUF_PIXEL_PATHS:
+    case CPT_BUF_SECOND_MOMENT:
+    case CPT_BUF_ALLOC_ERROR:
+      if (!is_adaptive(r->mode)) return fail(CPT_E_ARG, "%s: buffer %d exists only in the adaptive modes", fn, which);
+      if (which == CPT_BUF_FILTERED && !uses_filter(r->mode))
+        return fail(CPT_E_ARG, "%s: the empirical-variance mode does not filter", fn);
+      switch (which) {
+        case CPT_BUF_GBUF_POS: src = r->gpos.p + 3 * off; per = 12; break;
+        case CPT_BUF_GBUF_NORM: src = r->gnrm.p + 3 * off; per = 12; break;
+        case CPT_BUF_GBUF_DIFF: src = r->gdif.p + 3 * off; per = 12; break;
+        case CPT_BUF_FILTERED: src = r->filtered.p + 3 * off; per = 12; break;
+        case CPT_BUF_SECOND_MOMENT: src = r->sq.p + 3 * off; per = 12; break;
+        case CPT_BUF_ERROR: src = r->err.p + off; break;
+        case CPT_BUF_SAMPLE_COUNTS: src = r->counts.p + off; break;
+        case CPT_BUF_PIXEL_PATHS: src = r->paths.p + off; break;
+        default: src = (r->mlp && r->mlp_e.p ? r->mlp_e.p : r->err.p) + off; break;
+      }
+      break;
+    default:
+      return fail(CPT_E_ARG, "%s: buffer %d not produced by this mode", fn, which);
+  }
+  HIP_TRY(hipMemcpyAsync(out, src, n * per, hipMemcpyDeviceToHost, r->stream));
+  HIP_TRY(hipStreamSynchronize(r->stream));
+  return CPT_OK;
+}
+
 }  // namespace
 
 extern "C" {
 
 int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t height) {
   if (!out) return fail(CPT_E_ARG, "cpt_renderer_create: null output");
-  if (mode < CPT_MODE_BDPT_STREAM || mode > CPT_MODE_PT_STREAM_REGEN)
+  if (mode < CPT_MODE_BDPT_STREAM || mode > CPT_MODE_PT_DISNEY_MIS)
     return fail(CPT_E_ARG, "cpt_renderer_create: unknown mode %d", mode);
   if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 26))
     return fail(CPT_E_LIMIT, "cpt_renderer_create: %ux%u outside 1..2^26 pixels", width, height);
@@ -1424,7 +1544,7 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
                   {"Avg Accum Time", 3, 0}};
   }
   // NORMALRAY_BOUND_MAX: 5 in the BDPT files, 10 in the path tracers (cudaRTPTStream.cu:7)
-  r->attribs.push_back({kEyeDepth, 0, is_pt(mode) ? 10.0 : 5.0});
+  r->attribs.push_back({kEyeDepth, 0, is_pt(mode) ? 10.0 : is_disney(mode) ? 8.0 : 5.0});
   r->attribs.push_back({kLightDepth, 0, 5});
   r->attribs.push_back({kLightPaths, 0, 128});
   r->attribs.push_back({kSeed, 0, (double)0x5EED0001u});
@@ -1507,56 +1627,7 @@ int cpt_renderer_reset(cpt_renderer* r) {
 
 int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out) {
   if (!r || !out) return fail(CPT_E_ARG, "cpt_renderer_read_buffer: null argument");
-  if (!r->allocated) return fail(CPT_E_STATE, "cpt_renderer_read_buffer: nothing rendered yet");
-  size_t n = (size_t)r->W * r->H;
-  switch (which) {
-    case CPT_BUF_RESULT:
-      HIP_TRY(hipMemcpyAsync(out, r->result.p, n * 12, hipMemcpyDeviceToHost, r->stream));
-      break;
-    case CPT_BUF_SAMPLE_N:
-      HIP_TRY(hipMemcpyAsync(out, r->sample_n.p, n * 4, hipMemcpyDeviceToHost, r->stream));
-      break;
-    case CPT_BUF_GBUF_POS:
-    case CPT_BUF_GBUF_NORM:
-    case CPT_BUF_GBUF_DIFF:
-    case CPT_BUF_FILTERED:
-    case CPT_BUF_ERROR:
-    case CPT_BUF_SAMPLE_COUNTS: {
-      if (!is_adaptive(r->mode))
-        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive modes", which);
-      if (which == CPT_BUF_FILTERED && !uses_filter(r->mode))
-        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: the empirical-variance mode does not filter");
-      const void* src = which == CPT_BUF_GBUF_POS ? (const void*)r->gpos.p
-                      : which == CPT_BUF_GBUF_NORM ? (const void*)r->gnrm.p
-                      : which == CPT_BUF_GBUF_DIFF ? (const void*)r->gdif.p
-                      : which == CPT_BUF_FILTERED ? (const void*)r->filtered.p
-                      : which == CPT_BUF_ERROR ? (const void*)r->err.p : (const void*)r->counts.p;
-      size_t bytes = (which == CPT_BUF_ERROR || which == CPT_BUF_SAMPLE_COUNTS) ? n * 4 : n * 12;
-      HIP_TRY(hipMemcpyAsync(out, src, bytes, hipMemcpyDeviceToHost, r->stream));
-      break;
-    }
-    case CPT_BUF_PIXEL_PATHS:
-      if (!is_adaptive(r->mode))
-        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive modes", which);
-      HIP_TRY(hipMemcpyAsync(out, r->paths.p, n * 4, hipMemcpyDeviceToHost, r->stream));
-      break;
-    case CPT_BUF_SECOND_MOMENT:
-      if (!is_adaptive(r->mode))
-        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive modes", which);
-      HIP_TRY(hipMemcpyAsync(out, r->sq.p, n * 12, hipMemcpyDeviceToHost, r->stream));
-      break;
-    case CPT_BUF_ALLOC_ERROR: {
-      if (!is_adaptive(r->mode))
-        return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d exists only in the adaptive modes", which);
-      const float* src = r->mlp && r->mlp_e.p ? r->mlp_e.p : r->err.p;
-      HIP_TRY(hipMemcpyAsync(out, src, n * 4, hipMemcpyDeviceToHost, r->stream));
-      break;
-    }
-    default:
-      return fail(CPT_E_ARG, "cpt_renderer_read_buffer: buffer %d not produced by this mode", which);
-  }
-  HIP_TRY(hipStreamSynchronize(r->stream));
-  return CPT_OK;
+  return read_pixels(r, which, 0, (size_t)r->W * r->H, out, "cpt_renderer_read_buffer");
 }
 
 int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset) {
@@ -1582,6 +1653,8 @@ int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset) {
   out->extend_tris = h.ext_tris;
   out->shadow_nodes = h.sh_nodes;
   out->shadow_tris = h.sh_tris;
+  out->extend_spills = h.ext_spill;
+  out->shadow_spills = h.sh_spill;
   out->extend_launches = r->extend_launches;
   out->extend_ms = r->ms[0];
   out->shadow_ms = r->ms[1];
@@ -1649,6 +1722,19 @@ int mlp_upload(cpt_mlp* m, const float* params) {
   return CPT_OK;
 }
 
+// One training step on `stream`: forward + backward into per-workgroup
+// partials, then the fixed-order sum and (apply) Adam.
+int mlp_step(cpt_mlp* m, const float* d_x, const float* d_target, uint32_t n, float lr, int apply, hipStream_t st) {
+  const u32 nwg = mlp_train_groups(n);
+  HIP_TRY(m->part.alloc((size_t)nwg * kMlpParams));
+  HIP_TRY(m->loss_part.alloc(nwg));
+  MlpState s = m->state();
+  HIP_TRY(launch_mlp_train(s, d_x, d_target, n, m->part.p, m->loss_part.p, st));
+  int step = apply ? m->step + 1 : (m->step > 0 ? m->step : 1);
+  HIP_TRY(launch_mlp_adam(s, m->part.p, nwg, m->loss_part.p, lr, step, apply != 0, st));
+  if (apply) m->step = step;
+  return CPT_OK;
+}
 }  // namespace
 
 extern "C" {
@@ -1663,10 +1749,10 @@ int cpt_mlp_create(cpt_mlp** out, uint64_t seed) {
   for (auto* a : {&m->w, &m->m, &m->v, &m->grad})
     if (a->alloc(kMlpParams) != hipSuccess) return bad(fail(CPT_E_NOMEM, "cpt_mlp_create: device allocation"));
   if (m->loss.alloc(1) || m->b1.alloc(64) || m->b2.alloc(64) || m->w3.alloc(64) || m->b3.alloc(1) ||
-      m->w1t.alloc(64 * 32) || m->w2.alloc(64 * 64) || m->w2t.alloc(64 * 64))
+      m->w1f.alloc(kMlpW1Frag) || m->w2tf.alloc(kMlpW2Frag) || m->w2f.alloc(kMlpW2Frag))
     return bad(fail(CPT_E_NOMEM, "cpt_mlp_create: device allocation"));
-  // feature columns 9..31 of W1^T stay zero
-  if (hipMemset(m->w1t.p, 0, sizeof(uint16_t) * 64 * 32) != hipSuccess)
+  // the K padding of layer 1 (features 9..15) stays zero
+  if (hipMemset(m->w1f.p, 0, sizeof(uint16_t) * kMlpW1Frag) != hipSuccess)
     return bad(fail(CPT_E_HIP, "cpt_mlp_create: hipMemset"));
   std::vector<float> p(kMlpParams, 0.f);
   HostPcg rng(seed, 0);
@@ -1718,20 +1804,36 @@ int cpt_mlp_train_step(cpt_mlp* m, const float* d_x, const float* d_target, uint
   if (n > (1u << 24)) return fail(CPT_E_LIMIT, "cpt_mlp_train_step: batch %u exceeds 2^24", n);
   if (!(lr >= 0.f)) return fail(CPT_E_ARG, "cpt_mlp_train_step: learning rate must be >= 0");
   hipStream_t st = stream ? (hipStream_t)stream : m->stream;
-  u32 nwg = (n + 63) / 64;
-  HIP_TRY(m->part.alloc((size_t)nwg * kMlpParams));
-  HIP_TRY(m->loss_part.alloc(nwg));
-  HIP_TRY(m->part2.alloc((size_t)mlp_reduce_chunks(nwg) * kMlpParams));
-  MlpState s = m->state();
-  HIP_TRY(launch_mlp_train(s, d_x, d_target, n, m->part.p, m->loss_part.p, st));
-  int step = apply ? m->step + 1 : (m->step > 0 ? m->step : 1);
-  HIP_TRY(launch_mlp_adam(s, m->part.p, nwg, m->loss_part.p, lr, step, apply != 0, m->part2.p, st));
-  if (apply) m->step = step;
+  int rc = mlp_step(m, d_x, d_target, n, lr, apply, st);
+  if (rc) return rc;
   if (loss) {
     HIP_TRY(hipStreamSynchronize(st));
     float l = 0.f;
     HIP_TRY(hipMemcpy(&l, m->loss.p, sizeof l, hipMemcpyDeviceToHost));
     *loss = l / (float)n;
+  }
+  return CPT_OK;
+}
+
+int cpt_mlp_train_steps(cpt_mlp* m, const float* d_x, const float* d_target, uint32_t n_samples, uint32_t batch,
+                        uint32_t first_batch, uint32_t steps, float lr, float* loss, void* stream) {
+  if (!m || !d_x || !d_target) return fail(CPT_E_ARG, "cpt_mlp_train_steps: null argument");
+  if (batch == 0 || steps == 0 || n_samples < batch)
+    return fail(CPT_E_ARG, "cpt_mlp_train_steps: need 0 < batch <= samples and steps > 0");
+  if (batch > (1u << 24)) return fail(CPT_E_LIMIT, "cpt_mlp_train_steps: batch %u exceeds 2^24", batch);
+  if (!(lr >= 0.f)) return fail(CPT_E_ARG, "cpt_mlp_train_steps: learning rate must be >= 0");
+  hipStream_t st = stream ? (hipStream_t)stream : m->stream;
+  const uint32_t nb = n_samples / batch;
+  for (uint32_t k = 0; k < steps; k++) {
+    const size_t off = (size_t)((first_batch + k) % nb) * batch;
+    int rc = mlp_step(m, d_x + off * kMlpIn, d_target + off, batch, lr, 1, st);
+    if (rc) return rc;
+  }
+  if (loss) {
+    HIP_TRY(hipStreamSynchronize(st));
+    float l = 0.f;
+    HIP_TRY(hipMemcpy(&l, m->loss.p, sizeof l, hipMemcpyDeviceToHost));
+    *loss = l / (float)batch;
   }
   return CPT_OK;
 }
@@ -1797,6 +1899,12 @@ struct cpt_batch {
   int cus = 0;
   double ms = 0.0;
   hipEvent_t e0 = nullptr, e1 = nullptr;
+  // per-view adaptive sampling (cpt_batch_set_adaptive): the views as spans
+  // of the concatenated buffers, their error sums and path totals
+  bool adaptive = false;
+  std::vector<ViewSpan> spans;
+  DevArray<float> vsum;
+  DevArray<u32> vtot;
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -1928,7 +2036,9 @@ int batch_prepare(cpt_batch* b, cpt_scene* s) {
     b->mat_v = s->mat_version;
   }
   cpt_renderer* r = b->r;
-  if (!r->allocated || (size_t)r->cap < (size_t)b->total * b->cfg.frames_in_flight) {
+  const size_t need = b->adaptive ? (size_t)b->total * (size_t)r->attr(kPassFrames) + r->slot_margin
+                                  : (size_t)b->total * b->cfg.frames_in_flight;
+  if (!r->allocated || (size_t)r->cap < need) {
     int rc = renderer_alloc(r);
     if (rc) return rc;
   }
@@ -1943,8 +2053,69 @@ int batch_prepare(cpt_batch* b, cpt_scene* s) {
     u32 zero = 0;
     HIP_TRY(hipMemcpyAsync(b->frame_dev.p, &zero, sizeof zero, hipMemcpyHostToDevice, r->stream));
     HIP_TRY(hipStreamSynchronize(r->stream));
+    r->started = false;                  // adaptive frames count on the host from 0 again
+    r->forget_light_caches();
     b->started = true;
   }
+  return CPT_OK;
+}
+
+// Per-view adaptive frames (cpt_batch_set_adaptive): the frame loop of a
+// one-view adaptive renderer over every view at once.  Warm-up frames trace
+// one path per pixel of every view; then per pass each view allocates
+// pass_frames x its pixel count paths over its own pixels (its own filter,
+// model features and error sum), and all views' paths are traced as one
+// wavefront.  Host-driven (no graph): a pass reads nothing back, but its
+// shape changes after the warm-up.
+int batch_render_adaptive(cpt_batch* b, cpt_scene* s, u32 frames) {
+  cpt_renderer* r = b->r;
+  const u32 radius = (u32)r->attr("Filter Radius");
+  if (radius > (u32)kMaxFilterRadius) return fail(CPT_E_LIMIT, "cpt_batch_render: filter radius %u exceeds 31", radius);
+  if (r->attr(kWarmup) < 1) return fail(CPT_E_LIMIT, "cpt_batch_render: warm-up frames must be >= 1");
+  const u32 pass = (u32)r->attr(kPassFrames);
+  if (pass < 1 || pass > kMaxInFlight)
+    return fail(CPT_E_LIMIT, "cpt_batch_render: %s %u not in 1..%u", kPassFrames, pass, kMaxInFlight);
+  int rc = batch_prepare(b, s);
+  if (rc) return rc;
+  if (r->mlp) {
+    rc = model_buffers(r);
+    if (rc) return rc;
+  }
+  FrameArgs fa;
+  std::memset(&fa, 0, sizeof fa);
+  fa.width = b->total;
+  fa.height = 1;
+  fa.eye_depth = b->cfg.eye_depth;
+  fa.light_depth = b->cfg.light_depth;
+  fa.light_paths = b->cfg.light_paths;
+  fa.seed = b->cfg.seed;
+  fa.lv_stride = b->cfg.light_paths * (b->cfg.light_depth + 1);
+  fa.flags = r->attr(kLiteralShadow) != 0 ? 1u : 0u;
+  fa.n_views = (u32)b->vd.size();
+  fa.views = b->d_views.p;
+  const ViewAlloc va{&b->spans, b->vsum.p, b->vtot.p};
+  HIP_TRY(hipEventRecord(b->e0, r->stream));
+  u32 next = r->started ? r->frame + 1 : 0u;
+  for (u32 done = 0; done < frames;) {
+    u32 k = std::min(pass, frames - done);
+    if (next < warmup_frames(r)) k = 1;
+    fa.frame0 = next;
+    fa.nframes = k;
+    rc = render_adaptive_frame(r, s, fa, &va);
+    if (rc) {
+      join_side(r);
+      return rc;
+    }
+    r->frame = next + k - 1;
+    r->started = true;
+    next += k;
+    done += k;
+  }
+  HIP_TRY(hipEventRecord(b->e1, r->stream));
+  HIP_TRY(hipEventSynchronize(b->e1));
+  float ms = 0.f;
+  HIP_TRY(hipEventElapsedTime(&ms, b->e0, b->e1));
+  b->ms += ms;
   return CPT_OK;
 }
 
@@ -1969,6 +2140,7 @@ int batch_capture(cpt_batch* b, cpt_scene* s, u32 k) {
 }
 
 int batch_render(cpt_batch* b, cpt_scene* s, u32 frames) {
+  if (b->adaptive) return batch_render_adaptive(b, s, frames);
   int rc = batch_prepare(b, s);
   if (rc) return rc;
   cpt_renderer* r = b->r;
@@ -2103,6 +2275,56 @@ int cpt_batch_sample_counts(cpt_batch* b, uint32_t view, uint32_t* out) {
   return CPT_OK;
 }
 
+int cpt_batch_read_buffer(cpt_batch* b, uint32_t view, int which, void* out) {
+  if (!b || !out) return fail(CPT_E_ARG, "cpt_batch_read_buffer: null argument");
+  if (view >= b->vd.size()) return fail(CPT_E_ARG, "cpt_batch_read_buffer: view %u out of range", view);
+  const ViewDesc& d = b->vd[view];
+  return read_pixels(b->r, which, d.offset, (size_t)d.width * d.height, out, "cpt_batch_read_buffer");
+}
+
+int cpt_batch_set_adaptive(cpt_batch* b, int mode, uint32_t pass_frames, cpt_mlp* model) {
+  if (!b) return fail(CPT_E_ARG, "cpt_batch_set_adaptive: null batch");
+  if (mode != CPT_MODE_BDPT_STREAM && mode != CPT_MODE_BDPT_ADAPTIVE_PLL)
+    return fail(CPT_E_ARG, "cpt_batch_set_adaptive: mode %d is neither plain streaming BDPT (0) nor the proportional "
+                           "adaptive mode (1)", mode);
+  const bool on = mode == CPT_MODE_BDPT_ADAPTIVE_PLL;
+  if (on && (pass_frames < 1 || pass_frames > kMaxInFlight))
+    return fail(CPT_E_LIMIT, "cpt_batch_set_adaptive: pass frames %u not in 1..%u", pass_frames, kMaxInFlight);
+  if (!on && model) return fail(CPT_E_ARG, "cpt_batch_set_adaptive: a model needs the adaptive mode");
+  const u32 nv = (u32)b->vd.size();
+  if (on && (b->vsum.alloc(nv) != hipSuccess || b->vtot.alloc(nv) != hipSuccess))
+    return fail(CPT_E_NOMEM, "cpt_batch_set_adaptive: device allocation");
+  cpt_renderer* r = nullptr;
+  int rc = cpt_renderer_create(&r, mode, b->total, 1);
+  if (rc) return rc;
+  cpt_renderer_destroy(b->r);
+  b->r = r;
+  b->adaptive = on;
+  b->spans.clear();
+  for (const ViewDesc& d : b->vd) b->spans.push_back(ViewSpan{d.offset, d.width, d.height, d.cam_pos});
+  r->slot_margin = 1024u * nv;              // each view's counts may round past its budget
+  r->set_attr(kPassFrames, on ? pass_frames : 1u);
+  r->mlp = model;
+  batch_apply_config(b);
+  return CPT_OK;
+}
+
+int cpt_batch_set_attrib(cpt_batch* b, const char* name, double value) {
+  if (!b || !name) return fail(CPT_E_ARG, "cpt_batch_set_attrib: null argument");
+  for (const char* own : {kEyeDepth, kLightDepth, kLightPaths, kSeed, kInFlight, kPassFrames, kLdsStack, kNodeFormat,
+                          kRefill, kRefillSecondary, kRefillGroups, kRefillMinPaths})
+    if (std::strcmp(name, own) == 0)
+      return fail(CPT_E_ARG, "cpt_batch_set_attrib: \"%s\" belongs to the batch configuration", name);
+  for (uint32_t i = 0; i < b->r->attribs.size(); i++)
+    if (std::strcmp(b->r->attribs[i].name, name) == 0) {
+      int rc = cpt_renderer_set_attrib(b->r, i, value);
+      if (rc) return rc;
+      b->started = false;                   // a changed estimator restarts accumulation
+      return CPT_OK;
+    }
+  return fail(CPT_E_ARG, "cpt_batch_set_attrib: no attribute \"%s\" in this batch's mode", name);
+}
+
 int cpt_batch_stats(cpt_batch* b, cpt_stats* out, int reset) {
   if (!b || !out) return fail(CPT_E_ARG, "cpt_batch_stats: null argument");
   int rc = cpt_renderer_stats(b->r, out, reset);
@@ -2132,6 +2354,9 @@ int cpt_batch_candidate(cpt_batch* b, uint32_t i, cpt_batch_config* cfg) {
 int cpt_batch_autotune(cpt_batch* b, cpt_scene* s, uint32_t iterations, cpt_batch_config* best, double* ms,
                        uint32_t n_ms) {
   if (!b || !s) return fail(CPT_E_ARG, "cpt_batch_autotune: null argument");
+  if (b->adaptive)
+    return fail(CPT_E_STATE, "cpt_batch_autotune: tune the plain batch, then call cpt_batch_set_adaptive (it keeps the "
+                             "schedule)");
   if (iterations < 1) return fail(CPT_E_ARG, "cpt_batch_autotune: iterations must be >= 1");
   const cpt_batch_config base = b->cfg;
   const u32 n = batch_candidate_count();

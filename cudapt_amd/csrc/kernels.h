@@ -39,6 +39,11 @@ struct DevScene {
   const float4* tex;           // texels (RGBA) of every texture, concatenated
   const uint4* tex_desc;       // per texture: first texel, width, height
   uint32_t n_tex;
+  // environment light (no reference counterpart): env_w x env_h RGB radiance
+  // texels (w unused) over directions in latitude-longitude order, row 0 at +y;
+  // env_w = 0: none
+  const float4* env;
+  uint32_t env_w, env_h;
 };
 
 // Per-path SoA state of the wavefront (one slot per live eye path).
@@ -126,6 +131,7 @@ struct DevStats {
   unsigned long long ext_rays, shadow_rays, eye_paths, frames;
   unsigned long long ext_nodes, ext_tris;
   unsigned long long sh_nodes, sh_tris;
+  unsigned long long ext_spill, sh_spill;   // stack pushes below the LDS top (k_trace_dyn)
 };
 
 constexpr int kMaxDepth = 15;
@@ -179,6 +185,7 @@ constexpr int kVariantReflOffset = 1;  // reflection offset of AdapBilatAtomic's
 constexpr int kVariantGbuf = 2;        // first-hit G-buffer
 constexpr int kVariantPT = 4;          // path tracing: no connection, total internal reflection reflects
 constexpr int kVariantPTRegen = 8;     // the regenerating path tracer's diffuse-to-specular cut
+constexpr int kVariantDisney = 16;     // Disney-BRDF path tracer with next-event estimation and MIS
 // eye != 0: the bounce's paths are the eye paths k_gen_eye would have written
 // to cur (bounce 0 of a fresh batch); they are recomputed from their slot and
 // cur is not read.
@@ -220,6 +227,12 @@ hipError_t launch_alloc_scan(const float* err, const float* sum, uint32_t n, uin
 hipError_t launch_cap_error(float* err, const uint32_t* paths, uint32_t maxp, uint32_t n, hipStream_t st);
 hipError_t launch_cap_counts(uint32_t* counts, const uint32_t* paths, uint32_t maxp, uint32_t n, hipStream_t st);
 hipError_t launch_add_paths(uint32_t* paths, const uint32_t* counts, uint32_t n, int reset, hipStream_t st);
+// Multi-view adaptive batches: view_totals (one per view) become their
+// exclusive scan and *total their sum; each pixel's offset then adds its
+// view's base.
+hipError_t launch_view_bases(uint32_t* view_totals, uint32_t n_views, uint32_t* total, hipStream_t st);
+hipError_t launch_offset_views(uint32_t* offsets, uint32_t n, const FrameArgs& fa, const uint32_t* base,
+                               hipStream_t st);
 hipError_t launch_gen_adaptive(const FrameArgs& fa, const uint32_t* offsets, const uint32_t* counts,
                                const uint32_t* total, uint32_t cap, PathBuf pb, SampleRec rec, hipStream_t st);
 hipError_t launch_accum_adaptive(const uint32_t* offsets, const uint32_t* counts, uint32_t n, uint32_t cap,

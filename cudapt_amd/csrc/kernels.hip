@@ -203,9 +203,12 @@ __device__ __forceinline__ void ray_begin(RayState& R, f3 o, f3 d, float maxd, f
 // (t, original id) minimum, which neither the test order nor the conservative
 // culling changes.  Returns true when the ray is finished.
 // lstk points at this thread's column of a [STK][kBlock] u64 array in LDS.
+// spill (an LDS word of the wave, or null) counts pushes that land below the
+// LDS top, in the private (scratch) stack: an LDS atomic on that rare event
+// instead of a counter register in the loop.
 template <bool ANY, int STK, bool Q>
 __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool cull, LdsU64* lstk, uint64_t* pstk,
-                                         uint32_t& nn, uint32_t& nt) {
+                                         uint32_t& nn, uint32_t& nt, uint32_t* spill) {
   NodeView nv;
   load_node<Q>(sc, R.node, nv);
   nn++;
@@ -282,8 +285,12 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool c
 #pragma unroll
     for (int c = 3; c >= 1; c--) {
       if (key[c] != ~0ull && R.sp < CPT_STACK_TOTAL) {
-        if (R.sp < STK) lstk[R.sp * kBlock] = key[c];
-        else pstk[R.sp - STK] = key[c];
+        if (R.sp < STK) {
+          lstk[R.sp * kBlock] = key[c];
+        } else {
+          pstk[R.sp - STK] = key[c];
+          if (spill) atomicAdd(spill, 1u);
+        }
         R.sp++;
       }
     }
@@ -307,7 +314,7 @@ __device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps,
   RayState R;
   ray_begin(R, o, d, maxd, eps);
   uint64_t pstk[CPT_STACK_TOTAL - STK];
-  while (!ray_step<ANY, STK, Q>(sc, R, cull, lstk, pstk, nn, nt)) {
+  while (!ray_step<ANY, STK, Q>(sc, R, cull, lstk, pstk, nn, nt, nullptr)) {
   }
   h = R.h;
   return h.leaf != kInvalid;
@@ -602,6 +609,31 @@ __device__ __forceinline__ float4 tex_sample(const DevScene& sc, uint32_t id, fl
 // The hit's surface from its triangle's 128-byte shading record: normals and
 // material in one round trip after the hit; texture u and the vertex positions
 // only for textured materials and first-hit G-buffers.
+// Radiance the environment sends along direction d (no reference counterpart:
+// the reference lights scenes with emissive triangles only).  Latitude-longitude
+// map: u = atan2(d.z, d.x) / 2pi + 1/2, v = acos(d.y) / pi, texel centres at
+// half-integers, bilinear between the four around (u W - 1/2, v H - 1/2), u
+// wrapping and v clamped to the first and last rows.  A non-finite d sees 0.
+__device__ __forceinline__ f3 env_radiance(const DevScene& sc, f3 d) {
+  const int W = (int)sc.env_w, H = (int)sc.env_h;
+  const float u = atan2f(d.z, d.x) * (float)(0.5 / kPiD) + 0.5f;
+  const float v = acosf(fminf(fmaxf(d.y, -1.f), 1.f)) * (float)(1.0 / kPiD);
+  const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
+  if (!(fabsf(x) < 1e30f) || !(fabsf(y) < 1e30f)) return mk3(0.f, 0.f, 0.f);
+  const float fx = floorf(x), fy = floorf(y);
+  const float a = x - fx, b = y - fy;
+  int i0 = (int)fx % W;
+  if (i0 < 0) i0 += W;
+  const int i1 = i0 + 1 == W ? 0 : i0 + 1;
+  const int j0 = min(max((int)fy, 0), H - 1), j1 = min(max((int)fy + 1, 0), H - 1);
+  const float4 t00 = sc.env[j0 * W + i0], t10 = sc.env[j0 * W + i1];
+  const float4 t01 = sc.env[j1 * W + i0], t11 = sc.env[j1 * W + i1];
+  const float ia = 1.f - a, ib = 1.f - b;
+  return mk3((t00.x * ia + t10.x * a) * ib + (t01.x * ia + t11.x * a) * b,
+             (t00.y * ia + t10.y * a) * ib + (t01.y * ia + t11.y * a) * b,
+             (t00.z * ia + t10.z * a) * ib + (t01.z * ia + t11.z * a) * b);
+}
+
 __device__ __forceinline__ Surf fetch_surf(const DevScene& sc, uint32_t leaf, float w, float u, float v, bool want_pos) {
   Surf s;
   const float4* R = sc.surf + 8 * (size_t)leaf;
@@ -935,6 +967,10 @@ __device__ __forceinline__ void unpack_dk(PathState& p, uint32_t dk) {
 }
 // The connection's shadow ray: origin and max distance, direction and epsilon,
 // contribution added to the path's accumulator when nothing occludes it.
+// A shadow ray's target slot with this bit set adds its contribution without
+// the connection's N += 4 (the Disney mode's light samples are part of their
+// path's one sample).  Slots stay below 2^31.
+constexpr uint32_t kShadowNoCount = 1u << 31;
 struct ShadowRay {
   float4 o, d, c;
 };
@@ -1221,6 +1257,9 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
   uint32_t ray = 0;
   RayState R;
   uint32_t nn = 0, nt = 0;
+  __shared__ uint32_t spill_all[kBlock / 64];
+  uint32_t* spill = spill_all + __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  if (lane == 0) *spill = 0u;
   for (;;) {
     const uint64_t idle = __ballot(!active);
     const uint32_t n_idle = (uint32_t)__popcll(idle);
@@ -1236,12 +1275,12 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
         out[ray] = r;
       } else if (!hit) {
         const float4 c = contrib[ray];
-        const uint32_t slot = __float_as_uint(c.w);
+        const uint32_t bits = __float_as_uint(c.w), slot = bits & ~kShadowNoCount;
         float4 acc = nxt.acc[slot];
         acc.x = acc.x + c.x;
         acc.y = acc.y + c.y;
         acc.z = acc.z + c.z;
-        acc.w = __uint_as_float(__float_as_uint(acc.w) + 4u);
+        acc.w = __uint_as_float(__float_as_uint(acc.w) + ((bits & kShadowNoCount) ? 0u : 4u));
         nxt.acc[slot] = acc;
       }
     }
@@ -1277,16 +1316,18 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
     }
     // every lane idle with nothing left to take: the results were written above
     if (__ballot(active) == 0ull) break;
-    if (active && ray_step<ANY, STK, Q>(sc, R, false, lstk, pstk, nn, nt)) {
+    if (active && ray_step<ANY, STK, Q>(sc, R, false, lstk, pstk, nn, nt, spill)) {
       active = false;
       pending = true;
     }
   }
   if (stats) {
     uint32_t sn = wave_sum(nn), st = wave_sum(nt);
+    const uint32_t ss = *spill;         // the wave's own word, after its last atomic
     if (lane == 0 && (sn | st)) {
       atomicAdd(OUT == 1 ? &stats->ext_nodes : &stats->sh_nodes, (unsigned long long)sn);
       atomicAdd(OUT == 1 ? &stats->ext_tris : &stats->sh_tris, (unsigned long long)st);
+      if (ss) atomicAdd(OUT == 1 ? &stats->ext_spill : &stats->sh_spill, (unsigned long long)ss);
     }
   }
 }
@@ -1420,6 +1461,8 @@ __device__ __forceinline__ void shade_path(const DevScene& sc, const FrameArgs& 
   uint32_t leaf = __float_as_uint(h.y);
   bool term = false;
   if (leaf == kInvalid) {
+    // an escaping eye path gathers the environment's radiance through its throughput
+    if (sc.env_w) psample = psample + vmul(env_radiance(sc, dir), mul);
     acc = acc + psample; N++; term = true;
   } else {
     float w = (1.f - h.z) - h.w;
@@ -1509,6 +1552,236 @@ __device__ __forceinline__ void shade_path(const DevScene& sc, const FrameArgs& 
   p.rng = st;
 }
 
+// ---- Disney-BRDF path tracer (extension; no reference counterpart) ----------
+// The principled BRDF of Burley (SIGGRAPH 2012 course notes and brdf.txt),
+// isotropic, specular tint 0, opaque, with next-event estimation to the
+// emissive triangles and power-heuristic MIS against the BRDF samples.  The
+// same operations, in the same order and drawing the same uniforms, as the
+// oracle's FP64 restatement (disney_path).
+struct DisneyMat {
+  f3 base;
+  float metallic, subsurface, specular, roughness, sheen, sheenTint, clearcoat, clearcoatGloss;
+};
+__device__ __forceinline__ float schlick_w(float c) {
+  float m = fminf(fmaxf(1.f - c, 0.f), 1.f);
+  return (m * m) * (m * m) * m;
+}
+__device__ __forceinline__ float gtr1(float nh, float a) {
+  if (a >= 1.f) return (float)(1.0 / kPiD);
+  const float a2 = a * a, t = 1.f + (a2 - 1.f) * nh * nh;
+  return (a2 - 1.f) / (kPiF * logf(a2) * t);
+}
+__device__ __forceinline__ float gtr2(float nh, float a) {
+  const float a2 = a * a, t = 1.f + (a2 - 1.f) * nh * nh;
+  return a2 / (kPiF * t * t);
+}
+__device__ __forceinline__ float smith_g(float n, float a) {
+  const float a2 = a * a, b = n * n;
+  return 1.f / (n + sqrtf(a2 + b - a2 * b));
+}
+__device__ __forceinline__ float disney_alpha(const DisneyMat& m) { return fmaxf(m.roughness * m.roughness, 1e-3f); }
+__device__ __forceinline__ float disney_cc_alpha(const DisneyMat& m) {
+  return 0.1f + (0.001f - 0.1f) * m.clearcoatGloss;
+}
+__device__ __forceinline__ void disney_lobes(const DisneyMat& m, float& pd, float& ps, float& pc) {
+  const float wd = 1.f - m.metallic, ws = 1.f, wc = 0.25f * m.clearcoat, t = (wd + ws) + wc;
+  pd = wd / t; ps = ws / t; pc = wc / t;
+}
+__device__ f3 disney_f(const DisneyMat& m, f3 n, f3 wo, f3 wi) {
+  const float nl = dot(n, wi), nv = dot(n, wo);
+  if (nl <= 0.f || nv <= 0.f) return mk3(0.f, 0.f, 0.f);
+  const f3 hv = wi + wo;
+  const float hl = length(hv);
+  if (!(hl > 0.f)) return mk3(0.f, 0.f, 0.f);
+  const f3 h = hv / hl;
+  const float nh = dot(n, h), lh = dot(wi, h);
+  const float lum = 0.3f * m.base.x + 0.6f * m.base.y + 0.1f * m.base.z;
+  const f3 tint = lum > 0.f ? m.base / lum : mk3(1.f, 1.f, 1.f);
+  const f3 one = mk3(1.f, 1.f, 1.f);
+  const f3 cspec0 = one * (0.08f * m.specular) + (m.base - one * (0.08f * m.specular)) * m.metallic;
+  const f3 csheen = one + (tint - one) * m.sheenTint;
+  const float fl = schlick_w(nl), fv = schlick_w(nv), fh = schlick_w(lh);
+  const float fd90 = 0.5f + 2.f * lh * lh * m.roughness;
+  const float fd = (1.f + (fd90 - 1.f) * fl) * (1.f + (fd90 - 1.f) * fv);
+  const float fss90 = lh * lh * m.roughness;
+  const float fss = (1.f + (fss90 - 1.f) * fl) * (1.f + (fss90 - 1.f) * fv);
+  const float ss = 1.25f * (fss * (1.f / (nl + nv) - 0.5f) + 0.5f);
+  const float a = disney_alpha(m);
+  const float ds = gtr2(nh, a);
+  const f3 fs = cspec0 + (one - cspec0) * fh;
+  const float gs = smith_g(nl, a) * smith_g(nv, a);
+  const f3 fsheen = csheen * (fh * m.sheen);
+  const float dr = gtr1(nh, disney_cc_alpha(m));
+  const float fr = 0.04f + (1.f - 0.04f) * fh;
+  const float gr = smith_g(nl, 0.25f) * smith_g(nv, 0.25f);
+  const f3 diffuse = (m.base * ((float)(1.0 / kPiD) * (fd + (ss - fd) * m.subsurface)) + fsheen) * (1.f - m.metallic);
+  return (diffuse + fs * (gs * ds)) + one * (0.25f * m.clearcoat * gr * fr * dr);
+}
+__device__ float disney_pdf(const DisneyMat& m, f3 n, f3 wo, f3 wi) {
+  const float nl = dot(n, wi), nv = dot(n, wo);
+  if (nl <= 0.f || nv <= 0.f) return 0.f;
+  const f3 hv = wi + wo;
+  const float hl = length(hv);
+  if (!(hl > 0.f)) return 0.f;
+  const f3 h = hv / hl;
+  const float nh = dot(n, h), vh = dot(wo, h);
+  if (vh <= 0.f) return 0.f;
+  float pd, ps, pc;
+  disney_lobes(m, pd, ps, pc);
+  float pdf = pd * nl * (float)(1.0 / kPiD);
+  pdf = pdf + ps * gtr2(nh, disney_alpha(m)) * nh / (4.f * vh);
+  pdf = pdf + pc * gtr1(nh, disney_cc_alpha(m)) * nh / (4.f * vh);
+  return pdf;
+}
+__device__ bool disney_sample(const DisneyMat& m, f3 n, f3 wo, float u0, float u1, float u2, f3& wi) {
+  float pd, ps, pc;
+  disney_lobes(m, pd, ps, pc);
+  f3 u, v;
+  basis(n, u, v);
+  const float phi = (float)(2.0 * kPiD * (double)u1);
+  if (u0 < pd) {
+    const float ct = sqrtf(u2), st = sqrtf(1.f - u2);
+    wi = normalize((n * ct + (u * st) * cosf(phi)) + (v * st) * sinf(phi));
+  } else {
+    const bool spec = u0 < pd + ps;
+    const float a = spec ? disney_alpha(m) : disney_cc_alpha(m);
+    float ct;
+    if (spec) ct = sqrtf((1.f - u2) / (1.f + (a * a - 1.f) * u2));
+    else ct = a >= 1.f ? sqrtf(1.f - u2) : sqrtf((1.f - powf(a * a, 1.f - u2)) / (1.f - a * a));
+    const float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
+    const f3 h = (n * ct + (u * st) * cosf(phi)) + (v * st) * sinf(phi);
+    wi = normalize(h * (2.f * dot(wo, h)) - wo);
+  }
+  return dot(n, wi) > 0.f;
+}
+__device__ __forceinline__ DisneyMat disney_of(const DevScene& sc, uint32_t leaf, const Surf& s) {
+  DisneyMat m;
+  m.base = s.diff; m.metallic = s.metal; m.specular = s.spec; m.roughness = s.rough;
+  const uint32_t mi = __float_as_uint(sc.surf[8 * (size_t)leaf + 7].w);
+  const DevMaterial& d = sc.mats[mi];
+  m.subsurface = d.subsurface; m.sheen = d.sheen; m.sheenTint = d.sheenTint;
+  m.clearcoat = d.clearcoat; m.clearcoatGloss = d.clearcoatGloss;
+  return m;
+}
+// Solid-angle density of the light sampler at the point at distance t along
+// unit d on leaf triangle `leaf`: 1 / n_lights, uniform by area, two faces.
+__device__ __forceinline__ float light_pdf(const DevScene& sc, uint32_t leaf, float t, f3 d) {
+  const f3 ng = cross(ld3(sc.tris[3 * (size_t)leaf + 1]), ld3(sc.tris[3 * (size_t)leaf + 2]));
+  const float a2 = length(ng);
+  const float c = fabsf(dot(ng, d)) / a2;
+  if (!(c > 0.f) || !(a2 > 0.f)) return 0.f;
+  return t * t / (c * (a2 * 0.5f) * (float)sc.n_light_tris);
+}
+// One bounce of the Disney path tracer: emission (MIS-weighted against the
+// previous bounce's light sample), the depth bound, Russian roulette, the light
+// sample (a shadow ray adding to the path's accumulator without a count) and
+// the BRDF sample.  smp.w carries the BRDF density of the ray being traced.
+__device__ __forceinline__ void shade_disney(const DevScene& sc, const FrameArgs& fa, const Accum& ac, PathState& p,
+                                             float4 h, bool& survive, bool& emit, ShadowRay& sr) {
+  const uint32_t pix = p.pix, sid = p.sid;
+  uint32_t depth = p.depth;
+  const uint32_t frame = frame_base(fa) + p.fk;
+  uint64_t st = p.rng;
+  const uint64_t inc = ((uint64_t)local_pixel(fa, pix) << 1) | 1u;
+  f3 orig = ld3(p.ro), dir = ld3(p.rd), T = ld3(p.mul), L = ld3(p.smp), acc = ld3(p.acc);
+  const float prev_pdf = p.smp.w;
+  uint32_t N = __float_as_uint(p.acc.w);
+  const uint32_t leaf = __float_as_uint(h.y);
+  bool term = false, dead = false;
+  float next_pdf = 0.f;
+  if (leaf == kInvalid) {
+    if (sc.env_w) L = L + vmul(T, env_radiance(sc, dir));
+    term = true;
+  } else {
+    const float w = (1.f - h.z) - h.w;
+    const Surf s = fetch_surf(sc, leaf, w, h.z, h.w, false);
+    if (length(s.emissive) > 0.f) {
+      float wt = 1.f;
+      if (depth > 0) {
+        const float pl = light_pdf(sc, leaf, h.x, dir);
+        wt = prev_pdf * prev_pdf / (prev_pdf * prev_pdf + pl * pl);
+      }
+      L = L + vmul(T, s.emissive) * wt;
+    }
+    const uint32_t lim = p.lim ? p.lim : fa.eye_depth;
+    if (depth + 1 >= lim) term = true;
+    if (!term && depth >= 3) {
+      const float q = fminf(0.95f, fmaxf(T.x, fmaxf(T.y, T.z)));
+      if (uniform01(st, inc) >= q) term = true;
+      else T = T / q;
+    }
+    if (!term) {
+      const f3 nl = dot(s.norm, dir) < 0.f ? s.norm : s.norm * -1.f;
+      const f3 pos = orig + dir * h.x, wo = dir * -1.f;
+      const DisneyMat m = disney_of(sc, leaf, s);
+      const uint32_t nL = sc.n_light_tris;
+      if (nL > 0) {
+        const float l0 = uniform01(st, inc);
+        float l1 = uniform01(st, inc), l2 = uniform01(st, inc);
+        uint32_t li = (uint32_t)(l0 * (float)nL);
+        if (li >= nL) li = nL - 1;
+        if (l1 + l2 > 1.f) { l1 = 1.f - l1; l2 = 1.f - l2; }
+        const uint32_t lt = sc.light_tris[li];
+        const float4 q0 = sc.tris[3 * (size_t)lt], q1 = sc.tris[3 * (size_t)lt + 1], q2 = sc.tris[3 * (size_t)lt + 2];
+        const f3 lp = (ld3(q0) + ld3(q1) * l1) + ld3(q2) * l2;
+        const f3 so = pos + nl * (float)kBiasEps;
+        const f3 toL = lp - so;
+        const float dist = length(toL);
+        if ((double)dist > kFltEps) {
+          const f3 wi = toL / dist;
+          const float cs = dot(nl, wi);
+          const float pl = light_pdf(sc, lt, dist, wi);
+          if (cs > 0.f && pl > 0.f) {
+            const f3 f = disney_f(m, nl, wo, wi);
+            const float pb = disney_pdf(m, nl, wo, wi);
+            const float wt = pl * pl / (pl * pl + pb * pb);
+            const f3 le = ld3(sc.surf[8 * (size_t)lt + 4]);
+            const f3 c = vmul(vmul(T, f), le) * (cs * wt / pl);
+            if (length(c) > 0.f) {
+              emit = true;
+              sr.o = make_float4(so.x, so.y, so.z, (float)((double)dist - 2.0 * kBiasEps));
+              sr.d = make_float4(wi.x, wi.y, wi.z, (float)kBiasEps);
+              sr.c = make_float4(c.x, c.y, c.z, 0.f);
+            }
+          }
+        }
+      }
+      const float b0 = uniform01(st, inc), b1 = uniform01(st, inc), b2 = uniform01(st, inc);
+      f3 wi;
+      const bool ok = disney_sample(m, nl, wo, b0, b1, b2, wi);
+      const float pdf = ok ? disney_pdf(m, nl, wo, wi) : 0.f;
+      if (!(pdf > 0.f)) {
+        // no continuation; a pending light sample still needs the slot: the
+        // path lives one more, empty segment (maximum distance 0, throughput 0)
+        if (emit) dead = true;
+        else term = true;
+      } else {
+        T = vmul(T, disney_f(m, nl, wo, wi)) * (dot(nl, wi) / pdf);
+        next_pdf = pdf;
+        orig = pos + wi * (float)kBiasEps;
+        dir = wi;
+      }
+    }
+  }
+  if (term) {
+    acc = acc + L;
+    N++;
+    if (ac.apply == 0) apply_pixel(ac.result, ac.sample_n, pix, frame, acc, N);
+    else ac.rec.acc[sid] = make_float4(acc.x, acc.y, acc.z, __uint_as_float(N));
+  } else {
+    survive = true;
+    if (dead) T = mk3(0.f, 0.f, 0.f);
+    p.ro = make_float4(orig.x, orig.y, orig.z, dead ? 0.f : kInfF);
+    p.rd = make_float4(dir.x, dir.y, dir.z, (float)kBiasEps);
+    p.mul = make_float4(T.x, T.y, T.z, 0.f);
+    p.smp = make_float4(L.x, L.y, L.z, next_pdf);
+    p.acc = make_float4(acc.x, acc.y, acc.z, __uint_as_float(N));
+    p.depth = depth + 1;
+    p.type = RT_DIFF;
+  }
+  p.rng = st;
+}
+
 // The shading pass of one bounce over the live paths: survivors compacted into
 // nxt, connection shadow rays into sb (contribution .w = the survivor's slot).
 // Workgroup of the shading pass: its compaction takes one atomic per counter
@@ -1516,7 +1789,11 @@ __device__ __forceinline__ void shade_path(const DevScene& sc, const FrameArgs& 
 constexpr int kShadeBlock = 256;
 // Five waves per SIMD: 96 VGPRs without spills; left to itself the compiler
 // takes 97 and four waves, and the pass is then 5-7 % slower.
-__global__ __launch_bounds__(kShadeBlock, 5) void k_shade(DevScene sc, FrameArgs fa, PathBuf cur, PathBuf nxt,
+// DISNEY: the Disney path tracer's bounce (shade_disney) in an instance of its
+// own, so that its larger live set does not spill the BDPT instance's.
+template <bool DISNEY>
+__global__ __launch_bounds__(kShadeBlock, DISNEY ? 4 : 5) void k_shade(DevScene sc, FrameArgs fa, PathBuf cur,
+                                                                       PathBuf nxt,
                                                   const float4* __restrict__ hits, FrameCounters* __restrict__ fc,
                                                   int bounce, ShadowBuf sb, const LightVert* __restrict__ lv,
                                                   Accum ac, int eye) {
@@ -1549,7 +1826,8 @@ __global__ __launch_bounds__(kShadeBlock, 5) void k_shade(DevScene sc, FrameArgs
       p.smp = cur.smp[i];
       p.acc = cur.acc[i];
     }
-    shade_path(sc, fa, fc, lv, ac, p, hits[i], survive, emit, sr);
+    if constexpr (DISNEY) shade_disney(sc, fa, ac, p, hits[i], survive, emit, sr);
+    else shade_path(sc, fa, fc, lv, ac, p, hits[i], survive, emit, sr);
   }
   uint32_t slot = 0, sslot = 0;
   block_append2_oct<kShadeBlock>(survive, survive ? octant(p.rd) : 8u, emit, emit ? octant(sr.d) : 8u,
@@ -1566,7 +1844,7 @@ __global__ __launch_bounds__(kShadeBlock, 5) void k_shade(DevScene sc, FrameArgs
     nxt.rng[slot] = p.rng;
   }
   if (emit) {
-    sr.c.w = __uint_as_float(slot);
+    sr.c.w = __uint_as_float(slot | (DISNEY ? kShadowNoCount : 0u));
     sb.ray_o[sslot] = sr.o;
     sb.ray_d[sslot] = sr.d;
     sb.contrib[sslot] = sr.c;
@@ -1582,8 +1860,11 @@ hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, Pa
   if (ac.apply == 1 && (!rec_ok(ac.rec) || !ac.gpos || !ac.gnrm || !ac.gdif)) return hipErrorInvalidValue;
   if (ac.apply == 2 && !ac.rec.acc) return hipErrorInvalidValue;
   if ((ac.variant & kVariantGbuf) && !rec_ok(ac.rec)) return hipErrorInvalidValue;
-  k_shade<<<(max_n + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, st>>>(sc, fa, cur, nxt, hits, fc, bounce, sb, lv,
-                                                                         ac, eye);
+  const uint32_t grid = (max_n + kShadeBlock - 1) / kShadeBlock;
+  if (ac.variant & kVariantDisney)
+    k_shade<true><<<grid, kShadeBlock, 0, st>>>(sc, fa, cur, nxt, hits, fc, bounce, sb, lv, ac, eye);
+  else
+    k_shade<false><<<grid, kShadeBlock, 0, st>>>(sc, fa, cur, nxt, hits, fc, bounce, sb, lv, ac, eye);
   return hipGetLastError();
 }
 
@@ -1599,12 +1880,12 @@ __global__ __launch_bounds__(kBlock, MINW) void k_shadow(DevScene sc, ShadowBuf 
     float4 a = sb.ray_o[i], b = sb.ray_d[i], c = sb.contrib[i];
     TraceHit h;
     if (!traverse4<true, STK, Q>(sc, ld3(a), ld3(b), a.w, b.w, false, lds_column(lstk), h, nn, nt)) {
-      uint32_t slot = __float_as_uint(c.w);
+      const uint32_t bits = __float_as_uint(c.w), slot = bits & ~kShadowNoCount;
       float4 acc = nxt.acc[slot];
       acc.x = acc.x + c.x;
       acc.y = acc.y + c.y;
       acc.z = acc.z + c.z;
-      acc.w = __uint_as_float(__float_as_uint(acc.w) + 4u);
+      acc.w = __uint_as_float(__float_as_uint(acc.w) + ((bits & kShadowNoCount) ? 0u : 4u));
       nxt.acc[slot] = acc;
     }
   }
@@ -2099,6 +2380,27 @@ hipError_t launch_alloc_scan(const float* err, const float* sum, uint32_t n, uin
   k_scan_local<<<nb, kBlock, 0, st>>>(err, sum, n, gen, counts, offsets, block_tot, paths, maxp);
   k_scan_totals<<<1, kBlock, 0, st>>>(block_tot, nb, total);
   k_scan_add<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(offsets, block_tot, n);
+  return hipGetLastError();
+}
+
+// Multi-view adaptive batches scan each view's counts on their own: view v's
+// offsets then move past the slots of views 0..v-1 (base: exclusive scan of
+// the per-view totals, by k_scan_totals), so all views form one queue.
+__global__ __launch_bounds__(kBlock) void k_offset_views(uint32_t* __restrict__ offsets, uint32_t n, FrameArgs fa,
+                                                         const uint32_t* __restrict__ base) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) offsets[i] += base[view_of(fa, i)];
+}
+hipError_t launch_view_bases(uint32_t* view_totals, uint32_t n_views, uint32_t* total, hipStream_t st) {
+  if (!view_totals || !total || n_views == 0) return hipErrorInvalidValue;
+  k_scan_totals<<<1, kBlock, 0, st>>>(view_totals, n_views, total);
+  return hipGetLastError();
+}
+hipError_t launch_offset_views(uint32_t* offsets, uint32_t n, const FrameArgs& fa, const uint32_t* base,
+                               hipStream_t st) {
+  if (!offsets || !base || !fa.views || fa.n_views == 0) return hipErrorInvalidValue;
+  if (n == 0) return hipSuccess;
+  k_offset_views<<<(n + kBlock - 1) / kBlock, kBlock, 0, st>>>(offsets, n, fa, base);
   return hipGetLastError();
 }
 

@@ -19,11 +19,14 @@ constexpr int kMlpOffB3 = kMlpOffW3 + kMlpHidden;                // 4864
 constexpr int kMlpParams = kMlpOffB3 + 1;                        // 4865
 static_assert(kMlpParams == 4865, "parameter count");
 
-// Operands in the layouts the MFMA tiles read: BF16 matrices, FP32 vectors.
+// Operands in the fragment order the MFMA instructions read, BF16 matrices
+// (one 8- or 16-byte load per lane and fragment), FP32 vectors.
+constexpr int kMlpW1Frag = 4 * 64 * 4;       // W1^T: 4 hidden blocks x 64 lanes x 4 (K = 16, features 9..15 zero)
+constexpr int kMlpW2Frag = 4 * 2 * 64 * 8;   // W2 or W2^T: 4 row blocks x 2 k-blocks x 64 lanes x 8
 struct MlpDevWeights {
-  uint16_t* w1t;  // [64][32]  W1^T, feature columns 9..31 zero
-  uint16_t* w2;   // [64][64]  W2
-  uint16_t* w2t;  // [64][64]  W2^T
+  uint16_t* w1f;  // layer 1 A operands
+  uint16_t* w2tf; // layer 2 A operands (W2^T, k-permuted)
+  uint16_t* w2f;  // backward A operands (W2, k-permuted)
   float* b1;      // [64]
   float* b2;      // [64]
   float* w3;      // [64]
@@ -39,18 +42,18 @@ struct MlpState {
   MlpDevWeights dev;
 };
 
-// Forward + backward of n samples (x: n x 9 row-major, t: n targets); writes
-// one gradient vector per 64-sample workgroup into `part` (ceil(n/64) x 4865)
-// and one loss partial per workgroup.  The loss is mean squared error.
+// Forward + backward of n samples (x: n x 9 row-major, t: n targets) on
+// mlp_train_groups(n) workgroups; writes one gradient vector per workgroup
+// into `part` (groups x 4865) and one loss partial (sum of squared residuals)
+// per workgroup.  The loss is mean squared error.
+constexpr uint32_t kMlpMaxTrainGroups = 256;
+uint32_t mlp_train_groups(uint32_t n);
 hipError_t launch_mlp_train(const MlpState& st, const float* x, const float* t, uint32_t n, float* part,
                             float* loss_part, hipStream_t s);
-// Sums the partials into st.grad / st.loss in a fixed order (chunks of
-// workgroups, then the chunks), and when `apply` is set performs Adam step
-// `step` (1-based) and refreshes the operands.  part2 holds
-// mlp_reduce_chunks(nwg) x 4865 floats.
+// Sums the nwg partials into st.grad / st.loss in a fixed order, and when
+// `apply` is set performs Adam step `step` (1-based); refreshes the operands.
 hipError_t launch_mlp_adam(const MlpState& st, const float* part, uint32_t nwg, const float* loss_part, float lr,
-                           int step, int apply, float* part2, hipStream_t s);
-uint32_t mlp_reduce_chunks(uint32_t nwg);
+                           int step, int apply, hipStream_t s);
 hipError_t launch_mlp_infer(const MlpState& st, const float* x, uint32_t n, float* y, hipStream_t s);
 // Rebuilds the operand pack from st.w.
 hipError_t launch_mlp_pack(const MlpState& st, hipStream_t s);

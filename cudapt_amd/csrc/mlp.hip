@@ -1,19 +1,20 @@
 // Per-pixel error MLP 9 -> 64 -> 64 -> 1 (GELU), BF16 operands on gfx950 MFMA
-// (v_mfma_f32_16x16x32_bf16) with FP32 accumulation, FP32 master weights and
-// Adam.  No reference counterpart: CudaPT links FANN but never calls it; the
-// network replaces the filter-residual error of the adaptive renderer
+// (v_mfma_f32_16x16x32_bf16, and v_mfma_f32_16x16x16_bf16 for the 9-feature
+// layer) with FP32 accumulation, FP32 master weights and Adam.  No reference
+// counterpart: CudaPT links FANN but never calls it; the network replaces the
+// filter-residual error of the adaptive renderer
 // (src/cudaRTBDPTStreamAdapBilatAtomic.cu:1143-1156) as the allocation signal.
 //
-// One workgroup = one wave = 64 pixels.  Layer matrices are 64 x 64 and are
-// computed as 4 x 4 tiles of 16 x 16; each tile is ceil(K/32) MFMAs.
+// Training: a workgroup of four waves; each wave takes 64 pixels at a time
+// and keeps the layer activations in registers between layers (the k-permuted
+// operand scheme below), accumulating its weight gradients over its chunks in
+// MFMA accumulators:
 //   forward   Z1 = X W1 + b1, H1 = gelu(Z1), Z2 = H1 W2 + b2, H2 = gelu(Z2), y = H2 w3 + b3
 //   backward  dy = 2 (y - t) / B, dZ2 = dy w3^T * gelu'(Z2), dH1 = dZ2 W2^T,
 //             dZ1 = dH1 * gelu'(Z1), dW2 = H1^T dZ2, dW1 = X^T dZ1 (all MFMA)
-// Operands sit in LDS in the orientation each MFMA reads contiguously (an
-// activation is stored once per orientation it is read in).  Per-wave weight
-// gradients go to a partials array; k_mlp_reduce sums them in chunks of
-// kReduceChunk workgroups and k_mlp_adam sums the chunk totals, both in a fixed
-// order (deterministic), and applies Adam (Kingma & Ba 2015).
+// The waves' gradients are summed in wave order into one partial per
+// workgroup; k_mlp_adam sums the partials in a fixed order (deterministic) and
+// applies Adam (Kingma & Ba 2015).
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -22,6 +23,7 @@
 namespace cpt {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 // float -> bf16 with round-to-nearest-even (finite inputs).
@@ -46,243 +48,391 @@ __device__ __forceinline__ void gelu_fwd(float x, float& g, float& dg) {
   dg = 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * c * (1.f + 3.f * a * x2);
 }
 
-// 16x16 tile of A (rows m0.., K contiguous, leading dim lda) times B^T given as
-// Bt (rows n0.., K contiguous, leading dim ldb), K a multiple of 32.
-// Lane l holds D[m0 + 4(l/16) + e][n0 + l%16], e = 0..3.
-__device__ __forceinline__ f32x4 mfma_tile(const uint16_t* A, int lda, int m0, const uint16_t* Bt, int ldb, int n0,
-                                           int K) {
-  const int l = threadIdx.x & 63;
-  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-  const int r = l & 15, kq = (l >> 4) * 8;
-  for (int k0 = 0; k0 < K; k0 += 32) {
-    bf16x8 a = *(const bf16x8*)(A + (m0 + r) * lda + k0 + kq);
-    bf16x8 b = *(const bf16x8*)(Bt + (n0 + r) * ldb + k0 + kq);
-    acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc, 0, 0, 0);
+// ---- register-resident layers ----------------------------------------------
+// A wave computes its 64 pixels transposed: activations are [unit][pixel]
+// 64 x 64 matrices held as 4 x 4 tiles of 16 x 16 in the MFMA output layout,
+// lane l = 16 g + c holding rows 4g..4g+3 of column c of every tile.  The next
+// layer's MFMA reads them as its B operand without leaving the registers: the
+// contraction index k of v_mfma_f32_16x16x32_bf16 may be permuted freely if
+// A and B agree, so k-slot j of lane group g in k-block kb stands for unit
+//   perm(kb, g, j) = (2 kb + j / 4) * 16 + 4 g + j % 4,
+// which is exactly what the lane holds of tiles 2kb and 2kb+1.  The weight
+// operands are packed on the device in that order (mlp_pack_one).  Only the
+// weight gradients contract over pixels and need the pixel index in the
+// k-slots: their operands go through LDS, one 8 KB [unit][pixel] BF16 plane
+// each.  Layer 1 (K = 9 features) uses v_mfma_f32_16x16x16_bf16 with K padded
+// to 16 (the 16x16x32 form would pad it to 32).
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f32x4 mfma16(s16x4 a, s16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ bf16x8 ld_frag8(const uint16_t* p) { return *(const bf16x8*)p; }
+__device__ __forceinline__ s16x4 ld_frag4(const uint16_t* p) { return *(const s16x4*)p; }
+__device__ __forceinline__ __bf16 as_bf(uint16_t h) { return __builtin_bit_cast(__bf16, h); }
+
+// LDS order within a wave: LDS instructions of one wave execute in issue
+// order; this keeps the compiler from moving them across the point and waits
+// for the outstanding ones.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+constexpr int kTrainWaves = 4;                  // waves per training workgroup
+// Training takes 32 pixels per wave at a time (two pixel blocks): the
+// activations of a chunk then fit the register file next to the
+// weight-gradient accumulators; the weight gradients contract over the
+// chunk's 32 pixels, one k-block of v_mfma_f32_16x16x32_bf16.
+constexpr int kTrainPB = 2;                      // 16-pixel blocks per training chunk
+constexpr int kTrainChunk = 16 * kTrainPB;
+struct MlpWaveLds {
+  alignas(16) uint16_t xt[16 * kTrainChunk];    // X^T  [feature][pixel] (features 9..15 zero)
+  alignas(16) uint16_t h1t[64 * kTrainChunk];   // H1^T [hidden][pixel], then dZ1^T
+  alignas(16) uint16_t dz2t[64 * kTrainChunk];  // dZ2^T [out][pixel]
+  float vec[3][64];                             // dW3, db2, db1 summed over the wave's chunks
+};
+// Weight operands and vectors, shared by the workgroup's waves.
+struct MlpWeightLds {
+  alignas(16) uint16_t w1f[kMlpW1Frag];
+  alignas(16) uint16_t w2tf[kMlpW2Frag];
+  alignas(16) uint16_t w2f[kMlpW2Frag];
+  float b1[64], b2[64], w3[64];
+};
+struct MlpTrainLds {
+  MlpWeightLds w;
+  union {
+    MlpWaveLds wave[kTrainWaves];
+    float red[kMlpParams + 3];                  // the workgroup's gradient, summed wave by wave
+  } u;
+};
+
+template <bool TRAIN>
+__device__ __forceinline__ void stage_weights(MlpWeightLds& s, const MlpDevWeights& w, int tid, int nthreads) {
+  typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+  for (int i = tid; i < kMlpW1Frag / 8; i += nthreads) ((u32x4*)s.w1f)[i] = ((const u32x4*)w.w1f)[i];
+  for (int i = tid; i < kMlpW2Frag / 8; i += nthreads) {
+    ((u32x4*)s.w2tf)[i] = ((const u32x4*)w.w2tf)[i];
+    if constexpr (TRAIN) ((u32x4*)s.w2f)[i] = ((const u32x4*)w.w2f)[i];
   }
+  if (tid < 64) { s.b1[tid] = w.b1[tid]; s.b2[tid] = w.b2[tid]; s.w3[tid] = w.w3[tid]; }
+}
+
+// Layer-1 B operands of the wave's pixels: lane (g, c) holds features
+// 4g..4g+3 of pixel base + 16 pb + c (zero past feature 8 and past n).
+template <int PB>
+__device__ __forceinline__ void load_x(s16x4 xb[PB], const float* __restrict__ x, uint32_t base, uint32_t n, int g,
+                                       int c) {
+#pragma unroll
+  for (int pb = 0; pb < PB; pb++) {
+    const uint32_t px = base + pb * 16 + c;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int f = g * 4 + j;
+      const float v = (f < kMlpIn && px < n) ? x[(size_t)px * kMlpIn + f] : 0.f;
+      xb[pb][j] = (short)f2bf(v);
+    }
+  }
+}
+
+// Z1^T tile [hidden block hb][pixel block pb].
+__device__ __forceinline__ f32x4 layer1_tile(const uint16_t* w1f, int hb, const s16x4 xb, int l) {
+  return mfma16(ld_frag4(w1f + (hb * 64 + l) * 4), xb, f32x4{0.f, 0.f, 0.f, 0.f});
+}
+// Tile [row block rb][pixel block pb] of A (64 x 64, packed per row block and
+// k-block) times the 64 x 64 register matrix given by its k-permuted B
+// operands bk[kb][pb].
+template <int PB>
+__device__ __forceinline__ f32x4 layer64_tile(const uint16_t* a, int rb, int pb, const bf16x8 bk[2][PB], int l) {
+  f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+  acc = mfma32(ld_frag8(a + ((rb * 2 + 0) * 64 + l) * 8), bk[0][pb], acc);
+  acc = mfma32(ld_frag8(a + ((rb * 2 + 1) * 64 + l) * 8), bk[1][pb], acc);
   return acc;
 }
 
-// LDS of the forward pass (inference): operands of the wave's 64 pixels.
-struct MlpFwdLds {
-  alignas(16) uint16_t w1t[64 * 32];   // W1^T  [hidden][feature], features >= 9 zero
-  alignas(16) uint16_t w2t[64 * 64];   // W2^T  [out][in]
-  float b1[64], b2[64], w3[64], b3;
-  alignas(16) uint16_t x[64 * 32];     // X     [pixel][feature]
-  alignas(16) uint16_t h1[64 * 64];    // H1    [pixel][hidden]
-  float ys[64];                        // y per pixel
-};
-// Training adds the other operand orientations and the backward buffers.
-// Training adds the other operand orientations and the backward buffers,
-// within 80 KB so that two workgroups share a CU: dZ1^T reuses H1's space (H1
-// is dead after the forward pass), dZ2 and dZ2^T reuse H2's (dead after dW3),
-// and the GELU derivative at Z1 is recomputed in the backward pass from X.
-struct MlpLds : MlpFwdLds {
-  alignas(16) uint16_t w2[64 * 64];    // W2    [in][out]
-  alignas(16) uint16_t xt[32 * 64];    // X^T   [feature][pixel]
-  alignas(16) uint16_t h1t[64 * 64];   // H1^T  [hidden][pixel]
-  union alignas(16) {
-    float h2[64 * 65];                 // H2    [pixel][hidden], FP32, padded rows (forward)
-    struct {
-      uint16_t dz2[64 * 64];           // dZ2   [pixel][out]   (backward)
-      uint16_t dz2t[64 * 64];          // dZ2^T [out][pixel]
-    } bw;
-  } u;
-  float dy[64];
-  float dg2[64 * 64];                  // GELU derivative at Z2, MFMA output layout [tile * 4 + element][lane]
-};
-static_assert(sizeof(MlpLds) <= 80 * 1024, "two training workgroups per CU");
-
-template <bool TRAIN, class L>
-__device__ __forceinline__ void load_weights(L& s, const MlpDevWeights& w) {
-  const int l = threadIdx.x;
-  for (int i = l; i < 64 * 32; i += 64) s.w1t[i] = w.w1t[i];
-  for (int i = l; i < 64 * 64; i += 64) {
-    s.w2t[i] = w.w2t[i];
-    if constexpr (TRAIN) s.w2[i] = w.w2[i];
-  }
-  s.b1[l] = w.b1[l]; s.b2[l] = w.b2[l]; s.w3[l] = w.w3[l];
-  if (l == 0) s.b3 = w.b3[0];
-}
-
-// Forward of the wave's 64 pixels; returns y of the lane's pixel.  With TRAIN
-// the transposed operands, H2 and the GELU derivatives stay in LDS for the
-// backward pass.
-template <bool TRAIN, class L>
-__device__ __forceinline__ float forward(L& s, const float* __restrict__ x, uint32_t base, uint32_t n) {
-  const int l = threadIdx.x;
-  const int col = l & 15, rq = (l >> 4) * 4;
-  // X as bf16; rows past n are zero
-  for (int f = 0; f < 32; f++) {
-    float v = (f < kMlpIn && base + l < n) ? x[(size_t)(base + l) * kMlpIn + f] : 0.f;
-    uint16_t h = f2bf(v);
-    s.x[l * 32 + f] = h;
-    if constexpr (TRAIN) s.xt[f * 64 + l] = h;
-  }
-  __syncthreads();
-  // layer 1
-  for (int t = 0; t < 16; t++) {
-    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
-    f32x4 z = mfma_tile(s.x, 32, m0, s.w1t, 32, n0, 32);
+typedef _Float16 h16x2 __attribute__((ext_vector_type(2)));
+// Keeps the scheduler from hoisting the next tile's loads and MFMAs above the
+// current tile's element work: with every tile in flight at once the live
+// set exceeds the register file.
+__device__ __forceinline__ void tile_fence() { __builtin_amdgcn_sched_barrier(0); }
+// Sums v over the 16 pixel lanes of each lane group and adds it to acc[unit]
+// (lane c == 0 of each group writes its four units).
+__device__ __forceinline__ void add_unit_sums(float* acc, float v[4][4], int g, int c, bool first) {
+#pragma unroll
+  for (int b = 0; b < 4; b++)
 #pragma unroll
     for (int e = 0; e < 4; e++) {
-      const int px = m0 + rq + e, hd = n0 + col;
-      float g, dg;
-      gelu_fwd(z[e] + s.b1[hd], g, dg);
-      uint16_t hb = f2bf(g);
-      s.h1[px * 64 + hd] = hb;
-      if constexpr (TRAIN) s.h1t[hd * 64 + px] = hb;
-    }
-  }
-  __syncthreads();
-  // layer 2, and layer 3 as a reduction over the hidden units: per row block,
-  // each lane sums its 4 column tiles, then the 16 lanes of a column group add
-  // up by butterfly.
-  for (int mb = 0; mb < 4; mb++) {
-    float part[4] = {0.f, 0.f, 0.f, 0.f};
+      float t = v[b][e];
 #pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const int t = mb * 4 + c, m0 = mb * 16, n0 = c * 16;
-      f32x4 z = mfma_tile(s.h1, 64, m0, s.w2t, 64, n0, 64);
-#pragma unroll
-      for (int e = 0; e < 4; e++) {
-        const int px = m0 + rq + e, hd = n0 + col;
-        float g, dg;
-        gelu_fwd(z[e] + s.b2[hd], g, dg);
-        part[e] = fmaf(g, s.w3[hd], part[e]);
-        if constexpr (TRAIN) {
-          s.dg2[(t * 4 + e) * 64 + l] = dg;
-          s.u.h2[px * 65 + hd] = g;
-        }
+      for (int o = 1; o < 16; o <<= 1) t += __shfl_xor(t, o, 64);
+      if (c == 0) {
+        const int u = b * 16 + g * 4 + e;
+        acc[u] = first ? t : acc[u] + t;
       }
     }
-#pragma unroll
-    for (int e = 0; e < 4; e++) {
-      float v = part[e];
-      for (int o = 1; o < 16; o <<= 1) v += __shfl_xor(v, o, 64);
-      if (col == e) s.ys[mb * 16 + rq + e] = v;
-    }
-  }
-  __syncthreads();
-  return s.ys[l] + s.b3;
 }
 
-__global__ __launch_bounds__(64) void k_mlp_train(MlpDevWeights w, const float* __restrict__ x,
-                                                  const float* __restrict__ target, uint32_t n, float inv_batch,
-                                                  float* __restrict__ part, float* __restrict__ loss_part) {
-  __shared__ MlpLds s;
-  const int l = threadIdx.x;
-  const uint32_t base = blockIdx.x * 64;
-  const int col = l & 15, rq = (l >> 4) * 4;
-  load_weights<true>(s, w);
-  const float y = forward<true>(s, x, base, n);
-  const bool valid = base + l < n;
-  const float d = valid ? (y - target[base + l]) : 0.f;
+__global__ __launch_bounds__(64 * kTrainWaves, 1) void k_mlp_train(MlpDevWeights w, const float* __restrict__ x,
+                                                                   const float* __restrict__ target, uint32_t n,
+                                                                   float inv_batch, float* __restrict__ part,
+                                                                   float* __restrict__ loss_part) {
+  __shared__ MlpTrainLds s;
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, g = l >> 4, c = l & 15;
+  stage_weights<true>(s.w, w, tid, 64 * kTrainWaves);
+  for (int i = l; i < 3 * 64; i += 64) s.u.wave[wave].vec[i / 64][i % 64] = 0.f;
+  __syncthreads();
+  const float b3 = w.b3[0];
+  MlpWaveLds& L = s.u.wave[wave];
+  // weight-gradient accumulators over the wave's chunks (MFMA accumulators)
+  f32x4 aw2[4][4], aw1[4];
+  float ab3 = 0.f, aloss = 0.f;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    aw1[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int j = 0; j < 4; j++) aw2[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  }
+  constexpr int PB = kTrainPB, C = kTrainChunk;
+  const uint32_t nchunks = (n + C - 1) / C;
+  for (uint32_t ch = blockIdx.x * kTrainWaves + wave; ch < nchunks; ch += gridDim.x * kTrainWaves) {
+    const uint32_t base = ch * C;
+    s16x4 xb[PB];
+    load_x<PB>(xb, x, base, n, g, c);
+#pragma unroll
+    for (int pb = 0; pb < PB; pb++)
+#pragma unroll
+      for (int j = 0; j < 4; j++) L.xt[(g * 4 + j) * C + pb * 16 + c] = (uint16_t)xb[pb][j];
+    // ---- forward; the GELU derivatives and H2 are kept as FP16 pairs
+    bf16x8 hk[2][PB];                                      // H1^T as layer-2 B operands
+    h16x2 dg1[4][PB][2];
+#pragma unroll
+    for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+      for (int pb = 0; pb < PB; pb++) {
+        const f32x4 z = layer1_tile(s.w.w1f, hb, xb[pb], l);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int u = hb * 16 + g * 4 + e;
+          float gv, dg;
+          gelu_fwd(z[e] + s.w.b1[u], gv, dg);
+          dg1[hb][pb][e >> 1][e & 1] = (_Float16)dg;
+          const uint16_t hbits = f2bf(gv);
+          hk[hb >> 1][pb][(hb & 1) * 4 + e] = as_bf(hbits);
+          L.h1t[u * C + pb * 16 + c] = hbits;
+        }
+        tile_fence();
+      }
+    h16x2 h2[4][PB][2], dg2[4][PB][2];
+    float ys[PB];
+#pragma unroll
+    for (int pb = 0; pb < PB; pb++) ys[pb] = 0.f;
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++)
+#pragma unroll
+      for (int pb = 0; pb < PB; pb++) {
+        const f32x4 z = layer64_tile<PB>(s.w.w2tf, ob, pb, hk, l);  // Z2^T
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int u = ob * 16 + g * 4 + e;
+          float gv, dg;
+          gelu_fwd(z[e] + s.w.b2[u], gv, dg);
+          h2[ob][pb][e >> 1][e & 1] = (_Float16)gv;
+          dg2[ob][pb][e >> 1][e & 1] = (_Float16)dg;
+          ys[pb] = fmaf(gv, s.w.w3[u], ys[pb]);
+        }
+        tile_fence();
+      }
+    float dy[PB];
+#pragma unroll
+    for (int pb = 0; pb < PB; pb++) {
+      float y = ys[pb];
+      y += __shfl_xor(y, 16, 64);
+      y += __shfl_xor(y, 32, 64);
+      y += b3;
+      const uint32_t px = base + pb * 16 + c;
+      const float d = px < n ? y - target[px] : 0.f;
+      if (g == 0) aloss = fmaf(d, d, aloss);
+      dy[pb] = 2.f * d * inv_batch;
+      if (g == 0) ab3 += dy[pb];
+    }
+    // ---- backward
+    bf16x8 dk[2][PB];                                      // dZ2^T as B operands
+    float v3[4][4], v2[4][4];
+#pragma unroll
+    for (int ob = 0; ob < 4; ob++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int u = ob * 16 + g * 4 + e;
+        const float w3u = s.w.w3[u];
+        v3[ob][e] = v2[ob][e] = 0.f;
+#pragma unroll
+        for (int pb = 0; pb < PB; pb++) {
+          v3[ob][e] = fmaf((float)h2[ob][pb][e >> 1][e & 1], dy[pb], v3[ob][e]);
+          const float dz = dy[pb] * w3u * (float)dg2[ob][pb][e >> 1][e & 1];
+          v2[ob][e] += dz;
+          const uint16_t bits = f2bf(dz);
+          dk[ob >> 1][pb][(ob & 1) * 4 + e] = as_bf(bits);
+          L.dz2t[u * C + pb * 16 + c] = bits;
+        }
+        tile_fence();
+      }
+    add_unit_sums(L.vec[0], v3, g, c, false);
+    add_unit_sums(L.vec[1], v2, g, c, false);
+    wave_lds_sync();
+    // dW2[in][out] += H1^T . dZ2 over the chunk's pixels
+#pragma unroll
+    for (int kb = 0; kb < C / 32; kb++) {
+      bf16x8 bo[4];
+#pragma unroll
+      for (int ob = 0; ob < 4; ob++) bo[ob] = ld_frag8(&L.dz2t[(ob * 16 + c) * C + kb * 32 + g * 8]);
+#pragma unroll
+      for (int ib = 0; ib < 4; ib++) {
+        const bf16x8 ai = ld_frag8(&L.h1t[(ib * 16 + c) * C + kb * 32 + g * 8]);
+#pragma unroll
+        for (int ob = 0; ob < 4; ob++) aw2[ib][ob] = mfma32(ai, bo[ob], aw2[ib][ob]);
+      }
+    }
+    wave_lds_sync();                                       // H1^T reads done: dZ1^T takes its place
+    // dH1^T = W2 . dZ2^T; dZ1 = dH1 * gelu'(Z1)
+    float v1[4][4];
+#pragma unroll
+    for (int ib = 0; ib < 4; ib++) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) v1[ib][e] = 0.f;
+#pragma unroll
+      for (int pb = 0; pb < PB; pb++) {
+        const f32x4 dh = layer64_tile<PB>(s.w.w2f, ib, pb, dk, l);
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+          const int u = ib * 16 + g * 4 + e;
+          const float dz = dh[e] * (float)dg1[ib][pb][e >> 1][e & 1];
+          v1[ib][e] += dz;
+          L.h1t[u * C + pb * 16 + c] = f2bf(dz);
+        }
+        tile_fence();
+      }
+    }
+    add_unit_sums(L.vec[2], v1, g, c, false);
+    wave_lds_sync();
+    // dW1[feature][hidden] += X^T . dZ1 over the chunk's pixels
+#pragma unroll
+    for (int kb = 0; kb < C / 32; kb++) {
+      const bf16x8 ax = ld_frag8(&L.xt[c * C + kb * 32 + g * 8]);
+#pragma unroll
+      for (int hb = 0; hb < 4; hb++)
+        aw1[hb] = mfma32(ax, ld_frag8(&L.h1t[(hb * 16 + c) * C + kb * 32 + g * 8]), aw1[hb]);
+    }
+    wave_lds_sync();                                       // the next chunk rewrites the planes
+  }
+  for (int o = 1; o < 64; o <<= 1) {
+    ab3 += __shfl_xor(ab3, o, 64);
+    aloss += __shfl_xor(aloss, o, 64);
+  }
+  // per-unit vectors of the wave into registers before the planes are reused
+  const float vw3 = L.vec[0][l], vb2 = L.vec[1][l], vb1 = L.vec[2][l];
+  __syncthreads();                                         // every wave's planes are dead: red takes them
+  // the workgroup's gradient: waves add theirs in order
+  for (int wv = 0; wv < kTrainWaves; wv++) {
+    if (wave == wv) {
+      auto put = [&](int p, float v) { s.u.red[p] = wv == 0 ? v : s.u.red[p] + v; };
+#pragma unroll
+      for (int ib = 0; ib < 4; ib++)
+#pragma unroll
+        for (int ob = 0; ob < 4; ob++)
+#pragma unroll
+          for (int e = 0; e < 4; e++) put(kMlpOffW2 + (ib * 16 + g * 4 + e) * 64 + ob * 16 + c, aw2[ib][ob][e]);
+#pragma unroll
+      for (int hb = 0; hb < 4; hb++)
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          if (g * 4 + e < kMlpIn) put(kMlpOffW1 + (g * 4 + e) * 64 + hb * 16 + c, aw1[hb][e]);
+      put(kMlpOffW3 + l, vw3);
+      put(kMlpOffB2 + l, vb2);
+      put(kMlpOffB1 + l, vb1);
+      if (l == 0) {
+        put(kMlpOffB3, ab3);
+        put(kMlpParams, aloss);
+      }
+    }
+    __syncthreads();
+  }
   float* P = part + (size_t)blockIdx.x * kMlpParams;
-  // loss partial (sum of squares, workgroup order)
-  float sq = d * d;
-  for (int o = 32; o >= 1; o >>= 1) sq += __shfl_xor(sq, o, 64);
-  if (l == 0) loss_part[blockIdx.x] = sq;
-  s.dy[l] = 2.f * d * inv_batch;
+  for (int p = tid; p < kMlpParams; p += 64 * kTrainWaves) P[p] = s.u.red[p];
+  if (tid == 0) loss_part[blockIdx.x] = s.u.red[kMlpParams];
+}
+
+__global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_infer(MlpDevWeights w, const float* __restrict__ x,
+                                                               uint32_t n, float* __restrict__ y) {
+  __shared__ MlpWeightLds s;
+  const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, g = l >> 4, c = l & 15;
+  stage_weights<false>(s, w, tid, 64 * kTrainWaves);
   __syncthreads();
-  // dW3 (lane = hidden unit), db3
-  {
-    float g3 = 0.f;
-    for (int px = 0; px < 64; px++) g3 = fmaf(s.u.h2[px * 65 + l], s.dy[px], g3);
-    P[kMlpOffW3 + l] = g3;
-    float db3 = s.dy[l];
-    for (int o = 32; o >= 1; o >>= 1) db3 += __shfl_xor(db3, o, 64);
-    if (l == 0) P[kMlpOffB3] = db3;
-  }
-  __syncthreads();                                 // H2 is dead: its space takes dZ2
-  // dZ2 = dy w3^T * gelu'(Z2), both orientations; db2 = column sums
-  float db2[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-  for (int t = 0; t < 16; t++) {
-    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
+  const float b3 = w.b3[0];
+  const uint32_t nchunks = (n + 63) / 64;
+  for (uint32_t ch = blockIdx.x * kTrainWaves + wave; ch < nchunks; ch += gridDim.x * kTrainWaves) {
+    const uint32_t base = ch * 64;
+    s16x4 xb[4];
+    load_x<4>(xb, x, base, n, g, c);
+    bf16x8 hk[2][4];
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int px = m0 + rq + e, hd = n0 + col;
-      float g = s.dy[px] * s.w3[hd] * s.dg2[(t * 4 + e) * 64 + l];
-      db2[t & 3] += g;
-      uint16_t hb = f2bf(g);
-      s.u.bw.dz2[px * 64 + hd] = hb;
-      s.u.bw.dz2t[hd * 64 + px] = hb;
-    }
-  }
+    for (int hb = 0; hb < 4; hb++)
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    float v = db2[c];
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    if (l < 16) P[kMlpOffB2 + c * 16 + l] = v;
-  }
-  __syncthreads();
-  // dH1 = dZ2 W2^T -> dZ1 = dH1 * gelu'(Z1) (transposed, into H1's space), db1;
-  // Z1 is recomputed by the same MFMA as in the forward pass
-  uint16_t* dz1t = s.h1;
-  float db1[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll 4
-  for (int t = 0; t < 16; t++) {
-    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
-    f32x4 dh = mfma_tile(s.u.bw.dz2, 64, m0, s.w2, 64, n0, 64);
-    f32x4 z = mfma_tile(s.x, 32, m0, s.w1t, 32, n0, 32);
+      for (int pb = 0; pb < 4; pb++) {
+        const f32x4 z = layer1_tile(s.w1f, hb, xb[pb], l);
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int px = m0 + rq + e, hd = n0 + col;
-      float g1, dg1;
-      gelu_fwd(z[e] + s.b1[hd], g1, dg1);
-      float g = dh[e] * dg1;
-      db1[t & 3] += g;
-      dz1t[hd * 64 + px] = f2bf(g);
-    }
-  }
+        for (int e = 0; e < 4; e++) {
+          float gv, dg;
+          gelu_fwd(z[e] + s.b1[hb * 16 + g * 4 + e], gv, dg);
+          hk[hb >> 1][pb][(hb & 1) * 4 + e] = as_bf(f2bf(gv));
+        }
+      }
+    float ys[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-  for (int c = 0; c < 4; c++) {
-    float v = db1[c];
-    v += __shfl_xor(v, 16, 64);
-    v += __shfl_xor(v, 32, 64);
-    if (l < 16) P[kMlpOffB1 + c * 16 + l] = v;
-  }
-  __syncthreads();
-  // dW2[in][out] = H1^T dZ2 : A = H1^T [in][px], Bt = dZ2^T [out][px]
-  for (int t = 0; t < 16; t++) {
-    const int m0 = (t >> 2) * 16, n0 = (t & 3) * 16;
-    f32x4 g = mfma_tile(s.h1t, 64, m0, s.u.bw.dz2t, 64, n0, 64);
+    for (int ob = 0; ob < 4; ob++)
 #pragma unroll
-    for (int e = 0; e < 4; e++) P[kMlpOffW2 + (m0 + rq + e) * 64 + n0 + col] = g[e];
-  }
-  // dW1[feature][hidden] = X^T dZ1 : A = X^T [feature][px] (rows 0..15), Bt = dZ1^T [hidden][px]
-  for (int c = 0; c < 4; c++) {
-    f32x4 g = mfma_tile(s.xt, 64, 0, dz1t, 64, c * 16, 64);
+      for (int pb = 0; pb < 4; pb++) {
+        const f32x4 z = layer64_tile<4>(s.w2tf, ob, pb, hk, l);
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const int f = rq + e;
-      if (f < kMlpIn) P[kMlpOffW1 + f * 64 + c * 16 + col] = g[e];
+        for (int e = 0; e < 4; e++) {
+          const int u = ob * 16 + g * 4 + e;
+          float gv, dg;
+          gelu_fwd(z[e] + s.b2[u], gv, dg);
+          ys[pb] = fmaf(gv, s.w3[u], ys[pb]);
+        }
+      }
+#pragma unroll
+    for (int pb = 0; pb < 4; pb++) {
+      float v = ys[pb];
+      v += __shfl_xor(v, 16, 64);
+      v += __shfl_xor(v, 32, 64);
+      const uint32_t px = base + pb * 16 + c;
+      if (g == 0 && px < n) y[px] = v + b3;
     }
   }
 }
 
-__global__ __launch_bounds__(64) void k_mlp_infer(MlpDevWeights w, const float* __restrict__ x, uint32_t n,
-                                                  float* __restrict__ y) {
-  __shared__ MlpFwdLds s;
-  const uint32_t base = blockIdx.x * 64;
-  load_weights<false>(s, w);
-  const float out = forward<false>(s, x, base, n);
-  if (base + threadIdx.x < n) y[base + threadIdx.x] = out;
-}
-
-// Writes parameter p (value w) into the BF16/FP32 operand pack.
+// Writes parameter p (value w) into the operand pack: the k-permuted
+// fragment order the register-resident layers read (see above).
 __device__ __forceinline__ void mlp_pack_one(MlpDevWeights d, uint32_t p, float w) {
-  if (p < (uint32_t)kMlpOffB1) {                       // W1[f][h] -> W1^T[h][f]
-    uint32_t f = p / 64, h = p % 64;
-    d.w1t[h * 32 + f] = f2bf(w);
+  if (p < (uint32_t)kMlpOffB1) {                       // W1[f][h]: layer-1 A operand, row h, k = f
+    const uint32_t f = p / 64, h = p % 64;
+    d.w1f[((h / 16) * 64 + (f / 4) * 16 + h % 16) * 4 + f % 4] = f2bf(w);
   } else if (p < (uint32_t)kMlpOffW2) {
     d.b1[p - kMlpOffB1] = w;
   } else if (p < (uint32_t)kMlpOffB2) {                // W2[i][o]
-    uint32_t q = p - kMlpOffW2, i = q / 64, o = q % 64;
-    uint16_t h = f2bf(w);
-    d.w2[i * 64 + o] = h;
-    d.w2t[o * 64 + i] = h;
+    const uint32_t q = p - kMlpOffW2, i = q / 64, o = q % 64;
+    const uint16_t h = f2bf(w);
+    // W2^T: row o, k-slot of unit i; W2: row i, k-slot of unit o
+    auto slot = [](uint32_t unit, uint32_t& kb, uint32_t& lane_g, uint32_t& j) {
+      const uint32_t blk = unit / 16, r = unit % 16;
+      kb = blk / 2; lane_g = r / 4; j = (blk % 2) * 4 + r % 4;
+    };
+    uint32_t kb, gg, j;
+    slot(i, kb, gg, j);
+    d.w2tf[(((o / 16) * 2 + kb) * 64 + gg * 16 + o % 16) * 8 + j] = h;
+    slot(o, kb, gg, j);
+    d.w2f[(((i / 16) * 2 + kb) * 64 + gg * 16 + i % 16) * 8 + j] = h;
   } else if (p < (uint32_t)kMlpOffW3) {
     d.b2[p - kMlpOffB2] = w;
   } else if (p < (uint32_t)kMlpOffB3) {
@@ -292,49 +442,42 @@ __device__ __forceinline__ void mlp_pack_one(MlpDevWeights d, uint32_t p, float 
   }
 }
 
-// Sum of the per-workgroup gradients (workgroup order), Adam, and the BF16
-// operand copies in the layouts the MFMA tiles read.
-// Chunk c of the per-workgroup gradients: part2[c][p] = sum over workgroups
-// c*kReduceChunk .. of part[wg][p], in workgroup order.
-constexpr uint32_t kReduceChunk = 16;
-__global__ __launch_bounds__(256) void k_mlp_reduce(const float* __restrict__ part, uint32_t nwg,
-                                                    float* __restrict__ part2) {
-  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= (uint32_t)kMlpParams) return;
-  const uint32_t i0 = blockIdx.y * kReduceChunk, i1 = min(i0 + kReduceChunk, nwg);
-  float g = 0.f;
-  for (uint32_t i = i0; i < i1; i++) g += part[(size_t)i * kMlpParams + p];
-  part2[(size_t)blockIdx.y * kMlpParams + p] = g;
-}
-
-// The last workgroup sums the loss partials (strided per thread, then a fixed
-// tree); the others sum the chunk totals of one parameter each and step Adam.
-__global__ __launch_bounds__(256) void k_mlp_adam(MlpState st, const float* __restrict__ part2, uint32_t nch,
-                                                  const float* __restrict__ loss_part, uint32_t nwg, float lr,
-                                                  float beta1, float beta2, float eps, float bc1, float bc2,
-                                                  int apply) {
+// Gradient sum over the training workgroups' partials and Adam (Kingma & Ba
+// 2015).  Each workgroup takes 64 parameters; its four waves sum every fourth
+// partial, in order, and the four sums are added in wave order: a fixed order,
+// so the step is deterministic.  The last workgroup sums the loss partials.
+constexpr int kAdamParams = 64;
+__global__ __launch_bounds__(256) void k_mlp_adam(MlpState st, const float* __restrict__ part, uint32_t nwg,
+                                                  const float* __restrict__ loss_part, float lr, float beta1,
+                                                  float beta2, float eps, float bc1, float bc2, int apply) {
+  __shared__ float red[4][kAdamParams + 1];
   if (blockIdx.x == gridDim.x - 1) {
-    __shared__ float red[256];
+    __shared__ float lr_[256];
     float L = 0.f;
     for (uint32_t i = threadIdx.x; i < nwg; i += 256) L += loss_part[i];
-    red[threadIdx.x] = L;
+    lr_[threadIdx.x] = L;
     __syncthreads();
     for (int o = 128; o >= 1; o >>= 1) {
-      if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      if ((int)threadIdx.x < o) lr_[threadIdx.x] += lr_[threadIdx.x + o];
       __syncthreads();
     }
-    if (threadIdx.x == 0) st.loss[0] = red[0];
+    if (threadIdx.x == 0) st.loss[0] = lr_[0];
     return;
   }
-  const uint32_t p = blockIdx.x * 256 + threadIdx.x;
-  if (p >= (uint32_t)kMlpParams) return;
-  float g = 0.f;
-  for (uint32_t c = 0; c < nch; c++) g += part2[(size_t)c * kMlpParams + p];
-  st.grad[p] = g;
+  const int q = threadIdx.x & (kAdamParams - 1), grp = threadIdx.x / kAdamParams;
+  const uint32_t p = blockIdx.x * kAdamParams + q;
+  float gsum = 0.f;
+  if (p < (uint32_t)kMlpParams)
+    for (uint32_t i = grp; i < nwg; i += 4) gsum += part[(size_t)i * kMlpParams + p];
+  red[grp][q] = gsum;
+  __syncthreads();
+  if (grp != 0 || p >= (uint32_t)kMlpParams) return;
+  const float gr = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+  st.grad[p] = gr;
   float w = st.w[p];
   if (apply) {
-    float m = beta1 * st.m[p] + (1.f - beta1) * g;
-    float v = beta2 * st.v[p] + (1.f - beta2) * g * g;
+    float m = beta1 * st.m[p] + (1.f - beta1) * gr;
+    float v = beta2 * st.v[p] + (1.f - beta2) * gr * gr;
     st.m[p] = m;
     st.v[p] = v;
     w = w - lr * (m / bc1) / (sqrtf(v / bc2) + eps);
@@ -343,28 +486,34 @@ __global__ __launch_bounds__(256) void k_mlp_adam(MlpState st, const float* __re
   mlp_pack_one(st.dev, p, w);
 }
 
+uint32_t mlp_train_groups(uint32_t n) {
+  const uint32_t chunks = (n + kTrainChunk - 1) / kTrainChunk;
+  const uint32_t g = (chunks + kTrainWaves - 1) / kTrainWaves;
+  return g < kMlpMaxTrainGroups ? g : kMlpMaxTrainGroups;
+}
+
 hipError_t launch_mlp_train(const MlpState& st, const float* x, const float* t, uint32_t n, float* part,
                             float* loss_part, hipStream_t s) {
   if (!x || !t || !part || !loss_part || n == 0) return hipErrorInvalidValue;
-  uint32_t nwg = (n + 63) / 64;
-  k_mlp_train<<<nwg, 64, 0, s>>>(st.dev, x, t, n, 1.f / (float)n, part, loss_part);
+  k_mlp_train<<<mlp_train_groups(n), 64 * kTrainWaves, 0, s>>>(st.dev, x, t, n, 1.f / (float)n, part, loss_part);
   return hipGetLastError();
 }
 hipError_t launch_mlp_adam(const MlpState& st, const float* part, uint32_t nwg, const float* loss_part, float lr,
-                           int step, int apply, float* part2, hipStream_t s) {
-  if (!part || !loss_part || !part2 || nwg == 0) return hipErrorInvalidValue;
+                           int step, int apply, hipStream_t s) {
+  if (!part || !loss_part || nwg == 0) return hipErrorInvalidValue;
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
-  const uint32_t nch = mlp_reduce_chunks(nwg);
-  const uint32_t pb = (kMlpParams + 255) / 256;
-  k_mlp_reduce<<<dim3(pb, nch), 256, 0, s>>>(part, nwg, part2);
-  k_mlp_adam<<<pb + 1, 256, 0, s>>>(st, part2, nch, loss_part, nwg, lr, b1, b2, eps, bc1, bc2, apply);
+  const uint32_t pb = (kMlpParams + kAdamParams - 1) / kAdamParams;
+  k_mlp_adam<<<pb + 1, 256, 0, s>>>(st, part, nwg, loss_part, lr, b1, b2, eps, bc1, bc2, apply);
   return hipGetLastError();
 }
 hipError_t launch_mlp_infer(const MlpState& st, const float* x, uint32_t n, float* y, hipStream_t s) {
   if (!x || !y) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
-  k_mlp_infer<<<(n + 63) / 64, 64, 0, s>>>(st.dev, x, n, y);
+  const uint32_t chunks = (n + 63) / 64;
+  uint32_t grid = (chunks + kTrainWaves - 1) / kTrainWaves;
+  if (grid > 2048) grid = 2048;
+  k_mlp_infer<<<grid, 64 * kTrainWaves, 0, s>>>(st.dev, x, n, y);
   return hipGetLastError();
 }
 __global__ void k_mlp_pack(MlpState st) {
@@ -453,7 +602,6 @@ hipError_t launch_mlp_err(const float* y, const uint32_t* sample_n, uint32_t nd,
   return hipGetLastError();
 }
 
-size_t mlp_lds_bytes() { return sizeof(MlpLds); }
-uint32_t mlp_reduce_chunks(uint32_t nwg) { return (nwg + kReduceChunk - 1) / kReduceChunk; }
+size_t mlp_lds_bytes() { return sizeof(MlpTrainLds); }
 
 }  // namespace cpt

@@ -58,29 +58,37 @@ def sphere_grid_poses(n, seed0=POSE_SEED, grid=64, spacing=2.0):
     return orbit_poses((0.0, 0.0, 0.0), (half * 0.6, half + 30.0), (15.0, 45.0), half / 2.0, n, seed0)
 
 
+def heightfield_poses(n, seed0=POSE_SEED, width=1920, height=1080):
+    """n seeded views over the heightfield (scenes.heightfield_poses), all at
+    width x height: pose i from seed seed0 + i."""
+    from .scenes import heightfield_poses as hp
+    return [dict(p, width=width, height=height) for p in hp(n, seed0=seed0, resolutions=((width, height),))]
+
+
 def converged(scene: Scene, cam, W, H, spp, depth=8, in_flight=64, seed=REF_SEED):
     R = Renderer(W, H, mode=CPT_MODE_BDPT_STREAM, eye_depth=depth, light_depth=depth, seed=seed)
     R.set_attrib("Frames In Flight", min(in_flight, spp))
     return R.render(scene, cam, frames=spp)
 
 
-def _adaptive(W, H, depth, pass_frames=1, signal=0, mode=CPT_MODE_BDPT_ADAPTIVE_PLL, **attribs):
-    R = Renderer(W, H, mode=mode, eye_depth=depth, light_depth=depth, **attribs)
+def _adaptive(W, H, depth, pass_frames=1, signal=0, mode=CPT_MODE_BDPT_ADAPTIVE_PLL, attribs=None):
+    R = Renderer(W, H, mode=mode, eye_depth=depth, light_depth=depth, **(attribs or {}))
     R.set_attrib("Adaptive Pass Frames", pass_frames)
     R.set_attrib("Learned Signal", signal)
     return R
 
 
 def collect(scene: Scene, cams, refs, W, H, checkpoints=(6, 8, 12, 16, 24, 32, 48, 64), keep=0.125, depth=8,
-            seed=0xDA7A, pass_frames=1, clamp=CLAMP):
-    """Features and targets of plain adaptive runs at the checkpoint frames."""
+            seed=0xDA7A, pass_frames=1, clamp=CLAMP, attribs=None):
+    """Features and targets of plain adaptive runs at the checkpoint frames
+    (attribs: adaptive renderer attributes, e.g. scenes.ADAPTIVE_C3)."""
     rng = np.random.default_rng(seed)
     n = W * H
     dx, dt = DeviceBuffer(n * 36), DeviceBuffer(n * 4)
     X, T = [], []
     for cam, ref in zip(cams, refs):
         d_ref = DeviceBuffer.from_array(np.ascontiguousarray(ref, np.float32))
-        R = _adaptive(W, H, depth, pass_frames)
+        R = _adaptive(W, H, depth, pass_frames, attribs=attribs)
         done = 0
         for c in checkpoints:
             R.render(scene, cam, frames=c - done, result=False)
@@ -105,12 +113,14 @@ def train(model: Mlp, X, T, steps=2000, batch=65536, lr=1e-3, seed=0x7EA1, log_e
     synchronize()
     t0 = time.perf_counter()
     nb = m // batch
-    for s in range(steps):
-        off = (s % nb) * batch
-        want = (s + 1) % log_every == 0 or s == 0 or s == steps - 1
-        loss = model.train_step_device(dx, dt, batch, lr=lr, want_loss=want, x_offset=off)
-        if want:
-            losses.append((s + 1, loss))
+    first = model.train_step_device(dx, dt, batch, lr=lr, want_loss=True)      # step 1 reads batch 0
+    losses.append((1, first))
+    done = 1
+    while done < steps:                       # the rest in device-side runs between loss readings
+        k = min(log_every - done % log_every, steps - done)
+        loss = model.train_steps_device(dx, dt, nb * batch, batch, k, lr=lr, first_batch=done % nb, want_loss=True)
+        done += k
+        losses.append((done, loss))
     synchronize()
     return losses, time.perf_counter() - t0
 
@@ -122,12 +132,12 @@ def mse(img, ref, clamp=CLAMP) -> float:
 
 
 def mse_curve(scene: Scene, cam, ref, W, H, frames, model: Mlp | None = None, mode=CPT_MODE_BDPT_ADAPTIVE_PLL,
-              depth=8, stop_below=None, pass_frames=1, clamp=CLAMP, signal=0):
+              depth=8, stop_below=None, pass_frames=1, clamp=CLAMP, signal=0, attribs=None):
     """(frames, cumulative render seconds, MSE) after each render call of
     pass_frames frames.  With stop_below the run ends at the first call whose
     MSE is at or below it."""
     if mode != CPT_MODE_BDPT_STREAM:
-        R = _adaptive(W, H, depth, pass_frames, signal, mode=mode)
+        R = _adaptive(W, H, depth, pass_frames, signal, mode=mode, attribs=attribs)
         if model is not None:
             R.set_mlp(model)
     else:
@@ -173,19 +183,20 @@ def time_to_reach(ts, ms, target):
 
 
 def time_to_equal_mse(scene: Scene, model: Mlp, cams, refs, W, H, frames=64, depth=8, max_frames=None,
-                      pass_frames=1, clamp=CLAMP, signal=0):
+                      pass_frames=1, clamp=CLAMP, signal=0, attribs=None):
     """Per held-out pose: plain adaptive time, learned time and plain BDPT time
     to plain adaptive's MSE after `frames` frames."""
     max_frames = max_frames or 4 * frames
     # warm the code paths once (module load, first allocations)
-    mse_curve(scene, cams[0], refs[0], W, H, 2 * pass_frames + 5, model, depth=depth, pass_frames=pass_frames)
+    mse_curve(scene, cams[0], refs[0], W, H, 2 * pass_frames + 5, model, depth=depth, pass_frames=pass_frames,
+              attribs=attribs)
     rows = []
     for cam, ref in zip(cams, refs):
         fp, tp, mp = mse_curve(scene, cam, ref, W, H, frames, None, depth=depth, pass_frames=pass_frames,
-                               clamp=clamp)
+                               clamp=clamp, attribs=attribs)
         target = float(mp[-1])
         fl, tl, ml = mse_curve(scene, cam, ref, W, H, max_frames, model, depth=depth, stop_below=target,
-                               pass_frames=pass_frames, clamp=clamp, signal=signal)
+                               pass_frames=pass_frames, clamp=clamp, signal=signal, attribs=attribs)
         fu, tu, mu = mse_curve(scene, cam, ref, W, H, max_frames, None, mode=CPT_MODE_BDPT_STREAM, depth=depth,
                                stop_below=target, pass_frames=pass_frames, clamp=clamp)
         at = np.searchsorted(fl, frames)

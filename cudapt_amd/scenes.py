@@ -284,17 +284,66 @@ def _value_noise(rng, shape, cells):
     return (a * (1 - fx) + b * fx) * (1 - fy) + (c * (1 - fx) + d * fx) * fy
 
 
+# Adaptive-sampling settings of the heightfield workload, as renderer attributes:
+# filter radius 7, colour / normal / relative-depth sigmas 0.2 / 0.1 / 0.05, a
+# spatial sigma of radius / 2 (the reference would take the colour sigma, which
+# at 0.2 leaves only the centre tap), the position term off (the depth term
+# replaces it), and 4 to 256 eye paths per pixel.
+ADAPTIVE_C3 = {"Filter Radius": 7, "Filter Color EuD": 0.2, "Filter Norm EuD": 0.1, "Filter Depth EuD": 0.05,
+               "Filter Spatial EuD": 3.5, "Filter Pos EuD": 1e4, "Min Pixel Paths": 4, "Max Pixel Paths": 256}
+
+
+
+def env_directions(width, height):
+    """Unit directions of the texel centres of a width x height latitude-longitude
+    map (u = atan2(z, x) / 2pi + 1/2 across, v = acos(y) / pi down)."""
+    u = (np.arange(width) + 0.5) / width
+    v = (np.arange(height) + 0.5) / height
+    phi = (u - 0.5) * 2 * math.pi
+    theta = v * math.pi
+    st = np.sin(theta)[:, None]
+    return np.stack([st * np.cos(phi)[None, :], np.cos(theta)[:, None] * np.ones((1, width)),
+                     st * np.sin(phi)[None, :]], -1)
+
+
+def env_lobes(width=512, height=256, seed=0xE7, lobes=8):
+    """HDR environment of seeded Gaussian lobes: a sky gradient over the upper
+    hemisphere, a dim ground below, a sun lobe and `lobes - 1` broader coloured
+    lobes above the horizon, each I * exp(k (d . mu - 1)).  FP32 RGB, H x W x 3."""
+    rng = np.random.default_rng(seed)
+    d = env_directions(width, height)
+    up = np.clip(d[..., 1], 0.0, 1.0)[..., None]
+    sky = np.array([0.35, 0.45, 0.65]) * (0.4 + 0.6 * up) * (d[..., 1:2] >= 0)
+    ground = np.array([0.08, 0.07, 0.06]) * (d[..., 1:2] < 0)
+    img = sky + ground
+    for k in range(lobes):
+        el = rng.uniform(math.radians(10), math.radians(70))
+        az = rng.uniform(0, 2 * math.pi)
+        mu = np.array([math.cos(el) * math.cos(az), math.sin(el), math.cos(el) * math.sin(az)])
+        if k == 0:                                        # the sun: small and bright
+            kappa, inten, col = 2000.0, 60.0, np.array([1.0, 0.92, 0.8])
+        else:
+            kappa, inten, col = rng.uniform(20, 300), rng.uniform(0.5, 4.0), rng.uniform(0.3, 1.0, 3)
+        img = img + inten * col * np.exp(kappa * (d @ mu - 1.0))[..., None]
+    return img.astype(np.float32)
+
+
 def heightfield(nx=1024, nz=512, boxes=200, seed=0xC3, extent=(256.0, 128.0), amplitude=18.0, octaves=6,
-                light_radiance=12.0):
+                light_radiance=12.0, light="spheres"):
     """Adaptive-sampling workload: an nx x nz fBm heightfield (octaves of value
     noise, 2 triangles per cell: 1,048,576 triangles at 1024 x 512) with
-    `boxes` random boxes (12 triangles each) standing on it, lit by four
-    emissive icospheres above it.
+    `boxes` random boxes (12 triangles each) standing on it.
 
-    The stated workload lights the scene with an HDR environment map, which the
-    reference estimator cannot evaluate (it knows emissive triangles only,
-    cudaRTBDPTStream.cu:57-77); the emitters here take its place.
+    light = "env": lit by the stated 512 x 256 FP32 HDR environment of seeded
+    Gaussian lobes (env_lobes), the environment-light extension; no emissive
+    triangle, so there are no light subpaths and eye paths gather the
+    environment when they leave the scene.
+    light = "spheres": lit by four emissive icospheres above it instead, the
+    reference estimator's own kind of light (it knows emissive triangles only,
+    cudaRTBDPTStream.cu:57-77).
     """
+    if light not in ("spheres", "env"):
+        raise ValueError("light: 'spheres' or 'env'")
     rng = np.random.default_rng(seed)
     ex, ez = extent
     hgt = np.zeros((nz + 1, nx + 1))
@@ -330,11 +379,12 @@ def heightfield(nx=1024, nz=512, boxes=200, seed=0xC3, extent=(256.0, 128.0), am
         mats.append(material(diffuse=tuple(0.2 + 0.6 * par[:3]), metallic=par[3], roughness=0.05 + 0.95 * par[4],
                              specular=par[5], sheen=rng.uniform(), clearcoat=rng.uniform()))
         bld.add_box((bx, hgt[iz, ix] + half[1] * 0.8, bz), tuple(half), rng.uniform(0, 2 * math.pi), m)
-    lm = len(mats)
-    mats.append(material(diffuse=(0.0, 0.0, 0.0), emissive=(light_radiance,) * 3))
-    for lx, lz in ((-ex / 4, -ez / 4), (ex / 4, -ez / 4), (-ex / 4, ez / 4), (ex / 4, ez / 4)):
-        p, n, f = icosphere((lx, amplitude * 2.5 + 30.0, lz), 6.0, 1)
-        bld.add_mesh(p, n, f, lm)
+    if light == "spheres":
+        lm = len(mats)
+        mats.append(material(diffuse=(0.0, 0.0, 0.0), emissive=(light_radiance,) * 3))
+        for lx, lz in ((-ex / 4, -ez / 4), (ex / 4, -ez / 4), (-ex / 4, ez / 4), (ex / 4, ez / 4)):
+            p, n, f = icosphere((lx, amplitude * 2.5 + 30.0, lz), 6.0, 1)
+            bld.add_mesh(p, n, f, lm)
     mats_arr = np.zeros(len(mats), MATERIAL_DTYPE)
     for j, m in enumerate(mats):
         mats_arr[j] = m
@@ -345,7 +395,8 @@ def heightfield(nx=1024, nz=512, boxes=200, seed=0xC3, extent=(256.0, 128.0), am
     return dict(pos=np.concatenate([pos.astype(np.float32), np.asarray(bld.pos, np.float32).reshape(-1, 3)]),
                 nrm=np.concatenate([nrm.astype(np.float32), np.asarray(bld.nrm, np.float32).reshape(-1, 3)]),
                 idx=np.concatenate([idx.astype(np.uint32), np.asarray(bld.idx, np.uint32).reshape(-1, 3)]),
-                mat=np.concatenate(mat + [extra_mat]), mats=mats_arr, camera=cam)
+                mat=np.concatenate(mat + [extra_mat]), mats=mats_arr, camera=cam,
+                env=env_lobes(seed=seed ^ 0xE7) if light == "env" else None)
 
 
 def heightfield_poses(n, seed0=0xCA3E0000, resolutions=((256, 256), (640, 480), (1280, 720), (1920, 1080)),

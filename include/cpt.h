@@ -101,6 +101,16 @@ int cpt_scene_update_materials(cpt_scene* s);
 int cpt_scene_add_texture(cpt_scene* s, uint32_t width, uint32_t height, const float* rgba, uint32_t* id);
 int cpt_scene_clear_textures(cpt_scene* s);
 
+/* No reference counterpart (the reference lights scenes with emissive
+ * triangles only, src/cudaRTBDPTStream.cu:57-77): an environment light, a
+ * width x height latitude-longitude map of RGB radiance (3 floats per texel,
+ * row-major, row 0 looking up +y; u = atan2(d.z, d.x) / 2pi + 1/2 across,
+ * v = acos(d.y) / pi down, bilinear, u wrapping).  An eye path that leaves the
+ * scene adds the map's radiance along its direction times its throughput;
+ * light subpaths still start on emissive triangles only.  width = 0 removes
+ * it.  The next render uploads it and restarts accumulation. */
+int cpt_scene_set_environment(cpt_scene* s, uint32_t width, uint32_t height, const float* rgb);
+
 /* NPBVHHelper::CreateBVH + CompactBVH::InitialCompactBVH
  * (include/bvhhelper.h:43-54, src/bvhhelper.cpp:55-311): binned SAH, 16 bins,
  * depth <= max_depth (reference default 32), then the triangle reorder of
@@ -210,8 +220,13 @@ enum {
                                         variance, rounds with the window at the maximum error */
   CPT_MODE_PT_STREAM = 4,            /* cudaRTPTStream (src/cudaRTPTStream.cu): unidirectional path tracing
                                         over the same stream, eye depth 10, a per-frame running mean */
-  CPT_MODE_PT_STREAM_REGEN = 5       /* cudaRTPTStreamRegen (src/cudaRTPTStreamRegen.cu): as 4, and a pixel
+  CPT_MODE_PT_STREAM_REGEN = 5,      /* cudaRTPTStreamRegen (src/cudaRTPTStreamRegen.cu): as 4, and a pixel
                                         whose path ends while the frame is still tracing restarts once */
+  CPT_MODE_PT_DISNEY_MIS = 6         /* no reference counterpart (the reference loads the Disney parameters and
+                                        never reads them, src/cudaRTCommon.cu:398-405): a path tracer with
+                                        Burley's Disney BRDF (isotropic, opaque), next-event estimation to the
+                                        emissive triangles and power-heuristic MIS, Russian roulette from the
+                                        fourth bounce, eye depth 8; one sample per path */
 };
 
 /* RTRenderer::Init(w, h) + SetRendererMode(mode) (src/raytracer.cpp:574-608). */
@@ -266,6 +281,8 @@ typedef struct cpt_stats {
   double filter_ms, error_ms, gen_ms, accum_ms;   /* adaptive phases (PERFBREAKDOWN, AdapBilatAtomic.cu:1409-1536) */
   double mlp_ms;                                  /* learned error model: features + inference */
   uint64_t shadow_nodes, shadow_tris;             /* 4-wide nodes visited / triangles tested by shadow rays */
+  uint64_t extend_spills, shadow_spills;          /* traversal-stack pushes that went past the LDS entries into
+                                                     the private (scratch) stack, refilling-wave kernels */
 } cpt_stats;
 int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset);
 
@@ -317,8 +334,29 @@ int cpt_batch_set_config(cpt_batch* b, const cpt_batch_config* cfg);
 int cpt_batch_render(cpt_batch* b, cpt_scene* s, uint32_t frames);
 int cpt_batch_read_result(cpt_batch* b, uint32_t view, float* out);      /* W*H*3 running mean */
 int cpt_batch_sample_counts(cpt_batch* b, uint32_t view, uint32_t* out); /* W*H */
+/* One view's buffer (CPT_BUF_*, as cpt_renderer_read_buffer reads it for a
+ * one-view renderer): the pixels of that view. */
+int cpt_batch_read_buffer(cpt_batch* b, uint32_t view, int which, void* out);
 int cpt_batch_stats(cpt_batch* b, cpt_stats* out, int reset);
 int cpt_batch_reset(cpt_batch* b);
+/* Per-view adaptive sampling.  mode CPT_MODE_BDPT_STREAM: plain frames (the
+ * default).  CPT_MODE_BDPT_ADAPTIVE_PLL: every view runs the frames of
+ * cudaRTBDPTStreamAdapBilatAtomic::Render (AdapBilatAtomic.cu:1338-1538) -
+ * warm-up frames, then per pass of pass_frames frames its own cross-bilateral
+ * filter and error sum (or, with `model`, the learned error estimate) and a
+ * proportional allocation of pass_frames x its pixel count paths over its own
+ * pixels; the paths of all views are traced as one wavefront.  Each view's
+ * image equals that of a one-view adaptive renderer of its camera with the
+ * same attributes, seed and model, bit for bit.  Replaces the batch's
+ * renderer: attributes set before are dropped (set them after), the
+ * configuration is kept, accumulation restarts.  Adaptive batches are not
+ * autotuned (tune the plain batch first). */
+int cpt_batch_set_adaptive(cpt_batch* b, int mode, uint32_t pass_frames, struct cpt_mlp* model);
+/* Sets an attribute of the batch's renderer by name (e.g. the adaptive
+ * mode's "Filter Radius", "Max Pixel Paths", "Learned Signal"); those the
+ * configuration owns (depths, seed, frames in flight, schedule) are refused.
+ * Accumulation restarts. */
+int cpt_batch_set_attrib(cpt_batch* b, const char* name, double value);
 /* Autotuner: times `iterations` iterations of each candidate configuration
  * (extend chunk x workgroups per CU x traversal variant, with the current
  * frames in flight) and keeps the fastest.  *best receives it; ms (may be
@@ -363,6 +401,13 @@ int cpt_mlp_get_grad(cpt_mlp* m, float* grad, uint32_t n);
  * makes the call synchronous.  stream NULL = the model's own stream. */
 int cpt_mlp_train_step(cpt_mlp* m, const float* d_x, const float* d_target, uint32_t n, float lr, int apply,
                        float* loss, void* stream);
+/* No reference counterpart: `steps` training steps with Adam at rate lr over
+ * a device data set of n_samples samples (x n_samples x 9, target n_samples),
+ * step k taking the batch of samples [j b, (j + 1) b), j = (first_batch + k)
+ * mod (n_samples / b); one host call, no synchronisation unless *loss (host,
+ * may be NULL) asks for the last step's loss. */
+int cpt_mlp_train_steps(cpt_mlp* m, const float* d_x, const float* d_target, uint32_t n_samples, uint32_t batch,
+                        uint32_t first_batch, uint32_t steps, float lr, float* loss, void* stream);
 /* Inference of n samples into d_y (device FP32). */
 int cpt_mlp_infer(cpt_mlp* m, const float* d_x, uint32_t n, float* d_y, void* stream);
 

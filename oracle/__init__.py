@@ -35,6 +35,8 @@ def lib():
         L.orc_scene_create.argtypes = [f32p, f32p, c32, u32p, u32p, c32, vp, c32, c32]
         L.orc_scene_destroy.argtypes = [vp]
         L.orc_scene_set_uv.argtypes = [vp, f32p]
+        L.orc_scene_set_env.argtypes = [vp, c32, c32, vp]
+        L.orc_env_radiance.argtypes = [vp, f64p, ctypes.c_int, f64p]
         L.orc_scene_add_texture.restype = c32
         L.orc_scene_add_texture.argtypes = [vp, c32, c32, f32p]
         L.orc_tex_sample.argtypes = [vp, c32, ctypes.c_double, ctypes.c_double, ctypes.c_int,
@@ -71,6 +73,8 @@ def lib():
         L.orc_adaptive_state.argtypes = [vp, c32, vp, vp]
         L.orc_bilateral_defined.argtypes = [f32p, f32p, f32p, vp, c32, c32, ctypes.c_float, ctypes.c_float,
                                             ctypes.c_float, ctypes.c_float, c32, f32p]
+        L.orc_bilateral_ext.argtypes = [f32p, f32p, f32p, vp, c32, c32] + [ctypes.c_float] * 6 + [f32p, c32, c32,
+                                                                                                  f32p]
         L.orc_expf.restype = ctypes.c_float
         L.orc_expf.argtypes = [ctypes.c_float]
         d, i32p = ctypes.c_double, np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
@@ -84,6 +88,10 @@ def lib():
         L.orc_basis.argtypes = [ci, f64p, f64p, f64p]
         L.orc_shading.argtypes = [ci, f64p, f64p]
         L.orc_sample_bounce_many.argtypes = [ci, f64p, f64p, d, f64p, f64p, c64, c32, ci, i32p, f64p, f64p, f64p]
+        L.orc_disney_f.argtypes = [f64p] * 4 + [f64p]
+        L.orc_disney_pdf.restype = d
+        L.orc_disney_pdf.argtypes = [f64p] * 4
+        L.orc_disney_sample_many.argtypes = [f64p, f64p, f64p, c64, c32, f64p, i32p]
         L.orc_tree_sum.restype = ctypes.c_double
         L.orc_tree_sum.argtypes = [f32p, c32]
         _lib = L
@@ -98,7 +106,7 @@ RENDER_PARAMS_DTYPE = np.dtype([
     ("seed", "<u8"), ("row0", "<u4"), ("rows", "<u4"),
     ("mode", "<u4"), ("nd", "<u4"), ("sc", "<f4"), ("sp", "<f4"), ("sn", "<f4"), ("sd", "<f4"),
     ("radius", "<u4"), ("filter_diff", "<u4"), ("warmup", "<u4"), ("pass_frames", "<u4"),
-    ("min_prob", "<f4"), ("pad3", "<u4")])
+    ("min_prob", "<f4"), ("pad3", "<u4"), ("ss", "<f4"), ("sz", "<f4"), ("min_paths", "<u4"), ("max_paths", "<u4")])
 
 
 def pcg32(initstate: int, initseq: int, n: int) -> np.ndarray:
@@ -128,6 +136,21 @@ class Scene:
         self.textures = []
         for t in sc.get("textures", ()):
             self.add_texture(t)
+        if sc.get("env") is not None:
+            self.set_environment(sc["env"])
+
+    def set_environment(self, rgb):
+        """H x W x 3 latitude-longitude radiance map (None removes it)."""
+        if rgb is None:
+            lib().orc_scene_set_env(self.h, 0, 0, None)
+            return
+        self.env = np.ascontiguousarray(rgb, np.float32)
+        lib().orc_scene_set_env(self.h, self.env.shape[1], self.env.shape[0], self.env.ctypes.data)
+
+    def env_radiance(self, d, prec=64):
+        out = np.zeros(3)
+        lib().orc_env_radiance(self.h, np.ascontiguousarray(d, np.float64), prec, out)
+        return out
 
     def add_texture(self, rgba) -> int:
         t = np.ascontiguousarray(rgba, np.float32)
@@ -186,7 +209,7 @@ def max_threads() -> int:
 
 def render_params(width, height, camera, eye_depth=5, light_depth=5, light_paths=128, seed=0x5EED0001, flags=0,
                   row0=0, rows=0, mode=0, nd=5, sc=50.0, sp=1.0, sn=0.25, sd=0.1, radius=5, filter_diff=0,
-                  warmup=5, pass_frames=1, min_prob=0.0):
+                  warmup=5, pass_frames=1, min_prob=0.0, ss=0.0, sz=0.0, min_paths=0, max_paths=0):
     """Reference-estimator parameters; mode 1 = adaptive proportional allocation with the
     reference's defaults (AdapBilatAtomic.cu:28-38)."""
     p = np.zeros((), RENDER_PARAMS_DTYPE)
@@ -201,6 +224,7 @@ def render_params(width, height, camera, eye_depth=5, light_depth=5, light_paths
     p["radius"], p["filter_diff"], p["warmup"] = radius, filter_diff, warmup
     p["pass_frames"] = pass_frames
     p["min_prob"] = min_prob
+    p["ss"], p["sz"], p["min_paths"], p["max_paths"] = ss, sz, min_paths, max_paths
     return p
 
 
@@ -246,7 +270,8 @@ class Renderer:
         for k, name, shape, dt in ((0, "gpos", (self.H, self.W, 3), np.float32), (1, "gnorm", (self.H, self.W, 3), np.float32),
                                    (2, "gdiff", (self.H, self.W, 3), np.float32), (3, "filtered", (self.H, self.W, 3), np.float32),
                                    (4, "err", (self.H, self.W), np.float32), (5, "counts", (self.H, self.W), np.uint32),
-                                   (6, "sq", (self.H, self.W, 3), np.float32), (7, "rounds", (1,), np.uint32)):
+                                   (6, "sq", (self.H, self.W, 3), np.float32), (7, "rounds", (1,), np.uint32),
+                                   (8, "paths", (self.H, self.W), np.uint32)):
             a = np.zeros(shape, dt)
             s = np.zeros(1, np.float32)
             lib().orc_adaptive_state(self.h, k, a.ctypes.data, s.ctypes.data)
@@ -318,6 +343,32 @@ def sample_bounce_many(ro, rd, t, norm, diff, trans, specular, metallic, roughne
     return ty, o, d, m
 
 
+def _disney_m(m):
+    """Material vector of the Disney checks from a dict (base, metallic, subsurface,
+    specular, roughness, sheen, sheenTint, clearcoat, clearcoatGloss)."""
+    keys = ("metallic", "subsurface", "specular", "roughness", "sheen", "sheenTint", "clearcoat", "clearcoatGloss")
+    return np.ascontiguousarray(list(m["base"]) + [float(m.get(k, 0.0)) for k in keys], np.float64)
+
+
+def disney_f(m, n, wo, wi):
+    """The Disney BRDF of the extension mode (FP64)."""
+    out = np.zeros(3)
+    lib().orc_disney_f(_disney_m(m), _v(n), _v(wo), _v(wi), out)
+    return out
+
+
+def disney_pdf(m, n, wo, wi):
+    return lib().orc_disney_pdf(_disney_m(m), _v(n), _v(wo), _v(wi))
+
+
+def disney_sample_many(m, n, wo, seed, count):
+    """count BRDF samples: (directions count x 3, valid flags)."""
+    wi = np.zeros((count, 3))
+    ok = np.zeros(count, np.int32)
+    lib().orc_disney_sample_many(_disney_m(m), _v(n), _v(wo), int(seed), int(count), wi.reshape(-1), ok)
+    return wi, ok.astype(bool)
+
+
 def sample_bounce(ro, rd, t, norm, diff, trans, specular, metallic, roughness, nl, state, seq, tir=False):
     """One FP32 BRDF sample at a hit: (origin, direction, throughput, ray type 1 diffuse / 2 specular)."""
     f = lambda v: np.ascontiguousarray(v, np.float32)
@@ -386,6 +437,19 @@ def bilateral_defined(color, pos, norm, emit, sc, sp, sn, se, radius):
                                 np.ascontiguousarray(pos, np.float32).ravel(),
                                 np.ascontiguousarray(norm, np.float32).ravel(), e.ctypes.data if e is not None else None,
                                 W, H, sc, sp, sn, se, radius, out.ravel())
+    return out
+
+
+def bilateral_ext(color, pos, norm, emit, sc, sp, sn, se, radius, ss=0.0, sz=0.0, cam=(0.0, 0.0, 0.0), prec=32):
+    """The filter with a spatial sigma of its own (ss > 0) and the relative-depth
+    term ((z_q - z_p) / z_p)^2 / 2 sz^2 (sz > 0), z = |position - cam|; prec 32 the
+    defined arithmetic of the device kernel, 64 the literal FP64 product."""
+    H, W = color.shape[:2]
+    out = np.zeros((H, W, 3), np.float32)
+    f = lambda a: np.ascontiguousarray(a, np.float32).reshape(-1)
+    e = f(emit) if emit is not None else None
+    lib().orc_bilateral_ext(f(color), f(pos), f(norm), e.ctypes.data if e is not None else None, W, H, sc, sp, sn, se,
+                            ss, sz, f(cam), radius, prec, out.reshape(-1))
     return out
 
 

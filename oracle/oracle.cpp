@@ -197,6 +197,8 @@ struct Scene {
   std::vector<float> uv;             // 2 per vertex (zeros unless set)
   std::vector<float> tex;            // RTScene::m_pTextures: RGBA texels, concatenated
   std::vector<u32> texDesc;          // 3 per texture: first texel, width, height
+  std::vector<float> env;            // environment light: envW x envH RGB texels (empty: none)
+  u32 envW = 0, envH = 0;
 };
 
 // ---------------------------------------------------------------- BVH build
@@ -563,6 +565,118 @@ static inline V3<R> shading(V3<R> outDir, V3<R> inDir, V3<R> irrad, V3<R> n, V3<
   return vmul(irrad * nol, specI * sw + diffI * dw);
 }
 
+// ---------------------------------------------------------------- Disney BRDF (extension)
+// No reference counterpart: CudaPT reads the Disney parameters into RTMaterial
+// (include/raytracer.h:82-100) and never uses them (src/cudaRTCommon.cu:398-405).
+// The principled BRDF of Burley, "Physically Based Shading at Disney" (SIGGRAPH
+// 2012 course notes, and its reference shader brdf.txt), isotropic, specular tint
+// 0 (RTMaterial has none), opaque:
+//   f = (1 - metallic) [ lerp(Fd, ss, subsurface) base / pi + Fsheen ]
+//       + Ds Fs Gs + clearcoat / 4 * Dr Fr Gr
+//   Fd = (1 + (Fd90 - 1) FL)(1 + (Fd90 - 1) FV), Fd90 = 1/2 + 2 LH^2 r
+//   ss = 1.25 (Fss (1 / (NL + NV) - 1/2) + 1/2), Fss from Fss90 = LH^2 r
+//   Fsheen = (1 - LH)^5 sheen lerp(1, tint, sheenTint), tint = base / lum(base)
+//   Ds = GTR2(NH, a), a = max(1e-3, r^2); Fs = lerp(Cspec0, 1, (1 - LH)^5),
+//   Cspec0 = lerp(0.08 specular, base, metallic); Gs = sG(NL, a) sG(NV, a),
+//   sG(n, a) = 1 / (n + sqrt(a^2 + n^2 - a^2 n^2)) (Smith GGX with the 1 / 4 NL NV)
+//   Dr = GTR1(NH, lerp(0.1, 0.001, clearcoatGloss)), Fr = lerp(0.04, 1, (1 - LH)^5),
+//   Gr = sG(NL, 0.25) sG(NV, 0.25)
+// Sampling: a lobe with probability proportional to (1 - metallic) (diffuse:
+// cosine-weighted), 1 (specular: GGX half vector) and clearcoat / 4 (GTR1 half
+// vector); the pdf of a direction is the mixture of the three.
+template <class R> struct Disney {
+  V3<R> base;
+  R metallic, subsurface, specular, roughness, sheen, sheenTint, clearcoat, clearcoatGloss;
+};
+template <class R> static inline R sqr(R x) { return x * x; }
+template <class R> static inline R schlick_w(R c) { R m = (R)1 - c; m = m < (R)0 ? (R)0 : (m > (R)1 ? (R)1 : m); return (m * m) * (m * m) * m; }
+template <class R> static inline R gtr1(R nh, R a) {
+  if (a >= (R)1) return (R)(1.0 / kPi);
+  R a2 = a * a, t = (R)1 + (a2 - (R)1) * nh * nh;
+  return (a2 - (R)1) / ((R)kPi * std::log(a2) * t);
+}
+template <class R> static inline R gtr2(R nh, R a) {
+  R a2 = a * a, t = (R)1 + (a2 - (R)1) * nh * nh;
+  return a2 / ((R)kPi * t * t);
+}
+template <class R> static inline R smith_g(R n, R a) { R a2 = a * a, b = n * n; return (R)1 / (n + std::sqrt(a2 + b - a2 * b)); }
+template <class R> static inline R disney_alpha(const Disney<R>& m) { R a = m.roughness * m.roughness; return a < (R)1e-3 ? (R)1e-3 : a; }
+template <class R> static inline R disney_cc_alpha(const Disney<R>& m) { return lerp((R)0.1, (R)0.001, m.clearcoatGloss); }
+template <class R> static inline void disney_lobes(const Disney<R>& m, R& pd, R& ps, R& pc) {
+  R wd = (R)1 - m.metallic, ws = (R)1, wc = (R)0.25 * m.clearcoat;
+  R t = (wd + ws) + wc;
+  pd = wd / t; ps = ws / t; pc = wc / t;
+}
+template <class R>
+static V3<R> disney_f(const Disney<R>& m, V3<R> n, V3<R> wo, V3<R> wi) {
+  R nl = dot(n, wi), nv = dot(n, wo);
+  if (nl <= (R)0 || nv <= (R)0) return mk((R)0, (R)0, (R)0);
+  V3<R> hv = wi + wo;
+  R hl = length(hv);
+  if (!(hl > (R)0)) return mk((R)0, (R)0, (R)0);
+  V3<R> h = hv / hl;
+  R nh = dot(n, h), lh = dot(wi, h);
+  R lum = (R)0.3 * m.base.x + (R)0.6 * m.base.y + (R)0.1 * m.base.z;
+  V3<R> tint = lum > (R)0 ? m.base / lum : mk((R)1, (R)1, (R)1);
+  V3<R> cspec0 = vlerp(mk((R)1, (R)1, (R)1) * ((R)0.08 * m.specular), m.base, m.metallic);
+  V3<R> csheen = vlerp(mk((R)1, (R)1, (R)1), tint, m.sheenTint);
+  R fl = schlick_w(nl), fv = schlick_w(nv), fh = schlick_w(lh);
+  R fd90 = (R)0.5 + (R)2 * lh * lh * m.roughness;
+  R fd = ((R)1 + (fd90 - (R)1) * fl) * ((R)1 + (fd90 - (R)1) * fv);
+  R fss90 = lh * lh * m.roughness;
+  R fss = ((R)1 + (fss90 - (R)1) * fl) * ((R)1 + (fss90 - (R)1) * fv);
+  R ss = (R)1.25 * (fss * ((R)1 / (nl + nv) - (R)0.5) + (R)0.5);
+  R a = disney_alpha(m);
+  R ds = gtr2(nh, a);
+  V3<R> fs = vlerp(cspec0, mk((R)1, (R)1, (R)1), fh);
+  R gs = smith_g(nl, a) * smith_g(nv, a);
+  V3<R> fsheen = csheen * (fh * m.sheen);
+  R dr = gtr1(nh, disney_cc_alpha(m));
+  R fr = lerp((R)0.04, (R)1, fh);
+  R gr = smith_g(nl, (R)0.25) * smith_g(nv, (R)0.25);
+  V3<R> diffuse = (m.base * ((R)(1.0 / kPi) * lerp(fd, ss, m.subsurface)) + fsheen) * ((R)1 - m.metallic);
+  return (diffuse + fs * (gs * ds)) + mk((R)1, (R)1, (R)1) * ((R)0.25 * m.clearcoat * gr * fr * dr);
+}
+template <class R>
+static R disney_pdf(const Disney<R>& m, V3<R> n, V3<R> wo, V3<R> wi) {
+  R nl = dot(n, wi), nv = dot(n, wo);
+  if (nl <= (R)0 || nv <= (R)0) return (R)0;
+  V3<R> hv = wi + wo;
+  R hl = length(hv);
+  if (!(hl > (R)0)) return (R)0;
+  V3<R> h = hv / hl;
+  R nh = dot(n, h), vh = dot(wo, h);
+  if (vh <= (R)0) return (R)0;
+  R pd, ps, pc;
+  disney_lobes(m, pd, ps, pc);
+  R pdf = pd * nl * (R)(1.0 / kPi);
+  pdf = pdf + ps * gtr2(nh, disney_alpha(m)) * nh / ((R)4 * vh);
+  pdf = pdf + pc * gtr1(nh, disney_cc_alpha(m)) * nh / ((R)4 * vh);
+  return pdf;
+}
+// Direction by lobe choice u0 and (u1, u2); false when it leaves below the surface.
+template <class R>
+static bool disney_sample(const Disney<R>& m, V3<R> n, V3<R> wo, R u0, R u1, R u2, V3<R>& wi) {
+  R pd, ps, pc;
+  disney_lobes(m, pd, ps, pc);
+  V3<R> u, v; basis(n, u, v);
+  R phi = (R)(2.0 * kPi * (double)u1);
+  if (u0 < pd) {                                   // cosine-weighted hemisphere
+    R ct = std::sqrt(u2), st = std::sqrt((R)1 - u2);
+    wi = normalize((n * ct + (u * st) * std::cos(phi)) + (v * st) * std::sin(phi));
+  } else {
+    R a = u0 < pd + ps ? disney_alpha(m) : disney_cc_alpha(m);
+    R ct;
+    if (u0 < pd + ps) ct = std::sqrt(((R)1 - u2) / ((R)1 + (a * a - (R)1) * u2));                       // GGX
+    else ct = a >= (R)1 ? std::sqrt((R)1 - u2)
+                        : std::sqrt(((R)1 - std::pow(a * a, (R)1 - u2)) / ((R)1 - a * a));                // GTR1
+    R st = std::sqrt(std::fmax((R)0, (R)1 - ct * ct));
+    V3<R> h = (n * ct + (u * st) * std::cos(phi)) + (v * st) * std::sin(phi);
+    wi = normalize(h * ((R)2 * dot(wo, h)) - wo);
+  }
+  return dot(n, wi) > (R)0;
+}
+
 // ---------------------------------------------------------------- BDPT
 struct RenderParams {
   u32 width, height;
@@ -575,7 +689,8 @@ struct RenderParams {
   u32 mode;        // 0 plain (cudaRTBDPTStream), 1 adaptive proportional (AdapBilatAtomic),
                    // 2 rounds with bisection (AdapBilatBinary), 3 empirical variance rounds (AdapEmp),
                    // 4 path tracing (cudaRTPTStream), 5 with regeneration (cudaRTPTStreamRegen),
-                   // 6 as 5 through the reference's own loop (render_pt_literal)
+                   // 6 as 5 through the reference's own loop (render_pt_literal),
+                   // 7 the Disney-BRDF path tracer with next-event estimation and MIS (extension)
   u32 nd;          // DesiredSamplingN
   float sc, sp, sn, sd;   // filter colour / position / normal / diffuse sigmas
   u32 radius;      // filter radius
@@ -585,6 +700,11 @@ struct RenderParams {
                    // error map, traced under the pass's first frame (0 or 1: one frame a pass)
   float min_prob;  // MinTraceProb (rounds)
   u32 pad3;
+  // extensions with no reference counterpart, 0 = off (the reference's behaviour)
+  float ss;        // spatial sigma of the filter table (0: the colour sigma, AdapBilatAtomic.cu:1327)
+  float sz;        // relative-depth sigma: (z_q - z_p) / z_p, z = |first-hit position - camera|
+  u32 min_paths;   // warm-up frames raised to this many
+  u32 max_paths;   // a pixel with this many eye paths gets no more
 };
 
 // Shadow-segment end for a connection.  The reference passes dist - eps
@@ -631,6 +751,34 @@ static void tex_sample(const Scene& s, u32 id, R u, R v, R out[4]) {
   const R ia = (R)1 - a, ib = (R)1 - b;
   for (int c = 0; c < 4; c++)
     out[c] = ((R)t00[c] * ia + (R)t10[c] * a) * ib + ((R)t01[c] * ia + (R)t11[c] * a) * b;
+}
+
+// Environment light (no reference counterpart): latitude-longitude map, u =
+// atan2(d.z, d.x) / 2pi + 1/2, v = acos(d.y) / pi, bilinear between the four
+// texel centres around (u W - 1/2, v H - 1/2), u wrapping, v clamped to the
+// first and last rows; a non-finite direction sees 0.
+template <class R>
+static V3<R> env_radiance(const Scene& s, V3<R> d) {
+  const int W = (int)s.envW, H = (int)s.envH;
+  const R u = std::atan2(d.z, d.x) * (R)(0.5 / kPi) + (R)0.5;
+  const R v = std::acos(std::fmin(std::fmax(d.y, (R)-1), (R)1)) * (R)(1.0 / kPi);
+  const R x = u * (R)W - (R)0.5, y = v * (R)H - (R)0.5;
+  if (!(std::fabs(x) < (R)1e30) || !(std::fabs(y) < (R)1e30)) return mk((R)0, (R)0, (R)0);
+  const R fx = std::floor(x), fy = std::floor(y);
+  const R a = x - fx, b = y - fy;
+  int i0 = (int)fx % W;
+  if (i0 < 0) i0 += W;
+  const int i1 = i0 + 1 == W ? 0 : i0 + 1;
+  const int j0 = std::min(std::max((int)fy, 0), H - 1), j1 = std::min(std::max((int)fy + 1, 0), H - 1);
+  const float* t00 = &s.env[((size_t)j0 * W + i0) * 3];
+  const float* t10 = &s.env[((size_t)j0 * W + i1) * 3];
+  const float* t01 = &s.env[((size_t)j1 * W + i0) * 3];
+  const float* t11 = &s.env[((size_t)j1 * W + i1) * 3];
+  const R ia = (R)1 - a, ib = (R)1 - b;
+  R c[3];
+  for (int k = 0; k < 3; k++)
+    c[k] = ((R)t00[k] * ia + (R)t10[k] * a) * ib + ((R)t01[k] * ia + (R)t11[k] * a) * b;
+  return mk(c[0], c[1], c[2]);
 }
 
 template <class R>
@@ -760,6 +908,7 @@ template <class R> struct FrameState {
   u32 rounds = 0;                      // last round-based allocation: rounds run
   std::vector<float> filtered, err;    // last adaptive frame: filter output, error map
   std::vector<u32> counts;             // last adaptive frame: samples per pixel
+  std::vector<u32> paths;              // eye paths traced per pixel since frame 0
   float err_sum = 0.f;
   u64 ext_rays = 0, shadow_rays = 0;
   std::vector<LightVertex<R>> lv;
@@ -845,6 +994,8 @@ static PathResult<R> eye_path(const Scene& s, const RenderParams& p, const Camer
     HitR<R> h;
     pr.ext_rays++;
     if (!trace_defined<R>(s, orig, dir, (R)kInf, (R)kBiasEps, false, h)) {
+      // an escaping eye path gathers the environment's radiance through its throughput
+      if (s.envW) psample = psample + vmul(env_radiance<R>(s, dir), mul);
       pr.accum = pr.accum + psample; pr.N++; break;
     }
     R w = ((R)1 - h.u) - h.v;
@@ -961,6 +1112,7 @@ static bool pt_step(const Scene& s, PtPath<R>& P, u32 limit, bool regen, u64& ra
   HitR<R> h;
   rays++;
   if (!trace_defined<R>(s, P.orig, P.dir, (R)kInf, (R)kBiasEps, false, h)) {
+    if (s.envW) P.psample = P.psample + vmul(env_radiance<R>(s, P.dir), P.mul);
     P.accum = P.accum + P.psample; P.N++; P.ended = true;
     return true;
   }
@@ -979,6 +1131,136 @@ static bool pt_step(const Scene& s, PtPath<R>& P, u32 limit, bool regen, u64& ra
   P.mul = vmul(P.mul, b.mul); P.dir = b.dir; P.depth++;
   P.orig = b.orig; P.type = b.type;
   return false;
+}
+
+// ---------------------------------------------------------------- Disney path tracer (extension)
+// Mode 7, no reference counterpart: a unidirectional path tracer with the
+// Disney BRDF above and next-event estimation, each emissive-triangle hit
+// weighted against the light sample by the power heuristic (Veach 1997,
+// section 9.2.4).  Light sampling: an emissive triangle uniformly among the
+// scene's (the reference's leaf order), a uniform point on it; emitters emit
+// on both faces, as the reference's do.  Per bounce the stream gives, in
+// order: the Russian-roulette uniform (bounce >= 3; survival
+// min(0.95, max T), survivors reweighted), the light sample (three uniforms,
+// when the scene has emissive triangles) and the BRDF sample (three).  A path
+// has at most eye_depth segments, the light sample's included.  Surfaces are
+// opaque (transparency is not read).  The environment light, when set, is
+// gathered by escaping paths only, with weight 1 (it is never light-sampled).
+// flags bits of the mode's estimator checks: light samples off (emitter hits
+// weighted 1), emitter hits off past the first segment (light samples
+// weighted 1), Russian roulette off; every variant has the same expectation.
+constexpr u32 kDisneyBsdfOnly = 2u, kDisneyLightOnly = 4u, kDisneyNoRoulette = 8u;
+template <class R>
+static Disney<R> disney_of(const Scene& s, u32 leaf, const HitShade<R>& hs) {
+  const Material& mt = s.mats[s.tmat[leaf]];
+  Disney<R> m;
+  m.base = hs.diff; m.metallic = hs.metallic; m.specular = hs.specular; m.roughness = hs.roughness;
+  m.subsurface = mt.subsurface; m.sheen = mt.sheen; m.sheenTint = mt.sheenTint;
+  m.clearcoat = mt.clearcoat; m.clearcoatGloss = mt.clearcoatGloss;
+  return m;
+}
+// Solid-angle density of sampling the point at distance t along unit d on
+// leaf triangle `leaf` by the light sampler.
+template <class R>
+static R light_pdf(const Scene& s, u32 leaf, R t, V3<R> d, u32 nL) {
+  V3<R> ng = cross(cvt<R>(&s.e0[leaf * 3]), cvt<R>(&s.e1[leaf * 3]));
+  R a2 = length(ng);
+  R c = std::fabs(dot(ng, d)) / a2;
+  if (!(c > (R)0) || !(a2 > (R)0)) return (R)0;
+  return t * t / (c * (a2 * (R)0.5) * (R)nL);
+}
+template <class R>
+static void disney_path(const Scene& s, const RenderParams& p, const Camera<R>& cam, u32 frame, u32 x, u32 y,
+                        V3<R>& L, u64& rays, u64& srays) {
+  const u32 W = p.width, H = p.height, pix = y * W + x;
+  Pcg32 rng; pcg32_seed(rng, stream_key(p.seed, frame, DOMAIN_EYE, 0), pix);
+  V3<R> dir = eye_dir(cam, x, y, (R)W, (R)H, rng), orig = cam.pos, T = mk((R)1, (R)1, (R)1);
+  L = mk((R)0, (R)0, (R)0);
+  R prev_pdf = (R)0;
+  const u32 nL = (u32)s.lightTri.size();
+  for (u32 d = 0;; d++) {
+    HitR<R> h;
+    rays++;
+    if (!trace_defined<R>(s, orig, dir, (R)kInf, (R)kBiasEps, false, h)) {
+      if (s.envW) L = L + vmul(T, env_radiance<R>(s, dir));
+      return;
+    }
+    R w = ((R)1 - h.u) - h.v;
+    HitShade<R> hs = fetch_hit<R>(s, h.leaf, w, h.u, h.v);
+    if (length(hs.emissive) > (R)0) {
+      R wt = (R)1;
+      if (d > 0 && !(p.flags & kDisneyBsdfOnly)) {
+        R pl = light_pdf<R>(s, h.leaf, h.t, dir, nL);
+        wt = (p.flags & kDisneyLightOnly) ? (R)0 : prev_pdf * prev_pdf / (prev_pdf * prev_pdf + pl * pl);
+      }
+      L = L + vmul(T, hs.emissive) * wt;
+    }
+    if (d + 1 >= p.eye_depth) return;
+    if (d >= 3 && !(p.flags & kDisneyNoRoulette)) {
+      R q = std::fmin((R)0.95, std::fmax(T.x, std::fmax(T.y, T.z)));
+      if (uniform<R>(rng) >= q) return;
+      T = T / q;
+    }
+    V3<R> nl = dot(hs.norm, dir) < (R)0 ? hs.norm : hs.norm * (R)-1;
+    V3<R> pos = orig + dir * h.t, wo = dir * (R)-1;
+    Disney<R> m = disney_of<R>(s, h.leaf, hs);
+    if (nL > 0 && !(p.flags & kDisneyBsdfOnly)) {
+      R l0 = uniform<R>(rng), l1 = uniform<R>(rng), l2 = uniform<R>(rng);
+      u32 li = (u32)(l0 * (R)nL); if (li >= nL) li = nL - 1;
+      if (l1 + l2 > (R)1) { l1 = (R)1 - l1; l2 = (R)1 - l2; }
+      const u32 lt = s.lightTri[li];
+      V3<R> e0 = cvt<R>(&s.e0[lt * 3]), e1 = cvt<R>(&s.e1[lt * 3]);
+      V3<R> lp = (cvt<R>(&s.p0[lt * 3]) + e0 * l1) + e1 * l2;
+      V3<R> so = pos + nl * (R)kBiasEps;
+      V3<R> toL = lp - so;
+      R dist = length(toL);
+      if ((double)dist > kFltEps) {
+        V3<R> wi = toL / dist;
+        R cs = dot(nl, wi);
+        R pl = light_pdf<R>(s, lt, dist, wi, nL);
+        if (cs > (R)0 && pl > (R)0) {
+          V3<R> f = disney_f(m, nl, wo, wi);
+          R pb = disney_pdf(m, nl, wo, wi);
+          R wt = (p.flags & kDisneyLightOnly) ? (R)1 : pl * pl / (pl * pl + pb * pb);
+          const Material& lm = s.mats[s.tmat[lt]];
+          V3<R> c = vmul(vmul(T, f), cvt<R>(lm.emissive)) * (cs * wt / pl);
+          if (length(c) > (R)0) {
+            HitR<R> sh;
+            srays++;
+            if (!trace_defined<R>(s, so, wi, (R)((double)dist - 2.0 * kBiasEps), (R)kBiasEps, false, sh, true))
+              L = L + c;
+          }
+        }
+      }
+    }
+    R b0 = uniform<R>(rng), b1 = uniform<R>(rng), b2 = uniform<R>(rng);
+    V3<R> wi;
+    if (!disney_sample(m, nl, wo, b0, b1, b2, wi)) return;
+    R pdf = disney_pdf(m, nl, wo, wi);
+    if (!(pdf > (R)0)) return;
+    T = vmul(T, disney_f(m, nl, wo, wi)) * (dot(nl, wi) / pdf);
+    prev_pdf = pdf;
+    orig = pos + wi * (R)kBiasEps;
+    dir = wi;
+  }
+}
+template <class R>
+static void render_disney(const Scene& s, const RenderParams& p, u32 frame0, u32 nframes, FrameState<R>& fs) {
+  Camera<R> cam = make_camera<R>(p);
+  const u32 W = p.width, H = p.height;
+  for (u32 f = frame0; f < frame0 + nframes; f++) {
+    if (f == 0) std::fill(fs.N.begin(), fs.N.end(), 0u);
+    u64 er = 0, sr = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : er, sr)
+    for (i32 y = 0; y < (i32)H; y++)
+      for (u32 x = 0; x < W; x++) {
+        V3<R> Lr;
+        disney_path<R>(s, p, cam, f, x, (u32)y, Lr, er, sr);
+        const u32 pix = (u32)y * W + x;
+        apply_sample(&fs.result[pix * 3], fs.N[pix], Lr, 1u);
+      }
+    fs.ext_rays += er; fs.shadow_rays += sr;
+  }
 }
 
 // pt_applyPathQueueResult_kernel (cudaRTPTStream.cu:303-325): the frame's
@@ -1112,15 +1394,22 @@ static inline R euclid(const float* a, const float* b, R d) {
   return std::exp(-m / ((R)2 * d * d));
 }
 // bilaterial_posnorm_kernel / _posnormemit_kernel (cudaBilateralFilter.cu:66-142)
+// Depth of a first-hit position: its distance from the camera.
+template <class R> static inline R depth_of(const float* p, const float* cam) {
+  R dx = (R)(p[0] - cam[0]), dy = (R)(p[1] - cam[1]), dz = (R)(p[2] - cam[2]);
+  return std::sqrt((dx * dx + dy * dy) + dz * dz);
+}
 template <class R>
 static void bilateral(const float* color, const float* pos, const float* norm, const float* emit, u32 W, u32 H,
-                      float sc, float sp, float sn, float se, u32 radius, const std::vector<float>& g, float* out) {
+                      float sc, float sp, float sn, float se, u32 radius, const std::vector<float>& g, float* out,
+                      float sz = 0.f, const float* cam = nullptr) {
 #pragma omp parallel for schedule(static)
   for (i32 yy = 0; yy < (i32)H; yy++) {
     u32 y = (u32)yy;
     for (u32 x = 0; x < W; x++) {
       u32 ind = x + y * W;
       R sum = 0, t0 = 0, t1 = 0, t2 = 0;
+      const R zc = sz > 0.f ? depth_of<R>(&pos[ind * 3], cam) : (R)0;
       for (int i = -(int)radius; i <= (int)radius; i++) {
         for (int j = -(int)radius; j <= (int)radius; j++) {
           u32 c = ref_clamp(x, j, W) + ref_clamp(y, i, H) * W;
@@ -1129,6 +1418,10 @@ static void bilateral(const float* color, const float* pos, const float* norm, c
           f = f * euclid<R>(&pos[c * 3], &pos[ind * 3], (R)sp);
           f = f * euclid<R>(&norm[c * 3], &norm[ind * 3], (R)sn);
           if (emit) f = f * euclid<R>(&emit[c * 3], &emit[ind * 3], (R)se);
+          if (sz > 0.f) {
+            const R dz = zc > (R)0 ? (depth_of<R>(&pos[c * 3], cam) - zc) / zc : (R)0;
+            f = f * std::exp(-(dz * dz) / ((R)2 * (R)sz * (R)sz));
+          }
           t0 += f * (R)color[c * 3]; t1 += f * (R)color[c * 3 + 1]; t2 += f * (R)color[c * 3 + 2];
           sum += f;
         }
@@ -1167,22 +1460,30 @@ static inline float sqdist(const float* a, const float* b) {
 // w = (G[i] G[j]) * exp(-(|dc|^2/2sc^2 + |dp|^2/2sp^2 + |dn|^2/2sn^2 [+ |dd|^2/2sd^2])),
 // the product of the reference's per-term exponentials (cudaBilateralFilter.cu:89-92)
 // folded into one; FP64 runs keep the literal product (bilateral<double>).
+// The optional depth term adds ((z_q - z_p) / z_p)^2 / 2sz^2 last, z the
+// correctly rounded distance of the first hit from the camera.
 static void bilateral_defined(const float* color, const float* pos, const float* norm, const float* emit, u32 W, u32 H,
                               float sc, float sp, float sn, float se, u32 radius, const std::vector<float>& g,
-                              float* out) {
-  const float kc = 2.f * sc * sc, kp = 2.f * sp * sp, kn = 2.f * sn * sn, ke = 2.f * se * se;
+                              float* out, float sz = 0.f, const float* cam = nullptr) {
+  const float kc = 2.f * sc * sc, kp = 2.f * sp * sp, kn = 2.f * sn * sn, ke = 2.f * se * se, kz = 2.f * sz * sz;
 #pragma omp parallel for schedule(static)
   for (i32 yy = 0; yy < (i32)H; yy++) {
     u32 y = (u32)yy;
     for (u32 x = 0; x < W; x++) {
       u32 ind = x + y * W;
       float sum = 0.f, t0 = 0.f, t1 = 0.f, t2 = 0.f;
+      const float zc = kz > 0.f ? std::sqrt(sqdist(&pos[ind * 3], cam)) : 0.f;
       for (int i = -(int)radius; i <= (int)radius; i++) {
         for (int j = -(int)radius; j <= (int)radius; j++) {
           u32 c = ref_clamp(x, j, W) + ref_clamp(y, i, H) * W;
           float a = sqdist(&color[c * 3], &color[ind * 3]) / kc + sqdist(&pos[c * 3], &pos[ind * 3]) / kp;
           a = a + sqdist(&norm[c * 3], &norm[ind * 3]) / kn;
           if (emit) a = a + sqdist(&emit[c * 3], &emit[ind * 3]) / ke;
+          if (kz > 0.f) {
+            const float zq = std::sqrt(sqdist(&pos[c * 3], cam));
+            const float dz = zc > 0.f ? (zq - zc) / zc : 0.f;
+            a = a + (dz * dz) / kz;
+          }
           float f = (g[i + radius] * g[j + radius]) * cpt_expf(-a);
           t0 = t0 + f * color[c * 3]; t1 = t1 + f * color[c * 3 + 1]; t2 = t2 + f * color[c * 3 + 2];
           sum = sum + f;
@@ -1306,15 +1607,15 @@ static void adaptive_frame(const Scene& s, const RenderParams& p, const Camera<R
     }
   } else {
     std::vector<float> g;
-    gaussian_table(p.sc, p.radius, g);                     // spatial sigma = colour sigma (:1327)
+    gaussian_table(p.ss > 0.f ? p.ss : p.sc, p.radius, g);   // spatial sigma = colour sigma (:1327)
     fs.filtered.assign(n * 3, 0.f);
     const float* emit = p.filter_diff ? dif.data() : nullptr;
     if (sizeof(R) == 4) {
       bilateral_defined(col.data(), pos.data(), nrm.data(), emit, W, H, p.sc, p.sp, p.sn, p.sd, p.radius, g,
-                        fs.filtered.data());
+                        fs.filtered.data(), p.sz, p.cam_pos);
     } else {
       bilateral<double>(col.data(), pos.data(), nrm.data(), emit, W, H, p.sc, p.sp, p.sn, p.sd, p.radius, g,
-                        fs.filtered.data());
+                        fs.filtered.data(), p.sz, p.cam_pos);
     }
     // pt_calculateSquareError_kernel (:1143-1156; AdapBilatBinary.cu:1031-1044)
     for (u32 x = 0; x < n; x++) {
@@ -1327,6 +1628,12 @@ static void adaptive_frame(const Scene& s, const RenderParams& p, const Camera<R
       fs.err[x] = red * ((d0 * d0 + d1 * d1) + d2 * d2) / 3.f;
     }
   }
+  // per-pixel cap: a pixel that has max_paths eye paths gets no weight, and its
+  // count stops at max_paths - paths
+  auto room = [&](u32 x) { return fs.paths[x] < p.max_paths ? p.max_paths - fs.paths[x] : 0u; };
+  if (p.max_paths)
+    for (u32 x = 0; x < n; x++)
+      if (fs.paths[x] >= p.max_paths) fs.err[x] = 0.f;
   if (p.mode == 1) {
     float sum = (float)tree_sum(fs.err.data(), n);
     fs.err_sum = sum;
@@ -1335,6 +1642,9 @@ static void adaptive_frame(const Scene& s, const RenderParams& p, const Camera<R
     alloc_rounds(p, f, fs.err, n, gen, p.mode == 2, (float)n / (float)(p.nd ? p.nd : 1u), p.min_prob, fs.counts,
                  fs.rounds);
   }
+  if (p.max_paths)
+    for (u32 x = 0; x < n; x++) fs.counts[x] = std::min(fs.counts[x], room(x));
+  for (u32 x = 0; x < n; x++) fs.paths[x] += fs.counts[x];
   const int variant = p.mode == 1 ? 1 : 0;      // reflection offset of AdapBilatAtomic's eye bounce
   u64 er = 0, sr = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : er, sr)
@@ -1381,15 +1691,18 @@ static void render_adaptive(const Scene& s, const RenderParams& p, u32 frame0, u
   if (fs.gpos.size() != (size_t)n * 3) {
     fs.gpos.assign(n * 3, (R)0); fs.gnorm.assign(n * 3, (R)0); fs.gdiff.assign(n * 3, (R)0);
     fs.sq.assign(n * 3, (R)0);
+    fs.paths.assign(n, 0u);
   }
   const int variant = p.mode == 1 ? 1 : 0;
+  const u32 warmup = std::max(p.warmup, p.min_paths);       // every pixel gets min_paths paths
   for (u32 f = frame0; f < frame0 + nframes;) {
     // the cache of the frame's three-frame group (built at its first frame)
     if (f % 3 == 0 || fs.light_group != f / 3) light_pass<R>(s, p, f - f % 3, fs);
     fs.light_group = f / 3;
     u32 k = 1;
-    if (f < p.warmup) {
-      if (f == 0) std::fill(fs.N.begin(), fs.N.end(), 0u);
+    if (f < warmup) {
+      if (f == 0) { std::fill(fs.N.begin(), fs.N.end(), 0u); std::fill(fs.paths.begin(), fs.paths.end(), 0u); }
+      for (u32 x = 0; x < n; x++) fs.paths[x]++;
       u64 er = 0, sr = 0;
 #pragma omp parallel for schedule(dynamic, 64) reduction(+ : er, sr)
       for (i32 ii = 0; ii < (i32)n; ii++) {
@@ -1475,6 +1788,18 @@ void* orc_scene_create(const float* pos3, const float* nrm3, u32 nv, const u32* 
   return s;
 }
 void orc_scene_destroy(void* h) { delete (Scene*)h; }
+// Environment light: W x H RGB texels (W = 0 removes it).
+void orc_scene_set_env(void* h, u32 W, u32 H, const float* rgb) {
+  Scene* s = (Scene*)h;
+  s->envW = W; s->envH = H;
+  s->env.assign(rgb, rgb + (size_t)W * H * 3 * (W && H ? 1 : 0));
+  if (!W || !H) { s->envW = s->envH = 0; s->env.clear(); }
+}
+void orc_env_radiance(void* h, const double* d, int prec, double* out) {
+  const Scene* s = (const Scene*)h;
+  if (prec == 64) put3(env_radiance<double>(*s, v3d<double>(d)), out);
+  else put3(env_radiance<float>(*s, v3d<float>(d)), out);
+}
 // Per-vertex texture coordinates (2 floats per vertex).
 void orc_scene_set_uv(void* h, const float* uv2) {
   Scene* s = (Scene*)h;
@@ -1553,6 +1878,33 @@ void orc_sample_bounce_many(int prec, const double* ro, const double* rd, double
                             double* dir, double* mul) {
   if (prec == 64) bounce_many<double>(ro, rd, t, hs, nl, seed, n, refl_offset, type, orig, dir, mul);
   else bounce_many<float>(ro, rd, t, hs, nl, seed, n, refl_offset, type, orig, dir, mul);
+}
+
+// Disney BRDF checks.  m = base3 metallic subsurface specular roughness sheen
+// sheenTint clearcoat clearcoatGloss (11 values).
+static Disney<double> disney_from(const double* m) {
+  Disney<double> d;
+  d.base = mk(m[0], m[1], m[2]); d.metallic = m[3]; d.subsurface = m[4]; d.specular = m[5]; d.roughness = m[6];
+  d.sheen = m[7]; d.sheenTint = m[8]; d.clearcoat = m[9]; d.clearcoatGloss = m[10];
+  return d;
+}
+void orc_disney_f(const double* m, const double* n, const double* wo, const double* wi, double* out) {
+  put3(disney_f<double>(disney_from(m), v3d<double>(n), v3d<double>(wo), v3d<double>(wi)), out);
+}
+double orc_disney_pdf(const double* m, const double* n, const double* wo, const double* wi) {
+  return disney_pdf<double>(disney_from(m), v3d<double>(n), v3d<double>(wo), v3d<double>(wi));
+}
+// count samples (independent hashed streams from seed): direction and valid flag
+void orc_disney_sample_many(const double* m, const double* n, const double* wo, u64 seed, u32 count, double* wi,
+                            i32* valid) {
+  const Disney<double> d = disney_from(m);
+  for (u32 i = 0; i < count; i++) {
+    Pcg32 rng; pcg32_seed(rng, splitmix64_mix(seed + (u64)i * 0x9E3779B97F4A7C15ull), i);
+    double u0 = uniform<double>(rng), u1 = uniform<double>(rng), u2 = uniform<double>(rng);
+    V3<double> w;
+    valid[i] = disney_sample<double>(d, v3d<double>(n), v3d<double>(wo), u0, u1, u2, w) ? 1 : 0;
+    put3(w, wi + (size_t)i * 3);
+  }
 }
 
 void orc_tex_sample(void* h, u32 id, double u, double v, int bits, double* out) {
@@ -1663,7 +2015,8 @@ void orc_render(void* scene, void* rh, u32 frame0, u32 nframes, float* result, u
   size_t n = (size_t)r->p.width * r->p.height;
   if (r->prec == 64) {
     std::vector<double> g; if (gbuf) g.assign(n * 9, 0.0);
-    if (r->p.mode == 6) render_pt_literal<double>(*s, r->p, frame0, nframes, r->f64);
+    if (r->p.mode == 7) render_disney<double>(*s, r->p, frame0, nframes, r->f64);
+    else if (r->p.mode == 6) render_pt_literal<double>(*s, r->p, frame0, nframes, r->f64);
     else if (r->p.mode >= 4) render_pt<double>(*s, r->p, frame0, nframes, r->f64);
     else if (r->p.mode != 0) render_adaptive<double>(*s, r->p, frame0, nframes, r->f64);
     else render_frames<double>(*s, r->p, frame0, nframes, r->f64, gbuf ? g.data() : nullptr);
@@ -1673,7 +2026,8 @@ void orc_render(void* scene, void* rh, u32 frame0, u32 nframes, float* result, u
     if (lvN) *lvN = r->f64.lvN;
   } else {
     std::vector<float> g; if (gbuf) g.assign(n * 9, 0.f);
-    if (r->p.mode == 6) render_pt_literal<float>(*s, r->p, frame0, nframes, r->f32);
+    if (r->p.mode == 7) render_disney<float>(*s, r->p, frame0, nframes, r->f32);
+    else if (r->p.mode == 6) render_pt_literal<float>(*s, r->p, frame0, nframes, r->f32);
     else if (r->p.mode >= 4) render_pt<float>(*s, r->p, frame0, nframes, r->f32);
     else if (r->p.mode != 0) render_adaptive<float>(*s, r->p, frame0, nframes, r->f32);
     else render_frames<float>(*s, r->p, frame0, nframes, r->f32, gbuf ? g.data() : nullptr);
@@ -1701,6 +2055,7 @@ void orc_adaptive_state(void* rh, u32 which, void* out, float* sum) {
     else if (which == 5) { if (fs.counts.size()) std::memcpy(out, fs.counts.data(), n * 4); }
     else if (which == 6) cp(fs.sq);
     else if (which == 7) *(u32*)out = fs.rounds;
+    else if (which == 8) { if (fs.paths.size()) std::memcpy(out, fs.paths.data(), n * 4); }
     if (sum) *sum = fs.err_sum;
   };
   if (r->prec == 64) dump(r->f64); else dump(r->f32);
@@ -1711,6 +2066,16 @@ void orc_bilateral_defined(const float* color, const float* pos, const float* no
                            float sc, float sp, float sn, float se, u32 radius, float* out) {
   std::vector<float> g; gaussian_table(sc, radius, g);
   bilateral_defined(color, pos, norm, emit, W, H, sc, sp, sn, se, radius, g, out);
+}
+// The filter with the extension terms: spatial sigma ss (0: the colour sigma)
+// and the relative-depth term sz (0: off) about camera position cam.
+// prec 32: the defined one-exponential arithmetic; 64: the literal product.
+void orc_bilateral_ext(const float* color, const float* pos, const float* norm, const float* emit, u32 W, u32 H,
+                       float sc, float sp, float sn, float se, float ss, float sz, const float* cam, u32 radius,
+                       u32 prec, float* out) {
+  std::vector<float> g; gaussian_table(ss > 0.f ? ss : sc, radius, g);
+  if (prec == 64) bilateral<double>(color, pos, norm, emit, W, H, sc, sp, sn, se, radius, g, out, sz, cam);
+  else bilateral_defined(color, pos, norm, emit, W, H, sc, sp, sn, se, radius, g, out, sz, cam);
 }
 float orc_expf(float x) { return cpt_expf(x); }
 double orc_tree_sum(const float* e, u32 n) { return tree_sum(e, n); }

@@ -77,3 +77,11 @@ def rel_mse(a, ref):
     a = np.asarray(a, np.float64)
     ref = np.asarray(ref, np.float64)
     return float(((a - ref) ** 2).sum() / max((ref ** 2).sum(), 1e-300))
+
+
+def oracle_adaptive(attrs):
+    """Renderer adaptive attributes (e.g. scenes.ADAPTIVE_C3) as oracle render_params keywords."""
+    keys = {"Filter Radius": "radius", "Filter Color EuD": "sc", "Filter Norm EuD": "sn", "Filter Depth EuD": "sz",
+            "Filter Spatial EuD": "ss", "Filter Pos EuD": "sp", "Min Pixel Paths": "min_paths",
+            "Max Pixel Paths": "max_paths"}
+    return {keys[k]: v for k, v in attrs.items()}

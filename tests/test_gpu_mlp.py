@@ -53,7 +53,7 @@ def test_forward_matches_oracle(cpt, n):
     assert np.abs(y - yb).mean() < np.abs(y - ye).mean()
 
 
-@pytest.mark.parametrize("n", [64, 777, 8192])
+@pytest.mark.parametrize("n", [64, 777, 8192, 65536, 100000])
 def test_gradient_matches_oracle(cpt, n):
     m = cpt.Mlp(1)
     p = _params(11)
@@ -85,6 +85,22 @@ def test_adam_steps_match_recurrence(cpt):
         g = m.grad().astype(np.float64)           # the gradient the step used
         pm, mm, vm = M.adam(pm, mm, vm, g, step, lr=1e-2)
         np.testing.assert_allclose(m.params(), pm, rtol=1e-5, atol=1e-7)
+
+
+def test_multi_step_call_equals_single_steps(cpt):
+    """cpt_mlp_train_steps runs the same kernels in the same order as one
+    cpt_mlp_train_step per batch: identical parameters, bit for bit."""
+    p0 = _params(17)
+    X, t = _data(21, 4 * 4096)
+    dx, dt = cpt.DeviceBuffer.from_array(X), cpt.DeviceBuffer.from_array(t)
+    a, b = cpt.Mlp(1), cpt.Mlp(1)
+    a.set_params(p0)
+    b.set_params(p0)
+    for k in range(7):                                   # batches 2, 3, 0, 1, 2, 3, 0
+        a.train_step_device(dx, dt, 4096, lr=5e-3, want_loss=False, x_offset=((2 + k) % 4) * 4096)
+    loss = b.train_steps_device(dx, dt, 4 * 4096, 4096, 7, lr=5e-3, first_batch=2, want_loss=True)
+    assert np.array_equal(a.params().view(np.uint32), b.params().view(np.uint32))
+    assert np.isfinite(loss)
 
 
 def test_training_reduces_loss(cpt):
