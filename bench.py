@@ -171,7 +171,7 @@ def learned_leg(ca, args):
         r["learned_over_plain_adaptive"] = r["learned_s"] / r["plain_adaptive_s"] if r["learned_s"] else None
     mlp = mlp_step_throughput(ca, m, X, T)
     del S
-    return {
+    return m, {
         "learned_over_plain_adaptive": round(gm, 4) if gm is not None else None,
         "plain_bdpt_over_plain_adaptive": round(float(np.exp(np.mean(np.log(bdpt)))), 4) if bdpt else None,
         "target": "<= 0.5",
@@ -220,9 +220,8 @@ def adaptive_leg(ca, args):
                        "4..256 paths per pixel",
            "seconds": round(wall, 4), "frames": F,
            "mrays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / wall / 1e6, 2),
-           "host_readback": "none in the timed region: the running mean stays in HBM (the reference copies it "
-                         "to the host after every frame, src/cudaRTBDPTStream.cu:949, 11 MB per 1280x720 image)",
-        "samples_per_s": round(st["eye_paths"] / wall, 1),
+           "host_readback": "none in the timed region (the running mean stays in HBM)",
+           "samples_per_s": round(st["eye_paths"] / wall, 1),
            "paths_per_pixel": {"mean": round(float(paths.mean()), 3), "min": int(paths.min()),
                                "max": int(paths.max())},
            "filter_ms_per_frame": round(st["filter_ms"] / max(filt_frames, 1), 4),
@@ -360,8 +359,9 @@ def baselines_leg(ca, S, sc, args):
     return out
 
 
-def multiview_leg(ca, args):
-    """Batched multi-view rendering: autotune, then 32 spp of every view."""
+def multiview_leg(ca, args, model=None):
+    """Batched multi-view rendering: autotune, then 32 spp of every view, plain
+    and with per-view (learned) adaptive sampling."""
     from cudapt_amd import scenes
     t0 = time.perf_counter()
     sc = scenes.heightfield(light="env")
@@ -383,11 +383,32 @@ def multiview_leg(ca, args):
     res = {"views": len(views), "pixels": px, "spp": args.mv_spp, "triangles": int(len(sc["idx"])),
            "seconds": round(wall, 4), "gpu_ms": round(st["total_ms"], 3),
            "mrays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / wall / 1e6, 2),
-           "host_readback": "none in the timed region: the running mean stays in HBM (the reference copies it "
-                         "to the host after every frame, src/cudaRTBDPTStream.cu:949, 11 MB per 1280x720 image)",
-        "samples_per_s": round(st["eye_paths"] / wall, 1),
+           "host_readback": "none in the timed region (the running mean stays in HBM)",
+           "samples_per_s": round(st["eye_paths"] / wall, 1),
            "resolutions": sorted({(v["width"], v["height"]) for v in views}),
            "autotuned": best, "candidates_ms_per_iteration": [round(m, 3) for m in cand_ms]}
+    # the same views with per-view adaptive sampling (learned when a model is
+    # given): 32 paths per pixel on average per view (5 warm-up frames, then
+    # one-frame passes), the adaptive workload's filter and path bounds
+    B.set_adaptive(ca.CPT_MODE_BDPT_ADAPTIVE_PLL, 1, model, **scenes.ADAPTIVE_C3)
+    if model is not None:
+        B.set_attrib("Learned Signal", args.learn_signal)
+    B.render(S, 6)                                   # allocations, outside the timing
+    B.reset()
+    B.stats(reset=True)
+    ca.synchronize()
+    t0 = time.perf_counter()
+    B.render(S, args.mv_spp)
+    wall = time.perf_counter() - t0
+    st = B.stats()
+    paths = np.concatenate([B.read(v, ca.CPT_BUF_PIXEL_PATHS).ravel() for v in range(len(views))])
+    res["adaptive"] = {"learned": model is not None, "seconds": round(wall, 4),
+                       "mrays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / wall / 1e6, 2),
+                       "samples_per_s": round(st["eye_paths"] / wall, 1),
+                       "paths_per_pixel": {"mean": round(float(paths.mean()), 3), "min": int(paths.min()),
+                                           "max": int(paths.max())},
+                       "phase_ms": {k: round(st[k], 3) for k in ("extend_ms", "shade_ms", "shadow_ms", "filter_ms",
+                                                               "error_ms", "gen_ms", "accum_ms", "mlp_ms")}}
     del B, S
     return res
 
@@ -431,6 +452,8 @@ def main():
     ap.add_argument("--learn-signal", type=int, default=3, help="allocation signal of the learned renderer")
     ap.add_argument("--pass-frames", type=int, default=1, help="adaptive pass size in frames (learned leg)")
     ap.add_argument("--adaptive-spp", type=int, default=32, help="average spp of the adaptive leg (0: skip)")
+    ap.add_argument("--mlp-only", action="store_true",
+                    help="only the model's training-step throughput on synthetic features (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the library renders on one "
@@ -448,6 +471,13 @@ def main():
         ca.synchronize()
         log(f"calibration copy: {nbytes} bytes device to device")
         del src, dst
+    if args.mlp_only:
+        rng = np.random.default_rng(7)
+        X = rng.normal(size=(65536, 9)).astype(np.float32)
+        T = (X[:, :3].sum(axis=1) + 0.1 * rng.normal(size=65536)).astype(np.float32)
+        m = ca.Mlp(0xA22)
+        print(json.dumps({"mlp": mlp_step_throughput(ca, m, X, T)}), flush=True)
+        return
     t0 = time.perf_counter()
     sc = scenes.sphere_grid()
     S = ca.Scene(sc)
@@ -480,6 +510,7 @@ def main():
     wall = time.perf_counter() - t_start
     st = R.stats(reset=True)
     R.set_profiling(False)
+    bounce_rays = R.read(ca.CPT_BUF_BOUNCE_RAYS)      # the last step's per-bounce queue lengths
     rays = st["extension_rays"] + st["shadow_rays"]
     mrays = rays / wall / 1e6
     launches = max(st["extend_launches"], 1)
@@ -519,7 +550,9 @@ def main():
         "host_readback": "none in the timed region: the running mean stays in HBM (the reference copies it "
                          "to the host after every frame, src/cudaRTBDPTStream.cu:949, 11 MB per 1280x720 image)",
         "samples_per_s": round(st["eye_paths"] / wall, 1),
-        "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"]},
+        "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"],
+                 "extension_per_bounce": [int(x) for x in bounce_rays[0][:EYE_DEPTH]],
+                 "shadow_per_bounce": [int(x) for x in bounce_rays[1][:EYE_DEPTH]]},
         "kernel_ms": {"extend": round(st["extend_ms"], 3), "shade": round(st["shade_ms"], 3),
                       "shadow": round(st["shadow_ms"], 3), "light_pass": round(st["light_ms"], 3),
                       "all": round(st["total_ms"], 3)},
@@ -554,11 +587,12 @@ def main():
     del S
     res["adaptive"] = adaptive_leg(ca, args) if args.adaptive_spp > 0 else None
     res["mlp"] = None
+    model = None
     if not args.no_learned:
-        res["time_to_equal_mse"], res["mlp"] = learned_leg(ca, args)
+        model, res["time_to_equal_mse"], res["mlp"] = learned_leg(ca, args)
     res["multiview"] = None
     if not args.no_multiview:
-        res["multiview"] = multiview_leg(ca, args)
+        res["multiview"] = multiview_leg(ca, args, model)
     if not args.no_traffic:
         tr = counter_passes(args)
         rl = res["roofline"]
@@ -570,13 +604,16 @@ def main():
             gbs = tr["bytes_per_launch"] / launch_s / 1e9
             rl["hbm_measured"] = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                   "frac": round(gbs / HBM_PEAK_GBS, 4)}
-            # VALU issue: a wave64 vector instruction occupies one 16-lane SIMD
-            # for 4 cycles; 256 CUs x 4 SIMDs at 2.4 GHz
+            # VALU issue: a 32-bit wave64 vector instruction holds one of a CU's
+            # four 16-lane SIMDs for 4 cycles, so a CU completes about one per
+            # cycle: 256 CUs at 2.4 GHz.  Instructions that do not hold a SIMD
+            # that long can take the measured rate past 1.
             valu_rate = tr["valu_insts_per_launch"] / launch_s
-            valu_peak = 256 * 4 * 2.4e9 / 4
+            valu_peak = 256 * 2.4e9
             rl["valu"] = {"wave_insts_per_launch": tr["valu_insts_per_launch"],
                           "achieved": round(valu_rate / 1e9, 1), "peak": round(valu_peak / 1e9, 1),
-                          "unit": "G wave64 instructions/s", "frac": round(valu_rate / valu_peak, 4),
+                          "unit": "G wave64 instructions/s (peak: one per CU-cycle)",
+                          "frac": round(valu_rate / valu_peak, 4),
                           "per_ray": round(tr["valu_insts_per_launch"] * 64 / max(st["extension_rays"] / launches, 1),
                                            1)}
             hf, vf = rl["hbm_measured"]["frac"], rl["valu"]["frac"]

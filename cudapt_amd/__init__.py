@@ -25,7 +25,8 @@ CPT_MODE_PT_STREAM = 4
 CPT_MODE_PT_STREAM_REGEN = 5
 CPT_MODE_PT_DISNEY_MIS = 6
 (CPT_BUF_RESULT, CPT_BUF_SAMPLE_N, CPT_BUF_GBUF_POS, CPT_BUF_GBUF_NORM, CPT_BUF_GBUF_DIFF, CPT_BUF_FILTERED,
- CPT_BUF_ERROR, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_ALLOC_ERROR, CPT_BUF_SECOND_MOMENT, CPT_BUF_PIXEL_PATHS) = range(11)
+ CPT_BUF_ERROR, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_ALLOC_ERROR, CPT_BUF_SECOND_MOMENT, CPT_BUF_PIXEL_PATHS,
+ CPT_BUF_BOUNCE_RAYS) = range(12)
 MLP_INPUTS, MLP_HIDDEN, MLP_PARAMS = 9, 64, 4865
 
 RAY_DTYPE = np.dtype([("o", "<f4", 3), ("max_dist", "<f4"), ("d", "<f4", 3), ("eps", "<f4")])
@@ -378,8 +379,11 @@ class Renderer:
         return out
 
     def read(self, which: int):
-        """A renderer buffer: CPT_BUF_* (result, counts, G-buffer, filtered, error, per-frame sample counts)."""
-        if which in (CPT_BUF_SAMPLE_N, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_PIXEL_PATHS):
+        """A renderer buffer: CPT_BUF_* (result, counts, G-buffer, filtered, error, per-frame sample counts;
+        CPT_BUF_BOUNCE_RAYS: the last wavefront's extension and shadow rays per bounce, 2 x 16)."""
+        if which == CPT_BUF_BOUNCE_RAYS:
+            out = np.zeros((2, 16), np.uint32)
+        elif which in (CPT_BUF_SAMPLE_N, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_PIXEL_PATHS):
             out = np.zeros((self.H, self.W), np.uint32)
         elif which in (CPT_BUF_ERROR, CPT_BUF_ALLOC_ERROR):
             out = np.zeros((self.H, self.W), np.float32)

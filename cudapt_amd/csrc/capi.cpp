@@ -1627,6 +1627,15 @@ int cpt_renderer_reset(cpt_renderer* r) {
 
 int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out) {
   if (!r || !out) return fail(CPT_E_ARG, "cpt_renderer_read_buffer: null argument");
+  if (which == CPT_BUF_BOUNCE_RAYS) {
+    if (!r->allocated) return fail(CPT_E_STATE, "cpt_renderer_read_buffer: nothing rendered yet");
+    FrameCounters h;
+    HIP_TRY(hipMemcpyAsync(&h, r->fc.p, sizeof h, hipMemcpyDeviceToHost, r->stream));
+    HIP_TRY(hipStreamSynchronize(r->stream));
+    std::memcpy(out, h.active, sizeof h.active);
+    std::memcpy((uint32_t*)out + 16, h.shadow, sizeof h.shadow);
+    return CPT_OK;
+  }
   return read_pixels(r, which, 0, (size_t)r->W * r->H, out, "cpt_renderer_read_buffer");
 }
 

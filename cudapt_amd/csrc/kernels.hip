@@ -2262,8 +2262,9 @@ hipError_t launch_filter_error(const FilterArgs& fa, const float* color, const f
   return hipGetLastError();
 }
 
-// Fixed-order double sum: pairwise tree inside 256-element blocks, then the
-// block sums in order (stands in for thrust::reduce, AdapBilatAtomic.cu:1444).
+// Fixed-order double sum (stands in for thrust::reduce, AdapBilatAtomic.cu:1444):
+// a pairwise tree inside each 256-element block; the same tree over each chunk
+// of 256 block sums; the chunk sums in order.
 __global__ __launch_bounds__(kBlock) void k_err_block_sums(const float* __restrict__ e, uint32_t n,
                                                            double* __restrict__ bs) {
   __shared__ double s[kBlock];
@@ -2276,17 +2277,38 @@ __global__ __launch_bounds__(kBlock) void k_err_block_sums(const float* __restri
   }
   if (threadIdx.x == 0) bs[blockIdx.x] = s[0];
 }
-__global__ void k_err_final(const double* __restrict__ bs, uint32_t nb, float* __restrict__ sum) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    double t = 0.0;
-    for (uint32_t b = 0; b < nb; b++) t += bs[b];
-    *sum = (float)t;
+// One workgroup of four 256-thread groups, each reducing one chunk of block
+// sums at a time (their loads in parallel, where one thread walking the block
+// sums waits on every load in turn); the chunk sums wait in LDS for thread 0.
+constexpr uint32_t kErrChunks = 1024;   // 2^26 pixels: 262,144 block sums
+__global__ __launch_bounds__(1024) void k_err_final(const double* __restrict__ bs, uint32_t nb,
+                                                    float* __restrict__ sum) {
+  __shared__ double s[1024];
+  __shared__ double cs[kErrChunks];
+  const uint32_t g = threadIdx.x >> 8, t = threadIdx.x & 255u;
+  const uint32_t nc = (nb + 255u) / 256u;
+  for (uint32_t c0 = 0; c0 < nc; c0 += 4) {
+    const uint32_t c = c0 + g, i = c * 256u + t;
+    s[threadIdx.x] = (c < nc && i < nb) ? bs[i] : 0.0;
+    __syncthreads();
+    for (uint32_t stp = 128; stp >= 1; stp >>= 1) {
+      if (t < stp) s[threadIdx.x] += s[threadIdx.x + stp];
+      __syncthreads();
+    }
+    if (t == 0 && c < nc) cs[c] = s[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double tot = 0.0;
+    for (uint32_t c = 0; c < nc; c++) tot += cs[c];
+    *sum = (float)tot;
   }
 }
 hipError_t launch_error_sum(const float* err, uint32_t n, double* block_sums, float* sum_out, hipStream_t st) {
   uint32_t nb = (n + kBlock - 1) / kBlock;
+  if (n == 0 || (nb + 255u) / 256u > kErrChunks) return hipErrorInvalidValue;
   k_err_block_sums<<<nb, kBlock, 0, st>>>(err, n, block_sums);
-  k_err_final<<<1, 64, 0, st>>>(block_sums, nb, sum_out);
+  k_err_final<<<1, 1024, 0, st>>>(block_sums, nb, sum_out);
   return hipGetLastError();
 }
 

@@ -133,9 +133,9 @@ def mse(img, ref, clamp=CLAMP) -> float:
 
 def mse_curve(scene: Scene, cam, ref, W, H, frames, model: Mlp | None = None, mode=CPT_MODE_BDPT_ADAPTIVE_PLL,
               depth=8, stop_below=None, pass_frames=1, clamp=CLAMP, signal=0, attribs=None):
-    """(frames, cumulative render seconds, MSE) after each render call of
-    pass_frames frames.  With stop_below the run ends at the first call whose
-    MSE is at or below it."""
+    """(frames, cumulative render seconds, MSE, eye paths per pixel) after each
+    render call of pass_frames frames.  With stop_below the run ends at the
+    first call whose MSE is at or below it."""
     if mode != CPT_MODE_BDPT_STREAM:
         R = _adaptive(W, H, depth, pass_frames, signal, mode=mode, attribs=attribs)
         if model is not None:
@@ -148,7 +148,8 @@ def mse_curve(scene: Scene, cam, ref, W, H, frames, model: Mlp | None = None, mo
     warm = dict(cam, pos=tuple(np.asarray(cam["pos"]) + 1e-3))
     R.render(scene, warm, frames=min(frames, 6 + 2 * pass_frames), result=False)
     synchronize()
-    fs, ts, ms = [], [], []
+    R.stats(reset=True)
+    fs, ts, ms, ps = [], [], [], []
     total = 0.0
     done = 0
     while done < frames:
@@ -163,9 +164,10 @@ def mse_curve(scene: Scene, cam, ref, W, H, frames, model: Mlp | None = None, mo
         fs.append(done)
         ts.append(total)
         ms.append(e)
+        ps.append(R.stats()["eye_paths"] / (W * H))
         if stop_below is not None and e <= stop_below:
             break
-    return np.array(fs), np.array(ts), np.array(ms)
+    return np.array(fs), np.array(ts), np.array(ms), np.array(ps)
 
 
 def time_to_reach(ts, ms, target):
@@ -192,16 +194,18 @@ def time_to_equal_mse(scene: Scene, model: Mlp, cams, refs, W, H, frames=64, dep
               attribs=attribs)
     rows = []
     for cam, ref in zip(cams, refs):
-        fp, tp, mp = mse_curve(scene, cam, ref, W, H, frames, None, depth=depth, pass_frames=pass_frames,
+        fp, tp, mp, pp = mse_curve(scene, cam, ref, W, H, frames, None, depth=depth, pass_frames=pass_frames,
                                clamp=clamp, attribs=attribs)
         target = float(mp[-1])
-        fl, tl, ml = mse_curve(scene, cam, ref, W, H, max_frames, model, depth=depth, stop_below=target,
+        fl, tl, ml, pl = mse_curve(scene, cam, ref, W, H, max_frames, model, depth=depth, stop_below=target,
                                pass_frames=pass_frames, clamp=clamp, signal=signal, attribs=attribs)
-        fu, tu, mu = mse_curve(scene, cam, ref, W, H, max_frames, None, mode=CPT_MODE_BDPT_STREAM, depth=depth,
+        fu, tu, mu, pu = mse_curve(scene, cam, ref, W, H, max_frames, None, mode=CPT_MODE_BDPT_STREAM, depth=depth,
                                stop_below=target, pass_frames=pass_frames, clamp=clamp)
         at = np.searchsorted(fl, frames)
         rows.append(dict(target_mse=target, plain_adaptive_s=float(tp[-1]),
                          learned_s=time_to_reach(tl, ml, target), plain_bdpt_s=time_to_reach(tu, mu, target),
                          learned_frames=int(fl[-1]), plain_bdpt_frames=int(fu[-1]),
+                         plain_adaptive_paths_per_pixel=float(pp[-1]), learned_paths_per_pixel=float(pl[-1]),
+                         plain_bdpt_paths_per_pixel=float(pu[-1]),
                          learned_mse_at_equal_frames=float(ml[min(at, len(ml) - 1)])))
     return rows

@@ -266,8 +266,12 @@ enum {
   CPT_BUF_ALLOC_ERROR = 8,   /* float x1: the error signal the last allocation used (the learned
                                 estimate when a model is attached, else CPT_BUF_ERROR) */
   CPT_BUF_SECOND_MOMENT = 9, /* float x3: running mean of squared samples (squareResult, AdapEmp.cu:927-944) */
-  CPT_BUF_PIXEL_PATHS = 10   /* uint32: eye paths traced per pixel since accumulation restarted (adaptive
+  CPT_BUF_PIXEL_PATHS = 10,  /* uint32: eye paths traced per pixel since accumulation restarted (adaptive
                                 modes; what "Min Pixel Paths" / "Max Pixel Paths" bound) */
+  CPT_BUF_BOUNCE_RAYS = 11   /* uint32 x 32, not per pixel: the last wavefront's extension rays entering
+                                bounce 0..15, then its shadow rays emitted at bounce 0..15 (the
+                                per-bounce queue lengths, what decides whether refilling finished
+                                paths could keep more lanes busy) */
 };
 int cpt_renderer_read_buffer(cpt_renderer* r, int which, void* out);
 

@@ -1497,15 +1497,26 @@ static void bilateral_defined(const float* color, const float* pos, const float*
 // Fixed-order double sum: pairwise tree inside 256-element blocks, blocks in
 // order.  Stands in for thrust::reduce (AdapBilatAtomic.cu:1444), whose float
 // order is unspecified; the device reduction uses the same tree.
-static double tree_sum(const float* e, size_t n) {
-  double total = 0.0;
+// The device's fixed-order sum (k_err_block_sums, k_err_final): a pairwise
+// tree over each 256-element block, the same tree over each chunk of 256 block
+// sums, then the chunk sums in order.
+static double tree256(const double* v, size_t n) {
   double s[256];
+  for (size_t t = 0; t < 256; t++) s[t] = t < n ? v[t] : 0.0;
+  for (int st = 128; st >= 1; st >>= 1)
+    for (int t = 0; t < st; t++) s[t] += s[t + st];
+  return s[0];
+}
+static double tree_sum(const float* e, size_t n) {
+  std::vector<double> bs;
+  double blk[256];
   for (size_t b = 0; b < n; b += 256) {
-    for (int t = 0; t < 256; t++) s[t] = (b + t < n) ? (double)e[b + t] : 0.0;
-    for (int st = 128; st >= 1; st >>= 1)
-      for (int t = 0; t < st; t++) s[t] += s[t + st];
-    total += s[0];
+    const size_t m = std::min<size_t>(256, n - b);
+    for (size_t t = 0; t < m; t++) blk[t] = (double)e[b + t];
+    bs.push_back(tree256(blk, m));
   }
+  double total = 0.0;
+  for (size_t c = 0; c < bs.size(); c += 256) total += tree256(bs.data() + c, std::min<size_t>(256, bs.size() - c));
   return total;
 }
 

@@ -290,3 +290,21 @@ def test_debug_mode_checks(cpt):
     R.set_attrib("Debug Mode", 5)
     with pytest.raises(cpt.CptError, match="not in 0..4"):
         R.render(S, sc["camera"], frames=1)
+
+
+def test_proportional_allocation_bit_exact_across_sum_chunks(cpt):
+    """480 x 270 pixels: the error sum spans two chunks of 256 block sums
+    (k_err_final); counts and offsets from the device's own error map equal
+    the oracle's sum and proportional rule bit for bit."""
+    print("PLL allocation at 480x270 (two error-sum chunks)", flush=True)
+    sc = scenes.cornell()
+    S = cpt.Scene(sc)
+    W, H = 480, 270
+    R = cpt.Renderer(W, H, mode=cpt.CPT_MODE_BDPT_ADAPTIVE_PLL)
+    R.render(S, sc["camera"], frames=6, result=False)
+    err = R.read(cpt.CPT_BUF_ERROR)
+    counts = R.read(cpt.CPT_BUF_SAMPLE_COUNTS)
+    assert (W * H + 255) // 256 > 256
+    total = np.float32(oracle.tree_sum(err))
+    want = oracle.alloc_proportional(err, total, W * H)
+    assert np.array_equal(counts.ravel(), np.asarray(want).ravel())

@@ -208,3 +208,20 @@ def test_unknown_traversal_variant_is_an_error(cpt, cornell_runs, attr):
     R.set_attrib(attr, 805)
     with pytest.raises(cpt.CptError, match="traversal variant"):
         R.render(S, sc["camera"], frames=1)
+
+
+def test_bounce_ray_counts_sum_to_the_ray_counters(cpt):
+    """CPT_BUF_BOUNCE_RAYS of a one-wavefront render: extension rays per
+    bounce start at every path, never grow, and sum to the extension-ray
+    counter; shadow rays per bounce sum to the shadow-ray counter."""
+    sc = scenes.cornell()
+    S = cpt.Scene(sc)
+    R = cpt.Renderer(64, 48)
+    R.set_attrib("Frames In Flight", 4)
+    R.render(S, sc["camera"], frames=4, result=False)
+    b = R.read(cpt.CPT_BUF_BOUNCE_RAYS)
+    st = R.stats()
+    assert b[0][0] == 64 * 48 * 4
+    assert all(b[0][i + 1] <= b[0][i] for i in range(15))
+    assert int(b[0].sum()) == st["extension_rays"]
+    assert int(b[1].sum()) == st["shadow_rays"] > 0

@@ -335,6 +335,25 @@ def test_error_estimators():
     assert c.sum() <= 1000 and c.sum() >= 1000 - 1000
 
 
+def test_tree_sum_exact_and_ordered():
+    """The fixed-order error sum: exact on sums that no order rounds (integers
+    below 2^53, sizes across the 256-element block and 256-block chunk
+    boundaries), within double rounding of math.fsum otherwise, and pairwise
+    inside a block: [2^53, 1 x 255] sums to 2^53 + 254 (the first pair rounds
+    2^53 + 1 to even, the other ones pair up exactly), where left to right
+    gives 2^53 and the exact sum is 2^53 + 255."""
+    import math
+    rng = np.random.default_rng(4)
+    for n in (1, 255, 256, 257, 65535, 65536, 65537, 300000):
+        ints = rng.integers(0, 1 << 20, size=n).astype(np.float32)
+        assert oracle.tree_sum(ints) == float(ints.astype(np.int64).sum())
+        u = rng.uniform(size=n).astype(np.float32)
+        assert abs(oracle.tree_sum(u) - math.fsum(u.astype(np.float64))) <= 1e-12 * n
+    v = np.ones(256, np.float32)
+    v[0] = 2.0 ** 53
+    assert oracle.tree_sum(v) == 2.0 ** 53 + 254
+
+
 # ---- round-based allocation (AdapBilatBinary / AdapEmp) ----------------------
 
 def test_rounds_constant_error_is_uniform():
