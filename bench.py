@@ -15,8 +15,8 @@ roofline: the closest-hit extension kernel (by default k_trace_dyn: persistent
   64 B (quantized layout) per 4-wide node visited + 48 B per triangle tested,
   over its HIP-event duration on the renderer stream.  Beside it, from
   rocprofv3 counter passes: the HBM bytes actually moved (FETCH_SIZE +
-  WRITE_SIZE, calibrated on a copy) and the VALU issue rate (SQ_INSTS_VALU),
-  and the limiter they name.
+  WRITE_SIZE, calibrated on a copy), the VALUs' busy share and lane use
+  (VALUBusy, VALUUtilization), and the limiter they name.
 cpu_baseline: the oracle's FP32 reference estimator on the host cores (OpenMP),
   a bounded sample of the same frame (a band of rows at 1 spp).
 time_to_equal_mse: the learned-vs-plain adaptive protocol of cudapt_amd/learned.py
@@ -233,16 +233,19 @@ def adaptive_leg(ca, args):
     return res
 
 
-def counter_per_launch(path, ext_name, calib):
+def counter_per_launch(path, ext_name, calib, counter=None):
     """(mean counter value per extension dispatch, scale) from one `rocprofv3
-    --pmc <counter> --output-format csv` file.  Dispatches whose name contains
-    ext_name are averaged.  For the byte counters (FETCH_SIZE / WRITE_SIZE,
-    KiB) the scale is calib / what the counter read for the largest copy,
-    which must be the calibration copy of calib bytes (the counter runs make no
-    larger one); calib None: raw values, scale 1.  None if missing."""
+    --pmc <counters> --output-format csv` file, over the rows of `counter`
+    (None: every row).  Dispatches whose name contains ext_name are averaged.
+    For the byte counters (FETCH_SIZE / WRITE_SIZE, KiB) the scale is calib /
+    what the counter read for the largest copy, which must be the calibration
+    copy of calib bytes (the counter runs make no larger one); calib None: raw
+    values, scale 1.  None if missing."""
     import csv
     ext, copy = [], []
     for row in csv.DictReader(open(path)):
+        if counter is not None and row["Counter_Name"] != counter:
+            continue
         v = float(row["Counter_Value"]) * (1024.0 if calib else 1.0)
         if ext_name in row["Kernel_Name"]:
             ext.append(v)
@@ -257,11 +260,18 @@ def counter_per_launch(path, ext_name, calib):
     return sum(ext) / len(ext), calib / max(copy)
 
 
+# One rocprofv3 --pmc pass each: the HBM byte counters, then the VALU
+# counters with the derived VALUBusy (SQ_ACTIVE_INST_VALU over GUI-active
+# cycles per CU: the share of time the VALUs work, dual issue counted once)
+# and VALUUtilization (active lanes per VALU instruction).
+COUNTER_PASSES = (("FETCH_SIZE",), ("WRITE_SIZE",), ("SQ_INSTS_VALU", "VALUBusy", "VALUUtilization"))
+
+
 def counter_passes(args):
-    """Per-extension-launch counters from one rocprofv3 --pmc pass each over one
-    step of the same workload: FETCH_SIZE and WRITE_SIZE (HBM bytes, each
-    scaled by what the counter reads for a 256 MiB device-to-device copy) and
-    SQ_INSTS_VALU (wave64 vector instructions issued)."""
+    """Per-extension-launch counters from rocprofv3 --pmc passes over one step
+    of the same workload: FETCH_SIZE and WRITE_SIZE (HBM bytes, each scaled by
+    what the counter reads for a 256 MiB device-to-device copy), and the VALU
+    counters of COUNTER_PASSES."""
     import shutil
     import subprocess
     import tempfile
@@ -271,9 +281,9 @@ def counter_passes(args):
     calib = 256 << 20
     ext_name = extend_kernel_name(args, demangled=True)
     out = {}
-    for counter in ("FETCH_SIZE", "WRITE_SIZE", "SQ_INSTS_VALU"):
+    for counters in COUNTER_PASSES:
         d = tempfile.mkdtemp(prefix="cpt_pmc_")
-        cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "t",
+        cmd = ["timeout", "-s", "KILL", "120", prof, "--pmc", *counters, "--output-format", "csv", "-d", d, "-o", "t",
                "--", sys.executable, os.path.abspath(__file__), "--steps", "1", "--warmup", "0", "--no-cpu",
                "--no-traffic", "--no-learned", "--no-multiview", "--baseline-steps", "0", "--adaptive-spp", "0",
                "--frames", str(args.frames), "--in-flight", str(args.in_flight),
@@ -283,22 +293,24 @@ def counter_passes(args):
                "--refill-groups", str(args.refill_groups),
                "--refill-shadow", str(args.refill_shadow),
                "--copy-calib", str(calib >> 20)]
-        log(f"counter pass: {counter}")
+        log(f"counter pass: {' '.join(counters)}")
         r = subprocess.run(cmd, capture_output=True, text=True)
         files = [os.path.join(root, f) for root, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
         if r.returncode != 0 or not files:
-            log(f"counter pass {counter} failed (rc {r.returncode}); reported as null")
+            log(f"counter pass {counters} failed (rc {r.returncode}); reported as null")
             return None
-        got = counter_per_launch(files[0], ext_name, calib if counter.endswith("_SIZE") else None)
+        for c in counters:
+            got = counter_per_launch(files[0], ext_name, calib if c.endswith("_SIZE") else None, c)
+            if got is None:
+                return None
+            out[c] = got
         shutil.rmtree(d, ignore_errors=True)
-        if got is None:
-            return None
-        out[counter] = got
     fetch, ffac = out["FETCH_SIZE"]
     write, wfac = out["WRITE_SIZE"]
     return {"bytes_per_launch": int(fetch * ffac + write * wfac), "fetch_bytes": int(fetch * ffac),
             "write_bytes": int(write * wfac), "fetch_scale": round(ffac, 3), "write_scale": round(wfac, 3),
-            "valu_insts_per_launch": int(out["SQ_INSTS_VALU"][0])}
+            "valu_insts_per_launch": int(out["SQ_INSTS_VALU"][0]), "valu_busy_pct": round(out["VALUBusy"][0], 2),
+            "valu_lane_util_pct": round(out["VALUUtilization"][0], 2)}
 
 
 def baselines_leg(ca, S, sc, args):
@@ -454,6 +466,8 @@ def main():
     ap.add_argument("--adaptive-spp", type=int, default=32, help="average spp of the adaptive leg (0: skip)")
     ap.add_argument("--mlp-only", action="store_true",
                     help="only the model's training-step throughput on synthetic features (for profiling)")
+    ap.add_argument("--multiview-only", action="store_true",
+                    help="only the multi-view leg, plain and adaptive without a model (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the library renders on one "
@@ -477,6 +491,9 @@ def main():
         T = (X[:, :3].sum(axis=1) + 0.1 * rng.normal(size=65536)).astype(np.float32)
         m = ca.Mlp(0xA22)
         print(json.dumps({"mlp": mlp_step_throughput(ca, m, X, T)}), flush=True)
+        return
+    if args.multiview_only:
+        print(json.dumps({"multiview": multiview_leg(ca, args)}), flush=True)
         return
     t0 = time.perf_counter()
     sc = scenes.sphere_grid()
@@ -604,22 +621,22 @@ def main():
             gbs = tr["bytes_per_launch"] / launch_s / 1e9
             rl["hbm_measured"] = {"achieved": round(gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                                   "frac": round(gbs / HBM_PEAK_GBS, 4)}
-            # VALU issue: a 32-bit wave64 vector instruction holds one of a CU's
-            # four 16-lane SIMDs for 4 cycles, so a CU completes about one per
-            # cycle: 256 CUs at 2.4 GHz.  Instructions that do not hold a SIMD
-            # that long can take the measured rate past 1.
+            # VALU: VALUBusy, the share of the kernel's time the CUs' VALUs
+            # work (gfx950 can issue two VALU instructions in one cycle, so the
+            # instruction rate alone does not bound it)
             valu_rate = tr["valu_insts_per_launch"] / launch_s
-            valu_peak = 256 * 2.4e9
-            rl["valu"] = {"wave_insts_per_launch": tr["valu_insts_per_launch"],
-                          "achieved": round(valu_rate / 1e9, 1), "peak": round(valu_peak / 1e9, 1),
-                          "unit": "G wave64 instructions/s (peak: one per CU-cycle)",
-                          "frac": round(valu_rate / valu_peak, 4),
+            rl["valu"] = {"busy_pct": tr["valu_busy_pct"], "frac": round(tr["valu_busy_pct"] / 100.0, 4),
+                          "lane_util_pct": tr["valu_lane_util_pct"],
+                          "wave_insts_per_launch": tr["valu_insts_per_launch"],
+                          "wave_insts_per_cu_cycle": round(valu_rate / (256 * 2.4e9), 3),
                           "per_ray": round(tr["valu_insts_per_launch"] * 64 / max(st["extension_rays"] / launches, 1),
                                            1)}
             hf, vf = rl["hbm_measured"]["frac"], rl["valu"]["frac"]
-            rl["limiter"] = ("HBM bandwidth" if hf >= 0.6 else "VALU issue" if vf >= 0.6 else
-                             f"latency of dependent node / triangle loads (HBM {hf:.0%} and VALU issue {vf:.0%} of "
-                             "peak; bytes served mostly by L2 / MALL)")
+            rl["limiter"] = ("HBM bandwidth" if hf >= 0.6 else
+                             f"VALU (busy {vf:.0%} of the kernel's time, {tr['valu_lane_util_pct']:.0f} % of lanes "
+                             f"active; HBM {hf:.0%} of peak)" if vf >= 0.6 else
+                             f"latency of dependent node / triangle loads (HBM {hf:.0%} of peak, VALU busy "
+                             f"{vf:.0%}; bytes served mostly by L2 / MALL)")
     if not args.no_cpu:
         log("cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(sc)

@@ -274,13 +274,44 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool c
         if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i; R.h.orig = oid; }
       }
     }
+    // (an occlusion ray that gets here hit nothing: its limit is unchanged and
+    // nothing is culled, but dropping the test costs the shadow kernel
+    // register spills)
     const float lim2 = R.h.t + cull_slack(R.h.t, R.eps);
 #pragma unroll
     for (int q = 0; q < 4; q++)
       if (key[q] != ~0ull && __uint_as_float((uint32_t)(key[q] >> 32)) > lim2) key[q] = ~0ull;
   }
-  cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[1], key[2]);
-  if (key[0] != ~0ull) {
+  if constexpr (ANY) {
+    // Occlusion: a segment that is not blocked visits every box it crosses in
+    // any order, and a blocked one stops at whichever hit comes first, so the
+    // children are taken in node order, unsorted: the first one next, the
+    // others pushed.
+    uint32_t m = 0;
+#pragma unroll
+    for (int c = 0; c < 4; c++) m |= key[c] != ~0ull ? 1u << c : 0u;
+    if (m) {
+      const uint32_t first = (uint32_t)__builtin_ctz(m);
+      R.node = (uint32_t)(first == 0 ? key[0] : first == 1 ? key[1] : first == 2 ? key[2] : key[3]);
+      m &= m - 1u;
+#pragma unroll
+      for (int c = 1; c < 4; c++) {
+        if ((m >> c) & 1u && R.sp < CPT_STACK_TOTAL) {
+          if (R.sp < STK) {
+            lstk[R.sp * kBlock] = key[c];
+          } else {
+            pstk[R.sp - STK] = key[c];
+            if (spill) atomicAdd(spill, 1u);
+          }
+          R.sp++;
+        }
+      }
+      return false;
+    }
+  } else {
+    cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[1], key[2]);
+  }
+  if (!ANY && key[0] != ~0ull) {
     R.node = (uint32_t)key[0];
 #pragma unroll
     for (int c = 3; c >= 1; c--) {

@@ -443,21 +443,24 @@ __device__ __forceinline__ void mlp_pack_one(MlpDevWeights d, uint32_t p, float 
 }
 
 // Gradient sum over the training workgroups' partials and Adam (Kingma & Ba
-// 2015).  Each workgroup takes 64 parameters; its four waves sum every fourth
-// partial, in order, and the four sums are added in wave order: a fixed order,
-// so the step is deterministic.  The last workgroup sums the loss partials.
+// 2015).  Each workgroup takes 64 parameters (consecutive lanes: coalesced
+// partial reads); its sixteen waves each sum every sixteenth partial, in
+// order, four loads in flight, and the sixteen sums are added pairwise in a
+// fixed tree: a fixed order, so the step is deterministic.  The last
+// workgroup sums the loss partials.
 constexpr int kAdamParams = 64;
-__global__ __launch_bounds__(256) void k_mlp_adam(MlpState st, const float* __restrict__ part, uint32_t nwg,
-                                                  const float* __restrict__ loss_part, float lr, float beta1,
-                                                  float beta2, float eps, float bc1, float bc2, int apply) {
-  __shared__ float red[4][kAdamParams + 1];
+constexpr int kAdamGroups = 16;
+__global__ __launch_bounds__(kAdamParams * kAdamGroups) void k_mlp_adam(
+    MlpState st, const float* __restrict__ part, uint32_t nwg, const float* __restrict__ loss_part, float lr,
+    float beta1, float beta2, float eps, float bc1, float bc2, int apply) {
+  __shared__ float red[kAdamGroups][kAdamParams];
   if (blockIdx.x == gridDim.x - 1) {
-    __shared__ float lr_[256];
+    __shared__ float lr_[kAdamParams * kAdamGroups];
     float L = 0.f;
-    for (uint32_t i = threadIdx.x; i < nwg; i += 256) L += loss_part[i];
+    for (uint32_t i = threadIdx.x; i < nwg; i += blockDim.x) L += loss_part[i];
     lr_[threadIdx.x] = L;
     __syncthreads();
-    for (int o = 128; o >= 1; o >>= 1) {
+    for (int o = kAdamParams * kAdamGroups / 2; o >= 1; o >>= 1) {
       if ((int)threadIdx.x < o) lr_[threadIdx.x] += lr_[threadIdx.x + o];
       __syncthreads();
     }
@@ -467,12 +470,18 @@ __global__ __launch_bounds__(256) void k_mlp_adam(MlpState st, const float* __re
   const int q = threadIdx.x & (kAdamParams - 1), grp = threadIdx.x / kAdamParams;
   const uint32_t p = blockIdx.x * kAdamParams + q;
   float gsum = 0.f;
-  if (p < (uint32_t)kMlpParams)
-    for (uint32_t i = grp; i < nwg; i += 4) gsum += part[(size_t)i * kMlpParams + p];
+  if (p < (uint32_t)kMlpParams) {
+#pragma unroll 4
+    for (uint32_t i = grp; i < nwg; i += kAdamGroups) gsum += part[(size_t)i * kMlpParams + p];
+  }
   red[grp][q] = gsum;
   __syncthreads();
+  for (int o = kAdamGroups / 2; o >= 1; o >>= 1) {
+    if (grp < o) red[grp][q] += red[grp + o][q];
+    __syncthreads();
+  }
   if (grp != 0 || p >= (uint32_t)kMlpParams) return;
-  const float gr = ((red[0][q] + red[1][q]) + red[2][q]) + red[3][q];
+  const float gr = red[0][q];
   st.grad[p] = gr;
   float w = st.w[p];
   if (apply) {
@@ -504,7 +513,7 @@ hipError_t launch_mlp_adam(const MlpState& st, const float* part, uint32_t nwg, 
   const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
   float bc1 = 1.f - powf(b1, (float)step), bc2 = 1.f - powf(b2, (float)step);
   const uint32_t pb = (kMlpParams + kAdamParams - 1) / kAdamParams;
-  k_mlp_adam<<<pb + 1, 256, 0, s>>>(st, part, nwg, loss_part, lr, b1, b2, eps, bc1, bc2, apply);
+  k_mlp_adam<<<pb + 1, kAdamParams * kAdamGroups, 0, s>>>(st, part, nwg, loss_part, lr, b1, b2, eps, bc1, bc2, apply);
   return hipGetLastError();
 }
 hipError_t launch_mlp_infer(const MlpState& st, const float* x, uint32_t n, float* y, hipStream_t s) {

@@ -187,7 +187,11 @@ def time_to_reach(ts, ms, target):
 def time_to_equal_mse(scene: Scene, model: Mlp, cams, refs, W, H, frames=64, depth=8, max_frames=None,
                       pass_frames=1, clamp=CLAMP, signal=0, attribs=None):
     """Per held-out pose: plain adaptive time, learned time and plain BDPT time
-    to plain adaptive's MSE after `frames` frames."""
+    to plain adaptive's MSE after `frames` frames; eye paths per pixel each
+    spent; the learned renderer's MSE after `frames` frames (a run of its own,
+    not stopped at the target); and plain BDPT's MSE after as many frames as
+    the learned renderer spent paths per pixel (rounded up): uniform sampling
+    at the learned renderer's sample count."""
     max_frames = max_frames or 4 * frames
     # warm the code paths once (module load, first allocations)
     mse_curve(scene, cams[0], refs[0], W, H, 2 * pass_frames + 5, model, depth=depth, pass_frames=pass_frames,
@@ -201,11 +205,15 @@ def time_to_equal_mse(scene: Scene, model: Mlp, cams, refs, W, H, frames=64, dep
                                pass_frames=pass_frames, clamp=clamp, signal=signal, attribs=attribs)
         fu, tu, mu, pu = mse_curve(scene, cam, ref, W, H, max_frames, None, mode=CPT_MODE_BDPT_STREAM, depth=depth,
                                stop_below=target, pass_frames=pass_frames, clamp=clamp)
-        at = np.searchsorted(fl, frames)
+        _, _, me, _ = mse_curve(scene, cam, ref, W, H, frames, model, depth=depth, pass_frames=pass_frames,
+                                clamp=clamp, signal=signal, attribs=attribs)
+        k = int(math.ceil(pl[-1]))                     # uniform frames = paths per pixel
+        at = np.nonzero(fu >= k)[0]
         rows.append(dict(target_mse=target, plain_adaptive_s=float(tp[-1]),
                          learned_s=time_to_reach(tl, ml, target), plain_bdpt_s=time_to_reach(tu, mu, target),
                          learned_frames=int(fl[-1]), plain_bdpt_frames=int(fu[-1]),
                          plain_adaptive_paths_per_pixel=float(pp[-1]), learned_paths_per_pixel=float(pl[-1]),
                          plain_bdpt_paths_per_pixel=float(pu[-1]),
-                         learned_mse_at_equal_frames=float(ml[min(at, len(ml) - 1)])))
+                         learned_mse_at_equal_frames=float(me[-1]),
+                         plain_bdpt_mse_at_learned_paths=float(mu[at[0]]) if at.size and fu[at[0]] == k else None))
     return rows
