@@ -291,7 +291,7 @@ def counter_passes(args):
                "--node-format", str(args.node_format),
                "--refill", str(args.refill), "--refill-secondary", str(args.refill_secondary),
                "--refill-groups", str(args.refill_groups),
-               "--refill-shadow", str(args.refill_shadow),
+               "--refill-shadow", str(args.refill_shadow), "--tail-bounce", str(args.tail_bounce),
                "--copy-calib", str(calib >> 20)]
         log(f"counter pass: {' '.join(counters)}")
         r = subprocess.run(cmd, capture_output=True, text=True)
@@ -443,6 +443,8 @@ def main():
                     help="the same for extension rays after the first bounce")
     ap.add_argument("--refill-shadow", type=int, default=16, help="the same for shadow rays")
     ap.add_argument("--refill-groups", type=int, default=8, help="workgroups per CU of the refilling grid")
+    ap.add_argument("--tail-bounce", type=int, default=0,
+                    help="bounces from this one on run as one k_tail launch (0: every bounce as wavefront passes)")
     ap.add_argument("--no-traffic", action="store_true", help="skip the rocprofv3 HBM-traffic passes")
     ap.add_argument("--copy-calib", type=int, default=0, metavar="MIB",
                     help="also run one device-to-device copy of MIB MiB (HBM counter calibration)")
@@ -468,6 +470,7 @@ def main():
                     help="only the model's training-step throughput on synthetic features (for profiling)")
     ap.add_argument("--multiview-only", action="store_true",
                     help="only the multi-view leg, plain and adaptive without a model (for profiling)")
+    ap.add_argument("--adaptive-only", action="store_true", help="only the adaptive leg (for profiling)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the library renders on one "
@@ -495,6 +498,9 @@ def main():
     if args.multiview_only:
         print(json.dumps({"multiview": multiview_leg(ca, args)}), flush=True)
         return
+    if args.adaptive_only:
+        print(json.dumps({"adaptive": adaptive_leg(ca, args)}), flush=True)
+        return
     t0 = time.perf_counter()
     sc = scenes.sphere_grid()
     S = ca.Scene(sc)
@@ -509,6 +515,7 @@ def main():
     R.set_attrib("Secondary Refill", args.refill_secondary)
     R.set_attrib("Refill Workgroups", args.refill_groups)
     R.set_attrib("Shadow Refill", args.refill_shadow)
+    R.set_attrib("Tail Bounce", args.tail_bounce)
     cam = sc["camera"]
     for w in range(args.warmup):
         R.render(S, cam, frames=args.frames, result=False)
@@ -536,9 +543,9 @@ def main():
     # ray read) when the refilling grid runs the first bounce over the
     # quantized nodes, as the renderer's schedule does above Refill Min Paths
     eye_in_kernel = args.refill >= 1 and args.node_format != 0 and WIDTH * HEIGHT * args.in_flight >= (1 << 20)
-    rays_read = st["extension_rays"] - (st["eye_paths"] if eye_in_kernel else 0)
-    bytes_alg = (32 * rays_read + 16 * st["extension_rays"] + node_bytes * st["extend_nodes"]
-                 + 48 * st["extend_tris"])
+    ext_rays = st["extension_rays"] - st["tail_rays"]     # the tail kernel's rays are not the extension kernel's
+    rays_read = ext_rays - (st["eye_paths"] if eye_in_kernel else 0)
+    bytes_alg = 32 * rays_read + 16 * ext_rays + node_bytes * st["extend_nodes"] + 48 * st["extend_tris"]
     ext_s = st["extend_ms"] / 1e3
     achieved = bytes_alg / ext_s / 1e9 if ext_s > 0 else 0.0
     launch_s = ext_s / launches
@@ -563,7 +570,7 @@ def main():
                    "shadow_variant": args.lds_stack_shadow or args.lds_stack, "node_format": args.node_format,
                    "traversal_refill": args.refill, "secondary_refill": args.refill_secondary,
                    "refill_groups": args.refill_groups,
-                   "shadow_refill": args.refill_shadow},
+                   "shadow_refill": args.refill_shadow, "tail_bounce": args.tail_bounce},
         "host_readback": "none in the timed region: the running mean stays in HBM (the reference copies it "
                          "to the host after every frame, src/cudaRTBDPTStream.cu:949, 11 MB per 1280x720 image)",
         "samples_per_s": round(st["eye_paths"] / wall, 1),
@@ -572,18 +579,18 @@ def main():
                  "shadow_per_bounce": [int(x) for x in bounce_rays[1][:EYE_DEPTH]]},
         "kernel_ms": {"extend": round(st["extend_ms"], 3), "shade": round(st["shade_ms"], 3),
                       "shadow": round(st["shadow_ms"], 3), "light_pass": round(st["light_ms"], 3),
-                      "all": round(st["total_ms"], 3)},
+                      "tail": round(st["tail_ms"], 3), "all": round(st["total_ms"], 3)},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "traffic_detail": None,
                      "kernel": extend_kernel_name(args),
                      "launches": launches, "avg_launch_us": round(st["extend_ms"] * 1e3 / launches, 2),
                      "bytes_per_launch": int(bytes_alg / launches),
-                     "nodes_per_ray": round(st["extend_nodes"] / max(st["extension_rays"], 1), 2),
-                     "tris_per_ray": round(st["extend_tris"] / max(st["extension_rays"], 1), 2),
+                     "nodes_per_ray": round(st["extend_nodes"] / max(ext_rays, 1), 2),
+                     "tris_per_ray": round(st["extend_tris"] / max(ext_rays, 1), 2),
                      "shadow_nodes_per_ray": round(st["shadow_nodes"] / max(st["shadow_rays"], 1), 2),
                      "shadow_tris_per_ray": round(st["shadow_tris"] / max(st["shadow_rays"], 1), 2),
                      "eye_rays_computed_in_kernel": bool(eye_in_kernel),
-                     "stack_spills_per_ray": round(st["extend_spills"] / max(st["extension_rays"], 1), 5),
+                     "stack_spills_per_ray": round(st["extend_spills"] / max(ext_rays, 1), 5),
                      "shadow_stack_spills_per_ray": round(st["shadow_spills"] / max(st["shadow_rays"], 1), 5),
                      "box_tests_per_s": round(4 * st["extend_nodes"] / ext_s, 1) if ext_s > 0 else None,
                      "tri_tests_per_s": round(st["extend_tris"] / ext_s, 1) if ext_s > 0 else None,
@@ -629,7 +636,7 @@ def main():
                           "lane_util_pct": tr["valu_lane_util_pct"],
                           "wave_insts_per_launch": tr["valu_insts_per_launch"],
                           "wave_insts_per_cu_cycle": round(valu_rate / (256 * 2.4e9), 3),
-                          "per_ray": round(tr["valu_insts_per_launch"] * 64 / max(st["extension_rays"] / launches, 1),
+                          "per_ray": round(tr["valu_insts_per_launch"] * 64 / max(ext_rays / launches, 1),
                                            1)}
             hf, vf = rl["hbm_measured"]["frac"], rl["valu"]["frac"]
             rl["limiter"] = ("HBM bandwidth" if hf >= 0.6 else

@@ -53,7 +53,9 @@ class _Stats(ctypes.Structure):
                                               "extend_launches", "extend_nodes", "extend_tris")] + \
                [(n, ctypes.c_double) for n in ("extend_ms", "shadow_ms", "shade_ms", "light_ms", "total_ms",
                                                "filter_ms", "error_ms", "gen_ms", "accum_ms", "mlp_ms")] + \
-               [(n, ctypes.c_uint64) for n in ("shadow_nodes", "shadow_tris", "extend_spills", "shadow_spills")]
+               [(n, ctypes.c_uint64) for n in ("shadow_nodes", "shadow_tris", "extend_spills", "shadow_spills",
+                                              "tail_rays", "tail_nodes", "tail_tris")] + \
+               [("tail_ms", ctypes.c_double)]
 
 
 def lib():

@@ -704,6 +704,7 @@ const char* const kLearnedSignal = "Learned Signal";
 // baseline) instead of the wavefront passes; same image, bit for bit.  Its
 // time is reported as extension time.
 const char* const kMegakernel = "Megakernel";
+const char* const kTailBounce = "Tail Bounce";
 const char* const kLdsStack = "Traversal LDS Stack";
 const char* const kLdsStackShadow = "Shadow LDS Stack";
 constexpr u32 kMaxInFlight = 90;   // 30 three-frame light-cache groups + partial ones at either end
@@ -754,7 +755,8 @@ struct cpt_renderer {
   struct Ev { hipEvent_t a, b; int kind; };
   std::vector<Ev> events;          // recorded pairs awaiting readback
   std::vector<hipEvent_t> pool;    // reusable events
-  double ms[10] = {0};  // extend, shadow, shade, light, total, filter, error sum, generation, accumulation, model
+  double ms[11] = {0};  // extend, shadow, shade, light, total, filter, error sum, generation, accumulation, model,
+                        // tail
   // shadow rays of bounce b traced on `side` while bounce b + 1's extension
   // rays run on `stream` (a captured batch iteration takes the side stream
   // into its graph through the same events)
@@ -986,7 +988,31 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
       gen_eye && big && r->attr(kRefill) >= 1.0 && r->attr(kNodeFormat) != 0 && fa.n_views == 0;
   if (gen_eye && !eye0) HIP_TRY(launch_gen_eye(fa, r->A.view(), r->stream));
   bool shadow_pending = false;
+  // "Tail Bounce" t > 0: bounces t.. run as one k_tail launch (plain BDPT's
+  // frames, whose accumulation goes through the sample records)
+  const double tail_attr = r->attr(kTailBounce);
+  const u32 tail = tail_attr >= 1.0 && tail_attr < (double)fa.eye_depth && ac.apply == 2 && !(ac.variant & kVariantGbuf) &&
+                           !(ac.variant & (kVariantPT | kVariantDisney))
+                       ? (u32)tail_attr
+                       : 0u;
   for (u32 b = 0; b < fa.eye_depth; b++) {
+    if (tail && b == tail) {
+      if (shadow_pending) {                      // the tail shades: bounce b - 1's occlusion results first
+        HIP_TRY(hipStreamWaitEvent(r->stream, r->ev_shadow, 0));
+        shadow_pending = false;
+      }
+      if (r->light_pending) {
+        HIP_TRY(hipStreamWaitEvent(r->stream, r->ev_light, 0));
+        r->light_pending = false;
+      }
+      const uint32_t grid =
+          (uint32_t)(r->cus > 0 ? r->cus : 256) * (uint32_t)std::min(32.0, std::max(1.0, r->attr(kRefillGroups)));
+      HIP_TRY(timed_begin(r, 10, &ev));
+      HIP_TRY(launch_tail(sc, fa, cur->view(), r->fc.p, b, grid, r->lv.p, ac, (int)r->attr(kLdsStack),
+                          r->attr(kNodeFormat) != 0, r->stats.p, r->stream));
+      HIP_TRY(timed_end(r, 10, ev));
+      break;
+    }
     r->extend_launches++;
     HIP_TRY(timed_begin(r, 0, &ev));
     const int quant = r->attr(kNodeFormat) != 0;
@@ -1017,7 +1043,10 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
     HIP_TRY(launch_shade(sc, fa, cur->view(), nxt->view(), r->hits.p, r->fc.p, (int)b, max_n, sb, r->lv.p, ac,
                          r->stream, eye0 && b == 0));
     HIP_TRY(timed_end(r, 2, ev));
-    if (b + 1 < fa.eye_depth) {
+    // no emissive triangle: no light vertex to connect to (and the path
+    // tracers connect nothing), so no shading emits an occlusion ray
+    const bool may_emit = sc.n_light_tris > 0 && !(ac.variant & kVariantPT);
+    if (b + 1 < fa.eye_depth && may_emit) {
       if (ov) {
         HIP_TRY(hipEventRecord(r->ev_shaded, r->stream));
         HIP_TRY(hipStreamWaitEvent(sst, r->ev_shaded, 0));
@@ -1561,7 +1590,10 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
     r->attribs.push_back({kLearnedSignal, 0, 0});
   } else {
     r->attribs.push_back({kInFlight, 0, 24});
-    if (mode == CPT_MODE_BDPT_STREAM) r->attribs.push_back({kMegakernel, 0, 0});
+    if (mode == CPT_MODE_BDPT_STREAM) {
+      r->attribs.push_back({kMegakernel, 0, 0});
+      r->attribs.push_back({kTailBounce, 0, 0});
+    }
   }
   // traversal kernel variant: 8 LDS stack entries, >= 6 waves per SIMD
   // (swept on the sphere grid: 1.58 vs 1.36 Grays/s for 16 entries)
@@ -1675,6 +1707,10 @@ int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset) {
   out->gen_ms = r->ms[7];
   out->accum_ms = r->ms[8];
   out->mlp_ms = r->ms[9];
+  out->tail_rays = h.tail_rays;
+  out->tail_nodes = h.tail_nodes;
+  out->tail_tris = h.tail_tris;
+  out->tail_ms = r->ms[10];
   if (reset) {
     HIP_TRY(hipMemset(r->stats.p, 0, sizeof(DevStats)));
     for (double& m : r->ms) m = 0;

@@ -132,6 +132,7 @@ struct DevStats {
   unsigned long long ext_nodes, ext_tris;
   unsigned long long sh_nodes, sh_tris;
   unsigned long long ext_spill, sh_spill;   // stack pushes below the LDS top (k_trace_dyn)
+  unsigned long long tail_rays, tail_nodes, tail_tris;   // k_tail: extension rays, nodes and triangles of both kinds
 };
 
 constexpr int kMaxDepth = 15;
@@ -195,6 +196,11 @@ hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, Pa
 // Whole batch of the plain stream renderer in one kernel: eye paths, every
 // bounce's closest hit, shading and connection occlusion per thread; results
 // into ac.rec.acc (ac.apply must be 2), counters into fc and stats.
+// The bounces from bounce0 on, one thread per live slot of cur, as k_mega
+// carries them (grid: workgroups of the persistent launch).
+hipError_t launch_tail(const DevScene& sc, const FrameArgs& fa, PathBuf cur, FrameCounters* fc, uint32_t bounce0,
+                       uint32_t grid, const LightVert* lv, const Accum& ac, int lds_stack, int quant,
+                       DevStats* stats, hipStream_t st);
 hipError_t launch_mega(const DevScene& sc, const FrameArgs& fa, FrameCounters* fc, const LightVert* lv,
                        const Accum& ac, int lds_stack, int quant, DevStats* stats, hipStream_t st);
 // Path regeneration's second wavefront: T per frame from ac.term_bounce, the

@@ -274,44 +274,16 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool c
         if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i; R.h.orig = oid; }
       }
     }
-    // (an occlusion ray that gets here hit nothing: its limit is unchanged and
-    // nothing is culled, but dropping the test costs the shadow kernel
-    // register spills)
     const float lim2 = R.h.t + cull_slack(R.h.t, R.eps);
 #pragma unroll
     for (int q = 0; q < 4; q++)
       if (key[q] != ~0ull && __uint_as_float((uint32_t)(key[q] >> 32)) > lim2) key[q] = ~0ull;
   }
-  if constexpr (ANY) {
-    // Occlusion: a segment that is not blocked visits every box it crosses in
-    // any order, and a blocked one stops at whichever hit comes first, so the
-    // children are taken in node order, unsorted: the first one next, the
-    // others pushed.
-    uint32_t m = 0;
-#pragma unroll
-    for (int c = 0; c < 4; c++) m |= key[c] != ~0ull ? 1u << c : 0u;
-    if (m) {
-      const uint32_t first = (uint32_t)__builtin_ctz(m);
-      R.node = (uint32_t)(first == 0 ? key[0] : first == 1 ? key[1] : first == 2 ? key[2] : key[3]);
-      m &= m - 1u;
-#pragma unroll
-      for (int c = 1; c < 4; c++) {
-        if ((m >> c) & 1u && R.sp < CPT_STACK_TOTAL) {
-          if (R.sp < STK) {
-            lstk[R.sp * kBlock] = key[c];
-          } else {
-            pstk[R.sp - STK] = key[c];
-            if (spill) atomicAdd(spill, 1u);
-          }
-          R.sp++;
-        }
-      }
-      return false;
-    }
-  } else {
-    cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[1], key[2]);
-  }
-  if (!ANY && key[0] != ~0ull) {
+  // Near-first for both ray kinds: occlusion rays taken in node order visit
+  // 15 % fewer nodes but test 23 % more triangles and run 1 % slower on the
+  // sphere grid.
+  cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[1], key[2]);
+  if (key[0] != ~0ull) {
     R.node = (uint32_t)key[0];
 #pragma unroll
     for (int c = 3; c >= 1; c--) {
@@ -2055,6 +2027,65 @@ hipError_t launch_regen(const FrameArgs& fa, PathBuf pb, const Accum& ac, FrameC
 // k_shadow), so the image is the same bit for bit; only the schedule differs:
 // no path buffers, no compaction, and lanes of a wave idle once their paths
 // end.  The frame counters receive the same per-bounce counts.
+// One bounce of a path that one thread carries (k_mega, k_tail): the closest
+// hit, shade_path, and the occlusion ray of the connection it sets up, whose
+// contribution is added when nothing occludes it.
+template <int STK, bool Q>
+__device__ __forceinline__ void carry_bounce(const DevScene& sc, const FrameArgs& fa, FrameCounters* fc,
+                                             const LightVert* lv, const Accum& ac, PathState& p, LdsU64* lstk,
+                                             bool& survive, bool& emit, uint32_t& en, uint32_t& et, uint32_t& sn,
+                                             uint32_t& st) {
+  TraceHit h;
+  const bool hit = traverse4<false, STK, Q>(sc, ld3(p.ro), ld3(p.rd), p.ro.w, p.rd.w, false, lstk, h, en, et);
+  const float4 hv = make_float4(hit ? h.t : kInfF, __uint_as_float(hit ? h.leaf : kInvalid), h.u, h.v);
+  ShadowRay sr;
+  sr.o = sr.d = sr.c = make_float4(0.f, 0.f, 0.f, 0.f);
+  shade_path(sc, fa, fc, lv, ac, p, hv, survive, emit, sr);
+  if (emit) {
+    TraceHit hs;
+    if (!traverse4<true, STK, Q>(sc, ld3(sr.o), ld3(sr.d), sr.o.w, sr.d.w, false, lstk, hs, sn, st)) {
+      p.acc.x = p.acc.x + sr.c.x;
+      p.acc.y = p.acc.y + sr.c.y;
+      p.acc.z = p.acc.z + sr.c.z;
+      p.acc.w = __uint_as_float(__float_as_uint(p.acc.w) + 4u);
+    }
+  }
+}
+
+// Per-bounce live and shadow counts of a workgroup into the frame counters,
+// and its traversal counters into the statistics (TAIL: the tail's own
+// counters, with nr extension rays).
+template <bool TAIL>
+__device__ __forceinline__ void carry_flush(FrameCounters* fc, DevStats* stats, const uint32_t* n_live,
+                                            const uint32_t* n_shadow, uint32_t depth, uint32_t en, uint32_t et,
+                                            uint32_t sn, uint32_t st, uint32_t nr = 0) {
+  if (threadIdx.x < depth) {
+    const uint32_t b = threadIdx.x;
+    if (b + 1 < 16 && n_live[b]) atomicAdd(&fc->active[b + 1], n_live[b]);
+    if (n_shadow[b]) atomicAdd(&fc->shadow[b], n_shadow[b]);
+  }
+  if (stats && TAIL) {
+    const uint32_t a0 = wave_sum(en + sn), a1 = wave_sum(et + st), a2 = wave_sum(nr);
+    if ((threadIdx.x & 63) == 0 && (a0 | a1 | a2)) {
+      atomicAdd(&stats->tail_nodes, (unsigned long long)a0);
+      atomicAdd(&stats->tail_tris, (unsigned long long)a1);
+      atomicAdd(&stats->tail_rays, (unsigned long long)a2);
+    }
+  } else if (stats) {
+    const uint32_t a0 = wave_sum(en), a1 = wave_sum(et), a2 = wave_sum(sn), a3 = wave_sum(st);
+    if ((threadIdx.x & 63) == 0) {
+      if (a0 | a1) {
+        atomicAdd(&stats->ext_nodes, (unsigned long long)a0);
+        atomicAdd(&stats->ext_tris, (unsigned long long)a1);
+      }
+      if (a2 | a3) {
+        atomicAdd(&stats->sh_nodes, (unsigned long long)a2);
+        atomicAdd(&stats->sh_tris, (unsigned long long)a3);
+      }
+    }
+  }
+}
+
 template <int STK, bool Q>
 __global__ __launch_bounds__(kBlock) void k_mega(DevScene sc, FrameArgs fa, FrameCounters* __restrict__ fc,
                                                  const LightVert* __restrict__ lv, Accum ac,
@@ -2080,21 +2111,7 @@ __global__ __launch_bounds__(kBlock) void k_mega(DevScene sc, FrameArgs fa, Fram
     if (__ballot(live) == 0ull) break;                    // wave-uniform
     bool survive = false, emit = false;
     if (live) {
-      TraceHit h;
-      const bool hit = traverse4<false, STK, Q>(sc, ld3(p.ro), ld3(p.rd), p.ro.w, p.rd.w, false, lstk, h, en, et);
-      const float4 hv = make_float4(hit ? h.t : kInfF, __uint_as_float(hit ? h.leaf : kInvalid), h.u, h.v);
-      ShadowRay sr;
-      sr.o = sr.d = sr.c = make_float4(0.f, 0.f, 0.f, 0.f);
-      shade_path(sc, fa, fc, lv, ac, p, hv, survive, emit, sr);
-      if (emit) {
-        TraceHit hs;
-        if (!traverse4<true, STK, Q>(sc, ld3(sr.o), ld3(sr.d), sr.o.w, sr.d.w, false, lstk, hs, sn, st)) {
-          p.acc.x = p.acc.x + sr.c.x;
-          p.acc.y = p.acc.y + sr.c.y;
-          p.acc.z = p.acc.z + sr.c.z;
-          p.acc.w = __uint_as_float(__float_as_uint(p.acc.w) + 4u);
-        }
-      }
+      carry_bounce<STK, Q>(sc, fa, fc, lv, ac, p, lstk, survive, emit, en, et, sn, st);
       live = survive;
     }
     const uint32_t cs = (uint32_t)__popcll(__ballot(survive)), ce = (uint32_t)__popcll(__ballot(emit));
@@ -2104,24 +2121,88 @@ __global__ __launch_bounds__(kBlock) void k_mega(DevScene sc, FrameArgs fa, Fram
     }
   }
   __syncthreads();
-  if (threadIdx.x < depth) {
-    const uint32_t b = threadIdx.x;
-    if (b + 1 < 16 && n_live[b]) atomicAdd(&fc->active[b + 1], n_live[b]);
-    if (n_shadow[b]) atomicAdd(&fc->shadow[b], n_shadow[b]);
-  }
-  if (stats) {
-    const uint32_t a0 = wave_sum(en), a1 = wave_sum(et), a2 = wave_sum(sn), a3 = wave_sum(st);
-    if (lane == 0) {
-      if (a0 | a1) {
-        atomicAdd(&stats->ext_nodes, (unsigned long long)a0);
-        atomicAdd(&stats->ext_tris, (unsigned long long)a1);
+  carry_flush<false>(fc, stats, n_live, n_shadow, depth, en, et, sn, st);
+}
+
+// The last bounces of the wavefront in one launch.  Once few paths are left,
+// a bounce's refilling grid runs partly empty and each of its three launches
+// (extension, shading, occlusion) waits for its slowest ray; from bounce
+// `bounce0` on, each thread instead takes live slots of `cur` (strided over a
+// persistent grid) and carries each path through the remaining bounces as
+// k_mega does, with the same per-path arithmetic: the image does not change.
+template <int STK, bool Q>
+__global__ __launch_bounds__(kBlock) void k_tail(DevScene sc, FrameArgs fa, PathBuf cur,
+                                                 FrameCounters* __restrict__ fc, uint32_t bounce0,
+                                                 const LightVert* __restrict__ lv, Accum ac,
+                                                 DevStats* __restrict__ stats) {
+  __shared__ uint64_t lstk_all[STK * kBlock];
+  __shared__ uint32_t n_live[16], n_shadow[16];
+  LdsU64* lstk = lds_column(lstk_all);
+  if (threadIdx.x < 16) { n_live[threadIdx.x] = 0u; n_shadow[threadIdx.x] = 0u; }
+  __syncthreads();
+  const uint32_t count = fc->active[bounce0];
+  const int lane = threadIdx.x & 63;
+  const uint32_t depth = fa.eye_depth < 16u ? fa.eye_depth : 16u;
+  uint32_t en = 0, et = 0, sn = 0, st = 0, nr = 0;
+  for (uint32_t base = blockIdx.x * kBlock; base < count; base += gridDim.x * kBlock) {   // workgroup-uniform
+    const uint32_t i = base + threadIdx.x;
+    bool live = i < count;
+    PathState p;
+    if (live) {
+      p.ro = cur.ray_o[i];
+      p.rd = cur.ray_d[i];
+      p.pix = cur.pix[i];
+      p.sid = cur.sid[i];
+      p.rng = cur.rng[i];
+      const uint32_t dk = cur.depth[i];
+      unpack_dk(p, dk);
+      if (dk & kFreshPath) {
+        p.mul = make_float4(1.f, 1.f, 1.f, 0.f);
+        p.smp = make_float4(0.f, 0.f, 0.f, 0.f);
+        p.acc = p.smp;
+      } else {
+        p.mul = cur.mul[i];
+        p.smp = cur.smp[i];
+        p.acc = cur.acc[i];
       }
-      if (a2 | a3) {
-        atomicAdd(&stats->sh_nodes, (unsigned long long)a2);
-        atomicAdd(&stats->sh_tris, (unsigned long long)a3);
+    }
+    for (uint32_t b = bounce0; b < depth; b++) {
+      if (__ballot(live) == 0ull) break;                  // wave-uniform
+      bool survive = false, emit = false;
+      if (live) {
+        carry_bounce<STK, Q>(sc, fa, fc, lv, ac, p, lstk, survive, emit, en, et, sn, st);
+        live = survive;
+        nr++;
+      }
+      const uint32_t cs = (uint32_t)__popcll(__ballot(survive)), ce = (uint32_t)__popcll(__ballot(emit));
+      if (lane == 0) {
+        if (cs) atomicAdd(&n_live[b], cs);
+        if (ce) atomicAdd(&n_shadow[b], ce);
       }
     }
   }
+  __syncthreads();
+  carry_flush<true>(fc, stats, n_live, n_shadow, depth, en, et, sn, st, nr);
+}
+hipError_t launch_tail(const DevScene& sc, const FrameArgs& fa, PathBuf cur, FrameCounters* fc, uint32_t bounce0,
+                       uint32_t grid, const LightVert* lv, const Accum& ac, int lds_stack, int quant,
+                       DevStats* stats, hipStream_t st) {
+  if (!fc || !lv || !ac.rec.acc || ac.apply != 2 || (ac.variant & kVariantGbuf) || fa.eye_depth > 16 ||
+      bounce0 == 0 || bounce0 >= fa.eye_depth || grid == 0 || !pathbuf_ok(cur) || !sc.tris ||
+      !(quant ? (const void*)sc.nodes4q : (const void*)sc.nodes4))
+    return hipErrorInvalidValue;
+#define CPT_TL(S, W, Q) k_tail<S, Q><<<grid, kBlock, 0, st>>>(sc, fa, cur, fc, bounce0, lv, ac, stats)
+#define CPT_TL_Q(S, W) CPT_TL(S, W, true)
+#define CPT_TL_F(S, W) CPT_TL(S, W, false)
+  if (quant) {
+    CPT_VARIANT_SWITCH(lds_stack, CPT_TL_Q)
+  } else {
+    CPT_VARIANT_SWITCH(lds_stack, CPT_TL_F)
+  }
+#undef CPT_TL_Q
+#undef CPT_TL_F
+#undef CPT_TL
+  return hipGetLastError();
 }
 hipError_t launch_mega(const DevScene& sc, const FrameArgs& fa, FrameCounters* fc, const LightVert* lv,
                        const Accum& ac, int lds_stack, int quant, DevStats* stats, hipStream_t st) {

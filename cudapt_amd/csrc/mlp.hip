@@ -87,10 +87,15 @@ constexpr int kTrainWaves = 4;                  // waves per training workgroup
 // chunk's 32 pixels, one k-block of v_mfma_f32_16x16x32_bf16.
 constexpr int kTrainPB = 2;                      // 16-pixel blocks per training chunk
 constexpr int kTrainChunk = 16 * kTrainPB;
+// Row stride of the transposed planes: 8 pads a 64-byte row to 80 bytes, so
+// the four lane groups' 16-bit stores of one unit quad and the 16 rows of a
+// 16-byte fragment read fall in different LDS banks (with 64-byte rows both
+// hit one bank four times over).
+constexpr int kPlaneStride = kTrainChunk + 8;
 struct MlpWaveLds {
-  alignas(16) uint16_t xt[16 * kTrainChunk];    // X^T  [feature][pixel] (features 9..15 zero)
-  alignas(16) uint16_t h1t[64 * kTrainChunk];   // H1^T [hidden][pixel], then dZ1^T
-  alignas(16) uint16_t dz2t[64 * kTrainChunk];  // dZ2^T [out][pixel]
+  alignas(16) uint16_t xt[16 * kPlaneStride];    // X^T  [feature][pixel] (features 9..15 zero)
+  alignas(16) uint16_t h1t[64 * kPlaneStride];   // H1^T [hidden][pixel], then dZ1^T
+  alignas(16) uint16_t dz2t[64 * kPlaneStride];  // dZ2^T [out][pixel]
   float vec[3][64];                             // dW3, db2, db1 summed over the wave's chunks
 };
 // Weight operands and vectors, shared by the workgroup's waves.
@@ -193,7 +198,7 @@ __global__ __launch_bounds__(64 * kTrainWaves, 1) void k_mlp_train(MlpDevWeights
 #pragma unroll
     for (int j = 0; j < 4; j++) aw2[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
-  constexpr int PB = kTrainPB, C = kTrainChunk;
+  constexpr int PB = kTrainPB, C = kTrainChunk, S = kPlaneStride;
   const uint32_t nchunks = (n + C - 1) / C;
   for (uint32_t ch = blockIdx.x * kTrainWaves + wave; ch < nchunks; ch += gridDim.x * kTrainWaves) {
     const uint32_t base = ch * C;
@@ -202,7 +207,7 @@ __global__ __launch_bounds__(64 * kTrainWaves, 1) void k_mlp_train(MlpDevWeights
 #pragma unroll
     for (int pb = 0; pb < PB; pb++)
 #pragma unroll
-      for (int j = 0; j < 4; j++) L.xt[(g * 4 + j) * C + pb * 16 + c] = (uint16_t)xb[pb][j];
+      for (int j = 0; j < 4; j++) L.xt[(g * 4 + j) * S + pb * 16 + c] = (uint16_t)xb[pb][j];
     // ---- forward; the GELU derivatives and H2 are kept as FP16 pairs
     bf16x8 hk[2][PB];                                      // H1^T as layer-2 B operands
     h16x2 dg1[4][PB][2];
@@ -219,7 +224,7 @@ __global__ __launch_bounds__(64 * kTrainWaves, 1) void k_mlp_train(MlpDevWeights
           dg1[hb][pb][e >> 1][e & 1] = (_Float16)dg;
           const uint16_t hbits = f2bf(gv);
           hk[hb >> 1][pb][(hb & 1) * 4 + e] = as_bf(hbits);
-          L.h1t[u * C + pb * 16 + c] = hbits;
+          L.h1t[u * S + pb * 16 + c] = hbits;
         }
         tile_fence();
       }
@@ -273,7 +278,7 @@ __global__ __launch_bounds__(64 * kTrainWaves, 1) void k_mlp_train(MlpDevWeights
           v2[ob][e] += dz;
           const uint16_t bits = f2bf(dz);
           dk[ob >> 1][pb][(ob & 1) * 4 + e] = as_bf(bits);
-          L.dz2t[u * C + pb * 16 + c] = bits;
+          L.dz2t[u * S + pb * 16 + c] = bits;
         }
         tile_fence();
       }
@@ -285,10 +290,10 @@ __global__ __launch_bounds__(64 * kTrainWaves, 1) void k_mlp_train(MlpDevWeights
     for (int kb = 0; kb < C / 32; kb++) {
       bf16x8 bo[4];
 #pragma unroll
-      for (int ob = 0; ob < 4; ob++) bo[ob] = ld_frag8(&L.dz2t[(ob * 16 + c) * C + kb * 32 + g * 8]);
+      for (int ob = 0; ob < 4; ob++) bo[ob] = ld_frag8(&L.dz2t[(ob * 16 + c) * S + kb * 32 + g * 8]);
 #pragma unroll
       for (int ib = 0; ib < 4; ib++) {
-        const bf16x8 ai = ld_frag8(&L.h1t[(ib * 16 + c) * C + kb * 32 + g * 8]);
+        const bf16x8 ai = ld_frag8(&L.h1t[(ib * 16 + c) * S + kb * 32 + g * 8]);
 #pragma unroll
         for (int ob = 0; ob < 4; ob++) aw2[ib][ob] = mfma32(ai, bo[ob], aw2[ib][ob]);
       }
@@ -308,7 +313,7 @@ __global__ __launch_bounds__(64 * kTrainWaves, 1) void k_mlp_train(MlpDevWeights
           const int u = ib * 16 + g * 4 + e;
           const float dz = dh[e] * (float)dg1[ib][pb][e >> 1][e & 1];
           v1[ib][e] += dz;
-          L.h1t[u * C + pb * 16 + c] = f2bf(dz);
+          L.h1t[u * S + pb * 16 + c] = f2bf(dz);
         }
         tile_fence();
       }
@@ -318,10 +323,10 @@ __global__ __launch_bounds__(64 * kTrainWaves, 1) void k_mlp_train(MlpDevWeights
     // dW1[feature][hidden] += X^T . dZ1 over the chunk's pixels
 #pragma unroll
     for (int kb = 0; kb < C / 32; kb++) {
-      const bf16x8 ax = ld_frag8(&L.xt[c * C + kb * 32 + g * 8]);
+      const bf16x8 ax = ld_frag8(&L.xt[c * S + kb * 32 + g * 8]);
 #pragma unroll
       for (int hb = 0; hb < 4; hb++)
-        aw1[hb] = mfma32(ax, ld_frag8(&L.h1t[(hb * 16 + c) * C + kb * 32 + g * 8]), aw1[hb]);
+        aw1[hb] = mfma32(ax, ld_frag8(&L.h1t[(hb * 16 + c) * S + kb * 32 + g * 8]), aw1[hb]);
     }
     wave_lds_sync();                                       // the next chunk rewrites the planes
   }

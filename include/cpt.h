@@ -287,6 +287,10 @@ typedef struct cpt_stats {
   uint64_t shadow_nodes, shadow_tris;             /* 4-wide nodes visited / triangles tested by shadow rays */
   uint64_t extend_spills, shadow_spills;          /* traversal-stack pushes that went past the LDS entries into
                                                      the private (scratch) stack, refilling-wave kernels */
+  uint64_t tail_rays, tail_nodes, tail_tris;      /* "Tail Bounce": extension rays traced by the tail kernel, and
+                                                     its nodes / triangles of both ray kinds (the extend_ and
+                                                     shadow_ counters above leave them out) */
+  double tail_ms;                                 /* the tail kernel's time */
 } cpt_stats;
 int cpt_renderer_stats(cpt_renderer* r, cpt_stats* out, int reset);
 

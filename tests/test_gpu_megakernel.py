@@ -79,3 +79,45 @@ def test_megakernel_unknown_variant_is_an_error(cpt, scene_objs):
     R.set_attrib("Traversal LDS Stack", 805)
     with pytest.raises(cpt.CptError):
         R.render(S, sc["camera"], frames=1)
+
+
+@pytest.mark.parametrize("name", list(SCENES))
+@pytest.mark.parametrize("tail", [1, 4, 7])
+def test_tail_bounces_in_one_launch_same_image(cpt, scene_objs, name, tail):
+    """"Tail Bounce": bounces tail.. of every path run in one k_tail launch
+    (one thread carries each live slot through the remaining bounces).  The
+    frames, counts and ray counters equal the wavefront's; the tail's own
+    node counters take the rest of the traversal work."""
+    sc, S = scene_objs[name]
+    a, b = _pair(cpt, 96, 64, **{"in_flight": 5, "eye_depth": 8})
+    b.set_attrib("megakernel", 0)
+    b.set_attrib("Tail Bounce", tail)
+    ra = a.render(S, sc["camera"], frames=10)
+    rb = b.render(S, sc["camera"], frames=10)
+    assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
+    assert np.array_equal(a.sample_counts(), b.sample_counts())
+    sa, sb = a.stats(), b.stats()
+    for k in ("extension_rays", "shadow_rays", "eye_paths", "frames"):
+        assert sa[k] == sb[k], k
+    assert sb["tail_rays"] > 0
+    assert sb["extend_nodes"] + sb["shadow_nodes"] + sb["tail_nodes"] == sa["extend_nodes"] + sa["shadow_nodes"]
+    assert sb["extend_tris"] + sb["shadow_tris"] + sb["tail_tris"] == sa["extend_tris"] + sa["shadow_tris"]
+
+
+def test_tail_bounce_on_a_refilling_batch(cpt, scene_objs):
+    """Above the refill threshold (refilling waves, computed eye rays, the side
+    stream) and in a multi-view batch (graph replays) the tail keeps the image."""
+    sc, S = scene_objs["sphere_grid"]
+    a, b = _pair(cpt, 320, 240, **{"in_flight": 16})
+    b.set_attrib("megakernel", 0)
+    b.set_attrib("Tail Bounce", 3)
+    ra = a.render(S, sc["camera"], frames=16)
+    rb = b.render(S, sc["camera"], frames=16)
+    assert np.array_equal(ra.view(np.uint32), rb.view(np.uint32))
+    views = [dict(width=64, height=48, **sc["camera"]), dict(width=40, height=30, **sc["camera"])]
+    B1, B2 = cpt.Batch(views, frames_in_flight=4), cpt.Batch(views, frames_in_flight=4)
+    B2.set_attrib("Tail Bounce", 2)
+    B1.render(S, 8)
+    B2.render(S, 8)
+    for v in range(2):
+        assert np.array_equal(B1.result(v).view(np.uint32), B2.result(v).view(np.uint32))
