@@ -71,6 +71,23 @@ def extend_kernel_name(args, demangled=False) -> str:
                 f"EYE at the first bounce: eye rays computed in the kernel)")
     return f"k_trace<{layout},false,1,{stk},{minw}> (closest-hit extension, one thread per ray)"
 
+def queue_bytes(bounce_rays, eye_in_kernel):
+    """Algorithmic SoA queue bytes of one wavefront per bounce (path buffers,
+    hits and the shadow queue; not the scene): the extension reads each ray
+    (32 B) and writes its hit (16 B); shading reads the hit and the path state
+    (100 B) and writes each survivor's state (100 B) and each shadow ray
+    (48 B); the shadow kernel reads the shadow ray and updates the path's
+    accumulator (16 B read + 16 B written).  First-bounce eye rays computed in
+    the kernels read no ray and no state."""
+    ext, sh = [int(x) for x in bounce_rays[0]], [int(x) for x in bounce_rays[1]]
+    out = []
+    for b in range(EYE_DEPTH):
+        fresh = b == 0 and eye_in_kernel
+        nxt = ext[b + 1] if b + 1 < len(ext) else 0
+        out.append(ext[b] * (16 + 16 + (0 if fresh else 32 + 100)) + nxt * 100 + sh[b] * (48 + 48 + 32))
+    return out
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -318,7 +335,9 @@ def baselines_leg(ca, S, sc, args):
     the megakernel schedule of the same estimator (one thread per sample through
     every bounce; one step compared bit for bit with the wavefront's), and the
     unidirectional path tracers (cudaRTPTStream, cudaRTPTStreamRegen, eye depth
-    10), each timed over `--baseline-steps` steps."""
+    10) and the "Path Regen" BDPT (cudaRTBDPTRegen: one thread per pixel, camera
+    paths restarted in place, its own 640-vertex light cache), each timed over
+    `--baseline-steps` steps."""
     def renderer(mode=0, mega=False):
         R = ca.Renderer(WIDTH, HEIGHT, mode=mode)
         if mode == 0:
@@ -368,6 +387,13 @@ def baselines_leg(ca, S, sc, args):
         out[name] = timed(R, name)
         out[name]["eye_depth"] = 10
         del R
+    R = ca.Renderer(WIDTH, HEIGHT, mode=ca.CPT_MODE_BDPT_REGEN)
+    R.set_attrib("Eye Depth", EYE_DEPTH)
+    R.set_attrib("Traversal LDS Stack", args.lds_stack)
+    R.set_attrib("Node Format", args.node_format)
+    out["bdpt_path_regen"] = timed(R, "bdpt_path_regen")
+    out["bdpt_path_regen"]["eye_depth"] = EYE_DEPTH
+    del R
     return out
 
 
@@ -576,7 +602,8 @@ def main():
         "samples_per_s": round(st["eye_paths"] / wall, 1),
         "rays": {"extension": st["extension_rays"], "shadow": st["shadow_rays"],
                  "extension_per_bounce": [int(x) for x in bounce_rays[0][:EYE_DEPTH]],
-                 "shadow_per_bounce": [int(x) for x in bounce_rays[1][:EYE_DEPTH]]},
+                 "shadow_per_bounce": [int(x) for x in bounce_rays[1][:EYE_DEPTH]],
+                 "queue_bytes_per_bounce": queue_bytes(bounce_rays, eye_in_kernel)},
         "kernel_ms": {"extend": round(st["extend_ms"], 3), "shade": round(st["shade_ms"], 3),
                       "shadow": round(st["shadow_ms"], 3), "light_pass": round(st["light_ms"], 3),
                       "tail": round(st["tail_ms"], 3), "all": round(st["total_ms"], 3)},
@@ -608,6 +635,8 @@ def main():
         res["baselines"] = baselines_leg(ca, S, sc, args)
         res["baselines"]["wavefront_over_megakernel"] = round(
             mrays / max(res["baselines"]["megakernel"]["mrays_per_s"], 1e-9), 3)
+        res["baselines"]["wavefront_over_path_regen"] = round(
+            mrays / max(res["baselines"]["bdpt_path_regen"]["mrays_per_s"], 1e-9), 3)
     del S
     res["adaptive"] = adaptive_leg(ca, args) if args.adaptive_spp > 0 else None
     res["mlp"] = None

@@ -24,6 +24,7 @@ CPT_MODE_BDPT_ADAPTIVE_EMP = 3
 CPT_MODE_PT_STREAM = 4
 CPT_MODE_PT_STREAM_REGEN = 5
 CPT_MODE_PT_DISNEY_MIS = 6
+CPT_MODE_BDPT_REGEN = 7
 (CPT_BUF_RESULT, CPT_BUF_SAMPLE_N, CPT_BUF_GBUF_POS, CPT_BUF_GBUF_NORM, CPT_BUF_GBUF_DIFF, CPT_BUF_FILTERED,
  CPT_BUF_ERROR, CPT_BUF_SAMPLE_COUNTS, CPT_BUF_ALLOC_ERROR, CPT_BUF_SECOND_MOMENT, CPT_BUF_PIXEL_PATHS,
  CPT_BUF_BOUNCE_RAYS) = range(12)

@@ -618,6 +618,7 @@ namespace {
 bool is_adaptive(int mode) { return mode >= CPT_MODE_BDPT_ADAPTIVE_PLL && mode <= CPT_MODE_BDPT_ADAPTIVE_EMP; }
 bool is_pt(int mode) { return mode == CPT_MODE_PT_STREAM || mode == CPT_MODE_PT_STREAM_REGEN; }
 bool is_disney(int mode) { return mode == CPT_MODE_PT_DISNEY_MIS; }
+bool is_bdpt_regen(int mode) { return mode == CPT_MODE_BDPT_REGEN; }
 bool uses_filter(int mode) { return mode == CPT_MODE_BDPT_ADAPTIVE_PLL || mode == CPT_MODE_BDPT_ADAPTIVE_BINARY; }
 
 struct Attrib {
@@ -890,6 +891,7 @@ int renderer_alloc(cpt_renderer* r) {
 
 int lv_alloc(cpt_renderer* r) {
   size_t need = (size_t)r->attr(kLightPaths) * ((size_t)r->attr(kLightDepth) + 1) * kLightSlots;
+  if (is_bdpt_regen(r->mode)) need = kRegenLightVerts;     // one fixed 128 x 5 cache
   if (need > r->lv.n) r->forget_light_caches();
   HIP_TRY(r->lv.alloc(need));
   return CPT_OK;
@@ -1315,8 +1317,32 @@ int render_adaptive_frame(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa, co
 // fa.nframes frames of cudaRTBDPTStream::Render traced as one wavefront
 // (src/cudaRTBDPTStream.cu:874-946), then applied to the running mean in
 // frame order: identical to fa.nframes sequential calls.
+// cudaRTBDPTRegen::Render (src/cudaRTBDPTRegen.cu:728-792), frame by frame: the
+// light cache when frame % 3 == 0, then one thread per pixel.
+int render_bdpt_regen(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa0) {
+  DevScene sc = s->dev();
+  FrameArgs fa = fa0;
+  fa.nframes = 1;
+  hipEvent_t ev = nullptr;
+  for (u32 k = 0; k < fa0.nframes; k++) {
+    fa.frame0 = fa0.frame0 + k;
+    if (fa.frame0 % 3 == 0) {
+      HIP_TRY(timed_begin(r, 3, &ev));
+      HIP_TRY(launch_regen_light(sc, fa, r->lv.p, r->stream));
+      HIP_TRY(timed_end(r, 3, ev));
+    }
+    r->extend_launches++;
+    HIP_TRY(timed_begin(r, 0, &ev));
+    HIP_TRY(launch_regen_eye(sc, fa, r->lv.p, r->result.p, r->sample_n.p, (int)r->attr(kLdsStack),
+                             r->attr(kNodeFormat) != 0, r->stats.p, r->stream));
+    HIP_TRY(timed_end(r, 0, ev));
+  }
+  return CPT_OK;
+}
+
 int render_batch(cpt_renderer* r, cpt_scene* s, const FrameArgs& fa) {
   if (is_adaptive(r->mode)) return render_adaptive_frame(r, s, fa);
+  if (is_bdpt_regen(r->mode)) return render_bdpt_regen(r, s, fa);
   DevScene sc = s->dev();
   const u32 n = r->W * r->H * fa.nframes;
   hipEvent_t ev = nullptr;
@@ -1435,6 +1461,8 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
       return fail(CPT_E_ARG, "render: Debug Mode 4 (Filtered Result) needs a filtering mode; the empirical mode has none");
   }
   if (eye_depth < 1 || eye_depth > (u32)kMaxDepth) return fail(CPT_E_LIMIT, "render: eye depth %u not in 1..15", eye_depth);
+  if (is_bdpt_regen(r->mode) && eye_depth > 8)
+    return fail(CPT_E_LIMIT, "render: eye depth %u: the Path Regen renderer carries at most 8 levels", eye_depth);
   if (light_depth < 1 || light_depth > 64) return fail(CPT_E_LIMIT, "render: light depth %u not in 1..64", light_depth);
   if (light_paths < 1 || light_paths > (u32)kBlock) return fail(CPT_E_LIMIT, "render: light paths %u not in 1..256", light_paths);
   for (const char* a : {kLdsStack, kLdsStackShadow})
@@ -1548,7 +1576,7 @@ extern "C" {
 
 int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t height) {
   if (!out) return fail(CPT_E_ARG, "cpt_renderer_create: null output");
-  if (mode < CPT_MODE_BDPT_STREAM || mode > CPT_MODE_PT_DISNEY_MIS)
+  if (mode < CPT_MODE_BDPT_STREAM || mode > CPT_MODE_BDPT_REGEN)
     return fail(CPT_E_ARG, "cpt_renderer_create: unknown mode %d", mode);
   if (width == 0 || height == 0 || (uint64_t)width * height > (1ull << 26))
     return fail(CPT_E_LIMIT, "cpt_renderer_create: %ux%u outside 1..2^26 pixels", width, height);
@@ -1573,6 +1601,7 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
                   {"Avg Accum Time", 3, 0}};
   }
   // NORMALRAY_BOUND_MAX: 5 in the BDPT files, 10 in the path tracers (cudaRTPTStream.cu:7)
+  // (cudaRTBDPTRegen: NORMALRAY_BOUND_MAX 5, src/cudaRTBDPTRegen.cu:4)
   r->attribs.push_back({kEyeDepth, 0, is_pt(mode) ? 10.0 : is_disney(mode) ? 8.0 : 5.0});
   r->attribs.push_back({kLightDepth, 0, 5});
   r->attribs.push_back({kLightPaths, 0, 128});
@@ -1589,7 +1618,7 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
     r->attribs.push_back({kPassFrames, 0, 1});
     r->attribs.push_back({kLearnedSignal, 0, 0});
   } else {
-    r->attribs.push_back({kInFlight, 0, 24});
+    r->attribs.push_back({kInFlight, 0, is_bdpt_regen(mode) ? 1.0 : 24.0});
     if (mode == CPT_MODE_BDPT_STREAM) {
       r->attribs.push_back({kMegakernel, 0, 0});
       r->attribs.push_back({kTailBounce, 0, 0});

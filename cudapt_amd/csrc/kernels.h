@@ -196,6 +196,14 @@ hipError_t launch_shade(const DevScene& sc, const FrameArgs& fa, PathBuf cur, Pa
 // Whole batch of the plain stream renderer in one kernel: eye paths, every
 // bounce's closest hit, shading and connection occlusion per thread; results
 // into ac.rec.acc (ac.apply must be 2), counters into fc and stats.
+// The reference's "Path Regen" megakernel BDPT (cudaRTBDPTRegen): the
+// 640-vertex light cache of frame fa.frame0, and one frame of one thread per
+// pixel (eye_depth <= 8 levels) into the running mean.  lv holds
+// kRegenLightVerts vertices.
+constexpr uint32_t kRegenLightVerts = 640;
+hipError_t launch_regen_light(const DevScene& sc, const FrameArgs& fa, LightVert* lv, hipStream_t st);
+hipError_t launch_regen_eye(const DevScene& sc, const FrameArgs& fa, const LightVert* lv, float* result,
+                            uint32_t* sample_n, int lds_stack, int quant, DevStats* stats, hipStream_t st);
 // The bounces from bounce0 on, one thread per live slot of cur, as k_mega
 // carries them (grid: workgroups of the persistent launch).
 hipError_t launch_tail(const DevScene& sc, const FrameArgs& fa, PathBuf cur, FrameCounters* fc, uint32_t bounce0,

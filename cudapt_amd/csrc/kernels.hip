@@ -2016,6 +2016,237 @@ hipError_t launch_regen(const FrameArgs& fa, PathBuf pb, const Accum& ac, FrameC
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- "Path Regen" BDPT
+// The reference's megakernel BDPT renderer (src/cudaRTBDPTRegen.cu, the
+// "Path Regen" baseline), one thread per pixel.
+//  - k_regen_light, every third frame: 128 light paths of 5 vertices each fill
+//    a 640-vertex cache (bdpt_light_kernel :314-330, bdpt_light_ray
+//    :80-309).  A path starts on a random emissive triangle (vertex: irrad =
+//    emissive, no direction) and then carries the throughput (1, 1, 1), not
+//    the emission (:179, :307); later vertices store the throughput they
+//    arrived with and the hit's interpolated normal; a miss stores an empty
+//    vertex, and the next vertex starts a new path, as does a throughput
+//    below 1e-4.
+//  - k_regen_eye: pt0_normalRay's recursion (:433-625) as a loop over at most
+//    eye_depth levels with the per-level factors kept in registers: the BRDF
+//    bounce (refraction past the critical angle reflects, :501-505), Russian
+//    roulette on the pixel weight, whose failure restarts a camera path for
+//    the same pixel (:562-571) and whose success connects a diffuse bounce to a
+//    uniformly chosen cache vertex; then the radiance summed back up the
+//    levels, r = f * r' + emission + connection, banked where a camera path
+//    starts (:598-606), and the pixel's running mean over the count of camera
+//    paths plus 4 per contributing connection (:655-664).
+constexpr uint32_t kRegenLightN = kRegenLightVerts, kRegenLightDepth = 5, kRegenMaxLevels = 8;
+constexpr uint32_t kDomainRegenLight = 4u, kDomainRegenEye = 5u;
+
+__global__ __launch_bounds__(kRegenLightN / kRegenLightDepth) void k_regen_light(DevScene sc, FrameArgs fa,
+                                                                                  LightVert* __restrict__ lv) {
+  __shared__ uint64_t lstk_all[CPT_LDS_STACK * kBlock];   // column layout: the stride is kBlock
+  LdsU64* lstk = lds_column(lstk_all);
+  const uint32_t x = threadIdx.x;
+  const uint64_t inc = ((uint64_t)x << 1) | 1u;
+  uint64_t st;
+  pcg32_seed(st, inc, stream_key(fa.seed, fa.frame0, kDomainRegenLight, 0u));
+  bool valid = false;
+  f3 o = mk3(0.f, 0.f, 0.f), d = o, irr = o;
+  const Surf none = {o, o, o, o, 0.f, 0.f, 0.f, 0.f};
+  for (uint32_t k = 0; k < kRegenLightDepth; k++) {
+    f3 mulT = mk3(1.f, 1.f, 1.f), no = o, nd = d;
+    bool nvalid = false;
+    LightVert& V = lv[x * kRegenLightDepth + k];
+    store_lv(V, o * 0.f, o * 0.f, o * 0.f, o * 0.f, none);
+    if (!valid || length(irr) < (float)kBiasEps) {
+      if (sc.n_light_tris > 0) {                         // a new path on an emitter (:94-158)
+        uint32_t id = (uint32_t)(uniform01(st, inc) * (float)sc.n_light_tris);
+        if (id >= sc.n_light_tris) id = sc.n_light_tris - 1;
+        float lw = uniform01(st, inc), lu = uniform01(st, inc);
+        if (lw + lu > 1.f) { lw = 1.f - lw; lu = 1.f - lu; }
+        const float lvv = (1.f - lw) - lu;
+        const Surf s = fetch_surf(sc, sc.light_tris[id], lw, lu, lvv, true);
+        f3 u, v;
+        basis(s.norm, u, v);
+        const float r1 = (float)(2.0 * kPiD * (double)uniform01(st, inc));
+        const float r2cos = sqrtf(uniform01(st, inc));
+        const float r2sin = 1.f - r2cos * r2cos;
+        nd = normalize((s.norm * r2cos + (u * r2sin) * cosf(r1)) + (v * r2sin) * sinf(r1));
+        no = s.pos + s.norm * (float)kBiasEps;
+        nvalid = true;
+        store_lv(V, s.pos, s.norm, s.emissive, mk3(0.f, 0.f, 0.f), s);
+      }
+    } else {
+      TraceHit h;
+      uint32_t nn = 0, nt = 0;
+      if (traverse4<false>(sc, o, d, kInfF, (float)kBiasEps, false, lstk, h, nn, nt)) {   // :159-299
+        const float w = (1.f - h.u) - h.v;
+        const Surf s = fetch_surf(sc, h.leaf, w, h.u, h.v, true);
+        const f3 nl = dot(s.norm, d) < 0.f ? s.norm : s.norm * -1.f;
+        const Bounce b = sample_bounce(st, inc, o, d, h.t, s, nl, RT_LIGHT, false, true);
+        no = b.orig;
+        nd = b.dir;
+        nvalid = true;
+        mulT = vmul(b.mul, irr);
+        store_lv(V, s.pos, s.norm, irr, d * -1.f, s);
+      }
+    }
+    valid = nvalid;
+    o = no;
+    d = nd;
+    irr = mulT;
+  }
+}
+
+template <int STK, bool Q>
+__global__ __launch_bounds__(kBlock) void k_regen_eye(DevScene sc, FrameArgs fa, const LightVert* __restrict__ lv,
+                                                      float* __restrict__ result, uint32_t* __restrict__ sample_n,
+                                                      DevStats* __restrict__ stats) {
+  __shared__ uint64_t lstk_all[STK * kBlock];
+  LdsU64* lstk = lds_column(lstk_all);
+  const uint32_t W = fa.width, H = fa.height, pix = blockIdx.x * kBlock + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  uint32_t ext = 0, shr = 0, paths = 0, en = 0, et = 0, sn = 0, stc = 0;
+  if (pix < W * H) {
+    const uint32_t levels = fa.eye_depth < kRegenMaxLevels ? fa.eye_depth : kRegenMaxLevels;
+    const uint32_t y = pix / W, x = pix - y * W;
+    const uint64_t inc = ((uint64_t)pix << 1) | 1u;
+    uint64_t st;
+    pcg32_seed(st, inc, stream_key(fa.seed, fa.frame0, kDomainRegenEye, 0u));
+    const f3 cpos = mk3(fa.cam_pos[0], fa.cam_pos[1], fa.cam_pos[2]);
+    const f3 cright = mk3(fa.cam_right[0], fa.cam_right[1], fa.cam_right[2]);
+    const f3 cup = mk3(fa.cam_up[0], fa.cam_up[1], fa.cam_up[2]);
+    const f3 cdir = mk3(fa.cam_dir[0], fa.cam_dir[1], fa.cam_dir[2]);
+    const float Wf = (float)W, Hf = (float)H, tf = tanf(fa.fov * 0.5f);
+    const float pu = ((((2.f * ((float)x + 0.5f)) / Wf - 1.f) * tf) * Wf) / Hf;   // pt0_kernel :646-647
+    const float pv = ((2.f * ((float)y + 0.5f)) / Hf - 1.f) * tf;
+    auto camera_ray = [&]() {
+      const float au = pu + ((uniform01(st, inc) - 0.5f) / Hf) * tf;
+      const float av = pv + ((uniform01(st, inc) - 0.5f) / Hf) * tf;
+      return normalize((cright * au + cup * av) + cdir);
+    };
+    f3 o = cpos, d = camera_ray();
+    float weight = 1.f;
+    uint32_t depth = 0, count = 0, D = levels;
+    f3 mul[kRegenMaxLevels], em[kRegenMaxLevels], con[kRegenMaxLevels];   // per level: f, emission, connection
+    uint32_t bank = 0;                                     // levels where a camera path starts
+    paths = 1;
+    for (uint32_t L = 0; L < levels; L++) {
+      TraceHit h;
+      ext++;
+      if (!traverse4<false, STK, Q>(sc, o, d, kInfF, (float)kBiasEps, false, lstk, h, en, et)) {   // :620-623
+        D = L;
+        break;
+      }
+      const float w = (1.f - h.u) - h.v;
+      const Surf s = fetch_surf(sc, h.leaf, w, h.u, h.v, true);
+      const f3 nl = dot(s.norm, d) < 0.f ? s.norm : s.norm * -1.f;
+      const Bounce b = sample_bounce(st, inc, o, d, h.t, s, nl, RT_EYE, false, true);
+      weight = weight * length(b.mul);
+      if (length(s.emissive) > 0.f) weight = 0.f;
+      if (depth == 0) bank |= 1u << L;
+      f3 c = mk3(0.f, 0.f, 0.f), no = b.orig, nd = b.dir;
+      if (!prob_rand(st, inc, weight)) {                 // a new camera path for the pixel (:562-571)
+        weight = 1.f;
+        depth = 0;
+        count++;
+        no = cpos;
+        nd = camera_ray();
+        paths++;
+      } else {
+        depth++;
+        if (b.type == RT_DIFF) {                          // GetLightFromRandLightVertices (:400-431)
+          const f3 P = s.pos + nl * (float)kBiasEps;
+          uint32_t li = (uint32_t)(uniform01(st, inc) * (float)kRegenLightN);
+          if (li >= kRegenLightN) li = kRegenLightN - 1;
+          const LightVert& LV = lv[li];
+          const f3 lpos = mk3(LV.pos[0], LV.pos[1], LV.pos[2]), lirr = mk3(LV.irrad[0], LV.irrad[1], LV.irrad[2]);
+          const f3 ldir = mk3(LV.irradDir[0], LV.irradDir[1], LV.irradDir[2]);
+          f3 irrad = mk3(0.f, 0.f, 0.f), irradDir = irrad;
+          const f3 toL = normalize(lpos - P);
+          const float dist = length(lpos - P);
+          if (length(lirr) > 0.f && dot(nl, toL) > 0.f) {
+            TraceHit hs;
+            shr++;
+            if (!traverse4<true, STK, Q>(sc, P, toL, (float)shadow_max((double)dist, fa.flags), (float)kBiasEps,
+                                         false, lstk, hs, sn, stc)) {
+              if (length(ldir) > (float)kFltEps)
+                irrad = shading(toL * -1.f, ldir, lirr, mk3(LV.norm[0], LV.norm[1], LV.norm[2]),
+                                mk3(LV.diff[0], LV.diff[1], LV.diff[2]), LV.metallic, LV.roughness, LV.specular,
+                                1.f, 1.f) +
+                        mk3(LV.emissive[0], LV.emissive[1], LV.emissive[2]);
+              else
+                irrad = lirr;
+              irradDir = toL;
+            }
+          }
+          c = vmax0(shading(d * -1.f, irradDir, irrad, nl, s.diff, s.metal, s.rough, s.spec, 1.f - s.trans, 1.f));
+          if (length(c) > 0.f) count += 4;
+        }
+      }
+      mul[L] = b.mul;
+      em[L] = s.emissive;
+      con[L] = c;
+      o = no;
+      d = nd;
+    }
+    count++;                                               // the miss or the level bound (:620-630)
+    f3 r = mk3(0.f, 0.f, 0.f), accum = r;
+    for (uint32_t k = D; k-- > 0;) {
+      r = (vmul(mul[k], r) + em[k]) + con[k];
+      if ((bank >> k) & 1u) {
+        accum = accum + r;
+        r = mk3(0.f, 0.f, 0.f);
+      }
+    }
+    const uint32_t nold = fa.frame0 == 0 ? 0u : sample_n[pix], nn = nold + count;
+    const float ri = 1.f / (float)nn, oi = (float)nold * ri;
+    float* res = result + (size_t)pix * 3;
+    res[0] = fmaxf(ri * accum.x + oi * res[0], 0.f);
+    res[1] = fmaxf(ri * accum.y + oi * res[1], 0.f);
+    res[2] = fmaxf(ri * accum.z + oi * res[2], 0.f);
+    sample_n[pix] = nn;
+  }
+  if (stats) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(&stats->frames, 1ull);
+    const uint32_t a0 = wave_sum(ext), a1 = wave_sum(shr), a2 = wave_sum(paths);
+    const uint32_t b0 = wave_sum(en), b1 = wave_sum(et), b2 = wave_sum(sn), b3 = wave_sum(stc);
+    if (lane == 0 && (a0 | a2)) {
+      atomicAdd(&stats->ext_rays, (unsigned long long)a0);
+      atomicAdd(&stats->shadow_rays, (unsigned long long)a1);
+      atomicAdd(&stats->eye_paths, (unsigned long long)a2);
+      atomicAdd(&stats->ext_nodes, (unsigned long long)b0);
+      atomicAdd(&stats->ext_tris, (unsigned long long)b1);
+      atomicAdd(&stats->sh_nodes, (unsigned long long)b2);
+      atomicAdd(&stats->sh_tris, (unsigned long long)b3);
+    }
+  }
+}
+
+hipError_t launch_regen_light(const DevScene& sc, const FrameArgs& fa, LightVert* lv, hipStream_t st) {
+  if (!lv || !sc.tris || (sc.n_light_tris && !sc.light_tris)) return hipErrorInvalidValue;
+  k_regen_light<<<1, kRegenLightN / kRegenLightDepth, 0, st>>>(sc, fa, lv);
+  return hipGetLastError();
+}
+hipError_t launch_regen_eye(const DevScene& sc, const FrameArgs& fa, const LightVert* lv, float* result,
+                            uint32_t* sample_n, int lds_stack, int quant, DevStats* stats, hipStream_t st) {
+  const uint64_t n = (uint64_t)fa.width * fa.height;
+  if (n == 0) return hipSuccess;
+  if (!lv || !result || !sample_n || n > 0xFFFFFFFFull - kBlock || fa.eye_depth < 1 ||
+      fa.eye_depth > kRegenMaxLevels || !sc.tris || !(quant ? (const void*)sc.nodes4q : (const void*)sc.nodes4))
+    return hipErrorInvalidValue;
+  const uint32_t grid = (uint32_t)((n + kBlock - 1) / kBlock);
+#define CPT_RG(S, W, Q) k_regen_eye<S, Q><<<grid, kBlock, 0, st>>>(sc, fa, lv, result, sample_n, stats)
+#define CPT_RG_Q(S, W) CPT_RG(S, W, true)
+#define CPT_RG_F(S, W) CPT_RG(S, W, false)
+  if (quant) {
+    CPT_VARIANT_SWITCH(lds_stack, CPT_RG_Q)
+  } else {
+    CPT_VARIANT_SWITCH(lds_stack, CPT_RG_F)
+  }
+#undef CPT_RG_Q
+#undef CPT_RG_F
+#undef CPT_RG
+  return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- megakernel
 // The reference's regeneration baseline (cudaRTBDPTRegen: one thread carries
 // its sample through every bounce, src/cudaRTBDPTRegen.cu) over this library's

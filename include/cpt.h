@@ -222,11 +222,15 @@ enum {
                                         over the same stream, eye depth 10, a per-frame running mean */
   CPT_MODE_PT_STREAM_REGEN = 5,      /* cudaRTPTStreamRegen (src/cudaRTPTStreamRegen.cu): as 4, and a pixel
                                         whose path ends while the frame is still tracing restarts once */
-  CPT_MODE_PT_DISNEY_MIS = 6         /* no reference counterpart (the reference loads the Disney parameters and
+  CPT_MODE_PT_DISNEY_MIS = 6,        /* no reference counterpart (the reference loads the Disney parameters and
                                         never reads them, src/cudaRTCommon.cu:398-405): a path tracer with
                                         Burley's Disney BRDF (isotropic, opaque), next-event estimation to the
                                         emissive triangles and power-heuristic MIS, Russian roulette from the
                                         fourth bounce, eye depth 8; one sample per path */
+  CPT_MODE_BDPT_REGEN = 7            /* cudaRTBDPTRegen (src/cudaRTBDPTRegen.cu), the "Path Regen" baseline: a
+                                        megakernel, one thread per pixel; a 640-vertex light cache every third
+                                        frame; camera paths restarted for the same pixel when Russian roulette
+                                        ends one, at most 5 levels (eye depth, <= 8) */
 };
 
 /* RTRenderer::Init(w, h) + SetRendererMode(mode) (src/raytracer.cpp:574-608). */

@@ -77,7 +77,9 @@ class RTRenderer {
     BDPT_STREAM_ADAP_BILAT_BINARY = CPT_MODE_BDPT_ADAPTIVE_BINARY,
     BDPT_STREAM_ADAP_EMP = CPT_MODE_BDPT_ADAPTIVE_EMP,
     PT_STREAM = CPT_MODE_PT_STREAM,               // DEFINE_RT(PTSC, cudaRTPTStream)
-    PT_STREAM_REGEN = CPT_MODE_PT_STREAM_REGEN    // DEFINE_RT(PTSCRegen, cudaRTPTStreamRegen), commented out there
+    PT_STREAM_REGEN = CPT_MODE_PT_STREAM_REGEN,   // DEFINE_RT(PTSCRegen, cudaRTPTStreamRegen), commented out there
+    PT_DISNEY_MIS = CPT_MODE_PT_DISNEY_MIS,       // no reference entry: Disney BRDF path tracer with MIS
+    BDPT_REGEN = CPT_MODE_BDPT_REGEN              // DEFINE_RT(BDPTRegen, cudaRTBDPTRegen)
   };
   RTRenderer() = default;
   ~RTRenderer() { release(); }

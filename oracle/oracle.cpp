@@ -690,7 +690,8 @@ struct RenderParams {
                    // 2 rounds with bisection (AdapBilatBinary), 3 empirical variance rounds (AdapEmp),
                    // 4 path tracing (cudaRTPTStream), 5 with regeneration (cudaRTPTStreamRegen),
                    // 6 as 5 through the reference's own loop (render_pt_literal),
-                   // 7 the Disney-BRDF path tracer with next-event estimation and MIS (extension)
+                   // 7 the Disney-BRDF path tracer with next-event estimation and MIS (extension),
+                   // 8 the megakernel BDPT with camera-path restarts (cudaRTBDPTRegen, "Path Regen")
   u32 nd;          // DesiredSamplingN
   float sc, sp, sn, sd;   // filter colour / position / normal / diffuse sigmas
   u32 radius;      // filter radius
@@ -1258,6 +1259,193 @@ static void render_disney(const Scene& s, const RenderParams& p, u32 frame0, u32
         disney_path<R>(s, p, cam, f, x, (u32)y, Lr, er, sr);
         const u32 pix = (u32)y * W + x;
         apply_sample(&fs.result[pix * 3], fs.N[pix], Lr, 1u);
+      }
+    fs.ext_rays += er; fs.shadow_rays += sr;
+  }
+}
+
+// ---------------------------------------------------------------- "Path Regen" BDPT (cudaRTBDPTRegen)
+// The reference's megakernel BDPT renderer (src/cudaRTBDPTRegen.cu), one
+// thread per pixel, restated without recursion:
+//  - every third frame a light pass fills a 640-vertex cache: 128 light paths
+//    of 5 vertices each (bdpt_light_kernel :314-330, bdpt_light_ray :80-309).
+//    A path starts on a random emissive triangle (its vertex: irrad =
+//    emissive, no direction) and then carries the throughput (1, 1, 1), not the
+//    emission (:179, :307); each later vertex stores the throughput it arrived
+//    with and the hit's interpolated normal; a miss stores an empty vertex and
+//    the next vertex starts a new path, as does a throughput below 1e-4;
+//  - the eye sample runs at most `eye_depth` levels of pt0_normalRay
+//    (:433-625).  Per level: the BRDF bounce (refraction past the critical
+//    angle reflects, :501-505), the pixel weight multiplied by |f|, set to 0
+//    on an emitter; Russian roulette on it (:562): a failed test restarts a
+//    camera path for the same pixel (count + 1), a passed one connects a
+//    diffuse bounce to a uniformly chosen cache vertex (count + 4 when it
+//    contributes).  A miss or the level bound adds 1 to the count.  Radiance
+//    is summed back up the levels, r = f * r' + emission + connection, and
+//    a level where a camera path starts banks r into the pixel's sum (:598-606).
+static const u32 kRegenLightN = 640, kRegenLightDepth = 5, kRegenMaxLevels = 8;
+enum { DOMAIN_REGEN_LIGHT = 4, DOMAIN_REGEN_EYE = 5 };
+
+template <class R> static LightVertex<R> zero_vertex() {
+  LightVertex<R> V;
+  V.pos = V.norm = V.irrad = V.irradDir = V.diff = V.emissive = mk((R)0, (R)0, (R)0);
+  V.specular = V.metallic = V.roughness = (R)0;
+  return V;
+}
+template <class R> static LightVertex<R> shade_vertex(const HitShade<R>& hs, V3<R> irrad, V3<R> irradDir) {
+  LightVertex<R> V;
+  V.pos = hs.pos; V.norm = hs.norm; V.irrad = irrad; V.irradDir = irradDir;
+  V.diff = hs.diff; V.emissive = hs.emissive; V.specular = hs.specular; V.metallic = hs.metallic;
+  V.roughness = hs.roughness;
+  return V;
+}
+
+template <class R>
+static void regen_light_pass(const Scene& s, const RenderParams& p, u32 frame, FrameState<R>& fs) {
+  fs.lv.assign(kRegenLightN, zero_vertex<R>());
+  fs.lvN = kRegenLightN;
+  const u32 nlt = (u32)s.lightTri.size();
+  for (u32 x = 0; x < kRegenLightN / kRegenLightDepth; x++) {
+    Pcg32 rng; pcg32_seed(rng, stream_key(p.seed, frame, DOMAIN_REGEN_LIGHT, 0), x);
+    bool valid = false;
+    V3<R> o = mk((R)0, (R)0, (R)0), d = o, irr = o;
+    for (u32 k = 0; k < kRegenLightDepth; k++) {
+      V3<R> mulT = mk((R)1, (R)1, (R)1), no = o, nd = d;
+      bool nvalid = false;
+      LightVertex<R> V = zero_vertex<R>();
+      if (!valid || length(irr) < (R)kBiasEps) {
+        if (nlt > 0) {                                   // a new path on an emitter (:94-158)
+          u32 id = (u32)(uniform<R>(rng) * (R)nlt); if (id >= nlt) id = nlt - 1;
+          R lw = uniform<R>(rng), lu = uniform<R>(rng);
+          if (lw + lu > (R)1) { lw = (R)1 - lw; lu = (R)1 - lu; }
+          R lvv = ((R)1 - lw) - lu;
+          HitShade<R> hs = fetch_hit<R>(s, s.lightTri[id], lw, lu, lvv);
+          V3<R> u, v; basis(hs.norm, u, v);
+          R r1 = (R)(2.0 * kPi * (double)uniform<R>(rng));
+          R r2cos = std::sqrt(uniform<R>(rng));
+          R r2sin = (R)1 - r2cos * r2cos;
+          nd = normalize((hs.norm * r2cos + (u * r2sin) * std::cos(r1)) + (v * r2sin) * std::sin(r1));
+          no = hs.pos + hs.norm * (R)kBiasEps;
+          nvalid = true;
+          V = shade_vertex(hs, hs.emissive, mk((R)0, (R)0, (R)0));
+        }
+      } else {
+        HitR<R> h;
+        if (trace_defined<R>(s, o, d, (R)kInf, (R)kBiasEps, false, h)) {   // :159-299
+          R w = ((R)1 - h.u) - h.v;
+          HitShade<R> hs = fetch_hit<R>(s, h.leaf, w, h.u, h.v);
+          V3<R> nl = dot(hs.norm, d) < (R)0 ? hs.norm : hs.norm * (R)-1;
+          Bounce<R> b = sample_bounce<R>(rng, o, d, h.t, hs, nl, RT_LIGHT, false, true);
+          no = b.orig; nd = b.dir; nvalid = true;
+          mulT = vmul(b.mul, irr);
+          V = shade_vertex(hs, irr, d * (R)-1);
+        }
+      }
+      fs.lv[x * kRegenLightDepth + k] = V;
+      valid = nvalid; o = no; d = nd; irr = mulT;
+    }
+  }
+}
+
+template <class R> struct RegenSample { V3<R> accum; u32 N; u32 ext_rays, shadow_rays, paths; };
+
+template <class R>
+static RegenSample<R> regen_eye(const Scene& s, const RenderParams& p, const Camera<R>& cam, const FrameState<R>& fs,
+                                u32 frame, u32 x, u32 y) {
+  const u32 W = p.width, H = p.height, pix = y * W + x;
+  const u32 levels = p.eye_depth < kRegenMaxLevels ? p.eye_depth : kRegenMaxLevels;
+  Pcg32 rng; pcg32_seed(rng, stream_key(p.seed, frame, DOMAIN_REGEN_EYE, 0), pix);
+  RegenSample<R> rs; rs.accum = mk((R)0, (R)0, (R)0); rs.N = 0; rs.ext_rays = rs.shadow_rays = 0; rs.paths = 1;
+  const R tf = std::tan(cam.fov * (R)0.5), Wf = (R)W, Hf = (R)H;
+  const R pu = ((((R)2 * ((R)x + (R)0.5)) / Wf - (R)1) * tf * Wf) / Hf;   // pt0_kernel :646-647
+  const R pv = (((R)2 * ((R)y + (R)0.5)) / Hf - (R)1) * tf;
+  auto camera_ray = [&]() {
+    R au = pu + ((uniform<R>(rng) - (R)0.5) / Hf) * tf;
+    R av = pv + ((uniform<R>(rng) - (R)0.5) / Hf) * tf;
+    return normalize((cam.right * au + cam.up * av) + cam.dir);
+  };
+  V3<R> o = cam.pos, d = camera_ray();
+  R weight = (R)1;
+  u32 depth = 0, count = 0, D = levels;
+  V3<R> mul[kRegenMaxLevels], em[kRegenMaxLevels], lc[kRegenMaxLevels];
+  bool bank[kRegenMaxLevels];
+  for (u32 L = 0; L < levels; L++) {
+    HitR<R> h;
+    rs.ext_rays++;
+    if (!trace_defined<R>(s, o, d, (R)kInf, (R)kBiasEps, false, h)) { D = L; break; }   // :620-623
+    R w = ((R)1 - h.u) - h.v;
+    HitShade<R> hs = fetch_hit<R>(s, h.leaf, w, h.u, h.v);
+    V3<R> nl = dot(hs.norm, d) < (R)0 ? hs.norm : hs.norm * (R)-1;
+    Bounce<R> b = sample_bounce<R>(rng, o, d, h.t, hs, nl, RT_EYE, false, true);
+    weight = weight * length(b.mul);
+    if (length(hs.emissive) > (R)0) weight = (R)0;
+    bank[L] = depth == 0;
+    V3<R> c = mk((R)0, (R)0, (R)0), no = b.orig, nd = b.dir;
+    if (!prob_rand(rng, weight)) {                       // a new camera path for the pixel (:562-571)
+      weight = (R)1; depth = 0; count++;
+      no = cam.pos; nd = camera_ray();
+      rs.paths++;
+    } else {
+      depth++;
+      if (b.type == RT_DIFF) {                            // GetLightFromRandLightVertices (:400-431)
+        V3<R> P = hs.pos + nl * (R)kBiasEps;
+        u32 li = (u32)(uniform<R>(rng) * (R)kRegenLightN); if (li >= kRegenLightN) li = kRegenLightN - 1;
+        const LightVertex<R>& LV = fs.lv[li];
+        V3<R> irrad = mk((R)0, (R)0, (R)0), irradDir = irrad;
+        V3<R> toL = normalize(LV.pos - P);
+        R dist = length(LV.pos - P);
+        if (length(LV.irrad) > (R)0 && dot(nl, toL) > (R)0) {
+          HitR<R> sh;
+          rs.shadow_rays++;
+          if (!trace_defined<R>(s, P, toL, (R)shadow_max((double)dist, p.flags), (R)kBiasEps, false, sh, true)) {
+            if (length(LV.irradDir) > (R)kFltEps)
+              irrad = shading(toL * (R)-1, LV.irradDir, LV.irrad, LV.norm, LV.diff, LV.metallic, LV.roughness,
+                              LV.specular, (R)1, (R)1) + LV.emissive;
+            else irrad = LV.irrad;
+            irradDir = toL;
+          }
+        }
+        c = vmax(mk((R)0, (R)0, (R)0), shading(d * (R)-1, irradDir, irrad, nl, hs.diff, hs.metallic, hs.roughness,
+                                               hs.specular, (R)1 - hs.trans, (R)1));
+        if (length(c) > (R)0) count += 4;
+      }
+    }
+    mul[L] = b.mul; em[L] = hs.emissive; lc[L] = c;
+    o = no; d = nd;
+  }
+  count++;                                                 // the miss or the level bound (:620-630)
+  V3<R> r = mk((R)0, (R)0, (R)0);
+  for (u32 k = D; k-- > 0;) {
+    r = (vmul(mul[k], r) + em[k]) + lc[k];
+    if (bank[k]) { rs.accum = rs.accum + r; r = mk((R)0, (R)0, (R)0); }
+  }
+  rs.N = count;
+  return rs;
+}
+
+// pt0_kernel's running mean (:655-664): the count is every camera path started
+// plus 4 per contributing connection, reset at frame 0.
+template <class R>
+static void render_bdpt_regen(const Scene& s, const RenderParams& p, u32 frame0, u32 nframes, FrameState<R>& fs) {
+  Camera<R> cam = make_camera<R>(p);
+  const u32 W = p.width, H = p.height;
+  for (u32 f = frame0; f < frame0 + nframes; f++) {
+    if (f % 3 == 0) regen_light_pass<R>(s, p, f, fs);
+    if (f == 0) std::fill(fs.N.begin(), fs.N.end(), 0u);
+    u64 er = 0, sr = 0;
+#pragma omp parallel for schedule(dynamic, 16) reduction(+ : er, sr)
+    for (i32 y = 0; y < (i32)H; y++)
+      for (u32 x = 0; x < W; x++) {
+        const RegenSample<R> rs = regen_eye<R>(s, p, cam, fs, f, x, (u32)y);
+        const u32 pix = (u32)y * W + x;
+        R* res = &fs.result[pix * 3];
+        const u32 nn = fs.N[pix] + rs.N;
+        const R ri = (R)1 / (R)nn, oi = (R)fs.N[pix] * ri;
+        res[0] = std::fmax(ri * rs.accum.x + oi * res[0], (R)0);
+        res[1] = std::fmax(ri * rs.accum.y + oi * res[1], (R)0);
+        res[2] = std::fmax(ri * rs.accum.z + oi * res[2], (R)0);
+        fs.N[pix] = nn;
+        er += rs.ext_rays; sr += rs.shadow_rays;
       }
     fs.ext_rays += er; fs.shadow_rays += sr;
   }
@@ -2026,7 +2214,8 @@ void orc_render(void* scene, void* rh, u32 frame0, u32 nframes, float* result, u
   size_t n = (size_t)r->p.width * r->p.height;
   if (r->prec == 64) {
     std::vector<double> g; if (gbuf) g.assign(n * 9, 0.0);
-    if (r->p.mode == 7) render_disney<double>(*s, r->p, frame0, nframes, r->f64);
+    if (r->p.mode == 8) render_bdpt_regen<double>(*s, r->p, frame0, nframes, r->f64);
+    else if (r->p.mode == 7) render_disney<double>(*s, r->p, frame0, nframes, r->f64);
     else if (r->p.mode == 6) render_pt_literal<double>(*s, r->p, frame0, nframes, r->f64);
     else if (r->p.mode >= 4) render_pt<double>(*s, r->p, frame0, nframes, r->f64);
     else if (r->p.mode != 0) render_adaptive<double>(*s, r->p, frame0, nframes, r->f64);
@@ -2037,7 +2226,8 @@ void orc_render(void* scene, void* rh, u32 frame0, u32 nframes, float* result, u
     if (lvN) *lvN = r->f64.lvN;
   } else {
     std::vector<float> g; if (gbuf) g.assign(n * 9, 0.f);
-    if (r->p.mode == 7) render_disney<float>(*s, r->p, frame0, nframes, r->f32);
+    if (r->p.mode == 8) render_bdpt_regen<float>(*s, r->p, frame0, nframes, r->f32);
+    else if (r->p.mode == 7) render_disney<float>(*s, r->p, frame0, nframes, r->f32);
     else if (r->p.mode == 6) render_pt_literal<float>(*s, r->p, frame0, nframes, r->f32);
     else if (r->p.mode >= 4) render_pt<float>(*s, r->p, frame0, nframes, r->f32);
     else if (r->p.mode != 0) render_adaptive<float>(*s, r->p, frame0, nframes, r->f32);

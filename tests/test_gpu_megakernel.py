@@ -1,6 +1,7 @@
-"""The megakernel schedule (one thread carries its sample through every bounce,
-the reference's regeneration baseline, src/cudaRTBDPTRegen.cu) against the
-wavefront passes.
+"""The megakernel schedule of the streaming estimator (one thread carries its
+sample through every bounce, as the reference's Path Regen renderer,
+src/cudaRTBDPTRegen.cu, schedules its own estimator) against the wavefront
+passes.
 
 Both run the same per-path functions in the same order, so the gate is
 equality: the same frames bit for bit, the same sample counts, and the same
