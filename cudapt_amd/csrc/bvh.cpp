@@ -428,6 +428,7 @@ bool quantize_bvh4(BvhHost& b) {
   const std::vector<Node4>& in = b.nodes4;
   std::vector<Node4Q>& out = b.nodes4q;
   out.assign(in.size(), Node4Q());
+  float qmag = 1e-30f;
   for (size_t i = 0; i < in.size(); i++) {
     const Node4& n = in[i];
     Node4Q& q = out[i];
@@ -449,6 +450,7 @@ bool quantize_bvh4(BvhHost& b) {
       if (e == 0) return false;
       const float step = ldexpf(1.f, (int)e - 127);
       q.o[a] = o;
+      qmag = std::max(qmag, std::max(std::fabs(o), std::fabs(q_decode(o, 255u, step))));
       exps |= e << (8 * a);
       for (int c = 0; c < 4; c++) {
         if (n.child[c] == kInvalid) { qlo[a][c] = 255; qhi[a][c] = 0; continue; }
@@ -470,6 +472,7 @@ bool quantize_bvh4(BvhHost& b) {
       q.child[c] = n.child[c];
     }
   }
+  b.qmag = qmag;
   return true;
 }
 

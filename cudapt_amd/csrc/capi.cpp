@@ -117,6 +117,7 @@ struct cpt_scene {
     s.env = d_env.p;
     s.env_w = env.empty() ? 0u : env_w;
     s.env_h = env.empty() ? 0u : env_h;
+    s.qmag = bvh.qmag;
     return s;
   }
   ~cpt_scene() {

@@ -60,6 +60,7 @@ struct BvhHost {
   std::vector<u32> dreorder;       // device leaf order -> original triangle id
   std::vector<Node4> nodes4;       // 4-wide re-layout (BFS order, root 0)
   std::vector<Node4Q> nodes4q;     // the same nodes quantized to 64 bytes
+  float qmag = 1e-30f;             // max |o + q * step| over nodes4q's planes, at least 1e-30
   u32 depth4 = 0, leaves4 = 0;
 };
 

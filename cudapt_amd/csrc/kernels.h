@@ -44,6 +44,7 @@ struct DevScene {
   // env_w = 0: none
   const float4* env;
   uint32_t env_w, env_h;
+  float qmag;                  // bound on |o + q s| over every slab plane of nodes4q (>= 1e-30)
 };
 
 // Per-path SoA state of the wavefront (one slot per live eye path).

@@ -105,50 +105,26 @@ __device__ __forceinline__ void cswap(uint64_t& a, uint64_t& b) {
   a = lo; b = hi;
 }
 
-// Child boxes and links of a 4-wide node, from the 128-byte FP32 layout or
-// decoded from the 64-byte quantized one (lo = o + q * step, the arithmetic
-// the host checked containment with).
+// Child boxes and links of a 4-wide node in the 128-byte FP32 layout (the
+// 64-byte quantized layout is tested straight from its words, box4_q).
 struct NodeView {
   float lx[4], hx[4], ly[4], hy[4], lz[4], hz[4];
   uint32_t ch[4], cn[4];
 };
-template <bool Q>
 __device__ __forceinline__ void load_node(const DevScene& sc, uint32_t node, NodeView& v) {
-  if (!Q) {
-    const Node4* N = sc.nodes4 + node;
-    const float4 lx = *(const float4*)N->lox, hx = *(const float4*)N->hix;
-    const float4 ly = *(const float4*)N->loy, hy = *(const float4*)N->hiy;
-    const float4 lz = *(const float4*)N->loz, hz = *(const float4*)N->hiz;
-    const uint4 ch = *(const uint4*)N->child, cn = *(const uint4*)N->count;
-    v.lx[0] = lx.x; v.lx[1] = lx.y; v.lx[2] = lx.z; v.lx[3] = lx.w;
-    v.hx[0] = hx.x; v.hx[1] = hx.y; v.hx[2] = hx.z; v.hx[3] = hx.w;
-    v.ly[0] = ly.x; v.ly[1] = ly.y; v.ly[2] = ly.z; v.ly[3] = ly.w;
-    v.hy[0] = hy.x; v.hy[1] = hy.y; v.hy[2] = hy.z; v.hy[3] = hy.w;
-    v.lz[0] = lz.x; v.lz[1] = lz.y; v.lz[2] = lz.z; v.lz[3] = lz.w;
-    v.hz[0] = hz.x; v.hz[1] = hz.y; v.hz[2] = hz.z; v.hz[3] = hz.w;
-    v.ch[0] = ch.x; v.ch[1] = ch.y; v.ch[2] = ch.z; v.ch[3] = ch.w;
-    v.cn[0] = cn.x; v.cn[1] = cn.y; v.cn[2] = cn.z; v.cn[3] = cn.w;
-  } else {
-    const uint4* N = (const uint4*)(sc.nodes4q + node);
-    const uint4 A = N[0], B = N[1], C = N[2], D = N[3];
-    const float ox = __uint_as_float(A.x), oy = __uint_as_float(A.y), oz = __uint_as_float(A.z);
-    const float sx = __uint_as_float((A.w & 0xFFu) << 23), sy = __uint_as_float(((A.w >> 8) & 0xFFu) << 23);
-    const float sz = __uint_as_float(((A.w >> 16) & 0xFFu) << 23);
-    // q * 2^k is exact for 8-bit q, so fma(q, step, o) rounds once, exactly
-    // as o + q * step does: the same bits as the host's containment check
-#pragma unroll
-    for (int c = 0; c < 4; c++) {
-      const int sh = 8 * c;
-      v.lx[c] = __builtin_fmaf((float)((B.x >> sh) & 0xFFu), sx, ox);
-      v.hx[c] = __builtin_fmaf((float)((B.y >> sh) & 0xFFu), sx, ox);
-      v.ly[c] = __builtin_fmaf((float)((B.z >> sh) & 0xFFu), sy, oy);
-      v.hy[c] = __builtin_fmaf((float)((B.w >> sh) & 0xFFu), sy, oy);
-      v.lz[c] = __builtin_fmaf((float)((C.x >> sh) & 0xFFu), sz, oz);
-      v.hz[c] = __builtin_fmaf((float)((C.y >> sh) & 0xFFu), sz, oz);
-    }
-    v.cn[0] = C.z & 0xFFFFu; v.cn[1] = C.z >> 16; v.cn[2] = C.w & 0xFFFFu; v.cn[3] = C.w >> 16;
-    v.ch[0] = D.x; v.ch[1] = D.y; v.ch[2] = D.z; v.ch[3] = D.w;
-  }
+  const Node4* N = sc.nodes4 + node;
+  const float4 lx = *(const float4*)N->lox, hx = *(const float4*)N->hix;
+  const float4 ly = *(const float4*)N->loy, hy = *(const float4*)N->hiy;
+  const float4 lz = *(const float4*)N->loz, hz = *(const float4*)N->hiz;
+  const uint4 ch = *(const uint4*)N->child, cn = *(const uint4*)N->count;
+  v.lx[0] = lx.x; v.lx[1] = lx.y; v.lx[2] = lx.z; v.lx[3] = lx.w;
+  v.hx[0] = hx.x; v.hx[1] = hx.y; v.hx[2] = hx.z; v.hx[3] = hx.w;
+  v.ly[0] = ly.x; v.ly[1] = ly.y; v.ly[2] = ly.z; v.ly[3] = ly.w;
+  v.hy[0] = hy.x; v.hy[1] = hy.y; v.hy[2] = hy.z; v.hy[3] = hy.w;
+  v.lz[0] = lz.x; v.lz[1] = lz.y; v.lz[2] = lz.z; v.lz[3] = lz.w;
+  v.hz[0] = hz.x; v.hz[1] = hz.y; v.hz[2] = hz.z; v.hz[3] = hz.w;
+  v.ch[0] = ch.x; v.ch[1] = ch.y; v.ch[2] = ch.z; v.ch[3] = ch.w;
+  v.cn[0] = cn.x; v.cn[1] = cn.y; v.cn[2] = cn.z; v.cn[3] = cn.w;
 }
 
 // Instantiates launch macro M(stack entries, min waves per SIMD) for the
@@ -177,23 +153,78 @@ __device__ __forceinline__ LdsU64* lds_column(uint64_t* shared_base) {
 // The state of one ray's traversal, advanced one node at a time by ray_step, so
 // that a lane whose ray has finished can start another while the rest of its
 // wave is still traversing (k_trace_dyn) or simply loop (traverse4).
+// pn, pf: -o * idir moved out by the quantized box test's slack, per axis.
 struct RayState {
-  f3 o, d, idir, o2;
+  f3 o, d, idir, o2, pn, pf;
   float eps;
   TraceHit h;
   uint32_t node;
   int sp;
 };
-__device__ __forceinline__ void ray_begin(RayState& R, f3 o, f3 d, float maxd, float eps) {
+// qmag bounds the magnitude of every quantized slab plane of the scene.
+__device__ __forceinline__ void ray_begin(RayState& R, f3 o, f3 d, float maxd, float eps, float qmag) {
   R.o = o;
   R.d = d;
   R.idir = mk3(1.f / escape_zero(d.x), 1.f / escape_zero(d.y), 1.f / escape_zero(d.z));
+  // Slack of the quantized box test (box4_q): 2^-20 = 16 u per unit of
+  // |idir| (qmag + |o|), twice the rounding bound of that test's distances.
+  const float k = 0x1p-20f;
+  const float sx = (fabsf(R.idir.x) * (qmag + fabsf(o.x))) * k;
+  const float sy = (fabsf(R.idir.y) * (qmag + fabsf(o.y))) * k;
+  const float sz = (fabsf(R.idir.z) * (qmag + fabsf(o.z))) * k;
+  const f3 oi = mk3(-(o.x * R.idir.x), -(o.y * R.idir.y), -(o.z * R.idir.z));
+  R.pn = mk3(oi.x - sx, oi.y - sy, oi.z - sz);
+  R.pf = mk3(oi.x + sx, oi.y + sy, oi.z + sz);
   R.o2 = o + d * eps;
   R.eps = eps;
   R.h.t = maxd; R.h.u = R.h.v = 0.f; R.h.leaf = kInvalid; R.h.orig = kInvalid;
   R.node = 0;
   R.sp = 0;
 }
+// The four child boxes of a 64-byte quantized node against the ray: entry and
+// exit distances tn, tf (entry clamped at 0), children and leaf counts.  A
+// slab plane is o_a + q s_a with s_a a power of two, so its distance
+// (o_a + q s_a - ray.o_a) idir_a is fma(q, s_a idir_a, c_a) with
+// c_a = fma(o_a, idir_a, -ray.o_a idir_a) per node (s_a idir_a is exact); the
+// near plane of each axis (by the sign of idir_a) gives the entry and the far
+// plane the exit, so no per-axis min / max is needed.  With M = sc.qmag,
+// u = 2^-24 and slack = 16 u |idir_a| (M + |ray.o_a|), the distance differs
+// from the exact one to the host's plane by at most six roundings (idir
+// against 1 / d, the plane's own, -ray.o_a idir_a, the slack's addition and
+// the two fmas), each within u (|idir_a| (M + |ray.o_a|) + slack): under half
+// the slack, which is subtracted from entries (R.pn) and added to exits
+// (R.pf), so no box that the ray meets in exact arithmetic is culled.
+__device__ __forceinline__ void box4_q(const DevScene& sc, const RayState& R, float tn[4], float tf[4],
+                                       uint32_t ch[4], uint32_t cn[4]) {
+  const uint4* N = (const uint4*)(sc.nodes4q + R.node);
+  const uint4 A = N[0], B = N[1], C = N[2], D = N[3];
+  const float sx = __uint_as_float((A.w & 0xFFu) << 23) * R.idir.x;
+  const float sy = __uint_as_float(((A.w >> 8) & 0xFFu) << 23) * R.idir.y;
+  const float sz = __uint_as_float(((A.w >> 16) & 0xFFu) << 23) * R.idir.z;
+  const float ox = __uint_as_float(A.x), oy = __uint_as_float(A.y), oz = __uint_as_float(A.z);
+  const float nx = __builtin_fmaf(ox, R.idir.x, R.pn.x), fx = __builtin_fmaf(ox, R.idir.x, R.pf.x);
+  const float ny = __builtin_fmaf(oy, R.idir.y, R.pn.y), fy = __builtin_fmaf(oy, R.idir.y, R.pf.y);
+  const float nz = __builtin_fmaf(oz, R.idir.z, R.pn.z), fz = __builtin_fmaf(oz, R.idir.z, R.pf.z);
+  const bool px = R.idir.x >= 0.f, py = R.idir.y >= 0.f, pz = R.idir.z >= 0.f;
+  const uint32_t qnx = px ? B.x : B.y, qfx = px ? B.y : B.x;
+  const uint32_t qny = py ? B.z : B.w, qfy = py ? B.w : B.z;
+  const uint32_t qnz = pz ? C.x : C.y, qfz = pz ? C.y : C.x;
+#pragma unroll
+  for (int c = 0; c < 4; c++) {
+    const int sh = 8 * c;
+    const float ex = __builtin_fmaf((float)((qnx >> sh) & 0xFFu), sx, nx);
+    const float ey = __builtin_fmaf((float)((qny >> sh) & 0xFFu), sy, ny);
+    const float ez = __builtin_fmaf((float)((qnz >> sh) & 0xFFu), sz, nz);
+    const float lx = __builtin_fmaf((float)((qfx >> sh) & 0xFFu), sx, fx);
+    const float ly = __builtin_fmaf((float)((qfy >> sh) & 0xFFu), sy, fy);
+    const float lz = __builtin_fmaf((float)((qfz >> sh) & 0xFFu), sz, fz);
+    tn[c] = fmaxf(fmaxf(ex, ey), fmaxf(ez, 0.f));
+    tf[c] = fminf(fminf(lx, ly), lz);
+  }
+  cn[0] = C.z & 0xFFFFu; cn[1] = C.z >> 16; cn[2] = C.w & 0xFFFFu; cn[3] = C.w >> 16;
+  ch[0] = D.x; ch[1] = D.y; ch[2] = D.z; ch[3] = D.w;
+}
+
 // One node: the four child boxes against the ray (culled beyond the best hit
 // plus slack); the leaf children hit are then tested as one run of triangles
 // (child order, triangle order), so that lanes whose leaves sit under different
@@ -209,24 +240,36 @@ __device__ __forceinline__ void ray_begin(RayState& R, f3 o, f3 d, float maxd, f
 template <bool ANY, int STK, bool Q>
 __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool cull, LdsU64* lstk, uint64_t* pstk,
                                          uint32_t& nn, uint32_t& nt, uint32_t* spill) {
-  NodeView nv;
-  load_node<Q>(sc, R.node, nv);
+  float tn[4], tf[4];
+  uint32_t ch[4], cn[4];
+  if constexpr (Q) {
+    box4_q(sc, R, tn, tf, ch, cn);
+  } else {
+    NodeView nv;
+    load_node(sc, R.node, nv);
+#pragma unroll
+    for (int c = 0; c < 4; c++) {
+      float t0x = (nv.lx[c] - R.o.x) * R.idir.x, t1x = (nv.hx[c] - R.o.x) * R.idir.x;
+      float t0y = (nv.ly[c] - R.o.y) * R.idir.y, t1y = (nv.hy[c] - R.o.y) * R.idir.y;
+      float t0z = (nv.lz[c] - R.o.z) * R.idir.z, t1z = (nv.hz[c] - R.o.z) * R.idir.z;
+      tn[c] = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
+      tf[c] = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * 1.0000004f;
+      ch[c] = nv.ch[c];
+      cn[c] = nv.cn[c];
+    }
+  }
   nn++;
   const float lim = R.h.t + cull_slack(R.h.t, R.eps);
   uint64_t key[4];
   uint32_t lf[4], ln[4];
 #pragma unroll
   for (int c = 0; c < 4; c++) {
-    float t0x = (nv.lx[c] - R.o.x) * R.idir.x, t1x = (nv.hx[c] - R.o.x) * R.idir.x;
-    float t0y = (nv.ly[c] - R.o.y) * R.idir.y, t1y = (nv.hy[c] - R.o.y) * R.idir.y;
-    float t0z = (nv.lz[c] - R.o.z) * R.idir.z, t1z = (nv.hz[c] - R.o.z) * R.idir.z;
-    float tnear = fmaxf(fmaxf(fminf(t0x, t1x), fminf(t0y, t1y)), fmaxf(fminf(t0z, t1z), 0.f));
-    float tfar = fminf(fminf(fmaxf(t0x, t1x), fmaxf(t0y, t1y)), fmaxf(t0z, t1z)) * 1.0000004f;
-    bool hit = nv.ch[c] != kInvalid && tnear <= tfar && tnear <= lim;
-    const bool leaf = nv.cn[c] > 0;
-    lf[c] = nv.ch[c];
-    ln[c] = hit && leaf ? nv.cn[c] : 0u;
-    key[c] = hit && !leaf ? ((uint64_t)__float_as_uint(tnear) << 32) | nv.ch[c] : ~0ull;
+    const float tnear = tn[c];
+    bool hit = ch[c] != kInvalid && tnear <= tf[c] && tnear <= lim;
+    const bool leaf = cn[c] > 0;
+    lf[c] = ch[c];
+    ln[c] = hit && leaf ? cn[c] : 0u;
+    key[c] = hit && !leaf ? ((uint64_t)__float_as_uint(tnear) << 32) | ch[c] : ~0ull;
   }
   // run position k lies in child c when p_c <= k < p_(c+1) (prefix sums of the
   // counts); its triangle is k + (first_c - p_c): selects, no branches
@@ -315,7 +358,7 @@ template <bool ANY, int STK = CPT_LDS_STACK, bool Q = false>
 __device__ bool traverse4(const DevScene& sc, f3 o, f3 d, float maxd, float eps, bool cull, LdsU64* lstk,
                           TraceHit& h, uint32_t& nn, uint32_t& nt) {
   RayState R;
-  ray_begin(R, o, d, maxd, eps);
+  ray_begin(R, o, d, maxd, eps, sc.qmag);
   uint64_t pstk[CPT_STACK_TOTAL - STK];
   while (!ray_step<ANY, STK, Q>(sc, R, cull, lstk, pstk, nn, nt, nullptr)) {
   }
@@ -1306,10 +1349,10 @@ __global__ __launch_bounds__(kBlock, MINW) void k_trace_dyn(DevScene sc, const f
           if (EYE) {
             uint32_t dk;
             const PathState e = eye_slot(fa, ray, dk);
-            ray_begin(R, ld3(e.ro), ld3(e.rd), e.ro.w, e.rd.w);
+            ray_begin(R, ld3(e.ro), ld3(e.rd), e.ro.w, e.rd.w, sc.qmag);
           } else {
             const float4 a = ro[ray], b4 = rd[ray];
-            ray_begin(R, ld3(a), ld3(b4), a.w, b4.w);
+            ray_begin(R, ld3(a), ld3(b4), a.w, b4.w, sc.qmag);
           }
           active = true;
         }
