@@ -269,7 +269,8 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool c
     const bool leaf = cn[c] > 0;
     lf[c] = ch[c];
     ln[c] = hit && leaf ? cn[c] : 0u;
-    key[c] = hit && !leaf ? ((uint64_t)__float_as_uint(tnear) << 32) | ch[c] : ~0ull;
+    // 0 marks no interior child (a child link is never the root, 0)
+    key[c] = hit && !leaf ? ((uint64_t)__float_as_uint(tnear) << 32) | ch[c] : 0ull;
   }
   // run position k lies in child c when p_c <= k < p_(c+1) (prefix sums of the
   // counts); its triangle is k + (first_c - p_c): selects, no branches
@@ -320,24 +321,41 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool c
     const float lim2 = R.h.t + cull_slack(R.h.t, R.eps);
 #pragma unroll
     for (int q = 0; q < 4; q++)
-      if (key[q] != ~0ull && __uint_as_float((uint32_t)(key[q] >> 32)) > lim2) key[q] = ~0ull;
+      if (__uint_as_float((uint32_t)(key[q] >> 32)) > lim2) key[q] = 0ull;
   }
   // Near-first for both ray kinds: occlusion rays taken in node order visit
   // 15 % fewer nodes but test 23 % more triangles and run 1 % slower on the
-  // sphere grid.
+  // sphere grid.  Sorted ascending, the n interior children hit are the last n
+  // keys: key[4 - n] is taken next and key[3], key[2], key[1] (those of them
+  // past it) are pushed farthest first.  Slots from sp on are free, so when
+  // all three fit the LDS top they are written unconditionally and sp moves
+  // by n - 1.
   cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[1], key[2]);
-  if (key[0] != ~0ull) {
-    R.node = (uint32_t)key[0];
+  const int n = ((uint32_t)key[0] != 0u) + ((uint32_t)key[1] != 0u) + ((uint32_t)key[2] != 0u) +
+                ((uint32_t)key[3] != 0u);
+  if (n) {
+    uint32_t next = (uint32_t)key[3];
+    next = n >= 2 ? (uint32_t)key[2] : next;
+    next = n >= 3 ? (uint32_t)key[1] : next;
+    next = n >= 4 ? (uint32_t)key[0] : next;
+    R.node = next;
+    if (R.sp + 3 <= STK) {
+      lstk[R.sp * kBlock] = key[3];
+      lstk[(R.sp + 1) * kBlock] = key[2];
+      lstk[(R.sp + 2) * kBlock] = key[1];
+      R.sp += n - 1;
+    } else {
 #pragma unroll
-    for (int c = 3; c >= 1; c--) {
-      if (key[c] != ~0ull && R.sp < CPT_STACK_TOTAL) {
-        if (R.sp < STK) {
-          lstk[R.sp * kBlock] = key[c];
-        } else {
-          pstk[R.sp - STK] = key[c];
-          if (spill) atomicAdd(spill, 1u);
+      for (int c = 3; c >= 1; c--) {
+        if (c + n > 4 && R.sp < CPT_STACK_TOTAL) {
+          if (R.sp < STK) {
+            lstk[R.sp * kBlock] = key[c];
+          } else {
+            pstk[R.sp - STK] = key[c];
+            if (spill) atomicAdd(spill, 1u);
+          }
+          R.sp++;
         }
-        R.sp++;
       }
     }
     return false;
@@ -2524,6 +2542,11 @@ __device__ __forceinline__ float sqdist3(const float* a, const float* b) {
   float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
   return (dx * dx + dy * dy) + dz * dz;
 }
+// |a - b|^2 of a filter range term: fma(dz, dz, fma(dy, dy, dx * dx)).
+__device__ __forceinline__ float sqd_fma(const float* a, const float* b) {
+  const float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return __builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx));
+}
 // Coordinate as the reference clamps it: (x + j) is unsigned, so a negative
 // coordinate wraps and lands on the high edge (cudaBilateralFilter.cu:85).
 __device__ __forceinline__ uint32_t ref_clamp_i(int v, uint32_t n) {
@@ -2536,6 +2559,9 @@ constexpr int kLdsRadius = 9;     // LDS tile path up to this radius, global rea
 // Cross-bilateral over colour/position/normal(/albedo) + the weighted squared
 // residual of pt_calculateSquareError_kernel, one 16x16 tile per workgroup.
 // The (16+2r)^2 input halo is staged in LDS once; every tap then reads LDS.
+// One exponential per tap: the range terms' exponents are summed, each scaled
+// by its 1 / 2 sigma^2 (correctly rounded once per thread) and accumulated
+// with fmas, so a tap takes no division.
 template <bool DIFF, bool LDS>
 __global__ __launch_bounds__(kTile * kTile) void k_filter_error(FilterArgs fa, const float* __restrict__ color,
                                                                 const float* __restrict__ gpos,
@@ -2585,6 +2611,9 @@ __global__ __launch_bounds__(kTile * kTile) void k_filter_error(FilterArgs fa, c
     cd[c] = DIFF ? gdif[ind * 3 + c] : 0.f;
   }
   const float zc = depth ? sqrtf(sqdist3(cp, cam)) : 0.f;
+  const float izc = zc > 0.f ? 1.f / zc : 0.f;
+  const float ikc = 1.f / fa.kc, ikp = 1.f / fa.kp, ikn = 1.f / fa.kn, ikd = 1.f / fa.kd;
+  const float ikz = depth ? 1.f / fa.kz : 0.f;
   float sum = 0.f, t0 = 0.f, t1 = 0.f, t2 = 0.f;
   for (int i = -R; i <= R; i++) {
     for (int j = -R; j <= R; j++) {
@@ -2601,17 +2630,18 @@ __global__ __launch_bounds__(kTile * kTile) void k_filter_error(FilterArgs fa, c
         if (DIFF) qd = gdif + g;
         if (depth) zq = sqrtf(sqdist3(qp, cam));
       }
-      float a = sqdist3(qc, cc) / fa.kc + sqdist3(qp, cp) / fa.kp;
-      a = a + sqdist3(qn, cn) / fa.kn;
-      if (DIFF) a = a + sqdist3(qd, cd) / fa.kd;
+      float a = sqd_fma(qc, cc) * ikc;
+      a = __builtin_fmaf(sqd_fma(qp, cp), ikp, a);
+      a = __builtin_fmaf(sqd_fma(qn, cn), ikn, a);
+      if (DIFF) a = __builtin_fmaf(sqd_fma(qd, cd), ikd, a);
       if (depth) {
-        const float dz = zc > 0.f ? (zq - zc) / zc : 0.f;
-        a = a + (dz * dz) / fa.kz;
+        const float dz = (zq - zc) * izc;
+        a = __builtin_fmaf(dz * dz, ikz, a);
       }
       float f = (sg[i + R] * sg[j + R]) * cpt_expf(-a);
-      t0 = t0 + f * qc[0];
-      t1 = t1 + f * qc[1];
-      t2 = t2 + f * qc[2];
+      t0 = __builtin_fmaf(f, qc[0], t0);
+      t1 = __builtin_fmaf(f, qc[1], t1);
+      t2 = __builtin_fmaf(f, qc[2], t2);
       sum = sum + f;
     }
   }

@@ -1644,16 +1644,26 @@ static inline float sqdist(const float* a, const float* b) {
   float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
   return (dx * dx + dy * dy) + dz * dz;
 }
+// |a - b|^2 of a range term, as the filter kernel evaluates it:
+// fma(dz, dz, fma(dy, dy, dx * dx)).
+static inline float sqd_fma(const float* a, const float* b) {
+  const float dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+  return std::fma(dz, dz, std::fma(dy, dy, dx * dx));
+}
 // Cross-bilateral with one exponential per tap:
 // w = (G[i] G[j]) * exp(-(|dc|^2/2sc^2 + |dp|^2/2sp^2 + |dn|^2/2sn^2 [+ |dd|^2/2sd^2])),
 // the product of the reference's per-term exponentials (cudaBilateralFilter.cu:89-92)
 // folded into one; FP64 runs keep the literal product (bilateral<double>).
 // The optional depth term adds ((z_q - z_p) / z_p)^2 / 2sz^2 last, z the
 // correctly rounded distance of the first hit from the camera.
+// The arithmetic is the device kernel's: each 1 / 2s^2 and 1 / z_p correctly
+// rounded once, the exponent accumulated with fmas in the order above
+// (c * (1/2sc^2), then fma per further term), the colour sums with fmas.
 static void bilateral_defined(const float* color, const float* pos, const float* norm, const float* emit, u32 W, u32 H,
                               float sc, float sp, float sn, float se, u32 radius, const std::vector<float>& g,
                               float* out, float sz = 0.f, const float* cam = nullptr) {
   const float kc = 2.f * sc * sc, kp = 2.f * sp * sp, kn = 2.f * sn * sn, ke = 2.f * se * se, kz = 2.f * sz * sz;
+  const float ikc = 1.f / kc, ikp = 1.f / kp, ikn = 1.f / kn, ike = 1.f / ke, ikz = kz > 0.f ? 1.f / kz : 0.f;
 #pragma omp parallel for schedule(static)
   for (i32 yy = 0; yy < (i32)H; yy++) {
     u32 y = (u32)yy;
@@ -1661,19 +1671,23 @@ static void bilateral_defined(const float* color, const float* pos, const float*
       u32 ind = x + y * W;
       float sum = 0.f, t0 = 0.f, t1 = 0.f, t2 = 0.f;
       const float zc = kz > 0.f ? std::sqrt(sqdist(&pos[ind * 3], cam)) : 0.f;
+      const float izc = zc > 0.f ? 1.f / zc : 0.f;
       for (int i = -(int)radius; i <= (int)radius; i++) {
         for (int j = -(int)radius; j <= (int)radius; j++) {
           u32 c = ref_clamp(x, j, W) + ref_clamp(y, i, H) * W;
-          float a = sqdist(&color[c * 3], &color[ind * 3]) / kc + sqdist(&pos[c * 3], &pos[ind * 3]) / kp;
-          a = a + sqdist(&norm[c * 3], &norm[ind * 3]) / kn;
-          if (emit) a = a + sqdist(&emit[c * 3], &emit[ind * 3]) / ke;
+          float a = sqd_fma(&color[c * 3], &color[ind * 3]) * ikc;
+          a = std::fma(sqd_fma(&pos[c * 3], &pos[ind * 3]), ikp, a);
+          a = std::fma(sqd_fma(&norm[c * 3], &norm[ind * 3]), ikn, a);
+          if (emit) a = std::fma(sqd_fma(&emit[c * 3], &emit[ind * 3]), ike, a);
           if (kz > 0.f) {
             const float zq = std::sqrt(sqdist(&pos[c * 3], cam));
-            const float dz = zc > 0.f ? (zq - zc) / zc : 0.f;
-            a = a + (dz * dz) / kz;
+            const float dz = (zq - zc) * izc;
+            a = std::fma(dz * dz, ikz, a);
           }
           float f = (g[i + radius] * g[j + radius]) * cpt_expf(-a);
-          t0 = t0 + f * color[c * 3]; t1 = t1 + f * color[c * 3 + 1]; t2 = t2 + f * color[c * 3 + 2];
+          t0 = std::fma(f, color[c * 3], t0);
+          t1 = std::fma(f, color[c * 3 + 1], t1);
+          t2 = std::fma(f, color[c * 3 + 2], t2);
           sum = sum + f;
         }
       }
