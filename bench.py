@@ -220,6 +220,7 @@ def adaptive_leg(ca, args):
     log(f"adaptive leg: scene in {time.perf_counter() - t0:.1f} s")
     R = ca.Renderer(W, H, mode=ca.CPT_MODE_BDPT_ADAPTIVE_PLL, eye_depth=EYE_DEPTH, light_depth=LIGHT_DEPTH,
                     **scenes.ADAPTIVE_C3)
+    R.set_attrib("Tail Bounce", args.adaptive_tail)
     cam = sc["camera"]
     R.render(S, dict(cam, pos=tuple(np.asarray(cam["pos"]) + 1e-3)), frames=8, result=False)   # allocations
     R.stats(reset=True)
@@ -235,7 +236,7 @@ def adaptive_leg(ca, args):
     res = {"workload": f"heightfield 1,050,976 tris + HDR environment, {W}x{H}, {F} spp average, proportional "
                        "allocation per frame, filter radius 7, sigmas colour/normal/depth 0.2/0.1/0.05, "
                        "4..256 paths per pixel",
-           "seconds": round(wall, 4), "frames": F,
+           "seconds": round(wall, 4), "frames": F, "tail_bounce": args.adaptive_tail,
            "mrays_per_s": round((st["extension_rays"] + st["shadow_rays"]) / wall / 1e6, 2),
            "host_readback": "none in the timed region (the running mean stays in HBM)",
            "samples_per_s": round(st["eye_paths"] / wall, 1),
@@ -244,8 +245,8 @@ def adaptive_leg(ca, args):
            "filter_ms_per_frame": round(st["filter_ms"] / max(filt_frames, 1), 4),
            "filter_gpix_per_s": round(W * H * filt_frames / (st["filter_ms"] / 1e3) / 1e9, 3)
            if st["filter_ms"] > 0 else None,
-           "phase_ms": {k: round(st[k], 3) for k in ("extend_ms", "shade_ms", "shadow_ms", "filter_ms", "error_ms",
-                                                   "gen_ms", "accum_ms", "total_ms")}}
+           "phase_ms": {k: round(st[k], 3) for k in ("extend_ms", "shade_ms", "shadow_ms", "tail_ms", "filter_ms",
+                                                   "error_ms", "gen_ms", "accum_ms", "total_ms")}}
     del R, S
     return res
 
@@ -497,6 +498,8 @@ def main():
     ap.add_argument("--multiview-only", action="store_true",
                     help="only the multi-view leg, plain and adaptive without a model (for profiling)")
     ap.add_argument("--adaptive-only", action="store_true", help="only the adaptive leg (for profiling)")
+    ap.add_argument("--adaptive-tail", type=int, default=2,
+                    help="Tail Bounce of the adaptive leg: bounces t.. of its adaptive frames in one launch (0: off)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the library renders on one "

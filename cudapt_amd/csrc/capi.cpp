@@ -992,9 +992,11 @@ int bounce_loop(cpt_renderer* r, const DevScene& sc, const FrameArgs& fa, u32 ma
   if (gen_eye && !eye0) HIP_TRY(launch_gen_eye(fa, r->A.view(), r->stream));
   bool shadow_pending = false;
   // "Tail Bounce" t > 0: bounces t.. run as one k_tail launch (plain BDPT's
-  // frames, whose accumulation goes through the sample records)
+  // frames and the adaptive frames after the warm-up, whose accumulation goes
+  // through the sample records; the first-hit G-buffer is written at bounce 0,
+  // before any tail)
   const double tail_attr = r->attr(kTailBounce);
-  const u32 tail = tail_attr >= 1.0 && tail_attr < (double)fa.eye_depth && ac.apply == 2 && !(ac.variant & kVariantGbuf) &&
+  const u32 tail = tail_attr >= 1.0 && tail_attr < (double)fa.eye_depth && ac.apply == 2 &&
                            !(ac.variant & (kVariantPT | kVariantDisney))
                        ? (u32)tail_attr
                        : 0u;
@@ -1618,6 +1620,10 @@ int cpt_renderer_create(cpt_renderer** out, int mode, uint32_t width, uint32_t h
     }
     r->attribs.push_back({kPassFrames, 0, 1});
     r->attribs.push_back({kLearnedSignal, 0, 0});
+    // a pass traces one frame's paths, most of which end within two bounces:
+    // from bounce 2 on one k_tail launch (90 ms against 102 ms for the 32-frame
+    // 1920 x 1080 adaptive workload on the environment-lit heightfield)
+    r->attribs.push_back({kTailBounce, 0, 2});
   } else {
     r->attribs.push_back({kInFlight, 0, is_bdpt_regen(mode) ? 1.0 : 24.0});
     if (mode == CPT_MODE_BDPT_STREAM) {

@@ -2479,7 +2479,9 @@ __global__ __launch_bounds__(kBlock) void k_tail(DevScene sc, FrameArgs fa, Path
 hipError_t launch_tail(const DevScene& sc, const FrameArgs& fa, PathBuf cur, FrameCounters* fc, uint32_t bounce0,
                        uint32_t grid, const LightVert* lv, const Accum& ac, int lds_stack, int quant,
                        DevStats* stats, hipStream_t st) {
-  if (!fc || !lv || !ac.rec.acc || ac.apply != 2 || (ac.variant & kVariantGbuf) || fa.eye_depth > 16 ||
+  // bounce0 >= 1: no path of the tail is at its first hit, so a G-buffer
+  // variant (written at depth 0 only) is allowed
+  if (!fc || !lv || !ac.rec.acc || ac.apply != 2 || fa.eye_depth > 16 ||
       bounce0 == 0 || bounce0 >= fa.eye_depth || grid == 0 || !pathbuf_ok(cur) || !sc.tris ||
       !(quant ? (const void*)sc.nodes4q : (const void*)sc.nodes4))
     return hipErrorInvalidValue;

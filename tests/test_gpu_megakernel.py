@@ -122,3 +122,41 @@ def test_tail_bounce_on_a_refilling_batch(cpt, scene_objs):
     B2.render(S, 8)
     for v in range(2):
         assert np.array_equal(B1.result(v).view(np.uint32), B2.result(v).view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", ["pll", "binary", "emp"])
+@pytest.mark.parametrize("tail", [1, 3])
+def test_tail_bounce_in_adaptive_frames(cpt, scene_objs, mode, tail):
+    """The adaptive frames after the warm-up (sample records, first-hit
+    G-buffer at bounce 0) with their last bounces in one k_tail launch: image,
+    sample counts, per-pixel paths and error map bit-identical."""
+    m = {"pll": cpt.CPT_MODE_BDPT_ADAPTIVE_PLL, "binary": cpt.CPT_MODE_BDPT_ADAPTIVE_BINARY,
+         "emp": cpt.CPT_MODE_BDPT_ADAPTIVE_EMP}[mode]
+    sc, S = scene_objs["textured_glass"]
+    out = []
+    for t in (0, tail):
+        R = cpt.Renderer(64, 48, mode=m, eye_depth=8)
+        R.set_attrib("Tail Bounce", t)
+        R.render(S, sc["camera"], frames=9, result=False)
+        out.append((R.read(cpt.CPT_BUF_RESULT), R.sample_counts(), R.read(cpt.CPT_BUF_PIXEL_PATHS),
+                    R.read(cpt.CPT_BUF_ERROR), R.stats()))
+    (ia, na, pa, ea, sa), (ib, nb, pb, eb, sb) = out
+    assert np.array_equal(ia.view(np.uint32), ib.view(np.uint32))
+    assert np.array_equal(na, nb) and np.array_equal(pa, pb)
+    assert np.array_equal(ea.view(np.uint32), eb.view(np.uint32))
+    assert sa["extension_rays"] == sb["extension_rays"] and sb["tail_rays"] > 0
+
+
+def test_tail_bounce_in_adaptive_environment_frames(cpt):
+    """The adaptive workload's scene (heightfield under its environment map) at
+    a small size: the tail keeps every adaptive frame bit for bit."""
+    from cudapt_amd import scenes
+    sc = scenes.heightfield(light="env")
+    S = cpt.Scene(sc)
+    cam = scenes.heightfield_poses(1, resolutions=((96, 64),))[0]
+    imgs = []
+    for t in (0, 2):
+        R = cpt.Renderer(96, 64, mode=cpt.CPT_MODE_BDPT_ADAPTIVE_PLL, eye_depth=8, **scenes.ADAPTIVE_C3)
+        R.set_attrib("Tail Bounce", t)
+        imgs.append(R.render(S, cam, frames=10))
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
