@@ -1502,6 +1502,7 @@ int render_frames(cpt_renderer* r, cpt_scene* s, const float* pos, const float* 
   for (int k = 0; k < 3; k++) { fa.cam_pos[k] = pos[k]; fa.cam_dir[k] = dir[k]; }
   camera_basis(dir, up, fa.cam_right, fa.cam_up);
   fa.fov = fov;
+  fa.tan_half = std::tan(fov * 0.5f);             // the oracle's tanf of the same float
   fa.eye_depth = eye_depth;
   fa.light_depth = light_depth;
   fa.light_paths = light_paths;
@@ -2292,6 +2293,7 @@ int cpt_batch_create(cpt_batch** out, const cpt_view* views, uint32_t n_views) {
     for (int k = 0; k < 3; k++) { d.cam_pos[k] = V.cam_pos[k]; d.cam_dir[k] = V.cam_dir[k]; }
     camera_basis(V.cam_dir, V.cam_up, d.cam_right, d.cam_up);
     d.fov = V.fov;
+    d.tan_half = std::tan(V.fov * 0.5f);
     b->vd.push_back(d);
     total += (uint64_t)V.width * V.height;
   }

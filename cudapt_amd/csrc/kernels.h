@@ -89,13 +89,14 @@ static_assert(sizeof(LightVert) == 96, "LightVert is 96 bytes");
 struct ViewDesc {
   uint32_t offset, width, height, pad;
   float cam_pos[3], cam_dir[3], cam_up[3], cam_right[3];
-  float fov, pad2[3];
+  float fov, tan_half, pad2[2];   // tan_half = tan(fov / 2), computed once on the host
 };
 
 struct FrameArgs {
   uint32_t width, height;
   float cam_pos[3], cam_dir[3], cam_up[3], cam_right[3];
   float fov;
+  float tan_half;        // tan(fov / 2), computed once on the host
   uint32_t eye_depth, light_depth, light_paths;
   uint32_t flags;        // bit 0: literal shadow endpoint dist - eps
   uint64_t seed;         // stream seed: per-path keys are derived on the device

@@ -1048,18 +1048,17 @@ struct ShadowRay {
 __device__ __forceinline__ PathState eye_path_at(const FrameArgs& fa, uint32_t pix, uint32_t sid, uint64_t st) {
   uint32_t lp = pix, vw = fa.width, vh = fa.height;
   const float *cpos = fa.cam_pos, *cdir = fa.cam_dir, *cup = fa.cam_up, *cright = fa.cam_right;
-  float fov = fa.fov;
+  float tf = fa.tan_half;
   if (fa.views) {
     const ViewDesc& V = fa.views[view_of(fa, pix)];
     lp = pix - V.offset;
     vw = V.width;
     vh = V.height;
     cpos = V.cam_pos; cdir = V.cam_dir; cup = V.cam_up; cright = V.cam_right;
-    fov = V.fov;
+    tf = V.tan_half;
   }
   uint32_t y = lp / vw, x = lp - y * vw;
   float W = (float)vw, H = (float)vh;
-  float tf = tanf(fov * 0.5f);
   float u = ((((2.f * ((float)x + 0.5f)) / W - 1.f) * tf) * W) / H;
   float v = ((2.f * ((float)y + 0.5f)) / H - 1.f) * tf;
   const uint64_t inc = ((uint64_t)lp << 1) | 1u;
@@ -2175,7 +2174,7 @@ __global__ __launch_bounds__(kBlock) void k_regen_eye(DevScene sc, FrameArgs fa,
     const f3 cright = mk3(fa.cam_right[0], fa.cam_right[1], fa.cam_right[2]);
     const f3 cup = mk3(fa.cam_up[0], fa.cam_up[1], fa.cam_up[2]);
     const f3 cdir = mk3(fa.cam_dir[0], fa.cam_dir[1], fa.cam_dir[2]);
-    const float Wf = (float)W, Hf = (float)H, tf = tanf(fa.fov * 0.5f);
+    const float Wf = (float)W, Hf = (float)H, tf = fa.tan_half;
     const float pu = ((((2.f * ((float)x + 0.5f)) / Wf - 1.f) * tf) * Wf) / Hf;   // pt0_kernel :646-647
     const float pv = ((2.f * ((float)y + 0.5f)) / Hf - 1.f) * tf;
     auto camera_ray = [&]() {
