@@ -597,8 +597,16 @@ __device__ __forceinline__ float d_ggx(float r, float noh) {
   float d = (noh * m2 - noh) * noh + 1.f;
   return (float)((double)m2 / (kPiD * (double)d * (double)d));
 }
+// x^5 of Schlick's term: the power in double (x^2 exact, x^4 and x^5 rounded
+// at 2^-53) rounded once to float, the correctly rounded x^5 except within
+// ~2^-51 of a rounding boundary (the reference's powf is within an ulp of it);
+// a few instructions where a general powf takes dozens.
+__device__ __forceinline__ float pow5f(float x) {
+  const double d = (double)x, d2 = d * d;
+  return (float)(d2 * d2 * d);
+}
 __device__ __forceinline__ f3 f_schlick(f3 s, float voh) {
-  float fc = powf(1.f - voh, 5.f);
+  float fc = pow5f(1.f - voh);
   double z = 50.0 * (double)s.z;
   float first = (float)(z < 0.0 ? 0.0 : (z > 1.0 ? 1.0 : z)) * fc;
   return mk3(first, first, first) + s * (1.f - fc);

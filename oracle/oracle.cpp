@@ -72,6 +72,7 @@
 // Build: oracle/Makefile (g++ -O2 -ffp-contract=off -fopenmp).
 
 #include <cmath>
+#include <type_traits>
 #include <cstdint>
 #include <cstring>
 #include <cstdio>
@@ -530,8 +531,20 @@ template <class R> static inline R d_ggx(R rough, R noh) {
   R d = (noh * m2 - noh) * noh + (R)1;
   return (R)((double)m2 / (kPi * (double)d * (double)d));
 }
+// x^5 of Schlick's term.  FP32 runs evaluate it as the device does: the
+// power in double (x^2 exact, x^4 and x^5 rounded at 2^-53) rounded once to
+// float, the correctly rounded x^5 except within ~2^-51 of a rounding
+// boundary; the reference's powf is within an ulp of the same value.
+template <class R> static inline R pow5(R x) {
+  if constexpr (std::is_same<R, float>::value) {
+    const double d = (double)x, d2 = d * d;
+    return (float)(d2 * d2 * d);
+  } else {
+    return std::pow(x, (R)5);
+  }
+}
 template <class R> static inline V3<R> f_schlick(V3<R> sc, R voh) {
-  R fc = std::pow((R)1 - voh, (R)5);
+  R fc = pow5((R)1 - voh);
   R first = (R)saturate(50.0 * (double)sc.z) * fc;
   return mk(first, first, first) + sc * ((R)1 - fc);
 }

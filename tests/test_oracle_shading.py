@@ -89,6 +89,39 @@ def test_sampler_frame_is_orthonormal():
 
 
 # ---------------------------------------------------------------- F_Schlick, Vis_SmithJointApprox
+def _rn32(q):
+    """The float32 nearest the rational q (ties to even)."""
+    from fractions import Fraction
+    f = np.float32(float(q))
+    best = None
+    for c in (np.nextafter(f, np.float32(-np.inf)), f, np.nextafter(f, np.float32(np.inf))):
+        e = abs(Fraction(float(c)) - q)
+        if best is None or e < best[0] or (e == best[0] and int(c.view(np.uint32)) % 2 == 0):
+            best = (e, c)
+    return best[1]
+
+
+def test_schlick_fifth_power_fp32_is_correctly_rounded():
+    """The FP32 restatement's (1 - VoH)^5, the device's double-evaluated power:
+    equal to the correctly rounded x^5 on (nearly) every argument and never
+    more than an ulp away (F = Fc exactly in channel x when f0 = (0, 0, 1))."""
+    from fractions import Fraction
+    rng = np.random.default_rng(5)
+    voh = rng.uniform(0.0, 1.0, 4000).astype(np.float32)
+    exact = same = 0
+    for v in voh:
+        x = np.float32(1.0) - v
+        fc = np.float32(oracle.f_schlick(np.array([0.0, 0.0, 1.0]), float(v), prec=32)[0])
+        want = _rn32(Fraction(float(x)) ** 5)
+        ulp = abs(int(fc.view(np.int32)) - int(want.view(np.int32)))
+        assert ulp <= 1, (v, fc, want)
+        same += ulp == 0
+        exact += 1
+    assert same >= 0.999 * exact
+    # powers of two are exact: x = 1/2 -> 1/32
+    assert np.float32(oracle.f_schlick(np.array([0.0, 0.0, 1.0]), 0.5, prec=32)[0]) == np.float32(1 / 32)
+
+
 def test_schlick_hand_values():
     f0 = np.array([0.04, 0.04, 0.04])
     # VoH = 1: Fc = 0, F = f0 exactly
