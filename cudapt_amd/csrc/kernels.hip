@@ -622,7 +622,9 @@ __device__ __forceinline__ f3 sample_ggx(float xi0, float xi1, float rough, f3 n
   float phi = (float)(2.0 * kPiD * (double)xi0);
   float ct = sqrtf((1.f - xi1) / (1.f + (a * a - 1.f) * xi1));
   float st = sqrtf(1.f - ct * ct);
-  f3 h = mk3(st * cosf(phi), st * sinf(phi), ct);
+  float sp, cp;
+  sincos_def(phi, sp, cp);
+  f3 h = mk3(st * cp, st * sp, ct);
   f3 u, v;
   basis(n, u, v);
   return (u * h.x + v * h.y) + n * h.z;
@@ -688,8 +690,8 @@ __device__ __forceinline__ float4 tex_sample(const DevScene& sc, uint32_t id, fl
 // wrapping and v clamped to the first and last rows.  A non-finite d sees 0.
 __device__ __forceinline__ f3 env_radiance(const DevScene& sc, f3 d) {
   const int W = (int)sc.env_w, H = (int)sc.env_h;
-  const float u = atan2f(d.z, d.x) * (float)(0.5 / kPiD) + 0.5f;
-  const float v = acosf(fminf(fmaxf(d.y, -1.f), 1.f)) * (float)(1.0 / kPiD);
+  const float u = atan2_def(d.z, d.x) * (float)(0.5 / kPiD) + 0.5f;
+  const float v = acos_def(fminf(fmaxf(d.y, -1.f), 1.f)) * (float)(1.0 / kPiD);
   const float x = u * (float)W - 0.5f, y = v * (float)H - 0.5f;
   if (!(fabsf(x) < 1e30f) || !(fabsf(y) < 1e30f)) return mk3(0.f, 0.f, 0.f);
   const float fx = floorf(x), fy = floorf(y);
@@ -799,7 +801,9 @@ __device__ Bounce sample_bounce(uint64_t& st, uint64_t inc, f3 ro, f3 rd, float 
     float r1 = (float)(2.0 * kPiD * (double)uniform01(st, inc));
     float r2cos = sqrtf(uniform01(st, inc));
     float r2sin = 1.f - r2cos * r2cos;
-    f3 dd = normalize((nl * r2cos + (u * r2sin) * cosf(r1)) + (v * r2sin) * sinf(r1));
+    float s1, c1;
+    sincos_def(r1, s1, c1);
+    f3 dd = normalize((nl * r2cos + (u * r2sin) * c1) + (v * r2sin) * s1);
     b.orig = (ro + rd * t) + dd * (float)kBiasEps;
     b.dir = dd;
     b.mul = lambert(s.diff) * (float)kPiD / ((1.f - refrProb) * (1.f - reflProb));
@@ -931,7 +935,9 @@ __global__ __launch_bounds__(kBlock) void k_light_pass(DevScene sc, FrameArgs fa
     float r1 = (float)(2.0 * kPiD * (double)uniform01(st, inc));
     float r2cos = sqrtf(uniform01(st, inc));
     float r2sin = 1.f - r2cos * r2cos;
-    f3 dd = normalize((s.norm * r2cos + (u * r2sin) * cosf(r1)) + (v * r2sin) * sinf(r1));
+    float s1, c1;
+    sincos_def(r1, s1, c1);
+    f3 dd = normalize((s.norm * r2cos + (u * r2sin) * c1) + (v * r2sin) * s1);
     orig = s.pos + s.norm * (float)kBiasEps;
     dir = dd;
     smp = s.emissive;
@@ -1712,7 +1718,9 @@ __device__ bool disney_sample(const DisneyMat& m, f3 n, f3 wo, float u0, float u
   const float phi = (float)(2.0 * kPiD * (double)u1);
   if (u0 < pd) {
     const float ct = sqrtf(u2), st = sqrtf(1.f - u2);
-    wi = normalize((n * ct + (u * st) * cosf(phi)) + (v * st) * sinf(phi));
+    float sp, cp;
+    sincos_def(phi, sp, cp);
+    wi = normalize((n * ct + (u * st) * cp) + (v * st) * sp);
   } else {
     const bool spec = u0 < pd + ps;
     const float a = spec ? disney_alpha(m) : disney_cc_alpha(m);
@@ -1720,7 +1728,9 @@ __device__ bool disney_sample(const DisneyMat& m, f3 n, f3 wo, float u0, float u
     if (spec) ct = sqrtf((1.f - u2) / (1.f + (a * a - 1.f) * u2));
     else ct = a >= 1.f ? sqrtf(1.f - u2) : sqrtf((1.f - powf(a * a, 1.f - u2)) / (1.f - a * a));
     const float st = sqrtf(fmaxf(0.f, 1.f - ct * ct));
-    const f3 h = (n * ct + (u * st) * cosf(phi)) + (v * st) * sinf(phi);
+    float sp, cp;
+    sincos_def(phi, sp, cp);
+    const f3 h = (n * ct + (u * st) * cp) + (v * st) * sp;
     wi = normalize(h * (2.f * dot(wo, h)) - wo);
   }
   return dot(n, wi) > 0.f;
@@ -2136,7 +2146,9 @@ __global__ __launch_bounds__(kRegenLightN / kRegenLightDepth) void k_regen_light
         const float r1 = (float)(2.0 * kPiD * (double)uniform01(st, inc));
         const float r2cos = sqrtf(uniform01(st, inc));
         const float r2sin = 1.f - r2cos * r2cos;
-        nd = normalize((s.norm * r2cos + (u * r2sin) * cosf(r1)) + (v * r2sin) * sinf(r1));
+        float s1, c1;
+        sincos_def(r1, s1, c1);
+        nd = normalize((s.norm * r2cos + (u * r2sin) * c1) + (v * r2sin) * s1);
         no = s.pos + s.norm * (float)kBiasEps;
         nvalid = true;
         store_lv(V, s.pos, s.norm, s.emissive, mk3(0.f, 0.f, 0.f), s);

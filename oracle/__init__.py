@@ -84,6 +84,9 @@ def lib():
         L.orc_vis_smith.restype = d
         L.orc_vis_smith.argtypes = [ci, d, d, d]
         L.orc_f_schlick.argtypes = [ci, f64p, d, f64p]
+        L.orc_sincos_def.argtypes = [f32p, c32, f32p, f32p]
+        L.orc_atan2_def.argtypes = [f32p, f32p, c32, f32p]
+        L.orc_acos_def.argtypes = [f32p, c32, f32p]
         L.orc_sample_ggx.argtypes = [ci, d, d, d, f64p, f64p]
         L.orc_basis.argtypes = [ci, f64p, f64p, f64p]
         L.orc_shading.argtypes = [ci, f64p, f64p]
@@ -299,6 +302,32 @@ def d_ggx(rough, noh, prec=64):
 def vis_smith(rough, nov, nol, prec=64):
     """Vis_SmithJointApprox (cudaRTCommon.cu:299-307)."""
     return lib().orc_vis_smith(prec, float(rough), float(nov), float(nol))
+
+
+def sincos_def(x):
+    """The FP32 sin and cos of x >= 0 (Cephes' octant reduction and polynomials),
+    the device's operation sequence."""
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    sn, cs = np.zeros_like(x), np.zeros_like(x)
+    lib().orc_sincos_def(x, x.size, sn, cs)
+    return sn, cs
+
+
+def atan2_def(y, x):
+    """The FP32 atan2 of the environment lookup (Cephes' atanf core by octant)."""
+    y = np.ascontiguousarray(y, np.float32).ravel()
+    x = np.ascontiguousarray(x, np.float32).ravel()
+    out = np.zeros_like(y)
+    lib().orc_atan2_def(y, x, y.size, out)
+    return out
+
+
+def acos_def(y):
+    """The FP32 acos of the environment lookup: atan2(sqrt((1 - y)(1 + y)), y)."""
+    y = np.ascontiguousarray(y, np.float32).ravel()
+    out = np.zeros_like(y)
+    lib().orc_acos_def(y, y.size, out)
+    return out
 
 
 def f_schlick(f0, voh, prec=64):

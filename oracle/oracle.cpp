@@ -138,7 +138,73 @@ template <class R> static inline V3<R> operator+(V3<R> a, V3<R> b) { return mk(a
 template <class R> static inline V3<R> operator-(V3<R> a, V3<R> b) { return mk(a.x - b.x, a.y - b.y, a.z - b.z); }
 template <class R> static inline V3<R> operator*(V3<R> a, R s) { return mk(a.x * s, a.y * s, a.z * s); }
 template <class R> static inline V3<R> operator*(R s, V3<R> a) { return a * s; }
-template <class R> static inline V3<R> operator/(V3<R> a, R s) { return mk(a.x / s, a.y / s, a.z / s); }
+// A vector over a scalar.  FP32 runs use the device's form: one correctly
+// rounded reciprocal, then three products (within an ulp of the reference's
+// three divisions); FP64 runs divide.
+template <class R> static inline V3<R> operator/(V3<R> a, R s) {
+  if constexpr (std::is_same<R, float>::value) {
+    const float r = 1.f / s;
+    return mk(a.x * r, a.y * r, a.z * r);
+  } else {
+    return mk(a.x / s, a.y / s, a.z / s);
+  }
+}
+// ---------------------------------------------------------------- trigonometry
+// FP32 runs evaluate sin/cos, atan2 and acos with the device's operation
+// sequence (devmath.h): Cephes' (S. L. Moshier) sinf/cosf octant reduction
+// and minimax polynomials for x >= 0, Cephes' atanf core on [0, 1] with the
+// octants of atan2, and acos(y) = atan2(sqrt((1 - y)(1 + y)), y).  FP64 runs
+// use libm.
+static inline void sincos_f(float x, float& s, float& c) {
+  int j = (int)(x * 1.27323954473516f);
+  j += j & 1;
+  const float y = (float)j;
+  float r = std::fma(-y, 0.78515625f, x);
+  r = std::fma(-y, 2.4187564849853515625e-4f, r);
+  r = std::fma(-y, 3.77489497744594108e-8f, r);
+  const float z = r * r;
+  const float ps = std::fma(std::fma(std::fma(-1.9515295891e-4f, z, 8.3321608736e-3f), z, -1.6666654611e-1f), z * r, r);
+  const float pc = std::fma(std::fma(std::fma(2.443315711809948e-5f, z, -1.388731625493765e-3f), z,
+                                     4.166664568298827e-2f),
+                            z * z, std::fma(-0.5f, z, 1.f));
+  const int q = (j >> 1) & 3;
+  s = (q & 1) ? pc : ps;
+  c = (q & 1) ? ps : pc;
+  if (q >= 2) s = -s;
+  if (q == 1 || q == 2) c = -c;
+}
+static inline float atan01_f(float t) {
+  float y0 = 0.f;
+  if (t > 0.4142135623730950f) {
+    y0 = 0.78539816339744831f;
+    t = (t - 1.f) / (t + 1.f);
+  }
+  const float z = t * t;
+  const float p = std::fma(std::fma(std::fma(8.05374449538e-2f, z, -1.38776856032e-1f), z, 1.99777106478e-1f), z,
+                           -3.33329491539e-1f);
+  return y0 + std::fma(p * z, t, t);
+}
+static inline float atan2_f(float y, float x) {
+  const float ax = std::fabs(x), ay = std::fabs(y);
+  float a;
+  if (ay <= ax) a = ax > 0.f ? atan01_f(ay / ax) : 0.f;
+  else a = 1.57079632679489662f - atan01_f(ax / ay);
+  if (x < 0.f) a = 3.14159265358979324f - a;
+  return y < 0.f ? -a : a;
+}
+template <class R> static inline void sincos_r(R x, R& s, R& c) {
+  if constexpr (std::is_same<R, float>::value) sincos_f(x, s, c);
+  else { s = std::sin(x); c = std::cos(x); }
+}
+template <class R> static inline R atan2_r(R y, R x) {
+  if constexpr (std::is_same<R, float>::value) return atan2_f(y, x);
+  else return std::atan2(y, x);
+}
+template <class R> static inline R acos_r(R y) {
+  if constexpr (std::is_same<R, float>::value) return atan2_f(std::sqrt((1.f - y) * (1.f + y)), y);
+  else return std::acos(y);
+}
+
 template <class R> static inline V3<R> vmul(V3<R> a, V3<R> b) { return mk(a.x * b.x, a.y * b.y, a.z * b.z); }
 template <class R> static inline V3<R> vdiv(V3<R> a, V3<R> b) { return mk(a.x / b.x, a.y / b.y, a.z / b.z); }
 template <class R> static inline V3<R> vmax(V3<R> a, V3<R> b) { return mk(std::fmax(a.x, b.x), std::fmax(a.y, b.y), std::fmax(a.z, b.z)); }
@@ -155,7 +221,7 @@ template <class R> static inline R sqrt_rd(R x) {
   return s;
 }
 template <class R> static inline R length(V3<R> a) { return sqrt_rd(fm(a.z, a.z, fm(a.y, a.y, a.x * a.x))); }
-template <class R> static inline V3<R> normalize(V3<R> a) { R l = length(a); return mk(a.x / l, a.y / l, a.z / l); }
+template <class R> static inline V3<R> normalize(V3<R> a) { return a / length(a); }
 template <class R> static inline R lerp(R x, R y, R d) { return x + (y - x) * d; }
 template <class R> static inline V3<R> vlerp(V3<R> x, V3<R> y, R d) { return x + (y - x) * d; }
 template <class R> static inline R saturate(R x) { return x < (R)0 ? (R)0 : (x > (R)1 ? (R)1 : x); }
@@ -559,7 +625,9 @@ template <class R> static inline V3<R> sample_ggx(R xi0, R xi1, R rough, V3<R> n
   R phi = (R)(2.0 * kPi * (double)xi0);
   R ct = std::sqrt(((R)1 - xi1) / ((R)1 + (a * a - (R)1) * xi1));
   R st = std::sqrt((R)1 - ct * ct);
-  V3<R> h = mk(st * std::cos(phi), st * std::sin(phi), ct);
+  R sp, cp;
+  sincos_r(phi, sp, cp);
+  V3<R> h = mk(st * cp, st * sp, ct);
   V3<R> u, v; basis(n, u, v);
   return (u * h.x + v * h.y) + n * h.z;
 }
@@ -676,7 +744,9 @@ static bool disney_sample(const Disney<R>& m, V3<R> n, V3<R> wo, R u0, R u1, R u
   R phi = (R)(2.0 * kPi * (double)u1);
   if (u0 < pd) {                                   // cosine-weighted hemisphere
     R ct = std::sqrt(u2), st = std::sqrt((R)1 - u2);
-    wi = normalize((n * ct + (u * st) * std::cos(phi)) + (v * st) * std::sin(phi));
+    R sp, cp;
+    sincos_r(phi, sp, cp);
+    wi = normalize((n * ct + (u * st) * cp) + (v * st) * sp);
   } else {
     R a = u0 < pd + ps ? disney_alpha(m) : disney_cc_alpha(m);
     R ct;
@@ -684,7 +754,9 @@ static bool disney_sample(const Disney<R>& m, V3<R> n, V3<R> wo, R u0, R u1, R u
     else ct = a >= (R)1 ? std::sqrt((R)1 - u2)
                         : std::sqrt(((R)1 - std::pow(a * a, (R)1 - u2)) / ((R)1 - a * a));                // GTR1
     R st = std::sqrt(std::fmax((R)0, (R)1 - ct * ct));
-    V3<R> h = (n * ct + (u * st) * std::cos(phi)) + (v * st) * std::sin(phi);
+    R sp, cp;
+    sincos_r(phi, sp, cp);
+    V3<R> h = (n * ct + (u * st) * cp) + (v * st) * sp;
     wi = normalize(h * ((R)2 * dot(wo, h)) - wo);
   }
   return dot(n, wi) > (R)0;
@@ -774,8 +846,8 @@ static void tex_sample(const Scene& s, u32 id, R u, R v, R out[4]) {
 template <class R>
 static V3<R> env_radiance(const Scene& s, V3<R> d) {
   const int W = (int)s.envW, H = (int)s.envH;
-  const R u = std::atan2(d.z, d.x) * (R)(0.5 / kPi) + (R)0.5;
-  const R v = std::acos(std::fmin(std::fmax(d.y, (R)-1), (R)1)) * (R)(1.0 / kPi);
+  const R u = atan2_r(d.z, d.x) * (R)(0.5 / kPi) + (R)0.5;
+  const R v = acos_r(std::fmin(std::fmax(d.y, (R)-1), (R)1)) * (R)(1.0 / kPi);
   const R x = u * (R)W - (R)0.5, y = v * (R)H - (R)0.5;
   if (!(std::fabs(x) < (R)1e30) || !(std::fabs(y) < (R)1e30)) return mk((R)0, (R)0, (R)0);
   const R fx = std::floor(x), fy = std::floor(y);
@@ -878,7 +950,9 @@ static Bounce<R> sample_bounce(Pcg32& rng, V3<R> ro, V3<R> rd, R t, const HitSha
     R r1 = (R)(2.0 * kPi * (double)uniform<R>(rng));
     R r2cos = std::sqrt(uniform<R>(rng));
     R r2sin = (R)1 - r2cos * r2cos;
-    V3<R> dd = normalize((nl * r2cos + (u * r2sin) * std::cos(r1)) + (v * r2sin) * std::sin(r1));
+    R s1, c1;
+    sincos_r(r1, s1, c1);
+    V3<R> dd = normalize((nl * r2cos + (u * r2sin) * c1) + (v * r2sin) * s1);
     b.orig = (ro + rd * t) + dd * (R)kBiasEps; b.dir = dd;
     b.mul = lambert(hs.diff) * (R)kPi / (((R)1 - refrProb) * ((R)1 - reflProb));
     b.type = RT_DIFF;
@@ -952,7 +1026,9 @@ static void light_pass(const Scene& s, const RenderParams& p, u32 frame, FrameSt
     R r1 = (R)(2.0 * kPi * (double)uniform<R>(P.rng));
     R r2cos = std::sqrt(uniform<R>(P.rng));
     R r2sin = (R)1 - r2cos * r2cos;
-    V3<R> dd = normalize((hs.norm * r2cos + (u * r2sin) * std::cos(r1)) + (v * r2sin) * std::sin(r1));
+    R s1, c1;
+    sincos_r(r1, s1, c1);
+    V3<R> dd = normalize((hs.norm * r2cos + (u * r2sin) * c1) + (v * r2sin) * s1);
     P.orig = hs.pos + hs.norm * (R)kBiasEps; P.dir = dd; P.sample = hs.emissive; P.mul = mk((R)1, (R)1, (R)1);
     P.type = RT_LIGHT; P.depth = 0; P.origin = x; P.term = false;
     LightVertex<R>& V = fs.lv[x];
@@ -1337,7 +1413,9 @@ static void regen_light_pass(const Scene& s, const RenderParams& p, u32 frame, F
           R r1 = (R)(2.0 * kPi * (double)uniform<R>(rng));
           R r2cos = std::sqrt(uniform<R>(rng));
           R r2sin = (R)1 - r2cos * r2cos;
-          nd = normalize((hs.norm * r2cos + (u * r2sin) * std::cos(r1)) + (v * r2sin) * std::sin(r1));
+          R s1, c1;
+          sincos_r(r1, s1, c1);
+          nd = normalize((hs.norm * r2cos + (u * r2sin) * c1) + (v * r2sin) * s1);
           no = hs.pos + hs.norm * (R)kBiasEps;
           nvalid = true;
           V = shade_vertex(hs, hs.emissive, mk((R)0, (R)0, (R)0));
@@ -2073,6 +2151,15 @@ double orc_d_ggx(int prec, double rough, double noh) {
 }
 double orc_vis_smith(int prec, double rough, double nov, double nol) {
   return prec == 64 ? vis_smith<double>(rough, nov, nol) : (double)vis_smith<float>((float)rough, (float)nov, (float)nol);
+}
+void orc_sincos_def(const float* x, uint32_t n, float* s, float* c) {
+  for (uint32_t i = 0; i < n; i++) sincos_f(x[i], s[i], c[i]);
+}
+void orc_atan2_def(const float* y, const float* x, uint32_t n, float* out) {
+  for (uint32_t i = 0; i < n; i++) out[i] = atan2_f(y[i], x[i]);
+}
+void orc_acos_def(const float* y, uint32_t n, float* out) {
+  for (uint32_t i = 0; i < n; i++) out[i] = acos_r<float>(y[i]);
 }
 void orc_f_schlick(int prec, const double* f0, double voh, double* out) {
   if (prec == 64) put3(f_schlick<double>(v3d<double>(f0), voh), out);
