@@ -2,12 +2,11 @@
 box with a glass sphere, one area light) loaded through the preset reader and
 rendered by the GPU streaming BDPT, against the oracle on the same scene.
 
-Refraction through the sphere carries last-bit differences (device vs host
-transcendentals) further than the diffuse Cornell box does: a flipped
-Russian-roulette or branch decision changes a whole path.  So the FP32 gate is
-per pixel (at least 97 % of pixels within 1e-4 relative of the host run) plus
-relative MSE 1e-4 over the image (measured: 8.3e-6), and 1e-3 against FP64
-(measured: 1.7e-4, the FP32 host run itself differs from FP64 by 1.8e-4)."""
+The FP32 host run is the device operation sequence, transcendentals included,
+so the FP32 GPU image and counts equal it exactly, refraction through the
+sphere and all.  Against FP64 the gate is 1e-3 (measured 1.7e-4: the FP32 host
+run itself differs from FP64 by 1.8e-4, a flipped Russian-roulette or branch
+decision through the glass changing a whole path)."""
 import os
 
 import numpy as np
@@ -46,6 +45,7 @@ def test_preset_scene_matches_fp32_oracle(runs):
     assert e <= 1e-4
     assert close >= 0.97
     assert (runs["n"] == runs["n32"]).mean() > 0.99
+    assert np.array_equal(g.view(np.uint32), r.view(np.uint32)) and np.array_equal(runs["n"], runs["n32"])
 
 
 def test_preset_scene_matches_fp64_oracle(runs):

@@ -48,6 +48,8 @@ def test_pt_cornell_vs_oracle(runs, mode):
     assert np.isfinite(r["gpu"]).all() and r["gpu"].mean() > 0.02
     e64 = rel_mse(r["gpu"], r["r64"])
     e32 = rel_mse(r["gpu"], r["r32"])
+    print(f"PT mode {mode} Cornell: rel MSE vs FP64 {e64:.3e}, vs FP32 {e32:.3e}, "
+          f"N equal {(r['n'] == r['n32']).mean():.5f}")
     assert e64 <= 1e-4, f"relative MSE vs FP64 {e64:.3e}"
     assert e32 <= 1e-6, f"relative MSE vs FP32 {e32:.3e}"
     assert (r["n"] == r["n32"]).mean() > 0.995
@@ -60,6 +62,8 @@ def test_pt_glass_vs_oracle(runs, mode):
     r = runs[("glass", mode)]
     d = np.abs(r["gpu"] - r["r32"]).max(axis=-1)
     tol = 1e-4 * np.maximum(1.0, np.abs(r["r32"]).max(axis=-1))
+    print(f"PT mode {mode} glass: rel MSE vs FP32 {rel_mse(r['gpu'], r['r32']):.3e}, "
+          f"N equal {(r['n'] == r['n32']).mean():.5f}")
     assert (d <= tol).mean() >= 0.99, f"{(d <= tol).mean():.4f} of pixels within 1e-4"
     assert abs(r["gpu"].mean() / r["r64"].mean() - 1) < 0.02
     assert (r["n"] == r["n32"]).mean() > 0.98
