@@ -617,7 +617,8 @@ __device__ __forceinline__ void basis(f3 w, f3& u, f3& v) {
   v = cross(w, u);
   u = cross(v, w);
 }
-__device__ __forceinline__ f3 sample_ggx(float xi0, float xi1, float rough, f3 n) {
+// (u, v, n): the tangent frame basis(n) gives.
+__device__ __forceinline__ f3 sample_ggx(float xi0, float xi1, float rough, f3 n, f3 u, f3 v) {
   float a = rough * rough;
   float phi = (float)(2.0 * kPiD * (double)xi0);
   float ct = sqrtf((1.f - xi1) / (1.f + (a * a - 1.f) * xi1));
@@ -625,8 +626,6 @@ __device__ __forceinline__ f3 sample_ggx(float xi0, float xi1, float rough, f3 n
   float sp, cp;
   sincos_def(phi, sp, cp);
   f3 h = mk3(st * cp, st * sp, ct);
-  f3 u, v;
-  basis(n, u, v);
   return (u * h.x + v * h.y) + n * h.z;
 }
 __device__ __forceinline__ f3 lambert(f3 diff) { return diff * (float)(1.0 / kPiD); }
@@ -757,7 +756,9 @@ struct Bounce {
 __device__ Bounce sample_bounce(uint64_t& st, uint64_t inc, f3 ro, f3 rd, float t, const Surf& s, f3 nl, int cur,
                                 bool refl_offset = false, bool tir_reflect = false) {
   float x0 = uniform01(st, inc), x1 = uniform01(st, inc);
-  f3 hDir = sample_ggx(x0, x1, s.rough, nl);
+  f3 u, v;                                           // one tangent frame for both samplers
+  basis(nl, u, v);
+  f3 hDir = sample_ggx(x0, x1, s.rough, nl, u, v);
   f3 ndir = rd * -1.f;
   float voH = dot(ndir, hDir);
   f3 f0 = vlerp(mk3(0.08f * s.spec, 0.08f * s.spec, 0.08f * s.spec), s.diff, s.metal);
@@ -796,8 +797,6 @@ __device__ Bounce sample_bounce(uint64_t& st, uint64_t inc, f3 ro, f3 rd, float 
     b.mul = s.diff * cosine / (refrProb * (1.f - reflProb));
     b.type = RT_SPEC;
   } else {
-    f3 u, v;
-    basis(nl, u, v);
     float r1 = (float)(2.0 * kPiD * (double)uniform01(st, inc));
     float r2cos = sqrtf(uniform01(st, inc));
     float r2sin = 1.f - r2cos * r2cos;
