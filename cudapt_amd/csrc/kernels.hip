@@ -225,23 +225,11 @@ __device__ __forceinline__ void box4_q(const DevScene& sc, const RayState& R, fl
   ch[0] = D.x; ch[1] = D.y; ch[2] = D.z; ch[3] = D.w;
 }
 
-// One node: the four child boxes against the ray (culled beyond the best hit
-// plus slack); the leaf children hit are then tested as one run of triangles
-// (child order, triangle order), so that lanes whose leaves sit under different
-// children test them in the same loop; interior children are culled again with
-// the updated best hit, sorted near-first, the nearest taken next and the rest
-// pushed (LDS stack top, private stack below it).  The closest hit is the
-// (t, original id) minimum, which neither the test order nor the conservative
-// culling changes.  Returns true when the ray is finished.
-// lstk points at this thread's column of a [STK][kBlock] u64 array in LDS.
-// spill (an LDS word of the wave, or null) counts pushes that land below the
-// LDS top, in the private (scratch) stack: an LDS atomic on that rare event
-// instead of a counter register in the loop.
-template <bool ANY, int STK, bool Q>
-__device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool cull, LdsU64* lstk, uint64_t* pstk,
-                                         uint32_t& nn, uint32_t& nt, uint32_t* spill) {
-  float tn[4], tf[4];
-  uint32_t ch[4], cn[4];
+// The four child boxes of node R.node (either layout): entry / exit
+// distances, children and leaf counts.
+template <bool Q>
+__device__ __forceinline__ void child_boxes(const DevScene& sc, const RayState& R, float tn[4], float tf[4],
+                                            uint32_t ch[4], uint32_t cn[4]) {
   if constexpr (Q) {
     box4_q(sc, R, tn, tf, ch, cn);
   } else {
@@ -258,6 +246,93 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool c
       cn[c] = nv.cn[c];
     }
   }
+}
+
+// The leaf children's triangles as one run (child order, triangle order):
+// child c contributes ln[c] triangles from lf[c] on.  Run position k lies in
+// child c when p_c <= k < p_(c+1) (prefix sums of the counts); its triangle is
+// k + (first_c - p_c): selects, no branches.  Returns true when an occlusion
+// ray (ANY) found a hit.
+template <bool ANY>
+__device__ __forceinline__ bool test_run(const DevScene& sc, RayState& R, bool cull, const uint32_t lf[4],
+                                         const uint32_t ln[4], uint32_t& nt) {
+  const uint32_t p1 = ln[0], p2 = p1 + ln[1], p3 = p2 + ln[2];
+  const uint32_t ntri = p3 + ln[3];
+  nt += ntri;
+  const uint32_t d0 = lf[0], d1 = lf[1] - p1, d2 = lf[2] - p2, d3 = lf[3] - p3;
+  // All of a triangle's 48 bytes are fetched at once (the empty asm needs the
+  // vertex before the determinant test, which would otherwise delay its load
+  // to a second round trip).  Occlusion rays fetch two triangles per round
+  // trip; closest-hit rays one (two would spill registers there).
+  if constexpr (ANY) {
+    for (uint32_t k = 0; k < ntri; k += 2) {
+      const uint32_t i0 = k + (k < p1 ? d0 : k < p2 ? d1 : k < p3 ? d2 : d3);
+      const uint32_t k1 = k + 1;
+      const bool two = k1 < ntri;
+      const uint32_t i1 = two ? k1 + (k1 < p1 ? d0 : k1 < p2 ? d1 : k1 < p3 ? d2 : d3) : i0;
+      const float4 a0 = sc.tris[3 * i0], b0 = sc.tris[3 * i0 + 1], c0 = sc.tris[3 * i0 + 2];
+      const float4 a1 = sc.tris[3 * i1], b1 = sc.tris[3 * i1 + 1], c1 = sc.tris[3 * i1 + 2];
+      asm volatile("" : : "v"(a0.x), "v"(a0.y), "v"(a0.z), "v"(a1.x), "v"(a1.y), "v"(a1.z));
+      {
+        float u = 0.f, v = 0.f;
+        const float t = isect_tri(R.o2, R.d, ld3(a0), ld3(b0), ld3(c0), cull, u, v);
+        const uint32_t oid = __float_as_uint(a0.w);
+        if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i0; R.h.orig = oid; }
+        if (R.h.leaf != kInvalid) return true;
+      }
+      if (two) {
+        float u = 0.f, v = 0.f;
+        const float t = isect_tri(R.o2, R.d, ld3(a1), ld3(b1), ld3(c1), cull, u, v);
+        const uint32_t oid = __float_as_uint(a1.w);
+        if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i1; R.h.orig = oid; }
+        if (R.h.leaf != kInvalid) return true;
+      }
+    }
+  } else {
+    for (uint32_t k = 0; k < ntri; k++) {
+      const uint32_t i = k + (k < p1 ? d0 : k < p2 ? d1 : k < p3 ? d2 : d3);
+      const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
+      asm volatile("" : : "v"(a.x), "v"(a.y), "v"(a.z));
+      float u = 0.f, v = 0.f;
+      const float t = isect_tri(R.o2, R.d, ld3(a), ld3(b), ld3(cc), cull, u, v);
+      const uint32_t oid = __float_as_uint(a.w);
+      if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i; R.h.orig = oid; }
+    }
+  }
+  return false;
+}
+
+// Pops the nearest pending node still in front of the best hit into R.node;
+// false when the stack runs out (the ray is finished).
+template <int STK>
+__device__ __forceinline__ bool pop_node(RayState& R, const LdsU64* lstk, const uint64_t* pstk) {
+  while (R.sp > 0) {
+    R.sp--;
+    uint64_t e = R.sp < STK ? lstk[R.sp * kBlock] : pstk[R.sp - STK];
+    float tn = __uint_as_float((uint32_t)(e >> 32));
+    if (tn <= R.h.t + cull_slack(R.h.t, R.eps)) { R.node = (uint32_t)e; return true; }
+  }
+  return false;
+}
+
+// One node: the four child boxes against the ray (culled beyond the best hit
+// plus slack); the leaf children hit are then tested as one run of triangles
+// (test_run), so that lanes whose leaves sit under different children test
+// them in the same loop; interior children are culled again with the updated
+// best hit, sorted near-first, the nearest taken next and the rest pushed
+// (LDS stack top, private stack below it).  The closest hit is the
+// (t, original id) minimum, which neither the test order nor the conservative
+// culling changes.  Returns true when the ray is finished.
+// lstk points at this thread's column of a [STK][kBlock] u64 array in LDS.
+// spill (an LDS word of the wave, or null) counts pushes that land below the
+// LDS top, in the private (scratch) stack: an LDS atomic on that rare event
+// instead of a counter register in the loop.
+template <bool ANY, int STK, bool Q>
+__device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool cull, LdsU64* lstk, uint64_t* pstk,
+                                         uint32_t& nn, uint32_t& nt, uint32_t* spill) {
+  float tn[4], tf[4];
+  uint32_t ch[4], cn[4];
+  child_boxes<Q>(sc, R, tn, tf, ch, cn);
   nn++;
   const float lim = R.h.t + cull_slack(R.h.t, R.eps);
   uint64_t key[4];
@@ -272,52 +347,8 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool c
     // 0 marks no interior child (a child link is never the root, 0)
     key[c] = hit && !leaf ? ((uint64_t)__float_as_uint(tnear) << 32) | ch[c] : 0ull;
   }
-  // run position k lies in child c when p_c <= k < p_(c+1) (prefix sums of the
-  // counts); its triangle is k + (first_c - p_c): selects, no branches
-  const uint32_t p1 = ln[0], p2 = p1 + ln[1], p3 = p2 + ln[2];
-  const uint32_t ntri = p3 + ln[3];
-  if (ntri) {
-    nt += ntri;
-    const uint32_t d0 = lf[0], d1 = lf[1] - p1, d2 = lf[2] - p2, d3 = lf[3] - p3;
-    // All of a triangle's 48 bytes are fetched at once (the empty asm needs the
-    // vertex before the determinant test, which would otherwise delay its load
-    // to a second round trip).  Occlusion rays fetch two triangles per round
-    // trip; closest-hit rays one (two would spill registers there).
-    if constexpr (ANY) {
-      for (uint32_t k = 0; k < ntri; k += 2) {
-        const uint32_t i0 = k + (k < p1 ? d0 : k < p2 ? d1 : k < p3 ? d2 : d3);
-        const uint32_t k1 = k + 1;
-        const bool two = k1 < ntri;
-        const uint32_t i1 = two ? k1 + (k1 < p1 ? d0 : k1 < p2 ? d1 : k1 < p3 ? d2 : d3) : i0;
-        const float4 a0 = sc.tris[3 * i0], b0 = sc.tris[3 * i0 + 1], c0 = sc.tris[3 * i0 + 2];
-        const float4 a1 = sc.tris[3 * i1], b1 = sc.tris[3 * i1 + 1], c1 = sc.tris[3 * i1 + 2];
-        asm volatile("" : : "v"(a0.x), "v"(a0.y), "v"(a0.z), "v"(a1.x), "v"(a1.y), "v"(a1.z));
-        {
-          float u = 0.f, v = 0.f;
-          const float t = isect_tri(R.o2, R.d, ld3(a0), ld3(b0), ld3(c0), cull, u, v);
-          const uint32_t oid = __float_as_uint(a0.w);
-          if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i0; R.h.orig = oid; }
-          if (R.h.leaf != kInvalid) return true;
-        }
-        if (two) {
-          float u = 0.f, v = 0.f;
-          const float t = isect_tri(R.o2, R.d, ld3(a1), ld3(b1), ld3(c1), cull, u, v);
-          const uint32_t oid = __float_as_uint(a1.w);
-          if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i1; R.h.orig = oid; }
-          if (R.h.leaf != kInvalid) return true;
-        }
-      }
-    } else {
-      for (uint32_t k = 0; k < ntri; k++) {
-        const uint32_t i = k + (k < p1 ? d0 : k < p2 ? d1 : k < p3 ? d2 : d3);
-        const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
-        asm volatile("" : : "v"(a.x), "v"(a.y), "v"(a.z));
-        float u = 0.f, v = 0.f;
-        const float t = isect_tri(R.o2, R.d, ld3(a), ld3(b), ld3(cc), cull, u, v);
-        const uint32_t oid = __float_as_uint(a.w);
-        if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i; R.h.orig = oid; }
-      }
-    }
+  if (ln[0] + ln[1] + ln[2] + ln[3]) {
+    if (test_run<ANY>(sc, R, cull, lf, ln, nt)) return true;
     const float lim2 = R.h.t + cull_slack(R.h.t, R.eps);
 #pragma unroll
     for (int q = 0; q < 4; q++)
@@ -360,14 +391,7 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool c
     }
     return false;
   }
-  // pop the nearest pending node still in front of the best hit
-  while (R.sp > 0) {
-    R.sp--;
-    uint64_t e = R.sp < STK ? lstk[R.sp * kBlock] : pstk[R.sp - STK];
-    float tn = __uint_as_float((uint32_t)(e >> 32));
-    if (tn <= R.h.t + cull_slack(R.h.t, R.eps)) { R.node = (uint32_t)e; return false; }
-  }
-  return true;
+  return !pop_node<STK>(R, lstk, pstk);
 }
 
 // Closest (ANY: any) hit of one ray, the loop of ray_step.
