@@ -262,42 +262,18 @@ __device__ __forceinline__ bool test_run(const DevScene& sc, RayState& R, bool c
   const uint32_t d0 = lf[0], d1 = lf[1] - p1, d2 = lf[2] - p2, d3 = lf[3] - p3;
   // All of a triangle's 48 bytes are fetched at once (the empty asm needs the
   // vertex before the determinant test, which would otherwise delay its load
-  // to a second round trip).  Occlusion rays fetch two triangles per round
-  // trip; closest-hit rays one (two would spill registers there).
-  if constexpr (ANY) {
-    for (uint32_t k = 0; k < ntri; k += 2) {
-      const uint32_t i0 = k + (k < p1 ? d0 : k < p2 ? d1 : k < p3 ? d2 : d3);
-      const uint32_t k1 = k + 1;
-      const bool two = k1 < ntri;
-      const uint32_t i1 = two ? k1 + (k1 < p1 ? d0 : k1 < p2 ? d1 : k1 < p3 ? d2 : d3) : i0;
-      const float4 a0 = sc.tris[3 * i0], b0 = sc.tris[3 * i0 + 1], c0 = sc.tris[3 * i0 + 2];
-      const float4 a1 = sc.tris[3 * i1], b1 = sc.tris[3 * i1 + 1], c1 = sc.tris[3 * i1 + 2];
-      asm volatile("" : : "v"(a0.x), "v"(a0.y), "v"(a0.z), "v"(a1.x), "v"(a1.y), "v"(a1.z));
-      {
-        float u = 0.f, v = 0.f;
-        const float t = isect_tri(R.o2, R.d, ld3(a0), ld3(b0), ld3(c0), cull, u, v);
-        const uint32_t oid = __float_as_uint(a0.w);
-        if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i0; R.h.orig = oid; }
-        if (R.h.leaf != kInvalid) return true;
-      }
-      if (two) {
-        float u = 0.f, v = 0.f;
-        const float t = isect_tri(R.o2, R.d, ld3(a1), ld3(b1), ld3(c1), cull, u, v);
-        const uint32_t oid = __float_as_uint(a1.w);
-        if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i1; R.h.orig = oid; }
-        if (R.h.leaf != kInvalid) return true;
-      }
-    }
-  } else {
-    for (uint32_t k = 0; k < ntri; k++) {
-      const uint32_t i = k + (k < p1 ? d0 : k < p2 ? d1 : k < p3 ? d2 : d3);
-      const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
-      asm volatile("" : : "v"(a.x), "v"(a.y), "v"(a.z));
-      float u = 0.f, v = 0.f;
-      const float t = isect_tri(R.o2, R.d, ld3(a), ld3(b), ld3(cc), cull, u, v);
-      const uint32_t oid = __float_as_uint(a.w);
-      if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i; R.h.orig = oid; }
-    }
+  // to a second round trip); one triangle per round trip for both ray kinds:
+  // fetching two for occlusion rays spilled five registers of the shadow
+  // kernel and ran it 2 % slower end to end.
+  for (uint32_t k = 0; k < ntri; k++) {
+    const uint32_t i = k + (k < p1 ? d0 : k < p2 ? d1 : k < p3 ? d2 : d3);
+    const float4 a = sc.tris[3 * i], b = sc.tris[3 * i + 1], cc = sc.tris[3 * i + 2];
+    asm volatile("" : : "v"(a.x), "v"(a.y), "v"(a.z));
+    float u = 0.f, v = 0.f;
+    const float t = isect_tri(R.o2, R.d, ld3(a), ld3(b), ld3(cc), cull, u, v);
+    const uint32_t oid = __float_as_uint(a.w);
+    if (t >= 0.f && better(t, oid, R.h)) { R.h.t = t; R.h.u = u; R.h.v = v; R.h.leaf = i; R.h.orig = oid; }
+    if (ANY && R.h.leaf != kInvalid) return true;
   }
   return false;
 }
