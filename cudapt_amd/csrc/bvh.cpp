@@ -7,6 +7,8 @@
 // directly.  Compiled with -ffp-contract=off so the host float arithmetic is
 // the reference's.
 #include <algorithm>
+#include <array>
+#include <limits>
 #include <cmath>
 #include <cstring>
 #include <queue>
@@ -310,6 +312,21 @@ bool build_bvh_device(const float* pos3, const u32* idx3, u32 nt, BvhHost& out) 
   return true;
 }
 
+// Cost weights of the collapse below: one visit of a 4-wide node (four box
+// tests, the near-first sort, push and pop) against one triangle test, in the
+// ratio of their VALU instructions in the traversal loop (about 230 and 82 on
+// gfx950).
+constexpr double kCollapseNodeCost = 2.8, kCollapseTriCost = 1.0;
+
+// SAH-optimal collapse of the binary device tree into 4-wide nodes, by dynamic
+// programming over the binary nodes.  F[x][i] (i = 1..4) is the least expected
+// cost of hanging binary subtree x below a 4-wide node as at most i entries,
+// each entry a leaf range (at most leaf_max triangles, or a binary leaf):
+// area(x) * tri * count, or a 4-wide node: area(x) * node + G[x][4].  G[x][i]
+// spreads at most i entries over x's children (k to the left, i - k to the
+// right), so F[x][i] = min(F[x][1], G[x][i]).  Children follow their parent in
+// the preorder, so one reverse sweep fills both tables.  Nodes are then
+// emitted breadth first from the root, whose entries are G[0][4]'s.
 void collapse_bvh4(const float* /*pos3*/, const u32* /*idx3*/, BvhHost& b, u32 leaf_max) {
   const std::vector<BinNode>& bn = b.dnodes;
   u32 N = (u32)bn.size();
@@ -323,10 +340,10 @@ void collapse_bvh4(const float* /*pos3*/, const u32* /*idx3*/, BvhHost& b, u32 l
     cnt[ii] = cnt[l] + cnt[r];
     first[ii] = first[l] != kInvalid ? first[l] : first[r];
   }
-  auto leafish = [&](u32 i) { return is_leaf(i) || cnt[i] <= leaf_max; };
   auto area = [&](u32 i) {
-    float dx = bn[i].mx[0] - bn[i].mn[0], dy = bn[i].mx[1] - bn[i].mn[1], dz = bn[i].mx[2] - bn[i].mn[2];
-    return dx * dy + dx * dz + dy * dz;
+    double dx = (double)bn[i].mx[0] - bn[i].mn[0], dy = (double)bn[i].mx[1] - bn[i].mn[1];
+    double dz = (double)bn[i].mx[2] - bn[i].mn[2];
+    return std::max(0.0, dx * dy + dx * dz + dy * dz);
   };
   b.nodes4.clear();
   b.depth4 = 0;
@@ -345,12 +362,9 @@ void collapse_bvh4(const float* /*pos3*/, const u32* /*idx3*/, BvhHost& b, u32 l
     }
   };
   if (N == 0) return;
-  // queue of (binary node, node4 slot, depth)
-  struct Item { u32 bin, slot, depth; };
-  std::queue<Item> q;
   b.nodes4.push_back(Node4());
   init4(b.nodes4[0]);
-  if (leafish(0)) {
+  if (is_leaf(0) || cnt[0] <= leaf_max) {
     set_child(b.nodes4[0], 0, 0);
     b.nodes4[0].child[0] = first[0];
     b.nodes4[0].count[0] = cnt[0];
@@ -358,38 +372,64 @@ void collapse_bvh4(const float* /*pos3*/, const u32* /*idx3*/, BvhHost& b, u32 l
     b.depth4 = 1;
     return;
   }
+  const double inf = std::numeric_limits<double>::infinity();
+  std::vector<std::array<double, 5>> F(N), G(N);
+  std::vector<std::array<uint8_t, 5>> gk(N);     // entries given to the left child (0: the one child takes all)
+  std::vector<uint8_t> as_leaf(N, 0);
+  for (u32 x = N; x-- > 0;) {
+    if (cnt[x] == 0) continue;
+    const double A = area(x);
+    const double leafC = (is_leaf(x) || cnt[x] <= leaf_max) ? A * kCollapseTriCost * cnt[x] : inf;
+    G[x].fill(inf);
+    gk[x].fill(0);
+    if (!is_leaf(x)) {
+      const u32 l = x + 1, r = x + bn[x].a;
+      for (int i = 1; i <= 4; i++) {
+        if (cnt[l] && cnt[r]) {
+          for (int k = 1; k < i; k++) {
+            const double c = F[l][k] + F[r][i - k];
+            if (c < G[x][i]) { G[x][i] = c; gk[x][i] = (uint8_t)k; }
+          }
+        } else {
+          G[x][i] = F[cnt[l] ? l : r][i];
+        }
+      }
+    }
+    const double nodeC = is_leaf(x) ? inf : A * kCollapseNodeCost + G[x][4];
+    as_leaf[x] = leafC <= nodeC;
+    F[x][0] = inf;
+    F[x][1] = std::min(leafC, nodeC);
+    for (int i = 2; i <= 4; i++) F[x][i] = std::min(F[x][1], G[x][i]);
+  }
+  // the entries F[x][i] (or, for spread = true, G[x][i]) stands for
+  std::vector<u32> ent;
+  auto take = [&](auto&& self, u32 x, int i, bool spread) -> void {
+    if (!spread && F[x][1] <= G[x][i]) { ent.push_back(x); return; }
+    const u32 l = x + 1, r = x + bn[x].a;
+    const int k = gk[x][i];
+    if (k == 0) {
+      self(self, cnt[l] ? l : r, i, false);
+    } else {
+      self(self, l, k, false);
+      self(self, r, i - k, false);
+    }
+  };
+  struct Item { u32 bin, slot, depth; };
+  std::queue<Item> q;
   q.push({0, 0, 1});
   while (!q.empty()) {
     Item it = q.front();
     q.pop();
     b.depth4 = std::max(b.depth4, it.depth);
-    u32 set[4];
-    int ns = 0;
-    u32 l = it.bin + 1, r = it.bin + bn[it.bin].a;
-    if (cnt[l]) set[ns++] = l;
-    if (cnt[r]) set[ns++] = r;
-    while (ns < 4) {
-      int best = -1;
-      float bestA = -1.f;
-      for (int i = 0; i < ns; i++)
-        if (!leafish(set[i]) && area(set[i]) > bestA) { bestA = area(set[i]); best = i; }
-      if (best < 0) break;
-      u32 x = set[best];
-      u32 xl = x + 1, xr = x + bn[x].a;
-      u32 repl[2];
-      int nr = 0;
-      if (cnt[xl]) repl[nr++] = xl;
-      if (cnt[xr]) repl[nr++] = xr;
-      set[best] = repl[0];
-      if (nr == 2) set[ns++] = repl[1];
-    }
+    ent.clear();
+    take(take, it.bin, 4, true);
     Node4 n4;
     init4(n4);
     std::vector<Item> kids;
-    for (int c = 0; c < ns; c++) {
-      u32 x = set[c];
+    for (int c = 0; c < (int)ent.size(); c++) {
+      const u32 x = ent[c];
       set_child(n4, c, x);
-      if (leafish(x)) {
+      if (as_leaf[x]) {
         n4.child[c] = first[x];
         n4.count[c] = cnt[x];
         b.leaves4++;

@@ -74,7 +74,9 @@ def test_errors_are_reported_not_fatal():
 
 
 def test_leaf_size_changes_the_4wide_tree_only():
-    """Smaller leaves: more 4-wide nodes and leaves, the binary SAH tree unchanged."""
+    """A smaller leaf bound: no fewer 4-wide nodes (the collapse may choose leaves
+    below a larger bound), one triangle per leaf at bound 1, and the binary SAH
+    tree unchanged."""
     S = ca.Scene(scenes.sphere_grid(grid=8), upload=False)
     b0 = S.bvh()
     info = {}
@@ -82,7 +84,8 @@ def test_leaf_size_changes_the_4wide_tree_only():
         S.set_leaf_size(L)
         info[L] = S.bvh4_info()
         assert all(np.array_equal(x, y) for x, y in zip(S.bvh(), b0))
-    assert info[1]["nodes"] > info[2]["nodes"] > info[4]["nodes"] > info[8]["nodes"] >= info[64]["nodes"]
+    assert info[1]["nodes"] > info[2]["nodes"] >= info[4]["nodes"] >= info[8]["nodes"] >= info[64]["nodes"]
+    assert info[2]["nodes"] > info[64]["nodes"]
     assert info[1]["leaves"] >= S.nt                       # one triangle per leaf
     S.set_leaf_size(2)
     assert S.bvh4_info() == info[2]
