@@ -34,18 +34,26 @@ __device__ __forceinline__ uint16_t f2bf(float f) {
 }
 __device__ __forceinline__ float bf2f(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
-// GELU, tanh form (Hendrycks & Gimpel 2016), and its derivative.
-// tanh(u) = 1 - 2 / (1 + e^(2u)) with the hardware exponential and reciprocal
-// (absolute error ~1e-7, far below the BF16 rounding of the activations).
-__device__ __forceinline__ float fast_tanh(float u) {
-  return 1.f - __fdividef(2.f, 1.f + __expf(2.f * u));
+// GELU, tanh form (Hendrycks & Gimpel 2016): 0.5 x (1 + tanh u) with
+// u = c (x + a x^3), evaluated as x sigmoid(2u) = x / (1 + 2^(-2u log2 e)) with
+// the hardware exponential and reciprocal (relative error ~1e-7, far below the
+// BF16 rounding of the activations).  One exponential, one reciprocal and four
+// other operations; through tanh with a division (correctly rounded under this
+// build's flags) inference took 248 us per 1920 x 1080 pass against 116 us.
+// Large |x| saturate to x and -0.
+__device__ __forceinline__ float gelu_sigmoid(float x, float x2) {
+  const float k = -2.f * 0.7978845608028654f * 1.4426950408889634f;   // -2 c log2(e)
+  return __builtin_amdgcn_rcpf(1.f + __builtin_amdgcn_exp2f(x * __builtin_fmaf(k * 0.044715f, x2, k)));
 }
+__device__ __forceinline__ float gelu_val(float x) { return x * gelu_sigmoid(x, x * x); }
+// With the derivative, for training: with s = sigmoid(2u),
+// d/dx [x s] = s + x s (1 - s) 2c (1 + 3 a x^2).
 __device__ __forceinline__ void gelu_fwd(float x, float& g, float& dg) {
   const float c = 0.7978845608028654f, a = 0.044715f;
-  float x2 = x * x;
-  float th = fast_tanh(c * (x + a * x2 * x));
-  g = 0.5f * x * (1.f + th);
-  dg = 0.5f * (1.f + th) + 0.5f * x * (1.f - th * th) * c * (1.f + 3.f * a * x2);
+  const float x2 = x * x;
+  const float sg = gelu_sigmoid(x, x2);
+  g = x * sg;
+  dg = __builtin_fmaf(g * (1.f - sg), (2.f * c) * __builtin_fmaf(3.f * a, x2, 1.f), sg);
 }
 
 // ---- register-resident layers ----------------------------------------------
@@ -367,47 +375,53 @@ __global__ __launch_bounds__(64 * kTrainWaves, 1) void k_mlp_train(MlpDevWeights
   if (tid == 0) loss_part[blockIdx.x] = s.u.red[kMlpParams];
 }
 
-__global__ __launch_bounds__(64 * kTrainWaves) void k_mlp_infer(MlpDevWeights w, const float* __restrict__ x,
-                                                               uint32_t n, float* __restrict__ y) {
+// Inference: a wave takes chunks of 16 * kInferBlocks pixels; kInferMinWaves
+// waves per SIMD bound its registers (115 of 128, no spills).
+constexpr int kInferBlocks = 4, kInferMinWaves = 4;
+constexpr uint32_t kInferMaxGroups = 2048;
+__global__ __launch_bounds__(64 * kTrainWaves, kInferMinWaves) void k_mlp_infer(MlpDevWeights w,
+                                                                               const float* __restrict__ x,
+                                                                               uint32_t n, float* __restrict__ y) {
+  constexpr int PB = kInferBlocks;
   __shared__ MlpWeightLds s;
   const int tid = threadIdx.x, wave = tid >> 6, l = tid & 63, g = l >> 4, c = l & 15;
   stage_weights<false>(s, w, tid, 64 * kTrainWaves);
   __syncthreads();
   const float b3 = w.b3[0];
-  const uint32_t nchunks = (n + 63) / 64;
+  const uint32_t nchunks = (n + 16 * PB - 1) / (16 * PB);
   for (uint32_t ch = blockIdx.x * kTrainWaves + wave; ch < nchunks; ch += gridDim.x * kTrainWaves) {
-    const uint32_t base = ch * 64;
-    s16x4 xb[4];
-    load_x<4>(xb, x, base, n, g, c);
-    bf16x8 hk[2][4];
+    // the weights are read from LDS again for every chunk: held in registers
+    // across chunks they would take about 90 of them and halve the waves
+    asm volatile("" ::: "memory");
+    const uint32_t base = ch * (16 * PB);
+    s16x4 xb[PB];
+    load_x<PB>(xb, x, base, n, g, c);
+    bf16x8 hk[2][PB];
 #pragma unroll
     for (int hb = 0; hb < 4; hb++)
 #pragma unroll
-      for (int pb = 0; pb < 4; pb++) {
+      for (int pb = 0; pb < PB; pb++) {
         const f32x4 z = layer1_tile(s.w1f, hb, xb[pb], l);
 #pragma unroll
-        for (int e = 0; e < 4; e++) {
-          float gv, dg;
-          gelu_fwd(z[e] + s.b1[hb * 16 + g * 4 + e], gv, dg);
-          hk[hb >> 1][pb][(hb & 1) * 4 + e] = as_bf(f2bf(gv));
-        }
+        for (int e = 0; e < 4; e++)
+          hk[hb >> 1][pb][(hb & 1) * 4 + e] = as_bf(f2bf(gelu_val(z[e] + s.b1[hb * 16 + g * 4 + e])));
       }
-    float ys[4] = {0.f, 0.f, 0.f, 0.f};
+    float ys[PB];
+#pragma unroll
+    for (int pb = 0; pb < PB; pb++) ys[pb] = 0.f;
 #pragma unroll
     for (int ob = 0; ob < 4; ob++)
 #pragma unroll
-      for (int pb = 0; pb < 4; pb++) {
-        const f32x4 z = layer64_tile<4>(s.w2tf, ob, pb, hk, l);
+      for (int pb = 0; pb < PB; pb++) {
+        const f32x4 z = layer64_tile<PB>(s.w2tf, ob, pb, hk, l);
 #pragma unroll
         for (int e = 0; e < 4; e++) {
           const int u = ob * 16 + g * 4 + e;
-          float gv, dg;
-          gelu_fwd(z[e] + s.b2[u], gv, dg);
-          ys[pb] = fmaf(gv, s.w3[u], ys[pb]);
+          ys[pb] = fmaf(gelu_val(z[e] + s.b2[u]), s.w3[u], ys[pb]);
         }
       }
 #pragma unroll
-    for (int pb = 0; pb < 4; pb++) {
+    for (int pb = 0; pb < PB; pb++) {
       float v = ys[pb];
       v += __shfl_xor(v, 16, 64);
       v += __shfl_xor(v, 32, 64);
@@ -524,9 +538,9 @@ hipError_t launch_mlp_adam(const MlpState& st, const float* part, uint32_t nwg, 
 hipError_t launch_mlp_infer(const MlpState& st, const float* x, uint32_t n, float* y, hipStream_t s) {
   if (!x || !y) return hipErrorInvalidValue;
   if (n == 0) return hipSuccess;
-  const uint32_t chunks = (n + 63) / 64;
+  const uint32_t chunks = (n + 16 * kInferBlocks - 1) / (16 * kInferBlocks);
   uint32_t grid = (chunks + kTrainWaves - 1) / kTrainWaves;
-  if (grid > 2048) grid = 2048;
+  if (grid > kInferMaxGroups) grid = kInferMaxGroups;
   k_mlp_infer<<<grid, 64 * kTrainWaves, 0, s>>>(st.dev, x, n, y);
   return hipGetLastError();
 }
