@@ -12,7 +12,8 @@ fann_* function), so parity is pinned by mathematics, not by reference output:
   bit patterns (ties, carries into the exponent, signed values);
 * ``forward/loss_grad(exact=False)``: the same network with operands rounded to
   bfloat16 where the device kernel rounds them (X, W1, W2, H1, dZ2, dZ1) and all
-  sums in FP64; the device result (FP32 accumulation, device tanh) is compared
+  sums in FP64; the device result (FP32 accumulation, the GELU through the
+  hardware exponential and reciprocal as x sigmoid(2u)) is compared
   with it under stated tolerances.
 
 Network: 9 -> 64 -> 64 -> 1, GELU (tanh form, Hendrycks & Gimpel 2016) after
