@@ -337,15 +337,17 @@ __device__ __forceinline__ bool ray_step(const DevScene& sc, RayState& R, bool c
   // past it) are pushed farthest first.  Slots from sp on are free, so when
   // all three fit the LDS top they are written unconditionally and sp moves
   // by n - 1.
-  cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[1], key[2]);
+  // With one interior child hit there is nothing to sort or push: its key is
+  // the OR of the four (the others are 0).
   const int n = ((uint32_t)key[0] != 0u) + ((uint32_t)key[1] != 0u) + ((uint32_t)key[2] != 0u) +
                 ((uint32_t)key[3] != 0u);
+  if (n == 1) {
+    R.node = (uint32_t)(key[0] | key[1] | key[2] | key[3]);
+    return false;
+  }
   if (n) {
-    uint32_t next = (uint32_t)key[3];
-    next = n >= 2 ? (uint32_t)key[2] : next;
-    next = n >= 3 ? (uint32_t)key[1] : next;
-    next = n >= 4 ? (uint32_t)key[0] : next;
-    R.node = next;
+    cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[1], key[2]);
+    R.node = n >= 3 ? (uint32_t)(n >= 4 ? key[0] : key[1]) : (uint32_t)key[2];
     if (R.sp + 3 <= STK) {
       lstk[R.sp * kBlock] = key[3];
       lstk[(R.sp + 1) * kBlock] = key[2];
