@@ -92,3 +92,41 @@ def test_leaf_size_changes_the_4wide_tree_only():
     for bad in (0, 65):
         with pytest.raises(ca.CptError, match="not in 1..64"):
             S.set_leaf_size(bad)
+
+
+def test_collapse_opens_small_subtrees_when_cheaper():
+    """SAH-optimal collapse: four unit triangles 10 apart along x.  The binary
+    tree pairs them; a pair's box is about 11 long, so a 2-triangle leaf costs
+    about 2 x 11 against 2 x 1 for two 1-triangle leaves, and the four
+    triangles hang as four leaves of the root.  (Taking every subtree within
+    the leaf bound as one leaf, as a greedy collapse does, gives two leaves of
+    two triangles.)  With the bound at 1 the tree is the same."""
+    pos = []
+    for k in range(4):
+        x = 10.0 * k
+        pos += [(x, 0.0, 0.0), (x + 1.0, 0.0, 0.0), (x, 1.0, 0.0)]
+    S = ca.Scene()
+    S.set_mesh(np.array(pos, np.float32), None, np.arange(12, dtype=np.uint32).reshape(4, 3),
+               np.zeros(4, np.uint32))
+    S.build_bvh()
+    for L in (2, 1):
+        S.set_leaf_size(L)
+        assert S.bvh4_info() == {"nodes": 1, "depth": 1, "leaves": 4}
+
+
+def test_collapse_keeps_a_leaf_of_coincident_triangles():
+    """Two triangles with one bounding box cost the same as one 2-triangle leaf
+    or two 1-triangle leaves; the collapse keeps the single leaf (ties go to
+    the entry that is already one), so four such pairs far apart give four
+    leaves under the root."""
+    pos = []
+    for k in range(4):
+        x = 10.0 * k
+        pos += [(x, 0.0, 0.0), (x + 1.0, 0.0, 0.0), (x, 1.0, 0.0),
+                (x + 1.0, 1.0, 0.0), (x, 1.0, 0.0), (x + 1.0, 0.0, 0.0)]
+    S = ca.Scene()
+    S.set_mesh(np.array(pos, np.float32), None, np.arange(24, dtype=np.uint32).reshape(8, 3),
+               np.zeros(8, np.uint32))
+    S.build_bvh()
+    S.set_leaf_size(2)
+    assert S.bvh4_info() == {"nodes": 1, "depth": 1, "leaves": 4}
