@@ -68,8 +68,8 @@ struct BvhHost {
 bool build_bvh_sah(const float* pos3, const u32* idx3, u32 nt, u32 max_depth, BvhHost& out);
 // The device tree: binned SAH over triangle bounds (dnodes / dreorder).
 bool build_bvh_device(const float* pos3, const u32* idx3, u32 nt, BvhHost& out);
-// Collapse the device binary tree into 4-wide nodes; subtrees of <= leaf_max
-// triangles become one leaf range.
+// Collapse the device binary tree into 4-wide nodes (SAH-optimal); a leaf
+// range holds at most leaf_max triangles.
 void collapse_bvh4(const float* pos3, const u32* idx3, BvhHost& b, u32 leaf_max);
 // Quantize nodes4 into nodes4q; false when a node cannot be encoded.
 bool quantize_bvh4(BvhHost& b);

@@ -123,8 +123,8 @@ int cpt_bvh_build_sah(cpt_scene* s, uint32_t max_depth);
 int cpt_bvh_export(cpt_scene* s, float* bounds6, uint32_t* off2, uint32_t* reorder, uint32_t* node_n);
 
 /* No reference counterpart: the most triangles a leaf of the 4-wide device BVH
- * holds (1..64, default 2).  Subtrees of the binary SAH tree with at most that
- * many triangles become leaves.  Applies to the current BVH (rebuilt at once,
+ * holds (1..64, default 2).  The SAH-optimal collapse of the binary tree picks
+ * leaves of at most that many triangles.  Applies to the current BVH (rebuilt at once,
  * uploaded by the next render) and to later builds; hits do not depend on it. */
 int cpt_bvh4_set_leaf_size(cpt_scene* s, uint32_t max_tris);
 
